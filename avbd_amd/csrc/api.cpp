@@ -1,0 +1,800 @@
+// api.cpp -- the avbd C ABI (include/avbd.h): context lifetime, FP64 host
+// staging pools, upload to FP32 device records, hipGraph capture of one time
+// step, readback and statistics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace avbd;
+
+struct avbd_ctx {
+    avbd_scene_desc desc{};
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // FP64 staging (ParticlePool / EnergyRegistry layout, reference Real = double)
+    std::vector<double> x, v, v_prev, x_prev, mass;
+    std::vector<uint8_t> fixed;
+    std::vector<HostNH> nh;
+    std::vector<HostGround> ground;
+    std::vector<uint32_t> scene_offsets;   // empty = one scene
+
+    bool planned = false;   // host colour plan done
+    bool built = false;     // device buffers uploaded
+    bool damping = false;
+    ColorPlan plan;
+    DevState d;
+    uint32_t* d_conflicts = nullptr;
+    std::vector<void*> allocations;
+    uint32_t reduce_blocks = 0;
+
+    LaunchCfg cfg;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    double solve_avg_ms = 0;
+    uint32_t solve_launches = 0;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+
+    uint64_t steps_taken = 0, vertex_updates = 0;
+    double last_step_ms = 0;
+};
+
+namespace {
+
+std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct CodeError : std::runtime_error {
+    int code;
+    CodeError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(avbd_ctx* c, F&& f) {
+    if (!c) return AVBD_E_ARG;
+    try {
+        f();
+        return AVBD_OK;
+    } catch (const CodeError& e) {
+        c->err = e.what();
+        return e.code;
+    } catch (const HipError& e) {
+        c->err = e.what();
+        return AVBD_E_HIP;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        const std::string m = e.what();
+        if (m.find("Ran out of colors") != std::string::npos) return AVBD_E_COLORS;
+        if (m.find("capacity") != std::string::npos) return AVBD_E_CAPACITY;
+        return AVBD_E_ARG;
+    }
+}
+
+template <class T>
+T* dev_alloc(avbd_ctx* c, size_t count) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    c->allocations.push_back(p);
+    return static_cast<T*>(p);
+}
+
+template <class T>
+void upload(avbd_ctx* c, T* dst, const std::vector<T>& src) {
+    if (src.empty()) return;
+    hip_check(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c->stream),
+              "hipMemcpyAsync H2D");
+}
+
+float4 f4(const double* p, double w) { return make_float4((float)p[0], (float)p[1], (float)p[2], (float)w); }
+
+GroundRec to_dev(const HostGround& g) {
+    GroundRec r{};
+    r.k_start = (float)g.k_start;
+    r.k_n = (float)g.k_n;
+    r.k_t = (float)g.k_t;
+    r.k_b = (float)g.k_b;
+    r.lambda_n = (float)g.lambda_n;
+    r.lambda_t = (float)g.lambda_t;
+    r.lambda_b = (float)g.lambda_b;
+    r.C_n_prev = (float)g.C_n_prev;
+    r.mu_s = (float)g.mu_s;
+    r.mu_k = (float)g.mu_k;
+    r.cp_x = (float)g.cp_x;
+    r.cp_z = (float)g.cp_z;
+    r.p = g.p;
+    r.flags = (g.use_static ? GF_USE_STATIC : 0u) | (g.unclamped ? GF_UNCLAMPED : 0u);
+    return r;
+}
+
+void require_built(avbd_ctx* c) {
+    if (!c->built) throw CodeError(AVBD_E_NOT_BUILT, "avbd_build_graph() has not been called");
+}
+
+void require_not_built(avbd_ctx* c) {
+    if (c->built) throw CodeError(AVBD_E_ARG, "pools are frozen after avbd_build_graph()");
+}
+
+void require_not_planned(avbd_ctx* c) {
+    if (c->planned) throw CodeError(AVBD_E_ARG, "pools are frozen after avbd_build_graph()");
+}
+
+void check_capacity(avbd_ctx* c, size_t needed, uint32_t cap) {
+    if (cap && needed > cap) throw CodeError(AVBD_E_CAPACITY, "Pool has exceeded capacity");
+    (void)c;
+}
+
+// Upload particle state [first, first+n) from the FP64 staging arrays.
+void upload_particle_range(avbd_ctx* c, uint32_t first, uint32_t n) {
+    std::vector<float4> bx(n), bxp(n), bv(n), bvp(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t p = first + i;
+        bx[i] = f4(&c->x[3 * p], c->mass[p]);
+        bxp[i] = f4(&c->x_prev[3 * p], 0.0);
+        bv[i] = f4(&c->v[3 * p], 0.0);
+        bvp[i] = f4(&c->v_prev[3 * p], 0.0);
+    }
+    auto cp = [&](float4* dst, const std::vector<float4>& src) {
+        hip_check(hipMemcpyAsync(dst + first, src.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream),
+                  "hipMemcpyAsync H2D");
+    };
+    cp(c->d.x, bx);
+    cp(c->d.x_prev, bxp);
+    cp(c->d.v, bv);
+    cp(c->d.v_prev, bvp);
+    // ParticlePool::addParticle initialises y, x_last, x_lastlast to the position
+    cp(c->d.y, bx);
+    cp(c->d.x_last, bx);
+    cp(c->d.x_lastlast, bx);
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+double omega_at(uint32_t i, double rho, double omega) {
+    // Chebyshev schedule, simulation/VBDSolver.hpp:65-74
+    if (i > 2) return 4 / (4 - rho * rho * omega);
+    if (i == 2) return 2 / (2 - rho * rho);
+    return 1;
+}
+
+// Enqueue one time step (VBDSolver::solve_Parallel, simulation/VBDSolver.hpp:48-103).
+void enqueue_step(avbd_ctx* c, bool with_events) {
+    const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
+    hipStream_t s = c->stream;
+    if (c->damping) launch_eprev(s, c->d, dt);
+    launch_step_begin(s, c->d, dt, g);
+    double omega = 1;
+    size_t ev = 0;
+    for (uint32_t i = 0; i < c->desc.solver_iters; i++) {
+        omega = omega_at(i, c->desc.iter_acceleration, omega);
+        for (uint32_t col = 0; col < c->plan.num_colors; col++) {
+            const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
+            if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
+            launch_solve_color(s, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x);
+            if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
+            const uint32_t cb = c->plan.conflict_offsets[col], ce = c->plan.conflict_offsets[col + 1];
+            if (ce > cb) launch_commit(s, c->d, c->d_conflicts + cb, ce - cb);
+        }
+        launch_cheb_dual(s, c->d, (float)omega);
+    }
+    launch_velocity(s, c->d, dt);
+    hip_check(hipGetLastError(), "kernel launch");
+}
+
+void destroy_graph(avbd_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+}
+
+void capture_graph(avbd_ctx* c) {
+    destroy_graph(c);
+    hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        enqueue_step(c, false);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(c->stream, &c->graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+// Autotuner: time the colour-phase solve of one iteration for each candidate
+// (lanes per vertex, block size), writing into a scratch sink so that the
+// state is untouched, and keep the fastest.
+void autotune(avbd_ctx* c) {
+    const uint32_t lanes_c[] = {4, 8, 16};
+    const uint32_t block_c[] = {64, 128, 256};
+    float best = 1e30f;
+    LaunchCfg best_cfg = c->cfg;
+    const float dt = (float)c->desc.dt;
+    for (uint32_t L : lanes_c) {
+        if (c->desc.lanes_per_vertex && L != c->desc.lanes_per_vertex) continue;
+        for (uint32_t B : block_c) {
+            if (c->desc.block_size && B != c->desc.block_size) continue;
+            LaunchCfg cfg{L, B};
+            auto run = [&] {
+                for (uint32_t col = 0; col < c->plan.num_colors; col++) {
+                    const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
+                    launch_solve_color(c->stream, c->d, cfg, b, e - b, c->damping, dt, c->d.x_scratch);
+                }
+            };
+            for (int w = 0; w < 3; w++) run();
+            hip_check(hipEventRecord(c->ev_a, c->stream), "hipEventRecord");
+            for (int r = 0; r < 10; r++) run();
+            hip_check(hipEventRecord(c->ev_b, c->stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(c->ev_b), "hipEventSynchronize");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, c->ev_a, c->ev_b), "hipEventElapsedTime");
+            if (ms < best) {
+                best = ms;
+                best_cfg = cfg;
+            }
+        }
+    }
+    c->cfg = best_cfg;
+}
+
+}  // namespace
+
+extern "C" {
+
+int avbd_create(const avbd_scene_desc* desc, avbd_ctx** out) {
+    if (!desc || !out) {
+        g_create_error = "null argument";
+        return AVBD_E_ARG;
+    }
+    *out = nullptr;
+    if (!(desc->dt > 0) || desc->solver_iters == 0) {
+        g_create_error = "dt must be > 0 and solver_iters >= 1";
+        return AVBD_E_ARG;
+    }
+    const uint32_t L = desc->lanes_per_vertex, B = desc->block_size;
+    if (L && L != 4 && L != 8 && L != 16 && L != 32) {
+        g_create_error = "lanes_per_vertex must be 0, 4, 8, 16 or 32";
+        return AVBD_E_ARG;
+    }
+    if (B && (B < 64 || B > 1024 || B % 64)) {
+        g_create_error = "block_size must be 0 or a multiple of 64 in [64, 1024]";
+        return AVBD_E_ARG;
+    }
+    // HIP is initialised lazily by avbd_build_graph(): staging and the host
+    // graph build work without a device.
+    auto* c = new avbd_ctx();
+    c->desc = *desc;
+    c->device = desc->device;
+    c->cfg.lanes = L ? L : 8;
+    c->cfg.block = B ? B : 256;
+    *out = c;
+    return AVBD_OK;
+}
+
+void avbd_destroy(avbd_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    destroy_graph(c);
+    for (void* p : c->allocations) (void)hipFree(p);
+    for (auto ev : c->events) (void)hipEventDestroy(ev);
+    if (c->ev_a) (void)hipEventDestroy(c->ev_a);
+    if (c->ev_b) (void)hipEventDestroy(c->ev_b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* avbd_last_error(avbd_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int avbd_add_particles(avbd_ctx* c, uint32_t n, const double* x, const double* v, const double* v_prev,
+                       const double* x_prev, const double* mass, const uint8_t* fixed, uint32_t* first) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (n && !x) throw std::runtime_error("x is required");
+        const size_t base = c->mass.size();
+        check_capacity(c, base + n, c->desc.particle_capacity);
+        for (uint32_t i = 0; i < n; i++) {
+            for (int k = 0; k < 3; k++) {
+                c->x.push_back(x[3 * i + k]);
+                c->v.push_back(v ? v[3 * i + k] : 0.0);
+                c->v_prev.push_back(v_prev ? v_prev[3 * i + k] : 0.0);
+                c->x_prev.push_back(x_prev ? x_prev[3 * i + k] : x[3 * i + k]);
+            }
+            c->mass.push_back(mass ? mass[i] : 0.0);
+            c->fixed.push_back(fixed ? fixed[i] : 0);
+        }
+        if (first) *first = (uint32_t)base;
+    });
+}
+
+int avbd_add_energies(avbd_ctx* c, int type, const void* records, uint32_t count) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (count && !records) throw std::runtime_error("records is null");
+        if (type == AVBD_NEO_HOOKEAN) {
+            check_capacity(c, c->nh.size() + count, c->desc.energy_capacity);
+            const auto* r = static_cast<const avbd_neo_hookean_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                HostNH h;
+                std::memcpy(h.idx, r[i].particle_indices, sizeof(h.idx));
+                h.lambda = r[i].lambda;
+                h.mu = r[i].mu;
+                h.kd = r[i].kd;
+                std::memcpy(h.Q, r[i].Q, sizeof(h.Q));
+                h.V = r[i].rest_volume;
+                std::memcpy(h.E_prev, r[i].E_prev, sizeof(h.E_prev));
+                c->nh.push_back(h);
+            }
+        } else if (type == AVBD_GROUND_COLLISION) {
+            check_capacity(c, c->ground.size() + count, c->desc.energy_capacity);
+            const auto* r = static_cast<const avbd_ground_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                HostGround g;
+                g.p = r[i].particle_index;
+                g.k_start = r[i].k_start;
+                g.k_n = r[i].k_n;
+                g.k_t = r[i].k_t;
+                g.k_b = r[i].k_b;
+                g.lambda_n = r[i].lambda_n;
+                g.lambda_t = r[i].lambda_t;
+                g.lambda_b = r[i].lambda_b;
+                g.C_n_prev = r[i].C_n_prev;
+                g.mu_s = r[i].mu_s;
+                g.mu_k = r[i].mu_k;
+                g.cp_x = r[i].cp_x;
+                g.cp_z = r[i].cp_z;
+                g.use_static = r[i].use_static != 0;
+                g.unclamped = r[i].unclamped_lambda_tb != 0;
+                c->ground.push_back(g);
+            }
+        } else {
+            throw CodeError(AVBD_E_ARG, "energy type not supported by this build");
+        }
+    });
+}
+
+// TetMeshObject::setup (simobject/TetMeshObject.cpp:46-96) and the per-vertex
+// ground energies of simulation/Simulation.hpp:117-126, in FP64.
+int avbd_add_tet_mesh(avbd_ctx* c, uint32_t nv, const double* rest_x, uint32_t nt, const uint32_t* tets, double E,
+                      double nu, double density, double kd, int with_ground, double mu_s, double mu_k,
+                      uint32_t* first) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (!rest_x || (nt && !tets)) throw std::runtime_error("null mesh arrays");
+        const uint32_t base = (uint32_t)c->mass.size();
+        check_capacity(c, (size_t)base + nv, c->desc.particle_capacity);
+        check_capacity(c, c->nh.size() + nt, c->desc.energy_capacity);
+        if (with_ground) check_capacity(c, c->ground.size() + nv, c->desc.energy_capacity);
+        for (uint32_t t = 0; t < nt; t++)
+            for (int k = 0; k < 4; k++)
+                if (tets[4 * t + k] >= nv) throw std::runtime_error("tet index out of range");
+        for (uint32_t i = 0; i < nv; i++) {
+            for (int k = 0; k < 3; k++) {
+                c->x.push_back(rest_x[3 * i + k]);
+                c->x_prev.push_back(rest_x[3 * i + k]);
+                c->v.push_back(0.0);
+                c->v_prev.push_back(0.0);
+            }
+            c->mass.push_back(0.0);
+            c->fixed.push_back(0);
+        }
+        const double mu = E / (2 * (1 + nu));
+        const double lambda = (E * nu) / ((1 + nu) * (1 - 2 * nu));
+        for (uint32_t t = 0; t < nt; t++) {
+            HostNH h{};
+            for (int k = 0; k < 4; k++) h.idx[k] = base + tets[4 * t + k];
+            const double* p[4];
+            for (int k = 0; k < 4; k++) p[k] = &c->x[3 * h.idx[k]];
+            double X[3][3];
+            for (int r = 0; r < 3; r++)
+                for (int col = 0; col < 3; col++) X[r][col] = p[col][r] - p[3][r];
+            const double det = X[0][0] * (X[1][1] * X[2][2] - X[1][2] * X[2][1]) -
+                               X[0][1] * (X[1][0] * X[2][2] - X[1][2] * X[2][0]) +
+                               X[0][2] * (X[1][0] * X[2][1] - X[1][1] * X[2][0]);
+            if (!(std::fabs(det) > 0)) throw std::runtime_error("degenerate tet in rest state");
+            const double inv = 1.0 / det;
+            h.Q[0] = (X[1][1] * X[2][2] - X[1][2] * X[2][1]) * inv;
+            h.Q[1] = (X[0][2] * X[2][1] - X[0][1] * X[2][2]) * inv;
+            h.Q[2] = (X[0][1] * X[1][2] - X[0][2] * X[1][1]) * inv;
+            h.Q[3] = (X[1][2] * X[2][0] - X[1][0] * X[2][2]) * inv;
+            h.Q[4] = (X[0][0] * X[2][2] - X[0][2] * X[2][0]) * inv;
+            h.Q[5] = (X[0][2] * X[1][0] - X[0][0] * X[1][2]) * inv;
+            h.Q[6] = (X[1][0] * X[2][1] - X[1][1] * X[2][0]) * inv;
+            h.Q[7] = (X[0][1] * X[2][0] - X[0][0] * X[2][1]) * inv;
+            h.Q[8] = (X[0][0] * X[1][1] - X[0][1] * X[1][0]) * inv;
+            h.V = std::fabs(det / 6.0);
+            h.lambda = lambda;
+            h.mu = mu;
+            h.kd = kd;
+            c->nh.push_back(h);
+            for (int k = 0; k < 4; k++) c->mass[h.idx[k]] += 0.25 * h.V * density;
+        }
+        if (with_ground) {
+            const double dt2 = c->desc.dt * c->desc.dt;
+            for (uint32_t i = 0; i < nv; i++) {
+                HostGround g{};
+                g.p = base + i;
+                g.k_start = g.k_n = g.k_t = g.k_b = c->mass[base + i] / dt2;
+                g.mu_s = mu_s;
+                g.mu_k = mu_k;
+                c->ground.push_back(g);
+            }
+        }
+        if (first) *first = base;
+    });
+}
+
+int avbd_set_particle_state(avbd_ctx* c, uint32_t first, uint32_t n, const double* x, const double* x_prev,
+                            const double* v, const double* v_prev) {
+    return guarded(c, [&] {
+        if ((size_t)first + n > c->mass.size()) throw std::runtime_error("particle range out of bounds");
+        if (c->built) {
+            // pull the current device state so that NULL fields stay unchanged
+            std::vector<double> cx(3 * c->mass.size()), cv(3 * c->mass.size());
+            if (avbd_download_state(c, cx.data(), cv.data()) != AVBD_OK) throw std::runtime_error(c->err);
+        }
+        for (uint32_t i = 0; i < 3 * n; i++) {
+            const size_t k = 3 * (size_t)first + i;
+            if (x) c->x[k] = x[i];
+            if (x_prev) c->x_prev[k] = x_prev[i];
+            if (v) c->v[k] = v[i];
+            if (v_prev) c->v_prev[k] = v_prev[i];
+        }
+        if (c->built) upload_particle_range(c, first, n);
+    });
+}
+
+int avbd_set_fixed(avbd_ctx* c, uint32_t first, uint32_t n, const uint8_t* fixed) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if ((size_t)first + n > c->mass.size() || !fixed) throw std::runtime_error("bad fixed range");
+        for (uint32_t i = 0; i < n; i++) c->fixed[first + i] = fixed[i];
+    });
+}
+
+int avbd_set_scenes(avbd_ctx* c, uint32_t n_scenes, const uint32_t* offsets) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (n_scenes == 0 || !offsets) throw std::runtime_error("need at least one scene");
+        if (offsets[0] != 0 || offsets[n_scenes] != c->mass.size())
+            throw std::runtime_error("scene offsets must start at 0 and end at the particle count");
+        for (uint32_t s = 0; s < n_scenes; s++)
+            if (offsets[s + 1] < offsets[s]) throw std::runtime_error("scene offsets must be non-decreasing");
+        c->scene_offsets.assign(offsets, offsets + n_scenes + 1);
+    });
+}
+
+int avbd_build_graph(avbd_ctx* c) {
+    return guarded(c, [&] {
+        require_not_built(c);
+        const uint32_t n = (uint32_t)c->mass.size();
+        if (!c->planned) {
+            std::vector<uint32_t> so = c->scene_offsets;
+            if (so.empty()) so = {0u, n};
+            if (so.back() != n) throw std::runtime_error("scene offsets do not cover the particle pool");
+            build_color_plan(n, c->nh, c->ground, c->fixed, so, c->plan);
+            c->planned = true;
+        }
+        // device part
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        if (!c->stream) hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        if (!c->ev_a) hip_check(hipEventCreate(&c->ev_a), "hipEventCreate");
+        if (!c->ev_b) hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
+        c->damping = false;
+        for (const auto& t : c->nh)
+            if (t.kd != 0.0) c->damping = true;
+
+        DevState& d = c->d;
+        d.n = n;
+        d.n_tets = (uint32_t)c->nh.size();
+        d.n_ground = (uint32_t)c->ground.size();
+        d.n_items = (uint32_t)c->plan.items.size();
+        d.n_inc = (uint32_t)c->plan.inc.size();
+        d.x = dev_alloc<float4>(c, n);
+        d.x_buf = dev_alloc<float4>(c, n);
+        d.y = dev_alloc<float4>(c, n);
+        d.x_prev = dev_alloc<float4>(c, n);
+        d.x_last = dev_alloc<float4>(c, n);
+        d.x_lastlast = dev_alloc<float4>(c, n);
+        d.v = dev_alloc<float4>(c, n);
+        d.v_prev = dev_alloc<float4>(c, n);
+        d.x_scratch = dev_alloc<float4>(c, n);
+        d.fixed = dev_alloc<uint8_t>(c, n);
+        d.ground_of = dev_alloc<int32_t>(c, n);
+        d.tets = dev_alloc<TetRec>(c, d.n_tets);
+        d.damp = dev_alloc<TetDamp>(c, d.n_tets);
+        d.ground = dev_alloc<GroundRec>(c, d.n_ground);
+        d.inc = dev_alloc<uint32_t>(c, d.n_inc);
+        d.items = dev_alloc<VItem>(c, d.n_items);
+        c->d_conflicts = dev_alloc<uint32_t>(c, c->plan.conflict_list.size());
+        // reduction slots: enough blocks for the stats passes at any launch config
+        c->reduce_blocks = std::max((d.n_items * 32 + 63) / 64, (d.n_tets + d.n_ground + 255) / 256) + 1;
+        d.reduce = dev_alloc<double>(c, (size_t)c->reduce_blocks * 4);
+
+        std::vector<TetRec> tr(d.n_tets);
+        std::vector<TetDamp> td(d.n_tets);
+        for (uint32_t t = 0; t < d.n_tets; t++) {
+            const HostNH& h = c->nh[t];
+            std::memcpy(tr[t].idx, h.idx, sizeof(h.idx));
+            for (int k = 0; k < 9; k++) tr[t].q[k] = (float)h.Q[k];
+            tr[t].VL = (float)(h.V * h.lambda);
+            tr[t].VM = (float)(h.V * h.mu);
+            tr[t].alpha = (float)(h.mu / h.lambda + 1.0);
+            td[t].e00 = (float)h.E_prev[0];
+            td[t].e11 = (float)h.E_prev[4];
+            td[t].e22 = (float)h.E_prev[8];
+            td[t].e01 = (float)h.E_prev[1];
+            td[t].e02 = (float)h.E_prev[2];
+            td[t].e12 = (float)h.E_prev[5];
+            td[t].damp = (float)(2.0 / c->desc.dt * h.V * h.kd);
+            td[t].pad = 0.f;
+        }
+        std::vector<GroundRec> gr(d.n_ground);
+        for (uint32_t i = 0; i < d.n_ground; i++) gr[i] = to_dev(c->ground[i]);
+        std::vector<int32_t> gof(n, -1);
+        for (uint32_t i = 0; i < d.n_ground; i++) gof[c->ground[i].p] = (int32_t)i;
+        upload(c, d.tets, tr);
+        upload(c, d.damp, td);
+        upload(c, d.ground, gr);
+        upload(c, d.ground_of, gof);
+        upload(c, d.fixed, c->fixed);
+        upload(c, d.inc, c->plan.inc);
+        upload(c, d.items, c->plan.items);
+        upload(c, c->d_conflicts, c->plan.conflict_list);
+        upload_particle_range(c, 0, n);
+        c->built = true;
+
+        const uint32_t n_solves = c->desc.solver_iters * c->plan.num_colors;
+        c->events.resize(2 * n_solves);
+        for (auto& ev : c->events) hip_check(hipEventCreate(&ev), "hipEventCreate");
+
+        if (c->desc.lanes_per_vertex == 0 || c->desc.block_size == 0) autotune(c);
+        if (!(c->desc.flags & AVBD_FLAG_NO_GRAPH)) capture_graph(c);
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    });
+}
+
+int avbd_step(avbd_ctx* c, uint32_t n_steps) {
+    return guarded(c, [&] {
+        require_built(c);
+        hip_check(hipEventRecord(c->ev_a, c->stream), "hipEventRecord");
+        if (c->timing) {
+            double total = 0;
+            uint32_t launches = 0;
+            for (uint32_t s = 0; s < n_steps; s++) {
+                enqueue_step(c, true);
+                hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+                for (size_t k = 0; k + 1 < c->events.size(); k += 2) {
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, c->events[k], c->events[k + 1]), "hipEventElapsedTime");
+                    total += ms;
+                    launches++;
+                }
+            }
+            c->solve_avg_ms = launches ? total / launches : 0;
+            c->solve_launches = launches;
+        } else if (c->graph_exec) {
+            for (uint32_t s = 0; s < n_steps; s++) hip_check(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
+        } else {
+            for (uint32_t s = 0; s < n_steps; s++) enqueue_step(c, false);
+        }
+        hip_check(hipEventRecord(c->ev_b, c->stream), "hipEventRecord");
+        c->steps_taken += n_steps;
+        c->vertex_updates += (uint64_t)n_steps * c->desc.solver_iters * c->d.n_items;
+    });
+}
+
+int avbd_synchronize(avbd_ctx* c) {
+    return guarded(c, [&] {
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        if (c->built) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, c->ev_a, c->ev_b) == hipSuccess) c->last_step_ms = ms;
+        }
+    });
+}
+
+int avbd_download_state(avbd_ctx* c, double* x, double* v) {
+    return guarded(c, [&] {
+        require_built(c);
+        const uint32_t n = c->d.n;
+        std::vector<float4> bx(n), bv(n);
+        hip_check(hipMemcpyAsync(bx.data(), c->d.x, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipMemcpyAsync(bv.data(), c->d.v, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        for (uint32_t i = 0; i < n; i++) {
+            if (x) {
+                x[3 * i] = bx[i].x;
+                x[3 * i + 1] = bx[i].y;
+                x[3 * i + 2] = bx[i].z;
+            }
+            if (v) {
+                v[3 * i] = bv[i].x;
+                v[3 * i + 1] = bv[i].y;
+                v[3 * i + 2] = bv[i].z;
+            }
+        }
+        // keep the staging copy current (used by avbd_set_particle_state)
+        std::vector<float4> bxp(n), bvp(n);
+        hip_check(hipMemcpyAsync(bxp.data(), c->d.x_prev, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream),
+                  "D2H");
+        hip_check(hipMemcpyAsync(bvp.data(), c->d.v_prev, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream),
+                  "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        for (uint32_t i = 0; i < n; i++) {
+            const float* fx = &bx[i].x;
+            const float* fv = &bv[i].x;
+            const float* fxp = &bxp[i].x;
+            const float* fvp = &bvp[i].x;
+            for (int k = 0; k < 3; k++) {
+                c->x[3 * i + k] = fx[k];
+                c->v[3 * i + k] = fv[k];
+                c->x_prev[3 * i + k] = fxp[k];
+                c->v_prev[3 * i + k] = fvp[k];
+            }
+        }
+    });
+}
+
+int avbd_download_inertial(avbd_ctx* c, double* y) {
+    return guarded(c, [&] {
+        require_built(c);
+        const uint32_t n = c->d.n;
+        std::vector<float4> by(n);
+        hip_check(hipMemcpyAsync(by.data(), c->d.y, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        for (uint32_t i = 0; i < n; i++) {
+            y[3 * i] = by[i].x;
+            y[3 * i + 1] = by[i].y;
+            y[3 * i + 2] = by[i].z;
+        }
+    });
+}
+
+int avbd_download_energies(avbd_ctx* c, int type, void* records, uint32_t count) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (type == AVBD_NEO_HOOKEAN) {
+            if (count > c->d.n_tets) throw std::runtime_error("count exceeds the pool");
+            std::vector<TetDamp> td(c->d.n_tets);
+            hip_check(hipMemcpyAsync(td.data(), c->d.damp, td.size() * sizeof(TetDamp), hipMemcpyDeviceToHost,
+                                     c->stream),
+                      "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            auto* r = static_cast<avbd_neo_hookean_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                const HostNH& h = c->nh[i];
+                std::memcpy(r[i].particle_indices, h.idx, sizeof(h.idx));
+                r[i].lambda = h.lambda;
+                r[i].mu = h.mu;
+                r[i].kd = h.kd;
+                std::memcpy(r[i].Q, h.Q, sizeof(h.Q));
+                r[i].rest_volume = h.V;
+                const double E[9] = {td[i].e00, td[i].e01, td[i].e02, td[i].e01, td[i].e11,
+                                     td[i].e12, td[i].e02, td[i].e12, td[i].e22};
+                std::memcpy(r[i].E_prev, E, sizeof(E));
+            }
+        } else if (type == AVBD_GROUND_COLLISION) {
+            if (count > c->d.n_ground) throw std::runtime_error("count exceeds the pool");
+            std::vector<GroundRec> gr(c->d.n_ground);
+            hip_check(hipMemcpyAsync(gr.data(), c->d.ground, gr.size() * sizeof(GroundRec), hipMemcpyDeviceToHost,
+                                     c->stream),
+                      "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            auto* r = static_cast<avbd_ground_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                const GroundRec& g = gr[i];
+                r[i].particle_index = g.p;
+                r[i].use_static = (g.flags & GF_USE_STATIC) ? 1 : 0;
+                r[i].unclamped_lambda_tb = (g.flags & GF_UNCLAMPED) ? 1 : 0;
+                r[i].pad_ = 0;
+                r[i].k_start = g.k_start;
+                r[i].k_n = g.k_n;
+                r[i].k_t = g.k_t;
+                r[i].k_b = g.k_b;
+                r[i].lambda_n = g.lambda_n;
+                r[i].lambda_t = g.lambda_t;
+                r[i].lambda_b = g.lambda_b;
+                r[i].C_n_prev = g.C_n_prev;
+                r[i].mu_s = g.mu_s;
+                r[i].mu_k = g.mu_k;
+                r[i].cp_x = g.cp_x;
+                r[i].cp_z = g.cp_z;
+            }
+        } else {
+            throw CodeError(AVBD_E_ARG, "energy type not supported by this build");
+        }
+    });
+}
+
+int avbd_download_coloring(avbd_ctx* c, uint32_t* color, uint32_t* num_colors) {
+    return guarded(c, [&] {
+        // the colouring is host-side: available once avbd_build_graph() has
+        // planned it, even if the device part of that call failed
+        if (!c->planned) throw CodeError(AVBD_E_NOT_BUILT, "avbd_build_graph() has not been called");
+        if (color) std::copy(c->plan.color.begin(), c->plan.color.end(), color);
+        if (num_colors) *num_colors = c->plan.num_colors;
+    });
+}
+
+int avbd_get_stats(avbd_ctx* c, avbd_stats* out) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!out) throw std::runtime_error("null output");
+        avbd_stats st{};
+        hip_check(hipMemsetAsync(c->d.reduce, 0, (size_t)c->reduce_blocks * 4 * sizeof(double), c->stream),
+                  "hipMemsetAsync");
+        launch_stats(c->stream, c->d, c->cfg, c->damping, (float)c->desc.dt, (float)c->desc.g_accel);
+        hip_check(hipGetLastError(), "stats launch");
+        std::vector<double> red((size_t)c->reduce_blocks * 4);
+        hip_check(hipMemcpyAsync(red.data(), c->d.reduce, red.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        double r2 = 0, mg2 = 0, ip = 0, en = 0;
+        for (uint32_t b = 0; b < c->reduce_blocks; b++) {
+            r2 += red[4 * b];
+            mg2 += red[4 * b + 1];
+            ip += red[4 * b + 2];
+            en += red[4 * b + 3];
+        }
+        st.residual_abs = std::sqrt(r2);
+        st.residual_rel = mg2 > 0 ? std::sqrt(r2) / std::sqrt(mg2) : 0;
+        st.incremental_potential = ip + en;
+        st.num_colors = c->plan.num_colors;
+        st.num_particles = c->d.n;
+        st.num_tets = c->d.n_tets;
+        st.num_ground = c->d.n_ground;
+        st.steps_taken = c->steps_taken;
+        st.vertex_updates = c->vertex_updates;
+        st.last_step_ms = c->last_step_ms;
+        *out = st;
+    });
+}
+
+uint32_t avbd_num_particles(avbd_ctx* c) { return c ? (uint32_t)c->mass.size() : 0; }
+
+uint32_t avbd_num_energies(avbd_ctx* c, int type) {
+    if (!c) return 0;
+    if (type == AVBD_NEO_HOOKEAN) return (uint32_t)c->nh.size();
+    if (type == AVBD_GROUND_COLLISION) return (uint32_t)c->ground.size();
+    return 0;
+}
+
+int avbd_get_launch_config(avbd_ctx* c, uint32_t* lanes, uint32_t* block) {
+    return guarded(c, [&] {
+        if (lanes) *lanes = c->cfg.lanes;
+        if (block) *block = c->cfg.block;
+    });
+}
+
+int avbd_set_timing(avbd_ctx* c, int enable) {
+    return guarded(c, [&] { c->timing = enable != 0; });
+}
+
+int avbd_get_solve_timing(avbd_ctx* c, double* avg_ms, uint32_t* launches) {
+    return guarded(c, [&] {
+        if (avg_ms) *avg_ms = c->solve_avg_ms;
+        if (launches) *launches = c->solve_launches;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+// host_graph.cpp -- setup-time graph work of the avbd context: vertex
+// adjacency, DSATUR colouring per scene, and the per-colour work-item layout
+// that the vertex-solve kernel consumes.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <set>
+#include <stdexcept>
+#include <thread>
+#include <tuple>
+
+namespace avbd {
+
+// ColorList::greedyColor2 (common/ColorList.hpp:149-241).  The reference scans
+// all uncoloured vertices each round and keeps the first vertex with the
+// lexicographically largest (saturation, degree); that is the minimum of the
+// key (-saturation, -degree, index), kept here in an ordered set so a round
+// costs O(log n) instead of O(n).  Saturation = number of distinct colours
+// among coloured neighbours, maintained with a 256-bit set per vertex (the
+// reference's ColorBitset<4> limit, common/ColorList.hpp:11-38).
+uint32_t dsatur_csr(uint32_t n, const uint32_t* off, const uint32_t* nb, uint32_t* color) {
+    constexpr uint32_t UNC = 0xFFFFFFFFu;
+    std::vector<std::array<uint64_t, 4>> seen(n, std::array<uint64_t, 4>{0, 0, 0, 0});
+    std::vector<uint32_t> sat(n, 0);
+    using Key = std::tuple<int64_t, int64_t, uint32_t>;
+    std::set<Key> heap;
+    for (uint32_t v = 0; v < n; v++) {
+        color[v] = UNC;
+        heap.emplace(0, -(int64_t)(off[v + 1] - off[v]), v);
+    }
+    uint32_t num_colors = 0;
+    while (!heap.empty()) {
+        const uint32_t best = std::get<2>(*heap.begin());
+        heap.erase(heap.begin());
+        std::array<uint64_t, 4> used = {0, 0, 0, 0};
+        for (uint32_t e = off[best]; e < off[best + 1]; e++) {
+            const uint32_t c = color[nb[e]];
+            if (c != UNC) used[c >> 6] |= 1ull << (c & 63);
+        }
+        uint32_t c = 0;
+        while (c < 256 && (used[c >> 6] >> (c & 63) & 1ull)) c++;
+        if (c >= 256) throw std::runtime_error("Ran out of colors!");
+        color[best] = c;
+        num_colors = std::max(num_colors, c + 1);
+        for (uint32_t e = off[best]; e < off[best + 1]; e++) {
+            const uint32_t w = nb[e];
+            if (color[w] != UNC) continue;
+            if (seen[w][c >> 6] >> (c & 63) & 1ull) continue;
+            heap.erase(Key(-(int64_t)sat[w], -(int64_t)(off[w + 1] - off[w]), w));
+            seen[w][c >> 6] |= 1ull << (c & 63);
+            sat[w]++;
+            heap.emplace(-(int64_t)sat[w], -(int64_t)(off[w + 1] - off[w]), w);
+        }
+    }
+    return num_colors;
+}
+
+namespace {
+
+struct SceneGraph {
+    std::vector<uint32_t> off, nb;
+};
+
+// Vertex-vertex adjacency of one scene (StaticParticleAdjacency::buildAdjacency,
+// common/StaticParticleAdjacency.hpp:42-133): deduplicated neighbour sets of the
+// static multi-particle energies.
+void scene_adjacency(uint32_t lo, uint32_t hi, const std::vector<HostNH>& nh,
+                     const std::vector<uint32_t>& tets_of_scene, SceneGraph& g) {
+    const uint32_t n = hi - lo;
+    std::vector<std::vector<uint32_t>> sets(n);
+    for (uint32_t t : tets_of_scene) {
+        const HostNH& r = nh[t];
+        for (int k = 0; k < 4; k++)
+            for (int k2 = 0; k2 < 4; k2++)
+                if (k != k2) sets[r.idx[k] - lo].push_back(r.idx[k2] - lo);
+    }
+    g.off.assign(n + 1, 0);
+    for (uint32_t v = 0; v < n; v++) {
+        auto& s = sets[v];
+        std::sort(s.begin(), s.end());
+        s.erase(std::unique(s.begin(), s.end()), s.end());
+        g.off[v + 1] = g.off[v] + (uint32_t)s.size();
+    }
+    g.nb.resize(g.off[n]);
+    for (uint32_t v = 0; v < n; v++) std::copy(sets[v].begin(), sets[v].end(), g.nb.begin() + g.off[v]);
+}
+
+}  // namespace
+
+void build_color_plan(uint32_t n, const std::vector<HostNH>& nh, const std::vector<HostGround>& ground,
+                      const std::vector<uint8_t>& fixed, const std::vector<uint32_t>& scene_offsets,
+                      ColorPlan& plan) {
+    const uint32_t n_scenes = (uint32_t)scene_offsets.size() - 1;
+    std::vector<uint32_t> scene_of(n, 0);
+    for (uint32_t s = 0; s < n_scenes; s++)
+        for (uint32_t v = scene_offsets[s]; v < scene_offsets[s + 1]; v++) scene_of[v] = s;
+
+    // tets per scene; every tet must lie inside one scene
+    std::vector<std::vector<uint32_t>> scene_tets(n_scenes);
+    for (uint32_t t = 0; t < nh.size(); t++) {
+        const uint32_t s = scene_of[nh[t].idx[0]];
+        for (int k = 0; k < 4; k++) {
+            if (nh[t].idx[k] >= n) throw std::runtime_error("tet references a particle out of range");
+            if (scene_of[nh[t].idx[k]] != s) throw std::runtime_error("tet spans two scenes");
+        }
+        scene_tets[s].push_back(t);
+    }
+
+    // colour every scene on its own (threads over scenes)
+    plan.color.assign(n, 0);
+    std::vector<uint32_t> scene_colors(n_scenes, 0);
+    std::vector<std::string> errors(n_scenes);
+    const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (uint32_t w = 0; w < hw; w++) {
+        pool.emplace_back([&, w] {
+            for (uint32_t s = w; s < n_scenes; s += hw) {
+                try {
+                    SceneGraph g;
+                    const uint32_t lo = scene_offsets[s], hi = scene_offsets[s + 1];
+                    scene_adjacency(lo, hi, nh, scene_tets[s], g);
+                    scene_colors[s] = dsatur_csr(hi - lo, g.off.data(), g.nb.data(), plan.color.data() + lo);
+                } catch (const std::exception& e) {
+                    errors[s] = e.what();
+                }
+            }
+        });
+    }
+    for (auto& t : pool) t.join();
+    for (uint32_t s = 0; s < n_scenes; s++)
+        if (!errors[s].empty()) throw std::runtime_error(errors[s]);
+    plan.num_colors = 0;
+    for (uint32_t s = 0; s < n_scenes; s++) plan.num_colors = std::max(plan.num_colors, scene_colors[s]);
+
+    // per-vertex NH incidences in pool order (ascending tet index, then local)
+    std::vector<uint32_t> inc_cnt(n + 1, 0);
+    for (const auto& t : nh)
+        for (int k = 0; k < 4; k++) inc_cnt[t.idx[k]]++;
+    std::vector<uint32_t> inc_off(n + 1, 0);
+    for (uint32_t v = 0; v < n; v++) inc_off[v + 1] = inc_off[v] + inc_cnt[v];
+    std::vector<uint32_t> inc_all(inc_off[n]);
+    std::vector<uint32_t> cur(inc_off.begin(), inc_off.end() - 1);
+    for (uint32_t t = 0; t < nh.size(); t++)
+        for (uint32_t k = 0; k < 4; k++) inc_all[cur[nh[t].idx[k]]++] = (t << 2) | k;
+
+    std::vector<int32_t> ground_of(n, -1);
+    for (uint32_t gi = 0; gi < ground.size(); gi++) {
+        const uint32_t p = ground[gi].p;
+        if (p >= n) throw std::runtime_error("ground energy references a particle out of range");
+        if (ground_of[p] >= 0) throw std::runtime_error("more than one ground energy on one particle");
+        ground_of[p] = (int32_t)gi;
+    }
+
+    // work items: colour-major, then scene order, then ascending vertex index
+    // (ColorList::rebuildWorkList order, common/ColorList.hpp:345-363); fixed
+    // particles are dropped (they return immediately, VBDSolver.hpp:288).
+    plan.items.clear();
+    plan.inc.clear();
+    plan.item_offsets.assign(plan.num_colors + 1, 0);
+    for (uint32_t c = 0; c < plan.num_colors; c++) {
+        plan.item_offsets[c] = (uint32_t)plan.items.size();
+        for (uint32_t v = 0; v < n; v++) {
+            if (plan.color[v] != c || fixed[v]) continue;
+            VItem it;
+            it.v = v;
+            it.inc_begin = (uint32_t)plan.inc.size();
+            it.inc_count = inc_cnt[v];
+            it.ground = ground_of[v];
+            plan.inc.insert(plan.inc.end(), inc_all.begin() + inc_off[v], inc_all.begin() + inc_off[v + 1]);
+            plan.items.push_back(it);
+        }
+    }
+    plan.item_offsets[plan.num_colors] = (uint32_t)plan.items.size();
+    plan.conflict_offsets.assign(plan.num_colors + 1, 0);
+    plan.conflict_list.clear();
+}
+
+}  // namespace avbd

@@ -1,0 +1,555 @@
+// vbd_kernels.hip -- gfx950 kernels of one AVBD time step.
+//
+// Phase structure of VBDSolver::solve_Parallel (simulation/VBDSolver.hpp:48-103),
+// one kernel per phase, every barrier of the reference becomes a kernel boundary:
+//   eprev        NeoHookeanEnergySolver::updateAfterTimeStep      (only with damping)
+//   step_begin   GroundCollisionEnergySolver::updateContactPoints +
+//                CollisionConstraintEnergySolver::updateAfterTimeStep +
+//                _particleInertialUpdate (adaptive initialisation), fused per particle
+//   solve_color  _solveParticle<3> for every vertex of one colour
+//   commit       _commitConflictedInColor
+//   cheb_dual    _particleChebyshevAcceleration + CollisionConstraintEnergySolver::updateAfterIteration
+//   velocity     _particleVelocityUpdate
+//
+// Vertex block solve: a group of L lanes (L | 64, groups never straddle a
+// wavefront) owns one vertex; lane j handles incidences j, j+L, ...; the 3
+// gradient and 6 (symmetric) Hessian partial sums are folded with
+// cross-lane xor shuffles; the group leader adds the ground AL terms,
+// solves the 3x3 system by LDL^T in registers and writes the vertex.
+#include "internal.hpp"
+
+namespace avbd {
+
+namespace {
+
+constexpr float kAlpha = 0.95f;   // CONSTRAINT_ALPHA, common/common.hpp:15
+constexpr float kBeta = 10.0f;    // STIFFNESS_BETA,   common/common.hpp:13
+constexpr float kGamma = 0.99f;   // STIFFNESS_GAMMA,  common/common.hpp:14
+
+struct Sym3 {
+    float a00, a11, a22, a01, a02, a12;
+};
+
+__device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator*(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Load a 64-byte tet record with four 16-byte loads.
+__device__ __forceinline__ void load_tet(const TetRec* __restrict__ tets, uint32_t t, uint4& idx, float4& r1,
+                                         float4& r2, float4& r3) {
+    const uint4* p = reinterpret_cast<const uint4*>(tets + t);
+    idx = p[0];
+    const float4* f = reinterpret_cast<const float4*>(p);
+    r1 = f[1];
+    r2 = f[2];
+    r3 = f[3];
+}
+
+// One tet incidence of the stable Neo-Hookean energy
+// (NeoHookeanEnergySolver::accumulate, energy/elastic/NeoHookeanEnergySolver.hpp:121-227).
+// q = Q.row(local) for local < 3, -(Q.row0 + Q.row1 + Q.row2) for local 3 (the
+// reference's sign = -1 folded into q; every term is linear in q or quadratic).
+template <bool DAMP>
+__device__ __forceinline__ void nh_incidence(const float4* x, const TetRec* __restrict__ tets,
+                                             const TetDamp* __restrict__ damp, uint32_t e, float inv_dt,
+                                             float3& g, Sym3& h) {
+    const uint32_t t = e >> 2, local = e & 3u;
+    uint4 idx;
+    float4 r1, r2, r3;
+    load_tet(tets, t, idx, r1, r2, r3);
+    // r1 = q00 q01 q02 q10 ; r2 = q11 q12 q20 q21 ; r3 = q22 VL VM alpha
+    const float3 p0 = f3(x[idx.x]), p1 = f3(x[idx.y]), p2 = f3(x[idx.z]), p3 = f3(x[idx.w]);
+    const float3 d0 = p0 - p3, d1 = p1 - p3, d2 = p2 - p3;
+    const float q00 = r1.x, q01 = r1.y, q02 = r1.z, q10 = r1.w, q11 = r2.x, q12 = r2.y, q20 = r2.z, q21 = r2.w,
+                q22 = r3.x;
+    const float VL = r3.y, VM = r3.z, alpha = r3.w;
+    // F = D Q: column c of F = d0 Q[0][c] + d1 Q[1][c] + d2 Q[2][c]
+    const float3 f0 = q00 * d0 + q10 * d1 + q20 * d2;
+    const float3 f1 = q01 * d0 + q11 * d1 + q21 * d2;
+    const float3 f2 = q02 * d0 + q12 * d1 + q22 * d2;
+    const float3 c0 = cross3(f1, f2), c1 = cross3(f2, f0), c2 = cross3(f0, f1);
+    const float J = dot3(f0, c0);
+    float3 q;
+    if (local == 0) q = make_float3(q00, q01, q02);
+    else if (local == 1) q = make_float3(q10, q11, q12);
+    else if (local == 2) q = make_float3(q20, q21, q22);
+    else q = make_float3(-(q00 + q10 + q20), -(q01 + q11 + q21), -(q02 + q12 + q22));
+    const float3 Fq = q.x * f0 + q.y * f1 + q.z * f2;
+    const float3 Fc = q.x * c0 + q.y * c1 + q.z * c2;
+    const float hyd = VL * (J - alpha);
+    const float q2 = dot3(q, q);
+    g = g + (hyd * Fc + VM * Fq);
+    const float dev = VM * q2;
+    h.a00 += VL * Fc.x * Fc.x + dev;
+    h.a11 += VL * Fc.y * Fc.y + dev;
+    h.a22 += VL * Fc.z * Fc.z + dev;
+    h.a01 += VL * Fc.x * Fc.y;
+    h.a02 += VL * Fc.x * Fc.z;
+    h.a12 += VL * Fc.y * Fc.z;
+    if constexpr (DAMP) {
+        const TetDamp dr = damp[t];
+        const float dm = dr.damp;
+        // Edot = (0.5 F^T F - 0.5 I - E_prev) / dt, symmetric
+        const float e00 = (0.5f * dot3(f0, f0) - 0.5f - dr.e00) * inv_dt;
+        const float e11 = (0.5f * dot3(f1, f1) - 0.5f - dr.e11) * inv_dt;
+        const float e22 = (0.5f * dot3(f2, f2) - 0.5f - dr.e22) * inv_dt;
+        const float e01 = (0.5f * dot3(f0, f1) - dr.e01) * inv_dt;
+        const float e02 = (0.5f * dot3(f0, f2) - dr.e02) * inv_dt;
+        const float e12 = (0.5f * dot3(f1, f2) - dr.e12) * inv_dt;
+        const float3 Eq = make_float3(e00 * q.x + e01 * q.y + e02 * q.z, e01 * q.x + e11 * q.y + e12 * q.z,
+                                      e02 * q.x + e12 * q.y + e22 * q.z);
+        const float3 FEq = Eq.x * f0 + Eq.y * f1 + Eq.z * f2;
+        g = g + dm * FEq;
+        const float diag = dm * dot3(q, Eq);
+        const float a = 0.5f * dm * inv_dt;
+        // F F^T rows: (F F^T)_ij = sum_c F_ic F_jc
+        const float ff00 = f0.x * f0.x + f1.x * f1.x + f2.x * f2.x;
+        const float ff11 = f0.y * f0.y + f1.y * f1.y + f2.y * f2.y;
+        const float ff22 = f0.z * f0.z + f1.z * f1.z + f2.z * f2.z;
+        const float ff01 = f0.x * f0.y + f1.x * f1.y + f2.x * f2.y;
+        const float ff02 = f0.x * f0.z + f1.x * f1.z + f2.x * f2.z;
+        const float ff12 = f0.y * f0.z + f1.y * f1.z + f2.y * f2.z;
+        h.a00 += diag + a * (q2 * ff00 + Fq.x * Fq.x);
+        h.a11 += diag + a * (q2 * ff11 + Fq.y * Fq.y);
+        h.a22 += diag + a * (q2 * ff22 + Fq.z * Fq.z);
+        h.a01 += a * (q2 * ff01 + Fq.x * Fq.y);
+        h.a02 += a * (q2 * ff02 + Fq.x * Fq.z);
+        h.a12 += a * (q2 * ff12 + Fq.y * Fq.z);
+    }
+}
+
+// Ground AL contact contribution (CollisionConstraintEnergySolver::accumulate<3>,
+// energy/collision/CollisionConstraintEnergySolver.hpp:180-285 with the ground
+// constraint of energy/collision/GroundCollisionEnergySolver.hpp:27-60).
+__device__ __forceinline__ void ground_accumulate(const GroundRec& c, float3 p, float3& g, Sym3& h) {
+    const float Cn = -p.y, Ct = p.x - c.cp_x, Cb = p.z - c.cp_z;
+    const float Ccorr = Cn - kAlpha * c.C_n_prev;
+    const float lnp = fmaxf(0.0f, c.k_n * Ccorr + c.lambda_n);
+    float lt = c.k_t * Ct + c.lambda_t;
+    float lb = c.k_b * Cb + c.lambda_b;
+    const float mag = sqrtf(lt * lt + lb * lb);
+    const float mu = (c.flags & GF_USE_STATIC) ? c.mu_s : c.mu_k;
+    const float lmax = mu * lnp;
+    if (mag > lmax) {
+        const float nrm = mag + 1e-8f;
+        lt = lt / nrm * lmax;
+        lb = lb / nrm * lmax;
+    }
+    g.x += lt;
+    g.y += -lnp;
+    g.z += lb;
+    h.a00 += c.k_t;
+    h.a11 += c.k_n;
+    h.a22 += c.k_b;
+}
+
+// Solve (m/dt^2 I + H) dx = b for symmetric positive definite H by LDL^T.
+__device__ __forceinline__ float3 ldlt3(const Sym3& A, float3 b) {
+    const float d0 = A.a00;
+    const float l10 = A.a01 / d0, l20 = A.a02 / d0;
+    const float d1 = A.a11 - l10 * A.a01;
+    const float l21 = (A.a12 - l20 * A.a01) / d1;
+    const float d2 = A.a22 - l20 * A.a02 - l21 * l21 * d1;
+    const float z0 = b.x;
+    const float z1 = b.y - l10 * z0;
+    const float z2 = b.z - l20 * z0 - l21 * z1;
+    const float x2 = z2 / d2;
+    const float x1 = z1 / d1 - l21 * x2;
+    const float x0 = z0 / d0 - l10 * x1 - l20 * x2;
+    return make_float3(x0, x1, x2);
+}
+
+template <int L>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <int L, bool DAMP>
+__global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restrict__ items, uint32_t n_items,
+                                                           const uint32_t* __restrict__ inc,
+                                                           const TetRec* __restrict__ tets,
+                                                           const TetDamp* __restrict__ damp, float4* x,
+                                                           const float4* __restrict__ y,
+                                                           const GroundRec* __restrict__ ground, float4* x_out,
+                                                           float4* x_buf, float inv_dt, float inv_dt2) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = tid / L;
+    const uint32_t lane = tid % L;
+    if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)
+    const VItem it = items[item];
+    const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
+    float3 g = make_float3(0.f, 0.f, 0.f);
+    Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t j = lane; j < cnt; j += L) nh_incidence<DAMP>(x, tets, damp, inc[it.inc_begin + j], inv_dt, g, h);
+    g.x = group_sum<L>(g.x);
+    g.y = group_sum<L>(g.y);
+    g.z = group_sum<L>(g.z);
+    h.a00 = group_sum<L>(h.a00);
+    h.a11 = group_sum<L>(h.a11);
+    h.a22 = group_sum<L>(h.a22);
+    h.a01 = group_sum<L>(h.a01);
+    h.a02 = group_sum<L>(h.a02);
+    h.a12 = group_sum<L>(h.a12);
+    if (lane != 0) return;
+    const float4 xv = x[it.v];
+    const float4 yv = y[it.v];
+    const float3 p = f3(xv);
+    if (it.ground >= 0) ground_accumulate(ground[it.ground], p, g, h);
+    const float md = xv.w * inv_dt2;   // m / dt^2
+    h.a00 += md;
+    h.a11 += md;
+    h.a22 += md;
+    const float3 b = make_float3(-md * (p.x - yv.x) - g.x, -md * (p.y - yv.y) - g.y, -md * (p.z - yv.z) - g.z);
+    const float3 dx = ldlt3(h, b);
+    const float4 nx = make_float4(p.x + dx.x, p.y + dx.y, p.z + dx.z, xv.w);
+    if (it.inc_count & VITEM_CONFLICT) x_buf[it.v] = nx;
+    else x_out[it.v] = nx;
+}
+
+__global__ void eprev_kernel(const TetRec* __restrict__ tets, TetDamp* __restrict__ damp,
+                             const float4* __restrict__ x_prev, uint32_t n_tets) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tets) return;
+    uint4 idx;
+    float4 r1, r2, r3;
+    load_tet(tets, t, idx, r1, r2, r3);
+    const float3 p0 = f3(x_prev[idx.x]), p1 = f3(x_prev[idx.y]), p2 = f3(x_prev[idx.z]), p3 = f3(x_prev[idx.w]);
+    const float3 d0 = p0 - p3, d1 = p1 - p3, d2 = p2 - p3;
+    const float3 f0 = r1.x * d0 + r1.w * d1 + r2.z * d2;
+    const float3 f1 = r1.y * d0 + r2.x * d1 + r2.w * d2;
+    const float3 f2 = r1.z * d0 + r2.y * d1 + r3.x * d2;
+    TetDamp dr = damp[t];
+    dr.e00 = 0.5f * (dot3(f0, f0) - 1.0f);
+    dr.e11 = 0.5f * (dot3(f1, f1) - 1.0f);
+    dr.e22 = 0.5f * (dot3(f2, f2) - 1.0f);
+    dr.e01 = 0.5f * dot3(f0, f1);
+    dr.e02 = 0.5f * dot3(f0, f2);
+    dr.e12 = 0.5f * dot3(f1, f2);
+    damp[t] = dr;
+}
+
+__global__ void step_begin_kernel(float4* __restrict__ x, float4* __restrict__ y, float4* __restrict__ x_last,
+                                  float4* __restrict__ x_lastlast, const float4* __restrict__ v,
+                                  const float4* __restrict__ v_prev, const uint8_t* __restrict__ fixed,
+                                  const int32_t* __restrict__ ground_of, GroundRec* __restrict__ ground,
+                                  uint32_t n, float dt, float g) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 xp = x[p];
+    const int32_t gi = ground_of[p];
+    if (gi >= 0) {
+        GroundRec c = ground[gi];
+        // updateContactPoints (GroundCollisionEnergySolver.hpp:81-91)
+        c.cp_x = xp.x;
+        c.cp_z = xp.z;
+        // updateAfterTimeStep (CollisionConstraintEnergySolver.hpp:141-167)
+        if (c.lambda_n > 0.f) c.C_n_prev = -xp.y;
+        if ((c.flags & GF_UNCLAMPED) && !(c.flags & GF_USE_STATIC)) c.flags |= GF_USE_STATIC;
+        c.lambda_n *= kAlpha * kGamma;
+        c.lambda_t *= kGamma;
+        c.lambda_b *= kGamma;
+        c.k_n = fmaxf(kGamma * c.k_n, c.k_start);
+        c.k_t = fmaxf(kGamma * c.k_t, c.k_start);
+        c.k_b = fmaxf(kGamma * c.k_b, c.k_start);
+        ground[gi] = c;
+    }
+    if (fixed[p]) return;
+    // _particleInertialUpdate (VBDSolver.hpp:208-248): gravity (0,-g,0)
+    const float4 vv = v[p], vp = v_prev[p];
+    const float ay = (vv.y - vp.y) / dt;
+    float a_tilde = 0.f;
+    if (g > 1e-12f) {
+        const float along = (ay * -g) / g;   // a . a_ext / |a_ext|
+        if (along > g) a_tilde = 1.f;
+        else if (along < 0.f) a_tilde = 0.f;
+        else a_tilde = along / g;
+    }
+    const float dt2 = dt * dt;
+    y[p] = make_float4(xp.x + dt * vv.x, xp.y + dt * vv.y + dt2 * -g, xp.z + dt * vv.z, 0.f);
+    xp.x = xp.x + dt * vv.x;
+    xp.y = xp.y + (dt * vv.y + dt2 * (a_tilde * -g));
+    xp.z = xp.z + dt * vv.z;
+    x[p] = xp;
+    x_last[p] = xp;
+    x_lastlast[p] = xp;
+}
+
+__global__ void commit_kernel(float4* __restrict__ x, const float4* __restrict__ x_buf,
+                              const uint32_t* __restrict__ list, uint32_t count) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t p = list[k];
+    x[p] = x_buf[p];
+}
+
+__global__ void cheb_dual_kernel(float4* __restrict__ x, float4* __restrict__ x_last, float4* __restrict__ x_lastlast,
+                                 const int32_t* __restrict__ ground_of, GroundRec* __restrict__ ground, uint32_t n,
+                                 float omega) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 xp = x[p];
+    const float4 xl = x_last[p], xll = x_lastlast[p];
+    // _particleChebyshevAcceleration (VBDSolver.hpp:513-536)
+    xp.x = omega * (xp.x - xll.x) + xll.x;
+    xp.y = omega * (xp.y - xll.y) + xll.y;
+    xp.z = omega * (xp.z - xll.z) + xll.z;
+    x[p] = xp;
+    x_lastlast[p] = xl;
+    x_last[p] = xp;
+    const int32_t gi = ground_of[p];
+    if (gi < 0) return;
+    // CollisionConstraintEnergySolver::updateAfterIteration (CollisionConstraintEnergySolver.hpp:60-134)
+    GroundRec c = ground[gi];
+    const float Cn = -xp.y, Ct = xp.x - c.cp_x, Cb = xp.z - c.cp_z;
+    const float Ccorr = Cn - kAlpha * c.C_n_prev;
+    const float lnp = c.k_n * Ccorr + c.lambda_n;
+    const float ltp = c.k_t * Ct + c.lambda_t;
+    const float lbp = c.k_b * Cb + c.lambda_b;
+    if (lnp > 0.f) c.k_n += kBeta * fabsf(Ccorr);
+    c.lambda_n = fmaxf(0.f, lnp);
+    const float mag = sqrtf(ltp * ltp + lbp * lbp);
+    float lmax = ((c.flags & GF_USE_STATIC) ? c.mu_s : c.mu_k) * c.lambda_n;
+    if (mag < lmax) {
+        c.k_t += kBeta * fabsf(Ct);
+        c.k_b += kBeta * fabsf(Cb);
+        c.lambda_t = ltp;
+        c.lambda_b = lbp;
+        c.flags |= GF_UNCLAMPED;
+    } else {
+        if (c.flags & GF_USE_STATIC) {
+            c.flags &= ~GF_USE_STATIC;
+            lmax = c.mu_k * lnp;
+        }
+        c.lambda_t = ltp / (mag + 1e-8f) * lmax;
+        c.lambda_b = lbp / (mag + 1e-8f) * lmax;
+        c.flags &= ~GF_UNCLAMPED;
+    }
+    ground[gi] = c;
+}
+
+__global__ void velocity_kernel(const float4* __restrict__ x, float4* __restrict__ x_prev, float4* __restrict__ v,
+                                float4* __restrict__ v_prev, uint32_t n, float inv_dt) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float4 xp = x[p], xo = x_prev[p], vv = v[p];
+    v_prev[p] = vv;
+    v[p] = make_float4((xp.x - xo.x) * inv_dt, (xp.y - xo.y) * inv_dt, (xp.z - xo.z) * inv_dt, 0.f);
+    x_prev[p] = xp;
+}
+
+// ---- residual / incremental potential (new; see avbd_get_stats) ----------
+// Per free vertex: r = m/dt^2 (x - y) + G(x); partial sums of |r|^2, |m g|^2,
+// and the inertia term m/(2 dt^2)|x-y|^2 per block.
+template <int L, bool DAMP>
+__global__ __launch_bounds__(1024) void stats_items_kernel(const VItem* __restrict__ items, uint32_t n_items,
+                                                           const uint32_t* __restrict__ inc,
+                                                           const TetRec* __restrict__ tets,
+                                                           const TetDamp* __restrict__ damp,
+                                                           const float4* __restrict__ x,
+                                                           const float4* __restrict__ y,
+                                                           const GroundRec* __restrict__ ground,
+                                                           double* __restrict__ out, float inv_dt, float inv_dt2,
+                                                           float g_acc) {
+    __shared__ double red[3][1024 / L];
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = tid / L;
+    const uint32_t lane = tid % L;
+    const uint32_t grp = threadIdx.x / L;
+    double r2 = 0, mg2 = 0, ip = 0;
+    if (item < n_items) {
+        const VItem it = items[item];
+        const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
+        float3 g = make_float3(0.f, 0.f, 0.f);
+        Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (uint32_t j = lane; j < cnt; j += L)
+            nh_incidence<DAMP>(x, tets, damp, inc[it.inc_begin + j], inv_dt, g, h);
+        g.x = group_sum<L>(g.x);
+        g.y = group_sum<L>(g.y);
+        g.z = group_sum<L>(g.z);
+        if (lane == 0) {
+            const float4 xv = x[it.v], yv = y[it.v];
+            if (it.ground >= 0) ground_accumulate(ground[it.ground], f3(xv), g, h);
+            const double md = (double)xv.w * (double)inv_dt2;
+            const double dx = (double)xv.x - yv.x, dy = (double)xv.y - yv.y, dz = (double)xv.z - yv.z;
+            const double rx = md * dx + g.x, ry = md * dy + g.y, rz = md * dz + g.z;
+            r2 = rx * rx + ry * ry + rz * rz;
+            const double mg = (double)xv.w * g_acc;
+            mg2 = mg * mg;
+            ip = 0.5 * md * (dx * dx + dy * dy + dz * dz);
+        }
+    } else {
+        (void)lane;
+    }
+    if (lane == 0) {
+        red[0][grp] = r2;
+        red[1][grp] = mg2;
+        red[2][grp] = ip;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double s = 0;
+        for (uint32_t k = 0; k < blockDim.x / L; k++) s += red[threadIdx.x][k];
+        out[blockIdx.x * 4 + threadIdx.x] = s;
+    }
+}
+
+// Energy of every tet (NeoHookeanEnergySolver::energy, NeoHookeanEnergySolver.hpp:20-53)
+// and ground record (AL form 1/2 k C^2 + lambda C) summed per block into out[b*4+3].
+template <bool DAMP>
+__global__ void stats_energy_kernel(const TetRec* __restrict__ tets, const TetDamp* __restrict__ damp,
+                                    uint32_t n_tets, const GroundRec* __restrict__ ground, uint32_t n_ground,
+                                    const float4* __restrict__ x, double* __restrict__ out, float inv_dt) {
+    __shared__ double red[256];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0;
+    if (k < n_tets) {
+        uint4 idx;
+        float4 r1, r2, r3;
+        load_tet(tets, k, idx, r1, r2, r3);
+        const float3 p0 = f3(x[idx.x]), p1 = f3(x[idx.y]), p2 = f3(x[idx.z]), p3 = f3(x[idx.w]);
+        const float3 d0 = p0 - p3, d1 = p1 - p3, d2 = p2 - p3;
+        const float3 f0 = r1.x * d0 + r1.w * d1 + r2.z * d2;
+        const float3 f1 = r1.y * d0 + r2.x * d1 + r2.w * d2;
+        const float3 f2 = r1.z * d0 + r2.y * d1 + r3.x * d2;
+        const double J = dot3(f0, cross3(f1, f2));
+        const double VL = r3.y, VM = r3.z, al = r3.w;
+        const double trc = (double)dot3(f0, f0) + dot3(f1, f1) + dot3(f2, f2);
+        e = 0.5 * VL * (J - al) * (J - al) + 0.5 * VM * (trc - 3.0);
+        if constexpr (DAMP) {
+            const TetDamp dr = damp[k];
+            const double E[6] = {0.5 * (dot3(f0, f0) - 1.0), 0.5 * (dot3(f1, f1) - 1.0), 0.5 * (dot3(f2, f2) - 1.0),
+                                 0.5 * dot3(f0, f1), 0.5 * dot3(f0, f2), 0.5 * dot3(f1, f2)};
+            const double P[6] = {dr.e00, dr.e11, dr.e22, dr.e01, dr.e02, dr.e12};
+            double s = 0;
+            for (int i = 0; i < 6; i++) {
+                const double d = (E[i] - P[i]) * inv_dt;
+                s += (i < 3 ? 1.0 : 2.0) * d * d;
+            }
+            e += 0.5 * dr.damp / inv_dt * s;   // V kd |Edot|^2, damp = 2 V kd / dt
+        }
+    } else if (k < n_tets + n_ground) {
+        const GroundRec c = ground[k - n_tets];
+        const float4 xp = x[c.p];
+        const double Cn = -xp.y, Ct = (double)xp.x - c.cp_x, Cb = (double)xp.z - c.cp_z;
+        const double Cc = Cn - (double)kAlpha * c.C_n_prev;
+        if ((double)c.k_n * Cc + c.lambda_n > 0) e += 0.5 * c.k_n * Cc * Cc + (double)c.lambda_n * Cc;
+        e += 0.5 * c.k_t * Ct * Ct + (double)c.lambda_t * Ct + 0.5 * c.k_b * Cb * Cb + (double)c.lambda_b * Cb;
+    }
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x * 4 + 3] = red[0];
+}
+
+inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+template <int L, bool DAMP>
+void solve_dispatch(hipStream_t s, const DevState& d, uint32_t block, uint32_t begin, uint32_t count, float dt,
+                    float4* x_out) {
+    const uint32_t threads = count * L;
+    hipLaunchKernelGGL((solve_color_kernel<L, DAMP>), dim3(blocks_for(threads, block)), dim3(block), 0, s,
+                       d.items + begin, count, d.inc, d.tets, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
+                       1.0f / dt, 1.0f / (dt * dt));
+}
+
+template <int L, bool DAMP>
+uint32_t stats_dispatch(hipStream_t s, const DevState& d, uint32_t block, float dt, float g) {
+    const uint32_t threads = d.n_items * L;
+    const uint32_t nb = blocks_for(threads, block);
+    if (nb == 0) return 0;
+    hipLaunchKernelGGL((stats_items_kernel<L, DAMP>), dim3(nb), dim3(block), 0, s, d.items, d.n_items, d.inc,
+                       d.tets, d.damp, d.x, d.y, d.ground, d.reduce, 1.0f / dt, 1.0f / (dt * dt), g);
+    return nb;
+}
+
+}  // namespace
+
+void launch_eprev(hipStream_t s, const DevState& d, float) {
+    if (d.n_tets == 0) return;
+    hipLaunchKernelGGL(eprev_kernel, dim3(blocks_for(d.n_tets, 256)), dim3(256), 0, s, d.tets, d.damp, d.x_prev,
+                       d.n_tets);
+}
+
+void launch_step_begin(hipStream_t s, const DevState& d, float dt, float g) {
+    if (d.n == 0) return;
+    hipLaunchKernelGGL(step_begin_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d.x, d.y, d.x_last,
+                       d.x_lastlast, d.v, d.v_prev, d.fixed, d.ground_of, d.ground, d.n, dt, g);
+}
+
+void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
+                        bool damping, float dt, float4* x_out) {
+    if (count == 0) return;
+#define AVBD_SOLVE_CASE(LL)                                                         \
+    case LL:                                                                         \
+        if (damping) solve_dispatch<LL, true>(s, d, cfg.block, begin, count, dt, x_out); \
+        else solve_dispatch<LL, false>(s, d, cfg.block, begin, count, dt, x_out);      \
+        break;
+    switch (cfg.lanes) {
+        AVBD_SOLVE_CASE(4)
+        AVBD_SOLVE_CASE(8)
+        AVBD_SOLVE_CASE(16)
+        AVBD_SOLVE_CASE(32)
+        default: break;
+    }
+#undef AVBD_SOLVE_CASE
+}
+
+void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(commit_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d.x, d.x_buf, list, count);
+}
+
+void launch_cheb_dual(hipStream_t s, const DevState& d, float omega) {
+    if (d.n == 0) return;
+    hipLaunchKernelGGL(cheb_dual_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d.x, d.x_last, d.x_lastlast,
+                       d.ground_of, d.ground, d.n, omega);
+}
+
+void launch_velocity(hipStream_t s, const DevState& d, float dt) {
+    if (d.n == 0) return;
+    hipLaunchKernelGGL(velocity_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d.x, d.x_prev, d.v, d.v_prev,
+                       d.n, 1.0f / dt);
+}
+
+uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt, float g) {
+    // the items pass writes slots 0..2 of its blocks; the energy pass slot 3 of its
+    // blocks; both grids are sized so the larger one is returned and the host
+    // zero-fills the rest before launching (see the caller).
+    uint32_t nb_items = 0;
+#define AVBD_STATS_CASE(LL)                                                                   \
+    case LL:                                                                                   \
+        nb_items = damping ? stats_dispatch<LL, true>(s, d, cfg.block, dt, g)                  \
+                           : stats_dispatch<LL, false>(s, d, cfg.block, dt, g);                \
+        break;
+    switch (cfg.lanes) {
+        AVBD_STATS_CASE(4)
+        AVBD_STATS_CASE(8)
+        AVBD_STATS_CASE(16)
+        AVBD_STATS_CASE(32)
+        default: break;
+    }
+#undef AVBD_STATS_CASE
+    const uint32_t ne = d.n_tets + d.n_ground;
+    const uint32_t nb_e = blocks_for(ne, 256);
+    if (ne) {
+        if (damping)
+            hipLaunchKernelGGL(stats_energy_kernel<true>, dim3(nb_e), dim3(256), 0, s, d.tets, d.damp, d.n_tets,
+                               d.ground, d.n_ground, d.x, d.reduce, 1.0f / dt);
+        else
+            hipLaunchKernelGGL(stats_energy_kernel<false>, dim3(nb_e), dim3(256), 0, s, d.tets, d.damp, d.n_tets,
+                               d.ground, d.n_ground, d.x, d.reduce, 1.0f / dt);
+    }
+    return nb_items > nb_e ? nb_items : nb_e;
+}
+
+}  // namespace avbd

@@ -1,0 +1,212 @@
+"""Synthetic scene generators for the solver's workloads (host-side, numpy).
+
+Axis convention is the reference's: y-up, gravity (0, -g, 0)
+(simulation/VBDSolver.hpp:51), ground plane y = 0
+(energy/collision/GroundCollisionEnergySolver.hpp:22-24).
+
+Workloads:
+  * ``hanging_cube``  Kuhn 8^3 cube, top face fixed, jittered, no contact
+  * ``dropped_cube``  Kuhn 32^3 cube, random rotation, lowest point at 0.5 m, ground AL
+  * ``cube_batch``    512 independent Kuhn cubes (n ~ U{4..16}) dropped on the ground
+All randomness comes from SplitMix64 so the inputs are reproducible bit for bit.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+_M64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014); uniform() uses the top 53 bits."""
+
+    def __init__(self, seed: int):
+        self.state = seed & _M64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _M64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+    def uniforms(self, n: int) -> np.ndarray:
+        return np.array([self.uniform() for _ in range(n)], dtype=np.float64)
+
+    def randint(self, lo: int, hi: int) -> int:
+        """Uniform integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1))
+
+
+def kuhn_cube(n: int, edge: float = 1.0):
+    """Kuhn (Freudenthal) triangulation of an n^3 grid of cubes with side `edge`.
+
+    Vertex (i, j, k) sits at (i, j, k) * edge/n with index (i*(n+1) + j)*(n+1) + k.
+    Each cell is split into 6 tets along its main diagonal, one per permutation
+    of the axes: v0 = corner, v1 = v0 + e_a, v2 = v1 + e_b, v3 = far corner.
+    Returns (vertices float64 [(n+1)^3, 3], tets uint32 [6 n^3, 4]).
+    """
+    m = n + 1
+    h = edge / n
+    ii, jj, kk = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    verts = np.stack([ii, jj, kk], axis=-1).reshape(-1, 3).astype(np.float64) * h
+
+    def vid(i, j, k):
+        return (i * m + j) * m + k
+
+    ci, cj, ck = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    out = np.empty((ci.size, 6, 4), dtype=np.uint32)
+    for pi, perm in enumerate(perms):
+        cur = [ci.copy(), cj.copy(), ck.copy()]
+        ids = [vid(*cur)]
+        for ax in perm:
+            cur[ax] = cur[ax] + 1
+            ids.append(vid(*cur))
+        out[:, pi, :] = np.stack(ids, axis=-1)
+    return verts, out.reshape(-1, 4)
+
+
+def surface_triangles(n: int) -> np.ndarray:
+    """Boundary triangles of the Kuhn cube (2 per boundary cell face): 12 n^2."""
+    m = n + 1
+
+    def vid(i, j, k):
+        return (i * m + j) * m + k
+
+    tris = []
+    for axis in range(3):
+        for side in (0, n):
+            a, b = [ax for ax in range(3) if ax != axis]
+            for u in range(n):
+                for w in range(n):
+                    c = [0, 0, 0]
+                    c[axis] = side
+                    quad = []
+                    for du, dw in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        cc = list(c)
+                        cc[a] = u + du
+                        cc[b] = w + dw
+                        quad.append(vid(*cc))
+                    tris.append((quad[0], quad[1], quad[2]))
+                    tris.append((quad[0], quad[2], quad[3]))
+    return np.asarray(tris, dtype=np.uint32)
+
+
+def random_rotation(rng: SplitMix64) -> np.ndarray:
+    """Rotation about an axis uniform on S^2 by an angle uniform in [0, pi]."""
+    z = 1.0 - 2.0 * rng.uniform()
+    phi = 2.0 * np.pi * rng.uniform()
+    r = np.sqrt(max(0.0, 1.0 - z * z))
+    axis = np.array([r * np.cos(phi), r * np.sin(phi), z])
+    ang = np.pi * rng.uniform()
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * (K @ K)
+
+
+@dataclass
+class TetMesh:
+    """One tet-mesh object: rest state plus material (TetMeshObjectConfig fields,
+    config/TetMeshObjectConfig.hpp:18-32) and its initial state."""
+    rest: np.ndarray            # [nv, 3] rest positions (already placed in the world)
+    tets: np.ndarray            # [nt, 4] uint32
+    E: float = 1e6
+    nu: float = 0.4
+    density: float = 1000.0
+    kd: float = 0.0
+    ground: bool = False
+    mu_s: float = 0.4
+    mu_k: float = 0.2
+    x0: Optional[np.ndarray] = None      # initial positions (default: rest)
+    v0: Optional[np.ndarray] = None
+    fixed: Optional[np.ndarray] = None   # uint8 [nv]
+    triangles: Optional[np.ndarray] = None
+
+
+@dataclass
+class Scene:
+    """A batch of independent scenes sharing one solver configuration."""
+    name: str
+    meshes: List[TetMesh] = field(default_factory=list)
+    dt: float = 1.0 / 60.0
+    g: float = 9.81
+    iters: int = 20
+    rho: float = 0.95
+    steps: int = 100
+
+    @property
+    def n_vertices(self) -> int:
+        return int(sum(m.rest.shape[0] for m in self.meshes))
+
+    @property
+    def n_tets(self) -> int:
+        return int(sum(m.tets.shape[0] for m in self.meshes))
+
+    @property
+    def n_ground(self) -> int:
+        return int(sum(m.rest.shape[0] for m in self.meshes if m.ground))
+
+
+def hanging_cube(n: int = 8, jitter: float = 1e-3, seed: int = 1, iters: int = 20, steps: int = 100) -> Scene:
+    """1 m Kuhn cube, top face (y = 1) fixed, E = 1e5, nu = 0.45, rho = 1000,
+    no contact; free vertices jittered by U(-jitter, jitter) per component
+    (SplitMix64(seed), vertex-index order) in both x and x_prev; v = 0."""
+    verts, tets = kuhn_cube(n)
+    fixed = (np.abs(verts[:, 1] - 1.0) < 1e-12).astype(np.uint8)
+    rng = SplitMix64(seed)
+    x0 = verts.copy()
+    for i in range(verts.shape[0]):
+        if fixed[i]:
+            continue
+        for k in range(3):
+            x0[i, k] += (2.0 * rng.uniform() - 1.0) * jitter
+    mesh = TetMesh(rest=verts, tets=tets, E=1e5, nu=0.45, density=1000.0, kd=0.0, ground=False,
+                   x0=x0, fixed=fixed, triangles=surface_triangles(n))
+    return Scene(name=f"hanging_cube_{n}", meshes=[mesh], iters=iters, steps=steps)
+
+
+def dropped_cube(n: int = 32, seed: int = 2, height: float = 0.5, E: float = 1e6, nu: float = 0.4,
+                 iters: int = 10, steps: int = 200) -> Scene:
+    """1 m Kuhn cube rotated about its centre by a random rotation, translated so
+    its lowest point is at y = height, with ground contact on every vertex."""
+    verts, tets = kuhn_cube(n)
+    R = random_rotation(SplitMix64(seed))
+    c = verts.mean(axis=0)
+    p = (verts - c) @ R.T
+    p[:, 1] += height - p[:, 1].min()
+    mesh = TetMesh(rest=p, tets=tets, E=E, nu=nu, density=1000.0, kd=0.0, ground=True,
+                   triangles=surface_triangles(n))
+    return Scene(name=f"dropped_cube_{n}", meshes=[mesh], iters=iters, steps=steps)
+
+
+def cube_batch(n_scenes: int = 512, seed: int = 4, n_min: int = 4, n_max: int = 16, iters: int = 10,
+               steps: int = 50) -> Scene:
+    """Independent Kuhn cubes: n ~ U{n_min..n_max}, E log-uniform [1e4, 1e6],
+    nu ~ U[0.3, 0.48], random rotation, lowest point at U[0.2, 1.0] m, ground
+    contact.  Draw order per scene: n, E, nu, rotation (3 draws), height."""
+    rng = SplitMix64(seed)
+    sc = Scene(name=f"cube_batch_{n_scenes}", iters=iters, steps=steps)
+    for _ in range(n_scenes):
+        n = rng.randint(n_min, n_max)
+        E = 10.0 ** (4.0 + 2.0 * rng.uniform())
+        nu = 0.3 + 0.18 * rng.uniform()
+        R = random_rotation(rng)
+        h = 0.2 + 0.8 * rng.uniform()
+        verts, tets = kuhn_cube(n)
+        p = (verts - verts.mean(axis=0)) @ R.T
+        p[:, 1] += h - p[:, 1].min()
+        sc.meshes.append(TetMesh(rest=p, tets=tets, E=E, nu=nu, density=1000.0, kd=0.0, ground=True))
+    return sc
+
+
+def algorithmic_bytes_per_step(n_v: int, n_tet: int, n_g: int, iters: int) -> int:
+    """FP32 gather model of one time step:
+    I*(112 N_v + 576 N_tet + 136 N_g) + 156 N_v + 136 N_tet + 100 N_g."""
+    return iters * (112 * n_v + 576 * n_tet + 136 * n_g) + 156 * n_v + 136 * n_tet + 100 * n_g

@@ -1,0 +1,252 @@
+"""Host-side mirror of the reference's solver API over the avbd C ABI.
+
+Names follow the reference so a caller reads the same:
+  * ``SimulationContext``  capacities + params (simulation/SimulationContext.hpp:18-60)
+  * ``VBDSolver``          ctor (ctx, solver_iters, iter_acceleration) and
+                           ``solve_Parallel`` (simulation/VBDSolver.hpp:37-103)
+  * ``Simulation``         setup / update / _timeStep (simulation/Simulation.cpp:48-220)
+The device does all per-step work; this layer only stages inputs and reads
+results back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .scenes import Scene, TetMesh
+
+
+def _dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _uptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _bptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+class SimulationContext:
+    """Owns one device context (avbd_ctx): particle and energy pools, colouring,
+    device buffers, stream and step graph."""
+
+    def __init__(self, dt: float, g_accel: float = 9.81, solver_iters: int = 20,
+                 iter_acceleration: float = 0.95, particle_capacity: int = 0, energy_capacity: int = 0,
+                 device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0, use_graph: bool = True):
+        desc = _lib.SceneDesc()
+        desc.particle_capacity = particle_capacity
+        desc.energy_capacity = energy_capacity
+        desc.dt = dt
+        desc.g_accel = g_accel
+        desc.solver_iters = solver_iters
+        desc.iter_acceleration = iter_acceleration
+        desc.flags = 0 if use_graph else _lib.FLAG_NO_GRAPH
+        desc.device = device
+        desc.lanes_per_vertex = lanes_per_vertex
+        desc.block_size = block_size
+        h = C.c_void_p()
+        rc = lib.avbd_create(C.byref(desc), C.byref(h))
+        check(rc, None)
+        self.handle = h
+        self.dt = dt
+        self.g_accel = g_accel
+        self.solver_iters = solver_iters
+        self.iter_acceleration = iter_acceleration
+        self.built = False
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.avbd_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        check(rc, self.handle)
+
+    # --- pools -----------------------------------------------------------
+    def add_tet_mesh(self, rest: np.ndarray, tets: np.ndarray, E: float, nu: float, density: float,
+                     kd: float = 0.0, with_ground: bool = False, mu_s: float = 0.4, mu_k: float = 0.2) -> int:
+        rest = np.ascontiguousarray(rest, dtype=np.float64)
+        tets = np.ascontiguousarray(tets, dtype=np.uint32)
+        first = C.c_uint32()
+        self._chk(lib.avbd_add_tet_mesh(self.handle, rest.shape[0], _dptr(rest), tets.shape[0], _uptr(tets),
+                                        E, nu, density, kd, int(with_ground), mu_s, mu_k, C.byref(first)))
+        return first.value
+
+    def add_particles(self, x, v=None, v_prev=None, x_prev=None, mass=None, fixed=None) -> int:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        v, v_prev, x_prev, mass = arr(v), arr(v_prev), arr(x_prev), arr(mass)
+        fixed = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        first = C.c_uint32()
+        self._chk(lib.avbd_add_particles(self.handle, x.shape[0], _dptr(x), _dptr(v), _dptr(v_prev),
+                                         _dptr(x_prev), _dptr(mass), _bptr(fixed), C.byref(first)))
+        return first.value
+
+    def add_energies(self, type_: int, records) -> None:
+        n = len(records)
+        self._chk(lib.avbd_add_energies(self.handle, type_, C.cast(records, C.c_void_p), n))
+
+    def set_particle_state(self, first: int, x=None, x_prev=None, v=None, v_prev=None) -> None:
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        x, x_prev, v, v_prev = arr(x), arr(x_prev), arr(v), arr(v_prev)
+        n = next(a.shape[0] for a in (x, x_prev, v, v_prev) if a is not None)
+        self._chk(lib.avbd_set_particle_state(self.handle, first, n, _dptr(x), _dptr(x_prev), _dptr(v),
+                                              _dptr(v_prev)))
+
+    def set_fixed(self, first: int, fixed: np.ndarray) -> None:
+        fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+        self._chk(lib.avbd_set_fixed(self.handle, first, fixed.shape[0], _bptr(fixed)))
+
+    def set_scenes(self, offsets: Sequence[int]) -> None:
+        off = np.ascontiguousarray(offsets, dtype=np.uint32)
+        self._chk(lib.avbd_set_scenes(self.handle, off.shape[0] - 1, _uptr(off)))
+
+    def build_graph(self) -> None:
+        self._chk(lib.avbd_build_graph(self.handle))
+        self.built = True
+
+    # --- stepping / readback ---------------------------------------------
+    @property
+    def num_particles(self) -> int:
+        return lib.avbd_num_particles(self.handle)
+
+    def step(self, n: int = 1) -> None:
+        self._chk(lib.avbd_step(self.handle, n))
+
+    def synchronize(self) -> None:
+        self._chk(lib.avbd_synchronize(self.handle))
+
+    def positions(self) -> np.ndarray:
+        x = np.empty((self.num_particles, 3))
+        self._chk(lib.avbd_download_state(self.handle, _dptr(x), None))
+        return x
+
+    def state(self):
+        x = np.empty((self.num_particles, 3))
+        v = np.empty((self.num_particles, 3))
+        self._chk(lib.avbd_download_state(self.handle, _dptr(x), _dptr(v)))
+        return x, v
+
+    def inertial_positions(self) -> np.ndarray:
+        y = np.empty((self.num_particles, 3))
+        self._chk(lib.avbd_download_inertial(self.handle, _dptr(y)))
+        return y
+
+    def coloring(self):
+        col = np.empty(self.num_particles, dtype=np.uint32)
+        nc = C.c_uint32()
+        self._chk(lib.avbd_download_coloring(self.handle, _uptr(col), C.byref(nc)))
+        return col, nc.value
+
+    def ground_records(self):
+        n = lib.avbd_num_energies(self.handle, _lib.GROUND_COLLISION)
+        recs = (_lib.GroundRecord * n)()
+        self._chk(lib.avbd_download_energies(self.handle, _lib.GROUND_COLLISION, C.cast(recs, C.c_void_p), n))
+        return recs
+
+    def tet_records(self):
+        n = lib.avbd_num_energies(self.handle, _lib.NEO_HOOKEAN)
+        recs = (_lib.NeoHookeanRecord * n)()
+        self._chk(lib.avbd_download_energies(self.handle, _lib.NEO_HOOKEAN, C.cast(recs, C.c_void_p), n))
+        return recs
+
+    def stats(self) -> _lib.Stats:
+        st = _lib.Stats()
+        self._chk(lib.avbd_get_stats(self.handle, C.byref(st)))
+        return st
+
+    def launch_config(self):
+        L, B = C.c_uint32(), C.c_uint32()
+        self._chk(lib.avbd_get_launch_config(self.handle, C.byref(L), C.byref(B)))
+        return L.value, B.value
+
+    def set_timing(self, enable: bool) -> None:
+        self._chk(lib.avbd_set_timing(self.handle, int(enable)))
+
+    def solve_timing(self):
+        ms, n = C.c_double(), C.c_uint32()
+        self._chk(lib.avbd_get_solve_timing(self.handle, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+class VBDSolver:
+    """VBDSolver(ctx, solver_iters, iter_acceleration) (simulation/VBDSolver.hpp:37-42).
+    The iteration count and Chebyshev rho are fixed when the context is created;
+    ``solve_Parallel`` runs one full time step on the device."""
+
+    def __init__(self, ctx: SimulationContext, solver_iters: int, iter_acceleration: float):
+        if solver_iters != ctx.solver_iters or iter_acceleration != ctx.iter_acceleration:
+            raise ValueError("solver_iters / iter_acceleration must match the SimulationContext")
+        self._ctx = ctx
+
+    def solve_Parallel(self, w_ctx=None) -> None:
+        # every SPMD worker of the reference enters here; only worker 0 issues work
+        if w_ctx is None or getattr(w_ctx, "idx", 0) == 0:
+            self._ctx.step(1)
+
+
+class Simulation:
+    """Simulation shell (simulation/Simulation.cpp:22-220): builds the context from
+    a Scene description, runs setup() (adjacency + colouring) and time steps."""
+
+    def __init__(self, scene: Scene, device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0,
+                 use_graph: bool = True):
+        self.scene = scene
+        self.ctx = SimulationContext(dt=scene.dt, g_accel=scene.g, solver_iters=scene.iters,
+                                     iter_acceleration=scene.rho, device=device,
+                                     lanes_per_vertex=lanes_per_vertex, block_size=block_size,
+                                     use_graph=use_graph)
+        self.solver = VBDSolver(self.ctx, scene.iters, scene.rho)
+        self.time = 0.0
+        self.offsets: List[int] = []
+        self._setup = False
+
+    def setup(self) -> None:
+        offs = [0]
+        for m in self.scene.meshes:
+            first = self._add_object(m)
+            offs.append(first + m.rest.shape[0])
+        self.offsets = offs
+        if len(self.scene.meshes) > 1:
+            self.ctx.set_scenes(offs)
+        self.ctx.build_graph()
+        self._setup = True
+
+    def _add_object(self, m: TetMesh) -> int:
+        first = self.ctx.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
+        if m.fixed is not None:
+            self.ctx.set_fixed(first, m.fixed)
+        if m.x0 is not None or m.v0 is not None:
+            x0 = m.rest if m.x0 is None else m.x0
+            v0 = np.zeros_like(x0) if m.v0 is None else m.v0
+            self.ctx.set_particle_state(first, x=x0, x_prev=x0, v=v0, v_prev=v0)
+        return first
+
+    def _timeStep(self) -> None:
+        self.solver.solve_Parallel()
+        self.time += self.scene.dt
+
+    def update(self, n_steps: Optional[int] = None) -> None:
+        if not self._setup:
+            self.setup()
+        n = self.scene.steps if n_steps is None else n_steps
+        self.ctx.step(n)
+        self.time += n * self.scene.dt
+
+    def positions(self) -> np.ndarray:
+        return self.ctx.positions()

@@ -1,0 +1,191 @@
+"""Benchmark of the per-step AVBD solve on one MI355X.
+
+Workload (default): the 32^3 Kuhn cube (35,937 vertices, 196,608 tets) dropped
+from 0.5 m onto the ground plane with AL contact on every vertex, dt = 1/60,
+10 VBD iterations per step, synthetic seeded input.  A step is one full
+solve_Parallel on the device (hipGraph replay); inputs are resident in HBM.
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the per-colour
+vertex-block solve): algorithmic bytes per launch (FP32 gather model: 40 B own
+state + 144 B per incident tet + 48 B per ground record, per vertex block)
+divided by its average launch duration measured with HIP events on the
+library's stream.  `cpu_baseline` times the FP64 C++ restatement of the
+reference's multithreaded CPU path on this machine's host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "sim steps/sec at fixed force residual, warm-start vs plain; vertex-updates/sec"
+HBM_PEAK_GBS = 8000.0
+
+
+def make_scene(name: str):
+    from avbd_amd import scenes
+    if name == "dropped_cube_32":
+        return scenes.dropped_cube(n=32)
+    if name == "hanging_cube_8":
+        return scenes.hanging_cube(n=8)
+    if name == "cube_batch_512":
+        return scenes.cube_batch(n_scenes=512)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def solve_bytes_per_launch(sim) -> float:
+    """Average algorithmic bytes of one colour-phase launch."""
+    sc = sim.scene
+    n_items = 0
+    n_inc = 0
+    n_g = 0
+    for m in sc.meshes:
+        free = np.ones(m.rest.shape[0], bool) if m.fixed is None else (m.fixed == 0)
+        val = np.bincount(m.tets.ravel().astype(np.int64), minlength=m.rest.shape[0])
+        n_items += int(free.sum())
+        n_inc += int(val[free].sum())
+        n_g += int(free.sum()) if m.ground else 0
+    _, k = sim.ctx.coloring()
+    total = 40 * n_items + 144 * n_inc + 48 * n_g
+    return total / k
+
+
+def cpu_baseline(sc, budget_s: float = 12.0):
+    """FP64 oracle (restatement of the reference's SPMD CPU path) on the host
+    cores: setup excluded, 1 warm-up step, then whole steps until ~budget_s."""
+    from oracle import oracle_py
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    o = oracle_py.from_scene(sc)
+    o.step(1, threads)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        o.step(1, threads)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 400:
+            break
+    return {
+        "value": n / el,
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sc.name}: {n} whole steps ({sc.iters} iterations each) after 1 warm-up step, "
+                  f"FP64, {threads} SPMD threads with a spin barrier per phase",
+        "vertex_updates_per_s": n * sc.iters * sum(m.rest.shape[0] for m in sc.meshes) / el,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="dropped_cube_32")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs the solver on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    from avbd_amd.sim import Simulation
+
+    sc = make_scene(args.workload)
+    print(f"[bench] workload {sc.name}: {sc.n_vertices} vertices, {sc.n_tets} tets, "
+          f"{sc.n_ground} ground contacts, {sc.iters} iterations", file=sys.stderr, flush=True)
+    t_setup = time.perf_counter()
+    sim = Simulation(sc)
+    sim.setup()
+    t_setup = time.perf_counter() - t_setup
+    lanes, block = sim.ctx.launch_config()
+    print(f"[bench] setup {t_setup:.2f}s, launch config lanes={lanes} block={block}", file=sys.stderr, flush=True)
+
+    sim.ctx.step(args.warmup)
+    sim.ctx.synchronize()
+    t0 = time.perf_counter()
+    sim.ctx.step(args.steps)
+    sim.ctx.synchronize()
+    el = time.perf_counter() - t0
+    steps_per_s = args.steps / el
+    st = sim.ctx.stats()
+    n_items = sum(int((np.ones(m.rest.shape[0]) if m.fixed is None else (m.fixed == 0)).sum()) for m in sc.meshes)
+    vups = steps_per_s * sc.iters * n_items
+    print(f"[bench] {args.steps} steps in {el * 1e3:.1f} ms -> {steps_per_s:.1f} steps/s, "
+          f"residual_rel {st.residual_rel:.3e}", file=sys.stderr, flush=True)
+
+    # live kernel timing of the dominant kernel (HIP events on the ctx stream)
+    sim.ctx.set_timing(True)
+    sim.ctx.step(2)
+    avg_ms, launches = sim.ctx.solve_timing()
+    sim.ctx.set_timing(False)
+    bytes_launch = solve_bytes_per_launch(sim)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roof = {
+        "kernel": "solve_color_kernel",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_us": round(avg_ms * 1e3, 3),
+        "algorithmic_bytes_per_launch": int(bytes_launch),
+        "launches_timed": launches,
+    }
+    prof = ROOT / "profiles" / f"solve_color_{sc.name}_traffic.json"
+    if prof.exists():
+        try:
+            roof["traffic"] = json.loads(prof.read_text()).get("bytes_per_launch")
+        except Exception:
+            pass
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(sc, args.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": sc.name,
+            "vertices": sc.n_vertices,
+            "tets": sc.n_tets,
+            "ground_contacts": sc.n_ground,
+            "iterations": sc.iters,
+            "dt": sc.dt,
+            "mode": "plain AVBD, fixed iteration count",
+            "self_collision": False,
+            "parallelism": "single GPU",
+        },
+        "vertex_updates_per_s": round(vups, 1),
+        "residual_rel_final": st.residual_rel,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "launch_config": {"lanes_per_vertex": lanes, "block_size": block},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

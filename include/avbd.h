@@ -1,0 +1,189 @@
+/*
+ * avbd.h -- C ABI of the MI355X-native AVBD per-step solver (libavbd_amd.so).
+ *
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ * Every entry point returns 0 on success or a negative AVBD_E_* code; the
+ * message for the last failure on a context is returned by avbd_last_error().
+ * Host arrays are borrowed for the duration of the call.  One caller thread
+ * per context (contexts are independent; each owns one HIP stream).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   VBDSolver(SimulationContext*, unsigned solver_iters, Real iter_acceleration)
+ *       + solve_Parallel(WorkerThreadContext&)     simulation/VBDSolver.hpp:37-103
+ *   SimulationExecutor::timeStep                   simulation/SimulationExecutor.hpp:107-156
+ *   SimulationContext(capacities...) + params      simulation/SimulationContext.hpp:18-60,
+ *                                                  simulation/SimulationParams.hpp:8-15
+ *   ParticlePool::addParticle / addOrientedParticle common/ParticlePool.hpp:213-263
+ *   EnergyRegistry pools' addEnergy(...)           energy/EnergyRegistry.hpp:17-173,
+ *                                                  energy/elastic/NeoHookeanEnergyPool.hpp:40-67,
+ *                                                  energy/collision/GroundCollisionEnergyPool.hpp:26-38
+ *   TetMeshObject::setup (setup formulas)          simobject/TetMeshObject.cpp:21-113
+ *   Simulation::_addGroundCollisionConstraintsForObject  simulation/Simulation.hpp:117-126
+ *   StaticParticleAdjacency::buildAdjacency + ColorList::buildInitialColorList
+ *                                                  simulation/Simulation.cpp:87-89
+ */
+#ifndef AVBD_H
+#define AVBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference: exceptions at common/TombstonePool.hpp:64,
+ *      common/ColorList.hpp:36, collision/CollisionDetector.cpp:211,686) ---- */
+#define AVBD_OK 0
+#define AVBD_E_ARG (-1)                 /* invalid argument / state                  */
+#define AVBD_E_CAPACITY (-2)            /* "Pool has exceeded capacity"              */
+#define AVBD_E_COLORS (-3)              /* "Ran out of colors!" (more than 256)      */
+#define AVBD_E_UNSUPPORTED_CONTACT (-4) /* "Unsupported collision!"                  */
+#define AVBD_E_NONFINITE (-5)           /* NaN/Inf detected in the state             */
+#define AVBD_E_HIP (-6)                 /* HIP runtime error                         */
+#define AVBD_E_NOT_BUILT (-7)           /* avbd_build_graph() has not been called    */
+
+typedef struct avbd_ctx avbd_ctx;
+
+/* ---- energy types, in the reference's EnergyType order (common/common.hpp:156-163) ---- */
+typedef enum {
+    AVBD_NEO_HOOKEAN = 0,
+    AVBD_COSSERAT_ROD = 1,
+    AVBD_GROUND_COLLISION = 2,
+    AVBD_RIGID_BODY_GROUND_COLLISION = 3,
+    AVBD_TRIANGLE_RIGID_COLLISION = 4,
+    AVBD_TRIANGLE_ROD_COLLISION = 5,
+    AVBD_ONE_SIDED_FIXED_JOINT = 6
+} avbd_energy_type;
+
+/* ---- context creation flags ---- */
+#define AVBD_FLAG_NO_GRAPH 1u    /* launch kernels eagerly instead of replaying a hipGraph */
+
+/* SimulationContext capacities + SimulationParams + VBDSolver arguments.
+ * Reference: simulation/SimulationContext.hpp:46-59, simulation/SimulationParams.hpp:8-15,
+ * simulation/VBDSolver.hpp:37-42.  Gravity is (0, -g_accel, 0) (y-up,
+ * simulation/VBDSolver.hpp:51). */
+typedef struct {
+    uint32_t particle_capacity;
+    uint32_t energy_capacity;      /* per energy type */
+    double dt;
+    double g_accel;
+    uint32_t solver_iters;
+    double iter_acceleration;      /* Chebyshev rho */
+    uint32_t flags;
+    int32_t device;                /* HIP device ordinal */
+    uint32_t lanes_per_vertex;     /* 0 = autotuned; else 4, 8, 16 or 32 */
+    uint32_t block_size;           /* 0 = autotuned; else 64..1024, multiple of 64 */
+} avbd_scene_desc;
+
+/* NeoHookeanEnergyInfo, energy/elastic/NeoHookeanEnergyPool.hpp:8-17 (FP64, row-major 3x3). */
+typedef struct {
+    uint32_t particle_indices[4];
+    double lambda, mu, kd;
+    double Q[9];
+    double rest_volume;
+    double E_prev[9];
+} avbd_neo_hookean_record;
+
+/* GroundCollisionEnergyInfo, energy/collision/CollisionConstraintEnergyPool.hpp:9-30
+ * + energy/collision/GroundCollisionEnergyPool.hpp:8-13. */
+typedef struct {
+    uint32_t particle_index;
+    uint32_t use_static;
+    uint32_t unclamped_lambda_tb;
+    uint32_t pad_;
+    double k_start, k_n, k_t, k_b;
+    double lambda_n, lambda_t, lambda_b;
+    double C_n_prev;
+    double mu_s, mu_k;
+    double cp_x, cp_z;
+} avbd_ground_record;
+
+/* Per-step statistics (new: the reference has no residual; see the
+ * definitions in the oracle and the solver sources). */
+typedef struct {
+    double residual_abs;    /* ||r||_2 over free particles                     */
+    double residual_rel;    /* ||r||_2 / ||M g||_2                              */
+    double incremental_potential;
+    uint32_t num_colors;
+    uint32_t num_particles;
+    uint32_t num_tets;
+    uint32_t num_ground;
+    uint64_t steps_taken;
+    uint64_t vertex_updates; /* cumulative vertex-block solves                 */
+    double last_step_ms;     /* device time of the last avbd_step() call        */
+} avbd_stats;
+
+/* Context lifetime -------------------------------------------------------- */
+int avbd_create(const avbd_scene_desc* desc, avbd_ctx** out_ctx);
+void avbd_destroy(avbd_ctx* ctx);
+/* Last error message for ctx (or the last avbd_create failure if ctx is NULL). */
+const char* avbd_last_error(avbd_ctx* ctx);
+
+/* Pools (host-side staging until avbd_build_graph) ------------------------- */
+
+/* ParticlePool::addParticle (common/ParticlePool.hpp:213-238) for n particles:
+ * x, v, v_prev, x_prev are 3n doubles (NULL = zeros / x), mass n doubles,
+ * fixed n bytes (NULL = not fixed).  *first receives the first new index. */
+int avbd_add_particles(avbd_ctx* ctx, uint32_t n, const double* x, const double* v, const double* v_prev,
+                       const double* x_prev, const double* mass, const uint8_t* fixed, uint32_t* first);
+
+/* EnergyRegistry pools: append count records of the given type
+ * (avbd_neo_hookean_record / avbd_ground_record). */
+int avbd_add_energies(avbd_ctx* ctx, int type, const void* records, uint32_t count);
+
+/* TetMeshObject::setup formulas (simobject/TetMeshObject.cpp:46-96) for a mesh
+ * given in its rest state: Lame parameters from (E, nu), Q = X^-1,
+ * V = |det X|/6, lumped masses 0.25 V density; plus, if with_ground, one
+ * ground energy per vertex with k_start = m/dt^2 (simulation/Simulation.hpp:117-126).
+ * Particles are appended; *first receives the first new particle index. */
+int avbd_add_tet_mesh(avbd_ctx* ctx, uint32_t nv, const double* rest_x, uint32_t nt, const uint32_t* tets,
+                      double E, double nu, double density, double kd, int with_ground, double mu_s,
+                      double mu_k, uint32_t* first);
+
+/* Overwrite the state of particles [first, first+n) (3n doubles each; NULL =
+ * leave unchanged).  Valid before and after avbd_build_graph. */
+int avbd_set_particle_state(avbd_ctx* ctx, uint32_t first, uint32_t n, const double* x, const double* x_prev,
+                            const double* v, const double* v_prev);
+int avbd_set_fixed(avbd_ctx* ctx, uint32_t first, uint32_t n, const uint8_t* fixed);
+
+/* Scene partition for batching: scene s owns particles
+ * [particle_offsets[s], particle_offsets[s+1]); each scene is coloured on its
+ * own.  Default: one scene holding every particle. */
+int avbd_set_scenes(avbd_ctx* ctx, uint32_t n_scenes, const uint32_t* particle_offsets);
+
+/* StaticParticleAdjacency::buildAdjacency + ColorList::buildInitialColorList
+ * (DSATUR) per scene, then upload every pool to the device as FP32 SoA. */
+int avbd_build_graph(avbd_ctx* ctx);
+
+/* Time stepping ------------------------------------------------------------ */
+
+/* n_steps calls of VBDSolver::solve_Parallel (asynchronous on the ctx stream). */
+int avbd_step(avbd_ctx* ctx, uint32_t n_steps);
+int avbd_synchronize(avbd_ctx* ctx);
+
+/* Readback (synchronising) -------------------------------------------------- */
+int avbd_download_state(avbd_ctx* ctx, double* x, double* v);
+int avbd_download_inertial(avbd_ctx* ctx, double* y);
+int avbd_download_energies(avbd_ctx* ctx, int type, void* records, uint32_t count);
+/* color: one entry per particle; *num_colors = max over scenes. */
+int avbd_download_coloring(avbd_ctx* ctx, uint32_t* color, uint32_t* num_colors);
+/* Force residual / incremental potential at the current state (device pass). */
+int avbd_get_stats(avbd_ctx* ctx, avbd_stats* out);
+
+/* Introspection for tests and the bench -------------------------------------- */
+uint32_t avbd_num_particles(avbd_ctx* ctx);
+uint32_t avbd_num_energies(avbd_ctx* ctx, int type);
+/* Kernel launch geometry actually in use (after autotuning). */
+int avbd_get_launch_config(avbd_ctx* ctx, uint32_t* lanes_per_vertex, uint32_t* block_size);
+/* Kernel timing: while enabled, avbd_step() launches eagerly and brackets
+ * every per-colour vertex-solve launch with HIP events on the ctx stream.
+ * avbd_get_solve_timing() returns the average duration (ms) of those
+ * launches over the last avbd_step() call and how many it covers. */
+int avbd_set_timing(avbd_ctx* ctx, int enable);
+int avbd_get_solve_timing(avbd_ctx* ctx, double* avg_ms, uint32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AVBD_H */

@@ -1,0 +1,234 @@
+// capi.cpp -- C entry points of the FP64 oracle for ctypes (test
+// infrastructure only: tests/, smoke() and bench.py's cpu_baseline leg).
+#include "oracle.hpp"
+
+#include <cstring>
+#include <exception>
+#include <string>
+
+using namespace oracle;
+
+namespace {
+thread_local std::string g_err;
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error() { return g_err.c_str(); }
+
+void* oracle_new(double dt, double g, uint32_t iters, double rho) {
+    auto* s = new Simulation();
+    s->params.dt = dt;
+    s->params.g = g;
+    s->params.iters = iters;
+    s->params.rho = rho;
+    return s;
+}
+
+void oracle_free(void* h) { delete static_cast<Simulation*>(h); }
+
+int64_t oracle_add_tet_mesh(void* h, uint32_t nv, const double* rest_x, uint32_t nt, const uint32_t* tets,
+                            double E, double nu, double density, double kd, int with_ground,
+                            double mu_s, double mu_k) {
+    int64_t base = -1;
+    int rc = guard([&] {
+        base = static_cast<Simulation*>(h)->add_tet_mesh(nv, rest_x, nt, tets, E, nu, density, kd,
+                                                          with_ground != 0, mu_s, mu_k);
+    });
+    return rc == 0 ? base : -1;
+}
+
+uint32_t oracle_add_particles(void* h, uint32_t n, const double* x, const double* m) {
+    return static_cast<Simulation*>(h)->add_particles(n, x, m);
+}
+
+uint32_t oracle_num_particles(void* h) { return (uint32_t)static_cast<Simulation*>(h)->x.size(); }
+uint32_t oracle_num_tets(void* h) { return (uint32_t)static_cast<Simulation*>(h)->nh.size(); }
+uint32_t oracle_num_ground(void* h) { return (uint32_t)static_cast<Simulation*>(h)->ground.size(); }
+
+// Overwrite positions (and previous positions) of particles [0, n).
+void oracle_set_positions(void* h, const double* x, int also_prev) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t i = 0; i < s->x.size(); i++) {
+        V3 p(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        s->x[i] = p;
+        if (also_prev) s->x_prev[i] = p;
+    }
+}
+void oracle_set_velocities(void* h, const double* v, const double* v_prev) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t i = 0; i < s->x.size(); i++) {
+        s->v[i] = V3(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+        s->v_prev[i] = V3(v_prev[3 * i], v_prev[3 * i + 1], v_prev[3 * i + 2]);
+    }
+}
+void oracle_set_fixed(void* h, const uint8_t* fixed) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t i = 0; i < s->x.size(); i++) s->fixed[i] = fixed[i];
+}
+
+int oracle_build_graph(void* h) {
+    return guard([&] { static_cast<Simulation*>(h)->build_graph(); });
+}
+
+int oracle_step(void* h, uint32_t n_steps, uint32_t n_threads) {
+    return guard([&] { static_cast<Simulation*>(h)->step_mt(n_threads, n_steps); });
+}
+
+void oracle_get_positions(void* h, double* out) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t i = 0; i < s->x.size(); i++) {
+        out[3 * i] = s->x[i].x; out[3 * i + 1] = s->x[i].y; out[3 * i + 2] = s->x[i].z;
+    }
+}
+void oracle_get_velocities(void* h, double* out) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t i = 0; i < s->x.size(); i++) {
+        out[3 * i] = s->v[i].x; out[3 * i + 1] = s->v[i].y; out[3 * i + 2] = s->v[i].z;
+    }
+}
+void oracle_get_inertial(void* h, double* out) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t i = 0; i < s->x.size(); i++) {
+        out[3 * i] = s->y[i].x; out[3 * i + 1] = s->y[i].y; out[3 * i + 2] = s->y[i].z;
+    }
+}
+void oracle_get_masses(void* h, double* out) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t i = 0; i < s->x.size(); i++) out[i] = s->mass[i];
+}
+
+// Tet records: per tet 4 indices, then Q (9, row-major), V, lambda, mu, kd, E_prev (9).
+void oracle_get_tets(void* h, uint32_t* idx, double* Q, double* V, double* lam, double* mu, double* kd,
+                     double* Eprev) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t e = 0; e < s->nh.size(); e++) {
+        const auto& t = s->nh[e];
+        for (int k = 0; k < 4; k++) idx[4 * e + k] = t.idx[k];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                Q[9 * e + 3 * i + j] = t.Q.m[i][j];
+                Eprev[9 * e + 3 * i + j] = t.E_prev.m[i][j];
+            }
+        V[e] = t.V; lam[e] = t.lambda; mu[e] = t.mu; kd[e] = t.kd;
+    }
+}
+void oracle_set_tet_eprev(void* h, const double* Eprev) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t e = 0; e < s->nh.size(); e++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) s->nh[e].E_prev.m[i][j] = Eprev[9 * e + 3 * i + j];
+}
+
+// Ground records: k_start,k_n,k_t,k_b,lambda_n,lambda_t,lambda_b,C_n_prev,mu_s,mu_k,cp_x,cp_z,use_static,unclamped
+void oracle_get_ground(void* h, uint32_t* p, double* rec14) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t e = 0; e < s->ground.size(); e++) {
+        const auto& c = s->ground[e];
+        p[e] = c.p;
+        double* r = rec14 + 14 * e;
+        r[0] = c.k_start; r[1] = c.k_n; r[2] = c.k_t; r[3] = c.k_b;
+        r[4] = c.lambda_n; r[5] = c.lambda_t; r[6] = c.lambda_b; r[7] = c.C_n_prev;
+        r[8] = c.mu_s; r[9] = c.mu_k; r[10] = c.cp_x; r[11] = c.cp_z;
+        r[12] = c.use_static ? 1 : 0; r[13] = c.unclamped_lambda_tb ? 1 : 0;
+    }
+}
+void oracle_set_ground(void* h, const double* rec14) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t e = 0; e < s->ground.size(); e++) {
+        auto& c = s->ground[e];
+        const double* r = rec14 + 14 * e;
+        c.k_start = r[0]; c.k_n = r[1]; c.k_t = r[2]; c.k_b = r[3];
+        c.lambda_n = r[4]; c.lambda_t = r[5]; c.lambda_b = r[6]; c.C_n_prev = r[7];
+        c.mu_s = r[8]; c.mu_k = r[9]; c.cp_x = r[10]; c.cp_z = r[11];
+        c.use_static = r[12] != 0; c.unclamped_lambda_tb = r[13] != 0;
+    }
+}
+
+uint32_t oracle_num_colors(void* h) { return static_cast<Simulation*>(h)->coloring.num_colors; }
+void oracle_get_colors(void* h, uint32_t* color) {
+    auto* s = static_cast<Simulation*>(h);
+    std::memcpy(color, s->coloring.color.data(), s->coloring.color.size() * sizeof(uint32_t));
+}
+uint32_t oracle_num_adj_entries(void* h) { return (uint32_t)static_cast<Simulation*>(h)->adj.e_entries.size(); }
+// e_offsets: (n+1)*5, entries: type/e_idx/local triplets
+void oracle_get_adjacency(void* h, uint32_t* e_offsets, uint32_t* entries, uint32_t* p_offsets,
+                          uint32_t* p_neighbors) {
+    auto* s = static_cast<Simulation*>(h);
+    const auto& a = s->adj;
+    for (size_t i = 0; i < a.e_offsets.size(); i++)
+        for (int t = 0; t < ST_COUNT; t++) e_offsets[ST_COUNT * i + t] = a.e_offsets[i][t];
+    for (size_t k = 0; k < a.e_entries.size(); k++) {
+        entries[3 * k] = a.e_entries[k].type;
+        entries[3 * k + 1] = a.e_entries[k].e_idx;
+        entries[3 * k + 2] = a.e_entries[k].local;
+    }
+    if (p_offsets) std::memcpy(p_offsets, a.p_offsets.data(), a.p_offsets.size() * sizeof(uint32_t));
+    if (p_neighbors) std::memcpy(p_neighbors, a.p_neighbors.data(), a.p_neighbors.size() * sizeof(uint32_t));
+}
+uint32_t oracle_num_p_neighbors(void* h) { return (uint32_t)static_cast<Simulation*>(h)->adj.p_neighbors.size(); }
+
+// Standalone DSATUR on a caller-provided CSR graph (for hand-made cases).
+uint32_t oracle_dsatur_csr(uint32_t n, const uint32_t* p_offsets, const uint32_t* p_neighbors, uint32_t* color_out) {
+    StaticAdjacency adj;
+    adj.p_offsets.assign(p_offsets, p_offsets + n + 1);
+    adj.p_neighbors.assign(p_neighbors, p_neighbors + p_offsets[n]);
+    Coloring c;
+    dsatur(adj, n, c);
+    std::memcpy(color_out, c.color.data(), n * sizeof(uint32_t));
+    return c.num_colors;
+}
+
+void oracle_stats(void* h, double* out3) {
+    Stats st = static_cast<Simulation*>(h)->stats();
+    out3[0] = st.residual_abs;
+    out3[1] = st.residual_rel;
+    out3[2] = st.ip;
+}
+
+// Single-tet kernels for unit tests.  x: 4 positions (12), Q row-major (9),
+// Eprev (9).  Returns the energy; writes G (3) and H (9, row-major) for `local`.
+double oracle_nh_single(const double* x12, const double* Q9, const double* Eprev9, double V, double lambda,
+                        double mu, double kd, double dt, uint32_t local, double* G3, double* H9, int use4) {
+    std::vector<V3> x(4);
+    for (int i = 0; i < 4; i++) x[i] = V3(x12[3 * i], x12[3 * i + 1], x12[3 * i + 2]);
+    NHTet t;
+    for (int k = 0; k < 4; k++) t.idx[k] = k;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            t.Q.m[i][j] = Q9[3 * i + j];
+            t.E_prev.m[i][j] = Eprev9[3 * i + j];
+        }
+    t.V = V; t.lambda = lambda; t.mu = mu; t.kd = kd;
+    V3 G; M3 H;
+    if (use4) {
+        // the same incidence four times through the 4-wide path, divided by 4
+        // (the reference's own check, test/test_neohookean_accumulate.cpp:51-84)
+        const NHTet* t4[4] = {&t, &t, &t, &t};
+        uint32_t l4[4] = {local, local, local, local};
+        nh_accumulate4(t4, x, l4, G, H, dt);
+        G = 0.25 * G;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) H.m[i][j] *= 0.25;
+    } else {
+        nh_accumulate(t, x, local, G, H, dt);
+    }
+    G3[0] = G.x; G3[1] = G.y; G3[2] = G.z;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) H9[3 * i + j] = H.m[i][j];
+    return nh_energy(t, x, dt);
+}
+
+void oracle_lu_solve(int n, const double* A, const double* b, double* x) { lu_solve(n, A, b, x); }
+
+}  // extern "C"

@@ -1,0 +1,244 @@
+"""ctypes wrapper of the FP64 oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker / the CPU baseline --
+never by the avbd_amd product code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+_D = C.POINTER(C.c_double)
+_U32 = C.POINTER(C.c_uint32)
+_U8 = C.POINTER(C.c_uint8)
+
+_SIG = {
+    "oracle_last_error": (C.c_char_p, []),
+    "oracle_new": (C.c_void_p, [C.c_double, C.c_double, C.c_uint32, C.c_double]),
+    "oracle_free": (None, [C.c_void_p]),
+    "oracle_add_tet_mesh": (C.c_int64, [C.c_void_p, C.c_uint32, _D, C.c_uint32, _U32, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, C.c_double, C.c_double]),
+    "oracle_add_particles": (C.c_uint32, [C.c_void_p, C.c_uint32, _D, _D]),
+    "oracle_num_particles": (C.c_uint32, [C.c_void_p]),
+    "oracle_num_tets": (C.c_uint32, [C.c_void_p]),
+    "oracle_num_ground": (C.c_uint32, [C.c_void_p]),
+    "oracle_set_positions": (None, [C.c_void_p, _D, C.c_int]),
+    "oracle_set_velocities": (None, [C.c_void_p, _D, _D]),
+    "oracle_set_fixed": (None, [C.c_void_p, _U8]),
+    "oracle_build_graph": (C.c_int, [C.c_void_p]),
+    "oracle_step": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "oracle_get_positions": (None, [C.c_void_p, _D]),
+    "oracle_get_velocities": (None, [C.c_void_p, _D]),
+    "oracle_get_inertial": (None, [C.c_void_p, _D]),
+    "oracle_get_masses": (None, [C.c_void_p, _D]),
+    "oracle_get_tets": (None, [C.c_void_p, _U32, _D, _D, _D, _D, _D, _D]),
+    "oracle_set_tet_eprev": (None, [C.c_void_p, _D]),
+    "oracle_get_ground": (None, [C.c_void_p, _U32, _D]),
+    "oracle_set_ground": (None, [C.c_void_p, _D]),
+    "oracle_num_colors": (C.c_uint32, [C.c_void_p]),
+    "oracle_get_colors": (None, [C.c_void_p, _U32]),
+    "oracle_num_adj_entries": (C.c_uint32, [C.c_void_p]),
+    "oracle_num_p_neighbors": (C.c_uint32, [C.c_void_p]),
+    "oracle_get_adjacency": (None, [C.c_void_p, _U32, _U32, _U32, _U32]),
+    "oracle_dsatur_csr": (C.c_uint32, [C.c_uint32, _U32, _U32, _U32]),
+    "oracle_stats": (None, [C.c_void_p, _D]),
+    "oracle_nh_single": (C.c_double, [_D, _D, _D, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_uint32, _D, _D, C.c_int]),
+    "oracle_lu_solve": (None, [C.c_int, _D, _D, _D]),
+}
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def _load():
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    for k, (r, a) in _SIG.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+    return lib
+
+
+lib = _load()
+
+
+def dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def up(a):
+    return None if a is None else a.ctypes.data_as(_U32)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _chk(rc):
+    if rc != 0:
+        raise OracleError(lib.oracle_last_error().decode())
+
+
+class OracleSim:
+    """One reference Simulation restated in FP64 (single scene)."""
+
+    def __init__(self, dt: float, g: float, iters: int, rho: float):
+        self.h = lib.oracle_new(dt, g, iters, rho)
+        self.dt, self.g, self.iters, self.rho = dt, g, iters, rho
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.oracle_free(self.h)
+            self.h = None
+
+    def add_tet_mesh(self, rest, tets, E, nu, density, kd=0.0, with_ground=False, mu_s=0.4, mu_k=0.2) -> int:
+        rest = np.ascontiguousarray(rest, dtype=np.float64)
+        tets = np.ascontiguousarray(tets, dtype=np.uint32)
+        base = lib.oracle_add_tet_mesh(self.h, rest.shape[0], dp(rest), tets.shape[0], up(tets), E, nu, density,
+                                       kd, int(with_ground), mu_s, mu_k)
+        if base < 0:
+            raise OracleError(lib.oracle_last_error().decode())
+        return int(base)
+
+    def add_particles(self, x, m) -> int:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        return int(lib.oracle_add_particles(self.h, x.shape[0], dp(x), dp(m)))
+
+    @property
+    def n(self) -> int:
+        return lib.oracle_num_particles(self.h)
+
+    def set_positions(self, x, also_prev=True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        lib.oracle_set_positions(self.h, dp(x), int(also_prev))
+
+    def set_velocities(self, v, v_prev=None):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        vp = v if v_prev is None else np.ascontiguousarray(v_prev, dtype=np.float64)
+        lib.oracle_set_velocities(self.h, dp(v), dp(vp))
+
+    def set_fixed(self, fixed):
+        f = np.ascontiguousarray(fixed, dtype=np.uint8)
+        lib.oracle_set_fixed(self.h, f.ctypes.data_as(_U8))
+
+    def build_graph(self):
+        _chk(lib.oracle_build_graph(self.h))
+
+    def step(self, n=1, threads=1):
+        _chk(lib.oracle_step(self.h, n, threads))
+
+    def positions(self):
+        out = np.empty((self.n, 3))
+        lib.oracle_get_positions(self.h, dp(out))
+        return out
+
+    def velocities(self):
+        out = np.empty((self.n, 3))
+        lib.oracle_get_velocities(self.h, dp(out))
+        return out
+
+    def inertial(self):
+        out = np.empty((self.n, 3))
+        lib.oracle_get_inertial(self.h, dp(out))
+        return out
+
+    def masses(self):
+        out = np.empty(self.n)
+        lib.oracle_get_masses(self.h, dp(out))
+        return out
+
+    def colors(self):
+        out = np.empty(self.n, dtype=np.uint32)
+        lib.oracle_get_colors(self.h, up(out))
+        return out, lib.oracle_num_colors(self.h)
+
+    def tets(self):
+        nt = lib.oracle_num_tets(self.h)
+        idx = np.empty((nt, 4), dtype=np.uint32)
+        Q = np.empty((nt, 9))
+        V = np.empty(nt)
+        lam = np.empty(nt)
+        mu = np.empty(nt)
+        kd = np.empty(nt)
+        Ep = np.empty((nt, 9))
+        lib.oracle_get_tets(self.h, up(idx), dp(Q), dp(V), dp(lam), dp(mu), dp(kd), dp(Ep))
+        return dict(idx=idx, Q=Q, V=V, lam=lam, mu=mu, kd=kd, E_prev=Ep)
+
+    def ground(self):
+        ng = lib.oracle_num_ground(self.h)
+        p = np.empty(ng, dtype=np.uint32)
+        rec = np.empty((ng, 14))
+        lib.oracle_get_ground(self.h, up(p), dp(rec))
+        return p, rec
+
+    def adjacency(self):
+        n = self.n
+        ne = lib.oracle_num_adj_entries(self.h)
+        nn = lib.oracle_num_p_neighbors(self.h)
+        eo = np.empty((n + 1, 5), dtype=np.uint32)
+        ent = np.empty((ne, 3), dtype=np.uint32)
+        po = np.empty(n + 1, dtype=np.uint32)
+        pn = np.empty(nn, dtype=np.uint32)
+        lib.oracle_get_adjacency(self.h, up(eo), up(ent), up(po), up(pn))
+        return eo, ent, po, pn
+
+    def stats(self):
+        out = np.empty(3)
+        lib.oracle_stats(self.h, dp(out))
+        return dict(residual_abs=out[0], residual_rel=out[1], ip=out[2])
+
+
+def dsatur_csr(offsets, neighbors):
+    off = np.ascontiguousarray(offsets, dtype=np.uint32)
+    nb = np.ascontiguousarray(neighbors, dtype=np.uint32)
+    n = off.shape[0] - 1
+    col = np.empty(n, dtype=np.uint32)
+    k = lib.oracle_dsatur_csr(n, up(off), up(nb), up(col))
+    return col, k
+
+
+def nh_single(x4, Q, Eprev, V, lam, mu, kd, dt, local, use4=False):
+    x4 = np.ascontiguousarray(x4, dtype=np.float64).reshape(12)
+    Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(9)
+    Ep = np.ascontiguousarray(Eprev, dtype=np.float64).reshape(9)
+    G = np.empty(3)
+    H = np.empty(9)
+    e = lib.oracle_nh_single(dp(x4), dp(Q), dp(Ep), V, lam, mu, kd, dt, local, dp(G), dp(H), int(use4))
+    return e, G, H.reshape(3, 3)
+
+
+def lu_solve(A, b):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    n = b.shape[0]
+    x = np.empty(n)
+    lib.oracle_lu_solve(n, dp(A.reshape(-1)), dp(b), dp(x))
+    return x
+
+
+def from_scene(scene, mesh_index=0) -> OracleSim:
+    """Oracle instance for one mesh of an avbd_amd.scenes.Scene (one reference Simulation)."""
+    m = scene.meshes[mesh_index]
+    o = OracleSim(scene.dt, scene.g, scene.iters, scene.rho)
+    o.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
+    if m.fixed is not None:
+        o.set_fixed(m.fixed)
+    if m.x0 is not None:
+        o.set_positions(m.x0, also_prev=True)
+    if m.v0 is not None:
+        o.set_velocities(m.v0, m.v0)
+    o.build_graph()
+    return o

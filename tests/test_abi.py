@@ -1,0 +1,150 @@
+"""The C ABI library loads, exports every symbol include/avbd.h declares, and its
+host-side pieces (staging, setup formulas, DSATUR colouring) behave -- no GPU
+needed: HIP is only touched by the device part of avbd_build_graph."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from avbd_amd import scenes
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "avbd.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|void|uint32_t|const char\*)\s+(avbd_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from avbd_amd import _lib
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(_lib.lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES), "ctypes signatures out of sync with include/avbd.h"
+
+
+def test_struct_layouts_match_header():
+    from avbd_amd import _lib
+    # 4*u32 + 3 doubles + 9 + 1 + 9 doubles
+    assert C.sizeof(_lib.NeoHookeanRecord) == 16 + 8 * 22
+    assert C.sizeof(_lib.GroundRecord) == 16 + 8 * 12
+
+
+def make_ctx(**kw):
+    from avbd_amd.sim import SimulationContext
+    return SimulationContext(dt=kw.pop("dt", 1 / 60), solver_iters=kw.pop("iters", 10), **kw)
+
+
+def test_create_rejects_bad_descriptors():
+    from avbd_amd import _lib
+    from avbd_amd.sim import SimulationContext
+    with pytest.raises(_lib.AvbdError) as e:
+        SimulationContext(dt=0.0)
+    assert e.value.code == _lib.AVBD_E_ARG
+    with pytest.raises(_lib.AvbdError):
+        SimulationContext(dt=1 / 60, lanes_per_vertex=3)
+    with pytest.raises(_lib.AvbdError):
+        SimulationContext(dt=1 / 60, block_size=100)
+
+
+def test_capacity_error_code():
+    from avbd_amd import _lib
+    v, t = scenes.kuhn_cube(2)
+    ctx = make_ctx(particle_capacity=v.shape[0] - 1)
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.add_tet_mesh(v, t, 1e5, 0.45, 1000.0)
+    assert e.value.code == _lib.AVBD_E_CAPACITY
+
+
+def test_not_built_error_code():
+    from avbd_amd import _lib
+    v, t = scenes.kuhn_cube(2)
+    ctx = make_ctx()
+    ctx.add_tet_mesh(v, t, 1e5, 0.45, 1000.0)
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.step(1)
+    assert e.value.code == _lib.AVBD_E_NOT_BUILT
+
+
+def _planned_coloring(ctx):
+    """avbd_build_graph plans on the host first; on a machine without a GPU the
+    device part fails (AVBD_E_HIP) but the colouring is already available."""
+    from avbd_amd import _lib
+    try:
+        ctx.build_graph()
+    except _lib.AvbdError as e:
+        assert e.code == _lib.AVBD_E_HIP
+    return ctx.coloring()
+
+
+@pytest.mark.parametrize("n", [3, 4, 8])
+def test_product_coloring_bit_exact_vs_oracle(oracle, n):
+    v, t = scenes.kuhn_cube(n)
+    ctx = make_ctx()
+    ctx.add_tet_mesh(v, t, 1e5, 0.45, 1000.0)
+    col, k = _planned_coloring(ctx)
+    o = oracle.OracleSim(1 / 60, 9.81, 10, 0.95)
+    o.add_tet_mesh(v, t, 1e5, 0.45, 1000.0)
+    o.build_graph()
+    ocol, ok = o.colors()
+    assert k == ok
+    assert np.array_equal(col, ocol)
+
+
+def test_product_coloring_bit_exact_rotated_dropped_cube(oracle):
+    sc = scenes.dropped_cube(n=12)
+    m = sc.meshes[0]
+    ctx = make_ctx()
+    ctx.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, with_ground=True)
+    col, k = _planned_coloring(ctx)
+    o = oracle.from_scene(sc)
+    ocol, ok = o.colors()
+    assert (k, col.tolist()) == (ok, ocol.tolist())
+
+
+def test_batched_coloring_is_per_scene(oracle):
+    sc = scenes.cube_batch(n_scenes=6, seed=11)
+    from avbd_amd.sim import Simulation
+    sim = Simulation(sc)
+    # set up without touching the device: stage + plan only
+    offs = [0]
+    for m in sc.meshes:
+        first = sim._add_object(m)
+        offs.append(first + m.rest.shape[0])
+    sim.ctx.set_scenes(offs)
+    col, k = _planned_coloring(sim.ctx)
+    kmax = 0
+    for i, m in enumerate(sc.meshes):
+        o = oracle.from_scene(sc, i)
+        ocol, ok = o.colors()
+        assert np.array_equal(col[offs[i]:offs[i + 1]], ocol)
+        kmax = max(kmax, ok)
+    assert k == kmax
+
+
+def test_pools_frozen_after_build():
+    from avbd_amd import _lib
+    v, t = scenes.kuhn_cube(2)
+    ctx = make_ctx()
+    ctx.add_tet_mesh(v, t, 1e5, 0.45, 1000.0)
+    _planned_coloring(ctx)
+    with pytest.raises(_lib.AvbdError):
+        ctx.add_tet_mesh(v, t, 1e5, 0.45, 1000.0)
+
+
+def test_scene_offsets_validated():
+    from avbd_amd import _lib
+    v, t = scenes.kuhn_cube(2)
+    ctx = make_ctx()
+    ctx.add_tet_mesh(v, t, 1e5, 0.45, 1000.0)
+    with pytest.raises(_lib.AvbdError):
+        ctx.set_scenes([0, 5])
+    # a tet spanning two scenes is rejected at build time
+    ctx.set_scenes([0, 5, v.shape[0]])
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.build_graph()
+    assert "spans two scenes" in str(e.value)

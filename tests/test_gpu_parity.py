@@ -1,0 +1,209 @@
+"""GPU parity: the HIP solver (through the C ABI) against the FP64 oracle on the
+same seeded inputs.
+
+Tolerances (stated against the bounding-box extent of the body):
+  * positions after 1 step: max |x_gpu - x_oracle| <= 1e-5 * bbox
+  * after 100 contact-free steps:               <= 1e-3 * bbox
+  * with ground contact (friction switches are discontinuous, FP32 vs FP64
+    may take a different branch near the cone boundary):  <= 2e-3 * bbox
+Colourings are compared bit for bit.
+"""
+import numpy as np
+import pytest
+
+from avbd_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def bbox(x):
+    return float(np.max(x.max(axis=0) - x.min(axis=0)))
+
+
+def gpu_sim(sc, **kw):
+    from avbd_amd.sim import Simulation
+    s = Simulation(sc, **kw)
+    s.setup()
+    return s
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_hanging_cube_one_step(oracle, lanes):
+    banner(f"hanging_cube one step lanes={lanes}")
+    sc = scenes.hanging_cube()
+    g = gpu_sim(sc, lanes_per_vertex=lanes, block_size=256)
+    o = oracle.from_scene(sc)
+    col, k = g.ctx.coloring()
+    ocol, ok = o.colors()
+    assert k == ok and np.array_equal(col, ocol)
+    g.update(1)
+    o.step(1)
+    xg, xo = g.positions(), o.positions()
+    err = np.abs(xg - xo).max() / bbox(xo)
+    banner(f"  rel err {err:.3e}")
+    assert err <= 1e-5
+    # fixed vertices did not move at all
+    m = sc.meshes[0]
+    assert np.array_equal(xg[m.fixed == 1], m.x0[m.fixed == 1].astype(np.float32).astype(np.float64))
+    assert np.allclose(g.ctx.inertial_positions()[m.fixed == 0], o.inertial()[m.fixed == 0], atol=1e-6)
+
+
+def test_hanging_cube_100_steps(oracle):
+    banner("hanging_cube 100 steps")
+    sc = scenes.hanging_cube()
+    g = gpu_sim(sc, lanes_per_vertex=8, block_size=256)
+    o = oracle.from_scene(sc)
+    g.update(100)
+    o.step(100)
+    xg, xo = g.positions(), o.positions()
+    err = np.abs(xg - xo).max() / bbox(xo)
+    banner(f"  rel err {err:.3e}")
+    assert err <= 1e-3
+    _, vg = g.ctx.state()
+    assert np.abs(vg - o.velocities()).max() < 1e-3 * bbox(xo) / sc.dt
+
+
+def test_damping_strain_rate_parity(oracle):
+    """kd > 0 exercises the E_prev pass (NeoHookeanEnergySolver::updateAfterTimeStep)
+    and the damping terms of the vertex solve."""
+    banner("hanging_cube with damping")
+    sc = scenes.hanging_cube(n=6)
+    sc.meshes[0].kd = 0.5
+    g = gpu_sim(sc, lanes_per_vertex=8, block_size=128)
+    o = oracle.from_scene(sc)
+    g.update(1)
+    o.step(1)
+    xo = o.positions()
+    assert np.abs(g.positions() - xo).max() / bbox(xo) <= 1e-5
+    g.update(19)
+    o.step(19)
+    xo = o.positions()
+    err = np.abs(g.positions() - xo).max() / bbox(xo)
+    banner(f"  20-step rel err {err:.3e}")
+    assert err <= 1e-3
+    # E_prev of the last step start (from x_prev) agrees
+    recs = g.ctx.tet_records()
+    Eg = np.array([list(r.E_prev) for r in recs])
+    Eo = o.tets()["E_prev"]
+    assert np.abs(Eg - Eo).max() <= 1e-4 * max(1e-3, np.abs(Eo).max())
+
+
+def test_dropped_cube_ground_contact(oracle):
+    banner("dropped_cube n=8 with ground, 1 and 60 steps")
+    sc = scenes.dropped_cube(n=8, seed=2)
+    g = gpu_sim(sc, lanes_per_vertex=8, block_size=256)
+    o = oracle.from_scene(sc)
+    g.update(1)
+    o.step(1)
+    xo = o.positions()
+    assert np.abs(g.positions() - xo).max() / bbox(xo) <= 1e-5
+    g.update(59)
+    o.step(59)
+    xo = o.positions()
+    xg = g.positions()
+    err = np.abs(xg - xo).max() / bbox(xo)
+    banner(f"  60-step rel err {err:.3e}; min y gpu {xg[:, 1].min():.4f} oracle {xo[:, 1].min():.4f}")
+    assert xo[:, 1].min() < 0.05   # it has landed
+    assert err <= 2e-3
+    # AL duals: total normal multiplier agrees
+    recs = g.ctx.ground_records()
+    lam_g = np.array([r.lambda_n for r in recs])
+    _, rec = o.ground()
+    assert lam_g.sum() == pytest.approx(rec[:, 4].sum(), rel=2e-2, abs=1e-3)
+
+
+def test_free_particles_exact():
+    banner("free particles")
+    from avbd_amd.sim import SimulationContext
+    rng = np.random.default_rng(3)
+    x0 = rng.normal(size=(100, 3)) + np.array([0, 10.0, 0])
+    ctx = SimulationContext(dt=1 / 60, solver_iters=7, lanes_per_vertex=4, block_size=64)
+    ctx.add_particles(x0, mass=np.linspace(0.5, 2.0, 100))
+    ctx.build_graph()
+    dt, g = 1 / 60, 9.81
+    for n in range(1, 4):
+        ctx.step(1)
+        want = x0 + np.array([0, -g * dt * dt * n * (n + 1) / 2, 0])
+        assert np.abs(ctx.positions() - want).max() < 1e-5
+
+
+def test_batched_scenes_match_individual_oracles(oracle):
+    banner("batch of 6 scenes vs per-scene oracles")
+    sc = scenes.cube_batch(n_scenes=6, seed=11)
+    g = gpu_sim(sc, lanes_per_vertex=8, block_size=256)
+    offs = g.offsets
+    col, _ = g.ctx.coloring()
+    orcs = [oracle.from_scene(sc, i) for i in range(len(sc.meshes))]
+    for i, o in enumerate(orcs):
+        assert np.array_equal(col[offs[i]:offs[i + 1]], o.colors()[0])
+    g.update(1)
+    xg = g.positions()
+    for i, o in enumerate(orcs):
+        o.step(1)
+        xo = o.positions()
+        assert np.abs(xg[offs[i]:offs[i + 1]] - xo).max() / bbox(xo) <= 1e-5
+    g.update(29)
+    xg = g.positions()
+    for i, o in enumerate(orcs):
+        o.step(29)
+        xo = o.positions()
+        err = np.abs(xg[offs[i]:offs[i + 1]] - xo).max() / bbox(xo)
+        banner(f"  scene {i} 30-step rel err {err:.3e}")
+        assert err <= 2e-3
+
+
+def test_graph_replay_matches_eager(oracle):
+    """hipGraph replays are checked against eager launches of the same kernels,
+    with eager work and synchronisations between replays."""
+    banner("graph replay vs eager")
+    sc = scenes.dropped_cube(n=6, seed=5)
+    a = gpu_sim(sc, lanes_per_vertex=8, block_size=256, use_graph=True)
+    b = gpu_sim(sc, lanes_per_vertex=8, block_size=256, use_graph=False)
+    for k in range(5):
+        a.update(3)
+        b.update(3)
+        xa, xb = a.positions(), b.positions()
+        assert np.array_equal(xa, xb), f"replay {k} differs"
+        a.ctx.stats()   # eager work between replays
+
+
+def test_stats_match_oracle(oracle):
+    banner("residual / incremental potential")
+    sc = scenes.hanging_cube(n=6)
+    g = gpu_sim(sc, lanes_per_vertex=8, block_size=256)
+    o = oracle.from_scene(sc)
+    g.update(2)
+    o.step(2)
+    sg = g.ctx.stats()
+    so = o.stats()
+    banner(f"  residual gpu {sg.residual_rel:.4e} oracle {so['residual_rel']:.4e}; "
+           f"ip gpu {sg.incremental_potential:.6e} oracle {so['ip']:.6e}")
+    assert sg.residual_rel == pytest.approx(so["residual_rel"], rel=0.05, abs=1e-4)
+    assert sg.incremental_potential == pytest.approx(so["ip"], rel=1e-3)
+
+
+def test_full_size_dropped_cube_one_step(oracle):
+    """35,937 vertices / 196,608 tets with ground: one step against the oracle,
+    then size-independent properties over 40 more steps."""
+    banner("dropped_cube n=32 full size")
+    sc = scenes.dropped_cube(n=32)
+    g = gpu_sim(sc)
+    o = oracle.from_scene(sc)
+    col, k = g.ctx.coloring()
+    ocol, ok = o.colors()
+    assert k == ok == 4 and np.array_equal(col, ocol)
+    g.update(1)
+    o.step(1)
+    xo = o.positions()
+    err = np.abs(g.positions() - xo).max() / bbox(xo)
+    banner(f"  1-step rel err {err:.3e}")
+    assert err <= 1e-5
+    g.update(40)
+    x, v = g.ctx.state()
+    assert np.isfinite(x).all() and np.isfinite(v).all()
+    assert x[:, 1].min() > -0.02            # does not sink through the ground
+    assert bbox(x) == pytest.approx(bbox(xo), rel=0.05)   # no blow-up
