@@ -125,6 +125,7 @@ SIGNATURES = {
     "avbd_get_launch_config": (C.c_int, [_P, _U32, _U32]),
     "avbd_set_timing": (C.c_int, [_P, C.c_int]),
     "avbd_get_solve_timing": (C.c_int, [_P, _D, _U32]),
+    "avbd_calibrate_copy": (C.c_int, [_P, C.c_uint64, C.c_uint32, _D]),
 }
 
 

@@ -797,4 +797,30 @@ int avbd_get_solve_timing(avbd_ctx* c, double* avg_ms, uint32_t* launches) {
     });
 }
 
+int avbd_calibrate_copy(avbd_ctx* c, uint64_t bytes, uint32_t iters, double* avg_ms) {
+    return guarded(c, [&] {
+        require_built(c);
+        const uint64_t n = bytes / sizeof(float4);
+        if (n == 0 || iters == 0) throw std::runtime_error("bytes must be >= 16 and iters >= 1");
+        float4 *src = nullptr, *dst = nullptr;
+        hip_check(hipMalloc(&src, n * sizeof(float4)), "hipMalloc");
+        if (hipMalloc(&dst, n * sizeof(float4)) != hipSuccess) {
+            (void)hipFree(src);
+            throw HipError("hipMalloc failed");
+        }
+        hipError_t e = hipMemsetAsync(src, 0, n * sizeof(float4), c->stream);
+        if (e == hipSuccess) launch_calib_copy(c->stream, src, dst, n);   // warm-up
+        if (e == hipSuccess) e = hipEventRecord(c->ev_a, c->stream);
+        for (uint32_t i = 0; i < iters && e == hipSuccess; i++) launch_calib_copy(c->stream, src, dst, n);
+        if (e == hipSuccess) e = hipEventRecord(c->ev_b, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(c->ev_b);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev_a, c->ev_b);
+        (void)hipFree(src);
+        (void)hipFree(dst);
+        hip_check(e, "calibration copy");
+        if (avg_ms) *avg_ms = ms / iters;
+    });
+}
+
 }  // extern "C"

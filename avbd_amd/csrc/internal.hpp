@@ -98,6 +98,8 @@ void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, 
 void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count);
 void launch_cheb_dual(hipStream_t s, const DevState& d, float omega);
 void launch_velocity(hipStream_t s, const DevState& d, float dt);
+// plain streaming copy of n_float4 16-byte words (HBM counter calibration)
+void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n_float4);
 // residual/IP pass: writes 3 partial sums per block into d.reduce; returns block count
 uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt, float g);
 

@@ -553,3 +553,18 @@ uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bo
 }
 
 }  // namespace avbd
+
+namespace avbd {
+
+namespace {
+__global__ void calib_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n_float4) {
+    hipLaunchKernelGGL(calib_copy_kernel, dim3(2048), dim3(256), 0, s, src, dst, n_float4);
+}
+
+}  // namespace avbd

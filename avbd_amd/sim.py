@@ -178,6 +178,11 @@ class SimulationContext:
     def set_timing(self, enable: bool) -> None:
         self._chk(lib.avbd_set_timing(self.handle, int(enable)))
 
+    def calibrate_copy(self, nbytes: int, iters: int = 10) -> float:
+        ms = C.c_double()
+        self._chk(lib.avbd_calibrate_copy(self.handle, nbytes, iters, C.byref(ms)))
+        return ms.value
+
     def solve_timing(self):
         ms, n = C.c_double(), C.c_uint32()
         self._chk(lib.avbd_get_solve_timing(self.handle, C.byref(ms), C.byref(n)))

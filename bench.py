@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--workload", default="dropped_cube_32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--calibrate", type=int, default=0,
+                    help="also run one plain copy kernel of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the solver on a single MI355X "
@@ -149,6 +151,11 @@ def main():
         except Exception:
             pass
 
+    calib = None
+    if args.calibrate:
+        ms = sim.ctx.calibrate_copy(args.calibrate, 1)
+        calib = {"bytes": args.calibrate, "avg_ms": ms, "GBps_read_plus_write": 2 * args.calibrate / (ms * 1e-3) / 1e9}
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -184,6 +191,8 @@ def main():
         "cpu_baseline": cpu,
         "launch_config": {"lanes_per_vertex": lanes, "block_size": block},
     }
+    if calib:
+        out["copy_calibration"] = calib
     print(json.dumps(out), flush=True)
 
 

@@ -182,6 +182,12 @@ int avbd_get_launch_config(avbd_ctx* ctx, uint32_t* lanes_per_vertex, uint32_t* 
 int avbd_set_timing(avbd_ctx* ctx, int enable);
 int avbd_get_solve_timing(avbd_ctx* ctx, double* avg_ms, uint32_t* launches);
 
+/* Measurement helper with no counterpart in the reference: runs a plain
+ * device-to-device streaming copy kernel (calib_copy_kernel) of `bytes` bytes
+ * `iters` times on the ctx stream and returns the average duration, so that
+ * HBM byte counters can be scaled against a kernel of known traffic. */
+int avbd_calibrate_copy(avbd_ctx* ctx, uint64_t bytes, uint32_t iters, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
