@@ -15,7 +15,10 @@ OUT_DIR = PKG / "_build"
 LIB = OUT_DIR / "libavbd_amd.so"
 INCLUDE = PKG.parent / "include"
 
-SOURCES = ["api.cpp", "host_graph.cpp", "vbd_kernels.hip"]
+SOURCES = ["api.cpp", "host_graph.cpp", "vbd_kernels.hip", "lbvh_kernels.hip"]
+# FP64 Morton codes must round exactly like the reference's CPU code: no FMA
+# contraction in the LBVH kernels.
+EXTRA = {"lbvh_kernels.hip": ["-ffp-contract=off"]}
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 VALU ops
 # (v_pk_{add,mul,fma}_f32) in any kernel of this library.
@@ -44,7 +47,7 @@ def build(verbose: bool = False) -> Path:
     objs = []
     for src in SOURCES:
         obj = OUT_DIR / (Path(src).stem + ".o")
-        cmd = [cc, *HIP_FLAGS, f"-I{INCLUDE}", "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [cc, *HIP_FLAGS, *EXTRA.get(src, []), f"-I{INCLUDE}", "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

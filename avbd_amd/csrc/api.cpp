@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "lbvh.hpp"
 
 using namespace avbd;
 
@@ -27,6 +28,7 @@ struct avbd_ctx {
     std::vector<HostNH> nh;
     std::vector<HostGround> ground;
     std::vector<uint32_t> scene_offsets;   // empty = one scene
+    std::vector<uint32_t> tris;            // collision triangles (3 particles each)
 
     bool planned = false;   // host colour plan done
     bool built = false;     // device buffers uploaded
@@ -36,6 +38,7 @@ struct avbd_ctx {
     uint32_t* d_conflicts = nullptr;
     std::vector<void*> allocations;
     uint32_t reduce_blocks = 0;
+    LbvhDev lb;
 
     LaunchCfg cfg;
     hipGraph_t graph = nullptr;
@@ -179,6 +182,8 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
     const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
     hipStream_t s = c->stream;
     if (c->damping) launch_eprev(s, c->d, dt);
+    // CollisionDetector::detectCollisionsAndRecolor_Parallel runs before the solve
+    if (c->lb.n) launch_lbvh_build(s, c->lb, c->d.x, c->d.v, c->desc.dt);
     launch_step_begin(s, c->d, dt, g);
     double omega = 1;
     size_t ev = 0;
@@ -560,6 +565,37 @@ int avbd_build_graph(avbd_ctx* c) {
         upload(c, d.items, c->plan.items);
         upload(c, c->d_conflicts, c->plan.conflict_list);
         upload_particle_range(c, 0, n);
+
+        // LBVH broad phase buffers
+        const uint32_t nt = (uint32_t)(c->tris.size() / 3);
+        if (nt) {
+            for (uint32_t p : c->tris)
+                if (p >= n) throw std::runtime_error("collision triangle references a particle out of range");
+            LbvhDev& b = c->lb;
+            const uint32_t nn = 2 * nt - 1;
+            b.n = nt;
+            b.tris = dev_alloc<uint32_t>(c, 3 * (size_t)nt);
+            b.aabb = dev_alloc<double>(c, 6 * (size_t)nt);
+            b.sbox = dev_alloc<double>(c, 6);
+            b.keys = dev_alloc<uint64_t>(c, nt);
+            b.keys_sorted = dev_alloc<uint64_t>(c, nt);
+            b.sorted = dev_alloc<uint32_t>(c, nt);
+            b.left = dev_alloc<uint32_t>(c, nn);
+            b.right = dev_alloc<uint32_t>(c, nn);
+            b.parent = dev_alloc<uint32_t>(c, nn);
+            b.range_hi = dev_alloc<uint32_t>(c, nn);
+            b.visited = dev_alloc<uint32_t>(c, nt);
+            b.node_box = dev_alloc<double>(c, 6 * (size_t)nn);
+            b.pair_cap = std::max<uint32_t>(1024u, 64u * nt);
+            b.pairs = dev_alloc<uint32_t>(c, 2 * (size_t)b.pair_cap);
+            b.counters = dev_alloc<uint32_t>(c, 2);
+            upload(c, b.tris, c->tris);
+            // left/right of leaves stay "none" (0xFFFFFFFF) for the whole run
+            std::vector<uint32_t> none(nn, 0xFFFFFFFFu);
+            upload(c, b.left, none);
+            upload(c, b.right, none);
+            upload(c, b.parent, none);
+        }
         c->built = true;
 
         const uint32_t n_solves = c->desc.solver_iters * c->plan.num_colors;
@@ -820,6 +856,70 @@ int avbd_calibrate_copy(avbd_ctx* c, uint64_t bytes, uint32_t iters, double* avg
         (void)hipFree(dst);
         hip_check(e, "calibration copy");
         if (avg_ms) *avg_ms = ms / iters;
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int avbd_add_collision_triangles(avbd_ctx* c, uint32_t n, const uint32_t* tris) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (n && !tris) throw std::runtime_error("tris is null");
+        c->tris.insert(c->tris.end(), tris, tris + 3 * (size_t)n);
+    });
+}
+
+int avbd_detect_collisions(avbd_ctx* c) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!c->lb.n) throw CodeError(AVBD_E_ARG, "no collision primitives");
+        launch_lbvh_build(c->stream, c->lb, c->d.x, c->d.v, c->desc.dt);
+        hip_check(hipGetLastError(), "lbvh launch");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    });
+}
+
+int avbd_download_bvh(avbd_ctx* c, uint64_t* morton, uint32_t* sorted, uint32_t* left, uint32_t* right,
+                      uint32_t* parent, double* node_box, uint32_t* root) {
+    return guarded(c, [&] {
+        require_built(c);
+        const LbvhDev& b = c->lb;
+        if (!b.n) throw CodeError(AVBD_E_ARG, "no collision primitives");
+        const size_t nn = 2 * (size_t)b.n - 1;
+        auto d2h = [&](void* dst, const void* src, size_t bytes) {
+            if (dst) hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+        };
+        d2h(morton, b.keys, b.n * sizeof(uint64_t));
+        d2h(sorted, b.sorted, b.n * sizeof(uint32_t));
+        d2h(left, b.left, nn * sizeof(uint32_t));
+        d2h(right, b.right, nn * sizeof(uint32_t));
+        d2h(parent, b.parent, nn * sizeof(uint32_t));
+        d2h(node_box, b.node_box, 6 * nn * sizeof(double));
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        if (root) *root = b.n > 1 ? b.n : 0;
+    });
+}
+
+int avbd_download_candidates(avbd_ctx* c, uint32_t* pairs, uint32_t cap, uint32_t* count) {
+    return guarded(c, [&] {
+        require_built(c);
+        const LbvhDev& b = c->lb;
+        if (!b.n) throw CodeError(AVBD_E_ARG, "no collision primitives");
+        uint32_t ctr[2] = {0, 0};
+        hip_check(hipMemcpyAsync(ctr, b.counters, sizeof(ctr), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        if (ctr[1] & 1u) throw CodeError(AVBD_E_CAPACITY, "LBVH traversal stack overflow");
+        const uint32_t stored = std::min(ctr[0], b.pair_cap);
+        const uint32_t m = std::min(stored, cap);
+        if (pairs && m)
+            hip_check(hipMemcpyAsync(pairs, b.pairs, 2 * (size_t)m * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                     c->stream),
+                      "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        if (count) *count = ctr[0];
+        if (ctr[0] > b.pair_cap) throw CodeError(AVBD_E_CAPACITY, "candidate pair buffer exceeded");
     });
 }
 

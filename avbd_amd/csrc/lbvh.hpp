@@ -1,0 +1,35 @@
+// lbvh.hpp -- device buffers and launch wrapper of the LBVH broad phase.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace avbd {
+
+// Node numbering follows the reference (collision/LBVH.hpp:10-16): leaves
+// [0, n) in sorted order, internal nodes [n, 2n-1), root n (n > 1).
+struct LbvhDev {
+    uint32_t n = 0;             // triangles
+    uint32_t* tris = nullptr;   // 3 particle indices per triangle
+    double* aabb = nullptr;     // 6 per triangle (min xyz, max xyz), FP64
+    double* sbox = nullptr;     // scene box (6)
+    uint64_t* keys = nullptr;   // Morton code per triangle
+    uint64_t* keys_sorted = nullptr;
+    uint32_t* sorted = nullptr;     // sorted_order: original triangle per sorted slot
+    uint32_t* left = nullptr;       // 2n-1
+    uint32_t* right = nullptr;
+    uint32_t* parent = nullptr;
+    uint32_t* range_hi = nullptr;   // last leaf covered by each node
+    uint32_t* visited = nullptr;    // n-1 arrival counters (reset by the build)
+    double* node_box = nullptr;     // 6 per node
+    uint32_t* pairs = nullptr;      // candidate leaf pairs (a < b)
+    uint32_t pair_cap = 0;
+    uint32_t* counters = nullptr;   // [0] pair count, [1] status (bit0 stack overflow)
+};
+
+bool lbvh_lds_sort_fits(uint32_t n);
+// AABBs + Morton (FP64) -> sort -> Karras -> refit -> self traversal.
+void launch_lbvh_build(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt);
+
+}  // namespace avbd

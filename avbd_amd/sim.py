@@ -112,6 +112,13 @@ class SimulationContext:
         fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
         self._chk(lib.avbd_set_fixed(self.handle, first, fixed.shape[0], _bptr(fixed)))
 
+    def add_collision_triangles(self, tris: np.ndarray) -> None:
+        tris = np.ascontiguousarray(tris, dtype=np.uint32)
+        self._chk(lib.avbd_add_collision_triangles(self.handle, tris.shape[0], _uptr(tris)))
+
+    def detect_collisions(self) -> None:
+        self._chk(lib.avbd_detect_collisions(self.handle))
+
     def set_scenes(self, offsets: Sequence[int]) -> None:
         off = np.ascontiguousarray(offsets, dtype=np.uint32)
         self._chk(lib.avbd_set_scenes(self.handle, off.shape[0] - 1, _uptr(off)))
@@ -234,6 +241,11 @@ class Simulation:
 
     def _add_object(self, m: TetMesh) -> int:
         first = self.ctx.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
+        # CollisionPrimitivePool::addObject(TetMeshObject) registers the surface
+        # faces (collision/CollisionPrimitivePool.cpp:36-50); independent scenes of
+        # a batch are not collided with each other, so batches skip it
+        if m.triangles is not None and len(self.scene.meshes) == 1:
+            self.ctx.add_collision_triangles(m.triangles + first)
         if m.fixed is not None:
             self.ctx.set_fixed(first, m.fixed)
         if m.x0 is not None or m.v0 is not None:

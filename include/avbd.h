@@ -182,6 +182,26 @@ int avbd_get_launch_config(avbd_ctx* ctx, uint32_t* lanes_per_vertex, uint32_t* 
 int avbd_set_timing(avbd_ctx* ctx, int enable);
 int avbd_get_solve_timing(avbd_ctx* ctx, double* avg_ms, uint32_t* launches);
 
+/* Collision primitives and broad phase ------------------------------------
+ * CollisionPrimitivePool::addObject(TetMeshObject) (collision/CollisionPrimitivePool.cpp:36-50):
+ * n triangles (3 particle indices each) join the LBVH.  When any exist, every
+ * avbd_step() first rebuilds the LBVH and self-traverses it, as
+ * CollisionDetector::detectCollisionsAndRecolor_Parallel does before
+ * VBDSolver::solve_Parallel (simulation/SimulationExecutor.hpp:142-156). */
+int avbd_add_collision_triangles(avbd_ctx* ctx, uint32_t n, const uint32_t* tris);
+/* Run the broad phase now on the current state (LBVHBuilder::buildBVH +
+ * LBVHTraversal self traversal, collision/LBVHBuilder.cpp:13-25,
+ * collision/LBVHTraversal.hpp:30-88). */
+int avbd_detect_collisions(avbd_ctx* ctx);
+/* Last LBVH: morton[n] and sorted[n] per primitive, left/right/parent[2n-1],
+ * node_box[6(2n-1)] (min xyz, max xyz), *root (any pointer may be NULL). */
+int avbd_download_bvh(avbd_ctx* ctx, uint64_t* morton, uint32_t* sorted, uint32_t* left, uint32_t* right,
+                      uint32_t* parent, double* node_box, uint32_t* root);
+/* Candidate leaf pairs (a < b, sorted-order leaf indices) of the last
+ * traversal: up to cap pairs into pairs[2*cap]; *count = total found
+ * (> cap means truncated).  Order is unspecified. */
+int avbd_download_candidates(avbd_ctx* ctx, uint32_t* pairs, uint32_t cap, uint32_t* count);
+
 /* Measurement helper with no counterpart in the reference: runs a plain
  * device-to-device streaming copy kernel (calib_copy_kernel) of `bytes` bytes
  * `iters` times on the ctx stream and returns the average duration, so that

@@ -52,6 +52,12 @@ uint32_t oracle_add_particles(void* h, uint32_t n, const double* x, const double
     return static_cast<Simulation*>(h)->add_particles(n, x, m);
 }
 
+void oracle_set_collision_triangles(void* h, uint32_t n, const uint32_t* tris) {
+    auto* s = static_cast<Simulation*>(h);
+    s->tris.assign(tris, tris + 3 * (size_t)n);
+}
+uint64_t oracle_last_candidates(void* h) { return static_cast<Simulation*>(h)->last_candidates; }
+
 uint32_t oracle_num_particles(void* h) { return (uint32_t)static_cast<Simulation*>(h)->x.size(); }
 uint32_t oracle_num_tets(void* h) { return (uint32_t)static_cast<Simulation*>(h)->nh.size(); }
 uint32_t oracle_num_ground(void* h) { return (uint32_t)static_cast<Simulation*>(h)->ground.size(); }

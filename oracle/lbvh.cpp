@@ -242,3 +242,121 @@ uint32_t oracle_lbvh_self_pairs(uint32_t n, const uint32_t* left, const uint32_t
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Karras hierarchy from given codes (stable order by code, index tie-break):
+// the constructTree part of oracle_lbvh_build for unit tests.
+void oracle_lbvh_from_codes(uint32_t n, const uint64_t* morton, uint32_t* sorted, uint32_t* left, uint32_t* right,
+                            uint32_t* parent, uint32_t* root) {
+    // reuse the builder with degenerate geometry by sorting here directly
+    const uint32_t INV = 0xFFFFFFFFu;
+    std::iota(sorted, sorted + n, 0u);
+    std::stable_sort(sorted, sorted + n, [&](uint32_t a, uint32_t b) { return morton[a] < morton[b]; });
+    const uint32_t nn = 2 * n - 1;
+    for (uint32_t i = 0; i < nn; i++) left[i] = right[i] = parent[i] = INV;
+    const int N = (int)n;
+    auto delta = [&](int i, int j) {
+        if (j < 0 || j >= N) return -1;
+        return common_prefix_len(morton[sorted[i]], sorted[i], morton[sorted[j]], sorted[j]);
+    };
+    for (int i = 0; i < N - 1; i++) {
+        const int d = (delta(i, i + 1) - delta(i, i - 1) >= 0) ? 1 : -1;
+        const int dmin = delta(i, i - d);
+        int lmax = 128;
+        while (delta(i, i + lmax * d) > dmin) lmax *= 4;
+        int l = 0;
+        for (int t = lmax; t >= 1; t /= 2)
+            if (delta(i, i + (l + t) * d) > dmin) l += t;
+        const int j = i + l * d;
+        const int dnode = delta(i, j);
+        int s = 0;
+        for (int t = (l + 1) / 2; t >= 1; t = (t == 1 ? 0 : (t + 1) / 2)) {
+            const int ns = s + t;
+            if (ns < l && delta(i, i + ns * d) > dnode) s = ns;
+        }
+        const int gamma = i + s * d + std::min(d, 0);
+        const uint32_t g = (uint32_t)i + n;
+        const uint32_t lc = (std::min(i, j) == gamma) ? (uint32_t)gamma : n + (uint32_t)gamma;
+        const uint32_t rc = (std::max(i, j) == gamma + 1) ? (uint32_t)gamma + 1 : n + (uint32_t)gamma + 1;
+        left[g] = lc;
+        right[g] = rc;
+        parent[lc] = g;
+        parent[rc] = g;
+    }
+    uint32_t node = 0;
+    while (parent[node] != INV) node = parent[node];
+    *root = node;
+}
+
+}  // extern "C"
+
+// Broad phase used by the oracle's time step: build (serial) + self traversal
+// split over worker ranges of leaves (per-leaf descent; the union over leaves
+// is exactly the pair set of traverseSelfIterative).  Results are discarded by
+// the step -- the triangle-triangle narrow phase of the reference is a no-op
+// (collision/CollisionDetector.cpp:589-599) -- but the work is done, as the
+// reference does every step.
+#include "oracle.hpp"
+
+namespace oracle {
+
+void BroadPhase::build(const std::vector<uint32_t>& tris, const std::vector<V3>& x, const std::vector<V3>& v,
+                       double dt) {
+    n = (uint32_t)(tris.size() / 3);
+    if (!n) return;
+    xs.resize(3 * x.size());
+    vs.resize(3 * x.size());
+    for (size_t i = 0; i < x.size(); i++)
+        for (int k = 0; k < 3; k++) {
+            xs[3 * i + k] = x[i][k];
+            vs[3 * i + k] = v[i][k];
+        }
+    const uint32_t nn = 2 * n - 1;
+    aabb.resize(6 * n);
+    morton.resize(n);
+    sorted.resize(n);
+    left.resize(nn);
+    right.resize(nn);
+    parent.resize(nn);
+    node_box.resize(6 * nn);
+    hi.assign(nn, 0);
+    oracle_lbvh_build(n, tris.data(), xs.data(), vs.data(), dt, aabb.data(), morton.data(), sorted.data(),
+                      left.data(), right.data(), parent.data(), node_box.data(), &root);
+    // last leaf under each node (leaves are in sorted order)
+    for (uint32_t l = 0; l < n; l++) hi[l] = l;
+    std::vector<std::pair<uint32_t, bool>> st{{root, false}};
+    while (!st.empty() && n > 1) {
+        auto [k, ex] = st.back();
+        st.pop_back();
+        if (k < n) continue;
+        if (ex) hi[k] = hi[right[k]];
+        else {
+            st.push_back({k, true});
+            st.push_back({right[k], false});
+            st.push_back({left[k], false});
+        }
+    }
+}
+
+uint64_t BroadPhase::traverse_leaves(uint32_t lo, uint32_t hi_leaf) const {
+    uint64_t count = 0;
+    std::vector<uint32_t> stack;
+    for (uint32_t i = lo; i < hi_leaf; i++) {
+        const double* bi = node_box.data() + 6 * i;
+        stack.assign(1, root);
+        while (!stack.empty()) {
+            const uint32_t k = stack.back();
+            stack.pop_back();
+            if (hi[k] <= i || !overlaps(bi, node_box.data() + 6 * k)) continue;
+            if (k < n) count++;
+            else {
+                stack.push_back(right[k]);
+                stack.push_back(left[k]);
+            }
+        }
+    }
+    return count;
+}
+
+}  // namespace oracle

@@ -133,6 +133,20 @@ struct Stats {
     Real ip = 0;             // incremental potential
 };
 
+extern "C" void oracle_lbvh_build(uint32_t n, const uint32_t* tris, const double* x, const double* v, double dt,
+                                  double* aabb, uint64_t* morton, uint32_t* sorted, uint32_t* left,
+                                  uint32_t* right, uint32_t* parent, double* node_box, uint32_t* root);
+
+// LBVH over the collision triangles, rebuilt every step (lbvh.cpp).
+struct BroadPhase {
+    uint32_t n = 0, root = 0;
+    std::vector<double> xs, vs, aabb, node_box;
+    std::vector<uint64_t> morton;
+    std::vector<uint32_t> sorted, left, right, parent, hi;
+    void build(const std::vector<uint32_t>& tris, const std::vector<V3>& x, const std::vector<V3>& v, double dt);
+    uint64_t traverse_leaves(uint32_t lo, uint32_t hi_leaf) const;   // candidate count
+};
+
 class Simulation {
 public:
     Params params;
@@ -143,6 +157,9 @@ public:
 
     std::vector<NHTet> nh;
     std::vector<GroundContact> ground;
+    std::vector<uint32_t> tris;     // collision triangles (3 particle indices each)
+    BroadPhase bp;
+    uint64_t last_candidates = 0;
 
     StaticAdjacency adj;
     Coloring coloring;

@@ -27,6 +27,8 @@ _SIG = {
                                         C.c_double, C.c_double, C.c_int, C.c_double, C.c_double]),
     "oracle_add_particles": (C.c_uint32, [C.c_void_p, C.c_uint32, _D, _D]),
     "oracle_num_particles": (C.c_uint32, [C.c_void_p]),
+    "oracle_set_collision_triangles": (None, [C.c_void_p, C.c_uint32, _U32]),
+    "oracle_last_candidates": (C.c_uint64, [C.c_void_p]),
     "oracle_num_tets": (C.c_uint32, [C.c_void_p]),
     "oracle_num_ground": (C.c_uint32, [C.c_void_p]),
     "oracle_set_positions": (None, [C.c_void_p, _D, C.c_int]),
@@ -116,6 +118,13 @@ class OracleSim:
         x = np.ascontiguousarray(x, dtype=np.float64)
         m = np.ascontiguousarray(m, dtype=np.float64)
         return int(lib.oracle_add_particles(self.h, x.shape[0], dp(x), dp(m)))
+
+    def set_collision_triangles(self, tris):
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        lib.oracle_set_collision_triangles(self.h, t.shape[0], up(t))
+
+    def last_candidates(self) -> int:
+        return int(lib.oracle_last_candidates(self.h))
 
     @property
     def n(self) -> int:
@@ -229,7 +238,7 @@ def lu_solve(A, b):
     return x
 
 
-def from_scene(scene, mesh_index=0) -> OracleSim:
+def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
     """Oracle instance for one mesh of an avbd_amd.scenes.Scene (one reference Simulation)."""
     m = scene.meshes[mesh_index]
     o = OracleSim(scene.dt, scene.g, scene.iters, scene.rho)
@@ -240,5 +249,63 @@ def from_scene(scene, mesh_index=0) -> OracleSim:
         o.set_positions(m.x0, also_prev=True)
     if m.v0 is not None:
         o.set_velocities(m.v0, m.v0)
+    if collide and m.triangles is not None and len(scene.meshes) == 1:
+        o.set_collision_triangles(m.triangles)
     o.build_graph()
     return o
+
+
+# --- LBVH ------------------------------------------------------------------
+_U64 = C.POINTER(C.c_uint64)
+for _k, (_r, _a) in {
+    "oracle_expand_bits21": (C.c_uint64, [C.c_uint64]),
+    "oracle_lbvh_build": (None, [C.c_uint32, _U32, _D, _D, C.c_double, _D, _U64, _U32, _U32, _U32, _U32, _D, _U32]),
+    "oracle_lbvh_self_pairs": (C.c_uint32, [C.c_uint32, _U32, _U32, _D, C.c_uint32, _U32, C.c_uint32]),
+    "oracle_lbvh_from_codes": (None, [C.c_uint32, _U64, _U32, _U32, _U32, _U32, _U32]),
+}.items():
+    _f = getattr(lib, _k)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+def lbvh_build(tris, x, v, dt):
+    """FP64 LBVH over triangles; returns a dict of the reference's arrays."""
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    n = tris.shape[0]
+    nn = 2 * n - 1
+    out = dict(aabb=np.empty((n, 6)), morton=np.empty(n, dtype=np.uint64), sorted=np.empty(n, dtype=np.uint32),
+               left=np.empty(nn, dtype=np.uint32), right=np.empty(nn, dtype=np.uint32),
+               parent=np.empty(nn, dtype=np.uint32), node_box=np.empty((nn, 6)))
+    root = C.c_uint32()
+    lib.oracle_lbvh_build(n, up(tris), dp(x), dp(v), dt, dp(out["aabb"]), out["morton"].ctypes.data_as(_U64),
+                          up(out["sorted"]), up(out["left"]), up(out["right"]), up(out["parent"]),
+                          dp(out["node_box"]), C.byref(root))
+    out["root"] = root.value
+    return out
+
+
+def lbvh_self_pairs(bvh):
+    n = bvh["sorted"].shape[0]
+    cap = 1 << 22
+    buf = np.empty(2 * cap, dtype=np.uint32)
+    cnt = lib.oracle_lbvh_self_pairs(n, up(bvh["left"]), up(bvh["right"]), dp(bvh["node_box"]), bvh["root"],
+                                     up(buf), cap)
+    assert cnt <= cap
+    return buf[:2 * cnt].reshape(-1, 2)
+
+
+def lbvh_from_codes(codes):
+    codes = np.ascontiguousarray(codes, dtype=np.uint64)
+    n = codes.shape[0]
+    nn = 2 * n - 1
+    s = np.empty(n, dtype=np.uint32)
+    l, r, p = (np.empty(nn, dtype=np.uint32) for _ in range(3))
+    root = C.c_uint32()
+    lib.oracle_lbvh_from_codes(n, codes.ctypes.data_as(_U64), up(s), up(l), up(r), up(p), C.byref(root))
+    return dict(sorted=s, left=l, right=r, parent=p, root=root.value)
+
+
+def expand_bits21(v: int) -> int:
+    return int(lib.oracle_expand_bits21(v))

@@ -470,6 +470,11 @@ static inline Real omega_at(uint32_t i, Real rho, Real omega) {
 
 void Simulation::step() {
     uint32_t n = (uint32_t)x.size();
+    if (!tris.empty()) {
+        // CollisionDetector::detectCollisionsAndRecolor_Parallel precedes the solve
+        bp.build(tris, x, v, params.dt);
+        last_candidates = bp.traverse_leaves(0, bp.n);
+    }
     phase_after_timestep(0, (uint32_t)nh.size(), 0, (uint32_t)ground.size());
     phase_inertial(0, n);
     Real omega = 1;
@@ -514,10 +519,22 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
         return;
     }
     SpinBarrier bar(nthreads);
+    std::vector<uint64_t> cand(nthreads, 0);
     auto worker = [&](uint32_t w) {
         uint32_t n = (uint32_t)x.size();
         uint32_t lo, hi, lo2, hi2;
         for (uint32_t s = 0; s < n_steps; s++) {
+            if (!tris.empty()) {
+                if (w == 0) bp.build(tris, x, v, params.dt);
+                bar.arrive_and_wait();
+                range(bp.n, w, nthreads, lo, hi);
+                cand[w] = bp.traverse_leaves(lo, hi);
+                bar.arrive_and_wait();
+                if (w == 0) {
+                    last_candidates = 0;
+                    for (auto cc : cand) last_candidates += cc;
+                }
+            }
             range((uint32_t)nh.size(), w, nthreads, lo, hi);
             phase_after_timestep(lo, hi, 0, 0);
             bar.arrive_and_wait();
