@@ -126,6 +126,10 @@ SIGNATURES = {
     "avbd_set_timing": (C.c_int, [_P, C.c_int]),
     "avbd_get_solve_timing": (C.c_int, [_P, _D, _U32]),
     "avbd_calibrate_copy": (C.c_int, [_P, C.c_uint64, C.c_uint32, _D]),
+    "avbd_add_collision_triangles": (C.c_int, [_P, C.c_uint32, _U32]),
+    "avbd_detect_collisions": (C.c_int, [_P]),
+    "avbd_download_bvh": (C.c_int, [_P, C.POINTER(C.c_uint64), _U32, _U32, _U32, _U32, _D, _U32]),
+    "avbd_download_candidates": (C.c_int, [_P, _U32, C.c_uint32, _U32]),
 }
 
 
