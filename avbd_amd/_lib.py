@@ -125,6 +125,7 @@ SIGNATURES = {
     "avbd_get_launch_config": (C.c_int, [_P, _U32, _U32]),
     "avbd_set_timing": (C.c_int, [_P, C.c_int]),
     "avbd_get_solve_timing": (C.c_int, [_P, _D, _U32]),
+    "avbd_get_broadphase_timing": (C.c_int, [_P, _D]),
     "avbd_calibrate_copy": (C.c_int, [_P, C.c_uint64, C.c_uint32, _D]),
     "avbd_add_collision_triangles": (C.c_int, [_P, C.c_uint32, _U32]),
     "avbd_detect_collisions": (C.c_int, [_P]),

@@ -47,6 +47,8 @@ struct avbd_ctx {
     bool timing = false;
     std::vector<hipEvent_t> events;
     double solve_avg_ms = 0;
+    double lbvh_avg_ms = 0;
+    hipEvent_t ev_lb0 = nullptr, ev_lb1 = nullptr;
     uint32_t solve_launches = 0;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
 
@@ -183,7 +185,11 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
     hipStream_t s = c->stream;
     if (c->damping) launch_eprev(s, c->d, dt);
     // CollisionDetector::detectCollisionsAndRecolor_Parallel runs before the solve
-    if (c->lb.n) launch_lbvh_build(s, c->lb, c->d.x, c->d.v, c->desc.dt);
+    if (c->lb.n) {
+        if (with_events) hip_check(hipEventRecord(c->ev_lb0, s), "hipEventRecord");
+        launch_lbvh_build(s, c->lb, c->d.x, c->d.v, c->desc.dt);
+        if (with_events) hip_check(hipEventRecord(c->ev_lb1, s), "hipEventRecord");
+    }
     launch_step_begin(s, c->d, dt, g);
     double omega = 1;
     size_t ev = 0;
@@ -301,6 +307,8 @@ void avbd_destroy(avbd_ctx* c) {
     destroy_graph(c);
     for (void* p : c->allocations) (void)hipFree(p);
     for (auto ev : c->events) (void)hipEventDestroy(ev);
+    if (c->ev_lb0) (void)hipEventDestroy(c->ev_lb0);
+    if (c->ev_lb1) (void)hipEventDestroy(c->ev_lb1);
     if (c->ev_a) (void)hipEventDestroy(c->ev_a);
     if (c->ev_b) (void)hipEventDestroy(c->ev_b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -584,7 +592,10 @@ int avbd_build_graph(avbd_ctx* c) {
             b.right = dev_alloc<uint32_t>(c, nn);
             b.parent = dev_alloc<uint32_t>(c, nn);
             b.range_hi = dev_alloc<uint32_t>(c, nn);
-            b.visited = dev_alloc<uint32_t>(c, nt);
+            b.bucket_count = dev_alloc<uint32_t>(c, 1u << 16);
+            b.bucket_offset = dev_alloc<uint32_t>(c, (1u << 16) + 1);
+            b.slots = dev_alloc<uint32_t>(c, nt);
+            b.table = dev_alloc<double>(c, 6 * (size_t)lbvh_table_entries(nt));
             b.node_box = dev_alloc<double>(c, 6 * (size_t)nn);
             b.pair_cap = std::max<uint32_t>(1024u, 64u * nt);
             b.pairs = dev_alloc<uint32_t>(c, 2 * (size_t)b.pair_cap);
@@ -600,6 +611,8 @@ int avbd_build_graph(avbd_ctx* c) {
 
         const uint32_t n_solves = c->desc.solver_iters * c->plan.num_colors;
         c->events.resize(2 * n_solves);
+        hip_check(hipEventCreate(&c->ev_lb0), "hipEventCreate");
+        hip_check(hipEventCreate(&c->ev_lb1), "hipEventCreate");
         for (auto& ev : c->events) hip_check(hipEventCreate(&ev), "hipEventCreate");
 
         if (c->desc.lanes_per_vertex == 0 || c->desc.block_size == 0) autotune(c);
@@ -613,7 +626,7 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
         require_built(c);
         hip_check(hipEventRecord(c->ev_a, c->stream), "hipEventRecord");
         if (c->timing) {
-            double total = 0;
+            double total = 0, lbvh_total = 0;
             uint32_t launches = 0;
             for (uint32_t s = 0; s < n_steps; s++) {
                 enqueue_step(c, true);
@@ -624,8 +637,14 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
                     total += ms;
                     launches++;
                 }
+                if (c->lb.n) {
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, c->ev_lb0, c->ev_lb1), "hipEventElapsedTime");
+                    lbvh_total += ms;
+                }
             }
             c->solve_avg_ms = launches ? total / launches : 0;
+            c->lbvh_avg_ms = lbvh_total / std::max(1u, n_steps);
             c->solve_launches = launches;
         } else if (c->graph_exec) {
             for (uint32_t s = 0; s < n_steps; s++) hip_check(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
@@ -824,6 +843,12 @@ int avbd_get_launch_config(avbd_ctx* c, uint32_t* lanes, uint32_t* block) {
 
 int avbd_set_timing(avbd_ctx* c, int enable) {
     return guarded(c, [&] { c->timing = enable != 0; });
+}
+
+int avbd_get_broadphase_timing(avbd_ctx* c, double* avg_ms) {
+    return guarded(c, [&] {
+        if (avg_ms) *avg_ms = c->lbvh_avg_ms;
+    });
 }
 
 int avbd_get_solve_timing(avbd_ctx* c, double* avg_ms, uint32_t* launches) {

@@ -21,14 +21,17 @@ struct LbvhDev {
     uint32_t* right = nullptr;
     uint32_t* parent = nullptr;
     uint32_t* range_hi = nullptr;   // last leaf covered by each node
-    uint32_t* visited = nullptr;    // n-1 arrival counters (reset by the build)
+    uint32_t* bucket_count = nullptr;    // 65,536 (zeroed by the build)
+    uint32_t* bucket_offset = nullptr;   // 65,537
+    uint32_t* slots = nullptr;           // n: primitives grouped by bucket
+    double* table = nullptr;        // aligned range unions, 6 per entry (lbvh_table_entries)
     double* node_box = nullptr;     // 6 per node
     uint32_t* pairs = nullptr;      // candidate leaf pairs (a < b)
     uint32_t pair_cap = 0;
     uint32_t* counters = nullptr;   // [0] pair count, [1] status (bit0 stack overflow)
 };
 
-bool lbvh_lds_sort_fits(uint32_t n);
+uint32_t lbvh_table_entries(uint32_t n);
 // AABBs + Morton (FP64) -> sort -> Karras -> refit -> self traversal.
 void launch_lbvh_build(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt);
 

@@ -11,9 +11,9 @@
 //   * the hierarchy is Karras (2012) with the reference's delta, including the
 //     index tie-break of commonPrefixLen (collision/LBVHBuilder.hpp:33-41,
 //     collision/LBVHBuilder.cpp:181-286);
-//   * refit is bottom-up min/max with an arrival counter per internal node
-//     (collision/LBVHBuilder.cpp:410-453) -- min/max are exact, so the order of
-//     arrival does not change a bit;
+//   * node boxes are the min/max over the leaves a node covers -- what the
+//     reference's bottom-up refit computes (collision/LBVHBuilder.cpp:410-453);
+//     min/max are exact, so any evaluation order gives the same bits;
 //   * traversal reports every leaf pair (a < b) whose closed boxes overlap,
 //     the set produced by LBVHTraversal::traverseSelfIterative
 //     (collision/LBVHTraversal.hpp:30-88); one thread walks the tree per leaf.
@@ -24,8 +24,6 @@ namespace avbd {
 namespace {
 
 constexpr uint32_t INV = 0xFFFFFFFFu;
-constexpr int SORT_THREADS = 1024;
-constexpr int SORT_CHUNK = 12;   // LDS sort handles n <= 12288
 
 __device__ __forceinline__ uint64_t expand_bits21(uint64_t v) {
     v &= 0x1FFFFFull;
@@ -71,11 +69,16 @@ __global__ void tri_aabb_kernel(const uint32_t* __restrict__ tris, uint32_t n, c
     for (int k = 0; k < 6; k++) aabb[6 * t + k] = b[k];
 }
 
+constexpr uint32_t BUCKET_SHIFT = 47;   // codes have 63 bits: the top 16 pick the bucket
+constexpr uint32_t N_BUCKETS = 1u << 16;
+
 // One block: scene box = union of the primitive boxes; also resets the pair
-// counter and status word of this build.
+// counter, the status word and the bucket counts of this build.
 __global__ __launch_bounds__(1024) void scene_box_kernel(const double* __restrict__ aabb, uint32_t n,
-                                                         double* __restrict__ sbox, uint32_t* __restrict__ counters) {
+                                                         double* __restrict__ sbox, uint32_t* __restrict__ counters,
+                                                         uint32_t* __restrict__ bucket_count) {
     __shared__ double red[6][1024];
+    for (uint32_t k = threadIdx.x; k < N_BUCKETS; k += blockDim.x) bucket_count[k] = 0;
     double b[6] = {__DBL_MAX__, __DBL_MAX__, __DBL_MAX__, -__DBL_MAX__, -__DBL_MAX__, -__DBL_MAX__};
     for (uint32_t t = threadIdx.x; t < n; t += blockDim.x)
         for (int k = 0; k < 3; k++) {
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(1024) void scene_box_kernel(const double* __restric
 }
 
 __global__ void morton_kernel(const double* __restrict__ aabb, uint32_t n, const double* __restrict__ sbox,
-                              uint64_t* __restrict__ keys) {
+                              uint64_t* __restrict__ keys, uint32_t* __restrict__ bucket_count) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     uint64_t code = 0;
@@ -110,141 +113,167 @@ __global__ void morton_kernel(const double* __restrict__ aabb, uint32_t n, const
         code |= morton_axis((c - sbox[k]) / ext) << k;
     }
     keys[t] = code;
+    atomicAdd(&bucket_count[code >> BUCKET_SHIFT], 1u);
 }
 
-// Stable LSD radix sort of (key, original index), 4-bit digits, in LDS, one
-// block of 1024 threads; thread t owns elements [12t, 12t+12) so the pass is
-// stable.  Per-thread digit counts are packed 16 bits per digit into four
-// 64-bit words and scanned across the block.
-__device__ __forceinline__ uint32_t field16(uint64_t w0, uint64_t w1, uint64_t w2, uint64_t w3, uint32_t d) {
-    const uint64_t w = (d < 4) ? w0 : (d < 8) ? w1 : (d < 12) ? w2 : w3;
-    return static_cast<uint32_t>((w >> (16 * (d & 3))) & 0xFFFFull);
-}
-
-__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v, int lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint64_t o = __shfl_up(v, off);
-        if (lane >= off) v += o;
-    }
-    return v;
-}
-
-__global__ __launch_bounds__(SORT_THREADS) void lds_radix_sort_kernel(const uint64_t* __restrict__ keys_in, uint32_t n,
-                                                                      uint64_t* __restrict__ keys_out,
-                                                                      uint32_t* __restrict__ vals_out) {
-    __shared__ uint64_t skey[SORT_THREADS * SORT_CHUNK];
-    __shared__ uint32_t sval[SORT_THREADS * SORT_CHUNK];
-    __shared__ uint64_t wsum[16][4];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    uint64_t k_[SORT_CHUNK];
-    uint32_t v_[SORT_CHUNK];
-#pragma unroll
-    for (int k = 0; k < SORT_CHUNK; k++) {
-        const uint32_t e = t * SORT_CHUNK + k;
-        k_[k] = e < n ? keys_in[e] : ~0ull;   // sentinels sort last, in index order
-        v_[k] = e;
-    }
-    for (int shift = 0; shift < 64; shift += 4) {
-        uint64_t c[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < SORT_CHUNK; k++) {
-            const uint32_t d = (k_[k] >> shift) & 15u;
-            const uint64_t inc = 1ull << (16 * (d & 3));
-            c[0] += (d >> 2) == 0 ? inc : 0;
-            c[1] += (d >> 2) == 1 ? inc : 0;
-            c[2] += (d >> 2) == 2 ? inc : 0;
-            c[3] += (d >> 2) == 3 ? inc : 0;
-        }
-        uint64_t incl[4];
-#pragma unroll
-        for (int w = 0; w < 4; w++) incl[w] = wave_incl_scan64(c[w], lane);
-        if (lane == 63)
-            for (int w = 0; w < 4; w++) wsum[wave][w] = incl[w];
+// Sort by (code, original index) -- the order a stable LSD radix sort of the
+// codes produces -- in short fully parallel passes: morton_kernel counts codes
+// per bucket of their top 16 bits; one block scans the 65,536 counts; every
+// primitive takes a slot in its bucket; every primitive counts the keys of
+// its bucket that precede it and lands at bucket offset + count.
+__global__ __launch_bounds__(1024) void bucket_scan_kernel(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
+                                                           uint32_t n) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;   // 64 consecutive buckets per thread
+    uint32_t local = 0;
+    for (uint32_t k = 0; k < 64; k++) local += count[64 * t + k];
+    part[t] = local;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0;
         __syncthreads();
-        uint64_t pre[4] = {0, 0, 0, 0}, tot[4] = {0, 0, 0, 0};
-        for (int q = 0; q < 16; q++)
-            for (int w = 0; w < 4; w++) {
-                if (q < wave) pre[w] += wsum[q][w];
-                tot[w] += wsum[q][w];
-            }
-        uint64_t excl[4];
-        for (int w = 0; w < 4; w++) excl[w] = pre[w] + incl[w] - c[w];
-        uint32_t base[16];
-        uint32_t run = 0;
-        for (uint32_t d = 0; d < 16; d++) {
-            base[d] = run;
-            run += field16(tot[0], tot[1], tot[2], tot[3], d);
-        }
-        uint64_t seen[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < SORT_CHUNK; k++) {
-            const uint32_t d = (k_[k] >> shift) & 15u;
-            uint32_t b = 0;
-            for (uint32_t q = 0; q < 16; q++) b = (q == d) ? base[q] : b;
-            const uint32_t pos = b + field16(excl[0], excl[1], excl[2], excl[3], d) +
-                                 field16(seen[0], seen[1], seen[2], seen[3], d);
-            const uint64_t inc = 1ull << (16 * (d & 3));
-            seen[0] += (d >> 2) == 0 ? inc : 0;
-            seen[1] += (d >> 2) == 1 ? inc : 0;
-            seen[2] += (d >> 2) == 2 ? inc : 0;
-            seen[3] += (d >> 2) == 3 ? inc : 0;
-            skey[pos] = k_[k];
-            sval[pos] = v_[k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < SORT_CHUNK; k++) {
-            k_[k] = skey[t * SORT_CHUNK + k];
-            v_[k] = sval[t * SORT_CHUNK + k];
-        }
+        part[t] += v;
         __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k < SORT_CHUNK; k++) {
-        const uint32_t e = t * SORT_CHUNK + k;
-        if (e < n) {
-            keys_out[e] = k_[k];
-            vals_out[e] = v_[k];
-        }
+    uint32_t run = part[t] - local;
+    for (uint32_t k = 0; k < 64; k++) {
+        const uint32_t b = 64 * t + k;
+        const uint32_t c = count[b];
+        offset[b] = run;
+        run += c;
+        count[b] = 0;   // reused as the fill cursor by bucket_place_kernel
     }
+    if (t == 0) offset[N_BUCKETS] = n;
 }
 
-// Any n: rank of (key, index) by counting smaller keys (tiles staged in LDS).
-__global__ __launch_bounds__(256) void rank_sort_kernel(const uint64_t* __restrict__ keys, uint32_t n,
-                                                        uint64_t* __restrict__ keys_out,
-                                                        uint32_t* __restrict__ vals_out) {
-    __shared__ uint64_t tile[256];
+__global__ void bucket_place_kernel(const uint64_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ offset,
+                                    uint32_t* __restrict__ cursor, uint32_t* __restrict__ slots) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t ki = i < n ? keys[i] : 0;
-    uint32_t rank = 0;
-    for (uint32_t base = 0; base < n; base += 256) {
-        const uint32_t j = base + threadIdx.x;
-        tile[threadIdx.x] = j < n ? keys[j] : ~0ull;
-        __syncthreads();
-        const uint32_t m = n - base < 256 ? n - base : 256;
-        for (uint32_t q = 0; q < m; q++) {
-            const uint64_t kj = tile[q];
-            rank += (kj < ki || (kj == ki && base + q < i)) ? 1u : 0u;
+    if (i >= n) return;
+    const uint32_t b = (uint32_t)(keys[i] >> BUCKET_SHIFT);
+    slots[offset[b] + atomicAdd(&cursor[b], 1u)] = i;
+}
+
+__global__ void bucket_rank_kernel(const uint64_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ offset,
+                                   const uint32_t* __restrict__ slots, uint64_t* __restrict__ keys_out,
+                                   uint32_t* __restrict__ vals_out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t ki = keys[i];
+    const uint32_t b = (uint32_t)(ki >> BUCKET_SHIFT);
+    const uint32_t lo = offset[b], hi = offset[b + 1];
+    uint32_t rank = lo;
+    for (uint32_t s = lo; s < hi; s++) {
+        const uint32_t j = slots[s];
+        const uint64_t kj = keys[j];
+        rank += (kj < ki || (kj == ki && j < i)) ? 1u : 0u;
+    }
+    keys_out[rank] = ki;
+    vals_out[rank] = i;
+}
+
+// Aligned range-union table over the sorted leaves: entry (k, j) of level k is
+// the union of leaves [j 2^k, (j+1) 2^k).  An internal node covers a
+// contiguous leaf range, so its box is the union of O(2 log n) entries.
+__device__ __forceinline__ uint32_t level_offset(uint32_t n, uint32_t k) {
+    uint32_t off = 0;
+    for (uint32_t q = 0; q < k; q++) off += (n + (1u << q) - 1) >> q;
+    return off;
+}
+
+__global__ __launch_bounds__(256) void leaf_table_kernel(const double* __restrict__ aabb,
+                                                         const uint32_t* __restrict__ sorted, uint32_t n,
+                                                         double* __restrict__ node_box, double* __restrict__ table,
+                                                         uint32_t levels_local) {
+    __shared__ double sb[6][256];
+    const uint32_t l = blockIdx.x * 256 + threadIdx.x;
+    double b[6];
+    if (l < n) {
+        const uint32_t prim = sorted[l];
+        for (int k = 0; k < 6; k++) b[k] = aabb[6 * prim + k];
+        for (int k = 0; k < 6; k++) {
+            node_box[6 * l + k] = b[k];
+            table[6 * l + k] = b[k];
+        }
+    } else {
+        for (int k = 0; k < 3; k++) {
+            b[k] = __DBL_MAX__;
+            b[3 + k] = -__DBL_MAX__;
+        }
+    }
+    for (int k = 0; k < 6; k++) sb[k][threadIdx.x] = b[k];
+    __syncthreads();
+    for (uint32_t lev = 1; lev <= levels_local; lev++) {
+        const uint32_t w = 1u << lev;
+        if ((threadIdx.x & (w - 1)) == 0) {
+            const uint32_t o = threadIdx.x + (w >> 1);
+            for (int k = 0; k < 3; k++) {
+                sb[k][threadIdx.x] = fmin(sb[k][threadIdx.x], sb[k][o]);
+                sb[3 + k][threadIdx.x] = fmax(sb[3 + k][threadIdx.x], sb[3 + k][o]);
+            }
+            const uint32_t j = l >> lev;
+            if (j < ((n + w - 1) >> lev)) {
+                const uint32_t base = level_offset(n, lev) + j;
+                for (int k = 0; k < 6; k++) table[6 * base + k] = sb[k][threadIdx.x];
+            }
         }
         __syncthreads();
     }
-    if (i < n) {
-        keys_out[rank] = ki;
-        vals_out[rank] = i;
+}
+
+// Levels above one block: a single block folds pairs level by level.
+__global__ __launch_bounds__(1024) void upper_table_kernel(uint32_t n, double* table, uint32_t first_level,
+                                                           uint32_t last_level) {
+    for (uint32_t lev = first_level; lev <= last_level; lev++) {
+        const uint32_t cnt = (n + (1u << lev) - 1) >> lev;
+        const uint32_t prev_cnt = (n + (1u << (lev - 1)) - 1) >> (lev - 1);
+        const uint32_t po = level_offset(n, lev - 1), o = level_offset(n, lev);
+        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+            const double* a = table + 6 * (po + 2 * j);
+            double b[6];
+            for (int k = 0; k < 6; k++) b[k] = a[k];
+            if (2 * j + 1 < prev_cnt) {
+                const double* c = a + 6;
+                for (int k = 0; k < 3; k++) {
+                    b[k] = fmin(b[k], c[k]);
+                    b[3 + k] = fmax(b[3 + k], c[3 + k]);
+                }
+            }
+            for (int k = 0; k < 6; k++) table[6 * (o + j) + k] = b[k];
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void range_union(const double* __restrict__ table, uint32_t n, uint32_t lo, uint32_t hi,
+                                            double* box) {
+    for (int k = 0; k < 3; k++) {
+        box[k] = __DBL_MAX__;
+        box[3 + k] = -__DBL_MAX__;
+    }
+    uint32_t l = lo;
+    const uint32_t r = hi + 1;
+    while (l < r) {
+        uint32_t k = 31u - (uint32_t)__builtin_clz(r - l);   // floor(log2(r - l))
+        if (l) k = min(k, (uint32_t)__builtin_ctz(l));       // alignment of l
+        const double* e = table + 6 * (level_offset(n, k) + (l >> k));
+        for (int c = 0; c < 3; c++) {
+            box[c] = fmin(box[c], e[c]);
+            box[3 + c] = fmax(box[3 + c], e[3 + c]);
+        }
+        l += 1u << k;
     }
 }
 
 __global__ void karras_kernel(const uint64_t* __restrict__ code, const uint32_t* __restrict__ sorted, uint32_t n,
                               uint32_t* __restrict__ left, uint32_t* __restrict__ right,
                               uint32_t* __restrict__ parent, uint32_t* __restrict__ range_hi,
-                              uint32_t* __restrict__ visited) {
+                              const double* __restrict__ table, double* __restrict__ node_box) {
     const int N = (int)n;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) parent[N > 1 ? N : 0] = INV;   // the root has no parent
     if (i < N) range_hi[i] = (uint32_t)i;      // a leaf covers itself
     if (i >= N - 1) return;
-    visited[i] = 0;
     auto delta = [&](int a, int b) -> int {
         if (b < 0 || b >= N) return -1;
         return common_prefix_len(code[a], sorted[a], code[b], sorted[b]);
@@ -272,72 +301,71 @@ __global__ void karras_kernel(const uint64_t* __restrict__ code, const uint32_t*
     parent[lc] = g;
     parent[rc] = g;
     range_hi[g] = (uint32_t)max(i, j);
-}
-
-__device__ __forceinline__ double aload(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ void refit_kernel(const double* __restrict__ aabb, const uint32_t* __restrict__ sorted, uint32_t n,
-                             const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
-                             const uint32_t* __restrict__ parent, uint32_t* __restrict__ visited,
-                             double* node_box) {
-    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
-    const uint32_t prim = sorted[l];
-    for (int k = 0; k < 6; k++) node_box[6 * l + k] = aabb[6 * prim + k];
-    __threadfence();
-    uint32_t node = parent[l];
-    while (node != INV) {
-        if (atomicAdd(&visited[node - n], 1u) == 0) return;   // the sibling finishes the union
-        __threadfence();
-        const uint32_t a = left[node], b = right[node];
-        double box[6];
-        for (int k = 0; k < 3; k++) {
-            box[k] = fmin(aload(node_box + 6 * a + k), aload(node_box + 6 * b + k));
-            box[3 + k] = fmax(aload(node_box + 6 * a + 3 + k), aload(node_box + 6 * b + 3 + k));
-        }
-        for (int k = 0; k < 6; k++) node_box[6 * node + k] = box[k];
-        __threadfence();
-        node = parent[node];
-    }
+    double box[6];
+    range_union(table, n, (uint32_t)min(i, j), (uint32_t)max(i, j), box);
+    for (int k = 0; k < 6; k++) node_box[6 * g + k] = box[k];
 }
 
 __device__ __forceinline__ bool overlap(const double* a, const double* b) {
     return a[0] <= b[3] && a[3] >= b[0] && a[1] <= b[4] && a[4] >= b[1] && a[2] <= b[5] && a[5] >= b[2];
 }
 
-constexpr int TRAVERSE_STACK = 128;
+constexpr int TRAVERSE_STACK = 64;
+constexpr int TRAVERSE_BLOCK = 64;
 
-__global__ void traverse_kernel(uint32_t n, uint32_t root, const uint32_t* __restrict__ left,
-                                const uint32_t* __restrict__ right, const uint32_t* __restrict__ range_hi,
-                                const double* __restrict__ node_box, uint32_t* __restrict__ pairs, uint32_t cap,
-                                uint32_t* __restrict__ counters) {
+// One thread per leaf i; descends from the root testing both children of a
+// node at once, keeps the deferred child on a per-thread stack in LDS, and
+// emits (i, j) for overlapping leaves j > i (subtrees whose last leaf is <= i
+// are skipped).
+__global__ __launch_bounds__(TRAVERSE_BLOCK) void traverse_kernel(uint32_t n, uint32_t root,
+                                                                  const uint32_t* __restrict__ left,
+                                                                  const uint32_t* __restrict__ right,
+                                                                  const uint32_t* __restrict__ range_hi,
+                                                                  const double* __restrict__ node_box,
+                                                                  uint32_t* __restrict__ pairs, uint32_t cap,
+                                                                  uint32_t* __restrict__ counters) {
+    __shared__ uint32_t stack_mem[TRAVERSE_STACK * TRAVERSE_BLOCK];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || n < 2) return;
     double bi[6];
     for (int k = 0; k < 6; k++) bi[k] = node_box[6 * i + k];
-    uint32_t stack[TRAVERSE_STACK];
+    auto hit = [&](uint32_t c) { return range_hi[c] > i && overlap(bi, node_box + 6 * c); };
+    auto emit = [&](uint32_t j) {
+        const uint32_t slot = atomicAdd(&counters[0], 1u);
+        if (slot < cap) {
+            pairs[2 * slot] = i;
+            pairs[2 * slot + 1] = j;
+        }
+    };
     int sp = 0;
-    stack[sp++] = root;
-    while (sp > 0) {
-        const uint32_t node = stack[--sp];
-        if (range_hi[node] <= i) continue;             // every leaf below is <= i
-        if (!overlap(bi, node_box + 6 * node)) continue;
-        if (node < n) {
-            const uint32_t slot = atomicAdd(&counters[0], 1u);
-            if (slot < cap) {
-                pairs[2 * slot] = i;
-                pairs[2 * slot + 1] = node;
+    uint32_t node = root;
+    if (!hit(root)) return;
+    while (true) {
+        const uint32_t a = left[node], b = right[node];
+        bool ha = hit(a), hb = hit(b);
+        if (ha && a < n) {
+            emit(a);
+            ha = false;
+        }
+        if (hb && b < n) {
+            emit(b);
+            hb = false;
+        }
+        if (ha && hb) {
+            if (sp >= TRAVERSE_STACK) {
+                atomicOr(&counters[1], 1u);   // stack overflow: reported, traversal truncated
+                return;
             }
-            continue;
+            stack_mem[(sp++) * TRAVERSE_BLOCK + threadIdx.x] = b;
+            node = a;
+        } else if (ha) {
+            node = a;
+        } else if (hb) {
+            node = b;
+        } else {
+            if (sp == 0) return;
+            node = stack_mem[(--sp) * TRAVERSE_BLOCK + threadIdx.x];
         }
-        if (sp + 2 > TRAVERSE_STACK) {
-            atomicOr(&counters[1], 1u);   // stack overflow: reported, traversal truncated
-            break;
-        }
-        stack[sp++] = right[node];
-        stack[sp++] = left[node];
     }
 }
 
@@ -345,26 +373,33 @@ inline uint32_t nblk(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 }  // namespace
 
-bool lbvh_lds_sort_fits(uint32_t n) { return n <= (uint32_t)(SORT_THREADS * SORT_CHUNK); }
+uint32_t lbvh_table_entries(uint32_t n) {
+    uint32_t total = 0;
+    for (uint32_t k = 0; (1u << k) < 2 * n; k++) total += (n + (1u << k) - 1) >> k;
+    return total;
+}
 
 void launch_lbvh_build(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt) {
     const uint32_t n = b.n;
     if (n == 0) return;
+    uint32_t top = 0;   // highest table level: 2^top >= n
+    while ((1u << top) < n) top++;
     hipLaunchKernelGGL(tri_aabb_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.tris, n, x, v, dt, b.aabb);
-    hipLaunchKernelGGL(scene_box_kernel, dim3(1), dim3(1024), 0, s, b.aabb, n, b.sbox, b.counters);
-    hipLaunchKernelGGL(morton_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.aabb, n, b.sbox, b.keys);
-    if (lbvh_lds_sort_fits(n))
-        hipLaunchKernelGGL(lds_radix_sort_kernel, dim3(1), dim3(SORT_THREADS), 0, s, b.keys, n, b.keys_sorted,
-                           b.sorted);
-    else
-        hipLaunchKernelGGL(rank_sort_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.keys, n, b.keys_sorted,
-                           b.sorted);
+    hipLaunchKernelGGL(scene_box_kernel, dim3(1), dim3(1024), 0, s, b.aabb, n, b.sbox, b.counters, b.bucket_count);
+    hipLaunchKernelGGL(morton_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.aabb, n, b.sbox, b.keys,
+                       b.bucket_count);
+    hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(1024), 0, s, b.bucket_count, b.bucket_offset, n);
+    hipLaunchKernelGGL(bucket_place_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.keys, n, b.bucket_offset,
+                       b.bucket_count, b.slots);
+    hipLaunchKernelGGL(bucket_rank_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.keys, n, b.bucket_offset, b.slots,
+                       b.keys_sorted, b.sorted);
+    hipLaunchKernelGGL(leaf_table_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.aabb, b.sorted, n, b.node_box,
+                       b.table, top < 8 ? top : 8u);
+    if (top > 8) hipLaunchKernelGGL(upper_table_kernel, dim3(1), dim3(1024), 0, s, n, b.table, 9u, top);
     hipLaunchKernelGGL(karras_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.keys_sorted, b.sorted, n, b.left,
-                       b.right, b.parent, b.range_hi, b.visited);
-    hipLaunchKernelGGL(refit_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.aabb, b.sorted, n, b.left, b.right,
-                       b.parent, b.visited, b.node_box);
-    hipLaunchKernelGGL(traverse_kernel, dim3(nblk(n, 128)), dim3(128), 0, s, n, n > 1 ? n : 0u, b.left, b.right,
-                       b.range_hi, b.node_box, b.pairs, b.pair_cap, b.counters);
+                       b.right, b.parent, b.range_hi, b.table, b.node_box);
+    hipLaunchKernelGGL(traverse_kernel, dim3(nblk(n, TRAVERSE_BLOCK)), dim3(TRAVERSE_BLOCK), 0, s, n,
+                       n > 1 ? n : 0u, b.left, b.right, b.range_hi, b.node_box, b.pairs, b.pair_cap, b.counters);
 }
 
 }  // namespace avbd

@@ -171,6 +171,17 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
+// each with its own L2.  Remap so that XCD x runs a contiguous range of
+// logical blocks: work items are ordered by vertex index, i.e. spatially, so
+// each XCD's L2 keeps the tets and positions of one slab of the mesh across
+// colour phases and iterations.
+__device__ __forceinline__ uint32_t xcd_logical_block() {
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    const uint32_t x = b % 8u, q = b / 8u;
+    return x * (G / 8u) + min(x, G % 8u) + q;
+}
+
 template <int L, bool DAMP>
 __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restrict__ items, uint32_t n_items,
                                                            const uint32_t* __restrict__ inc,
@@ -179,7 +190,7 @@ __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restri
                                                            const float4* __restrict__ y,
                                                            const GroundRec* __restrict__ ground, float4* x_out,
                                                            float4* x_buf, float inv_dt, float inv_dt2) {
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L;
     const uint32_t lane = tid % L;
     if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)

@@ -190,6 +190,11 @@ class SimulationContext:
         self._chk(lib.avbd_calibrate_copy(self.handle, nbytes, iters, C.byref(ms)))
         return ms.value
 
+    def broadphase_timing(self) -> float:
+        ms = C.c_double()
+        self._chk(lib.avbd_get_broadphase_timing(self.handle, C.byref(ms)))
+        return ms.value
+
     def solve_timing(self):
         ms, n = C.c_double(), C.c_uint32()
         self._chk(lib.avbd_get_solve_timing(self.handle, C.byref(ms), C.byref(n)))

@@ -181,6 +181,9 @@ int avbd_get_launch_config(avbd_ctx* ctx, uint32_t* lanes_per_vertex, uint32_t* 
  * launches over the last avbd_step() call and how many it covers. */
 int avbd_set_timing(avbd_ctx* ctx, int enable);
 int avbd_get_solve_timing(avbd_ctx* ctx, double* avg_ms, uint32_t* launches);
+/* Average per-step duration (ms) of the broad phase over the last timed
+ * avbd_step() call (0 without collision triangles). */
+int avbd_get_broadphase_timing(avbd_ctx* ctx, double* avg_ms);
 
 /* Collision primitives and broad phase ------------------------------------
  * CollisionPrimitivePool::addObject(TetMeshObject) (collision/CollisionPrimitivePool.cpp:36-50):
