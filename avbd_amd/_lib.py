@@ -30,6 +30,8 @@ TRIANGLE_ROD_COLLISION = 5
 ONE_SIDED_FIXED_JOINT = 6
 
 FLAG_NO_GRAPH = 1
+FLAG_SERIAL_BROADPHASE = 2
+FLAG_NO_PERSISTENT = 4
 
 
 class SceneDesc(C.Structure):
@@ -123,6 +125,7 @@ SIGNATURES = {
     "avbd_num_particles": (C.c_uint32, [_P]),
     "avbd_num_energies": (C.c_uint32, [_P, C.c_int]),
     "avbd_get_launch_config": (C.c_int, [_P, _U32, _U32]),
+    "avbd_get_step_mode": (C.c_int, [_P, _U32]),
     "avbd_set_timing": (C.c_int, [_P, C.c_int]),
     "avbd_get_solve_timing": (C.c_int, [_P, _D, _U32]),
     "avbd_get_broadphase_timing": (C.c_int, [_P, _D]),

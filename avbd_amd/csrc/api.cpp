@@ -21,6 +21,8 @@ struct avbd_ctx {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side_stream = nullptr;   // broad phase, concurrent with the solve
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     // FP64 staging (ParticlePool / EnergyRegistry layout, reference Real = double)
     std::vector<double> x, v, v_prev, x_prev, mass;
@@ -39,6 +41,13 @@ struct avbd_ctx {
     std::vector<void*> allocations;
     uint32_t reduce_blocks = 0;
     LbvhDev lb;
+
+    // persistent step kernel
+    bool persistent = false;
+    uint32_t pgrid = 0;
+    StepArgs sargs;
+    hipEvent_t ev_st0 = nullptr, ev_st1 = nullptr;
+    double step_kernel_avg_ms = 0;
 
     LaunchCfg cfg;
     hipGraph_t graph = nullptr;
@@ -183,13 +192,36 @@ double omega_at(uint32_t i, double rho, double omega) {
 void enqueue_step(avbd_ctx* c, bool with_events) {
     const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
     hipStream_t s = c->stream;
-    if (c->damping) launch_eprev(s, c->d, dt);
-    // CollisionDetector::detectCollisionsAndRecolor_Parallel runs before the solve
+    // CollisionDetector::detectCollisionsAndRecolor_Parallel runs before the solve.
+    // Triangle-triangle pairs produce no contact energy (the reference's narrow
+    // phase for them is a no-op, collision/CollisionDetector.cpp:589-599), so
+    // nothing in this step's solve consumes the broad-phase output: after the
+    // bounds kernel has read the start-of-step state, the rest of the build
+    // and the traversal run on a second stream, concurrently with the solve,
+    // and join at the end of the step.
+    const bool fork = c->lb.n && c->side_stream;
     if (c->lb.n) {
+        hipStream_t sb = fork ? c->side_stream : s;
         if (with_events) hip_check(hipEventRecord(c->ev_lb0, s), "hipEventRecord");
-        launch_lbvh_build(s, c->lb, c->d.x, c->d.v, c->desc.dt);
-        if (with_events) hip_check(hipEventRecord(c->ev_lb1, s), "hipEventRecord");
+        launch_lbvh_bounds(s, c->lb, c->d.x, c->d.v, c->desc.dt);
+        if (fork) {
+            hip_check(hipEventRecord(c->ev_fork, s), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(sb, c->ev_fork, 0), "hipStreamWaitEvent");
+        }
+        launch_lbvh_tree(sb, c->lb);
+        if (with_events) hip_check(hipEventRecord(c->ev_lb1, sb), "hipEventRecord");
+        if (fork) hip_check(hipEventRecord(c->ev_join, sb), "hipEventRecord");
     }
+    if (c->persistent) {
+        // the whole solve in one launch with grid barriers between phases
+        if (with_events) hip_check(hipEventRecord(c->ev_st0, s), "hipEventRecord");
+        launch_step_persistent(s, c->sargs, c->cfg.lanes, c->damping, c->pgrid);
+        if (with_events) hip_check(hipEventRecord(c->ev_st1, s), "hipEventRecord");
+        if (fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
+        hip_check(hipGetLastError(), "kernel launch");
+        return;
+    }
+    if (c->damping) launch_eprev(s, c->d, dt);
     launch_step_begin(s, c->d, dt, g);
     double omega = 1;
     size_t ev = 0;
@@ -206,7 +238,19 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
         launch_cheb_dual(s, c->d, (float)omega);
     }
     launch_velocity(s, c->d, dt);
+    if (fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
     hip_check(hipGetLastError(), "kernel launch");
+}
+
+// After a synchronisation: report (and clear) an aborted persistent step.
+void check_abort(avbd_ctx* c) {
+    if (!c->persistent || !c->sargs.bar) return;
+    GridBarrier b{};
+    hip_check(hipMemcpy(&b, c->sargs.bar, sizeof(b), hipMemcpyDeviceToHost), "D2H");
+    if (b.abort) {
+        hip_check(hipMemset(c->sargs.bar, 0, sizeof(GridBarrier)), "hipMemset");
+        throw HipError("persistent step kernel: a grid barrier wait timed out and the step was aborted");
+    }
 }
 
 void destroy_graph(avbd_ctx* c) {
@@ -307,10 +351,15 @@ void avbd_destroy(avbd_ctx* c) {
     destroy_graph(c);
     for (void* p : c->allocations) (void)hipFree(p);
     for (auto ev : c->events) (void)hipEventDestroy(ev);
+    if (c->ev_st0) (void)hipEventDestroy(c->ev_st0);
+    if (c->ev_st1) (void)hipEventDestroy(c->ev_st1);
     if (c->ev_lb0) (void)hipEventDestroy(c->ev_lb0);
     if (c->ev_lb1) (void)hipEventDestroy(c->ev_lb1);
     if (c->ev_a) (void)hipEventDestroy(c->ev_a);
     if (c->ev_b) (void)hipEventDestroy(c->ev_b);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -509,6 +558,11 @@ int avbd_build_graph(avbd_ctx* c) {
         // device part
         hip_check(hipSetDevice(c->device), "hipSetDevice");
         if (!c->stream) hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        if (!c->tris.empty() && !(c->desc.flags & AVBD_FLAG_SERIAL_BROADPHASE)) {
+            hip_check(hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
+        }
         if (!c->ev_a) hip_check(hipEventCreate(&c->ev_a), "hipEventCreate");
         if (!c->ev_b) hip_check(hipEventCreate(&c->ev_b), "hipEventCreate");
         c->damping = false;
@@ -592,8 +646,8 @@ int avbd_build_graph(avbd_ctx* c) {
             b.right = dev_alloc<uint32_t>(c, nn);
             b.parent = dev_alloc<uint32_t>(c, nn);
             b.range_hi = dev_alloc<uint32_t>(c, nn);
-            b.bucket_count = dev_alloc<uint32_t>(c, 1u << 16);
-            b.bucket_offset = dev_alloc<uint32_t>(c, (1u << 16) + 1);
+            b.bucket_count = dev_alloc<uint32_t>(c, 1u << 13);
+            b.bucket_offset = dev_alloc<uint32_t>(c, (1u << 13) + 1);
             b.slots = dev_alloc<uint32_t>(c, nt);
             b.table = dev_alloc<double>(c, 6 * (size_t)lbvh_table_entries(nt));
             b.node_box = dev_alloc<double>(c, 6 * (size_t)nn);
@@ -607,6 +661,33 @@ int avbd_build_graph(avbd_ctx* c) {
             upload(c, b.right, none);
             upload(c, b.parent, none);
         }
+        // persistent step kernel: per-colour offsets, Chebyshev schedule, barrier
+        {
+            std::vector<uint32_t> co(c->plan.item_offsets.begin(), c->plan.item_offsets.end());
+            std::vector<float> om(c->desc.solver_iters);
+            double w = 1;
+            for (uint32_t i = 0; i < c->desc.solver_iters; i++) om[i] = (float)(w = omega_at(i, c->desc.iter_acceleration, w));
+            uint32_t* d_co = dev_alloc<uint32_t>(c, co.size());
+            uint32_t* d_cfo = dev_alloc<uint32_t>(c, c->plan.conflict_offsets.size());
+            float* d_om = dev_alloc<float>(c, om.size());
+            GridBarrier* bar = dev_alloc<GridBarrier>(c, 1);
+            upload(c, d_co, co);
+            upload(c, d_cfo, c->plan.conflict_offsets);
+            upload(c, d_om, om);
+            hip_check(hipMemsetAsync(bar, 0, sizeof(GridBarrier), c->stream), "hipMemsetAsync");
+            c->sargs.d = c->d;
+            c->sargs.color_offsets = d_co;
+            c->sargs.conflict_offsets = d_cfo;
+            c->sargs.conflicts = c->d_conflicts;
+            c->sargs.omega = d_om;
+            c->sargs.bar = bar;
+            c->sargs.iters = c->desc.solver_iters;
+            c->sargs.num_colors = c->plan.num_colors;
+            c->sargs.dt = (float)c->desc.dt;
+            c->sargs.g = (float)c->desc.g_accel;
+            hip_check(hipEventCreate(&c->ev_st0), "hipEventCreate");
+            hip_check(hipEventCreate(&c->ev_st1), "hipEventCreate");
+        }
         c->built = true;
 
         const uint32_t n_solves = c->desc.solver_iters * c->plan.num_colors;
@@ -616,6 +697,10 @@ int avbd_build_graph(avbd_ctx* c) {
         for (auto& ev : c->events) hip_check(hipEventCreate(&ev), "hipEventCreate");
 
         if (c->desc.lanes_per_vertex == 0 || c->desc.block_size == 0) autotune(c);
+        if (!(c->desc.flags & AVBD_FLAG_NO_PERSISTENT)) {
+            c->pgrid = persistent_grid(c->device, c->cfg.lanes, c->damping);
+            c->persistent = c->pgrid >= 8;
+        }
         if (!(c->desc.flags & AVBD_FLAG_NO_GRAPH)) capture_graph(c);
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     });
@@ -631,7 +716,13 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
             for (uint32_t s = 0; s < n_steps; s++) {
                 enqueue_step(c, true);
                 hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-                for (size_t k = 0; k + 1 < c->events.size(); k += 2) {
+                if (c->persistent) {
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, c->ev_st0, c->ev_st1), "hipEventElapsedTime");
+                    total += ms;
+                    launches++;
+                }
+                for (size_t k = 0; !c->persistent && k + 1 < c->events.size(); k += 2) {
                     float ms = 0;
                     hip_check(hipEventElapsedTime(&ms, c->events[k], c->events[k + 1]), "hipEventElapsedTime");
                     total += ms;
@@ -663,6 +754,7 @@ int avbd_synchronize(avbd_ctx* c) {
         if (c->built) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, c->ev_a, c->ev_b) == hipSuccess) c->last_step_ms = ms;
+            check_abort(c);
         }
     });
 }
@@ -675,6 +767,7 @@ int avbd_download_state(avbd_ctx* c, double* x, double* v) {
         hip_check(hipMemcpyAsync(bx.data(), c->d.x, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipMemcpyAsync(bv.data(), c->d.v, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        check_abort(c);
         for (uint32_t i = 0; i < n; i++) {
             if (x) {
                 x[3 * i] = bx[i].x;
@@ -837,8 +930,14 @@ uint32_t avbd_num_energies(avbd_ctx* c, int type) {
 int avbd_get_launch_config(avbd_ctx* c, uint32_t* lanes, uint32_t* block) {
     return guarded(c, [&] {
         if (lanes) *lanes = c->cfg.lanes;
-        if (block) *block = c->cfg.block;
+        if (block) *block = c->persistent ? 256u : c->cfg.block;
     });
+}
+
+int avbd_get_step_mode(avbd_ctx* c, uint32_t* persistent_grid) {
+    if (!c) return AVBD_E_ARG;
+    if (persistent_grid) *persistent_grid = c->persistent ? c->pgrid : 0u;
+    return AVBD_OK;
 }
 
 int avbd_set_timing(avbd_ctx* c, int enable) {

@@ -91,6 +91,30 @@ struct LaunchCfg {
     uint32_t block = 256;
 };
 
+// Grid barrier of the persistent step kernel (zeroed once at build time).
+struct GridBarrier {
+    uint32_t count;   // blocks arrived at the current barrier
+    uint32_t gen;     // barrier generation
+    uint32_t abort;   // set when a wait timed out; every block then leaves
+    uint32_t pad;
+};
+
+struct StepArgs {
+    DevState d;
+    const uint32_t* color_offsets = nullptr;      // num_colors + 1
+    const uint32_t* conflict_offsets = nullptr;   // num_colors + 1
+    const uint32_t* conflicts = nullptr;
+    const float* omega = nullptr;                 // Chebyshev omega per iteration
+    GridBarrier* bar = nullptr;
+    uint32_t iters = 0, num_colors = 0;
+    float dt = 0.f, g = 0.f;
+};
+
+// Blocks of 256 threads that fit on the device at once for the persistent
+// kernel with `lanes` lanes per vertex.
+uint32_t persistent_grid(int device, uint32_t lanes, bool damping);
+void launch_step_persistent(hipStream_t s, const StepArgs& a, uint32_t lanes, bool damping, uint32_t grid);
+
 void launch_eprev(hipStream_t s, const DevState& d, float dt);
 void launch_step_begin(hipStream_t s, const DevState& d, float dt, float g);
 void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t item_begin,

@@ -21,8 +21,8 @@ struct LbvhDev {
     uint32_t* right = nullptr;
     uint32_t* parent = nullptr;
     uint32_t* range_hi = nullptr;   // last leaf covered by each node
-    uint32_t* bucket_count = nullptr;    // 65,536 (zeroed by the build)
-    uint32_t* bucket_offset = nullptr;   // 65,537
+    uint32_t* bucket_count = nullptr;    // 8,192 (zeroed by the build)
+    uint32_t* bucket_offset = nullptr;   // 8,193
     uint32_t* slots = nullptr;           // n: primitives grouped by bucket
     double* table = nullptr;        // aligned range unions, 6 per entry (lbvh_table_entries)
     double* node_box = nullptr;     // 6 per node
@@ -34,5 +34,10 @@ struct LbvhDev {
 uint32_t lbvh_table_entries(uint32_t n);
 // AABBs + Morton (FP64) -> sort -> Karras -> refit -> self traversal.
 void launch_lbvh_build(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt);
+// The same in two parts: the speculative bounds read the particle state; the
+// rest (codes, sort, hierarchy, boxes, traversal) reads only the bounds, so
+// it may run concurrently with work that moves the particles.
+void launch_lbvh_bounds(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt);
+void launch_lbvh_tree(hipStream_t s, const LbvhDev& b);
 
 }  // namespace avbd

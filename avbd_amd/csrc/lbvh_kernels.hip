@@ -69,8 +69,9 @@ __global__ void tri_aabb_kernel(const uint32_t* __restrict__ tris, uint32_t n, c
     for (int k = 0; k < 6; k++) aabb[6 * t + k] = b[k];
 }
 
-constexpr uint32_t BUCKET_SHIFT = 47;   // codes have 63 bits: the top 16 pick the bucket
-constexpr uint32_t N_BUCKETS = 1u << 16;
+constexpr uint32_t BUCKET_SHIFT = 50;   // codes have 63 bits: the top 13 pick the bucket
+constexpr uint32_t N_BUCKETS = 1u << 13;
+constexpr uint32_t BUCKETS_PER_THREAD = N_BUCKETS / 1024;
 
 // One block: scene box = union of the primitive boxes; also resets the pair
 // counter, the status word and the bucket counts of this build.
@@ -118,15 +119,15 @@ __global__ void morton_kernel(const double* __restrict__ aabb, uint32_t n, const
 
 // Sort by (code, original index) -- the order a stable LSD radix sort of the
 // codes produces -- in short fully parallel passes: morton_kernel counts codes
-// per bucket of their top 16 bits; one block scans the 65,536 counts; every
+// per bucket of their top 13 bits; one block scans the 8,192 counts; every
 // primitive takes a slot in its bucket; every primitive counts the keys of
 // its bucket that precede it and lands at bucket offset + count.
 __global__ __launch_bounds__(1024) void bucket_scan_kernel(uint32_t* __restrict__ count, uint32_t* __restrict__ offset,
                                                            uint32_t n) {
     __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;   // 64 consecutive buckets per thread
+    const uint32_t t = threadIdx.x;   // BUCKETS_PER_THREAD consecutive buckets per thread
     uint32_t local = 0;
-    for (uint32_t k = 0; k < 64; k++) local += count[64 * t + k];
+    for (uint32_t k = 0; k < BUCKETS_PER_THREAD; k++) local += count[BUCKETS_PER_THREAD * t + k];
     part[t] = local;
     __syncthreads();
     for (uint32_t off = 1; off < 1024; off <<= 1) {
@@ -136,8 +137,8 @@ __global__ __launch_bounds__(1024) void bucket_scan_kernel(uint32_t* __restrict_
         __syncthreads();
     }
     uint32_t run = part[t] - local;
-    for (uint32_t k = 0; k < 64; k++) {
-        const uint32_t b = 64 * t + k;
+    for (uint32_t k = 0; k < BUCKETS_PER_THREAD; k++) {
+        const uint32_t b = BUCKETS_PER_THREAD * t + k;
         const uint32_t c = count[b];
         offset[b] = run;
         run += c;
@@ -312,11 +313,13 @@ __device__ __forceinline__ bool overlap(const double* a, const double* b) {
 
 constexpr int TRAVERSE_STACK = 64;
 constexpr int TRAVERSE_BLOCK = 64;
+constexpr int TRAVERSE_LOCAL = 32;   // pairs buffered per thread before the block flush
 
 // One thread per leaf i; descends from the root testing both children of a
 // node at once, keeps the deferred child on a per-thread stack in LDS, and
-// emits (i, j) for overlapping leaves j > i (subtrees whose last leaf is <= i
-// are skipped).
+// records j for overlapping leaves j > i (subtrees whose last leaf is <= i
+// are skipped).  Pairs are buffered in LDS and written with one global atomic
+// reservation per block (a pair counter hit by every pair serialises at L2).
 __global__ __launch_bounds__(TRAVERSE_BLOCK) void traverse_kernel(uint32_t n, uint32_t root,
                                                                   const uint32_t* __restrict__ left,
                                                                   const uint32_t* __restrict__ right,
@@ -325,46 +328,73 @@ __global__ __launch_bounds__(TRAVERSE_BLOCK) void traverse_kernel(uint32_t n, ui
                                                                   uint32_t* __restrict__ pairs, uint32_t cap,
                                                                   uint32_t* __restrict__ counters) {
     __shared__ uint32_t stack_mem[TRAVERSE_STACK * TRAVERSE_BLOCK];
+    __shared__ uint32_t out_j[TRAVERSE_LOCAL * TRAVERSE_BLOCK];
+    __shared__ uint32_t block_base;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || n < 2) return;
-    double bi[6];
-    for (int k = 0; k < 6; k++) bi[k] = node_box[6 * i + k];
-    auto hit = [&](uint32_t c) { return range_hi[c] > i && overlap(bi, node_box + 6 * c); };
+    uint32_t cnt = 0;
     auto emit = [&](uint32_t j) {
-        const uint32_t slot = atomicAdd(&counters[0], 1u);
+        if (cnt < TRAVERSE_LOCAL) {
+            out_j[cnt * TRAVERSE_BLOCK + threadIdx.x] = j;
+        } else {   // rare overflow of the local buffer: reserve directly
+            const uint32_t slot = atomicAdd(&counters[0], 1u);
+            if (slot < cap) {
+                pairs[2 * slot] = i;
+                pairs[2 * slot + 1] = j;
+            }
+        }
+        cnt++;
+    };
+    if (i < n && n > 1) {
+        double bi[6];
+        for (int k = 0; k < 6; k++) bi[k] = node_box[6 * i + k];
+        auto hit = [&](uint32_t c) { return range_hi[c] > i && overlap(bi, node_box + 6 * c); };
+        int sp = 0;
+        uint32_t node = root;
+        bool live = hit(root);
+        while (live) {
+            const uint32_t a = left[node], b = right[node];
+            bool ha = hit(a), hb = hit(b);
+            if (ha && a < n) {
+                emit(a);
+                ha = false;
+            }
+            if (hb && b < n) {
+                emit(b);
+                hb = false;
+            }
+            if (ha && hb) {
+                if (sp >= TRAVERSE_STACK) {
+                    atomicOr(&counters[1], 1u);   // stack overflow: reported, traversal truncated
+                    break;
+                }
+                stack_mem[(sp++) * TRAVERSE_BLOCK + threadIdx.x] = b;
+                node = a;
+            } else if (ha) {
+                node = a;
+            } else if (hb) {
+                node = b;
+            } else if (sp > 0) {
+                node = stack_mem[(--sp) * TRAVERSE_BLOCK + threadIdx.x];
+            } else {
+                live = false;
+            }
+        }
+    }
+    // block flush: exclusive prefix of the buffered counts (one wave per block)
+    const uint32_t mine = cnt < TRAVERSE_LOCAL ? cnt : TRAVERSE_LOCAL;
+    uint32_t incl = mine;
+    for (int off = 1; off < TRAVERSE_BLOCK; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if ((int)threadIdx.x >= off) incl += o;
+    }
+    if (threadIdx.x == TRAVERSE_BLOCK - 1) block_base = incl ? atomicAdd(&counters[0], incl) : 0u;
+    __syncthreads();
+    const uint32_t base = block_base + incl - mine;
+    for (uint32_t k = 0; k < mine; k++) {
+        const uint32_t slot = base + k;
         if (slot < cap) {
             pairs[2 * slot] = i;
-            pairs[2 * slot + 1] = j;
-        }
-    };
-    int sp = 0;
-    uint32_t node = root;
-    if (!hit(root)) return;
-    while (true) {
-        const uint32_t a = left[node], b = right[node];
-        bool ha = hit(a), hb = hit(b);
-        if (ha && a < n) {
-            emit(a);
-            ha = false;
-        }
-        if (hb && b < n) {
-            emit(b);
-            hb = false;
-        }
-        if (ha && hb) {
-            if (sp >= TRAVERSE_STACK) {
-                atomicOr(&counters[1], 1u);   // stack overflow: reported, traversal truncated
-                return;
-            }
-            stack_mem[(sp++) * TRAVERSE_BLOCK + threadIdx.x] = b;
-            node = a;
-        } else if (ha) {
-            node = a;
-        } else if (hb) {
-            node = b;
-        } else {
-            if (sp == 0) return;
-            node = stack_mem[(--sp) * TRAVERSE_BLOCK + threadIdx.x];
+            pairs[2 * slot + 1] = out_j[k * TRAVERSE_BLOCK + threadIdx.x];
         }
     }
 }
@@ -379,12 +409,16 @@ uint32_t lbvh_table_entries(uint32_t n) {
     return total;
 }
 
-void launch_lbvh_build(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt) {
+void launch_lbvh_bounds(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt) {
+    if (b.n == 0) return;
+    hipLaunchKernelGGL(tri_aabb_kernel, dim3(nblk(b.n, 256)), dim3(256), 0, s, b.tris, b.n, x, v, dt, b.aabb);
+}
+
+void launch_lbvh_tree(hipStream_t s, const LbvhDev& b) {
     const uint32_t n = b.n;
     if (n == 0) return;
     uint32_t top = 0;   // highest table level: 2^top >= n
     while ((1u << top) < n) top++;
-    hipLaunchKernelGGL(tri_aabb_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.tris, n, x, v, dt, b.aabb);
     hipLaunchKernelGGL(scene_box_kernel, dim3(1), dim3(1024), 0, s, b.aabb, n, b.sbox, b.counters, b.bucket_count);
     hipLaunchKernelGGL(morton_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.aabb, n, b.sbox, b.keys,
                        b.bucket_count);
@@ -400,6 +434,11 @@ void launch_lbvh_build(hipStream_t s, const LbvhDev& b, const float4* x, const f
                        b.right, b.parent, b.range_hi, b.table, b.node_box);
     hipLaunchKernelGGL(traverse_kernel, dim3(nblk(n, TRAVERSE_BLOCK)), dim3(TRAVERSE_BLOCK), 0, s, n,
                        n > 1 ? n : 0u, b.left, b.right, b.range_hi, b.node_box, b.pairs, b.pair_cap, b.counters);
+}
+
+void launch_lbvh_build(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt) {
+    launch_lbvh_bounds(s, b, x, v, dt);
+    launch_lbvh_tree(s, b);
 }
 
 }  // namespace avbd

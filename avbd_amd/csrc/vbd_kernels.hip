@@ -182,19 +182,14 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
     return x * (G / 8u) + min(x, G % 8u) + q;
 }
 
+// One vertex block solve (_solveParticle<3>, simulation/VBDSolver.hpp:285-449,487-510)
+// by the L lanes of a group; every lane of the group must call it.
 template <int L, bool DAMP>
-__global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restrict__ items, uint32_t n_items,
-                                                           const uint32_t* __restrict__ inc,
-                                                           const TetRec* __restrict__ tets,
-                                                           const TetDamp* __restrict__ damp, float4* x,
-                                                           const float4* __restrict__ y,
-                                                           const GroundRec* __restrict__ ground, float4* x_out,
-                                                           float4* x_buf, float inv_dt, float inv_dt2) {
-    const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
-    const uint32_t item = tid / L;
-    const uint32_t lane = tid % L;
-    if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)
-    const VItem it = items[item];
+__device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const uint32_t* __restrict__ inc,
+                                           const TetRec* __restrict__ tets, const TetDamp* __restrict__ damp,
+                                           float4* x, const float4* __restrict__ y,
+                                           const GroundRec* __restrict__ ground, float4* x_out, float4* x_buf,
+                                           float inv_dt, float inv_dt2) {
     const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
     float3 g = make_float3(0.f, 0.f, 0.f);
     Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -224,10 +219,25 @@ __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restri
     else x_out[it.v] = nx;
 }
 
-__global__ void eprev_kernel(const TetRec* __restrict__ tets, TetDamp* __restrict__ damp,
-                             const float4* __restrict__ x_prev, uint32_t n_tets) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_tets) return;
+template <int L, bool DAMP>
+__global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restrict__ items, uint32_t n_items,
+                                                           const uint32_t* __restrict__ inc,
+                                                           const TetRec* __restrict__ tets,
+                                                           const TetDamp* __restrict__ damp, float4* x,
+                                                           const float4* __restrict__ y,
+                                                           const GroundRec* __restrict__ ground, float4* x_out,
+                                                           float4* x_buf, float inv_dt, float inv_dt2) {
+    const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
+    const uint32_t item = tid / L;
+    const uint32_t lane = tid % L;
+    if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)
+    solve_item<L, DAMP>(items[item], lane, inc, tets, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2);
+}
+
+// E_prev = 0.5 (F_prev^T F_prev - I) from previous positions
+// (NeoHookeanEnergySolver::updateAfterTimeStep, energy/elastic/NeoHookeanEnergySolver.hpp:70-90).
+__device__ __forceinline__ void eprev_one(uint32_t t, const TetRec* __restrict__ tets, TetDamp* __restrict__ damp,
+                                          const float4* __restrict__ x_prev) {
     uint4 idx;
     float4 r1, r2, r3;
     load_tet(tets, t, idx, r1, r2, r3);
@@ -246,13 +256,20 @@ __global__ void eprev_kernel(const TetRec* __restrict__ tets, TetDamp* __restric
     damp[t] = dr;
 }
 
-__global__ void step_begin_kernel(float4* __restrict__ x, float4* __restrict__ y, float4* __restrict__ x_last,
-                                  float4* __restrict__ x_lastlast, const float4* __restrict__ v,
-                                  const float4* __restrict__ v_prev, const uint8_t* __restrict__ fixed,
-                                  const int32_t* __restrict__ ground_of, GroundRec* __restrict__ ground,
-                                  uint32_t n, float dt, float g) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+__global__ void eprev_kernel(const TetRec* __restrict__ tets, TetDamp* __restrict__ damp,
+                             const float4* __restrict__ x_prev, uint32_t n_tets) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_tets) eprev_one(t, tets, damp, x_prev);
+}
+
+// Start of step for one particle: ground contact point + AL warm start, then
+// the inertial target and adaptive initial guess.
+__device__ __forceinline__ void step_begin_one(uint32_t p, float4* __restrict__ x, float4* __restrict__ y,
+                                               float4* __restrict__ x_last, float4* __restrict__ x_lastlast,
+                                               const float4* __restrict__ v, const float4* __restrict__ v_prev,
+                                               const uint8_t* __restrict__ fixed,
+                                               const int32_t* __restrict__ ground_of,
+                                               GroundRec* __restrict__ ground, float dt, float g) {
     float4 xp = x[p];
     const int32_t gi = ground_of[p];
     if (gi >= 0) {
@@ -292,6 +309,15 @@ __global__ void step_begin_kernel(float4* __restrict__ x, float4* __restrict__ y
     x_lastlast[p] = xp;
 }
 
+__global__ void step_begin_kernel(float4* __restrict__ x, float4* __restrict__ y, float4* __restrict__ x_last,
+                                  float4* __restrict__ x_lastlast, const float4* __restrict__ v,
+                                  const float4* __restrict__ v_prev, const uint8_t* __restrict__ fixed,
+                                  const int32_t* __restrict__ ground_of, GroundRec* __restrict__ ground,
+                                  uint32_t n, float dt, float g) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) step_begin_one(p, x, y, x_last, x_lastlast, v, v_prev, fixed, ground_of, ground, dt, g);
+}
+
 __global__ void commit_kernel(float4* __restrict__ x, const float4* __restrict__ x_buf,
                               const uint32_t* __restrict__ list, uint32_t count) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -300,11 +326,10 @@ __global__ void commit_kernel(float4* __restrict__ x, const float4* __restrict__
     x[p] = x_buf[p];
 }
 
-__global__ void cheb_dual_kernel(float4* __restrict__ x, float4* __restrict__ x_last, float4* __restrict__ x_lastlast,
-                                 const int32_t* __restrict__ ground_of, GroundRec* __restrict__ ground, uint32_t n,
-                                 float omega) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+// Chebyshev step and ground dual / stiffness update for one particle.
+__device__ __forceinline__ void cheb_dual_one(uint32_t p, float4* __restrict__ x, float4* __restrict__ x_last,
+                                              float4* __restrict__ x_lastlast, const int32_t* __restrict__ ground_of,
+                                              GroundRec* __restrict__ ground, float omega) {
     float4 xp = x[p];
     const float4 xl = x_last[p], xll = x_lastlast[p];
     // _particleChebyshevAcceleration (VBDSolver.hpp:513-536)
@@ -345,14 +370,112 @@ __global__ void cheb_dual_kernel(float4* __restrict__ x, float4* __restrict__ x_
     ground[gi] = c;
 }
 
-__global__ void velocity_kernel(const float4* __restrict__ x, float4* __restrict__ x_prev, float4* __restrict__ v,
-                                float4* __restrict__ v_prev, uint32_t n, float inv_dt) {
+__global__ void cheb_dual_kernel(float4* __restrict__ x, float4* __restrict__ x_last, float4* __restrict__ x_lastlast,
+                                 const int32_t* __restrict__ ground_of, GroundRec* __restrict__ ground, uint32_t n,
+                                 float omega) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p < n) cheb_dual_one(p, x, x_last, x_lastlast, ground_of, ground, omega);
+}
+
+// _particleVelocityUpdate (VBDSolver.hpp:538-566) for one particle.
+__device__ __forceinline__ void velocity_one(uint32_t p, const float4* __restrict__ x, float4* __restrict__ x_prev,
+                                             float4* __restrict__ v, float4* __restrict__ v_prev, float inv_dt) {
     const float4 xp = x[p], xo = x_prev[p], vv = v[p];
     v_prev[p] = vv;
     v[p] = make_float4((xp.x - xo.x) * inv_dt, (xp.y - xo.y) * inv_dt, (xp.z - xo.z) * inv_dt, 0.f);
     x_prev[p] = xp;
+}
+
+__global__ void velocity_kernel(const float4* __restrict__ x, float4* __restrict__ x_prev, float4* __restrict__ v,
+                                float4* __restrict__ v_prev, uint32_t n, float inv_dt) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) velocity_one(p, x, x_prev, v, v_prev, inv_dt);
+}
+
+// ---- persistent step kernel ------------------------------------------------
+// One launch runs a whole time step; every barrier of solve_Parallel becomes a
+// grid barrier.  The grid is sized so that all blocks are resident at once
+// (occupancy query at build time); a barrier wait is bounded: after
+// kBarrierSpinLimit polls the waiting block raises the abort word, every
+// block leaves at its next barrier, and the host reports the failure -- no
+// wave can wait forever.
+constexpr uint32_t kBarrierSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint32_t aload_u32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns false (uniformly over the block) when the step was aborted.
+__device__ __forceinline__ bool grid_sync(GridBarrier* bar, uint32_t nblocks, uint32_t& gen, uint32_t* ok) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t target = gen + 1;
+        __threadfence();   // release this block's writes
+        if (atomicAdd(&bar->count, 1u) == nblocks - 1) {
+            atomicExch(&bar->count, 0u);
+            __threadfence();
+            atomicExch(&bar->gen, target);
+        } else {
+            uint32_t spins = 0;
+            while (aload_u32(&bar->gen) != target) {
+                if (aload_u32(&bar->abort)) break;
+                if (++spins > kBarrierSpinLimit) {
+                    atomicExch(&bar->abort, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        *ok = aload_u32(&bar->abort) == 0;
+    }
+    gen = gen + 1;
+    __syncthreads();
+    __threadfence();   // acquire: other blocks' writes become visible (L1 invalidated)
+    return *ok != 0;
+}
+
+template <int L, bool DAMP>
+__global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
+    const DevState& d = a.d;
+    const uint32_t G = gridDim.x, B = blockDim.x;
+    const uint32_t lb = xcd_logical_block();
+    const uint32_t gtid = lb * B + threadIdx.x, gthreads = G * B;
+    __shared__ uint32_t gen_s, ok_s;
+    if (threadIdx.x == 0) gen_s = aload_u32(&a.bar->gen);
+    __syncthreads();
+    uint32_t gen = gen_s;
+    const float inv_dt = 1.0f / a.dt, inv_dt2 = inv_dt * inv_dt;
+
+    if constexpr (DAMP) {
+        for (uint32_t t = gtid; t < d.n_tets; t += gthreads) eprev_one(t, d.tets, d.damp, d.x_prev);
+    }
+    for (uint32_t p = gtid; p < d.n; p += gthreads)
+        step_begin_one(p, d.x, d.y, d.x_last, d.x_lastlast, d.v, d.v_prev, d.fixed, d.ground_of, d.ground, a.dt, a.g);
+    if (!grid_sync(a.bar, G, gen, &ok_s)) return;
+
+    const uint32_t groups = B / L, lane = threadIdx.x % L, grp = threadIdx.x / L;
+    for (uint32_t it = 0; it < a.iters; it++) {
+        for (uint32_t c = 0; c < a.num_colors; c++) {
+            const uint32_t beg = a.color_offsets[c], cnt = a.color_offsets[c + 1] - beg;
+            // contiguous share per logical block (XCD-local slabs)
+            const uint32_t per = (cnt + G - 1) / G;
+            const uint32_t lo = min(cnt, lb * per), hi = min(cnt, lo + per);
+            for (uint32_t k = lo + grp; k < hi; k += groups)
+                solve_item<L, DAMP>(d.items[beg + k], lane, d.inc, d.tets, d.damp, d.x, d.y, d.ground, d.x, d.x_buf,
+                                    inv_dt, inv_dt2);
+            if (!grid_sync(a.bar, G, gen, &ok_s)) return;
+            const uint32_t cb = a.conflict_offsets[c], ce = a.conflict_offsets[c + 1];
+            if (ce > cb) {
+                for (uint32_t k = cb + gtid; k < ce; k += gthreads) d.x[a.conflicts[k]] = d.x_buf[a.conflicts[k]];
+                if (!grid_sync(a.bar, G, gen, &ok_s)) return;
+            }
+        }
+        const float omega = a.omega[it];
+        for (uint32_t p = gtid; p < d.n; p += gthreads)
+            cheb_dual_one(p, d.x, d.x_last, d.x_lastlast, d.ground_of, d.ground, omega);
+        if (!grid_sync(a.bar, G, gen, &ok_s)) return;
+    }
+    for (uint32_t p = gtid; p < d.n; p += gthreads) velocity_one(p, d.x, d.x_prev, d.v, d.v_prev, inv_dt);
 }
 
 // ---- residual / incremental potential (new; see avbd_get_stats) ----------
@@ -576,6 +699,52 @@ __global__ void calib_copy_kernel(const float4* __restrict__ src, float4* __rest
 
 void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n_float4) {
     hipLaunchKernelGGL(calib_copy_kernel, dim3(2048), dim3(256), 0, s, src, dst, n_float4);
+}
+
+}  // namespace avbd
+
+namespace avbd {
+
+namespace {
+template <int L, bool DAMP>
+const void* persistent_fn() {
+    return reinterpret_cast<const void*>(&step_persistent_kernel<L, DAMP>);
+}
+const void* persistent_fn(uint32_t lanes, bool damping) {
+    switch (lanes) {
+        case 4: return damping ? persistent_fn<4, true>() : persistent_fn<4, false>();
+        case 8: return damping ? persistent_fn<8, true>() : persistent_fn<8, false>();
+        case 16: return damping ? persistent_fn<16, true>() : persistent_fn<16, false>();
+        default: return damping ? persistent_fn<32, true>() : persistent_fn<32, false>();
+    }
+}
+}  // namespace
+
+uint32_t persistent_grid(int device, uint32_t lanes, bool damping) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_fn(lanes, damping), 256, 0) != hipSuccess)
+        return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    // two resident blocks per CU (of the occupancy limit) keep every block on
+    // the chip together with a concurrent broad-phase stream
+    const int use = per_cu < 2 ? per_cu : 2;
+    return (uint32_t)(use * cus);
+}
+
+void launch_step_persistent(hipStream_t s, const StepArgs& a, uint32_t lanes, bool damping, uint32_t grid) {
+#define AVBD_PERSIST_CASE(LL)                                                                                  \
+    case LL:                                                                                                    \
+        if (damping) hipLaunchKernelGGL((step_persistent_kernel<LL, true>), dim3(grid), dim3(256), 0, s, a);    \
+        else hipLaunchKernelGGL((step_persistent_kernel<LL, false>), dim3(grid), dim3(256), 0, s, a);           \
+        break;
+    switch (lanes) {
+        AVBD_PERSIST_CASE(4)
+        AVBD_PERSIST_CASE(8)
+        AVBD_PERSIST_CASE(16)
+        AVBD_PERSIST_CASE(32)
+        default: break;
+    }
+#undef AVBD_PERSIST_CASE
 }
 
 }  // namespace avbd

@@ -44,7 +44,8 @@ class SimulationContext:
 
     def __init__(self, dt: float, g_accel: float = 9.81, solver_iters: int = 20,
                  iter_acceleration: float = 0.95, particle_capacity: int = 0, energy_capacity: int = 0,
-                 device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0, use_graph: bool = True):
+                 device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0, use_graph: bool = True,
+                 serial_broadphase: bool = False, persistent: bool = True):
         desc = _lib.SceneDesc()
         desc.particle_capacity = particle_capacity
         desc.energy_capacity = energy_capacity
@@ -52,7 +53,8 @@ class SimulationContext:
         desc.g_accel = g_accel
         desc.solver_iters = solver_iters
         desc.iter_acceleration = iter_acceleration
-        desc.flags = 0 if use_graph else _lib.FLAG_NO_GRAPH
+        desc.flags = ((0 if use_graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_SERIAL_BROADPHASE if serial_broadphase else 0)
+                      | (0 if persistent else _lib.FLAG_NO_PERSISTENT))
         desc.device = device
         desc.lanes_per_vertex = lanes_per_vertex
         desc.block_size = block_size
@@ -177,6 +179,11 @@ class SimulationContext:
         self._chk(lib.avbd_get_stats(self.handle, C.byref(st)))
         return st
 
+    def persistent_grid(self) -> int:
+        g = C.c_uint32()
+        self._chk(lib.avbd_get_step_mode(self.handle, C.byref(g)))
+        return g.value
+
     def launch_config(self):
         L, B = C.c_uint32(), C.c_uint32()
         self._chk(lib.avbd_get_launch_config(self.handle, C.byref(L), C.byref(B)))
@@ -222,12 +229,13 @@ class Simulation:
     a Scene description, runs setup() (adjacency + colouring) and time steps."""
 
     def __init__(self, scene: Scene, device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0,
-                 use_graph: bool = True):
+                 use_graph: bool = True, serial_broadphase: bool = False, persistent: bool = True):
         self.scene = scene
         self.ctx = SimulationContext(dt=scene.dt, g_accel=scene.g, solver_iters=scene.iters,
                                      iter_acceleration=scene.rho, device=device,
                                      lanes_per_vertex=lanes_per_vertex, block_size=block_size,
-                                     use_graph=use_graph)
+                                     use_graph=use_graph, serial_broadphase=serial_broadphase,
+                                     persistent=persistent)
         self.solver = VBDSolver(self.ctx, scene.iters, scene.rho)
         self.time = 0.0
         self.offsets: List[int] = []

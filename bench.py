@@ -58,6 +58,13 @@ def solve_bytes_per_launch(sim) -> float:
     return total / k
 
 
+def step_bytes(sc) -> int:
+    """Algorithmic bytes of one whole step (FP32 gather model)."""
+    from avbd_amd.scenes import algorithmic_bytes_per_step
+    n_v = sum(int((np.ones(m.rest.shape[0]) if m.fixed is None else (m.fixed == 0)).sum()) for m in sc.meshes)
+    return algorithmic_bytes_per_step(n_v, sc.n_tets, sc.n_ground, sc.iters)
+
+
 def cpu_baseline(sc, budget_s: float = 12.0):
     """FP64 oracle (restatement of the reference's SPMD CPU path) on the host
     cores: setup excluded, 1 warm-up step, then whole steps until ~budget_s."""
@@ -131,10 +138,16 @@ def main():
     avg_ms, launches = sim.ctx.solve_timing()
     lbvh_ms = sim.ctx.broadphase_timing()
     sim.ctx.set_timing(False)
-    bytes_launch = solve_bytes_per_launch(sim)
+    pgrid = sim.ctx.persistent_grid()
+    if pgrid:
+        kernel = "step_persistent_kernel"        # one launch = one whole solve
+        bytes_launch = step_bytes(sc)
+    else:
+        kernel = "solve_color_kernel"            # one launch = one colour phase
+        bytes_launch = solve_bytes_per_launch(sim)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     roof = {
-        "kernel": "solve_color_kernel",
+        "kernel": kernel,
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
@@ -145,7 +158,7 @@ def main():
         "algorithmic_bytes_per_launch": int(bytes_launch),
         "launches_timed": launches,
     }
-    prof = ROOT / "profiles" / f"solve_color_{sc.name}_traffic.json"
+    prof = ROOT / "profiles" / f"{kernel}_{sc.name}_traffic.json"
     if prof.exists():
         try:
             roof["traffic"] = json.loads(prof.read_text()).get("bytes_per_launch")
@@ -189,11 +202,11 @@ def main():
         },
         "vertex_updates_per_s": round(vups, 1),
         "residual_rel_final": st.residual_rel,
-        "step_breakdown_us": {"broad_phase": round(lbvh_ms * 1e3, 2),
-                              "vertex_solves": round(avg_ms * 1e3 * launches / 2, 2)},
+        "step_breakdown_us": {"broad_phase_concurrent": round(lbvh_ms * 1e3, 2),
+                              "solve": round(avg_ms * 1e3 * launches / 2, 2)},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "launch_config": {"lanes_per_vertex": lanes, "block_size": block},
+        "launch_config": {"lanes_per_vertex": lanes, "block_size": block, "persistent_grid": pgrid},
     }
     if calib:
         out["copy_calibration"] = calib

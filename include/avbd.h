@@ -57,6 +57,10 @@ typedef enum {
 
 /* ---- context creation flags ---- */
 #define AVBD_FLAG_NO_GRAPH 1u    /* launch kernels eagerly instead of replaying a hipGraph */
+#define AVBD_FLAG_NO_PERSISTENT 4u  /* one kernel launch per solver phase instead of one
+                                       persistent launch per step with grid barriers */
+#define AVBD_FLAG_SERIAL_BROADPHASE 2u  /* run the triangle broad phase on the solve stream
+                                           instead of concurrently on a second stream */
 
 /* SimulationContext capacities + SimulationParams + VBDSolver arguments.
  * Reference: simulation/SimulationContext.hpp:46-59, simulation/SimulationParams.hpp:8-15,
@@ -175,6 +179,9 @@ uint32_t avbd_num_particles(avbd_ctx* ctx);
 uint32_t avbd_num_energies(avbd_ctx* ctx, int type);
 /* Kernel launch geometry actually in use (after autotuning). */
 int avbd_get_launch_config(avbd_ctx* ctx, uint32_t* lanes_per_vertex, uint32_t* block_size);
+/* *persistent_grid = blocks of the persistent step kernel, or 0 when the
+ * step runs as one launch per phase. */
+int avbd_get_step_mode(avbd_ctx* ctx, uint32_t* persistent_grid);
 /* Kernel timing: while enabled, avbd_step() launches eagerly and brackets
  * every per-colour vertex-solve launch with HIP events on the ctx stream.
  * avbd_get_solve_timing() returns the average duration (ms) of those
