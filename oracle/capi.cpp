@@ -1,6 +1,6 @@
 // capi.cpp -- C entry points of the FP64 oracle for ctypes (test
 // infrastructure only: tests/, smoke() and bench.py's cpu_baseline leg).
-#include "oracle.hpp"
+#include "simulation.hpp"
 
 #include <cstring>
 #include <exception>
@@ -236,5 +236,197 @@ double oracle_nh_single(const double* x12, const double* Q9, const double* Eprev
 }
 
 void oracle_lu_solve(int n, const double* A, const double* b, double* x) { lu_solve(n, A, b, x); }
+
+// ---------------------------------------------------------------------------
+// Oriented (6-DOF) particles.  Quaternions are passed as (w, x, y, z).
+// ---------------------------------------------------------------------------
+static inline Quat q_in(const double* q) { return Quat(q[0], q[1], q[2], q[3]); }
+static inline V3 v_in(const double* v) { return V3(v[0], v[1], v[2]); }
+static inline void q_out(const Quat& q, double* o) { o[0] = q.w; o[1] = q.x; o[2] = q.y; o[3] = q.z; }
+
+int64_t oracle_add_rod(void* h, uint32_t elements, double length, const double* curvature3, double E, double nu,
+                       double density, double diameter, const double* pos3, const double* rot4, const double* vel3,
+                       int base_fixed, int tip_fixed, int with_ground) {
+    int64_t first = -1;
+    int rc = guard([&] {
+        first = static_cast<Simulation*>(h)->add_rod(elements, length, v_in(curvature3), E, nu, density, diameter,
+                                                      v_in(pos3), q_in(rot4), v_in(vel3), base_fixed != 0,
+                                                      tip_fixed != 0, with_ground != 0);
+    });
+    return rc == 0 ? first : -1;
+}
+
+int64_t oracle_add_sphere(void* h, double radius, double density, const double* pos3, const double* rot4, int fixed,
+                          int with_ground) {
+    return static_cast<Simulation*>(h)->add_sphere(radius, density, v_in(pos3), q_in(rot4), fixed != 0,
+                                                    with_ground != 0);
+}
+
+int64_t oracle_add_oriented(void* h, const double* pos3, const double* rot4, double mass, const double* inertia3) {
+    return static_cast<Simulation*>(h)->add_oriented(v_in(pos3), q_in(rot4), mass, v_in(inertia3));
+}
+
+// Cosserat element between two existing oriented particles.
+void oracle_add_cosserat(void* h, uint32_t a, uint32_t b, const double* stiffness6, double L, const double* kappa3) {
+    CosseratRod r{};
+    r.idx[0] = a; r.idx[1] = b;
+    for (int k = 0; k < 6; k++) r.stiffness[k] = stiffness6[k];
+    r.L = L; r.kappa0 = v_in(kappa3);
+    static_cast<Simulation*>(h)->cos.push_back(r);
+}
+
+int64_t oracle_add_joint(void* h, uint32_t p, const double* pos_offset3, const double* rot_offset4,
+                         const double* ref_pos3, const double* ref_rot4) {
+    return static_cast<Simulation*>(h)->add_joint(p, v_in(pos_offset3), q_in(rot_offset4), v_in(ref_pos3),
+                                                   q_in(ref_rot4));
+}
+
+uint32_t oracle_num_cosserat(void* h) { return (uint32_t)static_cast<Simulation*>(h)->cos.size(); }
+uint32_t oracle_num_rground(void* h) { return (uint32_t)static_cast<Simulation*>(h)->rground.size(); }
+uint32_t oracle_num_joints(void* h) { return (uint32_t)static_cast<Simulation*>(h)->joints.size(); }
+
+// Per particle: orientation (4, identity for positional particles), angular
+// velocity (3), inertia (3), oriented flag.
+void oracle_get_rotations(void* h, double* q4, double* w3, double* inertia3, uint8_t* oriented) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t p = 0; p < s->x.size(); p++) {
+        const bool o = s->oriented((uint32_t)p);
+        if (oriented) oriented[p] = o ? 1 : 0;
+        const int32_t r = o ? s->rs.rot_of[p] : -1;
+        if (q4) q_out(o ? s->rs.q[r] : Quat(), q4 + 4 * p);
+        for (int k = 0; k < 3; k++) {
+            if (w3) w3[3 * p + k] = o ? s->rs.w[r][k] : 0.0;
+            if (inertia3) inertia3[3 * p + k] = o ? s->rs.inertia[r][k] : 0.0;
+        }
+    }
+}
+
+// Sets orientation and angular velocity (and their previous-step copies) of
+// every oriented particle; rows of positional particles are ignored.
+void oracle_set_rotation_state(void* h, const double* q4, const double* w3) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t p = 0; p < s->x.size(); p++) {
+        if (!s->oriented((uint32_t)p)) continue;
+        const int32_t r = s->rs.rot_of[p];
+        if (q4) {
+            const Quat q = q_in(q4 + 4 * p);
+            s->rs.q[r] = s->rs.q_prev[r] = s->rs.q_last[r] = s->rs.q_lastlast[r] = s->rs.q_buf[r] = q;
+        }
+        if (w3) s->rs.w[r] = s->rs.w_prev[r] = v_in(w3 + 3 * p);
+    }
+}
+
+void oracle_get_mass(void* h, double* m, uint8_t* fixed) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t p = 0; p < s->x.size(); p++) {
+        if (m) m[p] = s->mass[p];
+        if (fixed) fixed[p] = s->fixed[p];
+    }
+}
+
+// Rigid-ground records: k_start,k_n,k_t,k_b,lambda_n,lambda_t,lambda_b,C_n_prev,
+// cp_rb_local(3),cp_ground(3),use_static,unclamped (16 per record)
+void oracle_get_rground(void* h, double* rec16) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t e = 0; e < s->rground.size(); e++) {
+        const auto& c = s->rground[e];
+        double* r = rec16 + 16 * e;
+        r[0] = c.k_start; r[1] = c.k_n; r[2] = c.k_t; r[3] = c.k_b;
+        r[4] = c.lambda_n; r[5] = c.lambda_t; r[6] = c.lambda_b; r[7] = c.C_n_prev;
+        for (int k = 0; k < 3; k++) { r[8 + k] = c.cp_rb_local[k]; r[11 + k] = c.cp_ground[k]; }
+        r[14] = c.use_static; r[15] = c.unclamped_lambda_tb;
+    }
+}
+
+// Joint records: k(6), lambda(6), C_prev(6) (18 per record)
+void oracle_get_joints(void* h, double* rec18) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t e = 0; e < s->joints.size(); e++) {
+        const auto& j = s->joints[e];
+        for (int k = 0; k < 6; k++) {
+            rec18[18 * e + k] = j.k[k];
+            rec18[18 * e + 6 + k] = j.lambda[k];
+            rec18[18 * e + 12 + k] = j.C_prev[k];
+        }
+    }
+}
+
+// Single-energy evaluations for unit tests (G: 6, H: 36 row-major)
+static void out6(const V6& G, const M6& H, double* G6, double* H36) {
+    for (int i = 0; i < 6; i++) {
+        G6[i] = G.v[i];
+        for (int j = 0; j < 6; j++) H36[6 * i + j] = H.m[i][j];
+    }
+}
+
+double oracle_cosserat_single(const double* x6, const double* q8, const double* stiffness6, double L,
+                              const double* kappa3, uint32_t local, double* G6, double* H36) {
+    std::vector<V3> x = {v_in(x6), v_in(x6 + 3)};
+    RotState rs;
+    rs.rot_of = {0, 1};
+    rs.q = {q_in(q8), q_in(q8 + 4)};
+    CosseratRod r{};
+    r.idx[0] = 0; r.idx[1] = 1;
+    for (int k = 0; k < 6; k++) r.stiffness[k] = stiffness6[k];
+    r.L = L; r.kappa0 = v_in(kappa3);
+    V6 G; M6 H;
+    cosserat_accumulate(r, x, rs, local, G, H);
+    out6(G, H, G6, H36);
+    return cosserat_energy(r, x, rs);
+}
+
+// Joint at particle 0; writes the raw constraint vector C (6) as well.
+void oracle_joint_single(const double* x3, const double* q4, const double* pos_offset3, const double* rot_offset4,
+                         const double* ref_pos3, const double* ref_rot4, const double* k6, const double* lambda6,
+                         const double* C_prev6, double* C6, double* G6, double* H36) {
+    std::vector<V3> x = {v_in(x3)};
+    RotState rs;
+    rs.rot_of = {0};
+    rs.q = {q_in(q4)};
+    Joint j{};
+    j.p = 0; j.pos_offset = v_in(pos_offset3); j.rot_offset = q_in(rot_offset4);
+    j.ref_position = v_in(ref_pos3); j.ref_rotation = q_in(ref_rot4);
+    for (int k = 0; k < 6; k++) { j.k[k] = k6[k]; j.lambda[k] = lambda6[k]; j.C_prev[k] = C_prev6[k]; }
+    joint_eval(j, x, rs, C6);
+    V6 G; M6 H;
+    joint_accumulate(j, x, rs, G, H);
+    out6(G, H, G6, H36);
+}
+
+// Rigid ground at particle 0.  rec: k_n,k_t,k_b,lambda_n,lambda_t,lambda_b,C_n_prev,
+// mu_s,mu_k,use_static,radius,shape; cp_local3 / cp_ground3.  Writes C (3).
+void oracle_rground_single(const double* x3, const double* q4, const double* rec12, const double* cp_local3,
+                           const double* cp_ground3, double* C3, double* G6, double* H36) {
+    std::vector<V3> x = {v_in(x3)};
+    RotState rs;
+    rs.rot_of = {0};
+    rs.q = {q_in(q4)};
+    RigidGround c{};
+    c.p = 0;
+    c.k_n = rec12[0]; c.k_t = rec12[1]; c.k_b = rec12[2];
+    c.lambda_n = rec12[3]; c.lambda_t = rec12[4]; c.lambda_b = rec12[5]; c.C_n_prev = rec12[6];
+    c.mu_s = rec12[7]; c.mu_k = rec12[8]; c.use_static = rec12[9] != 0; c.radius = rec12[10];
+    c.shape = (uint32_t)rec12[11];
+    c.cp_rb_local = v_in(cp_local3); c.cp_ground = v_in(cp_ground3);
+    rground_eval(c, x, rs, C3[0], C3[1], C3[2]);
+    V6 G; M6 H;
+    rground_accumulate(c, x, rs, G, H);
+    out6(G, H, G6, H36);
+}
+
+// S3 / SO3 maps
+void oracle_exp_s3(const double* v3, double* q4) { q_out(exp_s3(v_in(v3)), q4); }
+void oracle_log_s3(const double* q4, double* v3) {
+    const V3 v = log_s3(q_in(q4));
+    v3[0] = v.x; v3[1] = v.y; v3[2] = v.z;
+}
+static void m3_out(const M3& m, double* o) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) o[3 * i + j] = m.m[i][j];
+}
+void oracle_qmat(const double* q4, double* M9) { m3_out(qmat(q_in(q4)), M9); }
+void oracle_exp_so3(const double* v3, double* M9) { m3_out(exp_so3(v_in(v3)), M9); }
+void oracle_jr(const double* v3, double* M9) { m3_out(jr(v_in(v3)), M9); }
+void oracle_jr_inv(const double* v3, double* M9) { m3_out(jr_inv(v_in(v3)), M9); }
 
 }  // extern "C"

@@ -1,6 +1,7 @@
 // graph.cpp -- FP64 oracle: static adjacency CSR and DSATUR colouring
 // (test infrastructure only; see oracle.hpp).
 #include "oracle.hpp"
+#include "rigid.hpp"
 
 #include <algorithm>
 #include <stdexcept>
@@ -12,8 +13,9 @@ namespace oracle {
 // local index k).  The vertex-vertex neighbour set is deduplicated; the
 // reference stores it in unordered_set iteration order, which no consumer
 // depends on (DSATUR only uses it as a set), so it is kept ascending here.
-void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh,
-                            const std::vector<GroundContact>& ground, StaticAdjacency& adj) {
+void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh, const std::vector<CosseratRod>& cos,
+                            const std::vector<GroundContact>& ground, const std::vector<RigidGround>& rground,
+                            const std::vector<Joint>& joints, StaticAdjacency& adj) {
     std::vector<std::array<uint32_t, ST_COUNT>> val(n);
     for (auto& a : val) a.fill(0);
     std::vector<std::vector<uint32_t>> nb(n);
@@ -23,7 +25,14 @@ void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh,
             for (int k2 = 0; k2 < 4; k2++)
                 if (k2 != k) nb[t.idx[k]].push_back(t.idx[k2]);
         }
+    for (const auto& r : cos)
+        for (int k = 0; k < 2; k++) {
+            val[r.idx[k]][ST_COSSERAT]++;
+            nb[r.idx[k]].push_back(r.idx[1 - k]);
+        }
     for (const auto& g : ground) val[g.p][ST_GROUND]++;
+    for (const auto& g : rground) val[g.p][ST_RIGID_GROUND]++;
+    for (const auto& j : joints) val[j.p][ST_JOINT]++;
 
     adj.e_offsets.assign(n + 1, {});
     adj.e_offsets[0][0] = 0;
@@ -49,9 +58,22 @@ void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh,
             uint32_t p = nh[e].idx[k];
             adj.e_entries[cursor[p][ST_NEO_HOOKEAN]++] = AdjEntry{ST_NEO_HOOKEAN, e, k};
         }
+    for (uint32_t e = 0; e < cos.size(); e++)
+        for (uint16_t k = 0; k < 2; k++) {
+            uint32_t p = cos[e].idx[k];
+            adj.e_entries[cursor[p][ST_COSSERAT]++] = AdjEntry{ST_COSSERAT, e, k};
+        }
     for (uint32_t e = 0; e < ground.size(); e++) {
         uint32_t p = ground[e].p;
         adj.e_entries[cursor[p][ST_GROUND]++] = AdjEntry{ST_GROUND, e, 0};
+    }
+    for (uint32_t e = 0; e < rground.size(); e++) {
+        uint32_t p = rground[e].p;
+        adj.e_entries[cursor[p][ST_RIGID_GROUND]++] = AdjEntry{ST_RIGID_GROUND, e, 0};
+    }
+    for (uint32_t e = 0; e < joints.size(); e++) {
+        uint32_t p = joints[e].p;
+        adj.e_entries[cursor[p][ST_JOINT]++] = AdjEntry{ST_JOINT, e, 0};
     }
     adj.p_neighbors.resize(adj.p_offsets[n]);
     for (uint32_t i = 0; i < n; i++)

@@ -147,54 +147,6 @@ struct BroadPhase {
     uint64_t traverse_leaves(uint32_t lo, uint32_t hi_leaf) const;   // candidate count
 };
 
-class Simulation {
-public:
-    Params params;
-    // ParticlePool, common/ParticlePool.hpp:78-90 (positional part)
-    std::vector<V3> x, x_buf, y, x_prev, x_last, x_lastlast, v, v_prev;
-    std::vector<Real> mass;
-    std::vector<uint8_t> fixed;
-
-    std::vector<NHTet> nh;
-    std::vector<GroundContact> ground;
-    std::vector<uint32_t> tris;     // collision triangles (3 particle indices each)
-    BroadPhase bp;
-    uint64_t last_candidates = 0;
-
-    StaticAdjacency adj;
-    Coloring coloring;
-    std::vector<uint8_t> is_conflicted;
-
-    // Setup (simobject/TetMeshObject.cpp:46-96 and simulation/Simulation.hpp:117-126).
-    // Returns the index of the first new vertex.
-    uint32_t add_tet_mesh(uint32_t nv, const Real* rest_x, uint32_t nt, const uint32_t* tets,
-                          Real E, Real nu, Real density, Real kd, bool with_ground,
-                          Real mu_s, Real mu_k);
-    // ParticlePool::addParticle (common/ParticlePool.hpp:213-238) for n free particles.
-    uint32_t add_particles(uint32_t n, const Real* x, const Real* m);
-    void build_graph();     // adjacency + DSATUR + work list (simulation/Simulation.cpp:87-89)
-
-    // One time step of VBDSolver::solve_Parallel, single threaded.
-    void step();
-    // Same step on n_threads SPMD workers with a spin barrier between phases.
-    void step_mt(uint32_t n_threads, uint32_t n_steps);
-
-    // Force residual and incremental potential at the current iterate (new
-    // definitions; the reference has none).
-    Stats stats() const;
-
-    // Per-vertex gradient/Hessian accumulation exactly as _solveParticle<3>.
-    void accumulate_vertex(uint32_t p, V3& G, M3& H) const;
-
-private:
-    void phase_after_timestep(uint32_t lo_nh, uint32_t hi_nh, uint32_t lo_g, uint32_t hi_g);
-    void phase_inertial(uint32_t lo, uint32_t hi);
-    void phase_color(uint32_t c, uint32_t lo, uint32_t hi);
-    void phase_chebyshev(Real omega, uint32_t lo, uint32_t hi);
-    void phase_dual(uint32_t lo, uint32_t hi);
-    void phase_velocity(uint32_t lo, uint32_t hi);
-    void solve_particle(uint32_t p);
-};
 
 // Energy kernels exposed for unit tests.
 Real nh_energy(const NHTet& t, const std::vector<V3>& x, Real dt);
@@ -210,8 +162,12 @@ void ground_update_after_timestep(GroundContact& c, const std::vector<V3>& x);
 void ground_update_after_iteration(GroundContact& c, const std::vector<V3>& x);
 
 // Graph pieces exposed for unit tests.
-void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh,
-                            const std::vector<GroundContact>& ground, StaticAdjacency& adj);
+struct CosseratRod;
+struct RigidGround;
+struct Joint;
+void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh, const std::vector<CosseratRod>& cos,
+                            const std::vector<GroundContact>& ground, const std::vector<RigidGround>& rground,
+                            const std::vector<Joint>& joints, StaticAdjacency& adj);
 void dsatur(const StaticAdjacency& adj, uint32_t n, Coloring& col);
 void rebuild_work_list(uint32_t n, Coloring& col);
 

@@ -309,3 +309,169 @@ def lbvh_from_codes(codes):
 
 def expand_bits21(v: int) -> int:
     return int(lib.oracle_expand_bits21(v))
+
+
+# --- oriented (6-DOF) particles ------------------------------------------------
+for _k, (_r, _a) in {
+    "oracle_add_rod": (C.c_int64, [C.c_void_p, C.c_uint32, C.c_double, _D, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, _D, _D, _D, C.c_int, C.c_int, C.c_int]),
+    "oracle_add_sphere": (C.c_int64, [C.c_void_p, C.c_double, C.c_double, _D, _D, C.c_int, C.c_int]),
+    "oracle_add_oriented": (C.c_int64, [C.c_void_p, _D, _D, C.c_double, _D]),
+    "oracle_add_cosserat": (None, [C.c_void_p, C.c_uint32, C.c_uint32, _D, C.c_double, _D]),
+    "oracle_add_joint": (C.c_int64, [C.c_void_p, C.c_uint32, _D, _D, _D, _D]),
+    "oracle_num_cosserat": (C.c_uint32, [C.c_void_p]),
+    "oracle_num_rground": (C.c_uint32, [C.c_void_p]),
+    "oracle_num_joints": (C.c_uint32, [C.c_void_p]),
+    "oracle_get_rotations": (None, [C.c_void_p, _D, _D, _D, _U8]),
+    "oracle_get_mass": (None, [C.c_void_p, _D, _U8]),
+    "oracle_set_rotation_state": (None, [C.c_void_p, _D, _D]),
+    "oracle_get_rground": (None, [C.c_void_p, _D]),
+    "oracle_get_joints": (None, [C.c_void_p, _D]),
+    "oracle_cosserat_single": (C.c_double, [_D, _D, _D, C.c_double, _D, C.c_uint32, _D, _D]),
+    "oracle_joint_single": (None, [_D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]),
+    "oracle_rground_single": (None, [_D, _D, _D, _D, _D, _D, _D, _D]),
+    "oracle_exp_s3": (None, [_D, _D]),
+    "oracle_log_s3": (None, [_D, _D]),
+    "oracle_qmat": (None, [_D, _D]),
+    "oracle_exp_so3": (None, [_D, _D]),
+    "oracle_jr": (None, [_D, _D]),
+    "oracle_jr_inv": (None, [_D, _D]),
+}.items():
+    _f = getattr(lib, _k)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+def _a(v, n=None):
+    a = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    if n is not None:
+        assert a.shape[0] == n
+    return a
+
+
+def _add_rod(self, elements, length, curvature=(0, 0, 0), E=1e7, nu=0.3, density=1000.0, diameter=0.01,
+             pos=(0, 0, 0), rot=(1, 0, 0, 0), vel=(0, 0, 0), base_fixed=False, tip_fixed=False, with_ground=False):
+    r = lib.oracle_add_rod(self.h, elements, length, dp(_a(curvature, 3)), E, nu, density, diameter,
+                           dp(_a(pos, 3)), dp(_a(rot, 4)), dp(_a(vel, 3)), int(base_fixed), int(tip_fixed),
+                           int(with_ground))
+    if r < 0:
+        raise OracleError(lib.oracle_last_error().decode())
+    return int(r)
+
+
+def _add_sphere(self, radius, density, pos, rot=(1, 0, 0, 0), fixed=False, with_ground=True):
+    return int(lib.oracle_add_sphere(self.h, radius, density, dp(_a(pos, 3)), dp(_a(rot, 4)), int(fixed),
+                                     int(with_ground)))
+
+
+def _add_oriented(self, pos, rot, mass, inertia):
+    return int(lib.oracle_add_oriented(self.h, dp(_a(pos, 3)), dp(_a(rot, 4)), mass, dp(_a(inertia, 3))))
+
+
+def _add_cosserat(self, a, b, stiffness, L, kappa=(0, 0, 0)):
+    lib.oracle_add_cosserat(self.h, a, b, dp(_a(stiffness, 6)), L, dp(_a(kappa, 3)))
+
+
+def _add_joint(self, p, pos_offset, rot_offset, ref_pos, ref_rot):
+    return int(lib.oracle_add_joint(self.h, p, dp(_a(pos_offset, 3)), dp(_a(rot_offset, 4)), dp(_a(ref_pos, 3)),
+                                    dp(_a(ref_rot, 4))))
+
+
+def _rotations(self):
+    n = self.n
+    q = np.empty((n, 4))
+    w = np.empty((n, 3))
+    inertia = np.empty((n, 3))
+    o = np.empty(n, dtype=np.uint8)
+    lib.oracle_get_rotations(self.h, dp(q), dp(w), dp(inertia), o.ctypes.data_as(_U8))
+    return dict(q=q, w=w, inertia=inertia, oriented=o.astype(bool))
+
+
+def _mass_fixed(self):
+    m = np.empty(self.n)
+    f = np.empty(self.n, dtype=np.uint8)
+    lib.oracle_get_mass(self.h, dp(m), f.ctypes.data_as(_U8))
+    return m, f
+
+
+def _rground(self):
+    n = lib.oracle_num_rground(self.h)
+    rec = np.empty((n, 16))
+    lib.oracle_get_rground(self.h, dp(rec))
+    return rec
+
+
+def _joints(self):
+    n = lib.oracle_num_joints(self.h)
+    rec = np.empty((n, 18))
+    lib.oracle_get_joints(self.h, dp(rec))
+    return dict(k=rec[:, :6], lam=rec[:, 6:12], C_prev=rec[:, 12:])
+
+
+OracleSim.add_rod = _add_rod
+OracleSim.add_sphere = _add_sphere
+OracleSim.add_oriented = _add_oriented
+OracleSim.add_cosserat = _add_cosserat
+OracleSim.add_joint = _add_joint
+def _set_rotation_state(self, q, w):
+    lib.oracle_set_rotation_state(self.h, dp(_a(q, 4 * self.n)), dp(_a(w, 3 * self.n)))
+
+
+OracleSim.rotations = _rotations
+OracleSim.set_rotation_state = _set_rotation_state
+OracleSim.mass_fixed = _mass_fixed
+OracleSim.rground = _rground
+OracleSim.joints = _joints
+
+
+def cosserat_single(x2, q2, stiffness, L, kappa, local):
+    G = np.empty(6)
+    H = np.empty(36)
+    e = lib.oracle_cosserat_single(dp(_a(x2, 6)), dp(_a(q2, 8)), dp(_a(stiffness, 6)), L, dp(_a(kappa, 3)), local,
+                                   dp(G), dp(H))
+    return e, G, H.reshape(6, 6)
+
+
+def joint_single(x, q, pos_offset, rot_offset, ref_pos, ref_rot, k, lam, C_prev=(0,) * 6):
+    Cv, G, H = np.empty(6), np.empty(6), np.empty(36)
+    lib.oracle_joint_single(dp(_a(x, 3)), dp(_a(q, 4)), dp(_a(pos_offset, 3)), dp(_a(rot_offset, 4)),
+                            dp(_a(ref_pos, 3)), dp(_a(ref_rot, 4)), dp(_a(k, 6)), dp(_a(lam, 6)), dp(_a(C_prev, 6)),
+                            dp(Cv), dp(G), dp(H))
+    return Cv, G, H.reshape(6, 6)
+
+
+def rground_single(x, q, rec12, cp_local, cp_ground):
+    Cv, G, H = np.empty(3), np.empty(6), np.empty(36)
+    lib.oracle_rground_single(dp(_a(x, 3)), dp(_a(q, 4)), dp(_a(rec12, 12)), dp(_a(cp_local, 3)),
+                              dp(_a(cp_ground, 3)), dp(Cv), dp(G), dp(H))
+    return Cv, G, H.reshape(6, 6)
+
+
+def _v3out(fn, arg, n):
+    out = np.empty(n)
+    fn(dp(_a(arg)), dp(out))
+    return out
+
+
+def exp_s3(v):
+    return _v3out(lib.oracle_exp_s3, v, 4)
+
+
+def log_s3(q):
+    return _v3out(lib.oracle_log_s3, q, 3)
+
+
+def qmat(q):
+    return _v3out(lib.oracle_qmat, q, 9).reshape(3, 3)
+
+
+def exp_so3(v):
+    return _v3out(lib.oracle_exp_so3, v, 9).reshape(3, 3)
+
+
+def jr(v):
+    return _v3out(lib.oracle_jr, v, 9).reshape(3, 3)
+
+
+def jr_inv(v):
+    return _v3out(lib.oracle_jr_inv, v, 9).reshape(3, 3)

@@ -1,0 +1,79 @@
+// simulation.hpp -- the oracle's restatement of one reference Simulation
+// (SimulationContext + VBDSolver), positional and oriented particles.
+// Test infrastructure only; see oracle.hpp.
+#pragma once
+
+#include "oracle.hpp"
+#include "rigid.hpp"
+
+namespace oracle {
+
+class Simulation {
+public:
+    Params params;
+    // ParticlePool, common/ParticlePool.hpp:78-90
+    std::vector<V3> x, x_buf, y, x_prev, x_last, x_lastlast, v, v_prev;
+    std::vector<Real> mass;
+    std::vector<uint8_t> fixed;
+    RotState rs;   // RotationPool through rotation_idx
+
+    // EnergyRegistry (energy/EnergyRegistry.hpp:17-27)
+    std::vector<NHTet> nh;
+    std::vector<CosseratRod> cos;
+    std::vector<GroundContact> ground;
+    std::vector<RigidGround> rground;
+    std::vector<Joint> joints;
+
+    std::vector<uint32_t> tris;     // collision triangles (3 particle indices each)
+    BroadPhase bp;
+    uint64_t last_candidates = 0;
+
+    StaticAdjacency adj;
+    Coloring coloring;
+    std::vector<uint8_t> is_conflicted;
+
+    // Setup (simobject/TetMeshObject.cpp:46-96 and simulation/Simulation.hpp:117-126).
+    uint32_t add_tet_mesh(uint32_t nv, const Real* rest_x, uint32_t nt, const uint32_t* tets,
+                          Real E, Real nu, Real density, Real kd, bool with_ground,
+                          Real mu_s, Real mu_k);
+    // ParticlePool::addParticle (common/ParticlePool.hpp:213-238) for n free particles.
+    uint32_t add_particles(uint32_t n, const Real* x, const Real* m);
+    // ParticlePool::addOrientedParticle (common/ParticlePool.hpp:248-263).
+    uint32_t add_oriented(V3 p, Quat q, Real m, V3 inertia);
+    // Rod ctor (simobject/Rod.cpp:10-78) + per-node rigid ground contacts
+    // (simulation/Simulation.hpp:128-138).  Returns the first node.
+    uint32_t add_rod(uint32_t elements, Real length, V3 curvature, Real E, Real nu, Real density, Real diameter,
+                     V3 pos, Quat rot, V3 vel, bool base_fixed, bool tip_fixed, bool with_ground);
+    // RigidSphere ctor (simobject/rigid/RigidSphere.cpp:8-22, RigidObjectBase.cpp:10-20)
+    // + ground contact (simulation/Simulation.hpp:108-115).
+    uint32_t add_sphere(Real radius, Real density, V3 pos, Quat rot, bool fixed_, bool with_ground);
+    // OneSidedFixedJointEnergyPool::addEnergy (energy/joint/OneSidedFixedJointEnergyPool.hpp:33-50).
+    uint32_t add_joint(uint32_t p, V3 pos_offset, Quat rot_offset, V3 ref_position, Quat ref_rotation);
+
+    void build_graph();     // adjacency + DSATUR + work list (simulation/Simulation.cpp:87-89)
+
+    // One time step of VBDSolver::solve_Parallel, single threaded.
+    void step();
+    // Same step on n_threads SPMD workers with a spin barrier between phases.
+    void step_mt(uint32_t n_threads, uint32_t n_steps);
+
+    Stats stats() const;
+
+    void accumulate_vertex(uint32_t p, V3& G, M3& H) const;
+    void accumulate_vertex6(uint32_t p, V6& G, M6& H) const;
+    bool oriented(uint32_t p) const { return !rs.rot_of.empty() && rs.rot_of[p] >= 0; }
+
+private:
+    void phase_after_timestep(uint32_t lo_nh, uint32_t hi_nh, uint32_t lo_g, uint32_t hi_g);
+    void phase_after_timestep_rigid(uint32_t lo, uint32_t hi);
+    void phase_inertial(uint32_t lo, uint32_t hi);
+    void phase_color(uint32_t c, uint32_t lo, uint32_t hi);
+    void phase_chebyshev(Real omega, uint32_t lo, uint32_t hi);
+    void phase_dual(uint32_t lo, uint32_t hi);
+    void phase_dual_rigid(uint32_t lo, uint32_t hi);
+    void phase_velocity(uint32_t lo, uint32_t hi);
+    void solve_particle(uint32_t p);
+    void solve_particle6(uint32_t p);
+};
+
+}  // namespace oracle

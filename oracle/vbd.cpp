@@ -1,6 +1,6 @@
 // vbd.cpp -- FP64 restatement of the reference AVBD step (test infrastructure
 // only; see oracle.hpp for the rules on who may load this).
-#include "oracle.hpp"
+#include "simulation.hpp"
 
 #include <algorithm>
 #include <stdexcept>
@@ -281,8 +281,17 @@ void ground_update_after_iteration(GroundContact& c, const std::vector<V3>& x) {
 }
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
 // Setup
 // ---------------------------------------------------------------------------
+
+static void push_particle(Simulation& s, V3 p, Real m) {
+    s.x.push_back(p); s.x_buf.push_back(p); s.y.push_back(p); s.x_prev.push_back(p);
+    s.x_last.push_back(p); s.x_lastlast.push_back(p);
+    s.v.push_back(V3()); s.v_prev.push_back(V3());
+    s.mass.push_back(m); s.fixed.push_back(0);
+    s.rs.rot_of.push_back(-1);
+}
 
 // TetMeshObject::setup, simobject/TetMeshObject.cpp:46-96, and the per-vertex
 // ground energies of simulation/Simulation.hpp:117-126.
@@ -290,14 +299,7 @@ uint32_t Simulation::add_tet_mesh(uint32_t nv, const Real* rest_x, uint32_t nt, 
                                   Real E, Real nu, Real density, Real kd, bool with_ground,
                                   Real mu_s, Real mu_k) {
     uint32_t base = (uint32_t)x.size();
-    for (uint32_t i = 0; i < nv; i++) {
-        V3 p(rest_x[3 * i], rest_x[3 * i + 1], rest_x[3 * i + 2]);
-        // ParticlePool::addParticle, common/ParticlePool.hpp:213-238
-        x.push_back(p); x_buf.push_back(p); y.push_back(p); x_prev.push_back(p);
-        x_last.push_back(p); x_lastlast.push_back(p);
-        v.push_back(V3()); v_prev.push_back(V3());
-        mass.push_back(0); fixed.push_back(0);
-    }
+    for (uint32_t i = 0; i < nv; i++) push_particle(*this, V3(rest_x[3 * i], rest_x[3 * i + 1], rest_x[3 * i + 2]), 0);
     Real mu = E / (2 * (1 + nu));
     Real lambda = (E * nu) / ((1 + nu) * (1 - 2 * nu));
     for (uint32_t e = 0; e < nt; e++) {
@@ -335,19 +337,106 @@ uint32_t Simulation::add_tet_mesh(uint32_t nv, const Real* rest_x, uint32_t nt, 
 
 uint32_t Simulation::add_particles(uint32_t n, const Real* xs, const Real* m) {
     uint32_t base = (uint32_t)x.size();
-    for (uint32_t i = 0; i < n; i++) {
-        V3 p(xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]);
-        x.push_back(p); x_buf.push_back(p); y.push_back(p); x_prev.push_back(p);
-        x_last.push_back(p); x_lastlast.push_back(p);
-        v.push_back(V3()); v_prev.push_back(V3());
-        mass.push_back(m[i]); fixed.push_back(0);
-    }
+    for (uint32_t i = 0; i < n; i++) push_particle(*this, V3(xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]), m[i]);
     return base;
+}
+
+uint32_t Simulation::add_oriented(V3 p, Quat q, Real m, V3 inertia) {
+    const uint32_t idx = (uint32_t)x.size();
+    push_particle(*this, p, m);
+    rs.rot_of[idx] = (int32_t)rs.q.size();
+    // RotationPool::addParticle, common/ParticlePool.hpp:37-51 (angular velocity starts at zero)
+    rs.q.push_back(q); rs.q_buf.push_back(q); rs.q_in.push_back(q); rs.q_prev.push_back(q);
+    rs.q_last.push_back(q); rs.q_lastlast.push_back(q);
+    rs.w.push_back(V3()); rs.w_prev.push_back(V3()); rs.inertia.push_back(inertia);
+    return idx;
+}
+
+static RigidGround make_rground(uint32_t p, uint32_t shape, Real radius, V3 cp_local, Real k_start, Real mu_s,
+                                Real mu_k) {
+    // RigidBodyGroundCollisionEnergyPool::addEnergy (+ CollisionConstraintEnergyPool::addEnergy)
+    RigidGround c{};
+    c.p = p; c.shape = shape; c.radius = radius; c.cp_rb_local = cp_local; c.cp_ground = V3();
+    c.n = V3(0, 1, 0); c.t = V3(1, 0, 0); c.b = V3(0, 0, 1);
+    c.k_start = c.k_n = c.k_t = c.k_b = k_start;
+    c.lambda_n = c.lambda_t = c.lambda_b = 0; c.C_n_prev = 0;
+    c.use_static = false; c.unclamped_lambda_tb = false;
+    c.mu_s = mu_s; c.mu_k = mu_k;
+    return c;
+}
+
+uint32_t Simulation::add_rod(uint32_t elements, Real length, V3 curvature, Real E, Real nu, Real density,
+                             Real diameter, V3 pos, Quat rot, V3 vel, bool base_fixed, bool tip_fixed,
+                             bool with_ground) {
+    const Real radius = diameter / 2;
+    const Real G = E / (2 * (1 + nu));
+    const uint32_t nn = elements + 1;
+    const Real ds = length / (nn - 1);
+    const V3 dR = ds * curvature;
+    std::vector<uint32_t> nodes(nn);
+    V3 cp = pos;
+    Quat cq = rot;
+    for (uint32_t i = 0; i < nn; i++) {
+        nodes[i] = add_oriented(cp, cq, 0, V3());
+        v[nodes[i]] = vel;   // previous_velocities stay zero (simobject/Rod.cpp:33)
+        cp = cp + qrot(cq, V3(0, 0, ds));
+        cq = qmul(cq, exp_s3(dR));
+    }
+    const Real pi = 3.14159265358979323846;
+    const Real area = pi * radius * radius;
+    const Real Ix = pi * radius * radius * radius * radius / 4.0, Iy = Ix, Iz = 2 * Ix;
+    const Real elem_len = length / (nn - 1);
+    const Real elem_mass = elem_len * area * density;
+    const V3 elem_inertia = (density * elem_len) * V3(Ix, Iy, Iz);
+    for (uint32_t e = 0; e < elements; e++) {
+        mass[nodes[e]] += 0.5 * elem_mass;
+        mass[nodes[e + 1]] += 0.5 * elem_mass;
+        rs.inertia[rs.rot_of[nodes[e]]] += 0.5 * elem_inertia;
+        rs.inertia[rs.rot_of[nodes[e + 1]]] += 0.5 * elem_inertia;
+    }
+    for (uint32_t e = 0; e < elements; e++) {
+        CosseratRod r{};
+        r.idx[0] = nodes[e]; r.idx[1] = nodes[e + 1];
+        const Real st[6] = {G * area, G * area, E * area, E * Ix, E * Iy, G * Iz};
+        for (int k = 0; k < 6; k++) r.stiffness[k] = st[k];
+        r.L = ds;
+        r.kappa0 = curvature;
+        cos.push_back(r);
+    }
+    if (base_fixed) fixed[nodes.front()] = 1;
+    if (tip_fixed) fixed[nodes.back()] = 1;
+    if (with_ground)
+        for (uint32_t n_idx : nodes)
+            rground.push_back(make_rground(n_idx, SHAPE_ROD, radius, V3(0, -radius, 0),
+                                           mass[n_idx] / (params.dt * params.dt), 0.5, 0.2));
+    return nodes.front();
+}
+
+uint32_t Simulation::add_sphere(Real radius, Real density, V3 pos, Quat rot, bool fixed_, bool with_ground) {
+    const Real pi = 3.14159265358979323846;
+    const uint32_t p = add_oriented(pos, rot, 0, V3());
+    fixed[p] = fixed_ ? 1 : 0;
+    const Real m = 4 * pi / 3 * radius * radius * radius * density;
+    mass[p] = m;
+    rs.inertia[rs.rot_of[p]] = (0.4 * m * radius * radius) * V3(1, 1, 1);
+    if (with_ground)
+        rground.push_back(make_rground(p, SHAPE_SPHERE, radius, V3(0, -radius, 0), m / (params.dt * params.dt),
+                                       0.5, 0.2));
+    return p;
+}
+
+uint32_t Simulation::add_joint(uint32_t p, V3 pos_offset, Quat rot_offset, V3 ref_position, Quat ref_rotation) {
+    Joint j{};
+    j.p = p; j.pos_offset = pos_offset; j.rot_offset = rot_offset;
+    j.ref_position = ref_position; j.ref_rotation = ref_rotation;
+    for (int i = 0; i < 6; i++) { j.k[i] = JOINT_K_START; j.lambda[i] = 0; j.C_prev[i] = 0; }
+    joints.push_back(j);
+    return (uint32_t)joints.size() - 1;
 }
 
 void Simulation::build_graph() {
     uint32_t n = (uint32_t)x.size();
-    build_static_adjacency(n, nh, ground, adj);
+    build_static_adjacency(n, nh, cos, ground, rground, joints, adj);
     dsatur(adj, n, coloring);
     rebuild_work_list(n, coloring);
     is_conflicted.assign(n, 0);
@@ -365,7 +454,19 @@ void Simulation::phase_after_timestep(uint32_t lo_nh, uint32_t hi_nh, uint32_t l
     }
 }
 
-// _particleInertialUpdate, VBDSolver.hpp:208-248 (adaptive initialisation)
+void Simulation::phase_after_timestep_rigid(uint32_t lo, uint32_t hi) {
+    // records [0, |rground|) then [|rground|, |rground| + |joints|)
+    for (uint32_t e = lo; e < hi; e++) {
+        if (e < rground.size()) {
+            rground_update_contact_points(rground[e], x, rs);
+            rground_update_after_timestep(rground[e], x, rs);
+        } else {
+            joint_update_after_timestep(joints[e - rground.size()], x, rs);
+        }
+    }
+}
+
+// _particleInertialUpdate + _particleRotationInertialUpdate, VBDSolver.hpp:208-283
 void Simulation::phase_inertial(uint32_t lo, uint32_t hi) {
     V3 a_ext(0, -params.g, 0);
     Real dt = params.dt;
@@ -386,6 +487,19 @@ void Simulation::phase_inertial(uint32_t lo, uint32_t hi) {
         x[p] = x[p] + (dt * v[p] + (dt * dt) * a_tilde_vec);
         x_last[p] = x[p];
         x_lastlast[p] = x[p];
+        if (oriented(p)) {
+            const int32_t r = rs.rot_of[p];
+            const V3 w = rs.w[r], I = rs.inertia[r];
+            const V3 Iinv(1 / I.x, 1 / I.y, 1 / I.z);
+            const V3 Iw(I.x * w.x, I.y * w.y, I.z * w.z);
+            const V3 c = cross(w, Iw);
+            const V3 corr(-Iinv.x * c.x, -Iinv.y * c.y, -Iinv.z * c.z);   // alpha_ext = 0
+            const V3 dq_in = dt * w + (dt * dt) * corr;
+            rs.q_in[r] = plus_s3(rs.q[r], dq_in);
+            rs.q[r] = plus_s3(rs.q[r], dt * w);
+            rs.q_last[r] = rs.q[r];
+            rs.q_lastlast[r] = rs.q[r];
+        }
     }
 }
 
@@ -411,12 +525,46 @@ void Simulation::accumulate_vertex(uint32_t p, V3& G, M3& H) const {
         } else if (ty == ST_GROUND) {
             for (; e < fe; e++) ground_accumulate(ground[adj.e_entries[e].e_idx], x, G, H);
         }
+        // Cosserat, rigid ground and joints support oriented particles only
+    }
+}
+
+// Static energies of an oriented particle in StaticEnergyType order
+// (VBDSolver.hpp:297-397): NH (positional only: skipped), Cosserat, ground
+// (accumulate<6>: positional block), rigid ground, joint.
+void Simulation::accumulate_vertex6(uint32_t p, V6& G, M6& H) const {
+    const auto& off = adj.e_offsets[p];
+    uint32_t end = adj.e_offsets[p + 1][0];
+    for (int ty = 0; ty < ST_COUNT; ty++) {
+        uint32_t e = off[ty];
+        uint32_t fe = (ty + 1 == ST_COUNT) ? end : off[ty + 1];
+        for (; e < fe; e++) {
+            const AdjEntry& en = adj.e_entries[e];
+            if (ty == ST_COSSERAT) {
+                cosserat_accumulate(cos[en.e_idx], x, rs, en.local, G, H);
+            } else if (ty == ST_GROUND) {
+                V3 g3; M3 h3;
+                ground_accumulate(ground[en.e_idx], x, g3, h3);
+                for (int i = 0; i < 3; i++) {
+                    G.v[i] += g3[i];
+                    for (int j = 0; j < 3; j++) H.m[i][j] += h3.m[i][j];
+                }
+            } else if (ty == ST_RIGID_GROUND) {
+                rground_accumulate(rground[en.e_idx], x, rs, G, H);
+            } else if (ty == ST_JOINT) {
+                joint_accumulate(joints[en.e_idx], x, rs, G, H);
+            }
+        }
     }
 }
 
 // _solveParticle<3>, VBDSolver.hpp:285-449,487-510
 void Simulation::solve_particle(uint32_t p) {
     if (fixed[p]) return;
+    if (oriented(p)) {
+        solve_particle6(p);
+        return;
+    }
     V3 G; M3 H;
     accumulate_vertex(p, G, H);
     Real dt = params.dt;
@@ -433,6 +581,38 @@ void Simulation::solve_particle(uint32_t p) {
     else x[p] += d;
 }
 
+// _solveParticle<6>, VBDSolver.hpp:450-486
+void Simulation::solve_particle6(uint32_t p) {
+    V6 G; M6 H;
+    accumulate_vertex6(p, G, H);
+    const Real dt = params.dt;
+    const Real m = mass[p];
+    const int32_t r = rs.rot_of[p];
+    const V3 I = rs.inertia[r];
+    const V3 dpos = (-1 / (dt * dt) * m) * (x[p] - y[p]);
+    const V3 th = minus_s3(rs.q[r], rs.q_in[r]);
+    const Real s = -1 / (dt * dt);
+    Real rhs[6] = {dpos.x, dpos.y, dpos.z, s * (I.x * th.x), s * (I.y * th.y), s * (I.z * th.z)};
+    for (int i = 0; i < 6; i++) rhs[i] -= G.v[i];
+    Real A[36];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) A[6 * i + j] = H.m[i][j];
+    for (int i = 0; i < 3; i++) {
+        A[6 * i + i] += m / (dt * dt);
+        A[6 * (3 + i) + 3 + i] += I[i] / (dt * dt);
+    }
+    Real dx[6];
+    lu_solve(6, A, rhs, dx);
+    const V3 dp(dx[0], dx[1], dx[2]), dth(dx[3], dx[4], dx[5]);
+    if (is_conflicted[p]) {
+        x_buf[p] = x[p] + dp;
+        rs.q_buf[r] = plus_s3(rs.q[r], dth);
+    } else {
+        x[p] += dp;
+        rs.q[r] = plus_s3(rs.q[r], dth);
+    }
+}
+
 void Simulation::phase_color(uint32_t c, uint32_t lo, uint32_t hi) {
     uint32_t start = coloring.color_offsets[c];
     for (uint32_t k = start + lo; k < start + hi; k++) solve_particle(coloring.work_list[k]);
@@ -442,13 +622,29 @@ void Simulation::phase_color(uint32_t c, uint32_t lo, uint32_t hi) {
 void Simulation::phase_chebyshev(Real omega, uint32_t lo, uint32_t hi) {
     for (uint32_t p = lo; p < hi; p++) {
         x[p] = omega * (x[p] - x_lastlast[p]) + x_lastlast[p];
+        if (oriented(p)) {
+            const int32_t r = rs.rot_of[p];
+            rs.q[r] = plus_s3(rs.q_lastlast[r], omega * minus_s3(rs.q[r], rs.q_lastlast[r]));
+        }
         x_lastlast[p] = x_last[p];
         x_last[p] = x[p];
+        if (oriented(p)) {
+            const int32_t r = rs.rot_of[p];
+            rs.q_lastlast[r] = rs.q_last[r];
+            rs.q_last[r] = rs.q[r];
+        }
     }
 }
 
 void Simulation::phase_dual(uint32_t lo, uint32_t hi) {
     for (uint32_t e = lo; e < hi; e++) ground_update_after_iteration(ground[e], x);
+}
+
+void Simulation::phase_dual_rigid(uint32_t lo, uint32_t hi) {
+    for (uint32_t e = lo; e < hi; e++) {
+        if (e < rground.size()) rground_update_after_iteration(rground[e], x, rs);
+        else joint_update_after_iteration(joints[e - rground.size()], x, rs);
+    }
 }
 
 // _particleVelocityUpdate, VBDSolver.hpp:538-566
@@ -458,6 +654,12 @@ void Simulation::phase_velocity(uint32_t lo, uint32_t hi) {
         v_prev[p] = v[p];
         v[p] = (x[p] - x_prev[p]) / dt;
         x_prev[p] = x[p];
+        if (oriented(p)) {
+            const int32_t r = rs.rot_of[p];
+            rs.w_prev[r] = rs.w[r];
+            rs.w[r] = minus_s3(rs.q[r], rs.q_prev[r]) / dt;
+            rs.q_prev[r] = rs.q[r];
+        }
     }
 }
 
@@ -475,7 +677,9 @@ void Simulation::step() {
         bp.build(tris, x, v, params.dt);
         last_candidates = bp.traverse_leaves(0, bp.n);
     }
+    const uint32_t n_rigid = (uint32_t)(rground.size() + joints.size());
     phase_after_timestep(0, (uint32_t)nh.size(), 0, (uint32_t)ground.size());
+    phase_after_timestep_rigid(0, n_rigid);
     phase_inertial(0, n);
     Real omega = 1;
     for (uint32_t i = 0; i < params.iters; i++) {
@@ -483,6 +687,7 @@ void Simulation::step() {
         for (uint32_t c = 0; c < coloring.num_colors; c++) phase_color(c, 0, coloring.color_counts[c]);
         phase_chebyshev(omega, 0, n);
         phase_dual(0, (uint32_t)ground.size());
+        phase_dual_rigid(0, n_rigid);
     }
     phase_velocity(0, n);
 }
@@ -520,6 +725,7 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
     }
     SpinBarrier bar(nthreads);
     std::vector<uint64_t> cand(nthreads, 0);
+    const uint32_t n_rigid = (uint32_t)(rground.size() + joints.size());
     auto worker = [&](uint32_t w) {
         uint32_t n = (uint32_t)x.size();
         uint32_t lo, hi, lo2, hi2;
@@ -541,6 +747,11 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
             range((uint32_t)ground.size(), w, nthreads, lo2, hi2);
             phase_after_timestep(0, 0, lo2, hi2);
             bar.arrive_and_wait();
+            if (n_rigid) {
+                range(n_rigid, w, nthreads, lo, hi);
+                phase_after_timestep_rigid(lo, hi);
+                bar.arrive_and_wait();
+            }
             range(n, w, nthreads, lo, hi);
             phase_inertial(lo, hi);
             bar.arrive_and_wait();
@@ -560,6 +771,11 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
                     phase_dual(lo, hi);
                     bar.arrive_and_wait();
                 }
+                if (n_rigid) {
+                    range(n_rigid, w, nthreads, lo, hi);
+                    phase_dual_rigid(lo, hi);
+                    bar.arrive_and_wait();
+                }
             }
             range(n, w, nthreads, lo, hi);
             phase_velocity(lo, hi);
@@ -576,10 +792,9 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
 // Force residual and incremental potential (new definitions)
 //   r_v = m_v/dt^2 (x_v - y_v) + sum_e dE_e/dx_v  (AL terms with lambda^+)
 //       = -RHS of VBDSolver.hpp:493 at the current iterate, free vertices only
+//       (positional block for oriented particles)
 //   rel = ||r||_2 / ||M g||_2 over free vertices
 //   IP  = sum m/(2dt^2)|x-y|^2 + sum_e E_e + sum_ground (1/2 k C^2 + lambda C)
-//         (the AL form of HardConstraintEnergySolver.hpp:17-35; normal term on
-//          the corrected constraint, clamped at lambda^+ = 0)
 // ---------------------------------------------------------------------------
 Stats Simulation::stats() const {
     Stats s;
@@ -588,8 +803,15 @@ Stats Simulation::stats() const {
     uint32_t n = (uint32_t)x.size();
     for (uint32_t p = 0; p < n; p++) {
         if (fixed[p]) continue;
-        V3 G; M3 H;
-        accumulate_vertex(p, G, H);
+        V3 G;
+        if (oriented(p)) {
+            V6 G6; M6 H6;
+            accumulate_vertex6(p, G6, H6);
+            G = V3(G6.v[0], G6.v[1], G6.v[2]);
+        } else {
+            M3 H;
+            accumulate_vertex(p, G, H);
+        }
         V3 r = (mass[p] / (dt * dt)) * (x[p] - y[p]) + G;
         r2 += dot(r, r);
         Real mg = mass[p] * params.g;
@@ -598,6 +820,7 @@ Stats Simulation::stats() const {
         s.ip += 0.5 * mass[p] / (dt * dt) * dot(d, d);
     }
     for (const auto& t : nh) s.ip += nh_energy(t, x, dt);
+    for (const auto& e : cos) s.ip += cosserat_energy(e, x, rs);
     for (const auto& c : ground) {
         Real Cn, Ct, Cb;
         ground_eval(c, x, Cn, Ct, Cb);
