@@ -82,6 +82,58 @@ class GroundRecord(C.Structure):
     ]
 
 
+# CollisionGeometryType of the reference (collision/CollisionGeometryType.hpp)
+SHAPE_ROD_SEGMENT = 1
+SHAPE_SPHERE = 2
+SHAPE_BOX = 3
+SHAPE_CAPSULE = 4
+
+
+class CosseratRecord(C.Structure):
+    _fields_ = [
+        ("particle_indices", C.c_uint32 * 2),
+        ("stiffness", C.c_double * 6),
+        ("rest_length", C.c_double),
+        ("rest_curvature", C.c_double * 3),
+    ]
+
+
+class RigidGroundRecord(C.Structure):
+    _fields_ = [
+        ("particle_index", C.c_uint32),
+        ("shape", C.c_uint32),
+        ("use_static", C.c_uint32),
+        ("unclamped_lambda_tb", C.c_uint32),
+        ("radius", C.c_double),
+        ("cp_rb_local", C.c_double * 3),
+        ("cp_ground", C.c_double * 3),
+        ("k_start", C.c_double),
+        ("k_n", C.c_double),
+        ("k_t", C.c_double),
+        ("k_b", C.c_double),
+        ("lambda_n", C.c_double),
+        ("lambda_t", C.c_double),
+        ("lambda_b", C.c_double),
+        ("C_n_prev", C.c_double),
+        ("mu_s", C.c_double),
+        ("mu_k", C.c_double),
+    ]
+
+
+class JointRecord(C.Structure):
+    _fields_ = [
+        ("particle_index", C.c_uint32),
+        ("pad_", C.c_uint32),
+        ("position_offset", C.c_double * 3),
+        ("rotation_offset", C.c_double * 4),
+        ("ref_position", C.c_double * 3),
+        ("ref_rotation", C.c_double * 4),
+        ("k", C.c_double * 6),
+        ("lambda_", C.c_double * 6),
+        ("C_prev", C.c_double * 6),
+    ]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("residual_abs", C.c_double),
@@ -109,6 +161,12 @@ SIGNATURES = {
     "avbd_last_error": (C.c_char_p, [_P]),
     "avbd_add_particles": (C.c_int, [_P, C.c_uint32, _D, _D, _D, _D, _D, _U8, _U32]),
     "avbd_add_energies": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_uint32]),
+    "avbd_add_oriented_particles": (C.c_int, [_P, C.c_uint32, _D, _D, _D, _D, _D, _D, _U8, _U32]),
+    "avbd_add_rod": (C.c_int, [_P, C.c_uint32, C.c_double, _D, C.c_double, C.c_double, C.c_double, C.c_double, _D,
+                               _D, _D, C.c_int, C.c_int, C.c_int, _U32]),
+    "avbd_add_sphere": (C.c_int, [_P, C.c_double, C.c_double, _D, _D, C.c_int, C.c_int, _U32]),
+    "avbd_set_rotation_state": (C.c_int, [_P, C.c_uint32, C.c_uint32, _D, _D]),
+    "avbd_download_rotations": (C.c_int, [_P, _D, _D]),
     "avbd_add_tet_mesh": (C.c_int, [_P, C.c_uint32, _D, C.c_uint32, _U32, C.c_double, C.c_double, C.c_double,
                                     C.c_double, C.c_int, C.c_double, C.c_double, _U32]),
     "avbd_set_particle_state": (C.c_int, [_P, C.c_uint32, C.c_uint32, _D, _D, _D, _D]),

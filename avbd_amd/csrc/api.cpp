@@ -31,6 +31,12 @@ struct avbd_ctx {
     std::vector<HostGround> ground;
     std::vector<uint32_t> scene_offsets;   // empty = one scene
     std::vector<uint32_t> tris;            // collision triangles (3 particles each)
+    // RotationPool (common/ParticlePool.hpp:7-69): slot per oriented particle
+    std::vector<int32_t> rot_of;           // per particle, -1 = positional
+    std::vector<double> q, w, inertia;     // per slot: (w, x, y, z), body-frame w, principal inertia
+    std::vector<HostCos> cos;
+    std::vector<HostRGround> rground;
+    std::vector<HostJoint> joints;
 
     bool planned = false;   // host colour plan done
     bool built = false;     // device buffers uploaded
@@ -139,6 +145,96 @@ GroundRec to_dev(const HostGround& g) {
     return r;
 }
 
+// quaternion (w, x, y, z) FP64 -> device float4 (x, y, z, w)
+float4 qf4(const double* q) { return make_float4((float)q[1], (float)q[2], (float)q[3], (float)q[0]); }
+
+CosRec to_dev(const HostCos& h) {
+    CosRec r{};
+    r.i0 = h.idx[0];
+    r.i1 = h.idx[1];
+    r.L = (float)h.L;
+    r.k0 = (float)h.stiffness[0];
+    r.k1 = (float)h.stiffness[1];
+    r.k2 = (float)h.stiffness[2];
+    r.k3 = (float)h.stiffness[3];
+    r.k4 = (float)h.stiffness[4];
+    r.k5 = (float)h.stiffness[5];
+    r.kx = (float)h.kappa0[0];
+    r.ky = (float)h.kappa0[1];
+    r.kz = (float)h.kappa0[2];
+    return r;
+}
+
+RGroundRec to_dev(const HostRGround& h) {
+    RGroundRec r{};
+    r.k_start = (float)h.k_start;
+    r.k_n = (float)h.k_n;
+    r.k_t = (float)h.k_t;
+    r.k_b = (float)h.k_b;
+    r.lambda_n = (float)h.lambda_n;
+    r.lambda_t = (float)h.lambda_t;
+    r.lambda_b = (float)h.lambda_b;
+    r.C_n_prev = (float)h.C_n_prev;
+    r.mu_s = (float)h.mu_s;
+    r.mu_k = (float)h.mu_k;
+    r.radius = (float)h.radius;
+    r.cpg_x = (float)h.cp_ground[0];
+    r.cpg_y = (float)h.cp_ground[1];
+    r.cpg_z = (float)h.cp_ground[2];
+    r.cpl_x = (float)h.cp_rb_local[0];
+    r.cpl_y = (float)h.cp_rb_local[1];
+    r.cpl_z = (float)h.cp_rb_local[2];
+    r.p = h.p;
+    r.shape = h.shape;
+    r.flags = (h.use_static ? GF_USE_STATIC : 0u) | (h.unclamped ? GF_UNCLAMPED : 0u);
+    return r;
+}
+
+JointRec to_dev(const HostJoint& h) {
+    JointRec r{};
+    r.pos_off = make_float4((float)h.pos_offset[0], (float)h.pos_offset[1], (float)h.pos_offset[2], 0.f);
+    r.rot_off = qf4(h.rot_offset);
+    r.ref_pos = make_float4((float)h.ref_position[0], (float)h.ref_position[1], (float)h.ref_position[2], 0.f);
+    r.ref_rot = qf4(h.ref_rotation);
+    for (int i = 0; i < 6; i++) {
+        r.k[i] = (float)h.k[i];
+        r.lambda[i] = (float)h.lambda[i];
+        r.C_prev[i] = (float)h.C_prev[i];
+    }
+    r.p = h.p;
+    r.pad = 0;
+    return r;
+}
+
+// Append one particle to the FP64 staging pools (ParticlePool::addParticle).
+void push_particle(avbd_ctx* c, const double* x, const double* v, const double* v_prev, const double* x_prev,
+                   double mass, uint8_t fixed) {
+    for (int k = 0; k < 3; k++) {
+        c->x.push_back(x[k]);
+        c->v.push_back(v ? v[k] : 0.0);
+        c->v_prev.push_back(v_prev ? v_prev[k] : 0.0);
+        c->x_prev.push_back(x_prev ? x_prev[k] : x[k]);
+    }
+    c->mass.push_back(mass);
+    c->fixed.push_back(fixed);
+    c->rot_of.push_back(-1);
+}
+
+// Append one oriented particle (ParticlePool::addOrientedParticle, common/ParticlePool.hpp:248-263).
+uint32_t push_oriented(avbd_ctx* c, const double* x, const double* q, const double* v, const double* w, double mass,
+                       const double* inertia, uint8_t fixed) {
+    const uint32_t idx = (uint32_t)c->mass.size();
+    push_particle(c, x, v, nullptr, nullptr, mass, fixed);
+    c->rot_of[idx] = (int32_t)(c->q.size() / 4);
+    const double qi[4] = {1, 0, 0, 0};
+    for (int k = 0; k < 4; k++) c->q.push_back(q ? q[k] : qi[k]);
+    for (int k = 0; k < 3; k++) {
+        c->w.push_back(w ? w[k] : 0.0);
+        c->inertia.push_back(inertia ? inertia[k] : 0.0);
+    }
+    return idx;
+}
+
 void require_built(avbd_ctx* c) {
     if (!c->built) throw CodeError(AVBD_E_NOT_BUILT, "avbd_build_graph() has not been called");
 }
@@ -178,6 +274,29 @@ void upload_particle_range(avbd_ctx* c, uint32_t first, uint32_t n) {
     cp(c->d.y, bx);
     cp(c->d.x_last, bx);
     cp(c->d.x_lastlast, bx);
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+// Upload orientation state of rotation slots [first, first+n): q and its
+// history copies take the staged orientation, w and w_prev the staged
+// angular velocity.
+void upload_rotation_range(avbd_ctx* c, uint32_t first, uint32_t n) {
+    if (n == 0) return;
+    std::vector<float4> bq(n), bw(n), bi(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t r = first + i;
+        bq[i] = qf4(&c->q[4 * r]);
+        bw[i] = f4(&c->w[3 * r], 0.0);
+        bi[i] = f4(&c->inertia[3 * r], 0.0);
+    }
+    auto cp = [&](float4* dst, const std::vector<float4>& src) {
+        hip_check(hipMemcpyAsync(dst + first, src.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream),
+                  "hipMemcpyAsync H2D");
+    };
+    for (float4* dst : {c->d.q, c->d.q_buf, c->d.q_in, c->d.q_prev, c->d.q_last, c->d.q_lastlast}) cp(dst, bq);
+    cp(c->d.w, bw);
+    cp(c->d.w_prev, bw);
+    cp(c->d.inertia, bi);
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
 }
 
@@ -231,6 +350,8 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
             const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
             if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
             launch_solve_color(s, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x);
+            const uint32_t ob = c->plan.oitem_offsets[col], oe = c->plan.oitem_offsets[col + 1];
+            launch_solve_ocolor(s, c->d, ob, oe - ob, dt);
             if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
             const uint32_t cb = c->plan.conflict_offsets[col], ce = c->plan.conflict_offsets[col + 1];
             if (ce > cb) launch_commit(s, c->d, c->d_conflicts + cb, ce - cb);
@@ -373,16 +494,9 @@ int avbd_add_particles(avbd_ctx* c, uint32_t n, const double* x, const double* v
         if (n && !x) throw std::runtime_error("x is required");
         const size_t base = c->mass.size();
         check_capacity(c, base + n, c->desc.particle_capacity);
-        for (uint32_t i = 0; i < n; i++) {
-            for (int k = 0; k < 3; k++) {
-                c->x.push_back(x[3 * i + k]);
-                c->v.push_back(v ? v[3 * i + k] : 0.0);
-                c->v_prev.push_back(v_prev ? v_prev[3 * i + k] : 0.0);
-                c->x_prev.push_back(x_prev ? x_prev[3 * i + k] : x[3 * i + k]);
-            }
-            c->mass.push_back(mass ? mass[i] : 0.0);
-            c->fixed.push_back(fixed ? fixed[i] : 0);
-        }
+        for (uint32_t i = 0; i < n; i++)
+            push_particle(c, x + 3 * i, v ? v + 3 * i : nullptr, v_prev ? v_prev + 3 * i : nullptr,
+                          x_prev ? x_prev + 3 * i : nullptr, mass ? mass[i] : 0.0, fixed ? fixed[i] : 0);
         if (first) *first = (uint32_t)base;
     });
 }
@@ -427,9 +541,210 @@ int avbd_add_energies(avbd_ctx* c, int type, const void* records, uint32_t count
                 g.unclamped = r[i].unclamped_lambda_tb != 0;
                 c->ground.push_back(g);
             }
+        } else if (type == AVBD_COSSERAT_ROD) {
+            check_capacity(c, c->cos.size() + count, c->desc.energy_capacity);
+            const auto* r = static_cast<const avbd_cosserat_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                HostCos h{};
+                h.idx[0] = r[i].particle_indices[0];
+                h.idx[1] = r[i].particle_indices[1];
+                std::memcpy(h.stiffness, r[i].stiffness, sizeof(h.stiffness));
+                h.L = r[i].rest_length;
+                std::memcpy(h.kappa0, r[i].rest_curvature, sizeof(h.kappa0));
+                if (!(h.L > 0)) throw std::runtime_error("Cosserat rest length must be > 0");
+                c->cos.push_back(h);
+            }
+        } else if (type == AVBD_RIGID_BODY_GROUND_COLLISION) {
+            check_capacity(c, c->rground.size() + count, c->desc.energy_capacity);
+            const auto* r = static_cast<const avbd_rigid_ground_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                HostRGround h{};
+                h.p = r[i].particle_index;
+                h.shape = r[i].shape;
+                h.radius = r[i].radius;
+                std::memcpy(h.cp_rb_local, r[i].cp_rb_local, sizeof(h.cp_rb_local));
+                std::memcpy(h.cp_ground, r[i].cp_ground, sizeof(h.cp_ground));
+                h.k_start = r[i].k_start;
+                h.k_n = r[i].k_n;
+                h.k_t = r[i].k_t;
+                h.k_b = r[i].k_b;
+                h.lambda_n = r[i].lambda_n;
+                h.lambda_t = r[i].lambda_t;
+                h.lambda_b = r[i].lambda_b;
+                h.C_n_prev = r[i].C_n_prev;
+                h.mu_s = r[i].mu_s;
+                h.mu_k = r[i].mu_k;
+                h.use_static = r[i].use_static != 0;
+                h.unclamped = r[i].unclamped_lambda_tb != 0;
+                if (h.shape < AVBD_SHAPE_ROD_SEGMENT || h.shape > AVBD_SHAPE_CAPSULE)
+                    throw std::runtime_error("rigid-ground shape must be a rod segment, sphere, box or capsule");
+                c->rground.push_back(h);
+            }
+        } else if (type == AVBD_ONE_SIDED_FIXED_JOINT) {
+            check_capacity(c, c->joints.size() + count, c->desc.energy_capacity);
+            const auto* r = static_cast<const avbd_joint_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                HostJoint h{};
+                h.p = r[i].particle_index;
+                std::memcpy(h.pos_offset, r[i].position_offset, sizeof(h.pos_offset));
+                std::memcpy(h.rot_offset, r[i].rotation_offset, sizeof(h.rot_offset));
+                std::memcpy(h.ref_position, r[i].ref_position, sizeof(h.ref_position));
+                std::memcpy(h.ref_rotation, r[i].ref_rotation, sizeof(h.ref_rotation));
+                std::memcpy(h.k, r[i].k, sizeof(h.k));
+                std::memcpy(h.lambda, r[i].lambda, sizeof(h.lambda));
+                std::memcpy(h.C_prev, r[i].C_prev, sizeof(h.C_prev));
+                c->joints.push_back(h);
+            }
         } else {
             throw CodeError(AVBD_E_ARG, "energy type not supported by this build");
         }
+    });
+}
+
+int avbd_add_oriented_particles(avbd_ctx* c, uint32_t n, const double* x, const double* q, const double* v,
+                                const double* w, const double* mass, const double* inertia, const uint8_t* fixed,
+                                uint32_t* first) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (n && (!x || !inertia)) throw std::runtime_error("x and inertia are required");
+        const size_t base = c->mass.size();
+        check_capacity(c, base + n, c->desc.particle_capacity);
+        for (uint32_t i = 0; i < n; i++)
+            push_oriented(c, x + 3 * i, q ? q + 4 * i : nullptr, v ? v + 3 * i : nullptr, w ? w + 3 * i : nullptr,
+                          mass ? mass[i] : 0.0, inertia + 3 * i, fixed ? fixed[i] : 0);
+        if (first) *first = (uint32_t)base;
+    });
+}
+
+namespace {
+// Hamilton product of (w, x, y, z) quaternions
+void qmul_d(const double* a, const double* b, double* o) {
+    const double w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    const double x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    const double y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    const double z = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    o[0] = w;
+    o[1] = x;
+    o[2] = y;
+    o[3] = z;
+}
+// rotate v by unit q (w, x, y, z)
+void qrot_d(const double* q, const double* v, double* o) {
+    const double u[3] = {q[1], q[2], q[3]};
+    const double t[3] = {2 * (u[1] * v[2] - u[2] * v[1]), 2 * (u[2] * v[0] - u[0] * v[2]),
+                         2 * (u[0] * v[1] - u[1] * v[0])};
+    o[0] = v[0] + q[0] * t[0] + (u[1] * t[2] - u[2] * t[1]);
+    o[1] = v[1] + q[0] * t[1] + (u[2] * t[0] - u[0] * t[2]);
+    o[2] = v[2] + q[0] * t[2] + (u[0] * t[1] - u[1] * t[0]);
+}
+// Exp_s3 (common/Math.hpp:197-221)
+void exp_s3_d(const double* v, double* q) {
+    const double th = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (th < 1e-6) {
+        double a[4] = {1.0, 0.5 * v[0], 0.5 * v[1], 0.5 * v[2]};
+        const double nq = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3]);
+        for (int k = 0; k < 4; k++) q[k] = a[k] / nq;
+        return;
+    }
+    const double s = std::sin(0.5 * th) / th;
+    q[0] = std::cos(0.5 * th);
+    q[1] = s * v[0];
+    q[2] = s * v[1];
+    q[3] = s * v[2];
+}
+HostRGround make_rground(uint32_t p, uint32_t shape, double radius, double k_start) {
+    // RigidBodyGroundCollisionEnergyPool::addEnergy with mu_s 0.5, mu_k 0.2
+    // (simulation/Simulation.hpp:108-138)
+    HostRGround h{};
+    h.p = p;
+    h.shape = shape;
+    h.radius = radius;
+    h.cp_rb_local[1] = -radius;
+    h.k_start = h.k_n = h.k_t = h.k_b = k_start;
+    h.mu_s = 0.5;
+    h.mu_k = 0.2;
+    return h;
+}
+}  // namespace
+
+int avbd_add_rod(avbd_ctx* c, uint32_t elements, double length, const double* curvature, double E, double nu,
+                 double density, double diameter, const double* pos, const double* rot, const double* vel,
+                 int base_fixed, int tip_fixed, int with_ground, uint32_t* first) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (elements == 0 || !(length > 0) || !(diameter > 0) || !pos)
+            throw std::runtime_error("rod needs elements >= 1, length > 0, diameter > 0 and a position");
+        const uint32_t nn = elements + 1;
+        check_capacity(c, c->mass.size() + nn, c->desc.particle_capacity);
+        check_capacity(c, c->cos.size() + elements, c->desc.energy_capacity);
+        if (with_ground) check_capacity(c, c->rground.size() + nn, c->desc.energy_capacity);
+        const double radius = diameter / 2, G = E / (2 * (1 + nu));
+        const double ds = length / (nn - 1);
+        const double kap[3] = {curvature ? curvature[0] : 0.0, curvature ? curvature[1] : 0.0,
+                               curvature ? curvature[2] : 0.0};
+        const double dR[3] = {kap[0] * ds, kap[1] * ds, kap[2] * ds};
+        double cp[3] = {pos[0], pos[1], pos[2]};
+        double cq[4] = {1, 0, 0, 0};
+        if (rot) std::memcpy(cq, rot, sizeof(cq));
+        const double zero3[3] = {0, 0, 0};
+        std::vector<uint32_t> nodes(nn);
+        for (uint32_t i = 0; i < nn; i++) {
+            nodes[i] = push_oriented(c, cp, cq, vel, nullptr, 0.0, zero3, 0);
+            double step[3];
+            const double ez[3] = {0, 0, ds};
+            qrot_d(cq, ez, step);
+            for (int k = 0; k < 3; k++) cp[k] += step[k];
+            double e[4], nq[4];
+            exp_s3_d(dR, e);
+            qmul_d(cq, e, nq);
+            std::memcpy(cq, nq, sizeof(cq));
+        }
+        const double pi = 3.14159265358979323846;
+        const double area = pi * radius * radius;
+        const double Ix = pi * radius * radius * radius * radius / 4.0, Iy = Ix, Iz = 2 * Ix;
+        const double elem_mass = ds * area * density;
+        const double elem_I[3] = {density * ds * Ix, density * ds * Iy, density * ds * Iz};
+        for (uint32_t e = 0; e < elements; e++)
+            for (uint32_t j = e; j <= e + 1; j++) {
+                c->mass[nodes[j]] += 0.5 * elem_mass;
+                for (int k = 0; k < 3; k++) c->inertia[3 * c->rot_of[nodes[j]] + k] += 0.5 * elem_I[k];
+            }
+        for (uint32_t e = 0; e < elements; e++) {
+            HostCos h{};
+            h.idx[0] = nodes[e];
+            h.idx[1] = nodes[e + 1];
+            const double st[6] = {G * area, G * area, E * area, E * Ix, E * Iy, G * Iz};
+            std::memcpy(h.stiffness, st, sizeof(st));
+            h.L = ds;
+            std::memcpy(h.kappa0, kap, sizeof(kap));
+            c->cos.push_back(h);
+        }
+        if (base_fixed) c->fixed[nodes.front()] = 1;
+        if (tip_fixed) c->fixed[nodes.back()] = 1;
+        if (with_ground) {
+            const double dt2 = c->desc.dt * c->desc.dt;
+            for (uint32_t nd : nodes)
+                c->rground.push_back(make_rground(nd, AVBD_SHAPE_ROD_SEGMENT, radius, c->mass[nd] / dt2));
+        }
+        if (first) *first = nodes.front();
+    });
+}
+
+int avbd_add_sphere(avbd_ctx* c, double radius, double density, const double* pos, const double* rot, int fixed,
+                    int with_ground, uint32_t* first) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (!(radius > 0) || !pos) throw std::runtime_error("sphere needs radius > 0 and a position");
+        check_capacity(c, c->mass.size() + 1, c->desc.particle_capacity);
+        if (with_ground) check_capacity(c, c->rground.size() + 1, c->desc.energy_capacity);
+        const double pi = 3.14159265358979323846;
+        const double m = 4 * pi / 3 * radius * radius * radius * density;
+        const double I = 0.4 * m * radius * radius;
+        const double inertia[3] = {I, I, I};
+        const uint32_t p = push_oriented(c, pos, rot, nullptr, nullptr, m, inertia, fixed ? 1 : 0);
+        if (with_ground)
+            c->rground.push_back(make_rground(p, AVBD_SHAPE_SPHERE, radius, m / (c->desc.dt * c->desc.dt)));
+        if (first) *first = p;
     });
 }
 
@@ -448,16 +763,7 @@ int avbd_add_tet_mesh(avbd_ctx* c, uint32_t nv, const double* rest_x, uint32_t n
         for (uint32_t t = 0; t < nt; t++)
             for (int k = 0; k < 4; k++)
                 if (tets[4 * t + k] >= nv) throw std::runtime_error("tet index out of range");
-        for (uint32_t i = 0; i < nv; i++) {
-            for (int k = 0; k < 3; k++) {
-                c->x.push_back(rest_x[3 * i + k]);
-                c->x_prev.push_back(rest_x[3 * i + k]);
-                c->v.push_back(0.0);
-                c->v_prev.push_back(0.0);
-            }
-            c->mass.push_back(0.0);
-            c->fixed.push_back(0);
-        }
+        for (uint32_t i = 0; i < nv; i++) push_particle(c, rest_x + 3 * i, nullptr, nullptr, nullptr, 0.0, 0);
         const double mu = E / (2 * (1 + nu));
         const double lambda = (E * nu) / ((1 + nu) * (1 - 2 * nu));
         for (uint32_t t = 0; t < nt; t++) {
@@ -532,6 +838,33 @@ int avbd_set_fixed(avbd_ctx* c, uint32_t first, uint32_t n, const uint8_t* fixed
     });
 }
 
+int avbd_set_rotation_state(avbd_ctx* c, uint32_t first, uint32_t n, const double* q, const double* w) {
+    return guarded(c, [&] {
+        if ((size_t)first + n > c->mass.size()) throw std::runtime_error("particle range out of bounds");
+        if (c->built) {
+            // pull the current device orientations so that NULL fields stay unchanged
+            const size_t np = c->mass.size();
+            std::vector<double> cq(4 * np), cw(3 * np);
+            if (avbd_download_rotations(c, cq.data(), cw.data()) != AVBD_OK) throw std::runtime_error(c->err);
+            for (size_t p = 0; p < np; p++) {
+                const int32_t r = c->rot_of[p];
+                if (r < 0) continue;
+                for (int k = 0; k < 4; k++) c->q[4 * r + k] = cq[4 * p + k];
+                for (int k = 0; k < 3; k++) c->w[3 * r + k] = cw[3 * p + k];
+            }
+        }
+        for (uint32_t i = 0; i < n; i++) {
+            const int32_t r = c->rot_of[first + i];
+            if (r < 0) continue;
+            if (q)
+                for (int k = 0; k < 4; k++) c->q[4 * r + k] = q[4 * i + k];
+            if (w)
+                for (int k = 0; k < 3; k++) c->w[3 * r + k] = w[3 * i + k];
+        }
+        if (c->built) upload_rotation_range(c, 0, c->d.n_rot);
+    });
+}
+
 int avbd_set_scenes(avbd_ctx* c, uint32_t n_scenes, const uint32_t* offsets) {
     return guarded(c, [&] {
         require_not_planned(c);
@@ -552,7 +885,17 @@ int avbd_build_graph(avbd_ctx* c) {
             std::vector<uint32_t> so = c->scene_offsets;
             if (so.empty()) so = {0u, n};
             if (so.back() != n) throw std::runtime_error("scene offsets do not cover the particle pool");
-            build_color_plan(n, c->nh, c->ground, c->fixed, so, c->plan);
+            HostModel m;
+            m.n = n;
+            m.nh = &c->nh;
+            m.ground = &c->ground;
+            m.cos = &c->cos;
+            m.rground = &c->rground;
+            m.joints = &c->joints;
+            m.fixed = &c->fixed;
+            m.rot_of = &c->rot_of;
+            m.scene_offsets = &so;
+            build_color_plan(m, c->plan);
             c->planned = true;
         }
         // device part
@@ -592,8 +935,51 @@ int avbd_build_graph(avbd_ctx* c) {
         d.inc = dev_alloc<uint32_t>(c, d.n_inc);
         d.items = dev_alloc<VItem>(c, d.n_items);
         c->d_conflicts = dev_alloc<uint32_t>(c, c->plan.conflict_list.size());
+
+        // oriented particles and their energies
+        d.n_rot = (uint32_t)(c->q.size() / 4);
+        d.n_cos = (uint32_t)c->cos.size();
+        d.n_rground = (uint32_t)c->rground.size();
+        d.n_joints = (uint32_t)c->joints.size();
+        d.n_oitems = (uint32_t)c->plan.oitems.size();
+        d.rot_of = dev_alloc<int32_t>(c, n);
+        d.q = dev_alloc<float4>(c, d.n_rot);
+        d.q_buf = dev_alloc<float4>(c, d.n_rot);
+        d.q_in = dev_alloc<float4>(c, d.n_rot);
+        d.q_prev = dev_alloc<float4>(c, d.n_rot);
+        d.q_last = dev_alloc<float4>(c, d.n_rot);
+        d.q_lastlast = dev_alloc<float4>(c, d.n_rot);
+        d.w = dev_alloc<float4>(c, d.n_rot);
+        d.w_prev = dev_alloc<float4>(c, d.n_rot);
+        d.inertia = dev_alloc<float4>(c, d.n_rot);
+        d.cos = dev_alloc<CosRec>(c, d.n_cos);
+        d.rground = dev_alloc<RGroundRec>(c, d.n_rground);
+        d.joints = dev_alloc<JointRec>(c, d.n_joints);
+        d.oitems = dev_alloc<OItem>(c, d.n_oitems);
+        d.oinc = dev_alloc<uint32_t>(c, c->plan.oinc.size());
+        d.prec_off = dev_alloc<uint32_t>(c, c->plan.prec_off.size());
+        d.prec = dev_alloc<uint32_t>(c, c->plan.prec.size());
+        {
+            std::vector<CosRec> cr(d.n_cos);
+            for (uint32_t i = 0; i < d.n_cos; i++) cr[i] = to_dev(c->cos[i]);
+            std::vector<RGroundRec> rr(d.n_rground);
+            for (uint32_t i = 0; i < d.n_rground; i++) rr[i] = to_dev(c->rground[i]);
+            std::vector<JointRec> jr(d.n_joints);
+            for (uint32_t i = 0; i < d.n_joints; i++) jr[i] = to_dev(c->joints[i]);
+            upload(c, d.cos, cr);
+            upload(c, d.rground, rr);
+            upload(c, d.joints, jr);
+            upload(c, d.rot_of, c->rot_of);
+            upload(c, d.oitems, c->plan.oitems);
+            upload(c, d.oinc, c->plan.oinc);
+            upload(c, d.prec_off, c->plan.prec_off);
+            upload(c, d.prec, c->plan.prec);
+            upload_rotation_range(c, 0, d.n_rot);
+        }
         // reduction slots: enough blocks for the stats passes at any launch config
-        c->reduce_blocks = std::max((d.n_items * 32 + 63) / 64, (d.n_tets + d.n_ground + 255) / 256) + 1;
+        c->reduce_blocks = std::max((d.n_items * 32 + 63) / 64 + (d.n_oitems + 255) / 256,
+                                    (d.n_tets + d.n_ground + d.n_cos + 255) / 256) +
+                           1;
         d.reduce = dev_alloc<double>(c, (size_t)c->reduce_blocks * 4);
 
         std::vector<TetRec> tr(d.n_tets);
@@ -668,6 +1054,8 @@ int avbd_build_graph(avbd_ctx* c) {
             double w = 1;
             for (uint32_t i = 0; i < c->desc.solver_iters; i++) om[i] = (float)(w = omega_at(i, c->desc.iter_acceleration, w));
             uint32_t* d_co = dev_alloc<uint32_t>(c, co.size());
+            uint32_t* d_oco = dev_alloc<uint32_t>(c, c->plan.oitem_offsets.size());
+            upload(c, d_oco, c->plan.oitem_offsets);
             uint32_t* d_cfo = dev_alloc<uint32_t>(c, c->plan.conflict_offsets.size());
             float* d_om = dev_alloc<float>(c, om.size());
             GridBarrier* bar = dev_alloc<GridBarrier>(c, 1);
@@ -679,6 +1067,7 @@ int avbd_build_graph(avbd_ctx* c) {
             c->sargs.color_offsets = d_co;
             c->sargs.conflict_offsets = d_cfo;
             c->sargs.conflicts = c->d_conflicts;
+            c->sargs.ocolor_offsets = d_oco;
             c->sargs.omega = d_om;
             c->sargs.bar = bar;
             c->sargs.iters = c->desc.solver_iters;
@@ -744,7 +1133,7 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
         }
         hip_check(hipEventRecord(c->ev_b, c->stream), "hipEventRecord");
         c->steps_taken += n_steps;
-        c->vertex_updates += (uint64_t)n_steps * c->desc.solver_iters * c->d.n_items;
+        c->vertex_updates += (uint64_t)n_steps * c->desc.solver_iters * (c->d.n_items + c->d.n_oitems);
     });
 }
 
@@ -817,6 +1206,39 @@ int avbd_download_inertial(avbd_ctx* c, double* y) {
     });
 }
 
+int avbd_download_rotations(avbd_ctx* c, double* q, double* w) {
+    return guarded(c, [&] {
+        require_built(c);
+        const uint32_t n = c->d.n, nr = c->d.n_rot;
+        std::vector<float4> bq(nr), bw(nr);
+        if (nr) {
+            hip_check(hipMemcpyAsync(bq.data(), c->d.q, nr * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hip_check(hipMemcpyAsync(bw.data(), c->d.w, nr * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
+        }
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        check_abort(c);
+        for (uint32_t p = 0; p < n; p++) {
+            const int32_t r = c->rot_of[p];
+            if (q) {
+                if (r < 0) {
+                    q[4 * p] = 1;
+                    q[4 * p + 1] = q[4 * p + 2] = q[4 * p + 3] = 0;
+                } else {
+                    q[4 * p] = bq[r].w;
+                    q[4 * p + 1] = bq[r].x;
+                    q[4 * p + 2] = bq[r].y;
+                    q[4 * p + 3] = bq[r].z;
+                }
+            }
+            if (w) {
+                w[3 * p] = r < 0 ? 0.0 : bw[r].x;
+                w[3 * p + 1] = r < 0 ? 0.0 : bw[r].y;
+                w[3 * p + 2] = r < 0 ? 0.0 : bw[r].z;
+            }
+        }
+    });
+}
+
 int avbd_download_energies(avbd_ctx* c, int type, void* records, uint32_t count) {
     return guarded(c, [&] {
         require_built(c);
@@ -866,6 +1288,73 @@ int avbd_download_energies(avbd_ctx* c, int type, void* records, uint32_t count)
                 r[i].mu_k = g.mu_k;
                 r[i].cp_x = g.cp_x;
                 r[i].cp_z = g.cp_z;
+            }
+        } else if (type == AVBD_COSSERAT_ROD) {
+            if (count > c->d.n_cos) throw std::runtime_error("count exceeds the pool");
+            auto* r = static_cast<avbd_cosserat_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                const HostCos& h = c->cos[i];
+                r[i].particle_indices[0] = h.idx[0];
+                r[i].particle_indices[1] = h.idx[1];
+                std::memcpy(r[i].stiffness, h.stiffness, sizeof(h.stiffness));
+                r[i].rest_length = h.L;
+                std::memcpy(r[i].rest_curvature, h.kappa0, sizeof(h.kappa0));
+            }
+        } else if (type == AVBD_RIGID_BODY_GROUND_COLLISION) {
+            if (count > c->d.n_rground) throw std::runtime_error("count exceeds the pool");
+            std::vector<RGroundRec> gr(c->d.n_rground);
+            if (!gr.empty())
+                hip_check(hipMemcpyAsync(gr.data(), c->d.rground, gr.size() * sizeof(RGroundRec),
+                                         hipMemcpyDeviceToHost, c->stream),
+                          "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            auto* r = static_cast<avbd_rigid_ground_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                const RGroundRec& g = gr[i];
+                r[i].particle_index = g.p;
+                r[i].shape = g.shape;
+                r[i].use_static = (g.flags & GF_USE_STATIC) ? 1 : 0;
+                r[i].unclamped_lambda_tb = (g.flags & GF_UNCLAMPED) ? 1 : 0;
+                r[i].radius = g.radius;
+                r[i].cp_rb_local[0] = g.cpl_x;
+                r[i].cp_rb_local[1] = g.cpl_y;
+                r[i].cp_rb_local[2] = g.cpl_z;
+                r[i].cp_ground[0] = g.cpg_x;
+                r[i].cp_ground[1] = g.cpg_y;
+                r[i].cp_ground[2] = g.cpg_z;
+                r[i].k_start = g.k_start;
+                r[i].k_n = g.k_n;
+                r[i].k_t = g.k_t;
+                r[i].k_b = g.k_b;
+                r[i].lambda_n = g.lambda_n;
+                r[i].lambda_t = g.lambda_t;
+                r[i].lambda_b = g.lambda_b;
+                r[i].C_n_prev = g.C_n_prev;
+                r[i].mu_s = g.mu_s;
+                r[i].mu_k = g.mu_k;
+            }
+        } else if (type == AVBD_ONE_SIDED_FIXED_JOINT) {
+            if (count > c->d.n_joints) throw std::runtime_error("count exceeds the pool");
+            std::vector<JointRec> jr(c->d.n_joints);
+            if (!jr.empty())
+                hip_check(hipMemcpyAsync(jr.data(), c->d.joints, jr.size() * sizeof(JointRec), hipMemcpyDeviceToHost,
+                                         c->stream),
+                          "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            auto* r = static_cast<avbd_joint_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                const HostJoint& h = c->joints[i];
+                r[i].particle_index = h.p;
+                r[i].pad_ = 0;
+                std::memcpy(r[i].position_offset, h.pos_offset, sizeof(h.pos_offset));
+                std::memcpy(r[i].rotation_offset, h.rot_offset, sizeof(h.rot_offset));
+                std::memcpy(r[i].ref_position, h.ref_position, sizeof(h.ref_position));
+                std::memcpy(r[i].ref_rotation, h.ref_rotation, sizeof(h.ref_rotation));
+                for (int k = 0; k < 6; k++) {
+                    r[i].k[k] = jr[i].k[k];
+                    r[i].lambda[k] = jr[i].lambda[k];
+                    r[i].C_prev[k] = jr[i].C_prev[k];
+                }
             }
         } else {
             throw CodeError(AVBD_E_ARG, "energy type not supported by this build");
@@ -924,6 +1413,9 @@ uint32_t avbd_num_energies(avbd_ctx* c, int type) {
     if (!c) return 0;
     if (type == AVBD_NEO_HOOKEAN) return (uint32_t)c->nh.size();
     if (type == AVBD_GROUND_COLLISION) return (uint32_t)c->ground.size();
+    if (type == AVBD_COSSERAT_ROD) return (uint32_t)c->cos.size();
+    if (type == AVBD_RIGID_BODY_GROUND_COLLISION) return (uint32_t)c->rground.size();
+    if (type == AVBD_ONE_SIDED_FIXED_JOINT) return (uint32_t)c->joints.size();
     return 0;
 }
 

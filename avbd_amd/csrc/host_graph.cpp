@@ -63,16 +63,20 @@ struct SceneGraph {
 
 // Vertex-vertex adjacency of one scene (StaticParticleAdjacency::buildAdjacency,
 // common/StaticParticleAdjacency.hpp:42-133): deduplicated neighbour sets of the
-// static multi-particle energies.
-void scene_adjacency(uint32_t lo, uint32_t hi, const std::vector<HostNH>& nh,
-                     const std::vector<uint32_t>& tets_of_scene, SceneGraph& g) {
+// static multi-particle energies (Neo-Hookean tets and Cosserat elements).
+void scene_adjacency(uint32_t lo, uint32_t hi, const std::vector<HostNH>& nh, const std::vector<uint32_t>& tets,
+                     const std::vector<HostCos>& cos, const std::vector<uint32_t>& rods, SceneGraph& g) {
     const uint32_t n = hi - lo;
     std::vector<std::vector<uint32_t>> sets(n);
-    for (uint32_t t : tets_of_scene) {
+    for (uint32_t t : tets) {
         const HostNH& r = nh[t];
         for (int k = 0; k < 4; k++)
             for (int k2 = 0; k2 < 4; k2++)
                 if (k != k2) sets[r.idx[k] - lo].push_back(r.idx[k2] - lo);
+    }
+    for (uint32_t e : rods) {
+        sets[cos[e].idx[0] - lo].push_back(cos[e].idx[1] - lo);
+        sets[cos[e].idx[1] - lo].push_back(cos[e].idx[0] - lo);
     }
     g.off.assign(n + 1, 0);
     for (uint32_t v = 0; v < n; v++) {
@@ -85,26 +89,62 @@ void scene_adjacency(uint32_t lo, uint32_t hi, const std::vector<HostNH>& nh,
     for (uint32_t v = 0; v < n; v++) std::copy(sets[v].begin(), sets[v].end(), g.nb.begin() + g.off[v]);
 }
 
+// CSR of (particle -> entries) from (particle, entry) pairs in pool order.
+void csr_by_particle(uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>>& pairs, std::vector<uint32_t>& off,
+                     std::vector<uint32_t>& ent) {
+    off.assign(n + 1, 0);
+    for (const auto& pr : pairs) off[pr.first + 1]++;
+    for (uint32_t v = 0; v < n; v++) off[v + 1] += off[v];
+    ent.resize(pairs.size());
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (const auto& pr : pairs) ent[cur[pr.first]++] = pr.second;
+}
+
 }  // namespace
 
-void build_color_plan(uint32_t n, const std::vector<HostNH>& nh, const std::vector<HostGround>& ground,
-                      const std::vector<uint8_t>& fixed, const std::vector<uint32_t>& scene_offsets,
-                      ColorPlan& plan) {
+void build_color_plan(const HostModel& m, ColorPlan& plan) {
+    const uint32_t n = m.n;
+    const auto& nh = *m.nh;
+    const auto& ground = *m.ground;
+    const auto& cos = *m.cos;
+    const auto& rground = *m.rground;
+    const auto& joints = *m.joints;
+    const auto& fixed = *m.fixed;
+    const auto& rot_of = *m.rot_of;
+    const auto& scene_offsets = *m.scene_offsets;
     const uint32_t n_scenes = (uint32_t)scene_offsets.size() - 1;
     std::vector<uint32_t> scene_of(n, 0);
     for (uint32_t s = 0; s < n_scenes; s++)
         for (uint32_t v = scene_offsets[s]; v < scene_offsets[s + 1]; v++) scene_of[v] = s;
+    auto oriented = [&](uint32_t p) { return rot_of[p] >= 0; };
 
-    // tets per scene; every tet must lie inside one scene
-    std::vector<std::vector<uint32_t>> scene_tets(n_scenes);
+    // energies per scene; every multi-particle energy must lie inside one scene.
+    // Neo-Hookean supports positional particles only and the 6-DOF energies
+    // oriented ones only (SupportsPositional / SupportsOriented of the solvers).
+    std::vector<std::vector<uint32_t>> scene_tets(n_scenes), scene_rods(n_scenes);
     for (uint32_t t = 0; t < nh.size(); t++) {
-        const uint32_t s = scene_of[nh[t].idx[0]];
         for (int k = 0; k < 4; k++) {
             if (nh[t].idx[k] >= n) throw std::runtime_error("tet references a particle out of range");
-            if (scene_of[nh[t].idx[k]] != s) throw std::runtime_error("tet spans two scenes");
+            if (oriented(nh[t].idx[k])) throw std::runtime_error("Neo-Hookean energy on an oriented particle");
         }
+        const uint32_t s = scene_of[nh[t].idx[0]];
+        for (int k = 0; k < 4; k++)
+            if (scene_of[nh[t].idx[k]] != s) throw std::runtime_error("tet spans two scenes");
         scene_tets[s].push_back(t);
     }
+    for (uint32_t e = 0; e < cos.size(); e++) {
+        for (int k = 0; k < 2; k++) {
+            if (cos[e].idx[k] >= n) throw std::runtime_error("Cosserat element references a particle out of range");
+            if (!oriented(cos[e].idx[k])) throw std::runtime_error("Cosserat element on a positional particle");
+        }
+        if (cos[e].idx[0] == cos[e].idx[1]) throw std::runtime_error("Cosserat element with one particle twice");
+        if (scene_of[cos[e].idx[0]] != scene_of[cos[e].idx[1]]) throw std::runtime_error("rod spans two scenes");
+        scene_rods[scene_of[cos[e].idx[0]]].push_back(e);
+    }
+    for (const auto& r : rground)
+        if (r.p >= n || !oriented(r.p)) throw std::runtime_error("rigid-ground contact needs an oriented particle");
+    for (const auto& j : joints)
+        if (j.p >= n || !oriented(j.p)) throw std::runtime_error("joint needs an oriented particle");
 
     // colour every scene on its own (threads over scenes)
     plan.color.assign(n, 0);
@@ -118,7 +158,7 @@ void build_color_plan(uint32_t n, const std::vector<HostNH>& nh, const std::vect
                 try {
                     SceneGraph g;
                     const uint32_t lo = scene_offsets[s], hi = scene_offsets[s + 1];
-                    scene_adjacency(lo, hi, nh, scene_tets[s], g);
+                    scene_adjacency(lo, hi, nh, scene_tets[s], cos, scene_rods[s], g);
                     scene_colors[s] = dsatur_csr(hi - lo, g.off.data(), g.nb.data(), plan.color.data() + lo);
                 } catch (const std::exception& e) {
                     errors[s] = e.what();
@@ -132,16 +172,25 @@ void build_color_plan(uint32_t n, const std::vector<HostNH>& nh, const std::vect
     plan.num_colors = 0;
     for (uint32_t s = 0; s < n_scenes; s++) plan.num_colors = std::max(plan.num_colors, scene_colors[s]);
 
-    // per-vertex NH incidences in pool order (ascending tet index, then local)
-    std::vector<uint32_t> inc_cnt(n + 1, 0);
-    for (const auto& t : nh)
-        for (int k = 0; k < 4; k++) inc_cnt[t.idx[k]]++;
-    std::vector<uint32_t> inc_off(n + 1, 0);
-    for (uint32_t v = 0; v < n; v++) inc_off[v + 1] = inc_off[v] + inc_cnt[v];
-    std::vector<uint32_t> inc_all(inc_off[n]);
-    std::vector<uint32_t> cur(inc_off.begin(), inc_off.end() - 1);
+    // per-vertex incidences in pool order (ascending energy index, then local)
+    std::vector<std::pair<uint32_t, uint32_t>> pr;
     for (uint32_t t = 0; t < nh.size(); t++)
-        for (uint32_t k = 0; k < 4; k++) inc_all[cur[nh[t].idx[k]]++] = (t << 2) | k;
+        for (uint32_t k = 0; k < 4; k++) pr.emplace_back(nh[t].idx[k], (t << 2) | k);
+    std::vector<uint32_t> inc_off, inc_all;
+    csr_by_particle(n, pr, inc_off, inc_all);
+    pr.clear();
+    for (uint32_t e = 0; e < cos.size(); e++)
+        for (uint32_t k = 0; k < 2; k++) pr.emplace_back(cos[e].idx[k], (e << 1) | k);
+    std::vector<uint32_t> cos_off, cos_all;
+    csr_by_particle(n, pr, cos_off, cos_all);
+    pr.clear();
+    for (uint32_t e = 0; e < rground.size(); e++) pr.emplace_back(rground[e].p, e);
+    std::vector<uint32_t> rg_off, rg_all;
+    csr_by_particle(n, pr, rg_off, rg_all);
+    pr.clear();
+    for (uint32_t e = 0; e < joints.size(); e++) pr.emplace_back(joints[e].p, e);
+    std::vector<uint32_t> jt_off, jt_all;
+    csr_by_particle(n, pr, jt_off, jt_all);
 
     std::vector<int32_t> ground_of(n, -1);
     for (uint32_t gi = 0; gi < ground.size(); gi++) {
@@ -151,26 +200,65 @@ void build_color_plan(uint32_t n, const std::vector<HostNH>& nh, const std::vect
         ground_of[p] = (int32_t)gi;
     }
 
+    // per rotation slot: its rigid-ground then joint records (start-of-step
+    // and dual updates run per particle, fixed ones included)
+    uint32_t n_rot = 0;
+    for (uint32_t v = 0; v < n; v++)
+        if (oriented(v)) n_rot = std::max(n_rot, (uint32_t)rot_of[v] + 1);
+    std::vector<int32_t> particle_of_rot(n_rot, -1);
+    for (uint32_t v = 0; v < n; v++)
+        if (oriented(v)) particle_of_rot[rot_of[v]] = (int32_t)v;
+    plan.prec_off.assign(n_rot + 1, 0);
+    plan.prec.clear();
+    for (uint32_t r = 0; r < n_rot; r++) {
+        plan.prec_off[r] = (uint32_t)plan.prec.size();
+        const int32_t v = particle_of_rot[r];
+        if (v < 0) continue;
+        for (uint32_t k = rg_off[v]; k < rg_off[v + 1]; k++) plan.prec.push_back(rg_all[k]);
+        for (uint32_t k = jt_off[v]; k < jt_off[v + 1]; k++) plan.prec.push_back(jt_all[k] | PREC_JOINT);
+    }
+    plan.prec_off[n_rot] = (uint32_t)plan.prec.size();
+
     // work items: colour-major, then scene order, then ascending vertex index
     // (ColorList::rebuildWorkList order, common/ColorList.hpp:345-363); fixed
     // particles are dropped (they return immediately, VBDSolver.hpp:288).
     plan.items.clear();
     plan.inc.clear();
+    plan.oitems.clear();
+    plan.oinc.clear();
     plan.item_offsets.assign(plan.num_colors + 1, 0);
+    plan.oitem_offsets.assign(plan.num_colors + 1, 0);
     for (uint32_t c = 0; c < plan.num_colors; c++) {
         plan.item_offsets[c] = (uint32_t)plan.items.size();
+        plan.oitem_offsets[c] = (uint32_t)plan.oitems.size();
         for (uint32_t v = 0; v < n; v++) {
             if (plan.color[v] != c || fixed[v]) continue;
-            VItem it;
-            it.v = v;
-            it.inc_begin = (uint32_t)plan.inc.size();
-            it.inc_count = inc_cnt[v];
-            it.ground = ground_of[v];
-            plan.inc.insert(plan.inc.end(), inc_all.begin() + inc_off[v], inc_all.begin() + inc_off[v + 1]);
-            plan.items.push_back(it);
+            if (!oriented(v)) {
+                VItem it;
+                it.v = v;
+                it.inc_begin = (uint32_t)plan.inc.size();
+                it.inc_count = inc_off[v + 1] - inc_off[v];
+                it.ground = ground_of[v];
+                plan.inc.insert(plan.inc.end(), inc_all.begin() + inc_off[v], inc_all.begin() + inc_off[v + 1]);
+                plan.items.push_back(it);
+            } else {
+                OItem it{};
+                it.v = v;
+                it.rot = (uint32_t)rot_of[v];
+                it.inc_begin = (uint32_t)plan.oinc.size();
+                it.n_cos = cos_off[v + 1] - cos_off[v];
+                it.n_rg = rg_off[v + 1] - rg_off[v];
+                it.n_joint = jt_off[v + 1] - jt_off[v];
+                it.ground = ground_of[v];
+                plan.oinc.insert(plan.oinc.end(), cos_all.begin() + cos_off[v], cos_all.begin() + cos_off[v + 1]);
+                plan.oinc.insert(plan.oinc.end(), rg_all.begin() + rg_off[v], rg_all.begin() + rg_off[v + 1]);
+                plan.oinc.insert(plan.oinc.end(), jt_all.begin() + jt_off[v], jt_all.begin() + jt_off[v + 1]);
+                plan.oitems.push_back(it);
+            }
         }
     }
     plan.item_offsets[plan.num_colors] = (uint32_t)plan.items.size();
+    plan.oitem_offsets[plan.num_colors] = (uint32_t)plan.oitems.size();
     plan.conflict_offsets.assign(plan.num_colors + 1, 0);
     plan.conflict_list.clear();
 }

@@ -13,6 +13,11 @@
 
 namespace avbd {
 
+// AL constants CONSTRAINT_ALPHA, STIFFNESS_BETA, STIFFNESS_GAMMA (common/common.hpp:13-15)
+constexpr float kAlpha = 0.95f;
+constexpr float kBeta = 10.0f;
+constexpr float kGamma = 0.99f;
+
 // ---------------------------------------------------------------------------
 // Device records
 // ---------------------------------------------------------------------------
@@ -62,6 +67,57 @@ struct alignas(16) VItem {
 };
 constexpr uint32_t VITEM_CONFLICT = 0x80000000u;
 
+// Cosserat rod element (CosseratRodEnergyInfo, energy/elastic/CosseratRodEnergyPool.hpp:8-14):
+// nodes, rest length, stiffness diag(GA, GA, EA, EI, EI, GJ), rest curvature.
+struct alignas(16) CosRec {
+    uint32_t i0, i1;
+    float L, k0;
+    float k1, k2, k3, k4;
+    float k5, kx, ky, kz;
+};
+static_assert(sizeof(CosRec) == 48, "CosRec must be 48 bytes");
+
+// Rigid-body ground contact (RigidBodyGroundCollisionEnergyInfo,
+// energy/collision/RigidBodyGroundCollisionEnergyPool.hpp:8-15).  Normal and
+// tangents are the world axes y, x, z.  shape: the reference's
+// CollisionGeometryType (collision/CollisionGeometryType.hpp).
+struct alignas(16) RGroundRec {
+    float k_start, k_n, k_t, k_b;
+    float lambda_n, lambda_t, lambda_b, C_n_prev;
+    float mu_s, mu_k, radius, cpg_x;
+    float cpg_y, cpg_z, cpl_x, cpl_y;
+    float cpl_z;
+    uint32_t p, shape, flags;   // flags: GF_USE_STATIC | GF_UNCLAMPED
+};
+static_assert(sizeof(RGroundRec) == 80, "RGroundRec must be 80 bytes");
+constexpr uint32_t SHAPE_ROD_SEGMENT = 1u;
+constexpr uint32_t SHAPE_SPHERE = 2u;
+
+// One-sided fixed joint (OneSidedFixedJointEnergyInfo,
+// energy/joint/OneSidedFixedJointEnergyPool.hpp:8-16) with its vector AL state.
+struct alignas(16) JointRec {
+    float4 pos_off;   // xyz
+    float4 rot_off;   // quaternion (x, y, z, w)
+    float4 ref_pos;   // xyz
+    float4 ref_rot;   // quaternion (x, y, z, w)
+    float k[6], lambda[6], C_prev[6];
+    uint32_t p, pad;
+};
+static_assert(sizeof(JointRec) == 144, "JointRec must be 144 bytes");
+constexpr float kJointKStart = 1e4f;   // OneSidedFixedJointEnergyPool k_start
+
+// Work item of an oriented particle: vertex, rotation slot, first entry in
+// the oriented incidence list, counts of Cosserat incidences ((e << 1) | local,
+// bit 31 = conflicted), rigid-ground records and joints that follow it, and
+// the positional ground record (or -1).
+struct alignas(16) OItem {
+    uint32_t v, rot, inc_begin, n_cos;
+    uint32_t n_rg, n_joint;
+    int32_t ground;
+    uint32_t pad;
+};
+constexpr uint32_t PREC_JOINT = 0x80000000u;   // per-rotation record list: joint flag
+
 // ---------------------------------------------------------------------------
 // Kernel launch wrappers (defined in vbd_kernels.hip)
 // ---------------------------------------------------------------------------
@@ -84,6 +140,26 @@ struct DevState {
     VItem* items = nullptr;        // all colours back to back
     double* reduce = nullptr;      // residual / IP partial sums
     uint32_t n = 0, n_tets = 0, n_ground = 0, n_inc = 0, n_items = 0;
+
+    // oriented particles (RotationPool through rotation_idx, common/ParticlePool.hpp:7-69)
+    int32_t* rot_of = nullptr;     // rotation slot per particle or -1
+    float4* q = nullptr;           // orientation (x, y, z, w)
+    float4* q_buf = nullptr;
+    float4* q_in = nullptr;        // inertial target orientation
+    float4* q_prev = nullptr;
+    float4* q_last = nullptr;
+    float4* q_lastlast = nullptr;
+    float4* w = nullptr;           // angular velocity (body frame)
+    float4* w_prev = nullptr;
+    float4* inertia = nullptr;     // principal rotational inertia
+    CosRec* cos = nullptr;
+    RGroundRec* rground = nullptr;
+    JointRec* joints = nullptr;
+    OItem* oitems = nullptr;
+    uint32_t* oinc = nullptr;
+    uint32_t* prec_off = nullptr;  // per rotation slot: rigid-ground / joint records
+    uint32_t* prec = nullptr;
+    uint32_t n_rot = 0, n_cos = 0, n_rground = 0, n_joints = 0, n_oitems = 0;
 };
 
 struct LaunchCfg {
@@ -104,6 +180,7 @@ struct StepArgs {
     const uint32_t* color_offsets = nullptr;      // num_colors + 1
     const uint32_t* conflict_offsets = nullptr;   // num_colors + 1
     const uint32_t* conflicts = nullptr;
+    const uint32_t* ocolor_offsets = nullptr;     // oriented items per colour, num_colors + 1
     const float* omega = nullptr;                 // Chebyshev omega per iteration
     GridBarrier* bar = nullptr;
     uint32_t iters = 0, num_colors = 0;
@@ -116,6 +193,7 @@ uint32_t persistent_grid(int device, uint32_t lanes, bool damping);
 void launch_step_persistent(hipStream_t s, const StepArgs& a, uint32_t lanes, bool damping, uint32_t grid);
 
 void launch_eprev(hipStream_t s, const DevState& d, float dt);
+void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t item_begin, uint32_t item_count, float dt);
 void launch_step_begin(hipStream_t s, const DevState& d, float dt, float g);
 void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t item_begin,
                         uint32_t item_count, bool damping, float dt, float4* x_out);
@@ -143,21 +221,57 @@ struct HostGround {
     bool use_static, unclamped;
 };
 
+struct HostCos {
+    uint32_t idx[2];
+    double stiffness[6];
+    double L;
+    double kappa0[3];
+};
+struct HostRGround {
+    uint32_t p, shape;
+    double radius;
+    double cp_rb_local[3], cp_ground[3];
+    double k_start, k_n, k_t, k_b, lambda_n, lambda_t, lambda_b, C_n_prev, mu_s, mu_k;
+    bool use_static, unclamped;
+};
+struct HostJoint {
+    uint32_t p;
+    double pos_offset[3], rot_offset[4];   // quaternions (w, x, y, z)
+    double ref_position[3], ref_rotation[4];
+    double k[6], lambda[6], C_prev[6];
+};
+
+// Everything the colour plan is built from.
+struct HostModel {
+    uint32_t n = 0;
+    const std::vector<HostNH>* nh = nullptr;
+    const std::vector<HostGround>* ground = nullptr;
+    const std::vector<HostCos>* cos = nullptr;
+    const std::vector<HostRGround>* rground = nullptr;
+    const std::vector<HostJoint>* joints = nullptr;
+    const std::vector<uint8_t>* fixed = nullptr;
+    const std::vector<int32_t>* rot_of = nullptr;
+    const std::vector<uint32_t>* scene_offsets = nullptr;
+};
+
 struct ColorPlan {
     uint32_t num_colors = 0;
     std::vector<uint32_t> color;              // per particle
     std::vector<uint32_t> item_offsets;       // per colour: [begin, end) in items
     std::vector<VItem> items;
     std::vector<uint32_t> inc;
+    std::vector<uint32_t> oitem_offsets;      // per colour: [begin, end) in oitems
+    std::vector<OItem> oitems;
+    std::vector<uint32_t> oinc;
+    std::vector<uint32_t> prec_off;           // per rotation slot (n_rot + 1)
+    std::vector<uint32_t> prec;
     std::vector<uint32_t> conflict_offsets;   // per colour
     std::vector<uint32_t> conflict_list;
 };
 
 // Build adjacency and colour every scene with DSATUR; then lay out the
 // per-colour work items.  Throws std::runtime_error on failure.
-void build_color_plan(uint32_t n, const std::vector<HostNH>& nh, const std::vector<HostGround>& ground,
-                      const std::vector<uint8_t>& fixed, const std::vector<uint32_t>& scene_offsets,
-                      ColorPlan& plan);
+void build_color_plan(const HostModel& m, ColorPlan& plan);
 
 // DSATUR on one CSR graph (ColorList::greedyColor2, common/ColorList.hpp:149-241).
 uint32_t dsatur_csr(uint32_t n, const uint32_t* p_offsets, const uint32_t* p_neighbors, uint32_t* color);

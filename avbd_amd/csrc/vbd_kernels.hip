@@ -16,15 +16,14 @@
 // gradient and 6 (symmetric) Hessian partial sums are folded with
 // cross-lane xor shuffles; the group leader adds the ground AL terms,
 // solves the 3x3 system by LDL^T in registers and writes the vertex.
+// Oriented (6-DOF) vertices of a colour are solved in the same phase, one
+// thread each (rigid_device.hpp).
 #include "internal.hpp"
+#include "rigid_device.hpp"
 
 namespace avbd {
 
 namespace {
-
-constexpr float kAlpha = 0.95f;   // CONSTRAINT_ALPHA, common/common.hpp:15
-constexpr float kBeta = 10.0f;    // STIFFNESS_BETA,   common/common.hpp:13
-constexpr float kGamma = 0.99f;   // STIFFNESS_GAMMA,  common/common.hpp:14
 
 struct Sym3 {
     float a00, a11, a22, a01, a02, a12;
@@ -262,18 +261,14 @@ __global__ void eprev_kernel(const TetRec* __restrict__ tets, TetDamp* __restric
     if (t < n_tets) eprev_one(t, tets, damp, x_prev);
 }
 
-// Start of step for one particle: ground contact point + AL warm start, then
-// the inertial target and adaptive initial guess.
-__device__ __forceinline__ void step_begin_one(uint32_t p, float4* __restrict__ x, float4* __restrict__ y,
-                                               float4* __restrict__ x_last, float4* __restrict__ x_lastlast,
-                                               const float4* __restrict__ v, const float4* __restrict__ v_prev,
-                                               const uint8_t* __restrict__ fixed,
-                                               const int32_t* __restrict__ ground_of,
-                                               GroundRec* __restrict__ ground, float dt, float g) {
-    float4 xp = x[p];
-    const int32_t gi = ground_of[p];
+// Start of step for one particle: ground contact point + AL warm start, the
+// rigid-ground / joint records of an oriented particle, then the inertial
+// target and adaptive initial guess (and the rotation inertial update).
+__device__ __forceinline__ void step_begin_one(const DevState& d, uint32_t p, float dt, float g) {
+    float4 xp = d.x[p];
+    const int32_t gi = d.ground_of[p];
     if (gi >= 0) {
-        GroundRec c = ground[gi];
+        GroundRec c = d.ground[gi];
         // updateContactPoints (GroundCollisionEnergySolver.hpp:81-91)
         c.cp_x = xp.x;
         c.cp_z = xp.z;
@@ -286,11 +281,13 @@ __device__ __forceinline__ void step_begin_one(uint32_t p, float4* __restrict__ 
         c.k_n = fmaxf(kGamma * c.k_n, c.k_start);
         c.k_t = fmaxf(kGamma * c.k_t, c.k_start);
         c.k_b = fmaxf(kGamma * c.k_b, c.k_start);
-        ground[gi] = c;
+        d.ground[gi] = c;
     }
-    if (fixed[p]) return;
+    const int32_t r = d.n_rot ? d.rot_of[p] : -1;
+    if (r >= 0) rigid::records_step_begin(d, p, (uint32_t)r);
+    if (d.fixed[p]) return;
     // _particleInertialUpdate (VBDSolver.hpp:208-248): gravity (0,-g,0)
-    const float4 vv = v[p], vp = v_prev[p];
+    const float4 vv = d.v[p], vp = d.v_prev[p];
     const float ay = (vv.y - vp.y) / dt;
     float a_tilde = 0.f;
     if (g > 1e-12f) {
@@ -300,96 +297,106 @@ __device__ __forceinline__ void step_begin_one(uint32_t p, float4* __restrict__ 
         else a_tilde = along / g;
     }
     const float dt2 = dt * dt;
-    y[p] = make_float4(xp.x + dt * vv.x, xp.y + dt * vv.y + dt2 * -g, xp.z + dt * vv.z, 0.f);
+    d.y[p] = make_float4(xp.x + dt * vv.x, xp.y + dt * vv.y + dt2 * -g, xp.z + dt * vv.z, 0.f);
     xp.x = xp.x + dt * vv.x;
     xp.y = xp.y + (dt * vv.y + dt2 * (a_tilde * -g));
     xp.z = xp.z + dt * vv.z;
-    x[p] = xp;
-    x_last[p] = xp;
-    x_lastlast[p] = xp;
+    d.x[p] = xp;
+    d.x_last[p] = xp;
+    d.x_lastlast[p] = xp;
+    if (r >= 0) rigid::rotation_inertial(d, (uint32_t)r, dt);
 }
 
-__global__ void step_begin_kernel(float4* __restrict__ x, float4* __restrict__ y, float4* __restrict__ x_last,
-                                  float4* __restrict__ x_lastlast, const float4* __restrict__ v,
-                                  const float4* __restrict__ v_prev, const uint8_t* __restrict__ fixed,
-                                  const int32_t* __restrict__ ground_of, GroundRec* __restrict__ ground,
-                                  uint32_t n, float dt, float g) {
+__global__ void step_begin_kernel(DevState d, float dt, float g) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) step_begin_one(p, x, y, x_last, x_lastlast, v, v_prev, fixed, ground_of, ground, dt, g);
+    if (p < d.n) step_begin_one(d, p, dt, g);
 }
 
-__global__ void commit_kernel(float4* __restrict__ x, const float4* __restrict__ x_buf,
-                              const uint32_t* __restrict__ list, uint32_t count) {
+// _commitConflictedInColor (VBDSolver.hpp:108-119) for one buffered particle.
+__device__ __forceinline__ void commit_one(const DevState& d, uint32_t p) {
+    d.x[p] = d.x_buf[p];
+    const int32_t r = d.n_rot ? d.rot_of[p] : -1;
+    if (r >= 0) d.q[r] = d.q_buf[r];
+}
+
+__global__ void commit_kernel(DevState d, const uint32_t* __restrict__ list, uint32_t count) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    const uint32_t p = list[k];
-    x[p] = x_buf[p];
+    if (k < count) commit_one(d, list[k]);
 }
 
-// Chebyshev step and ground dual / stiffness update for one particle.
-__device__ __forceinline__ void cheb_dual_one(uint32_t p, float4* __restrict__ x, float4* __restrict__ x_last,
-                                              float4* __restrict__ x_lastlast, const int32_t* __restrict__ ground_of,
-                                              GroundRec* __restrict__ ground, float omega) {
-    float4 xp = x[p];
-    const float4 xl = x_last[p], xll = x_lastlast[p];
+// Chebyshev step (positions and orientations) and the dual / stiffness update
+// of the particle's contact and joint records.
+__device__ __forceinline__ void cheb_dual_one(const DevState& d, uint32_t p, float omega) {
+    float4 xp = d.x[p];
+    const float4 xl = d.x_last[p], xll = d.x_lastlast[p];
     // _particleChebyshevAcceleration (VBDSolver.hpp:513-536)
     xp.x = omega * (xp.x - xll.x) + xll.x;
     xp.y = omega * (xp.y - xll.y) + xll.y;
     xp.z = omega * (xp.z - xll.z) + xll.z;
-    x[p] = xp;
-    x_lastlast[p] = xl;
-    x_last[p] = xp;
-    const int32_t gi = ground_of[p];
-    if (gi < 0) return;
-    // CollisionConstraintEnergySolver::updateAfterIteration (CollisionConstraintEnergySolver.hpp:60-134)
-    GroundRec c = ground[gi];
-    const float Cn = -xp.y, Ct = xp.x - c.cp_x, Cb = xp.z - c.cp_z;
-    const float Ccorr = Cn - kAlpha * c.C_n_prev;
-    const float lnp = c.k_n * Ccorr + c.lambda_n;
-    const float ltp = c.k_t * Ct + c.lambda_t;
-    const float lbp = c.k_b * Cb + c.lambda_b;
-    if (lnp > 0.f) c.k_n += kBeta * fabsf(Ccorr);
-    c.lambda_n = fmaxf(0.f, lnp);
-    const float mag = sqrtf(ltp * ltp + lbp * lbp);
-    float lmax = ((c.flags & GF_USE_STATIC) ? c.mu_s : c.mu_k) * c.lambda_n;
-    if (mag < lmax) {
-        c.k_t += kBeta * fabsf(Ct);
-        c.k_b += kBeta * fabsf(Cb);
-        c.lambda_t = ltp;
-        c.lambda_b = lbp;
-        c.flags |= GF_UNCLAMPED;
-    } else {
-        if (c.flags & GF_USE_STATIC) {
-            c.flags &= ~GF_USE_STATIC;
-            lmax = c.mu_k * lnp;
+    d.x[p] = xp;
+    d.x_lastlast[p] = xl;
+    d.x_last[p] = xp;
+    const int32_t r = d.n_rot ? d.rot_of[p] : -1;
+    float4 q = make_float4(0.f, 0.f, 0.f, 1.f);
+    if (r >= 0) q = rigid::rotation_chebyshev(d, (uint32_t)r, omega);
+    const int32_t gi = d.ground_of[p];
+    if (gi >= 0) {
+        // CollisionConstraintEnergySolver::updateAfterIteration (CollisionConstraintEnergySolver.hpp:60-134)
+        GroundRec c = d.ground[gi];
+        const float Cn = -xp.y, Ct = xp.x - c.cp_x, Cb = xp.z - c.cp_z;
+        const float Ccorr = Cn - kAlpha * c.C_n_prev;
+        const float lnp = c.k_n * Ccorr + c.lambda_n;
+        const float ltp = c.k_t * Ct + c.lambda_t;
+        const float lbp = c.k_b * Cb + c.lambda_b;
+        if (lnp > 0.f) c.k_n += kBeta * fabsf(Ccorr);
+        c.lambda_n = fmaxf(0.f, lnp);
+        const float mag = sqrtf(ltp * ltp + lbp * lbp);
+        float lmax = ((c.flags & GF_USE_STATIC) ? c.mu_s : c.mu_k) * c.lambda_n;
+        if (mag < lmax) {
+            c.k_t += kBeta * fabsf(Ct);
+            c.k_b += kBeta * fabsf(Cb);
+            c.lambda_t = ltp;
+            c.lambda_b = lbp;
+            c.flags |= GF_UNCLAMPED;
+        } else {
+            if (c.flags & GF_USE_STATIC) {
+                c.flags &= ~GF_USE_STATIC;
+                lmax = c.mu_k * lnp;
+            }
+            c.lambda_t = ltp / (mag + 1e-8f) * lmax;
+            c.lambda_b = lbp / (mag + 1e-8f) * lmax;
+            c.flags &= ~GF_UNCLAMPED;
         }
-        c.lambda_t = ltp / (mag + 1e-8f) * lmax;
-        c.lambda_b = lbp / (mag + 1e-8f) * lmax;
-        c.flags &= ~GF_UNCLAMPED;
+        d.ground[gi] = c;
     }
-    ground[gi] = c;
+    if (r >= 0) rigid::records_dual(d, make_float3(xp.x, xp.y, xp.z), q, (uint32_t)r);
 }
 
-__global__ void cheb_dual_kernel(float4* __restrict__ x, float4* __restrict__ x_last, float4* __restrict__ x_lastlast,
-                                 const int32_t* __restrict__ ground_of, GroundRec* __restrict__ ground, uint32_t n,
-                                 float omega) {
+__global__ void cheb_dual_kernel(DevState d, float omega) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) cheb_dual_one(p, x, x_last, x_lastlast, ground_of, ground, omega);
+    if (p < d.n) cheb_dual_one(d, p, omega);
 }
 
 // _particleVelocityUpdate (VBDSolver.hpp:538-566) for one particle.
-__device__ __forceinline__ void velocity_one(uint32_t p, const float4* __restrict__ x, float4* __restrict__ x_prev,
-                                             float4* __restrict__ v, float4* __restrict__ v_prev, float inv_dt) {
-    const float4 xp = x[p], xo = x_prev[p], vv = v[p];
-    v_prev[p] = vv;
-    v[p] = make_float4((xp.x - xo.x) * inv_dt, (xp.y - xo.y) * inv_dt, (xp.z - xo.z) * inv_dt, 0.f);
-    x_prev[p] = xp;
+__device__ __forceinline__ void velocity_one(const DevState& d, uint32_t p, float inv_dt) {
+    const float4 xp = d.x[p], xo = d.x_prev[p], vv = d.v[p];
+    d.v_prev[p] = vv;
+    d.v[p] = make_float4((xp.x - xo.x) * inv_dt, (xp.y - xo.y) * inv_dt, (xp.z - xo.z) * inv_dt, 0.f);
+    d.x_prev[p] = xp;
+    const int32_t r = d.n_rot ? d.rot_of[p] : -1;
+    if (r >= 0) rigid::rotation_velocity(d, (uint32_t)r, inv_dt);
 }
 
-__global__ void velocity_kernel(const float4* __restrict__ x, float4* __restrict__ x_prev, float4* __restrict__ v,
-                                float4* __restrict__ v_prev, uint32_t n, float inv_dt) {
+__global__ void velocity_kernel(DevState d, float inv_dt) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) velocity_one(p, x, x_prev, v, v_prev, inv_dt);
+    if (p < d.n) velocity_one(d, p, inv_dt);
+}
+
+// Oriented vertices of one colour, one thread each.
+__global__ __launch_bounds__(256) void solve_ocolor_kernel(DevState d, uint32_t begin, uint32_t count,
+                                                           float inv_dt2) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) rigid::solve_oitem(d, d.oitems[begin + k], inv_dt2);
 }
 
 // ---- persistent step kernel ------------------------------------------------
@@ -449,8 +456,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
     if constexpr (DAMP) {
         for (uint32_t t = gtid; t < d.n_tets; t += gthreads) eprev_one(t, d.tets, d.damp, d.x_prev);
     }
-    for (uint32_t p = gtid; p < d.n; p += gthreads)
-        step_begin_one(p, d.x, d.y, d.x_last, d.x_lastlast, d.v, d.v_prev, d.fixed, d.ground_of, d.ground, a.dt, a.g);
+    for (uint32_t p = gtid; p < d.n; p += gthreads) step_begin_one(d, p, a.dt, a.g);
     if (!grid_sync(a.bar, G, gen, &ok_s)) return;
 
     const uint32_t groups = B / L, lane = threadIdx.x % L, grp = threadIdx.x / L;
@@ -463,19 +469,24 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
             for (uint32_t k = lo + grp; k < hi; k += groups)
                 solve_item<L, DAMP>(d.items[beg + k], lane, d.inc, d.tets, d.damp, d.x, d.y, d.ground, d.x, d.x_buf,
                                     inv_dt, inv_dt2);
+            if (d.n_oitems) {
+                const uint32_t ob = a.ocolor_offsets[c], ocnt = a.ocolor_offsets[c + 1] - ob;
+                const uint32_t oper = (ocnt + G - 1) / G;
+                const uint32_t olo = min(ocnt, lb * oper), ohi = min(ocnt, olo + oper);
+                for (uint32_t k = olo + threadIdx.x; k < ohi; k += B) rigid::solve_oitem(d, d.oitems[ob + k], inv_dt2);
+            }
             if (!grid_sync(a.bar, G, gen, &ok_s)) return;
             const uint32_t cb = a.conflict_offsets[c], ce = a.conflict_offsets[c + 1];
             if (ce > cb) {
-                for (uint32_t k = cb + gtid; k < ce; k += gthreads) d.x[a.conflicts[k]] = d.x_buf[a.conflicts[k]];
+                for (uint32_t k = cb + gtid; k < ce; k += gthreads) commit_one(d, a.conflicts[k]);
                 if (!grid_sync(a.bar, G, gen, &ok_s)) return;
             }
         }
         const float omega = a.omega[it];
-        for (uint32_t p = gtid; p < d.n; p += gthreads)
-            cheb_dual_one(p, d.x, d.x_last, d.x_lastlast, d.ground_of, d.ground, omega);
+        for (uint32_t p = gtid; p < d.n; p += gthreads) cheb_dual_one(d, p, omega);
         if (!grid_sync(a.bar, G, gen, &ok_s)) return;
     }
-    for (uint32_t p = gtid; p < d.n; p += gthreads) velocity_one(p, d.x, d.x_prev, d.v, d.v_prev, inv_dt);
+    for (uint32_t p = gtid; p < d.n; p += gthreads) velocity_one(d, p, inv_dt);
 }
 
 // ---- residual / incremental potential (new; see avbd_get_stats) ----------
@@ -539,7 +550,8 @@ __global__ __launch_bounds__(1024) void stats_items_kernel(const VItem* __restri
 template <bool DAMP>
 __global__ void stats_energy_kernel(const TetRec* __restrict__ tets, const TetDamp* __restrict__ damp,
                                     uint32_t n_tets, const GroundRec* __restrict__ ground, uint32_t n_ground,
-                                    const float4* __restrict__ x, double* __restrict__ out, float inv_dt) {
+                                    const float4* __restrict__ x, double* __restrict__ out, float inv_dt,
+                                    DevState d) {
     __shared__ double red[256];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0;
@@ -575,6 +587,10 @@ __global__ void stats_energy_kernel(const TetRec* __restrict__ tets, const TetDa
         const double Cc = Cn - (double)kAlpha * c.C_n_prev;
         if ((double)c.k_n * Cc + c.lambda_n > 0) e += 0.5 * c.k_n * Cc * Cc + (double)c.lambda_n * Cc;
         e += 0.5 * c.k_t * Ct * Ct + (double)c.lambda_t * Ct + 0.5 * c.k_b * Cb * Cb + (double)c.lambda_b * Cb;
+    } else if (k < n_tets + n_ground + d.n_cos) {
+        // CosseratRodEnergySolver::energy (CosseratRodEnergySolver.hpp:20-49)
+        const CosRec c = d.cos[k - n_tets - n_ground];
+        e = rigid::cos_energy(c, rigid::xyz(x[c.i0]), rigid::xyz(x[c.i1]), d.q[d.rot_of[c.i0]], d.q[d.rot_of[c.i1]]);
     }
     red[threadIdx.x] = e;
     __syncthreads();
@@ -583,6 +599,42 @@ __global__ void stats_energy_kernel(const TetRec* __restrict__ tets, const TetDa
         __syncthreads();
     }
     if (threadIdx.x == 0) out[blockIdx.x * 4 + 3] = red[0];
+}
+
+// Residual / inertia term of the oriented vertices (positional block of the
+// 6-DOF gradient), written to slots 0..2 of blocks [base_block, ...).
+__global__ __launch_bounds__(256) void stats_oitems_kernel(DevState d, double* __restrict__ out,
+                                                           uint32_t base_block, float inv_dt2, float g_acc) {
+    __shared__ double red[3][256];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    double r2 = 0, mg2 = 0, ip = 0;
+    if (k < d.n_oitems) {
+        const OItem it = d.oitems[k];
+        const float4 xv = d.x[it.v];
+        rot::Vec6 G;
+        rot::Sym6 H;
+        rot::vec6_zero(G);
+        rot::sym6_zero(H);
+        rigid::accumulate6(d, it, rigid::xyz(xv), d.q[it.rot], G, H);
+        const float4 yv = d.y[it.v];
+        const double md = (double)xv.w * (double)inv_dt2;
+        const double dx = (double)xv.x - yv.x, dy = (double)xv.y - yv.y, dz = (double)xv.z - yv.z;
+        const double rx = md * dx + G.v[0], ry = md * dy + G.v[1], rz = md * dz + G.v[2];
+        r2 = rx * rx + ry * ry + rz * rz;
+        const double mg = (double)xv.w * g_acc;
+        mg2 = mg * mg;
+        ip = 0.5 * md * (dx * dx + dy * dy + dz * dz);
+    }
+    red[0][threadIdx.x] = r2;
+    red[1][threadIdx.x] = mg2;
+    red[2][threadIdx.x] = ip;
+    __syncthreads();
+    for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int i = 0; i < 3; i++) red[i][threadIdx.x] += red[i][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[(base_block + blockIdx.x) * 4 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
@@ -616,8 +668,13 @@ void launch_eprev(hipStream_t s, const DevState& d, float) {
 
 void launch_step_begin(hipStream_t s, const DevState& d, float dt, float g) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(step_begin_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d.x, d.y, d.x_last,
-                       d.x_lastlast, d.v, d.v_prev, d.fixed, d.ground_of, d.ground, d.n, dt, g);
+    hipLaunchKernelGGL(step_begin_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, dt, g);
+}
+
+void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t begin, uint32_t count, float dt) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(solve_ocolor_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, begin, count,
+                       1.0f / (dt * dt));
 }
 
 void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
@@ -640,19 +697,17 @@ void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, 
 
 void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count) {
     if (count == 0) return;
-    hipLaunchKernelGGL(commit_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d.x, d.x_buf, list, count);
+    hipLaunchKernelGGL(commit_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, list, count);
 }
 
 void launch_cheb_dual(hipStream_t s, const DevState& d, float omega) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(cheb_dual_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d.x, d.x_last, d.x_lastlast,
-                       d.ground_of, d.ground, d.n, omega);
+    hipLaunchKernelGGL(cheb_dual_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, omega);
 }
 
 void launch_velocity(hipStream_t s, const DevState& d, float dt) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(velocity_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d.x, d.x_prev, d.v, d.v_prev,
-                       d.n, 1.0f / dt);
+    hipLaunchKernelGGL(velocity_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, 1.0f / dt);
 }
 
 uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt, float g) {
@@ -673,17 +728,21 @@ uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bo
         default: break;
     }
 #undef AVBD_STATS_CASE
-    const uint32_t ne = d.n_tets + d.n_ground;
+    const uint32_t ne = d.n_tets + d.n_ground + d.n_cos;
     const uint32_t nb_e = blocks_for(ne, 256);
     if (ne) {
         if (damping)
             hipLaunchKernelGGL(stats_energy_kernel<true>, dim3(nb_e), dim3(256), 0, s, d.tets, d.damp, d.n_tets,
-                               d.ground, d.n_ground, d.x, d.reduce, 1.0f / dt);
+                               d.ground, d.n_ground, d.x, d.reduce, 1.0f / dt, d);
         else
             hipLaunchKernelGGL(stats_energy_kernel<false>, dim3(nb_e), dim3(256), 0, s, d.tets, d.damp, d.n_tets,
-                               d.ground, d.n_ground, d.x, d.reduce, 1.0f / dt);
+                               d.ground, d.n_ground, d.x, d.reduce, 1.0f / dt, d);
     }
-    return nb_items > nb_e ? nb_items : nb_e;
+    const uint32_t nb_o = blocks_for(d.n_oitems, 256);
+    if (nb_o)
+        hipLaunchKernelGGL(stats_oitems_kernel, dim3(nb_o), dim3(256), 0, s, d, d.reduce, nb_items, 1.0f / (dt * dt),
+                           g);
+    return nb_items + nb_o > nb_e ? nb_items + nb_o : nb_e;
 }
 
 }  // namespace avbd

@@ -13,7 +13,7 @@ All randomness comes from SplitMix64 so the inputs are reproducible bit for bit.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Optional
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -131,10 +131,47 @@ class TetMesh:
 
 
 @dataclass
+class Rod:
+    """A Cosserat rod (RodConfig fields, config/RodConfig.hpp:18-57): elements+1
+    oriented nodes from `pos` along the body z axis of `rot` (quaternion
+    w, x, y, z).  base_joint holds the first node at its initial pose with a
+    one-sided fixed joint (the form left commented out at simobject/Rod.cpp:79-89)."""
+    elements: int
+    length: float = 1.0
+    E: float = 1e7
+    nu: float = 0.3
+    density: float = 1000.0
+    diameter: float = 0.01
+    pos: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    rot: Tuple[float, float, float, float] = (1.0, 0.0, 0.0, 0.0)
+    vel: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    curvature: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    base_fixed: bool = False
+    tip_fixed: bool = False
+    ground: bool = False
+    base_joint: bool = False
+
+
+@dataclass
+class Sphere:
+    """A rigid sphere (RigidSphereConfig, config/RigidSphereConfig.hpp) with an
+    initial angular velocity w0 (body frame)."""
+    radius: float
+    density: float
+    pos: Tuple[float, float, float]
+    rot: Tuple[float, float, float, float] = (1.0, 0.0, 0.0, 0.0)
+    w0: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    fixed: bool = False
+    ground: bool = True
+
+
+@dataclass
 class Scene:
     """A batch of independent scenes sharing one solver configuration."""
     name: str
     meshes: List[TetMesh] = field(default_factory=list)
+    rods: List[Rod] = field(default_factory=list)
+    spheres: List[Sphere] = field(default_factory=list)
     dt: float = 1.0 / 60.0
     g: float = 9.81
     iters: int = 20
@@ -204,6 +241,40 @@ def cube_batch(n_scenes: int = 512, seed: int = 4, n_min: int = 4, n_max: int = 
         p[:, 1] += h - p[:, 1].min()
         sc.meshes.append(TetMesh(rest=p, tets=tets, E=E, nu=nu, density=1000.0, kd=0.0, ground=True))
     return sc
+
+
+def quat_about(axis, angle: float) -> Tuple[float, float, float, float]:
+    """Unit quaternion (w, x, y, z) of a rotation by `angle` about `axis`."""
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    s = np.sin(0.5 * angle)
+    return (float(np.cos(0.5 * angle)), float(s * a[0]), float(s * a[1]), float(s * a[2]))
+
+
+def rod_cantilever(elements: int = 8, length: float = 0.2, E: float = 1e8, diameter: float = 0.02,
+                   iters: int = 100, dt: float = 1.0 / 600.0, steps: int = 500) -> Scene:
+    """One horizontal rod along +x, base node fixed, sagging under gravity."""
+    rod = Rod(elements=elements, length=length, E=E, nu=0.3, density=1000.0, diameter=diameter,
+              rot=quat_about((0, 1, 0), np.pi / 2), base_fixed=True)
+    return Scene("rod_cantilever", rods=[rod], dt=dt, iters=iters, rho=0.95, steps=steps)
+
+
+def rods_and_spheres(n_rods: int = 4, elements: int = 16, seed: int = 3, dt: float = 1.0 / 120.0,
+                     iters: int = 20, steps: int = 120) -> Scene:
+    """Hanging rods held at the top by one-sided fixed joints, swinging from a
+    tilted start and touching the ground, plus spinning spheres resting on the
+    ground (rigid-ground contacts on every node and sphere)."""
+    rng = SplitMix64(seed)
+    rods, spheres = [], []
+    for i in range(n_rods):
+        tilt = 0.3 + 0.4 * rng.uniform()
+        rot = quat_about((1.0, 0.0, 0.2 * (rng.uniform() - 0.5)), np.pi / 2 + tilt)
+        rods.append(Rod(elements=elements, length=0.5, E=1e7, nu=0.3, density=1100.0, diameter=0.01,
+                        pos=(0.1 * i, 0.49, 0.0), rot=rot, ground=True, base_joint=True))
+    for i in range(n_rods):
+        w0 = tuple(float(10.0 * (rng.uniform() - 0.5)) for _ in range(3))
+        spheres.append(Sphere(radius=0.02, density=1500.0, pos=(0.1 * i, 0.02, 0.3), w0=w0))
+    return Scene("rods_and_spheres", rods=rods, spheres=spheres, dt=dt, iters=iters, rho=0.95, steps=steps)
 
 
 def algorithmic_bytes_per_step(n_v: int, n_tet: int, n_g: int, iters: int) -> int:

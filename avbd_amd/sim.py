@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .scenes import Scene, TetMesh
+from .scenes import Rod, Scene, Sphere, TetMesh
 
 
 def _dptr(a: Optional[np.ndarray]):
@@ -102,6 +102,83 @@ class SimulationContext:
     def add_energies(self, type_: int, records) -> None:
         n = len(records)
         self._chk(lib.avbd_add_energies(self.handle, type_, C.cast(records, C.c_void_p), n))
+
+    # --- oriented particles (ParticlePool::addOrientedParticle, Rod, RigidSphere) ---
+    def add_oriented_particles(self, x, inertia, q=None, v=None, w=None, mass=None, fixed=None) -> int:
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        x, inertia, q, v, w, mass = arr(x), arr(inertia), arr(q), arr(v), arr(w), arr(mass)
+        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        first = C.c_uint32()
+        self._chk(lib.avbd_add_oriented_particles(self.handle, x.shape[0], _dptr(x), _dptr(q), _dptr(v), _dptr(w),
+                                                  _dptr(mass), _dptr(inertia), _bptr(fx), C.byref(first)))
+        return first.value
+
+    def add_rod(self, r: Rod) -> int:
+        v3 = lambda t: np.ascontiguousarray(t, dtype=np.float64)
+        first = C.c_uint32()
+        self._chk(lib.avbd_add_rod(self.handle, r.elements, r.length, _dptr(v3(r.curvature)), r.E, r.nu, r.density,
+                                   r.diameter, _dptr(v3(r.pos)), _dptr(v3(r.rot)), _dptr(v3(r.vel)),
+                                   int(r.base_fixed), int(r.tip_fixed), int(r.ground), C.byref(first)))
+        return first.value
+
+    def add_sphere(self, sp: Sphere) -> int:
+        v3 = lambda t: np.ascontiguousarray(t, dtype=np.float64)
+        first = C.c_uint32()
+        self._chk(lib.avbd_add_sphere(self.handle, sp.radius, sp.density, _dptr(v3(sp.pos)), _dptr(v3(sp.rot)),
+                                      int(sp.fixed), int(sp.ground), C.byref(first)))
+        return first.value
+
+    def add_joint(self, p: int, pos_offset, rot_offset, ref_position, ref_rotation) -> None:
+        """OneSidedFixedJointEnergyPool::addEnergy (energy/joint/OneSidedFixedJointEnergyPool.hpp:33-50):
+        k = k_start = 1e4, lambda = 0, C_prev = 0."""
+        r = _lib.JointRecord()
+        r.particle_index = p
+        for k in range(3):
+            r.position_offset[k] = pos_offset[k]
+            r.ref_position[k] = ref_position[k]
+        for k in range(4):
+            r.rotation_offset[k] = rot_offset[k]
+            r.ref_rotation[k] = ref_rotation[k]
+        for k in range(6):
+            r.k[k] = 1e4
+        recs = (_lib.JointRecord * 1)(r)
+        self.add_energies(_lib.ONE_SIDED_FIXED_JOINT, recs)
+
+    def add_cosserat(self, a: int, b: int, stiffness, rest_length: float, rest_curvature=(0, 0, 0)) -> None:
+        r = _lib.CosseratRecord()
+        r.particle_indices[0], r.particle_indices[1] = a, b
+        for k in range(6):
+            r.stiffness[k] = stiffness[k]
+        r.rest_length = rest_length
+        for k in range(3):
+            r.rest_curvature[k] = rest_curvature[k]
+        self.add_energies(_lib.COSSERAT_ROD, (_lib.CosseratRecord * 1)(r))
+
+    def set_rotation_state(self, first: int, q=None, w=None) -> None:
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        q, w = arr(q), arr(w)
+        n = q.shape[0] if q is not None else w.shape[0]
+        self._chk(lib.avbd_set_rotation_state(self.handle, first, n, _dptr(q), _dptr(w)))
+
+    def rotations(self):
+        """Per particle: orientation (w, x, y, z) and body-frame angular velocity."""
+        q = np.empty((self.num_particles, 4))
+        w = np.empty((self.num_particles, 3))
+        self._chk(lib.avbd_download_rotations(self.handle, _dptr(q), _dptr(w)))
+        return q, w
+
+    def rigid_ground_records(self):
+        n = lib.avbd_num_energies(self.handle, _lib.RIGID_BODY_GROUND_COLLISION)
+        recs = (_lib.RigidGroundRecord * n)()
+        self._chk(lib.avbd_download_energies(self.handle, _lib.RIGID_BODY_GROUND_COLLISION,
+                                             C.cast(recs, C.c_void_p), n))
+        return recs
+
+    def joint_records(self):
+        n = lib.avbd_num_energies(self.handle, _lib.ONE_SIDED_FIXED_JOINT)
+        recs = (_lib.JointRecord * n)()
+        self._chk(lib.avbd_download_energies(self.handle, _lib.ONE_SIDED_FIXED_JOINT, C.cast(recs, C.c_void_p), n))
+        return recs
 
     def set_particle_state(self, first: int, x=None, x_prev=None, v=None, v_prev=None) -> None:
         arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
@@ -247,10 +324,32 @@ class Simulation:
             first = self._add_object(m)
             offs.append(first + m.rest.shape[0])
         self.offsets = offs
+        self.rod_first: List[int] = []
+        for r in self.scene.rods:
+            self.rod_first.append(self._add_rod(r))
+        self.sphere_index: List[int] = []
+        for sp in self.scene.spheres:
+            self.sphere_index.append(self._add_sphere(sp))
         if len(self.scene.meshes) > 1:
+            if self.scene.rods or self.scene.spheres:
+                raise ValueError("batched tet scenes cannot be mixed with rods or spheres")
             self.ctx.set_scenes(offs)
         self.ctx.build_graph()
         self._setup = True
+
+    def _add_rod(self, r: Rod) -> int:
+        first = self.ctx.add_rod(r)
+        if r.base_joint:
+            # hold the first node at its initial pose
+            self.ctx.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
+        return first
+
+    def _add_sphere(self, sp: Sphere) -> int:
+        p = self.ctx.add_sphere(sp)
+        if any(c != 0.0 for c in sp.w0):
+            self.ctx.set_rotation_state(p, q=np.asarray([sp.rot], dtype=np.float64),
+                                        w=np.asarray([sp.w0], dtype=np.float64))
+        return p
 
     def _add_object(self, m: TetMesh) -> int:
         first = self.ctx.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)

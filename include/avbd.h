@@ -18,6 +18,9 @@
  *                                                  energy/elastic/NeoHookeanEnergyPool.hpp:40-67,
  *                                                  energy/collision/GroundCollisionEnergyPool.hpp:26-38
  *   TetMeshObject::setup (setup formulas)          simobject/TetMeshObject.cpp:21-113
+ *   Rod / RigidSphere constructors                 simobject/Rod.cpp:10-78,
+ *                                                  simobject/rigid/RigidSphere.cpp:8-22,
+ *                                                  simobject/rigid/RigidObjectBase.cpp:10-20
  *   Simulation::_addGroundCollisionConstraintsForObject  simulation/Simulation.hpp:117-126
  *   StaticParticleAdjacency::buildAdjacency + ColorList::buildInitialColorList
  *                                                  simulation/Simulation.cpp:87-89
@@ -102,6 +105,53 @@ typedef struct {
     double cp_x, cp_z;
 } avbd_ground_record;
 
+/* CosseratRodEnergyInfo, energy/elastic/CosseratRodEnergyPool.hpp:8-14:
+ * stiffness = diag(GA, GA, EA, EI, EI, GJ), rest length, rest curvature. */
+typedef struct {
+    uint32_t particle_indices[2];
+    double stiffness[6];
+    double rest_length;
+    double rest_curvature[3];
+} avbd_cosserat_record;
+
+/* Collision geometry of a rigid-ground contact: the reference's
+ * CollisionGeometryType values (collision/CollisionGeometryType.hpp). */
+#define AVBD_SHAPE_ROD_SEGMENT 1
+#define AVBD_SHAPE_SPHERE 2
+#define AVBD_SHAPE_BOX 3
+#define AVBD_SHAPE_CAPSULE 4
+
+/* RigidBodyGroundCollisionEnergyInfo (energy/collision/RigidBodyGroundCollisionEnergyPool.hpp:8-15)
+ * + the AL state of CollisionConstraintEnergyInfo; normal y, tangents x and z. */
+typedef struct {
+    uint32_t particle_index;
+    uint32_t shape;                /* AVBD_SHAPE_* */
+    uint32_t use_static;
+    uint32_t unclamped_lambda_tb;
+    double radius;                 /* sphere / rod radius (lowest-point contact) */
+    double cp_rb_local[3];         /* contact point in the body frame */
+    double cp_ground[3];
+    double k_start, k_n, k_t, k_b;
+    double lambda_n, lambda_t, lambda_b;
+    double C_n_prev;
+    double mu_s, mu_k;
+} avbd_rigid_ground_record;
+
+/* OneSidedFixedJointEnergyInfo (energy/joint/OneSidedFixedJointEnergyPool.hpp:8-16)
+ * + the vector AL state of HardConstraintEnergyPool.  Quaternions are
+ * (w, x, y, z). */
+typedef struct {
+    uint32_t particle_index;
+    uint32_t pad_;
+    double position_offset[3];
+    double rotation_offset[4];
+    double ref_position[3];
+    double ref_rotation[4];
+    double k[6];
+    double lambda[6];
+    double C_prev[6];
+} avbd_joint_record;
+
 /* Per-step statistics (new: the reference has no residual; see the
  * definitions in the oracle and the solver sources). */
 typedef struct {
@@ -131,8 +181,33 @@ const char* avbd_last_error(avbd_ctx* ctx);
 int avbd_add_particles(avbd_ctx* ctx, uint32_t n, const double* x, const double* v, const double* v_prev,
                        const double* x_prev, const double* mass, const uint8_t* fixed, uint32_t* first);
 
+/* ParticlePool::addOrientedParticle (common/ParticlePool.hpp:248-263) for n
+ * particles: x, v, w 3n doubles (NULL v/w = zero), q 4n doubles (w, x, y, z;
+ * NULL = identity), mass n, inertia 3n (principal moments), fixed n bytes
+ * (NULL = not fixed).  *first receives the first new index. */
+int avbd_add_oriented_particles(avbd_ctx* ctx, uint32_t n, const double* x, const double* q, const double* v,
+                                const double* w, const double* mass, const double* inertia, const uint8_t* fixed,
+                                uint32_t* first);
+
+/* Rod ctor (simobject/Rod.cpp:10-78): elements+1 oriented nodes from pos along
+ * the body z axis of rot (quaternion w, x, y, z), bent by the rest curvature;
+ * lumped masses and inertias; one Cosserat energy per element; base/tip
+ * fixed flags; with_ground adds one rigid-ground contact per node
+ * (simulation/Simulation.hpp:128-138, k_start = m/dt^2, mu 0.5/0.2).
+ * *first receives the first node. */
+int avbd_add_rod(avbd_ctx* ctx, uint32_t elements, double length, const double* curvature, double E, double nu,
+                 double density, double diameter, const double* pos, const double* rot, const double* vel,
+                 int base_fixed, int tip_fixed, int with_ground, uint32_t* first);
+
+/* RigidSphere ctor (simobject/rigid/RigidSphere.cpp:8-22, RigidObjectBase.cpp:10-20):
+ * one oriented particle of mass 4/3 pi r^3 density and inertia 2/5 m r^2;
+ * with_ground adds its rigid-ground contact (simulation/Simulation.hpp:108-115). */
+int avbd_add_sphere(avbd_ctx* ctx, double radius, double density, const double* pos, const double* rot, int fixed,
+                    int with_ground, uint32_t* first);
+
 /* EnergyRegistry pools: append count records of the given type
- * (avbd_neo_hookean_record / avbd_ground_record). */
+ * (avbd_neo_hookean_record / avbd_ground_record / avbd_cosserat_record /
+ * avbd_rigid_ground_record / avbd_joint_record). */
 int avbd_add_energies(avbd_ctx* ctx, int type, const void* records, uint32_t count);
 
 /* TetMeshObject::setup formulas (simobject/TetMeshObject.cpp:46-96) for a mesh
@@ -149,6 +224,10 @@ int avbd_add_tet_mesh(avbd_ctx* ctx, uint32_t nv, const double* rest_x, uint32_t
 int avbd_set_particle_state(avbd_ctx* ctx, uint32_t first, uint32_t n, const double* x, const double* x_prev,
                             const double* v, const double* v_prev);
 int avbd_set_fixed(avbd_ctx* ctx, uint32_t first, uint32_t n, const uint8_t* fixed);
+/* Orientation q (4n: w, x, y, z) and angular velocity w (3n) of the oriented
+ * particles among [first, first+n), also taken as their previous-step values
+ * (rows of positional particles are ignored; NULL = unchanged). */
+int avbd_set_rotation_state(avbd_ctx* ctx, uint32_t first, uint32_t n, const double* q, const double* w);
 
 /* Scene partition for batching: scene s owns particles
  * [particle_offsets[s], particle_offsets[s+1]); each scene is coloured on its
@@ -168,6 +247,9 @@ int avbd_synchronize(avbd_ctx* ctx);
 /* Readback (synchronising) -------------------------------------------------- */
 int avbd_download_state(avbd_ctx* ctx, double* x, double* v);
 int avbd_download_inertial(avbd_ctx* ctx, double* y);
+/* Per particle: orientation (4: w, x, y, z; identity for positional
+ * particles) and angular velocity (3; zero for positional particles). */
+int avbd_download_rotations(avbd_ctx* ctx, double* q, double* w);
 int avbd_download_energies(avbd_ctx* ctx, int type, void* records, uint32_t count);
 /* color: one entry per particle; *num_colors = max over scenes. */
 int avbd_download_coloring(avbd_ctx* ctx, uint32_t* color, uint32_t* num_colors);

@@ -239,18 +239,37 @@ def lu_solve(A, b):
 
 
 def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
-    """Oracle instance for one mesh of an avbd_amd.scenes.Scene (one reference Simulation)."""
-    m = scene.meshes[mesh_index]
+    """Oracle instance for one mesh of an avbd_amd.scenes.Scene (one reference
+    Simulation), followed by the scene's rods and spheres in order."""
     o = OracleSim(scene.dt, scene.g, scene.iters, scene.rho)
-    o.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
-    if m.fixed is not None:
-        o.set_fixed(m.fixed)
-    if m.x0 is not None:
-        o.set_positions(m.x0, also_prev=True)
-    if m.v0 is not None:
-        o.set_velocities(m.v0, m.v0)
-    if collide and m.triangles is not None and len(scene.meshes) == 1:
-        o.set_collision_triangles(m.triangles)
+    if scene.meshes:
+        m = scene.meshes[mesh_index]
+        o.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
+        if m.fixed is not None:
+            o.set_fixed(m.fixed)
+        if m.x0 is not None:
+            o.set_positions(m.x0, also_prev=True)
+        if m.v0 is not None:
+            o.set_velocities(m.v0, m.v0)
+        if collide and m.triangles is not None and len(scene.meshes) == 1:
+            o.set_collision_triangles(m.triangles)
+    for r in getattr(scene, "rods", []):
+        first = o.add_rod(r.elements, r.length, r.curvature, r.E, r.nu, r.density, r.diameter, r.pos, r.rot,
+                          r.vel, r.base_fixed, r.tip_fixed, r.ground)
+        if r.base_joint:
+            o.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
+    for sp in getattr(scene, "spheres", []):
+        p = o.add_sphere(sp.radius, sp.density, sp.pos, sp.rot, sp.fixed, sp.ground)
+        if any(c != 0.0 for c in sp.w0):
+            n = o.n
+            q = np.tile(np.array([1.0, 0, 0, 0]), (n, 1))
+            w = np.zeros((n, 3))
+            cur = o.rotations()
+            q[:] = cur["q"]
+            w[:] = cur["w"]
+            q[p] = sp.rot
+            w[p] = sp.w0
+            o.set_rotation_state(q, w)
     o.build_graph()
     return o
 

@@ -148,3 +148,76 @@ def test_scene_offsets_validated():
     with pytest.raises(_lib.AvbdError) as e:
         ctx.build_graph()
     assert "spans two scenes" in str(e.value)
+
+
+def test_record_layouts_match_c_compiler(tmp_path):
+    """sizeof/offsetof of every record as the C compiler lays it out from
+    include/avbd.h equals the ctypes mirror."""
+    import subprocess
+    from avbd_amd import _lib
+    src = tmp_path / "sizes.c"
+    recs = ["avbd_neo_hookean_record", "avbd_ground_record", "avbd_cosserat_record", "avbd_rigid_ground_record",
+            "avbd_joint_record", "avbd_stats", "avbd_scene_desc"]
+    body = "\n".join(f'printf("%zu\\n", sizeof({r}));' for r in recs)
+    src.write_text(f'#include <stdio.h>\n#include "avbd.h"\nint main(void) {{ {body} return 0; }}\n')
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    mirror = [_lib.NeoHookeanRecord, _lib.GroundRecord, _lib.CosseratRecord, _lib.RigidGroundRecord,
+              _lib.JointRecord, _lib.Stats, _lib.SceneDesc]
+    assert sizes == [C.sizeof(m) for m in mirror]
+
+
+def _rod_scene_ctx(scene):
+    from avbd_amd.sim import Simulation
+    sim = Simulation(scene)
+    for r in scene.rods:
+        sim._add_rod(r)
+    for sp in scene.spheres:
+        sim._add_sphere(sp)
+    return sim
+
+
+def test_rod_and_sphere_coloring_bit_exact_vs_oracle(oracle):
+    sc = scenes.rods_and_spheres(n_rods=3, elements=9)
+    sim = _rod_scene_ctx(sc)
+    col, k = _planned_coloring(sim.ctx)
+    o = oracle.from_scene(sc)
+    ocol, ok = o.colors()
+    assert (k, col.tolist()) == (ok, ocol.tolist())
+    assert k == 2
+    from avbd_amd import _lib
+    assert _lib.lib.avbd_num_energies(sim.ctx.handle, _lib.COSSERAT_ROD) == 27
+    assert _lib.lib.avbd_num_energies(sim.ctx.handle, _lib.RIGID_BODY_GROUND_COLLISION) == 30 + 3
+    assert _lib.lib.avbd_num_energies(sim.ctx.handle, _lib.ONE_SIDED_FIXED_JOINT) == 3
+
+
+def test_energy_particle_kind_validated():
+    """Neo-Hookean tets need positional particles; Cosserat, rigid-ground and
+    joint energies need oriented ones (the solvers' SupportsPositional /
+    SupportsOriented)."""
+    from avbd_amd import _lib
+    ctx = make_ctx()
+    ctx.add_particles(np.zeros((2, 3)), mass=np.ones(2))
+    ctx.add_cosserat(0, 1, [1.0] * 6, 0.1)
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.build_graph()
+    assert "positional particle" in str(e.value)
+
+    ctx = make_ctx()
+    ctx.add_oriented_particles(np.eye(4)[:, :3].copy(), inertia=np.ones((4, 3)), mass=np.ones(4))
+    r = _lib.NeoHookeanRecord()
+    for k in range(4):
+        r.particle_indices[k] = k
+    r.rest_volume = 1.0
+    ctx.add_energies(_lib.NEO_HOOKEAN, (_lib.NeoHookeanRecord * 1)(r))
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.build_graph()
+    assert "oriented particle" in str(e.value)
+
+    ctx = make_ctx()
+    ctx.add_particles(np.zeros((1, 3)), mass=np.ones(1))
+    ctx.add_joint(0, (0, 0, 0), (1, 0, 0, 0), (0, 0, 0), (1, 0, 0, 0))
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.build_graph()
+    assert "oriented particle" in str(e.value)
