@@ -32,6 +32,7 @@ ONE_SIDED_FIXED_JOINT = 6
 FLAG_NO_GRAPH = 1
 FLAG_SERIAL_BROADPHASE = 2
 FLAG_NO_PERSISTENT = 4
+FLAG_PERSISTENT = 8
 
 
 class SceneDesc(C.Structure):
@@ -146,6 +147,9 @@ class Stats(C.Structure):
         ("steps_taken", C.c_uint64),
         ("vertex_updates", C.c_uint64),
         ("last_step_ms", C.c_double),
+        ("iterations_total", C.c_uint64),
+        ("last_step_iterations", C.c_uint32),
+        ("pad_", C.c_uint32),
     ]
 
 
@@ -166,6 +170,8 @@ SIGNATURES = {
                                _D, _D, C.c_int, C.c_int, C.c_int, _U32]),
     "avbd_add_sphere": (C.c_int, [_P, C.c_double, C.c_double, _D, _D, C.c_int, C.c_int, _U32]),
     "avbd_set_rotation_state": (C.c_int, [_P, C.c_uint32, C.c_uint32, _D, _D]),
+    "avbd_set_residual_target": (C.c_int, [_P, C.c_double, C.c_uint32, C.c_uint32]),
+    "avbd_get_residual_history": (C.c_int, [_P, _D, C.c_uint32, _U32]),
     "avbd_download_rotations": (C.c_int, [_P, _D, _D]),
     "avbd_add_tet_mesh": (C.c_int, [_P, C.c_uint32, _D, C.c_uint32, _U32, C.c_double, C.c_double, C.c_double,
                                     C.c_double, C.c_int, C.c_double, C.c_double, _U32]),

@@ -69,6 +69,15 @@ struct avbd_ctx {
 
     uint64_t steps_taken = 0, vertex_updates = 0;
     double last_step_ms = 0;
+
+    // residual-targeted iteration (avbd_set_residual_target)
+    double res_tol = 0;            // 0: fixed iteration count
+    uint32_t res_max_iters = 0, res_check_every = 0;
+    double res_tol2 = 0;           // tol^2 ||M g||^2
+    ResCtl* rc = nullptr;
+
+    // iterations per step: the budget in residual mode, else solver_iters
+    uint32_t iters() const { return res_tol > 0 ? res_max_iters : desc.solver_iters; }
 };
 
 namespace {
@@ -342,22 +351,29 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
     }
     if (c->damping) launch_eprev(s, c->d, dt);
     launch_step_begin(s, c->d, dt, g);
+    const uint32_t* done = c->rc ? &c->rc->done : nullptr;
+    if (c->rc) launch_residual_reset(s, c->rc);
     double omega = 1;
     size_t ev = 0;
-    for (uint32_t i = 0; i < c->desc.solver_iters; i++) {
+    const uint32_t iters = c->iters();
+    for (uint32_t i = 0; i < iters; i++) {
         omega = omega_at(i, c->desc.iter_acceleration, omega);
         for (uint32_t col = 0; col < c->plan.num_colors; col++) {
             const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
             if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
-            launch_solve_color(s, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x);
+            launch_solve_color(s, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x, done);
             const uint32_t ob = c->plan.oitem_offsets[col], oe = c->plan.oitem_offsets[col + 1];
-            launch_solve_ocolor(s, c->d, ob, oe - ob, dt);
+            launch_solve_ocolor(s, c->d, ob, oe - ob, dt, done);
             if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
             const uint32_t cb = c->plan.conflict_offsets[col], ce = c->plan.conflict_offsets[col + 1];
-            if (ce > cb) launch_commit(s, c->d, c->d_conflicts + cb, ce - cb);
+            if (ce > cb) launch_commit(s, c->d, c->d_conflicts + cb, ce - cb, done);
         }
-        launch_cheb_dual(s, c->d, (float)omega);
+        launch_cheb_dual(s, c->d, (float)omega, done);
+        if (c->rc && (i + 1) % c->res_check_every == 0 && i + 1 < iters)
+            launch_residual_check(s, c->d, c->cfg, c->damping, dt, c->rc, (i + 1) / c->res_check_every - 1, i + 1,
+                                  c->res_tol2);
     }
+    if (c->rc) launch_residual_finish(s, c->rc, iters);
     launch_velocity(s, c->d, dt);
     if (fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
     hip_check(hipGetLastError(), "kernel launch");
@@ -865,6 +881,38 @@ int avbd_set_rotation_state(avbd_ctx* c, uint32_t first, uint32_t n, const doubl
     });
 }
 
+int avbd_set_residual_target(avbd_ctx* c, double rel_tol, uint32_t max_iters, uint32_t check_every) {
+    return guarded(c, [&] {
+        require_not_built(c);
+        if (rel_tol < 0) throw std::runtime_error("rel_tol must be >= 0");
+        if (rel_tol > 0) {
+            if (max_iters == 0 || check_every == 0) throw std::runtime_error("max_iters and check_every must be >= 1");
+            if ((max_iters + check_every - 1) / check_every > kMaxResidualChecks)
+                throw std::runtime_error("at most 64 residual checks per step (raise check_every)");
+        }
+        c->res_tol = rel_tol;
+        c->res_max_iters = max_iters;
+        c->res_check_every = check_every;
+    });
+}
+
+int avbd_get_residual_history(avbd_ctx* c, double* rel, uint32_t cap, uint32_t* count) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!c->rc) throw std::runtime_error("residual-targeted iteration is off");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        ResCtl rc{};
+        hip_check(hipMemcpy(&rc, c->rc, sizeof(ResCtl), hipMemcpyDeviceToHost), "D2H");
+        const uint32_t it = rc.last_iters;
+        // checks run after iterations K, 2K, ... below the budget, up to the one that converged
+        uint32_t n = 0;
+        for (uint32_t k = c->res_check_every; k <= it && k < c->res_max_iters; k += c->res_check_every) n++;
+        const double mg2 = c->res_tol2 / (c->res_tol * c->res_tol);
+        for (uint32_t k = 0; k < n && k < cap; k++) rel[k] = mg2 > 0 ? std::sqrt(rc.acc[k] / mg2) : 0.0;
+        if (count) *count = n;
+    });
+}
+
 int avbd_set_scenes(avbd_ctx* c, uint32_t n_scenes, const uint32_t* offsets) {
     return guarded(c, [&] {
         require_not_planned(c);
@@ -1050,9 +1098,9 @@ int avbd_build_graph(avbd_ctx* c) {
         // persistent step kernel: per-colour offsets, Chebyshev schedule, barrier
         {
             std::vector<uint32_t> co(c->plan.item_offsets.begin(), c->plan.item_offsets.end());
-            std::vector<float> om(c->desc.solver_iters);
+            std::vector<float> om(c->iters());
             double w = 1;
-            for (uint32_t i = 0; i < c->desc.solver_iters; i++) om[i] = (float)(w = omega_at(i, c->desc.iter_acceleration, w));
+            for (uint32_t i = 0; i < c->iters(); i++) om[i] = (float)(w = omega_at(i, c->desc.iter_acceleration, w));
             uint32_t* d_co = dev_alloc<uint32_t>(c, co.size());
             uint32_t* d_oco = dev_alloc<uint32_t>(c, c->plan.oitem_offsets.size());
             upload(c, d_oco, c->plan.oitem_offsets);
@@ -1070,7 +1118,19 @@ int avbd_build_graph(avbd_ctx* c) {
             c->sargs.ocolor_offsets = d_oco;
             c->sargs.omega = d_om;
             c->sargs.bar = bar;
-            c->sargs.iters = c->desc.solver_iters;
+            c->sargs.iters = c->iters();
+            if (c->res_tol > 0) {
+                // ||M g||^2 over free particles, as in the residual definition
+                double mg2 = 0;
+                for (uint32_t p = 0; p < n; p++)
+                    if (!c->fixed[p]) mg2 += (c->mass[p] * c->desc.g_accel) * (c->mass[p] * c->desc.g_accel);
+                c->res_tol2 = c->res_tol * c->res_tol * mg2;
+                c->rc = dev_alloc<ResCtl>(c, 1);
+                hip_check(hipMemsetAsync(c->rc, 0, sizeof(ResCtl), c->stream), "hipMemsetAsync");
+                c->sargs.rc = c->rc;
+                c->sargs.check_every = c->res_check_every;
+                c->sargs.tol2 = c->res_tol2;
+            }
             c->sargs.num_colors = c->plan.num_colors;
             c->sargs.dt = (float)c->desc.dt;
             c->sargs.g = (float)c->desc.g_accel;
@@ -1079,14 +1139,14 @@ int avbd_build_graph(avbd_ctx* c) {
         }
         c->built = true;
 
-        const uint32_t n_solves = c->desc.solver_iters * c->plan.num_colors;
+        const uint32_t n_solves = c->iters() * c->plan.num_colors;
         c->events.resize(2 * n_solves);
         hip_check(hipEventCreate(&c->ev_lb0), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_lb1), "hipEventCreate");
         for (auto& ev : c->events) hip_check(hipEventCreate(&ev), "hipEventCreate");
 
         if (c->desc.lanes_per_vertex == 0 || c->desc.block_size == 0) autotune(c);
-        if (!(c->desc.flags & AVBD_FLAG_NO_PERSISTENT)) {
+        if ((c->desc.flags & AVBD_FLAG_PERSISTENT) && !(c->desc.flags & AVBD_FLAG_NO_PERSISTENT)) {
             c->pgrid = persistent_grid(c->device, c->cfg.lanes, c->damping);
             c->persistent = c->pgrid >= 8;
         }
@@ -1133,7 +1193,7 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
         }
         hip_check(hipEventRecord(c->ev_b, c->stream), "hipEventRecord");
         c->steps_taken += n_steps;
-        c->vertex_updates += (uint64_t)n_steps * c->desc.solver_iters * (c->d.n_items + c->d.n_oitems);
+        if (!c->rc) c->vertex_updates += (uint64_t)n_steps * c->desc.solver_iters * (c->d.n_items + c->d.n_oitems);
     });
 }
 
@@ -1401,7 +1461,17 @@ int avbd_get_stats(avbd_ctx* c, avbd_stats* out) {
         st.num_tets = c->d.n_tets;
         st.num_ground = c->d.n_ground;
         st.steps_taken = c->steps_taken;
-        st.vertex_updates = c->vertex_updates;
+        if (c->rc) {
+            ResCtl rc{};
+            hip_check(hipMemcpy(&rc, c->rc, sizeof(ResCtl), hipMemcpyDeviceToHost), "D2H");
+            st.iterations_total = rc.total_iters;
+            st.last_step_iterations = rc.last_iters;
+            st.vertex_updates = rc.total_iters * (uint64_t)(c->d.n_items + c->d.n_oitems);
+        } else {
+            st.iterations_total = c->steps_taken * c->desc.solver_iters;
+            st.last_step_iterations = c->steps_taken ? c->desc.solver_iters : 0;
+            st.vertex_updates = c->vertex_updates;
+        }
         st.last_step_ms = c->last_step_ms;
         *out = st;
     });

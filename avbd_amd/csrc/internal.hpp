@@ -175,8 +175,23 @@ struct GridBarrier {
     uint32_t pad;
 };
 
+// Residual-targeted iteration: per step, the relative force residual is
+// checked every `check_every` iterations and the iteration loop ends once
+// ||r||^2 <= tol^2 ||M g||^2.  acc[k] holds the ||r||^2 of check k of the
+// current step (zeroed at step start).
+constexpr uint32_t kMaxResidualChecks = 64;
+struct ResCtl {
+    double acc[kMaxResidualChecks];
+    uint32_t done;         // set when the current step has converged
+    uint32_t last_iters;   // iterations used by the last step
+    unsigned long long total_iters;
+};
+
 struct StepArgs {
     DevState d;
+    ResCtl* rc = nullptr;         // null: fixed iteration count
+    uint32_t check_every = 0;
+    double tol2 = 0;              // tol^2 * ||M g||^2
     const uint32_t* color_offsets = nullptr;      // num_colors + 1
     const uint32_t* conflict_offsets = nullptr;   // num_colors + 1
     const uint32_t* conflicts = nullptr;
@@ -193,12 +208,21 @@ uint32_t persistent_grid(int device, uint32_t lanes, bool damping);
 void launch_step_persistent(hipStream_t s, const StepArgs& a, uint32_t lanes, bool damping, uint32_t grid);
 
 void launch_eprev(hipStream_t s, const DevState& d, float dt);
-void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t item_begin, uint32_t item_count, float dt);
+void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t item_begin, uint32_t item_count, float dt,
+                         const uint32_t* done = nullptr);
 void launch_step_begin(hipStream_t s, const DevState& d, float dt, float g);
 void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t item_begin,
-                        uint32_t item_count, bool damping, float dt, float4* x_out);
-void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count);
-void launch_cheb_dual(hipStream_t s, const DevState& d, float omega);
+                        uint32_t item_count, bool damping, float dt, float4* x_out, const uint32_t* done = nullptr);
+void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count,
+                   const uint32_t* done = nullptr);
+void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const uint32_t* done = nullptr);
+// residual-targeted iteration (per-phase path): reset at step start, one
+// check (residual pass + decision) after iteration `iters_done`, and the
+// bookkeeping at step end
+void launch_residual_reset(hipStream_t s, ResCtl* rc);
+void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
+                           ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2);
+void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters);
 void launch_velocity(hipStream_t s, const DevState& d, float dt);
 // plain streaming copy of n_float4 16-byte words (HBM counter calibration)
 void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n_float4);

@@ -225,7 +225,9 @@ __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restri
                                                            const TetDamp* __restrict__ damp, float4* x,
                                                            const float4* __restrict__ y,
                                                            const GroundRec* __restrict__ ground, float4* x_out,
-                                                           float4* x_buf, float inv_dt, float inv_dt2) {
+                                                           float4* x_buf, float inv_dt, float inv_dt2,
+                                                           const uint32_t* __restrict__ done) {
+    if (done && *done) return;   // residual target reached earlier in this step
     const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L;
     const uint32_t lane = tid % L;
@@ -319,7 +321,9 @@ __device__ __forceinline__ void commit_one(const DevState& d, uint32_t p) {
     if (r >= 0) d.q[r] = d.q_buf[r];
 }
 
-__global__ void commit_kernel(DevState d, const uint32_t* __restrict__ list, uint32_t count) {
+__global__ void commit_kernel(DevState d, const uint32_t* __restrict__ list, uint32_t count,
+                              const uint32_t* __restrict__ done) {
+    if (done && *done) return;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < count) commit_one(d, list[k]);
 }
@@ -372,7 +376,8 @@ __device__ __forceinline__ void cheb_dual_one(const DevState& d, uint32_t p, flo
     if (r >= 0) rigid::records_dual(d, make_float3(xp.x, xp.y, xp.z), q, (uint32_t)r);
 }
 
-__global__ void cheb_dual_kernel(DevState d, float omega) {
+__global__ void cheb_dual_kernel(DevState d, float omega, const uint32_t* __restrict__ done) {
+    if (done && *done) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < d.n) cheb_dual_one(d, p, omega);
 }
@@ -394,9 +399,89 @@ __global__ void velocity_kernel(DevState d, float inv_dt) {
 
 // Oriented vertices of one colour, one thread each.
 __global__ __launch_bounds__(256) void solve_ocolor_kernel(DevState d, uint32_t begin, uint32_t count,
-                                                           float inv_dt2) {
+                                                           float inv_dt2, const uint32_t* __restrict__ done) {
+    if (done && *done) return;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < count) rigid::solve_oitem(d, d.oitems[begin + k], inv_dt2);
+}
+
+// ---- force residual (residual-targeted iteration) --------------------------
+// r_v = m/dt^2 (x_v - y_v) + dE/dx_v for one positional work item: the
+// gradient of the vertex solve at the current iterate; |r_v|^2 is returned by
+// lane 0 of the group (0 elsewhere).  Every lane of the group must call it.
+template <int L, bool DAMP>
+__device__ __forceinline__ float residual_item(const VItem& it, uint32_t lane, const DevState& d, float inv_dt,
+                                               float inv_dt2) {
+    const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
+    float3 g = make_float3(0.f, 0.f, 0.f);
+    Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint32_t j = lane; j < cnt; j += L) nh_incidence<DAMP>(d.x, d.tets, d.damp, d.inc[it.inc_begin + j], inv_dt, g, h);
+    g.x = group_sum<L>(g.x);
+    g.y = group_sum<L>(g.y);
+    g.z = group_sum<L>(g.z);
+    if (lane != 0) return 0.f;
+    const float4 xv = d.x[it.v], yv = d.y[it.v];
+    if (it.ground >= 0) ground_accumulate(d.ground[it.ground], f3(xv), g, h);
+    const float md = xv.w * inv_dt2;
+    const float rx = md * (xv.x - yv.x) + g.x, ry = md * (xv.y - yv.y) + g.y, rz = md * (xv.z - yv.z) + g.z;
+    return rx * rx + ry * ry + rz * rz;
+}
+
+// Positional block of the residual of an oriented work item.
+__device__ __forceinline__ float residual_oitem(const DevState& d, const OItem& it, float inv_dt2) {
+    const float4 xv = d.x[it.v];
+    rot::Vec6 G;
+    rot::Sym6 H;
+    rot::vec6_zero(G);
+    rot::sym6_zero(H);
+    rigid::accumulate6(d, it, rigid::xyz(xv), d.q[it.rot], G, H);
+    const float4 yv = d.y[it.v];
+    const float md = xv.w * inv_dt2;
+    const float rx = md * (xv.x - yv.x) + G.v[0], ry = md * (xv.y - yv.y) + G.v[1], rz = md * (xv.z - yv.z) + G.v[2];
+    return rx * rx + ry * ry + rz * rz;
+}
+
+// Sum a per-thread double over the block (256 threads) into *dst with one atomic.
+__device__ __forceinline__ void block_atomic_sum(double v, double* dst, double* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(dst, red[0]);
+}
+
+template <int L, bool DAMP>
+__global__ __launch_bounds__(256) void residual_pass_kernel(DevState d, ResCtl* rc, uint32_t check, float inv_dt,
+                                                            float inv_dt2) {
+    if (rc->done) return;
+    __shared__ double red[256];
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = tid / L, lane = tid % L;
+    double part = 0;
+    if (item < d.n_items) part = residual_item<L, DAMP>(d.items[item], lane, d, inv_dt, inv_dt2);
+    if (tid < d.n_oitems) part += residual_oitem(d, d.oitems[tid], inv_dt2);
+    block_atomic_sum(part, &rc->acc[check], red);
+}
+
+__global__ void residual_decide_kernel(ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2) {
+    if (threadIdx.x == 0 && !rc->done && rc->acc[check] <= tol2) {
+        rc->done = 1;
+        rc->last_iters = iters_done;
+    }
+}
+
+__global__ void residual_reset_kernel(ResCtl* rc) {
+    if (threadIdx.x < kMaxResidualChecks) rc->acc[threadIdx.x] = 0.0;
+    if (threadIdx.x == 0) rc->done = 0;
+}
+
+__global__ void residual_finish_kernel(ResCtl* rc, uint32_t max_iters) {
+    if (threadIdx.x == 0) {
+        if (!rc->done) rc->last_iters = max_iters;
+        rc->total_iters += rc->last_iters;
+    }
 }
 
 // ---- persistent step kernel ------------------------------------------------
@@ -453,11 +538,18 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
     uint32_t gen = gen_s;
     const float inv_dt = 1.0f / a.dt, inv_dt2 = inv_dt * inv_dt;
 
+    __shared__ double red_s[256];
+    if (a.rc && blockIdx.x == 0) {
+        // read by other blocks only after at least one grid barrier
+        if (threadIdx.x < kMaxResidualChecks) a.rc->acc[threadIdx.x] = 0.0;
+        if (threadIdx.x == 0) a.rc->done = 0;
+    }
     if constexpr (DAMP) {
         for (uint32_t t = gtid; t < d.n_tets; t += gthreads) eprev_one(t, d.tets, d.damp, d.x_prev);
     }
     for (uint32_t p = gtid; p < d.n; p += gthreads) step_begin_one(d, p, a.dt, a.g);
     if (!grid_sync(a.bar, G, gen, &ok_s)) return;
+    uint32_t iters_used = a.iters;
 
     const uint32_t groups = B / L, lane = threadIdx.x % L, grp = threadIdx.x / L;
     for (uint32_t it = 0; it < a.iters; it++) {
@@ -485,6 +577,32 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
         const float omega = a.omega[it];
         for (uint32_t p = gtid; p < d.n; p += gthreads) cheb_dual_one(d, p, omega);
         if (!grid_sync(a.bar, G, gen, &ok_s)) return;
+        if (a.rc && (it + 1) % a.check_every == 0 && it + 1 < a.iters) {
+            // relative force residual at the current iterate: contiguous share of
+            // all work items per logical block, one atomic per block, a barrier,
+            // then every block reads the same total and takes the same branch
+            const uint32_t check = (it + 1) / a.check_every - 1;
+            double part = 0;
+            const uint32_t per = (d.n_items + G - 1) / G;
+            const uint32_t lo = min(d.n_items, lb * per), hi = min(d.n_items, lo + per);
+            for (uint32_t k = lo + grp; k < hi; k += groups)
+                part += residual_item<L, DAMP>(d.items[k], lane, d, inv_dt, inv_dt2);
+            const uint32_t oper = (d.n_oitems + G - 1) / G;
+            const uint32_t olo = min(d.n_oitems, lb * oper), ohi = min(d.n_oitems, olo + oper);
+            for (uint32_t k = olo + threadIdx.x; k < ohi; k += B) part += residual_oitem(d, d.oitems[k], inv_dt2);
+            block_atomic_sum(part, &a.rc->acc[check], red_s);
+            if (!grid_sync(a.bar, G, gen, &ok_s)) return;
+            const double total = __hip_atomic_load(&a.rc->acc[check], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (total <= a.tol2) {
+                iters_used = it + 1;
+                break;
+            }
+        }
+    }
+    if (a.rc && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.rc->done = iters_used < a.iters ? 1u : 0u;
+        a.rc->last_iters = iters_used;
+        a.rc->total_iters += iters_used;
     }
     for (uint32_t p = gtid; p < d.n; p += gthreads) velocity_one(d, p, inv_dt);
 }
@@ -641,11 +759,19 @@ inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 template <int L, bool DAMP>
 void solve_dispatch(hipStream_t s, const DevState& d, uint32_t block, uint32_t begin, uint32_t count, float dt,
-                    float4* x_out) {
+                    float4* x_out, const uint32_t* done) {
     const uint32_t threads = count * L;
     hipLaunchKernelGGL((solve_color_kernel<L, DAMP>), dim3(blocks_for(threads, block)), dim3(block), 0, s,
                        d.items + begin, count, d.inc, d.tets, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
-                       1.0f / dt, 1.0f / (dt * dt));
+                       1.0f / dt, 1.0f / (dt * dt), done);
+}
+
+template <int L, bool DAMP>
+void residual_dispatch(hipStream_t s, const DevState& d, float dt, ResCtl* rc, uint32_t check) {
+    const uint32_t threads = max(d.n_items * L, d.n_oitems);
+    if (threads == 0) return;
+    hipLaunchKernelGGL((residual_pass_kernel<L, DAMP>), dim3(blocks_for(threads, 256)), dim3(256), 0, s, d, rc,
+                       check, 1.0f / dt, 1.0f / (dt * dt));
 }
 
 template <int L, bool DAMP>
@@ -671,19 +797,46 @@ void launch_step_begin(hipStream_t s, const DevState& d, float dt, float g) {
     hipLaunchKernelGGL(step_begin_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, dt, g);
 }
 
-void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t begin, uint32_t count, float dt) {
+void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t begin, uint32_t count, float dt,
+                         const uint32_t* done) {
     if (count == 0) return;
     hipLaunchKernelGGL(solve_ocolor_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, begin, count,
-                       1.0f / (dt * dt));
+                       1.0f / (dt * dt), done);
+}
+
+void launch_residual_reset(hipStream_t s, ResCtl* rc) {
+    hipLaunchKernelGGL(residual_reset_kernel, dim3(1), dim3(64), 0, s, rc);
+}
+
+void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
+                           ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2) {
+#define AVBD_RES_CASE(LL)                                                   \
+    case LL:                                                                 \
+        if (damping) residual_dispatch<LL, true>(s, d, dt, rc, check);       \
+        else residual_dispatch<LL, false>(s, d, dt, rc, check);              \
+        break;
+    switch (cfg.lanes) {
+        AVBD_RES_CASE(4)
+        AVBD_RES_CASE(8)
+        AVBD_RES_CASE(16)
+        AVBD_RES_CASE(32)
+        default: break;
+    }
+#undef AVBD_RES_CASE
+    hipLaunchKernelGGL(residual_decide_kernel, dim3(1), dim3(64), 0, s, rc, check, iters_done, tol2);
+}
+
+void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters) {
+    hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(64), 0, s, rc, max_iters);
 }
 
 void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
-                        bool damping, float dt, float4* x_out) {
+                        bool damping, float dt, float4* x_out, const uint32_t* done) {
     if (count == 0) return;
-#define AVBD_SOLVE_CASE(LL)                                                         \
-    case LL:                                                                         \
-        if (damping) solve_dispatch<LL, true>(s, d, cfg.block, begin, count, dt, x_out); \
-        else solve_dispatch<LL, false>(s, d, cfg.block, begin, count, dt, x_out);      \
+#define AVBD_SOLVE_CASE(LL)                                                                \
+    case LL:                                                                                \
+        if (damping) solve_dispatch<LL, true>(s, d, cfg.block, begin, count, dt, x_out, done); \
+        else solve_dispatch<LL, false>(s, d, cfg.block, begin, count, dt, x_out, done);      \
         break;
     switch (cfg.lanes) {
         AVBD_SOLVE_CASE(4)
@@ -695,14 +848,14 @@ void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, 
 #undef AVBD_SOLVE_CASE
 }
 
-void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count) {
+void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count, const uint32_t* done) {
     if (count == 0) return;
-    hipLaunchKernelGGL(commit_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, list, count);
+    hipLaunchKernelGGL(commit_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, list, count, done);
 }
 
-void launch_cheb_dual(hipStream_t s, const DevState& d, float omega) {
+void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const uint32_t* done) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(cheb_dual_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, omega);
+    hipLaunchKernelGGL(cheb_dual_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, omega, done);
 }
 
 void launch_velocity(hipStream_t s, const DevState& d, float dt) {

@@ -45,7 +45,7 @@ class SimulationContext:
     def __init__(self, dt: float, g_accel: float = 9.81, solver_iters: int = 20,
                  iter_acceleration: float = 0.95, particle_capacity: int = 0, energy_capacity: int = 0,
                  device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0, use_graph: bool = True,
-                 serial_broadphase: bool = False, persistent: bool = True):
+                 serial_broadphase: bool = False, persistent: bool = False):
         desc = _lib.SceneDesc()
         desc.particle_capacity = particle_capacity
         desc.energy_capacity = energy_capacity
@@ -54,7 +54,7 @@ class SimulationContext:
         desc.solver_iters = solver_iters
         desc.iter_acceleration = iter_acceleration
         desc.flags = ((0 if use_graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_SERIAL_BROADPHASE if serial_broadphase else 0)
-                      | (0 if persistent else _lib.FLAG_NO_PERSISTENT))
+                      | (_lib.FLAG_PERSISTENT if persistent else 0))
         desc.device = device
         desc.lanes_per_vertex = lanes_per_vertex
         desc.block_size = block_size
@@ -198,6 +198,18 @@ class SimulationContext:
     def detect_collisions(self) -> None:
         self._chk(lib.avbd_detect_collisions(self.handle))
 
+    def set_residual_target(self, rel_tol: float, max_iters: int, check_every: int = 5) -> None:
+        """Iterate each step until ||r|| / ||M g|| <= rel_tol (checked every
+        check_every iterations) or max_iters; rel_tol = 0: fixed count."""
+        self._chk(lib.avbd_set_residual_target(self.handle, rel_tol, max_iters, check_every))
+
+    def residual_history(self) -> np.ndarray:
+        """Relative residuals at the checks of the last step (residual-targeted mode)."""
+        buf = np.empty(64)
+        n = C.c_uint32()
+        self._chk(lib.avbd_get_residual_history(self.handle, _dptr(buf), 64, C.byref(n)))
+        return buf[:min(64, n.value)].copy()
+
     def set_scenes(self, offsets: Sequence[int]) -> None:
         off = np.ascontiguousarray(offsets, dtype=np.uint32)
         self._chk(lib.avbd_set_scenes(self.handle, off.shape[0] - 1, _uptr(off)))
@@ -306,7 +318,8 @@ class Simulation:
     a Scene description, runs setup() (adjacency + colouring) and time steps."""
 
     def __init__(self, scene: Scene, device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0,
-                 use_graph: bool = True, serial_broadphase: bool = False, persistent: bool = True):
+                 use_graph: bool = True, serial_broadphase: bool = False, persistent: bool = False,
+                 residual_target: Optional[tuple] = None):
         self.scene = scene
         self.ctx = SimulationContext(dt=scene.dt, g_accel=scene.g, solver_iters=scene.iters,
                                      iter_acceleration=scene.rho, device=device,
@@ -314,6 +327,8 @@ class Simulation:
                                      use_graph=use_graph, serial_broadphase=serial_broadphase,
                                      persistent=persistent)
         self.solver = VBDSolver(self.ctx, scene.iters, scene.rho)
+        if residual_target is not None:
+            self.ctx.set_residual_target(*residual_target)
         self.time = 0.0
         self.offsets: List[int] = []
         self._setup = False

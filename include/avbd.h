@@ -60,8 +60,12 @@ typedef enum {
 
 /* ---- context creation flags ---- */
 #define AVBD_FLAG_NO_GRAPH 1u    /* launch kernels eagerly instead of replaying a hipGraph */
-#define AVBD_FLAG_NO_PERSISTENT 4u  /* one kernel launch per solver phase instead of one
-                                       persistent launch per step with grid barriers */
+#define AVBD_FLAG_NO_PERSISTENT 4u  /* one kernel launch per solver phase (the default) */
+#define AVBD_FLAG_PERSISTENT 8u     /* one persistent launch per step with grid barriers between
+                                       phases.  Opt-in: every barrier needs agent-scope release /
+                                       acquire fences, which write back and invalidate the L2 of
+                                       every XCD, so on MI355X it is slower than one launch per
+                                       phase replayed from a hipGraph */
 #define AVBD_FLAG_SERIAL_BROADPHASE 2u  /* run the triangle broad phase on the solve stream
                                            instead of concurrently on a second stream */
 
@@ -165,6 +169,9 @@ typedef struct {
     uint64_t steps_taken;
     uint64_t vertex_updates; /* cumulative vertex-block solves                 */
     double last_step_ms;     /* device time of the last avbd_step() call        */
+    uint64_t iterations_total;      /* Gauss-Seidel iterations over all steps   */
+    uint32_t last_step_iterations;  /* iterations of the last step              */
+    uint32_t pad_;
 } avbd_stats;
 
 /* Context lifetime -------------------------------------------------------- */
@@ -228,6 +235,18 @@ int avbd_set_fixed(avbd_ctx* ctx, uint32_t first, uint32_t n, const uint8_t* fix
  * particles among [first, first+n), also taken as their previous-step values
  * (rows of positional particles are ignored; NULL = unchanged). */
 int avbd_set_rotation_state(avbd_ctx* ctx, uint32_t first, uint32_t n, const double* q, const double* w);
+
+/* Residual-targeted iteration (new; the reference runs a fixed solver_iters):
+ * every step iterates until the relative force residual
+ * ||r||_2 / ||M g||_2 (free particles; r = m/dt^2 (x - y) + dE/dx at the
+ * current iterate) is <= rel_tol, checked every check_every iterations, or
+ * until max_iters iterations.  rel_tol = 0 restores the fixed count.  Call
+ * before avbd_build_graph.  At most 64 checks per step. */
+int avbd_set_residual_target(avbd_ctx* ctx, double rel_tol, uint32_t max_iters, uint32_t check_every);
+/* Relative residuals measured at the checks of the last step (residual-
+ * targeted mode): rel[k] after iteration (k+1)*check_every; *count = checks
+ * that ran (at most cap are written). */
+int avbd_get_residual_history(avbd_ctx* ctx, double* rel, uint32_t cap, uint32_t* count);
 
 /* Scene partition for batching: scene s owns particles
  * [particle_offsets[s], particle_offsets[s+1]); each scene is coloured on its

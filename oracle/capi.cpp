@@ -37,6 +37,16 @@ void* oracle_new(double dt, double g, uint32_t iters, double rho) {
 
 void oracle_free(void* h) { delete static_cast<Simulation*>(h); }
 
+// Residual-targeted iteration (tol = 0: fixed iteration count).
+void oracle_set_residual_target(void* h, double tol, uint32_t max_iters, uint32_t check_every) {
+    auto* s = static_cast<Simulation*>(h);
+    s->params.tol = tol;
+    s->params.max_iters = max_iters;
+    s->params.check_every = check_every;
+}
+uint32_t oracle_last_iters(void* h) { return static_cast<Simulation*>(h)->last_iters; }
+uint64_t oracle_total_iters(void* h) { return static_cast<Simulation*>(h)->total_iters; }
+
 int64_t oracle_add_tet_mesh(void* h, uint32_t nv, const double* rest_x, uint32_t nt, const uint32_t* tets,
                             double E, double nu, double density, double kd, int with_ground,
                             double mu_s, double mu_k) {

@@ -125,6 +125,10 @@ struct Params {
     Real g = 9.81;
     uint32_t iters = 20;
     Real rho = 0.95;   // iter-acceleration (Chebyshev)
+    // residual-targeted iteration (new; 0 = fixed `iters`): iterate until
+    // ||r|| <= tol ||M g||, checked every check_every iterations, at most max_iters
+    Real tol = 0;
+    uint32_t max_iters = 0, check_every = 0;
 };
 
 struct Stats {

@@ -23,6 +23,9 @@ _SIG = {
     "oracle_last_error": (C.c_char_p, []),
     "oracle_new": (C.c_void_p, [C.c_double, C.c_double, C.c_uint32, C.c_double]),
     "oracle_free": (None, [C.c_void_p]),
+    "oracle_set_residual_target": (None, [C.c_void_p, C.c_double, C.c_uint32, C.c_uint32]),
+    "oracle_last_iters": (C.c_uint32, [C.c_void_p]),
+    "oracle_total_iters": (C.c_uint64, [C.c_void_p]),
     "oracle_add_tet_mesh": (C.c_int64, [C.c_void_p, C.c_uint32, _D, C.c_uint32, _U32, C.c_double, C.c_double,
                                         C.c_double, C.c_double, C.c_int, C.c_double, C.c_double]),
     "oracle_add_particles": (C.c_uint32, [C.c_void_p, C.c_uint32, _D, _D]),
@@ -145,6 +148,15 @@ class OracleSim:
 
     def build_graph(self):
         _chk(lib.oracle_build_graph(self.h))
+
+    def set_residual_target(self, tol, max_iters, check_every):
+        lib.oracle_set_residual_target(self.h, tol, max_iters, check_every)
+
+    def last_iters(self) -> int:
+        return int(lib.oracle_last_iters(self.h))
+
+    def total_iters(self) -> int:
+        return int(lib.oracle_total_iters(self.h))
 
     def step(self, n=1, threads=1):
         _chk(lib.oracle_step(self.h, n, threads))

@@ -58,6 +58,11 @@ public:
     void step_mt(uint32_t n_threads, uint32_t n_steps);
 
     Stats stats() const;
+    // sum over free particles in [lo, hi) of |m/dt^2 (x - y) + G|^2 (positional block)
+    Real residual2(uint32_t lo, uint32_t hi) const;
+    Real mg2() const;
+    uint32_t last_iters = 0;
+    uint64_t total_iters = 0;
 
     void accumulate_vertex(uint32_t p, V3& G, M3& H) const;
     void accumulate_vertex6(uint32_t p, V6& G, M6& H) const;

@@ -682,14 +682,52 @@ void Simulation::step() {
     phase_after_timestep_rigid(0, n_rigid);
     phase_inertial(0, n);
     Real omega = 1;
-    for (uint32_t i = 0; i < params.iters; i++) {
+    const bool target = params.tol > 0;
+    const uint32_t iters = target ? params.max_iters : params.iters;
+    const Real tol2 = target ? params.tol * params.tol * mg2() : 0;
+    uint32_t used = iters;
+    for (uint32_t i = 0; i < iters; i++) {
         omega = omega_at(i, params.rho, omega);
         for (uint32_t c = 0; c < coloring.num_colors; c++) phase_color(c, 0, coloring.color_counts[c]);
         phase_chebyshev(omega, 0, n);
         phase_dual(0, (uint32_t)ground.size());
         phase_dual_rigid(0, n_rigid);
+        if (target && (i + 1) % params.check_every == 0 && i + 1 < iters && residual2(0, n) <= tol2) {
+            used = i + 1;
+            break;
+        }
     }
+    last_iters = used;
+    total_iters += used;
     phase_velocity(0, n);
+}
+
+Real Simulation::residual2(uint32_t lo, uint32_t hi) const {
+    const Real dt = params.dt;
+    Real r2 = 0;
+    for (uint32_t p = lo; p < hi; p++) {
+        if (fixed[p]) continue;
+        V3 G;
+        if (oriented(p)) {
+            V6 G6;
+            M6 H6;
+            accumulate_vertex6(p, G6, H6);
+            G = V3(G6.v[0], G6.v[1], G6.v[2]);
+        } else {
+            M3 H;
+            accumulate_vertex(p, G, H);
+        }
+        const V3 r = (mass[p] / (dt * dt)) * (x[p] - y[p]) + G;
+        r2 += dot(r, r);
+    }
+    return r2;
+}
+
+Real Simulation::mg2() const {
+    Real s = 0;
+    for (size_t p = 0; p < x.size(); p++)
+        if (!fixed[p]) s += (mass[p] * params.g) * (mass[p] * params.g);
+    return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -725,6 +763,10 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
     }
     SpinBarrier bar(nthreads);
     std::vector<uint64_t> cand(nthreads, 0);
+    std::vector<Real> part(nthreads, 0);
+    const bool target = params.tol > 0;
+    const uint32_t iters = target ? params.max_iters : params.iters;
+    const Real tol2 = target ? params.tol * params.tol * mg2() : 0;
     const uint32_t n_rigid = (uint32_t)(rground.size() + joints.size());
     auto worker = [&](uint32_t w) {
         uint32_t n = (uint32_t)x.size();
@@ -756,7 +798,8 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
             phase_inertial(lo, hi);
             bar.arrive_and_wait();
             Real omega = 1;
-            for (uint32_t i = 0; i < params.iters; i++) {
+            uint32_t used = iters;
+            for (uint32_t i = 0; i < iters; i++) {
                 omega = omega_at(i, params.rho, omega);
                 for (uint32_t c = 0; c < coloring.num_colors; c++) {
                     range(coloring.color_counts[c], w, nthreads, lo, hi);
@@ -776,6 +819,22 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
                     phase_dual_rigid(lo, hi);
                     bar.arrive_and_wait();
                 }
+                if (target && (i + 1) % params.check_every == 0 && i + 1 < iters) {
+                    range(n, w, nthreads, lo, hi);
+                    part[w] = residual2(lo, hi);
+                    bar.arrive_and_wait();
+                    Real tot = 0;
+                    for (Real v : part) tot += v;   // same order on every worker
+                    bar.arrive_and_wait();          // partials are read before reuse
+                    if (tot <= tol2) {
+                        used = i + 1;
+                        break;
+                    }
+                }
+            }
+            if (w == 0) {
+                last_iters = used;
+                total_iters += used;
             }
             range(n, w, nthreads, lo, hi);
             phase_velocity(lo, hi);

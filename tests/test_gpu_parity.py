@@ -236,3 +236,44 @@ def test_persistent_step_kernel_matches_phase_launches(damping):
         xa, va = a.ctx.state()
         xb, vb = b.ctx.state()
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
+def test_residual_target_mode(oracle):
+    """Residual-targeted iteration on the device: the step stops at the first
+    check whose relative residual is <= tol; the state it leaves satisfies the
+    tolerance (FP64 oracle evaluation of the same positions allows 10 % slack
+    for FP32 rounding), the persistent kernel and the per-phase launches take
+    the same decisions and produce the same bits, and one step equals the
+    oracle's fixed-count step with the iteration count the device used."""
+    banner("residual-targeted iteration")
+    from avbd_amd.sim import Simulation
+    sc = scenes.hanging_cube(n=6)
+    tol, K = 3e-3, 5
+    a = Simulation(sc, lanes_per_vertex=8, block_size=256, persistent=True, residual_target=(tol, 300, K))
+    b = Simulation(sc, lanes_per_vertex=8, block_size=256, persistent=False, residual_target=(tol, 300, K))
+    a.setup()
+    b.setup()
+    a.update(1)
+    b.update(1)
+    sa, sb = a.ctx.stats(), b.ctx.stats()
+    banner(f"  iterations persistent {sa.last_step_iterations} per-phase {sb.last_step_iterations}; "
+           f"residual {sa.residual_rel:.3e}")
+    assert sa.last_step_iterations == sb.last_step_iterations
+    assert np.array_equal(a.positions(), b.positions())
+    used = sa.last_step_iterations
+    assert used % K == 0 and used < 300
+    assert sa.residual_rel <= tol * 1.1
+    # the oracle with the same number of iterations
+    sc2 = scenes.hanging_cube(n=6)
+    sc2.iters = used
+    o = oracle.from_scene(sc2)
+    o.step(1)
+    xo = o.positions()
+    err = np.abs(a.positions() - xo).max() / bbox(xo)
+    banner(f"  vs oracle with {used} iterations: rel err {err:.3e}; oracle residual {o.stats()['residual_rel']:.3e}")
+    assert err <= 1e-5
+    a.update(4)
+    b.update(4)
+    sa, sb = a.ctx.stats(), b.ctx.stats()
+    assert sa.iterations_total == sb.iterations_total and np.array_equal(a.positions(), b.positions())
+    assert sa.vertex_updates == sa.iterations_total * int((sc.meshes[0].fixed == 0).sum())

@@ -151,8 +151,13 @@ def test_oriented_persistent_matches_phase_launches():
 
 
 def test_oriented_stats_match_oracle(oracle):
+    """Residual and incremental potential of a soft rod.  (On a stiff rod the
+    FP32 state itself bounds the residual: with axial stiffness EA/L the
+    rounding of positions alone, ~1e-8 m, gives forces of EA/L * 1e-8 that are
+    comparable to a node's weight; the soft rod keeps that floor far below
+    the compared values.)"""
     banner("oriented residual / incremental potential vs oracle")
-    sc = scenes.rod_cantilever()
+    sc = scenes.rod_cantilever(E=1e5, iters=20, dt=1.0 / 240.0)
     g = gpu_sim(sc)
     o = oracle.from_scene(sc)
     g.update(3)

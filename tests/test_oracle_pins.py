@@ -321,3 +321,42 @@ def test_hanging_cube_static_equilibrium(oracle):
     st = o.stats()
     assert st["residual_rel"] < 2e-2
     assert np.abs(o.velocities()).max() < 0.05
+
+
+def test_residual_target_stops_at_tolerance(oracle):
+    """Residual-targeted iteration: a step ends on the first check whose
+    relative residual is <= tol -- the state it leaves satisfies the tolerance
+    and a fixed-count step K iterations shorter does not -- and the SPMD step
+    takes the same decisions."""
+    from avbd_amd import scenes
+    sc = scenes.hanging_cube(n=5)
+    m = sc.meshes[0]
+    tol, K = 2e-3, 5
+    a = oracle.from_scene(sc)
+    a.set_residual_target(tol, 400, K)
+    b = oracle.from_scene(sc)
+    b.set_residual_target(tol, 400, K)
+    a.step(2, 1)
+    b.step(2, 4)
+    assert a.last_iters() == b.last_iters() and a.total_iters() == b.total_iters()
+    assert np.array_equal(a.positions(), b.positions())
+
+    first = oracle.from_scene(sc)
+    first.set_residual_target(tol, 400, K)
+    first.step(1)
+    used = first.last_iters()
+    assert used % K == 0 and K < used < 400
+
+    def fixed_count(iters):
+        h = oracle.OracleSim(sc.dt, sc.g, iters, sc.rho)
+        h.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
+        h.set_fixed(m.fixed)
+        h.set_positions(m.x0, also_prev=True)
+        h.build_graph()
+        h.step(1)
+        return h
+
+    h = fixed_count(used)
+    assert np.array_equal(h.positions(), first.positions())
+    assert h.stats()["residual_rel"] <= tol
+    assert fixed_count(used - K).stats()["residual_rel"] > tol
