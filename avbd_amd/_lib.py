@@ -150,6 +150,8 @@ class Stats(C.Structure):
         ("iterations_total", C.c_uint64),
         ("last_step_iterations", C.c_uint32),
         ("pad_", C.c_uint32),
+        ("warm_start_predictions", C.c_uint64),
+        ("warm_start_fallbacks", C.c_uint64),
     ]
 
 
@@ -157,6 +159,7 @@ _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _U32 = C.POINTER(C.c_uint32)
 _U8 = C.POINTER(C.c_uint8)
+_F = C.POINTER(C.c_float)
 
 # name -> (restype, argtypes); every symbol declared in include/avbd.h
 SIGNATURES = {
@@ -172,6 +175,17 @@ SIGNATURES = {
     "avbd_set_rotation_state": (C.c_int, [_P, C.c_uint32, C.c_uint32, _D, _D]),
     "avbd_set_residual_target": (C.c_int, [_P, C.c_double, C.c_uint32, C.c_uint32]),
     "avbd_get_residual_history": (C.c_int, [_P, _D, C.c_uint32, _U32]),
+    "avbd_mlp_param_count": (C.c_uint64, []),
+    "avbd_mlp_init": (C.c_int, [_P, C.c_uint64]),
+    "avbd_mlp_set_params": (C.c_int, [_P, _F, C.c_uint64]),
+    "avbd_mlp_get_params": (C.c_int, [_P, _F, C.c_uint64]),
+    "avbd_mlp_set_normalization": (C.c_int, [_P, _F]),
+    "avbd_mlp_forward_backward": (C.c_int, [_P, _F, _F, C.c_uint32, _F, C.c_uint64, _F]),
+    "avbd_mlp_predict": (C.c_int, [_P, _F, C.c_uint32, _F]),
+    "avbd_mlp_train": (C.c_int, [_P, _F, _F, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _F]),
+    "avbd_record_steps": (C.c_int, [_P, C.c_uint32, _F, _F, C.c_uint32, _U32]),
+    "avbd_set_warm_start": (C.c_int, [_P, C.c_int]),
+    "avbd_mlp_gemm_test": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, _F, _F, _F]),
     "avbd_download_rotations": (C.c_int, [_P, _D, _D]),
     "avbd_add_tet_mesh": (C.c_int, [_P, C.c_uint32, _D, C.c_uint32, _U32, C.c_double, C.c_double, C.c_double,
                                     C.c_double, C.c_int, C.c_double, C.c_double, _U32]),

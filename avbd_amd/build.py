@@ -15,7 +15,7 @@ OUT_DIR = PKG / "_build"
 LIB = OUT_DIR / "libavbd_amd.so"
 INCLUDE = PKG.parent / "include"
 
-SOURCES = ["api.cpp", "host_graph.cpp", "vbd_kernels.hip", "lbvh_kernels.hip"]
+SOURCES = ["api.cpp", "host_graph.cpp", "mlp_host.cpp", "vbd_kernels.hip", "lbvh_kernels.hip", "mlp_kernels.hip"]
 # FP64 Morton codes must round exactly like the reference's CPU code: no FMA
 # contraction in the LBVH kernels.  The solver kernels contract only within
 # one expression (-ffp-contract=on), so a device function rounds the same way

@@ -7,11 +7,13 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "internal.hpp"
+#include "mlp_model.hpp"
 #include "lbvh.hpp"
 
 using namespace avbd;
@@ -75,6 +77,16 @@ struct avbd_ctx {
     uint32_t res_max_iters = 0, res_check_every = 0;
     double res_tol2 = 0;           // tol^2 ||M g||^2
     ResCtl* rc = nullptr;
+
+    // learned warm start (mlp_host.cpp): the network, its in-step buffers,
+    // and the recording of training data
+    std::unique_ptr<MlpModel> mlp;
+    bool warm = false;             // prediction replaces the inertial start
+    bool recording = false;        // record features / targets of the steps
+    WarmIO wio;
+    uint32_t rows_pad = 0;
+    float* d_raw = nullptr;        // n_items x 24
+    float* d_tgt = nullptr;        // n_items x 3
 
     // iterations per step: the budget in residual mode, else solver_iters
     uint32_t iters() const { return res_tol > 0 ? res_max_iters : desc.solver_iters; }
@@ -340,17 +352,42 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
         if (with_events) hip_check(hipEventRecord(c->ev_lb1, sb), "hipEventRecord");
         if (fork) hip_check(hipEventRecord(c->ev_join, sb), "hipEventRecord");
     }
+    // learned warm start / recording: the start-of-step phase runs as separate
+    // launches, then the features, the network and the choice of the start
+    const bool pre = c->warm || c->recording;
+    WarmIO io = c->wio;
+    if (pre) {
+        if (c->damping) launch_eprev(s, c->d, dt);
+        launch_step_begin(s, c->d, dt, g);
+        launch_warm_save(s, c->d);
+        io.raw = c->recording ? c->d_raw : nullptr;
+        io.tgt = c->d_tgt;
+        io.norm = c->mlp->d_norm;
+        launch_warm_features(s, c->d, io, c->cfg.lanes, dt, g, c->rows_pad);
+        if (c->warm) {
+            c->mlp->forward((int)c->rows_pad, io.x16);
+            io.out = c->mlp->z32[kMlpLayers - 1];
+            launch_warm_apply(s, c->d, io, dt * dt * g);
+            launch_warm_ip(s, c->d, io, c->damping, dt);
+            launch_warm_choose(s, c->d, io);
+        }
+    }
     if (c->persistent) {
         // the whole solve in one launch with grid barriers between phases
         if (with_events) hip_check(hipEventRecord(c->ev_st0, s), "hipEventRecord");
-        launch_step_persistent(s, c->sargs, c->cfg.lanes, c->damping, c->pgrid);
+        StepArgs a = c->sargs;
+        a.skip_begin = pre ? 1u : 0u;
+        launch_step_persistent(s, a, c->cfg.lanes, c->damping, c->pgrid);
+        if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
         if (with_events) hip_check(hipEventRecord(c->ev_st1, s), "hipEventRecord");
         if (fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
         hip_check(hipGetLastError(), "kernel launch");
         return;
     }
-    if (c->damping) launch_eprev(s, c->d, dt);
-    launch_step_begin(s, c->d, dt, g);
+    if (!pre) {
+        if (c->damping) launch_eprev(s, c->d, dt);
+        launch_step_begin(s, c->d, dt, g);
+    }
     const uint32_t* done = c->rc ? &c->rc->done : nullptr;
     if (c->rc) launch_residual_reset(s, c->rc);
     double omega = 1;
@@ -374,6 +411,7 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
                                   c->res_tol2);
     }
     if (c->rc) launch_residual_finish(s, c->rc, iters);
+    if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
     launch_velocity(s, c->d, dt);
     if (fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
     hip_check(hipGetLastError(), "kernel launch");
@@ -486,6 +524,7 @@ void avbd_destroy(avbd_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     destroy_graph(c);
+    c->mlp.reset();
     for (void* p : c->allocations) (void)hipFree(p);
     for (auto ev : c->events) (void)hipEventDestroy(ev);
     if (c->ev_st0) (void)hipEventDestroy(c->ev_st0);
@@ -1030,6 +1069,43 @@ int avbd_build_graph(avbd_ctx* c) {
                            1;
         d.reduce = dev_alloc<double>(c, (size_t)c->reduce_blocks * 4);
 
+        // learned warm start buffers: features, per-particle material and
+        // scene, per-scene IP difference, counters, recording buffers
+        {
+            c->rows_pad = std::max<uint32_t>(16u, (d.n_items + 15u) / 16u * 16u);
+            c->wio.x16 = dev_alloc<uint16_t>(c, (size_t)c->rows_pad * kMlpInPad);
+            c->d_raw = dev_alloc<float>(c, (size_t)std::max<uint32_t>(1u, d.n_items) * kMlpIn);
+            c->d_tgt = dev_alloc<float>(c, (size_t)std::max<uint32_t>(1u, d.n_items) * 3);
+            std::vector<double> le(n, 0.0), lnu(n, 0.0), cnt(n, 0.0);
+            for (const auto& t : c->nh) {
+                const double E = t.mu * (3 * t.lambda + 2 * t.mu) / (t.lambda + t.mu);
+                const double nu = t.lambda / (2 * (t.lambda + t.mu));
+                for (int k = 0; k < 4; k++) {
+                    le[t.idx[k]] += std::log10(E);
+                    lnu[t.idx[k]] += nu;
+                    cnt[t.idx[k]] += 1;
+                }
+            }
+            std::vector<float2> mat(n);
+            for (uint32_t p = 0; p < n; p++)
+                mat[p] = cnt[p] > 0 ? make_float2((float)(le[p] / cnt[p]), (float)(lnu[p] / cnt[p])) : make_float2(0.f, 0.f);
+            std::vector<uint32_t> so = c->scene_offsets;
+            if (so.empty()) so = {0u, n};
+            std::vector<uint32_t> scene_of(n, 0);
+            for (uint32_t sc = 0; sc + 1 < so.size(); sc++)
+                for (uint32_t p = so[sc]; p < so[sc + 1]; p++) scene_of[p] = sc;
+            c->wio.n_scenes = (uint32_t)(so.size() - 1);
+            float2* d_mat = dev_alloc<float2>(c, n);
+            uint32_t* d_scene = dev_alloc<uint32_t>(c, n);
+            c->wio.ipd = dev_alloc<double>(c, c->wio.n_scenes);
+            c->wio.counters = dev_alloc<unsigned long long>(c, 2);
+            upload(c, d_mat, mat);
+            upload(c, d_scene, scene_of);
+            hip_check(hipMemsetAsync(c->wio.counters, 0, 2 * sizeof(unsigned long long), c->stream), "hipMemsetAsync");
+            c->wio.mat = d_mat;
+            c->wio.scene_of = d_scene;
+        }
+
         std::vector<TetRec> tr(d.n_tets);
         std::vector<TetDamp> td(d.n_tets);
         for (uint32_t t = 0; t < d.n_tets; t++) {
@@ -1461,6 +1537,12 @@ int avbd_get_stats(avbd_ctx* c, avbd_stats* out) {
         st.num_tets = c->d.n_tets;
         st.num_ground = c->d.n_ground;
         st.steps_taken = c->steps_taken;
+        {
+            unsigned long long wc[2] = {0, 0};
+            hip_check(hipMemcpy(wc, c->wio.counters, sizeof(wc), hipMemcpyDeviceToHost), "D2H");
+            st.warm_start_predictions = wc[0];
+            st.warm_start_fallbacks = wc[1];
+        }
         if (c->rc) {
             ResCtl rc{};
             hip_check(hipMemcpy(&rc, c->rc, sizeof(ResCtl), hipMemcpyDeviceToHost), "D2H");
@@ -1546,6 +1628,174 @@ int avbd_calibrate_copy(avbd_ctx* c, uint64_t bytes, uint32_t iters, double* avg
 }
 
 }  // extern "C"
+
+namespace {
+void ensure_model(avbd_ctx* c) {
+    require_built(c);
+    if (!c->mlp) c->mlp = std::make_unique<MlpModel>(c->stream);
+}
+void require_not_warm(avbd_ctx* c) {
+    if (c->warm) throw CodeError(AVBD_E_ARG, "disable the warm start before training or evaluating the network");
+}
+}  // namespace
+
+extern "C" {
+
+uint64_t avbd_mlp_param_count(void) { return MlpModel::param_count(); }
+
+int avbd_mlp_init(avbd_ctx* c, uint64_t seed) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        require_not_warm(c);
+        c->mlp->init(seed);
+    });
+}
+
+int avbd_mlp_set_params(avbd_ctx* c, const float* p, uint64_t n) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        require_not_warm(c);
+        if (!p || n != MlpModel::param_count()) throw std::runtime_error("parameter vector has the wrong length");
+        c->mlp->set_params(p);
+    });
+}
+
+int avbd_mlp_get_params(avbd_ctx* c, float* p, uint64_t n) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        if (!p || n != MlpModel::param_count()) throw std::runtime_error("parameter vector has the wrong length");
+        c->mlp->get_params(p);
+    });
+}
+
+int avbd_mlp_set_normalization(avbd_ctx* c, const float* norm) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        require_not_warm(c);
+        if (!norm) throw std::runtime_error("null normalisation");
+        for (int i = 0; i < kMlpIn; i++)
+            if (!(norm[kMlpIn + i] > 0)) throw std::runtime_error("feature std must be > 0");
+        c->mlp->set_norm(norm);
+    });
+}
+
+int avbd_mlp_forward_backward(avbd_ctx* c, const float* feats, const float* tgts, uint32_t rows, float* grads,
+                              uint64_t n, float* loss) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        require_not_warm(c);
+        if (!feats || !tgts || rows == 0) throw std::runtime_error("empty batch");
+        const float l = c->mlp->forward_backward(feats, tgts, (int)rows);
+        if (grads) {
+            if (n != MlpModel::param_count() - 2 * kMlpIn - 2 * kMlpOut)
+                throw std::runtime_error("gradient vector has the wrong length");
+            c->mlp->get_grads(grads);
+        }
+        if (loss) *loss = l;
+    });
+}
+
+int avbd_mlp_predict(avbd_ctx* c, const float* feats, uint32_t rows, float* out) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        require_not_warm(c);
+        if (!feats || !out || rows == 0) throw std::runtime_error("empty batch");
+        c->mlp->predict(feats, (int)rows, out);
+    });
+}
+
+int avbd_mlp_train(avbd_ctx* c, const float* feats, const float* tgts, uint32_t rows, uint32_t epochs,
+                   uint32_t batch, float lr, uint64_t seed, float* loss) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        require_not_warm(c);
+        if (!feats || !tgts || rows < batch || batch == 0 || batch % 16) throw std::runtime_error("bad training set / batch");
+        const float l = c->mlp->train(feats, tgts, (int)rows, (int)epochs, (int)batch, lr, seed);
+        if (loss) *loss = l;
+    });
+}
+
+int avbd_mlp_gemm_test(avbd_ctx* c, uint32_t M, uint32_t N, uint32_t K, int ta, int tb, const float* A,
+                       const float* B, float* C) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!A || !B || !C || M % 16 || N % 16 || K % 16 || M == 0 || N == 0 || K == 0 || (ta && tb))
+            throw std::runtime_error("gemm test: sizes must be multiples of 16 and at most one operand transposed");
+        // A is M x K and B is K x N (row-major FP32); stored as the kernel reads them
+        std::vector<uint16_t> a16((size_t)M * K), b16((size_t)K * N);
+        auto bf = [](float f) {
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            return (uint16_t)(u >> 16);
+        };
+        for (uint32_t m = 0; m < M; m++)
+            for (uint32_t k = 0; k < K; k++) a16[ta ? (size_t)k * M + m : (size_t)m * K + k] = bf(A[(size_t)m * K + k]);
+        for (uint32_t k = 0; k < K; k++)
+            for (uint32_t n = 0; n < N; n++) b16[tb ? (size_t)n * K + k : (size_t)k * N + n] = bf(B[(size_t)k * N + n]);
+        uint16_t *da = nullptr, *db = nullptr;
+        float* dc = nullptr;
+        hip_check(hipMalloc(&da, a16.size() * 2), "hipMalloc");
+        hipError_t e = hipMalloc(&db, b16.size() * 2);
+        if (e == hipSuccess) e = hipMalloc(&dc, (size_t)M * N * 4);
+        if (e == hipSuccess) e = hipMemcpy(da, a16.data(), a16.size() * 2, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(db, b16.data(), b16.size() * 2, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            launch_gemm_bf16(c->stream, ta != 0, tb != 0, MLP_EPI_F32, da, ta ? (int)M : (int)K, db,
+                             tb ? (int)K : (int)N, dc, nullptr, (int)N, nullptr, nullptr, 0, (int)M, (int)N, (int)K);
+            e = hipStreamSynchronize(c->stream);
+        }
+        if (e == hipSuccess) e = hipMemcpy(C, dc, (size_t)M * N * 4, hipMemcpyDeviceToHost);
+        (void)hipFree(da);
+        (void)hipFree(db);
+        (void)hipFree(dc);
+        hip_check(e, "gemm test");
+    });
+}
+
+int avbd_record_steps(avbd_ctx* c, uint32_t n_steps, float* feats, float* tgts, uint32_t cap_rows, uint32_t* rows) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        require_not_warm(c);
+        const uint32_t ni = c->d.n_items;
+        if ((uint64_t)n_steps * ni > cap_rows) throw std::runtime_error("record buffer too small");
+        c->recording = true;
+        std::vector<float> raw((size_t)ni * kMlpIn), tg((size_t)ni * 3);
+        try {
+            for (uint32_t st = 0; st < n_steps; st++) {
+                enqueue_step(c, false);
+                hip_check(hipMemcpyAsync(raw.data(), c->d_raw, raw.size() * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+                hip_check(hipMemcpyAsync(tg.data(), c->d_tgt, tg.size() * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
+                hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+                check_abort(c);
+                std::memcpy(feats + (size_t)st * ni * kMlpIn, raw.data(), raw.size() * 4);
+                std::memcpy(tgts + (size_t)st * ni * 3, tg.data(), tg.size() * 4);
+                c->steps_taken++;
+            }
+        } catch (...) {
+            c->recording = false;
+            throw;
+        }
+        c->recording = false;
+        if (rows) *rows = n_steps * ni;
+    });
+}
+
+int avbd_set_warm_start(avbd_ctx* c, int enable) {
+    return guarded(c, [&] {
+        ensure_model(c);
+        if (enable && !c->mlp->loaded) throw CodeError(AVBD_E_ARG, "the warm-start network has no trained parameters");
+        if (enable) c->mlp->ensure_batch((int)c->rows_pad);
+        c->warm = enable != 0;
+        if (!(c->desc.flags & AVBD_FLAG_NO_GRAPH)) {
+            capture_graph(c);
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        }
+    });
+}
+
+}  // extern "C"
+
 
 extern "C" {
 

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/avbd.h"
+#include "mlp.hpp"
 
 namespace avbd {
 
@@ -199,6 +200,7 @@ struct StepArgs {
     const float* omega = nullptr;                 // Chebyshev omega per iteration
     GridBarrier* bar = nullptr;
     uint32_t iters = 0, num_colors = 0;
+    uint32_t skip_begin = 0;      // the start-of-step phase already ran as separate launches
     float dt = 0.f, g = 0.f;
 };
 
@@ -223,6 +225,14 @@ void launch_residual_reset(hipStream_t s, ResCtl* rc);
 void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
                            ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2);
 void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters);
+// learned warm start
+void launch_warm_save(hipStream_t s, const DevState& d);
+void launch_warm_features(hipStream_t s, const DevState& d, const WarmIO& io, uint32_t lanes, float dt, float g,
+                          uint32_t rows_pad);
+void launch_warm_apply(hipStream_t s, const DevState& d, const WarmIO& io, float dt2g);
+void launch_warm_ip(hipStream_t s, const DevState& d, const WarmIO& io, bool damping, float dt);
+void launch_warm_choose(hipStream_t s, const DevState& d, const WarmIO& io);
+void launch_warm_target(hipStream_t s, const DevState& d, const WarmIO& io, float inv_dt2g);
 void launch_velocity(hipStream_t s, const DevState& d, float dt);
 // plain streaming copy of n_float4 16-byte words (HBM counter calibration)
 void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n_float4);

@@ -27,11 +27,26 @@ constexpr int MLP_EPI_SILU_BWD = 3;   // C16 = bf16(acc * silu'(aux))
 inline int mlp_layer_in(int l) { return l == 0 ? kMlpInPad : kMlpHidden; }
 inline int mlp_layer_out(int l) { return l == kMlpLayers - 1 ? kMlpOutPad : kMlpHidden; }
 
+// Device buffers of the in-step warm start (features -> network -> apply ->
+// incremental-potential comparison -> choice).
+struct WarmIO {
+    uint16_t* x16 = nullptr;             // rows_pad x 32 normalised bf16 features
+    float* raw = nullptr;                // n_items x 24 raw features (recording) or null
+    const float* out = nullptr;          // rows_pad x 16 network output (FP32)
+    const float* norm = nullptr;         // feature mean/std (24 + 24), target mean/std (3 + 3)
+    float* tgt = nullptr;                // n_items x 3 training targets (recording)
+    const float2* mat = nullptr;         // per particle: log10 E, nu of the incident material
+    const uint32_t* scene_of = nullptr;  // per particle
+    double* ipd = nullptr;               // per scene: IP(prediction) - IP(initial guess)
+    unsigned long long* counters = nullptr;   // [0] scene-steps predicted, [1] fallbacks to the inertial start
+    uint32_t n_scenes = 0;
+};
+
 void launch_gemm_bf16(hipStream_t s, bool ta, bool tb, int epi, const uint16_t* a, int lda, const uint16_t* b,
                       int ldb, float* c32, uint16_t* c16, int ldc, const float* bias, const float* aux, int ldaux,
                       int M, int N, int K);
-void launch_mse_grad(hipStream_t s, const float* y, const float* t, uint16_t* dy, int rows, int ld, int outs,
-                     float* loss);
+void launch_mse_grad(hipStream_t s, const float* y, const float* t, uint16_t* dy, int rows, int rows_pad, int ld,
+                     int outs, float* loss);
 void launch_colsum(hipStream_t s, const uint16_t* d, int rows, int ld, int cols, float* out);
 void launch_adam(hipStream_t s, float* w, uint16_t* w16, const float* gw, float* m1, float* m2, int n, float lr,
                  float b1, float b2, float eps, int t);

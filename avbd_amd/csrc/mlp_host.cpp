@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -182,8 +183,8 @@ void MlpModel::forward(int rows_pad, const uint16_t* x) {
 // loss sum into d_loss.
 void MlpModel::backward(int rows, int rows_pad, const uint16_t* x) {
     hchk(hipMemsetAsync(d_loss, 0, 4, stream), "memset");
-    launch_mse_grad(stream, z32[kMlpLayers - 1], t32, d16[kMlpLayers - 1], rows_pad, kMlpOutPad, kMlpOut, d_loss);
-    (void)rows;
+    launch_mse_grad(stream, z32[kMlpLayers - 1], t32, d16[kMlpLayers - 1], rows, rows_pad, kMlpOutPad, kMlpOut,
+                    d_loss);
     for (int l = kMlpLayers - 1; l >= 0; l--) {
         const Layer& L = layers[l];
         const uint16_t* act_in = l == 0 ? x : h16[l - 1];

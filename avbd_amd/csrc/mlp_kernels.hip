@@ -88,14 +88,15 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
 }
 
 // Output-layer loss gradient: dY = 2 (Y - T) / rows for the first `outs`
-// columns (zero elsewhere), as bf16; returns the loss sum in *loss.
+// columns of the first `rows` rows (zero in the padding), as bf16; adds the
+// squared-error sum to *loss.
 __global__ void mse_grad_kernel(const float* __restrict__ y, const float* __restrict__ t, uint16_t* __restrict__ dy,
-                                int rows, int ld, int outs, float* __restrict__ loss) {
+                                int rows, int rows_pad, int ld, int outs, float* __restrict__ loss) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows * ld) return;
+    if (i >= rows_pad * ld) return;
     const int col = i % ld, row = i / ld;
     float d = 0.f;
-    if (col < outs) {
+    if (col < outs && row < rows) {
         d = y[i] - t[(size_t)row * outs + col];
         atomicAdd(loss, d * d);
     }
@@ -159,10 +160,10 @@ void launch_gemm_bf16(hipStream_t s, bool ta, bool tb, int epi, const uint16_t* 
 #undef AVBD_GEMM
 }
 
-void launch_mse_grad(hipStream_t s, const float* y, const float* t, uint16_t* dy, int rows, int ld, int outs,
-                     float* loss) {
-    hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks_for(rows * ld, 256)), dim3(256), 0, s, y, t, dy, rows, ld,
-                       outs, loss);
+void launch_mse_grad(hipStream_t s, const float* y, const float* t, uint16_t* dy, int rows, int rows_pad, int ld,
+                     int outs, float* loss) {
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks_for(rows_pad * ld, 256)), dim3(256), 0, s, y, t, dy, rows,
+                       rows_pad, ld, outs, loss);
 }
 
 void launch_colsum(hipStream_t s, const uint16_t* d, int rows, int ld, int cols, float* out) {

@@ -263,6 +263,39 @@ __global__ void eprev_kernel(const TetRec* __restrict__ tets, TetDamp* __restric
     if (t < n_tets) eprev_one(t, tets, damp, x_prev);
 }
 
+// Stable Neo-Hookean energy of tet t at positions P (NeoHookeanEnergySolver::energy,
+// energy/elastic/NeoHookeanEnergySolver.hpp:20-53), with the strain-rate
+// damping term when DAMP, in FP64 accumulation.
+template <bool DAMP>
+__device__ __forceinline__ double nh_energy_dev(const float4* P, const TetRec* __restrict__ tets,
+                                                const TetDamp* __restrict__ damp, uint32_t t, float inv_dt) {
+    uint4 idx;
+    float4 r1, r2, r3;
+    load_tet(tets, t, idx, r1, r2, r3);
+    const float3 p0 = f3(P[idx.x]), p1 = f3(P[idx.y]), p2 = f3(P[idx.z]), p3 = f3(P[idx.w]);
+    const float3 d0 = p0 - p3, d1 = p1 - p3, d2 = p2 - p3;
+    const float3 f0 = r1.x * d0 + r1.w * d1 + r2.z * d2;
+    const float3 f1 = r1.y * d0 + r2.x * d1 + r2.w * d2;
+    const float3 f2 = r1.z * d0 + r2.y * d1 + r3.x * d2;
+    const double J = dot3(f0, cross3(f1, f2));
+    const double VL = r3.y, VM = r3.z, al = r3.w;
+    const double trc = (double)dot3(f0, f0) + dot3(f1, f1) + dot3(f2, f2);
+    double e = 0.5 * VL * (J - al) * (J - al) + 0.5 * VM * (trc - 3.0);
+    if constexpr (DAMP) {
+        const TetDamp dr = damp[t];
+        const double E[6] = {0.5 * (dot3(f0, f0) - 1.0), 0.5 * (dot3(f1, f1) - 1.0), 0.5 * (dot3(f2, f2) - 1.0),
+                             0.5 * dot3(f0, f1), 0.5 * dot3(f0, f2), 0.5 * dot3(f1, f2)};
+        const double Pv[6] = {dr.e00, dr.e11, dr.e22, dr.e01, dr.e02, dr.e12};
+        double sq = 0;
+        for (int i = 0; i < 6; i++) {
+            const double dd = (E[i] - Pv[i]) * inv_dt;
+            sq += (i < 3 ? 1.0 : 2.0) * dd * dd;
+        }
+        e += 0.5 * dr.damp / inv_dt * sq;   // V kd |Edot|^2 with damp = 2 V kd / dt
+    }
+    return e;
+}
+
 // Start of step for one particle: ground contact point + AL warm start, the
 // rigid-ground / joint records of an oriented particle, then the inertial
 // target and adaptive initial guess (and the rotation inertial update).
@@ -484,6 +517,246 @@ __global__ void residual_finish_kernel(ResCtl* rc, uint32_t max_iters) {
     }
 }
 
+// ---- learned warm start: features, prediction apply, IP comparison ---------
+// Green-strain norm ||E||_F, E = (F^T F - I)/2, of tet t at positions P;
+// nfixed += 1 when one of its vertices is fixed.
+__device__ __forceinline__ float tet_strain(const float4* P, const TetRec* __restrict__ tets,
+                                            const uint8_t* __restrict__ fixed, uint32_t t, float& nfixed) {
+    uint4 idx;
+    float4 r1, r2, r3;
+    load_tet(tets, t, idx, r1, r2, r3);
+    const float3 p0 = f3(P[idx.x]), p1 = f3(P[idx.y]), p2 = f3(P[idx.z]), p3 = f3(P[idx.w]);
+    const float3 d0 = p0 - p3, d1 = p1 - p3, d2 = p2 - p3;
+    const float3 f0 = r1.x * d0 + r1.w * d1 + r2.z * d2;
+    const float3 f1 = r1.y * d0 + r2.x * d1 + r2.w * d2;
+    const float3 f2 = r1.z * d0 + r2.y * d1 + r3.x * d2;
+    const float e00 = 0.5f * (dot3(f0, f0) - 1.f), e11 = 0.5f * (dot3(f1, f1) - 1.f), e22 = 0.5f * (dot3(f2, f2) - 1.f);
+    const float e01 = 0.5f * dot3(f0, f1), e02 = 0.5f * dot3(f0, f2), e12 = 0.5f * dot3(f1, f2);
+    nfixed += (fixed[idx.x] | fixed[idx.y] | fixed[idx.z] | fixed[idx.w]) ? 1.f : 0.f;
+    return sqrtf(e00 * e00 + e11 * e11 + e22 * e22 + 2.f * (e01 * e01 + e02 * e02 + e12 * e12));
+}
+
+template <int L>
+__device__ __forceinline__ float group_max(float v) {
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Per positional work item at the start of a step (after the inertial /
+// adaptive initialisation): the 24 features of the warm-start network.
+//   0-2   (y - x_prev) / (dt^2 g)          inertial displacement
+//   3-5   dt v / (dt^2 g)                   previous-step displacement
+//   6-8   dt (v - v_prev) / (dt^2 g)        change of displacement
+//   9     log10 m
+//   10,11 mean / max Green-strain norm of the incident tets at x_prev
+//   12    contact flag (ground multiplier > 0 or within 1 mm of the ground)
+//   13    fraction of incident tets that touch a fixed vertex
+//   14,15 log10 E, nu of the incident material
+//   16    height above the ground (0 without a ground record)
+//   17    ground normal multiplier / (m g)
+//   18-20 elastic force at x_prev / (m g)
+//   21-23 force residual at the initial guess / (m g)
+// Normalised features go to x16 (bf16, rows of 32, zero padding); raw ones
+// to raw (rows of 24) when it is given.
+template <int L>
+__global__ __launch_bounds__(256) void warm_features_kernel(DevState d, WarmIO io, float dt, float g_acc,
+                                                            uint32_t rows_pad) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t item = tid / L, lane = tid % L;
+    if (item >= rows_pad) return;   // whole groups
+    if (item >= d.n_items) {
+        if (lane == 0)
+            for (int f = 0; f < kMlpInPad; f++) io.x16[(size_t)item * kMlpInPad + f] = 0;
+        return;
+    }
+    const VItem it = d.items[item];
+    const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
+    const float inv_dt = 1.f / dt;
+    float3 gp = make_float3(0.f, 0.f, 0.f), gi = make_float3(0.f, 0.f, 0.f);
+    Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float ssum = 0.f, smax = 0.f, nfix = 0.f;
+    for (uint32_t j = lane; j < cnt; j += L) {
+        const uint32_t e = d.inc[it.inc_begin + j];
+        nh_incidence<false>(d.x_prev, d.tets, d.damp, e, inv_dt, gp, h);
+        nh_incidence<false>(d.x, d.tets, d.damp, e, inv_dt, gi, h);
+        const float sn = tet_strain(d.x_prev, d.tets, d.fixed, e >> 2, nfix);
+        ssum += sn;
+        smax = fmaxf(smax, sn);
+    }
+    gp.x = group_sum<L>(gp.x);
+    gp.y = group_sum<L>(gp.y);
+    gp.z = group_sum<L>(gp.z);
+    gi.x = group_sum<L>(gi.x);
+    gi.y = group_sum<L>(gi.y);
+    gi.z = group_sum<L>(gi.z);
+    ssum = group_sum<L>(ssum);
+    nfix = group_sum<L>(nfix);
+    smax = group_max<L>(smax);
+    if (lane != 0) return;
+    const float4 x0 = d.x[it.v], xp = d.x_prev[it.v], yv = d.y[it.v], vv = d.v[it.v], vpv = d.v_prev[it.v];
+    const float m = x0.w, mg = m * g_acc, sc = 1.f / (dt * dt * g_acc);
+    float f[kMlpIn];
+    f[0] = (yv.x - xp.x) * sc;
+    f[1] = (yv.y - xp.y) * sc;
+    f[2] = (yv.z - xp.z) * sc;
+    f[3] = dt * vv.x * sc;
+    f[4] = dt * vv.y * sc;
+    f[5] = dt * vv.z * sc;
+    f[6] = dt * (vv.x - vpv.x) * sc;
+    f[7] = dt * (vv.y - vpv.y) * sc;
+    f[8] = dt * (vv.z - vpv.z) * sc;
+    f[9] = log10f(fmaxf(m, 1e-30f));
+    f[10] = cnt ? ssum / (float)cnt : 0.f;
+    f[11] = smax;
+    f[12] = 0.f;
+    f[16] = 0.f;
+    f[17] = 0.f;
+    if (it.ground >= 0) {
+        const GroundRec c = d.ground[it.ground];
+        f[12] = (c.lambda_n > 0.f || xp.y < 1e-3f) ? 1.f : 0.f;
+        f[16] = xp.y;
+        f[17] = c.lambda_n / mg;
+        ground_accumulate(c, f3(x0), gi, h);
+    }
+    f[13] = cnt ? nfix / (float)cnt : 0.f;
+    const float2 mat = io.mat[it.v];
+    f[14] = mat.x;
+    f[15] = mat.y;
+    f[18] = gp.x / mg;
+    f[19] = gp.y / mg;
+    f[20] = gp.z / mg;
+    const float md = m / (dt * dt);
+    f[21] = (md * (x0.x - yv.x) + gi.x) / mg;
+    f[22] = (md * (x0.y - yv.y) + gi.y) / mg;
+    f[23] = (md * (x0.z - yv.z) + gi.z) / mg;
+    uint16_t* row = io.x16 + (size_t)item * kMlpInPad;
+    for (int k = 0; k < kMlpIn; k++) {
+        row[k] = to_bf16((f[k] - io.norm[k]) / io.norm[kMlpIn + k]);
+        if (io.raw) io.raw[(size_t)item * kMlpIn + k] = f[k];
+    }
+    for (int k = kMlpIn; k < kMlpInPad; k++) row[k] = 0;
+}
+
+// Keep the initial guess of every particle (x -> x_scratch) before the
+// prediction overwrites the free positional ones.
+__global__ void warm_save_kernel(DevState d) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < d.n) d.x_scratch[p] = d.x[p];
+}
+
+// x = x0 + dt^2 g * (network output, de-normalised) for every positional work item.
+__global__ void warm_apply_kernel(DevState d, WarmIO io, float s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n_items) return;
+    const uint32_t v = d.items[i].v;
+    const float* o = io.out + (size_t)i * kMlpOutPad;
+    const float* tn = io.norm + 2 * kMlpIn;   // target mean (3), std (3)
+    float4 x = d.x[v];
+    x.x += s * (o[0] * tn[3] + tn[0]);
+    x.y += s * (o[1] * tn[4] + tn[1]);
+    x.z += s * (o[2] * tn[5] + tn[2]);
+    d.x[v] = x;
+}
+
+// Add v to acc[scene]: one atomic per wavefront when every active lane has
+// the same scene (the common case: energies are laid out scene by scene),
+// one per lane otherwise.  Every lane of the wavefront must call it.
+__device__ __forceinline__ void add_per_scene(double* acc, uint32_t scene, double v, bool active) {
+    const unsigned long long act = __ballot(active);
+    if (act == 0) return;
+    const int first = __ffsll((long long)act) - 1;
+    const uint32_t s0 = __shfl(scene, first);
+    if (__all(!active || scene == s0)) {
+        double t = active ? v : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+        if ((int)__lane_id() == first) atomicAdd(&acc[s0], t);
+    } else if (active) {
+        atomicAdd(&acc[scene], v);
+    }
+}
+
+__device__ __forceinline__ double ground_al_energy(const GroundRec& c, float4 xp) {
+    const double Cn = -xp.y, Ct = (double)xp.x - c.cp_x, Cb = (double)xp.z - c.cp_z;
+    const double Cc = Cn - (double)kAlpha * c.C_n_prev;
+    double e = 0;
+    if ((double)c.k_n * Cc + c.lambda_n > 0) e += 0.5 * c.k_n * Cc * Cc + (double)c.lambda_n * Cc;
+    return e + 0.5 * c.k_t * Ct * Ct + (double)c.lambda_t * Ct + 0.5 * c.k_b * Cb * Cb + (double)c.lambda_b * Cb;
+}
+
+// Per scene: incremental potential at the prediction (x) minus at the
+// initial guess (x_scratch) -- Neo-Hookean (+ damping), ground AL and
+// inertia terms (the terms that differ between the two).
+template <bool DAMP>
+__global__ __launch_bounds__(256) void warm_ip_kernel(DevState d, WarmIO io, float dt) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float inv_dt = 1.f / dt;
+    double e = 0;
+    uint32_t scene = 0;
+    bool active = false;
+    if (k < d.n_tets) {
+        e = nh_energy_dev<DAMP>(d.x, d.tets, d.damp, k, inv_dt) - nh_energy_dev<DAMP>(d.x_scratch, d.tets, d.damp, k, inv_dt);
+        scene = io.scene_of[d.tets[k].idx[0]];
+        active = true;
+    } else if (k < d.n_tets + d.n_ground) {
+        const GroundRec c = d.ground[k - d.n_tets];
+        e = ground_al_energy(c, d.x[c.p]) - ground_al_energy(c, d.x_scratch[c.p]);
+        scene = io.scene_of[c.p];
+        active = true;
+    } else if (k < d.n_tets + d.n_ground + d.n_items) {
+        const uint32_t v = d.items[k - d.n_tets - d.n_ground].v;
+        const float4 x = d.x[v], x0 = d.x_scratch[v], y = d.y[v];
+        const double md = 0.5 * (double)x.w * inv_dt * inv_dt;
+        const double a = ((double)x.x - y.x) * (x.x - y.x) + ((double)x.y - y.y) * (x.y - y.y) + ((double)x.z - y.z) * (x.z - y.z);
+        const double b = ((double)x0.x - y.x) * (x0.x - y.x) + ((double)x0.y - y.y) * (x0.y - y.y) + ((double)x0.z - y.z) * (x0.z - y.z);
+        e = md * (a - b);
+        scene = io.scene_of[v];
+        active = true;
+    }
+    add_per_scene(io.ipd, scene, e, active);
+}
+
+__global__ void warm_ip_reset_kernel(WarmIO io) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < io.n_scenes) io.ipd[s] = 0.0;
+}
+
+// Keep the prediction of a scene only if it lowers the incremental potential;
+// restart the Chebyshev history at the chosen guess.
+__global__ void warm_choose_kernel(DevState d, WarmIO io) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < io.n_scenes) {
+        atomicAdd(&io.counters[0], 1ull);
+        if (io.ipd[i] > 0.0) atomicAdd(&io.counters[1], 1ull);
+    }
+    if (i >= d.n_items) return;
+    const uint32_t v = d.items[i].v;
+    float4 x = d.x[v];
+    if (io.ipd[io.scene_of[v]] > 0.0) x = d.x_scratch[v];
+    d.x[v] = x;
+    d.x_last[v] = x;
+    d.x_lastlast[v] = x;
+}
+
+// Training target of a recorded step: (x_end - x0) / (dt^2 g) per work item.
+__global__ void warm_target_kernel(DevState d, WarmIO io, float inv_s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.n_items) return;
+    const uint32_t v = d.items[i].v;
+    const float4 x = d.x[v], x0 = d.x_scratch[v];
+    io.tgt[(size_t)i * 3 + 0] = (x.x - x0.x) * inv_s;
+    io.tgt[(size_t)i * 3 + 1] = (x.y - x0.y) * inv_s;
+    io.tgt[(size_t)i * 3 + 2] = (x.z - x0.z) * inv_s;
+}
+
 // ---- persistent step kernel ------------------------------------------------
 // One launch runs a whole time step; every barrier of solve_Parallel becomes a
 // grid barrier.  The grid is sized so that all blocks are resident at once
@@ -544,10 +817,12 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
         if (threadIdx.x < kMaxResidualChecks) a.rc->acc[threadIdx.x] = 0.0;
         if (threadIdx.x == 0) a.rc->done = 0;
     }
-    if constexpr (DAMP) {
-        for (uint32_t t = gtid; t < d.n_tets; t += gthreads) eprev_one(t, d.tets, d.damp, d.x_prev);
+    if (!a.skip_begin) {
+        if constexpr (DAMP) {
+            for (uint32_t t = gtid; t < d.n_tets; t += gthreads) eprev_one(t, d.tets, d.damp, d.x_prev);
+        }
+        for (uint32_t p = gtid; p < d.n; p += gthreads) step_begin_one(d, p, a.dt, a.g);
     }
-    for (uint32_t p = gtid; p < d.n; p += gthreads) step_begin_one(d, p, a.dt, a.g);
     if (!grid_sync(a.bar, G, gen, &ok_s)) return;
     uint32_t iters_used = a.iters;
 
@@ -674,37 +949,10 @@ __global__ void stats_energy_kernel(const TetRec* __restrict__ tets, const TetDa
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0;
     if (k < n_tets) {
-        uint4 idx;
-        float4 r1, r2, r3;
-        load_tet(tets, k, idx, r1, r2, r3);
-        const float3 p0 = f3(x[idx.x]), p1 = f3(x[idx.y]), p2 = f3(x[idx.z]), p3 = f3(x[idx.w]);
-        const float3 d0 = p0 - p3, d1 = p1 - p3, d2 = p2 - p3;
-        const float3 f0 = r1.x * d0 + r1.w * d1 + r2.z * d2;
-        const float3 f1 = r1.y * d0 + r2.x * d1 + r2.w * d2;
-        const float3 f2 = r1.z * d0 + r2.y * d1 + r3.x * d2;
-        const double J = dot3(f0, cross3(f1, f2));
-        const double VL = r3.y, VM = r3.z, al = r3.w;
-        const double trc = (double)dot3(f0, f0) + dot3(f1, f1) + dot3(f2, f2);
-        e = 0.5 * VL * (J - al) * (J - al) + 0.5 * VM * (trc - 3.0);
-        if constexpr (DAMP) {
-            const TetDamp dr = damp[k];
-            const double E[6] = {0.5 * (dot3(f0, f0) - 1.0), 0.5 * (dot3(f1, f1) - 1.0), 0.5 * (dot3(f2, f2) - 1.0),
-                                 0.5 * dot3(f0, f1), 0.5 * dot3(f0, f2), 0.5 * dot3(f1, f2)};
-            const double P[6] = {dr.e00, dr.e11, dr.e22, dr.e01, dr.e02, dr.e12};
-            double s = 0;
-            for (int i = 0; i < 6; i++) {
-                const double d = (E[i] - P[i]) * inv_dt;
-                s += (i < 3 ? 1.0 : 2.0) * d * d;
-            }
-            e += 0.5 * dr.damp / inv_dt * s;   // V kd |Edot|^2, damp = 2 V kd / dt
-        }
+        e = nh_energy_dev<DAMP>(x, tets, damp, k, inv_dt);
     } else if (k < n_tets + n_ground) {
         const GroundRec c = ground[k - n_tets];
-        const float4 xp = x[c.p];
-        const double Cn = -xp.y, Ct = (double)xp.x - c.cp_x, Cb = (double)xp.z - c.cp_z;
-        const double Cc = Cn - (double)kAlpha * c.C_n_prev;
-        if ((double)c.k_n * Cc + c.lambda_n > 0) e += 0.5 * c.k_n * Cc * Cc + (double)c.lambda_n * Cc;
-        e += 0.5 * c.k_t * Ct * Ct + (double)c.lambda_t * Ct + 0.5 * c.k_b * Cb * Cb + (double)c.lambda_b * Cb;
+        e = ground_al_energy(c, x[c.p]);
     } else if (k < n_tets + n_ground + d.n_cos) {
         // CosseratRodEnergySolver::energy (CosseratRodEnergySolver.hpp:20-49)
         const CosRec c = d.cos[k - n_tets - n_ground];
@@ -861,6 +1109,52 @@ void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const uint3
 void launch_velocity(hipStream_t s, const DevState& d, float dt) {
     if (d.n == 0) return;
     hipLaunchKernelGGL(velocity_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, 1.0f / dt);
+}
+
+void launch_warm_save(hipStream_t s, const DevState& d) {
+    if (d.n) hipLaunchKernelGGL(warm_save_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d);
+}
+
+void launch_warm_features(hipStream_t s, const DevState& d, const WarmIO& io, uint32_t lanes, float dt, float g,
+                          uint32_t rows_pad) {
+    if (rows_pad == 0) return;
+#define AVBD_FEAT_CASE(LL)                                                                                  \
+    case LL:                                                                                                \
+        hipLaunchKernelGGL((warm_features_kernel<LL>), dim3(blocks_for(rows_pad * LL, 256)), dim3(256), 0, s, d, \
+                           io, dt, g, rows_pad);                                                            \
+        break;
+    switch (lanes) {
+        AVBD_FEAT_CASE(4)
+        AVBD_FEAT_CASE(8)
+        AVBD_FEAT_CASE(16)
+        AVBD_FEAT_CASE(32)
+        default: break;
+    }
+#undef AVBD_FEAT_CASE
+}
+
+void launch_warm_apply(hipStream_t s, const DevState& d, const WarmIO& io, float dt2g) {
+    if (d.n_items) hipLaunchKernelGGL(warm_apply_kernel, dim3(blocks_for(d.n_items, 256)), dim3(256), 0, s, d, io, dt2g);
+}
+
+void launch_warm_ip(hipStream_t s, const DevState& d, const WarmIO& io, bool damping, float dt) {
+    hipLaunchKernelGGL(warm_ip_reset_kernel, dim3(blocks_for(io.n_scenes, 256)), dim3(256), 0, s, io);
+    const uint32_t n = d.n_tets + d.n_ground + d.n_items;
+    if (n == 0) return;
+    if (damping)
+        hipLaunchKernelGGL(warm_ip_kernel<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, io, dt);
+    else
+        hipLaunchKernelGGL(warm_ip_kernel<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, io, dt);
+}
+
+void launch_warm_choose(hipStream_t s, const DevState& d, const WarmIO& io) {
+    const uint32_t n = d.n_items > io.n_scenes ? d.n_items : io.n_scenes;
+    if (n) hipLaunchKernelGGL(warm_choose_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, d, io);
+}
+
+void launch_warm_target(hipStream_t s, const DevState& d, const WarmIO& io, float inv_dt2g) {
+    if (d.n_items)
+        hipLaunchKernelGGL(warm_target_kernel, dim3(blocks_for(d.n_items, 256)), dim3(256), 0, s, d, io, inv_dt2g);
 }
 
 uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt, float g) {

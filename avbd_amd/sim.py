@@ -32,6 +32,16 @@ def _uptr(a: Optional[np.ndarray]):
     return a.ctypes.data_as(C.POINTER(C.c_uint32))
 
 
+def _fptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
 def _bptr(a: Optional[np.ndarray]):
     if a is None:
         return None
@@ -202,6 +212,69 @@ class SimulationContext:
         """Iterate each step until ||r|| / ||M g|| <= rel_tol (checked every
         check_every iterations) or max_iters; rel_tol = 0: fixed count."""
         self._chk(lib.avbd_set_residual_target(self.handle, rel_tol, max_iters, check_every))
+
+    # --- learned warm start (BF16 MFMA MLP) --------------------------------
+    IN, OUT = 24, 3
+
+    def mlp_init(self, seed: int = 0) -> None:
+        self._chk(lib.avbd_mlp_init(self.handle, seed))
+
+    def mlp_params(self) -> np.ndarray:
+        n = lib.avbd_mlp_param_count()
+        p = np.empty(n, dtype=np.float32)
+        self._chk(lib.avbd_mlp_get_params(self.handle, _fptr(p), n))
+        return p
+
+    def mlp_set_params(self, p) -> None:
+        p = _f32(p)
+        self._chk(lib.avbd_mlp_set_params(self.handle, _fptr(p), p.shape[0]))
+
+    def mlp_set_normalization(self, norm) -> None:
+        norm = _f32(norm)
+        assert norm.shape[0] == 2 * self.IN + 2 * self.OUT
+        self._chk(lib.avbd_mlp_set_normalization(self.handle, _fptr(norm)))
+
+    def mlp_forward_backward(self, X, T):
+        X, T = _f32(X), _f32(T)
+        n = lib.avbd_mlp_param_count() - 2 * self.IN - 2 * self.OUT
+        g = np.empty(n, dtype=np.float32)
+        loss = C.c_float()
+        self._chk(lib.avbd_mlp_forward_backward(self.handle, _fptr(X), _fptr(T), X.shape[0], _fptr(g), n,
+                                                C.byref(loss)))
+        return g, loss.value
+
+    def mlp_predict(self, X) -> np.ndarray:
+        X = _f32(X)
+        out = np.empty((X.shape[0], self.OUT), dtype=np.float32)
+        self._chk(lib.avbd_mlp_predict(self.handle, _fptr(X), X.shape[0], _fptr(out)))
+        return out
+
+    def mlp_train(self, X, T, epochs: int, batch: int, lr: float, seed: int = 0) -> float:
+        X, T = _f32(X), _f32(T)
+        loss = C.c_float()
+        self._chk(lib.avbd_mlp_train(self.handle, _fptr(X), _fptr(T), X.shape[0], epochs, batch, lr, seed,
+                                     C.byref(loss)))
+        return loss.value
+
+    def record_steps(self, n_steps: int, n_items: int):
+        """Run n_steps recording (features, targets) of every positional work item."""
+        cap = n_steps * n_items
+        X = np.empty((cap, self.IN), dtype=np.float32)
+        T = np.empty((cap, self.OUT), dtype=np.float32)
+        rows = C.c_uint32()
+        self._chk(lib.avbd_record_steps(self.handle, n_steps, _fptr(X), _fptr(T), cap, C.byref(rows)))
+        return X[:rows.value], T[:rows.value]
+
+    def set_warm_start(self, enable: bool) -> None:
+        self._chk(lib.avbd_set_warm_start(self.handle, int(enable)))
+
+    def mlp_gemm_test(self, A, B, ta: bool, tb: bool) -> np.ndarray:
+        A, B = _f32(A), _f32(B)
+        M, K = A.shape
+        N = B.shape[1]
+        Cm = np.empty((M, N), dtype=np.float32)
+        self._chk(lib.avbd_mlp_gemm_test(self.handle, M, N, K, int(ta), int(tb), _fptr(A), _fptr(B), _fptr(Cm)))
+        return Cm
 
     def residual_history(self) -> np.ndarray:
         """Relative residuals at the checks of the last step (residual-targeted mode)."""

@@ -1,0 +1,122 @@
+"""Learned warm start: training-data generation with the plain GPU solver,
+training of the per-vertex MLP on the GPU (the library's BF16 MFMA forward and
+backward kernels, Adam on FP32 master weights), and storage of the trained
+parameters.
+
+The network predicts, per free vertex, the correction from the inertial /
+adaptive initial guess to the converged end-of-step position, in units of
+dt^2 g; the solver starts from the prediction when it lowers the scene's
+incremental potential and then runs plain VBD iterations as usual.
+
+    python -m avbd_amd.warm_start --out avbd_amd/data/warm_start_mlp.npz
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+from pathlib import Path
+from typing import List, Tuple
+
+import numpy as np
+
+from . import scenes
+from .scenes import SplitMix64
+
+DEFAULT_PATH = Path(__file__).resolve().parent / "data" / "warm_start_mlp.npz"
+
+
+def training_scenes(n_scenes: int, seed: int, n_min: int = 8, n_max: int = 14) -> List[scenes.Scene]:
+    """Dropped Kuhn cubes in the style of the 32^3 workload: random size,
+    Young's modulus (log-uniform 1e5..1e7), Poisson ratio (0.3..0.48),
+    rotation and drop height (0.1..0.6 m)."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(n_scenes):
+        n = rng.randint(n_min, n_max)
+        E = float(10 ** (5 + 2 * rng.uniform()))
+        nu = 0.3 + 0.18 * rng.uniform()
+        h = 0.1 + 0.5 * rng.uniform()
+        out.append(scenes.dropped_cube(n=n, seed=1000 + 7 * seed + i, height=h, E=E, nu=nu))
+    return out
+
+
+def generate(scene_list, steps: int, tol: float, max_iters: int, check_every: int = 10, keep: float = 0.25,
+             seed: int = 0, log=print) -> Tuple[np.ndarray, np.ndarray]:
+    """Run every scene for `steps` steps with the plain solver iterating to the
+    residual target, recording features and targets; keep a random fraction
+    of the rows."""
+    from .sim import Simulation
+    rng = np.random.default_rng(seed)
+    Xs, Ts = [], []
+    for k, sc in enumerate(scene_list):
+        sim = Simulation(sc, residual_target=(tol, max_iters, check_every))
+        sim.setup()
+        n_items = int(sum(int((m.fixed == 0).sum()) if m.fixed is not None else m.rest.shape[0] for m in sc.meshes))
+        X, T = sim.ctx.record_steps(steps, n_items)
+        sel = rng.random(X.shape[0]) < keep
+        Xs.append(X[sel])
+        Ts.append(T[sel])
+        log(f"[warm-start] scene {k}: {sc.n_vertices} vertices, {steps} steps, {int(sel.sum())} rows kept")
+        sim.ctx.close()
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def normalization(X: np.ndarray, T: np.ndarray) -> np.ndarray:
+    fm, fs = X.mean(0), X.std(0)
+    fs = np.where(fs > 1e-6, fs, 1.0)
+    tm, ts = T.mean(0), T.std(0)
+    ts = np.where(ts > 1e-9, ts, 1.0)
+    return np.concatenate([fm, fs, tm, ts]).astype(np.float32)
+
+
+def train(ctx, X: np.ndarray, T: np.ndarray, epochs: int = 8, batch: int = 4096, lr: float = 1e-3,
+          seed: int = 0) -> Tuple[np.ndarray, float]:
+    """Train on the device; returns the parameter vector and the last epoch's loss."""
+    ctx.mlp_init(seed)
+    ctx.mlp_set_normalization(normalization(X, T))
+    batch = min(batch, (X.shape[0] // 16) * 16)
+    loss = ctx.mlp_train(X, T, epochs, batch, lr, seed)
+    return ctx.mlp_params(), loss
+
+
+def save(path: Path, params: np.ndarray, **meta) -> None:
+    path.parent.mkdir(parents=True, exist_ok=True)
+    np.savez(path, params=params.astype(np.float32), **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load(path: Path = DEFAULT_PATH) -> np.ndarray:
+    with np.load(path, allow_pickle=False) as z:
+        return z["params"].astype(np.float32)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=str(DEFAULT_PATH))
+    ap.add_argument("--scenes", type=int, default=12)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--tol", type=float, default=1e-2)
+    ap.add_argument("--max-iters", type=int, default=300)
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=5)
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    t0 = time.perf_counter()
+    X, T = generate(training_scenes(args.scenes, args.seed), args.steps, args.tol, args.max_iters, log=log)
+    log(f"[warm-start] {X.shape[0]} rows in {time.perf_counter() - t0:.1f}s")
+    from .sim import SimulationContext
+    ctx = SimulationContext(dt=1 / 60, solver_iters=1)
+    ctx.add_particles(np.zeros((1, 3)), mass=np.ones(1))
+    ctx.build_graph()
+    t1 = time.perf_counter()
+    params, loss = train(ctx, X, T, args.epochs, args.batch, args.lr, args.seed)
+    log(f"[warm-start] trained {args.epochs} epochs in {time.perf_counter() - t1:.1f}s, final loss {loss:.4f}")
+    save(Path(args.out), params, rows=X.shape[0], loss=loss, tol=args.tol, scenes=args.scenes, steps=args.steps)
+    log(f"[warm-start] wrote {args.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--max-iters", type=int, default=300, help="iteration budget per step of the residual leg")
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--res-steps", type=int, default=60, help="timed steps of the residual-targeted leg")
+    ap.add_argument("--warm-weights", default=None,
+                    help="trained warm-start parameters (.npz); default: avbd_amd/data/warm_start_mlp.npz if present")
     ap.add_argument("--calibrate", type=int, default=0,
                     help="also run one plain copy kernel of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -169,11 +171,14 @@ def main():
         except Exception:
             pass
 
-    # headline leg: plain AVBD iterating each step to the residual target
-    res = None
-    if args.res_steps > 0:
+    # headline leg: AVBD iterating each step to the residual target, plain
+    # and (when trained parameters exist) warm-started by the network
+    def residual_leg(params=None):
         rsim = Simulation(make_scene(args.workload), residual_target=(args.tol, args.max_iters, args.check_every))
         rsim.setup()
+        if params is not None:
+            rsim.ctx.mlp_set_params(params)
+            rsim.ctx.set_warm_start(True)
         rsim.ctx.step(args.warmup)
         rsim.ctx.synchronize()
         s0 = rsim.ctx.stats()
@@ -183,23 +188,41 @@ def main():
         rel = time.perf_counter() - t0
         s1 = rsim.ctx.stats()
         iters = s1.iterations_total - s0.iterations_total
-        hist = rsim.ctx.residual_history()
-        res = {
-            "tol": args.tol,
-            "max_iters": args.max_iters,
-            "check_every": args.check_every,
-            "steps": args.res_steps,
+        out = {
             "steps_per_s": round(args.res_steps / rel, 3),
             "ms_per_step": round(rel / args.res_steps * 1e3, 4),
             "mean_iterations": round(iters / args.res_steps, 2),
             "vertex_updates_per_s": round((s1.vertex_updates - s0.vertex_updates) / rel, 1),
             "last_step_iterations": s1.last_step_iterations,
-            "last_step_residual_history": [float(f"{v:.4g}") for v in hist],
+            "last_step_residual_history": [float(f"{v:.4g}") for v in rsim.ctx.residual_history()],
             "final_residual_rel": s1.residual_rel,
-            "warm_start": None,
         }
-        print(f"[bench] residual leg: {res['steps_per_s']:.1f} steps/s, mean {res['mean_iterations']} iterations, "
-              f"final residual {s1.residual_rel:.3e}", file=sys.stderr, flush=True)
+        if params is not None:
+            out["predictions"] = int(s1.warm_start_predictions - s0.warm_start_predictions)
+            out["fallbacks_to_inertial_start"] = int(s1.warm_start_fallbacks - s0.warm_start_fallbacks)
+        rsim.ctx.close()
+        return out
+
+    res = None
+    if args.res_steps > 0:
+        plain = residual_leg()
+        print(f"[bench] residual leg (plain): {plain['steps_per_s']:.1f} steps/s, mean {plain['mean_iterations']} "
+              f"iterations, final residual {plain['final_residual_rel']:.3e}", file=sys.stderr, flush=True)
+        warm = None
+        from avbd_amd import warm_start
+        wpath = Path(args.warm_weights) if args.warm_weights else warm_start.DEFAULT_PATH
+        if wpath.exists():
+            warm = residual_leg(warm_start.load(wpath))
+            warm["weights"] = str(wpath.relative_to(ROOT)) if wpath.is_relative_to(ROOT) else str(wpath)
+            print(f"[bench] residual leg (warm start): {warm['steps_per_s']:.1f} steps/s, mean "
+                  f"{warm['mean_iterations']} iterations, {warm['fallbacks_to_inertial_start']} fallbacks",
+                  file=sys.stderr, flush=True)
+        best = warm if warm is not None else plain
+        res = {"tol": args.tol, "max_iters": args.max_iters, "check_every": args.check_every,
+               "steps": args.res_steps, "plain": plain, "warm_start": warm,
+               "speedup_warm_vs_plain": round(warm["steps_per_s"] / plain["steps_per_s"], 3) if warm else None,
+               "steps_per_s": best["steps_per_s"], "ms_per_step": best["ms_per_step"],
+               "vertex_updates_per_s": best["vertex_updates_per_s"]}
 
     calib = None
     if args.calibrate:
@@ -233,8 +256,9 @@ def main():
             "ground_contacts": sc.n_ground,
             "iterations": sc.iters,
             "dt": sc.dt,
-            "mode": (f"plain AVBD iterating each step to relative force residual {args.tol:g} "
-                     f"(budget {args.max_iters}, checked every {args.check_every})") if res
+            "mode": (f"AVBD iterating each step to relative force residual {args.tol:g} "
+                     f"(budget {args.max_iters}, checked every {args.check_every}); "
+                     + ("warm-started by the learned MLP" if res and res["warm_start"] else "plain start")) if res
                     else "plain AVBD, fixed iteration count",
             "broad_phase": "LBVH over the 12,288 surface triangles rebuilt and self-traversed every step",
             "self_collision": False,

@@ -172,6 +172,8 @@ typedef struct {
     uint64_t iterations_total;      /* Gauss-Seidel iterations over all steps   */
     uint32_t last_step_iterations;  /* iterations of the last step              */
     uint32_t pad_;
+    uint64_t warm_start_predictions; /* scene-steps started from the network    */
+    uint64_t warm_start_fallbacks;   /* of those, kept the inertial start       */
 } avbd_stats;
 
 /* Context lifetime -------------------------------------------------------- */
@@ -312,6 +314,44 @@ int avbd_download_bvh(avbd_ctx* ctx, uint64_t* morton, uint32_t* sorted, uint32_
  * traversal: up to cap pairs into pairs[2*cap]; *count = total found
  * (> cap means truncated).  Order is unspecified. */
 int avbd_download_candidates(avbd_ctx* ctx, uint32_t* pairs, uint32_t cap, uint32_t* count);
+
+/* Learned warm start (new; replaces the inertial / adaptive initial guess of
+ * _particleInertialUpdate, simulation/VBDSolver.hpp:208-248, whenever it
+ * lowers the incremental potential of the scene).  A per-vertex MLP
+ * (24 features -> 128 -> 128 -> 128 -> 3, SiLU, BF16 weights and activations,
+ * FP32 accumulation on the matrix cores) predicts the step displacement from
+ * the initial guess.  Parameters: for each layer W (out x in, row-major, in
+ * padded 24 -> 32 and out 3 -> 16 with zeros) then b (out); then feature
+ * mean (24), feature std (24), target mean (3), target std (3).  Features and
+ * targets are FP32 rows of 24 and 3 (targets in units of dt^2 g). */
+uint64_t avbd_mlp_param_count(void);
+int avbd_mlp_init(avbd_ctx* ctx, uint64_t seed);
+int avbd_mlp_set_params(avbd_ctx* ctx, const float* params, uint64_t count);
+int avbd_mlp_get_params(avbd_ctx* ctx, float* params, uint64_t count);
+/* feature mean (24), feature std (24), target mean (3), target std (3) */
+int avbd_mlp_set_normalization(avbd_ctx* ctx, const float* norm);
+/* One forward + backward pass (MSE on the normalised targets); grads has the
+ * parameter layout without the normalisation block; *loss = mean loss. */
+int avbd_mlp_forward_backward(avbd_ctx* ctx, const float* feats, const float* tgts, uint32_t rows, float* grads,
+                              uint64_t count, float* loss);
+int avbd_mlp_predict(avbd_ctx* ctx, const float* feats, uint32_t rows, float* out);
+/* Adam over the data set for `epochs` passes of minibatches (batch % 16 == 0);
+ * *loss = mean loss of the last epoch. */
+int avbd_mlp_train(avbd_ctx* ctx, const float* feats, const float* tgts, uint32_t rows, uint32_t epochs,
+                   uint32_t batch, float lr, uint64_t seed, float* loss);
+/* n_steps time steps that also record, per positional work item, the
+ * features at the start of the step and the target (x_end - x_start)/(dt^2 g);
+ * rows = n_steps * items, in work-item order per step. */
+int avbd_record_steps(avbd_ctx* ctx, uint32_t n_steps, float* feats, float* tgts, uint32_t cap_rows,
+                      uint32_t* rows);
+/* Start every step from the network's prediction (with the per-scene
+ * incremental-potential fallback) or, with enable = 0, from the inertial guess. */
+int avbd_set_warm_start(avbd_ctx* ctx, int enable);
+/* Test hook: C (M x N) = A (M x K) B (K x N) through the BF16 MFMA GEMM
+ * kernel, with A (ta) or B (tb) stored transposed on the device; FP32 row-major
+ * host arrays, sizes multiples of 16. */
+int avbd_mlp_gemm_test(avbd_ctx* ctx, uint32_t M, uint32_t N, uint32_t K, int ta, int tb, const float* A,
+                       const float* B, float* C);
 
 /* Measurement helper with no counterpart in the reference: runs a plain
  * device-to-device streaming copy kernel (calib_copy_kernel) of `bytes` bytes

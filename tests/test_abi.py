@@ -15,7 +15,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def declared_symbols():
     text = (ROOT / "include" / "avbd.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|uint32_t|const char\*)\s+(avbd_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|uint32_t|uint64_t|const char\*)\s+(avbd_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

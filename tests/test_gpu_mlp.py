@@ -1,0 +1,181 @@
+"""GPU tests of the learned warm start: the BF16 MFMA GEMM (lane layout and
+transpositions against numpy), the network's forward and backward pass
+against a numpy restatement fed the same bf16-rounded operands, training that
+lowers the loss, and the warm-started solver against the plain one.
+
+Tolerances: GEMM results equal an FP64 product of the bf16 inputs to FP32
+accumulation error (rel 1e-5 of the row norm); gradients agree with the
+numpy restatement to 2 % of each layer's gradient norm (bf16 rounding of the
+back-propagated activations can flip at rounding boundaries)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+def bf16(a):
+    """Round to bfloat16 (nearest even), returned as float32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+IN, INP, HID, OUT, OUTP = 24, 32, 128, 3, 16
+SHAPES = [(HID, INP), (HID, HID), (HID, HID), (OUTP, HID)]
+
+
+def unpack(p):
+    layers, off = [], 0
+    for (o, i) in SHAPES:
+        W = p[off:off + o * i].reshape(o, i)
+        off += o * i
+        b = p[off:off + o]
+        off += o
+        layers.append((W, b))
+    norm = p[off:off + 2 * IN + 2 * OUT]
+    return layers, norm
+
+
+def reference_grads(p, X, T):
+    """numpy restatement of the device forward/backward: bf16 operands,
+    FP64 accumulation, MSE on the 3 outputs."""
+    layers, norm = unpack(p)
+    fm, fs, tm, ts = norm[:IN], norm[IN:2 * IN], norm[2 * IN:2 * IN + OUT], norm[2 * IN + OUT:]
+    rows = X.shape[0]
+    x = np.zeros((rows, INP), np.float32)
+    x[:, :IN] = bf16((X - fm) / fs)
+    t = (T - tm) / ts
+    acts, zs = [x.astype(np.float64)], []
+    h = x.astype(np.float64)
+    for li, (W, b) in enumerate(layers):
+        z = h @ bf16(W).astype(np.float64).T + b
+        zs.append(z)
+        if li < 3:
+            h = bf16(silu(z).astype(np.float32)).astype(np.float64)
+            acts.append(h)
+    y = zs[-1]
+    d = np.zeros_like(y)
+    d[:, :OUT] = 2.0 * (y[:, :OUT] - t) / rows
+    d = bf16(d.astype(np.float32)).astype(np.float64)
+    loss = float(np.sum((y[:, :OUT] - t) ** 2) / rows)
+    grads = [None] * 4
+    for li in range(3, -1, -1):
+        W, b = layers[li]
+        gW = d.T @ acts[li]
+        gb = d.sum(axis=0)
+        grads[li] = (gW, gb)
+        if li > 0:
+            dh = d @ bf16(W).astype(np.float64)
+            d = bf16((dh * dsilu(zs[li - 1])).astype(np.float32)).astype(np.float64)
+    return grads, loss
+
+
+def test_mfma_gemm_layout_and_transposes():
+    banner("bf16 MFMA GEMM vs numpy (all transposition variants)")
+    from avbd_amd.sim import SimulationContext
+    ctx = SimulationContext(dt=1 / 60, solver_iters=1)
+    ctx.add_particles(np.zeros((1, 3)), mass=np.ones(1))
+    ctx.build_graph()
+    rng = np.random.default_rng(0)
+    M, N, K = 48, 32, 64
+    A = bf16(rng.normal(size=(M, K)))
+    B = bf16(rng.normal(size=(K, N)))
+    for ta in (False, True):
+        for tb in (False, True):
+            if (ta, tb) == (True, True):
+                continue
+            C = ctx.mlp_gemm_test(A, B, ta, tb)
+            want = A.astype(np.float64) @ B.astype(np.float64)
+            err = np.abs(C - want).max() / np.abs(want).max()
+            banner(f"  ta={ta} tb={tb}: rel err {err:.2e}")
+            assert err < 1e-5
+
+
+def test_mlp_gradients_match_numpy():
+    banner("MLP forward/backward vs numpy restatement")
+    from avbd_amd.sim import SimulationContext
+    ctx = SimulationContext(dt=1 / 60, solver_iters=1)
+    ctx.add_particles(np.zeros((1, 3)), mass=np.ones(1))
+    ctx.build_graph()
+    ctx.mlp_init(7)
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(100, IN)).astype(np.float32)
+    T = rng.normal(size=(100, OUT)).astype(np.float32)
+    norm = np.concatenate([X.mean(0), X.std(0) + 1e-3, T.mean(0), T.std(0) + 1e-3]).astype(np.float32)
+    ctx.mlp_set_normalization(norm)
+    p = ctx.mlp_params()
+    g, loss = ctx.mlp_forward_backward(X, T)
+    ref, ref_loss = reference_grads(p, X, T)
+    banner(f"  loss gpu {loss:.6f} numpy {ref_loss:.6f}")
+    assert loss == pytest.approx(ref_loss, rel=1e-3)
+    off = 0
+    for li, (o, i) in enumerate(SHAPES):
+        gW = g[off:off + o * i].reshape(o, i)
+        off += o * i
+        gb = g[off:off + o]
+        off += o
+        rW, rb = ref[li]
+        eW = np.linalg.norm(gW - rW) / max(np.linalg.norm(rW), 1e-12)
+        eb = np.linalg.norm(gb - rb) / max(np.linalg.norm(rb), 1e-12)
+        banner(f"  layer {li}: |dW| rel err {eW:.2e}, |db| rel err {eb:.2e}")
+        assert eW < 2e-2 and eb < 2e-2
+
+
+def test_training_and_warm_started_solver():
+    """Record (features, targets) with the plain solver on small dropped cubes,
+    train on the device, then step a held-out cube warm-started and plain to
+    the same residual target: the network's loss falls, every step records a
+    prediction (with the incremental-potential fallback counted), both runs
+    stay together, and the iteration counts of the two are reported."""
+    banner("warm start: record, train, evaluate")
+    from avbd_amd import scenes
+    from avbd_amd import warm_start as W
+    from avbd_amd.sim import Simulation, SimulationContext
+    scs = W.training_scenes(3, seed=1, n_min=5, n_max=7)
+    X, T = W.generate(scs, steps=30, tol=1e-2, max_iters=200, keep=1.0, log=banner)
+    assert X.shape[1] == IN and T.shape[1] == OUT and np.isfinite(X).all() and np.isfinite(T).all()
+    ctx = SimulationContext(dt=1 / 60, solver_iters=1)
+    ctx.add_particles(np.zeros((1, 3)), mass=np.ones(1))
+    ctx.build_graph()
+    ctx.mlp_init(0)
+    ctx.mlp_set_normalization(W.normalization(X, T))
+    _, loss0 = ctx.mlp_forward_backward(X[:2048], T[:2048])
+    loss = ctx.mlp_train(X, T, 8, 512, 1e-3, 0)
+    banner(f"  {X.shape[0]} rows, loss {loss0:.4f} -> {loss:.4f}")
+    assert loss < 0.7 * loss0
+    params = ctx.mlp_params()
+
+    sc = scenes.dropped_cube(n=6, seed=77, height=0.3)
+    plain = Simulation(sc, residual_target=(1e-2, 200, 5))
+    warm = Simulation(sc, residual_target=(1e-2, 200, 5))
+    plain.setup()
+    warm.setup()
+    warm.ctx.mlp_set_params(params)
+    warm.ctx.set_warm_start(True)
+    plain.update(40)
+    warm.update(40)
+    sp, sw = plain.ctx.stats(), warm.ctx.stats()
+    banner(f"  iterations plain {sp.iterations_total} warm {sw.iterations_total}; "
+           f"predictions {sw.warm_start_predictions}, fallbacks {sw.warm_start_fallbacks}")
+    assert sw.warm_start_predictions == 40 and sp.warm_start_predictions == 0
+    xp, xw = plain.positions(), warm.positions()
+    assert np.isfinite(xw).all()
+    assert np.abs(xw - xp).max() < 0.05 * bbox(xp)
+
+
+def bbox(x):
+    return float(np.max(x.max(axis=0) - x.min(axis=0)))
