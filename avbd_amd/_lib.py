@@ -173,6 +173,7 @@ SIGNATURES = {
                                _D, _D, C.c_int, C.c_int, C.c_int, _U32]),
     "avbd_add_sphere": (C.c_int, [_P, C.c_double, C.c_double, _D, _D, C.c_int, C.c_int, _U32]),
     "avbd_set_rotation_state": (C.c_int, [_P, C.c_uint32, C.c_uint32, _D, _D]),
+    "avbd_add_mass": (C.c_int, [_P, C.c_uint32, C.c_double, _D]),
     "avbd_set_residual_target": (C.c_int, [_P, C.c_double, C.c_uint32, C.c_uint32]),
     "avbd_get_residual_history": (C.c_int, [_P, _D, C.c_uint32, _U32]),
     "avbd_mlp_param_count": (C.c_uint64, []),

@@ -454,7 +454,7 @@ void capture_graph(avbd_ctx* c) {
 // (lanes per vertex, block size), writing into a scratch sink so that the
 // state is untouched, and keep the fastest.
 void autotune(avbd_ctx* c) {
-    const uint32_t lanes_c[] = {4, 8, 16};
+    const uint32_t lanes_c[] = {4, 8, 16, 32};
     const uint32_t block_c[] = {64, 128, 256};
     float best = 1e30f;
     LaunchCfg best_cfg = c->cfg;
@@ -893,6 +893,17 @@ int avbd_set_fixed(avbd_ctx* c, uint32_t first, uint32_t n, const uint8_t* fixed
     });
 }
 
+int avbd_add_mass(avbd_ctx* c, uint32_t p, double dm, const double* dI) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (p >= c->mass.size()) throw std::runtime_error("particle out of range");
+        c->mass[p] += dm;
+        const int32_t r = c->rot_of[p];
+        if (dI && r >= 0)
+            for (int k = 0; k < 3; k++) c->inertia[3 * r + k] += dI[k];
+    });
+}
+
 int avbd_set_rotation_state(avbd_ctx* c, uint32_t first, uint32_t n, const double* q, const double* w) {
     return guarded(c, [&] {
         if ((size_t)first + n > c->mass.size()) throw std::runtime_error("particle range out of bounds");
@@ -1134,6 +1145,17 @@ int avbd_build_graph(avbd_ctx* c) {
         upload(c, d.ground_of, gof);
         upload(c, d.fixed, c->fixed);
         upload(c, d.inc, c->plan.inc);
+        {
+            // per-incidence record copies (work-item order) for the vertex solve
+            if (n >= (1u << 30)) throw std::runtime_error("more than 2^30 particles");
+            std::vector<TetRec> ir(c->plan.inc.size());
+            for (size_t k = 0; k < ir.size(); k++) {
+                ir[k] = tr[c->plan.inc[k] >> 2];
+                ir[k].idx[3] |= (c->plan.inc[k] & 3u) << 30;
+            }
+            d.irec = dev_alloc<TetRec>(c, ir.size());
+            upload(c, d.irec, ir);
+        }
         upload(c, d.items, c->plan.items);
         upload(c, c->d_conflicts, c->plan.conflict_list);
         upload_particle_range(c, 0, n);

@@ -138,6 +138,8 @@ struct DevState {
     TetDamp* damp = nullptr;
     GroundRec* ground = nullptr;
     uint32_t* inc = nullptr;       // (tet << 2) | local, grouped per work item
+    TetRec* irec = nullptr;        // copy of the tet record per incidence, in inc order,
+                                   // local vertex index in the top two bits of idx[3]
     VItem* items = nullptr;        // all colours back to back
     double* reduce = nullptr;      // residual / IP partial sums
     uint32_t n = 0, n_tets = 0, n_ground = 0, n_inc = 0, n_items = 0;

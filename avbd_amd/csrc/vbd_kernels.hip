@@ -53,14 +53,11 @@ __device__ __forceinline__ void load_tet(const TetRec* __restrict__ tets, uint32
 // (NeoHookeanEnergySolver::accumulate, energy/elastic/NeoHookeanEnergySolver.hpp:121-227).
 // q = Q.row(local) for local < 3, -(Q.row0 + Q.row1 + Q.row2) for local 3 (the
 // reference's sign = -1 folded into q; every term is linear in q or quadratic).
+// The record (idx, r1..r3) is already loaded; t indexes the damping record.
 template <bool DAMP>
-__device__ __forceinline__ void nh_incidence(const float4* x, const TetRec* __restrict__ tets,
-                                             const TetDamp* __restrict__ damp, uint32_t e, float inv_dt,
-                                             float3& g, Sym3& h) {
-    const uint32_t t = e >> 2, local = e & 3u;
-    uint4 idx;
-    float4 r1, r2, r3;
-    load_tet(tets, t, idx, r1, r2, r3);
+__device__ __forceinline__ void nh_incidence_rec(const float4* x, uint4 idx, float4 r1, float4 r2, float4 r3,
+                                                 uint32_t local, const TetDamp* __restrict__ damp, uint32_t t,
+                                                 float inv_dt, float3& g, Sym3& h) {
     // r1 = q00 q01 q02 q10 ; r2 = q11 q12 q20 q21 ; r3 = q22 VL VM alpha
     const float3 p0 = f3(x[idx.x]), p1 = f3(x[idx.y]), p2 = f3(x[idx.z]), p3 = f3(x[idx.w]);
     const float3 d0 = p0 - p3, d1 = p1 - p3, d2 = p2 - p3;
@@ -120,6 +117,31 @@ __device__ __forceinline__ void nh_incidence(const float4* x, const TetRec* __re
         h.a02 += a * (q2 * ff02 + Fq.x * Fq.z);
         h.a12 += a * (q2 * ff12 + Fq.y * Fq.z);
     }
+}
+
+template <bool DAMP>
+__device__ __forceinline__ void nh_incidence(const float4* x, const TetRec* __restrict__ tets,
+                                             const TetDamp* __restrict__ damp, uint32_t e, float inv_dt,
+                                             float3& g, Sym3& h) {
+    uint4 idx;
+    float4 r1, r2, r3;
+    load_tet(tets, e >> 2, idx, r1, r2, r3);
+    nh_incidence_rec<DAMP>(x, idx, r1, r2, r3, e & 3u, damp, e >> 2, inv_dt, g, h);
+}
+
+// Incidence k of the work-item order, read from the per-incidence copy of
+// the tet records (contiguous for an item's lanes, no index indirection);
+// the local vertex index sits in the top two bits of idx.w.
+template <bool DAMP>
+__device__ __forceinline__ void nh_incidence_k(const float4* x, const TetRec* __restrict__ irec,
+                                               const uint32_t* __restrict__ inc, const TetDamp* __restrict__ damp,
+                                               uint32_t k, float inv_dt, float3& g, Sym3& h) {
+    uint4 idx;
+    float4 r1, r2, r3;
+    load_tet(irec, k, idx, r1, r2, r3);
+    const uint32_t local = idx.w >> 30;
+    idx.w &= 0x3fffffffu;
+    nh_incidence_rec<DAMP>(x, idx, r1, r2, r3, local, damp, DAMP ? (inc[k] >> 2) : 0u, inv_dt, g, h);
 }
 
 // Ground AL contact contribution (CollisionConstraintEnergySolver::accumulate<3>,
@@ -185,14 +207,14 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 // by the L lanes of a group; every lane of the group must call it.
 template <int L, bool DAMP>
 __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const uint32_t* __restrict__ inc,
-                                           const TetRec* __restrict__ tets, const TetDamp* __restrict__ damp,
+                                           const TetRec* __restrict__ irec, const TetDamp* __restrict__ damp,
                                            float4* x, const float4* __restrict__ y,
                                            const GroundRec* __restrict__ ground, float4* x_out, float4* x_buf,
                                            float inv_dt, float inv_dt2) {
     const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
     float3 g = make_float3(0.f, 0.f, 0.f);
     Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (uint32_t j = lane; j < cnt; j += L) nh_incidence<DAMP>(x, tets, damp, inc[it.inc_begin + j], inv_dt, g, h);
+    for (uint32_t j = lane; j < cnt; j += L) nh_incidence_k<DAMP>(x, irec, inc, damp, it.inc_begin + j, inv_dt, g, h);
     g.x = group_sum<L>(g.x);
     g.y = group_sum<L>(g.y);
     g.z = group_sum<L>(g.z);
@@ -221,7 +243,7 @@ __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const
 template <int L, bool DAMP>
 __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restrict__ items, uint32_t n_items,
                                                            const uint32_t* __restrict__ inc,
-                                                           const TetRec* __restrict__ tets,
+                                                           const TetRec* __restrict__ irec,
                                                            const TetDamp* __restrict__ damp, float4* x,
                                                            const float4* __restrict__ y,
                                                            const GroundRec* __restrict__ ground, float4* x_out,
@@ -232,7 +254,7 @@ __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restri
     const uint32_t item = tid / L;
     const uint32_t lane = tid % L;
     if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)
-    solve_item<L, DAMP>(items[item], lane, inc, tets, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2);
+    solve_item<L, DAMP>(items[item], lane, inc, irec, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2);
 }
 
 // E_prev = 0.5 (F_prev^T F_prev - I) from previous positions
@@ -448,7 +470,7 @@ __device__ __forceinline__ float residual_item(const VItem& it, uint32_t lane, c
     const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
     float3 g = make_float3(0.f, 0.f, 0.f);
     Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (uint32_t j = lane; j < cnt; j += L) nh_incidence<DAMP>(d.x, d.tets, d.damp, d.inc[it.inc_begin + j], inv_dt, g, h);
+    for (uint32_t j = lane; j < cnt; j += L) nh_incidence_k<DAMP>(d.x, d.irec, d.inc, d.damp, it.inc_begin + j, inv_dt, g, h);
     g.x = group_sum<L>(g.x);
     g.y = group_sum<L>(g.y);
     g.z = group_sum<L>(g.z);
@@ -834,7 +856,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
             const uint32_t per = (cnt + G - 1) / G;
             const uint32_t lo = min(cnt, lb * per), hi = min(cnt, lo + per);
             for (uint32_t k = lo + grp; k < hi; k += groups)
-                solve_item<L, DAMP>(d.items[beg + k], lane, d.inc, d.tets, d.damp, d.x, d.y, d.ground, d.x, d.x_buf,
+                solve_item<L, DAMP>(d.items[beg + k], lane, d.inc, d.irec, d.damp, d.x, d.y, d.ground, d.x, d.x_buf,
                                     inv_dt, inv_dt2);
             if (d.n_oitems) {
                 const uint32_t ob = a.ocolor_offsets[c], ocnt = a.ocolor_offsets[c + 1] - ob;
@@ -1010,7 +1032,7 @@ void solve_dispatch(hipStream_t s, const DevState& d, uint32_t block, uint32_t b
                     float4* x_out, const uint32_t* done) {
     const uint32_t threads = count * L;
     hipLaunchKernelGGL((solve_color_kernel<L, DAMP>), dim3(blocks_for(threads, block)), dim3(block), 0, s,
-                       d.items + begin, count, d.inc, d.tets, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
+                       d.items + begin, count, d.inc, d.irec, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
                        1.0f / dt, 1.0f / (dt * dt), done);
 }
 

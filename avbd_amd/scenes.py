@@ -150,6 +150,10 @@ class Rod:
     tip_fixed: bool = False
     ground: bool = False
     base_joint: bool = False
+    # rigid sphere welded at the tip node: (radius, mass, body-frame angular
+    # velocity); adds its mass and inertia 2/5 m r^2 to the node and a sphere
+    # ground contact
+    tip_sphere: Optional[Tuple[float, float, Tuple[float, float, float]]] = None
 
 
 @dataclass
@@ -275,6 +279,27 @@ def rods_and_spheres(n_rods: int = 4, elements: int = 16, seed: int = 3, dt: flo
         w0 = tuple(float(10.0 * (rng.uniform() - 0.5)) for _ in range(3))
         spheres.append(Sphere(radius=0.02, density=1500.0, pos=(0.1 * i, 0.02, 0.3), w0=w0))
     return Scene("rods_and_spheres", rods=rods, spheres=spheres, dt=dt, iters=iters, rho=0.95, steps=steps)
+
+
+def rod_grid(nx: int = 16, nz: int = 16, segments: int = 128, seed: int = 3, dt: float = 1.0 / 120.0,
+             iters: int = 20, steps: int = 120) -> Scene:
+    """Rods + rigid bodies: nx x nz Cosserat rods at 5 cm spacing, 1 m long,
+    radius 5 mm, E = 1e7 Pa, nu = 0.3 (G = E/2.6), density 1100 kg/m^3, hanging
+    down from a one-sided fixed joint at the top node, a rigid sphere
+    (r = 2 cm, m = 0.05 kg) welded at the tip with angular velocity
+    U[-5, 5]^3 rad/s (SplitMix64(seed)); every node and sphere has a
+    rigid-ground contact.  Tops at y = 1.05 m, so the spheres hang 3 cm above
+    the ground."""
+    rng = SplitMix64(seed)
+    down = quat_about((1.0, 0.0, 0.0), np.pi / 2)   # body z -> world -y
+    rods = []
+    for i in range(nx):
+        for k in range(nz):
+            w0 = tuple(float(10.0 * rng.uniform() - 5.0) for _ in range(3))
+            rods.append(Rod(elements=segments, length=1.0, E=1e7, nu=0.3, density=1100.0, diameter=0.01,
+                            pos=(0.05 * i, 1.05, 0.05 * k), rot=down, ground=True, base_joint=True,
+                            tip_sphere=(0.02, 0.05, w0)))
+    return Scene("rod_grid", rods=rods, dt=dt, iters=iters, rho=0.95, steps=steps)
 
 
 def algorithmic_bytes_per_step(n_v: int, n_tet: int, n_g: int, iters: int) -> int:

@@ -138,6 +138,23 @@ class SimulationContext:
                                       int(sp.fixed), int(sp.ground), C.byref(first)))
         return first.value
 
+    def add_mass(self, p: int, dm: float, dI=None) -> None:
+        """Weld a rigid body's mass and principal inertia onto particle p."""
+        dI = None if dI is None else np.ascontiguousarray(dI, dtype=np.float64)
+        self._chk(lib.avbd_add_mass(self.handle, p, dm, _dptr(dI)))
+
+    def add_rigid_ground(self, p: int, shape: int, radius: float, k_start: float, mu_s: float = 0.5,
+                         mu_k: float = 0.2) -> None:
+        """RigidBodyGroundCollisionEnergyPool::addEnergy for a lowest-point shape."""
+        r = _lib.RigidGroundRecord()
+        r.particle_index = p
+        r.shape = shape
+        r.radius = radius
+        r.cp_rb_local[1] = -radius
+        r.k_start = r.k_n = r.k_t = r.k_b = k_start
+        r.mu_s, r.mu_k = mu_s, mu_k
+        self.add_energies(_lib.RIGID_BODY_GROUND_COLLISION, (_lib.RigidGroundRecord * 1)(r))
+
     def add_joint(self, p: int, pos_offset, rot_offset, ref_position, ref_rotation) -> None:
         """OneSidedFixedJointEnergyPool::addEnergy (energy/joint/OneSidedFixedJointEnergyPool.hpp:33-50):
         k = k_start = 1e4, lambda = 0, C_prev = 0."""
@@ -413,11 +430,15 @@ class Simulation:
             offs.append(first + m.rest.shape[0])
         self.offsets = offs
         self.rod_first: List[int] = []
+        self._spins: List[tuple] = []
         for r in self.scene.rods:
             self.rod_first.append(self._add_rod(r))
         self.sphere_index: List[int] = []
         for sp in self.scene.spheres:
             self.sphere_index.append(self._add_sphere(sp))
+        for p, _rot, w0 in self._spins:
+            if any(c != 0.0 for c in w0):
+                self.ctx.set_rotation_state(p, w=np.asarray([w0], dtype=np.float64))
         if len(self.scene.meshes) > 1:
             if self.scene.rods or self.scene.spheres:
                 raise ValueError("batched tet scenes cannot be mixed with rods or spheres")
@@ -430,7 +451,23 @@ class Simulation:
         if r.base_joint:
             # hold the first node at its initial pose
             self.ctx.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
+        if r.tip_sphere is not None:
+            rad, m, w0 = r.tip_sphere
+            tip = first + r.elements
+            self.ctx.add_mass(tip, m, np.full(3, 0.4 * m * rad * rad))
+            # the node's total mass sets k_start, as for any rigid-ground contact
+            node_mass = self._rod_node_mass(r, True) + m
+            self.ctx.add_rigid_ground(tip, _lib.SHAPE_SPHERE, rad, node_mass / (self.scene.dt * self.scene.dt))
+            self._spins.append((tip, r.rot, w0))
         return first
+
+    @staticmethod
+    def _rod_node_mass(r: Rod, end: bool) -> float:
+        """Lumped node mass of the Rod ctor (simobject/Rod.cpp:47-58): half an
+        element at the ends, a whole one inside."""
+        area = np.pi * (r.diameter / 2) ** 2
+        elem = r.length / r.elements * area * r.density
+        return 0.5 * elem if end else elem
 
     def _add_sphere(self, sp: Sphere) -> int:
         p = self.ctx.add_sphere(sp)

@@ -38,6 +38,8 @@ def make_scene(name: str):
         return scenes.hanging_cube(n=8)
     if name == "cube_batch_512":
         return scenes.cube_batch(n_scenes=512)
+    if name == "rod_grid":
+        return scenes.rod_grid()
     raise SystemExit(f"unknown workload {name}")
 
 

@@ -233,6 +233,10 @@ int avbd_add_tet_mesh(avbd_ctx* ctx, uint32_t nv, const double* rest_x, uint32_t
 int avbd_set_particle_state(avbd_ctx* ctx, uint32_t first, uint32_t n, const double* x, const double* x_prev,
                             const double* v, const double* v_prev);
 int avbd_set_fixed(avbd_ctx* ctx, uint32_t first, uint32_t n, const uint8_t* fixed);
+/* Add mass dm and principal inertia dI (3, NULL = none; oriented particles
+ * only) to particle p: a rigid body welded to it, e.g. a sphere at a rod tip
+ * (new; the reference builds such bodies as separate objects). */
+int avbd_add_mass(avbd_ctx* ctx, uint32_t p, double dm, const double* dI);
 /* Orientation q (4n: w, x, y, z) and angular velocity w (3n) of the oriented
  * particles among [first, first+n), also taken as their previous-step values
  * (rows of positional particles are ignored; NULL = unchanged). */

@@ -291,6 +291,33 @@ int64_t oracle_add_joint(void* h, uint32_t p, const double* pos_offset3, const d
                                                    q_in(ref_rot4));
 }
 
+// Add mass (and, for an oriented particle, principal inertia) to particle p:
+// a rigid body welded to it (a sphere at a rod tip).
+void oracle_add_mass(void* h, uint32_t p, double dm, const double* dI3) {
+    auto* s = static_cast<Simulation*>(h);
+    s->mass[p] += dm;
+    if (dI3 && s->oriented(p)) s->rs.inertia[s->rs.rot_of[p]] += v_in(dI3);
+}
+
+// RigidBodyGroundCollisionEnergyPool::addEnergy with a lowest-point shape
+// (shape 0 sphere, 1 rod) of the given radius.
+void oracle_add_rground(void* h, uint32_t p, uint32_t shape, double radius, double k_start, double mu_s, double mu_k) {
+    auto* s = static_cast<Simulation*>(h);
+    RigidGround c{};
+    c.p = p;
+    c.shape = shape;
+    c.radius = radius;
+    c.cp_rb_local = V3(0, -radius, 0);
+    c.cp_ground = V3();
+    c.n = V3(0, 1, 0);
+    c.t = V3(1, 0, 0);
+    c.b = V3(0, 0, 1);
+    c.k_start = c.k_n = c.k_t = c.k_b = k_start;
+    c.mu_s = mu_s;
+    c.mu_k = mu_k;
+    s->rground.push_back(c);
+}
+
 uint32_t oracle_num_cosserat(void* h) { return (uint32_t)static_cast<Simulation*>(h)->cos.size(); }
 uint32_t oracle_num_rground(void* h) { return (uint32_t)static_cast<Simulation*>(h)->rground.size(); }
 uint32_t oracle_num_joints(void* h) { return (uint32_t)static_cast<Simulation*>(h)->joints.size(); }

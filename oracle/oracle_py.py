@@ -265,23 +265,31 @@ def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
             o.set_velocities(m.v0, m.v0)
         if collide and m.triangles is not None and len(scene.meshes) == 1:
             o.set_collision_triangles(m.triangles)
+    spins = []
     for r in getattr(scene, "rods", []):
         first = o.add_rod(r.elements, r.length, r.curvature, r.E, r.nu, r.density, r.diameter, r.pos, r.rot,
                           r.vel, r.base_fixed, r.tip_fixed, r.ground)
         if r.base_joint:
             o.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
+        if getattr(r, "tip_sphere", None) is not None:
+            rad, m, w0 = r.tip_sphere
+            tip = first + r.elements
+            dI = np.full(3, 0.4 * m * rad * rad)
+            lib.oracle_add_mass(o.h, tip, m, dp(dI))
+            # sphere ground contact, k_start from the node's total mass (mu 0.5 / 0.2)
+            mass, _ = o.mass_fixed()
+            lib.oracle_add_rground(o.h, tip, 0, rad, mass[tip] / (scene.dt * scene.dt), 0.5, 0.2)
+            spins.append((tip, r.rot, w0))
     for sp in getattr(scene, "spheres", []):
         p = o.add_sphere(sp.radius, sp.density, sp.pos, sp.rot, sp.fixed, sp.ground)
         if any(c != 0.0 for c in sp.w0):
-            n = o.n
-            q = np.tile(np.array([1.0, 0, 0, 0]), (n, 1))
-            w = np.zeros((n, 3))
-            cur = o.rotations()
-            q[:] = cur["q"]
-            w[:] = cur["w"]
-            q[p] = sp.rot
-            w[p] = sp.w0
-            o.set_rotation_state(q, w)
+            spins.append((p, sp.rot, sp.w0))
+    if spins:
+        cur = o.rotations()
+        q, w = cur["q"].copy(), cur["w"].copy()
+        for p, _rot, w0 in spins:
+            w[p] = w0
+        o.set_rotation_state(q, w)
     o.build_graph()
     return o
 
@@ -356,6 +364,9 @@ for _k, (_r, _a) in {
     "oracle_get_rotations": (None, [C.c_void_p, _D, _D, _D, _U8]),
     "oracle_get_mass": (None, [C.c_void_p, _D, _U8]),
     "oracle_set_rotation_state": (None, [C.c_void_p, _D, _D]),
+    "oracle_add_mass": (None, [C.c_void_p, C.c_uint32, C.c_double, _D]),
+    "oracle_add_rground": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                  C.c_double]),
     "oracle_get_rground": (None, [C.c_void_p, _D]),
     "oracle_get_joints": (None, [C.c_void_p, _D]),
     "oracle_cosserat_single": (C.c_double, [_D, _D, _D, C.c_double, _D, C.c_uint32, _D, _D]),

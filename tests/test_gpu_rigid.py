@@ -168,3 +168,31 @@ def test_oriented_stats_match_oracle(oracle):
            f"ip gpu {sg.incremental_potential:.6e} oracle {so['ip']:.6e}")
     assert sg.incremental_potential == pytest.approx(so["ip"], rel=2e-2, abs=1e-9)
     assert sg.residual_rel == pytest.approx(so["residual_rel"], rel=0.1, abs=1e-3)
+
+
+def test_rod_grid_with_tip_spheres(oracle):
+    """The rods + rigid bodies workload at reduced size (3 x 3 rods of 16
+    segments): joints at the top, spinning spheres welded at the tips, ground
+    contacts on every node and sphere."""
+    banner("rod grid with tip spheres vs oracle: 1 and 30 steps")
+    sc = scenes.rod_grid(nx=3, nz=3, segments=16)
+    g = gpu_sim(sc)
+    o = oracle.from_scene(sc)
+    col, k = g.ctx.coloring()
+    ocol, ok = o.colors()
+    assert k == ok and np.array_equal(col, ocol)
+    _, wg0 = g.ctx.rotations()
+    assert np.allclose(wg0, o.rotations()["w"], atol=1e-6)
+    g.update(1)
+    o.step(1)
+    xo = o.positions()
+    err = np.abs(g.positions() - xo).max() / bbox(xo)
+    qg, _ = g.ctx.rotations()
+    banner(f"  1-step pos rel err {err:.3e}, q err {qerr(qg, o.rotations()['q']):.3e}")
+    assert err <= 1e-5 and qerr(qg, o.rotations()["q"]) <= 1e-5
+    g.update(29)
+    o.step(29)
+    xo = o.positions()
+    err = np.abs(g.positions() - xo).max() / bbox(xo)
+    banner(f"  30-step pos rel err {err:.3e}")
+    assert err <= 2e-3
