@@ -209,6 +209,7 @@ SIGNATURES = {
     "avbd_get_solve_timing": (C.c_int, [_P, _D, _U32]),
     "avbd_get_broadphase_timing": (C.c_int, [_P, _D]),
     "avbd_calibrate_copy": (C.c_int, [_P, C.c_uint64, C.c_uint32, _D]),
+    "avbd_time_solve_launches": (C.c_int, [_P, C.c_uint32, _D, _U32]),
     "avbd_add_collision_triangles": (C.c_int, [_P, C.c_uint32, _U32]),
     "avbd_detect_collisions": (C.c_int, [_P]),
     "avbd_download_bvh": (C.c_int, [_P, C.POINTER(C.c_uint64), _U32, _U32, _U32, _U32, _D, _U32]),

@@ -359,7 +359,7 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
     if (pre) {
         if (c->damping) launch_eprev(s, c->d, dt);
         launch_step_begin(s, c->d, dt, g);
-        launch_warm_save(s, c->d);
+        launch_warm_save(s, c->d, io);
         io.raw = c->recording ? c->d_raw : nullptr;
         io.tgt = c->d_tgt;
         io.norm = c->mlp->d_norm;
@@ -1108,7 +1108,8 @@ int avbd_build_graph(avbd_ctx* c) {
             c->wio.n_scenes = (uint32_t)(so.size() - 1);
             float2* d_mat = dev_alloc<float2>(c, n);
             uint32_t* d_scene = dev_alloc<uint32_t>(c, n);
-            c->wio.ipd = dev_alloc<double>(c, c->wio.n_scenes);
+            c->wio.ipd = dev_alloc<double>(c, (size_t)c->wio.n_scenes * kWarmAlphas);
+            c->wio.pred = dev_alloc<float4>(c, n);
             c->wio.counters = dev_alloc<unsigned long long>(c, 2);
             upload(c, d_mat, mat);
             upload(c, d_scene, scene_of);
@@ -1620,6 +1621,34 @@ int avbd_get_solve_timing(avbd_ctx* c, double* avg_ms, uint32_t* launches) {
     return guarded(c, [&] {
         if (avg_ms) *avg_ms = c->solve_avg_ms;
         if (launches) *launches = c->solve_launches;
+    });
+}
+
+int avbd_time_solve_launches(avbd_ctx* c, uint32_t reps, double* avg_ms, uint32_t* launches) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (reps == 0) throw std::runtime_error("reps must be >= 1");
+        const float dt = (float)c->desc.dt;
+        uint32_t n = 0;
+        auto round = [&] {
+            for (uint32_t col = 0; col < c->plan.num_colors; col++) {
+                const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
+                if (e > b) {
+                    launch_solve_color(c->stream, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x_scratch);
+                    n++;
+                }
+            }
+        };
+        round();   // warm-up
+        n = 0;
+        hip_check(hipEventRecord(c->ev_a, c->stream), "hipEventRecord");
+        for (uint32_t r = 0; r < reps; r++) round();
+        hip_check(hipEventRecord(c->ev_b, c->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(c->ev_b), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, c->ev_a, c->ev_b), "hipEventElapsedTime");
+        if (avg_ms) *avg_ms = n ? ms / n : 0.0;
+        if (launches) *launches = n;
     });
 }
 

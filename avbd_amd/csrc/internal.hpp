@@ -228,7 +228,7 @@ void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cf
                            ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2);
 void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters);
 // learned warm start
-void launch_warm_save(hipStream_t s, const DevState& d);
+void launch_warm_save(hipStream_t s, const DevState& d, const WarmIO& io);
 void launch_warm_features(hipStream_t s, const DevState& d, const WarmIO& io, uint32_t lanes, float dt, float g,
                           uint32_t rows_pad);
 void launch_warm_apply(hipStream_t s, const DevState& d, const WarmIO& io, float dt2g);

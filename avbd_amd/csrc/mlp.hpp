@@ -17,6 +17,7 @@ constexpr int kMlpHidden = 128;
 constexpr int kMlpOut = 3;
 constexpr int kMlpOutPad = 16;
 constexpr int kMlpLayers = 4;
+constexpr int kWarmAlphas = 3;   // step lengths tried along the prediction: 1, 1/2, 1/4
 
 // GEMM epilogues
 constexpr int MLP_EPI_F32 = 0;        // C32 = acc
@@ -37,7 +38,8 @@ struct WarmIO {
     float* tgt = nullptr;                // n_items x 3 training targets (recording)
     const float2* mat = nullptr;         // per particle: log10 E, nu of the incident material
     const uint32_t* scene_of = nullptr;  // per particle
-    double* ipd = nullptr;               // per scene: IP(prediction) - IP(initial guess)
+    float4* pred = nullptr;              // per particle: predicted step from the initial guess
+    double* ipd = nullptr;               // per step length and scene: IP(x0 + a pred) - IP(x0)
     unsigned long long* counters = nullptr;   // [0] scene-steps predicted, [1] fallbacks to the inertial start
     uint32_t n_scenes = 0;
 };

@@ -290,7 +290,8 @@ __global__ void eprev_kernel(const TetRec* __restrict__ tets, TetDamp* __restric
 // damping term when DAMP, in FP64 accumulation.
 template <bool DAMP>
 __device__ __forceinline__ double nh_energy_dev(const float4* P, const TetRec* __restrict__ tets,
-                                                const TetDamp* __restrict__ damp, uint32_t t, float inv_dt) {
+                                                const TetDamp* __restrict__ damp, uint32_t t, float inv_dt,
+                                                uint32_t damp_index) {
     uint4 idx;
     float4 r1, r2, r3;
     load_tet(tets, t, idx, r1, r2, r3);
@@ -304,7 +305,7 @@ __device__ __forceinline__ double nh_energy_dev(const float4* P, const TetRec* _
     const double trc = (double)dot3(f0, f0) + dot3(f1, f1) + dot3(f2, f2);
     double e = 0.5 * VL * (J - al) * (J - al) + 0.5 * VM * (trc - 3.0);
     if constexpr (DAMP) {
-        const TetDamp dr = damp[t];
+        const TetDamp dr = damp[damp_index];
         const double E[6] = {0.5 * (dot3(f0, f0) - 1.0), 0.5 * (dot3(f1, f1) - 1.0), 0.5 * (dot3(f2, f2) - 1.0),
                              0.5 * dot3(f0, f1), 0.5 * dot3(f0, f2), 0.5 * dot3(f1, f2)};
         const double Pv[6] = {dr.e00, dr.e11, dr.e22, dr.e01, dr.e02, dr.e12};
@@ -667,25 +668,23 @@ __global__ __launch_bounds__(256) void warm_features_kernel(DevState d, WarmIO i
     for (int k = kMlpIn; k < kMlpInPad; k++) row[k] = 0;
 }
 
-// Keep the initial guess of every particle (x -> x_scratch) before the
-// prediction overwrites the free positional ones.
-__global__ void warm_save_kernel(DevState d) {
+// Keep the initial guess of every particle (x -> x_scratch) and clear the
+// predicted step (pred = 0 for particles the network does not move).
+__global__ void warm_save_kernel(DevState d, WarmIO io) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < d.n) d.x_scratch[p] = d.x[p];
+    if (p >= d.n) return;
+    d.x_scratch[p] = d.x[p];
+    io.pred[p] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// x = x0 + dt^2 g * (network output, de-normalised) for every positional work item.
+// pred = dt^2 g * (network output, de-normalised) for every positional work item.
 __global__ void warm_apply_kernel(DevState d, WarmIO io, float s) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.n_items) return;
     const uint32_t v = d.items[i].v;
     const float* o = io.out + (size_t)i * kMlpOutPad;
     const float* tn = io.norm + 2 * kMlpIn;   // target mean (3), std (3)
-    float4 x = d.x[v];
-    x.x += s * (o[0] * tn[3] + tn[0]);
-    x.y += s * (o[1] * tn[4] + tn[1]);
-    x.z += s * (o[2] * tn[5] + tn[2]);
-    d.x[v] = x;
+    io.pred[v] = make_float4(s * (o[0] * tn[3] + tn[0]), s * (o[1] * tn[4] + tn[1]), s * (o[2] * tn[5] + tn[2]), 0.f);
 }
 
 // Add v to acc[scene]: one atomic per wavefront when every active lane has
@@ -714,55 +713,105 @@ __device__ __forceinline__ double ground_al_energy(const GroundRec& c, float4 xp
     return e + 0.5 * c.k_t * Ct * Ct + (double)c.lambda_t * Ct + 0.5 * c.k_b * Cb * Cb + (double)c.lambda_b * Cb;
 }
 
-// Per scene: incremental potential at the prediction (x) minus at the
-// initial guess (x_scratch) -- Neo-Hookean (+ damping), ground AL and
-// inertia terms (the terms that differ between the two).
+// Step lengths tried along the prediction: x0 + a * pred.
+__device__ constexpr float kWarmSteps[kWarmAlphas] = {1.0f, 0.5f, 0.25f};
+
+__device__ __forceinline__ float4 along(const float4* x0, const float4* pred, uint32_t p, float a) {
+    const float4 b = x0[p], d = pred[p];
+    return make_float4(b.x + a * d.x, b.y + a * d.y, b.z + a * d.z, b.w);
+}
+
+// Stable Neo-Hookean energy (+ damping) of tet t at x0 + a * pred.
+template <bool DAMP>
+__device__ __forceinline__ double nh_energy_along(const float4* x0, const float4* pred, float a,
+                                                  const TetRec* __restrict__ tets, const TetDamp* __restrict__ damp,
+                                                  uint32_t t, float inv_dt) {
+    const uint4 idx = reinterpret_cast<const uint4*>(tets + t)[0];
+    float4 P[4];
+    P[0] = along(x0, pred, idx.x, a);
+    P[1] = along(x0, pred, idx.y, a);
+    P[2] = along(x0, pred, idx.z, a);
+    P[3] = along(x0, pred, idx.w, a);
+    // local positions table: energy of the tet whose vertices are 0..3 of P
+    TetRec r = tets[t];
+    r.idx[0] = 0;
+    r.idx[1] = 1;
+    r.idx[2] = 2;
+    r.idx[3] = 3;
+    return nh_energy_dev<DAMP>(P, &r, damp, 0, inv_dt, t);
+}
+
+// Per scene and step length a: incremental potential at x0 + a * pred minus
+// at x0 -- Neo-Hookean (+ damping), ground AL and inertia terms (the terms
+// that change along the prediction).
 template <bool DAMP>
 __global__ __launch_bounds__(256) void warm_ip_kernel(DevState d, WarmIO io, float dt) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const float inv_dt = 1.f / dt;
-    double e = 0;
+    double e[kWarmAlphas] = {0.0, 0.0, 0.0};
     uint32_t scene = 0;
     bool active = false;
     if (k < d.n_tets) {
-        e = nh_energy_dev<DAMP>(d.x, d.tets, d.damp, k, inv_dt) - nh_energy_dev<DAMP>(d.x_scratch, d.tets, d.damp, k, inv_dt);
+        const double e0 = nh_energy_along<DAMP>(d.x_scratch, io.pred, 0.f, d.tets, d.damp, k, inv_dt);
+        for (int j = 0; j < kWarmAlphas; j++)
+            e[j] = nh_energy_along<DAMP>(d.x_scratch, io.pred, kWarmSteps[j], d.tets, d.damp, k, inv_dt) - e0;
         scene = io.scene_of[d.tets[k].idx[0]];
         active = true;
     } else if (k < d.n_tets + d.n_ground) {
         const GroundRec c = d.ground[k - d.n_tets];
-        e = ground_al_energy(c, d.x[c.p]) - ground_al_energy(c, d.x_scratch[c.p]);
+        const double e0 = ground_al_energy(c, d.x_scratch[c.p]);
+        for (int j = 0; j < kWarmAlphas; j++)
+            e[j] = ground_al_energy(c, along(d.x_scratch, io.pred, c.p, kWarmSteps[j])) - e0;
         scene = io.scene_of[c.p];
         active = true;
     } else if (k < d.n_tets + d.n_ground + d.n_items) {
         const uint32_t v = d.items[k - d.n_tets - d.n_ground].v;
-        const float4 x = d.x[v], x0 = d.x_scratch[v], y = d.y[v];
-        const double md = 0.5 * (double)x.w * inv_dt * inv_dt;
-        const double a = ((double)x.x - y.x) * (x.x - y.x) + ((double)x.y - y.y) * (x.y - y.y) + ((double)x.z - y.z) * (x.z - y.z);
-        const double b = ((double)x0.x - y.x) * (x0.x - y.x) + ((double)x0.y - y.y) * (x0.y - y.y) + ((double)x0.z - y.z) * (x0.z - y.z);
-        e = md * (a - b);
+        const float4 x0 = d.x_scratch[v], y = d.y[v];
+        const double md = 0.5 * (double)x0.w * inv_dt * inv_dt;
+        const double b = ((double)x0.x - y.x) * (x0.x - y.x) + ((double)x0.y - y.y) * (x0.y - y.y) +
+                         ((double)x0.z - y.z) * (x0.z - y.z);
+        for (int j = 0; j < kWarmAlphas; j++) {
+            const float4 x = along(d.x_scratch, io.pred, v, kWarmSteps[j]);
+            const double a = ((double)x.x - y.x) * (x.x - y.x) + ((double)x.y - y.y) * (x.y - y.y) +
+                             ((double)x.z - y.z) * (x.z - y.z);
+            e[j] = md * (a - b);
+        }
         scene = io.scene_of[v];
         active = true;
     }
-    add_per_scene(io.ipd, scene, e, active);
+    for (int j = 0; j < kWarmAlphas; j++) add_per_scene(io.ipd + j * io.n_scenes, scene, e[j], active);
 }
 
 __global__ void warm_ip_reset_kernel(WarmIO io) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < io.n_scenes) io.ipd[s] = 0.0;
+    if (s < io.n_scenes * kWarmAlphas) io.ipd[s] = 0.0;
 }
 
-// Keep the prediction of a scene only if it lowers the incremental potential;
-// restart the Chebyshev history at the chosen guess.
+// Per scene: the step length along the prediction with the lowest
+// incremental potential, if it is below the initial guess's, else the initial
+// guess; restart the Chebyshev history at the chosen start.
+__device__ __forceinline__ float warm_choice(const WarmIO& io, uint32_t s) {
+    double best = 0.0;
+    float a = 0.f;
+    for (int j = 0; j < kWarmAlphas; j++) {
+        const double v = io.ipd[j * io.n_scenes + s];
+        if (v < best) {
+            best = v;
+            a = kWarmSteps[j];
+        }
+    }
+    return a;
+}
+
 __global__ void warm_choose_kernel(DevState d, WarmIO io) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < io.n_scenes) {
         atomicAdd(&io.counters[0], 1ull);
-        if (io.ipd[i] > 0.0) atomicAdd(&io.counters[1], 1ull);
+        if (warm_choice(io, i) == 0.f) atomicAdd(&io.counters[1], 1ull);
     }
     if (i >= d.n_items) return;
     const uint32_t v = d.items[i].v;
-    float4 x = d.x[v];
-    if (io.ipd[io.scene_of[v]] > 0.0) x = d.x_scratch[v];
+    const float4 x = along(d.x_scratch, io.pred, v, warm_choice(io, io.scene_of[v]));
     d.x[v] = x;
     d.x_last[v] = x;
     d.x_lastlast[v] = x;
@@ -971,7 +1020,7 @@ __global__ void stats_energy_kernel(const TetRec* __restrict__ tets, const TetDa
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0;
     if (k < n_tets) {
-        e = nh_energy_dev<DAMP>(x, tets, damp, k, inv_dt);
+        e = nh_energy_dev<DAMP>(x, tets, damp, k, inv_dt, k);
     } else if (k < n_tets + n_ground) {
         const GroundRec c = ground[k - n_tets];
         e = ground_al_energy(c, x[c.p]);
@@ -1133,8 +1182,8 @@ void launch_velocity(hipStream_t s, const DevState& d, float dt) {
     hipLaunchKernelGGL(velocity_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, 1.0f / dt);
 }
 
-void launch_warm_save(hipStream_t s, const DevState& d) {
-    if (d.n) hipLaunchKernelGGL(warm_save_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d);
+void launch_warm_save(hipStream_t s, const DevState& d, const WarmIO& io) {
+    if (d.n) hipLaunchKernelGGL(warm_save_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, io);
 }
 
 void launch_warm_features(hipStream_t s, const DevState& d, const WarmIO& io, uint32_t lanes, float dt, float g,
@@ -1160,7 +1209,7 @@ void launch_warm_apply(hipStream_t s, const DevState& d, const WarmIO& io, float
 }
 
 void launch_warm_ip(hipStream_t s, const DevState& d, const WarmIO& io, bool damping, float dt) {
-    hipLaunchKernelGGL(warm_ip_reset_kernel, dim3(blocks_for(io.n_scenes, 256)), dim3(256), 0, s, io);
+    hipLaunchKernelGGL(warm_ip_reset_kernel, dim3(blocks_for(io.n_scenes * kWarmAlphas, 256)), dim3(256), 0, s, io);
     const uint32_t n = d.n_tets + d.n_ground + d.n_items;
     if (n == 0) return;
     if (damping)

@@ -376,6 +376,12 @@ class SimulationContext:
         self._chk(lib.avbd_calibrate_copy(self.handle, nbytes, iters, C.byref(ms)))
         return ms.value
 
+    def time_solve_launches(self, reps: int = 20):
+        """Average duration (ms) of one vertex-solve launch, back to back on the ctx stream."""
+        ms, n = C.c_double(), C.c_uint32()
+        self._chk(lib.avbd_time_solve_launches(self.handle, reps, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def broadphase_timing(self) -> float:
         ms = C.c_double()
         self._chk(lib.avbd_get_broadphase_timing(self.handle, C.byref(ms)))

@@ -93,18 +93,20 @@ def load(path: Path = DEFAULT_PATH) -> np.ndarray:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(DEFAULT_PATH))
-    ap.add_argument("--scenes", type=int, default=12)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--scenes", type=int, default=24)
+    ap.add_argument("--steps", type=int, default=80)
+    ap.add_argument("--keep", type=float, default=0.3)
     ap.add_argument("--tol", type=float, default=1e-2)
     ap.add_argument("--max-iters", type=int, default=300)
-    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=15)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     t0 = time.perf_counter()
-    X, T = generate(training_scenes(args.scenes, args.seed), args.steps, args.tol, args.max_iters, log=log)
+    X, T = generate(training_scenes(args.scenes, args.seed), args.steps, args.tol, args.max_iters, keep=args.keep,
+                    log=log)
     log(f"[warm-start] {X.shape[0]} rows in {time.perf_counter() - t0:.1f}s")
     from .sim import SimulationContext
     ctx = SimulationContext(dt=1 / 60, solver_iters=1)

@@ -357,6 +357,12 @@ int avbd_set_warm_start(avbd_ctx* ctx, int enable);
 int avbd_mlp_gemm_test(avbd_ctx* ctx, uint32_t M, uint32_t N, uint32_t K, int ta, int tb, const float* A,
                        const float* B, float* C);
 
+/* Measurement helper with no counterpart in the reference: `reps` rounds of
+ * the per-colour vertex-solve launches back to back on the ctx stream (into a
+ * scratch output, state untouched), bracketed by HIP events; *avg_ms = average
+ * duration per launch. */
+int avbd_time_solve_launches(avbd_ctx* ctx, uint32_t reps, double* avg_ms, uint32_t* launches);
+
 /* Measurement helper with no counterpart in the reference: runs a plain
  * device-to-device streaming copy kernel (calib_copy_kernel) of `bytes` bytes
  * `iters` times on the ctx stream and returns the average duration, so that
