@@ -212,6 +212,16 @@ __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const
                                            const GroundRec* __restrict__ ground, float4* x_out, float4* x_buf,
                                            float inv_dt, float inv_dt2) {
     const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
+    // the group leader's own state is loaded first, so that its latency
+    // overlaps the incidence gathers instead of following the reduction
+    float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), yv = xv;
+    GroundRec gr{};
+    const bool lead = lane == 0;
+    if (lead) {
+        xv = x[it.v];
+        yv = y[it.v];
+        if (it.ground >= 0) gr = ground[it.ground];
+    }
     float3 g = make_float3(0.f, 0.f, 0.f);
     Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (uint32_t j = lane; j < cnt; j += L) nh_incidence_k<DAMP>(x, irec, inc, damp, it.inc_begin + j, inv_dt, g, h);
@@ -224,11 +234,9 @@ __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const
     h.a01 = group_sum<L>(h.a01);
     h.a02 = group_sum<L>(h.a02);
     h.a12 = group_sum<L>(h.a12);
-    if (lane != 0) return;
-    const float4 xv = x[it.v];
-    const float4 yv = y[it.v];
+    if (!lead) return;
     const float3 p = f3(xv);
-    if (it.ground >= 0) ground_accumulate(ground[it.ground], p, g, h);
+    if (it.ground >= 0) ground_accumulate(gr, p, g, h);
     const float md = xv.w * inv_dt2;   // m / dt^2
     h.a00 += md;
     h.a11 += md;

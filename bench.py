@@ -135,8 +135,8 @@ def main():
     el = time.perf_counter() - t0
     steps_per_s = args.steps / el
     st = sim.ctx.stats()
-    n_items = sum(int((np.ones(m.rest.shape[0]) if m.fixed is None else (m.fixed == 0)).sum()) for m in sc.meshes)
-    vups = steps_per_s * sc.iters * n_items
+    # vertex-block solves per second (positional and oriented work items)
+    vups = st.vertex_updates / max(st.steps_taken, 1) * steps_per_s
     print(f"[bench] {args.steps} steps in {el * 1e3:.1f} ms -> {steps_per_s:.1f} steps/s, "
           f"residual_rel {st.residual_rel:.3e}", file=sys.stderr, flush=True)
 
@@ -157,7 +157,7 @@ def main():
     else:
         kernel = "solve_color_kernel"            # one launch = one colour phase
         bytes_launch = solve_bytes_per_launch(sim)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     roof = {
         "kernel": kernel,
         "bound": "hbm",
@@ -170,8 +170,12 @@ def main():
         "algorithmic_bytes_per_launch": int(bytes_launch),
         "launches_timed": launches,
     }
+    if launches == 0:
+        # no positional vertex blocks (an all-oriented scene such as the rod
+        # grid): the colour-phase kernel never runs, so there is no roofline
+        roof = None
     prof = ROOT / "profiles" / f"{kernel}_{sc.name}_traffic.json"
-    if prof.exists():
+    if roof is not None and prof.exists():
         try:
             roof["traffic"] = json.loads(prof.read_text()).get("bytes_per_launch")
         except Exception:
