@@ -167,6 +167,7 @@ class Sphere:
     w0: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     fixed: bool = False
     ground: bool = True
+    v0: Tuple[float, float, float] = (0.0, 0.0, 0.0)   # initial linear velocity (v and v_prev)
 
 
 @dataclass
@@ -181,6 +182,10 @@ class Scene:
     iters: int = 20
     rho: float = 0.95
     steps: int = 100
+    # collision margin (SimulationParams::collision_margin); when set, every
+    # step runs the triangle narrow phase against rods and spheres and keeps
+    # the resulting dynamic contacts (CollisionDetector::detectCollisionsAndRecolor)
+    contact_margin: Optional[float] = None
 
     @property
     def n_vertices(self) -> int:
@@ -300,6 +305,44 @@ def rod_grid(nx: int = 16, nz: int = 16, segments: int = 128, seed: int = 3, dt:
                             pos=(0.05 * i, 1.05, 0.05 * k), rot=down, ground=True, base_joint=True,
                             tip_sphere=(0.02, 0.05, w0)))
     return Scene("rod_grid", rods=rods, dt=dt, iters=iters, rho=0.95, steps=steps)
+
+
+def soft_block(n: int, edge: float, E: float = 2e4, nu: float = 0.45, density: float = 1000.0) -> TetMesh:
+    """Kuhn block [0, edge]^3 with its bottom layer (y = 0) held fixed and its
+    boundary triangles as collision primitives."""
+    verts, tets = kuhn_cube(n, edge)
+    fixed = (np.abs(verts[:, 1]) < 1e-12).astype(np.uint8)
+    return TetMesh(rest=verts, tets=tets, E=E, nu=nu, density=density, kd=0.0, ground=False, fixed=fixed,
+                   triangles=surface_triangles(n))
+
+
+def ball_on_block(n: int = 6, edge: float = 0.2, radius: float = 0.04, drop: float = 0.004, speed: float = 0.5,
+                  density: float = 2000.0, dt: float = 1.0 / 120.0, iters: int = 20, steps: int = 60,
+                  margin: float = 1e-3) -> Scene:
+    """A rigid sphere falling onto a soft block: triangle-rigid contacts found
+    by the continuous triangle-SDF test.  The sphere starts `drop` above the
+    top face, off the face centre, moving down at `speed` and spinning."""
+    block = soft_block(n, edge)
+    c = 0.5 * edge
+    sphere = Sphere(radius=radius, density=density, pos=(c + 0.013, edge + radius + drop, c - 0.007),
+                    w0=(0.0, 2.0, 0.0), ground=False, v0=(0.0, -speed, 0.0))
+    return Scene("ball_on_block", meshes=[block], spheres=[sphere], dt=dt, iters=iters, rho=0.95, steps=steps,
+                 contact_margin=margin)
+
+
+def rod_on_block(n: int = 6, edge: float = 0.2, elements: int = 12, length: float = 0.16, diameter: float = 0.01,
+                 drop: float = 0.003, dt: float = 1.0 / 120.0, iters: int = 20, steps: int = 60,
+                 margin: float = 1e-3) -> Scene:
+    """A Cosserat rod (E = 1e6 Pa) dropped across a soft block, as in the
+    reference's tissue scene (a rod falling onto a tet mesh): triangle-rod
+    contacts from the segment-triangle closest points."""
+    block = soft_block(n, edge)
+    c = 0.5 * edge
+    rod = Rod(elements=elements, length=length, E=1e6, nu=0.4, density=1000.0, diameter=diameter,
+              pos=(c - 0.5 * length, edge + 0.5 * diameter + drop, c + 0.011),
+              rot=quat_about((0.05, 1.0, 0.0), np.pi / 2))
+    return Scene("rod_on_block", meshes=[block], rods=[rod], dt=dt, iters=iters, rho=0.95, steps=steps,
+                 contact_margin=margin)
 
 
 def algorithmic_bytes_per_step(n_v: int, n_tet: int, n_g: int, iters: int) -> int:

@@ -450,10 +450,14 @@ class Simulation:
                 raise ValueError("batched tet scenes cannot be mixed with rods or spheres")
             self.ctx.set_scenes(offs)
         self.ctx.build_graph()
+        if self.scene.contact_margin is not None:
+            self.ctx.set_dynamic_contacts(self.scene.contact_margin)
         self._setup = True
 
     def _add_rod(self, r: Rod) -> int:
         first = self.ctx.add_rod(r)
+        if self.scene.contact_margin is not None:
+            self.ctx.add_collision_segments(np.arange(first, first + r.elements + 1), 0.5 * r.diameter)
         if r.base_joint:
             # hold the first node at its initial pose
             self.ctx.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
@@ -477,9 +481,14 @@ class Simulation:
 
     def _add_sphere(self, sp: Sphere) -> int:
         p = self.ctx.add_sphere(sp)
+        if self.scene.contact_margin is not None:
+            self.ctx.add_collision_sphere(p, sp.radius)
         if any(c != 0.0 for c in sp.w0):
             self.ctx.set_rotation_state(p, q=np.asarray([sp.rot], dtype=np.float64),
                                         w=np.asarray([sp.w0], dtype=np.float64))
+        if any(c != 0.0 for c in sp.v0):
+            v0 = np.asarray([sp.v0], dtype=np.float64)
+            self.ctx.set_particle_state(p, v=v0, v_prev=v0)
         return p
 
     def _add_object(self, m: TetMesh) -> int:

@@ -10,6 +10,10 @@ using namespace oracle;
 
 namespace {
 thread_local std::string g_err;
+void m3_out(const M3& m, double* o) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) o[3 * i + j] = m.m[i][j];
+}
 template <class F>
 int guard(F&& f) {
     try {
@@ -62,9 +66,90 @@ uint32_t oracle_add_particles(void* h, uint32_t n, const double* x, const double
     return static_cast<Simulation*>(h)->add_particles(n, x, m);
 }
 
+// Replace the collision triangles by one mesh object of n triangles.
 void oracle_set_collision_triangles(void* h, uint32_t n, const uint32_t* tris) {
     auto* s = static_cast<Simulation*>(h);
-    s->tris.assign(tris, tris + 3 * (size_t)n);
+    s->tris.clear();
+    s->prims.clear();
+    s->next_object = 0;
+    s->add_collision_triangles(n, tris);
+}
+// CollisionPrimitivePool::addObject, one object per call; primitives are
+// numbered in call order.
+uint32_t oracle_add_collision_triangles(void* h, uint32_t n, const uint32_t* tris) {
+    return static_cast<Simulation*>(h)->add_collision_triangles(n, tris);
+}
+uint32_t oracle_add_collision_segments(void* h, uint32_t n_nodes, const uint32_t* nodes, double radius) {
+    return static_cast<Simulation*>(h)->add_collision_segments(n_nodes, nodes, radius);
+}
+uint32_t oracle_add_collision_sphere(void* h, uint32_t p, double radius) {
+    return static_cast<Simulation*>(h)->add_collision_sphere(p, radius);
+}
+uint32_t oracle_num_prims(void* h) { return (uint32_t)static_cast<Simulation*>(h)->prims.size(); }
+void oracle_enable_dynamic_contacts(void* h, double margin, uint32_t capacity) {
+    static_cast<Simulation*>(h)->enable_dynamic_contacts(margin, capacity);
+}
+// Run the per-step detection + lifecycle + recolouring alone (tests).
+int oracle_detect_and_recolor(void* h) {
+    return guard([&] { static_cast<Simulation*>(h)->detect_and_recolor(); });
+}
+uint32_t oracle_num_contacts(void* h, uint32_t type) {
+    auto* s = static_cast<Simulation*>(h);
+    return type < DYN_COUNT ? s->dyn[type].sp.count : 0;
+}
+// Active contacts of one dynamic pool in slot order, 32 doubles each:
+// slot, idx[5], n, t, b, barys, s, cpl, k_start, k_n, k_t, k_b,
+// lambda_n, lambda_t, lambda_b, C_n_prev, use_static, unclamped.
+uint32_t oracle_get_contacts(void* h, uint32_t type, double* rec32, uint32_t cap) {
+    auto* s = static_cast<Simulation*>(h);
+    if (type >= DYN_COUNT) return 0;
+    const DynPool& P = s->dyn[type];
+    uint32_t k = 0;
+    for (uint32_t e = 0; e < P.sp.active.size(); e++) {
+        if (!P.sp.active[e]) continue;
+        if (k < cap) {
+            double* r = rec32 + 32 * (size_t)k;
+            const DynContact& c = P.data[e];
+            r[0] = e;
+            for (int i = 0; i < 5; i++) r[1 + i] = c.idx[i];
+            for (int i = 0; i < 3; i++) {
+                r[6 + i] = c.n[i];
+                r[9 + i] = c.t[i];
+                r[12 + i] = c.b[i];
+                r[15 + i] = c.barys[i];
+                r[19 + i] = c.cpl[i];
+            }
+            r[18] = c.s;
+            r[22] = c.k_start; r[23] = c.k_n; r[24] = c.k_t; r[25] = c.k_b;
+            r[26] = c.lambda_n; r[27] = c.lambda_t; r[28] = c.lambda_b; r[29] = c.C_n_prev;
+            r[30] = c.use_static; r[31] = c.unclamped;
+        }
+        k++;
+    }
+    return k;
+}
+// Contacts detected by the last detection, in key order: key, pool slot, type
+// (DetectedCollisionType).
+uint32_t oracle_get_detected(void* h, uint64_t* keys, uint32_t* slots, uint8_t* types, uint32_t cap) {
+    auto* s = static_cast<Simulation*>(h);
+    const uint32_t n = (uint32_t)s->cur_det.size();
+    for (uint32_t i = 0; i < n && i < cap; i++) {
+        const Detected& d = s->cur_det[s->cur_order[i]];
+        keys[i] = d.key;
+        slots[i] = d.e_idx;
+        types[i] = d.type;
+    }
+    return n;
+}
+// Conflicted particles (is_conflicted) after the last recolouring.
+uint32_t oracle_get_conflicted(void* h, uint8_t* flags) {
+    auto* s = static_cast<Simulation*>(h);
+    uint32_t k = 0;
+    for (size_t p = 0; p < s->is_conflicted.size(); p++) {
+        flags[p] = s->is_conflicted[p];
+        k += s->is_conflicted[p];
+    }
+    return k;
 }
 uint64_t oracle_last_candidates(void* h) { return static_cast<Simulation*>(h)->last_candidates; }
 
@@ -451,15 +536,127 @@ void oracle_rground_single(const double* x3, const double* q4, const double* rec
     out6(G, H, G6, H36);
 }
 
+// Geometry and narrow-phase hooks for the tests
+void oracle_closest_segment_triangle(const double* p0, const double* p1, const double* a, const double* b,
+                                     const double* c, double* s, double* barys3, double* dist) {
+    V3 bc;
+    closest_segment_triangle(v_in(p0), v_in(p1), v_in(a), v_in(b), v_in(c), *s, bc, *dist);
+    for (int k = 0; k < 3; k++) barys3[k] = bc[k];
+}
+void oracle_closest_point_triangle(const double* p, const double* a, const double* b, const double* c, double* out3) {
+    const V3 r = closest_point_triangle(v_in(p), v_in(a), v_in(b), v_in(c));
+    for (int k = 0; k < 3; k++) out3[k] = r[k];
+}
+double oracle_golden_section_quadratic(double a, double b, double c) { return golden_section_quadratic(a, b, c); }
+void oracle_unit_orthogonal(const double* n3, double* t3, double* b3) {
+    V3 t, b;
+    complete_basis(v_in(n3), t, b);
+    for (int k = 0; k < 3; k++) {
+        t3[k] = t[k];
+        b3[k] = b[k];
+    }
+}
+uint64_t oracle_contact_key(uint32_t type, uint32_t a, uint32_t b, uint32_t code) { return contact_key(type, a, b, code); }
+int oracle_tri_sphere_ccd(const double* tri9, const double* tv9, const double* pos3, const double* q4, const double* v3,
+                          const double* w3, double radius, double dt, double margin, double* normal3, double* barys3,
+                          double* cpl3) {
+    const V3 T[3] = {v_in(tri9), v_in(tri9 + 3), v_in(tri9 + 6)};
+    const V3 TV[3] = {v_in(tv9), v_in(tv9 + 3), v_in(tv9 + 6)};
+    V3 n, bc, cp;
+    const bool hit = tri_sphere_ccd(T, TV, v_in(pos3), q_in(q4), v_in(v3), v_in(w3), radius, dt, margin, n, bc, cp);
+    for (int k = 0; k < 3; k++) {
+        normal3[k] = n[k];
+        barys3[k] = bc[k];
+        cpl3[k] = cp[k];
+    }
+    return hit ? 1 : 0;
+}
+// One dynamic contact over particles 0..4 (0-2 triangle, 3 body or 3-4 rod
+// nodes; q20: quaternions (w, x, y, z) of particles 0..4, only 3-4 used).
+// rec: n, t, b, barys, s, cpl (19), then k_n, k_t, k_b, lambda_n, lambda_t,
+// lambda_b, C_n_prev, mu_s, mu_k, use_static (10).  Writes C (3), the raw
+// gradients g (3 x 6: n, t, b), the rotational Hessian blocks h (3 x 9) and
+// the accumulated G (dof) / H (dof x dof).
+void oracle_dyn_single(uint32_t type, const double* x15, const double* q20, const double* rec26, uint32_t local,
+                       uint32_t dof, double* C3, double* g18, double* h27, double* G, double* H) {
+    std::vector<V3> x;
+    RotState rs;
+    for (int i = 0; i < 5; i++) {
+        x.push_back(v_in(x15 + 3 * i));
+        rs.rot_of.push_back(i >= 3 ? i - 3 : -1);
+    }
+    rs.q = {q_in(q20 + 12), q_in(q20 + 16)};
+    DynContact c;
+    for (int i = 0; i < 5; i++) c.idx[i] = i;
+    c.n = v_in(rec26);
+    c.t = v_in(rec26 + 3);
+    c.b = v_in(rec26 + 6);
+    c.barys = v_in(rec26 + 9);
+    c.s = rec26[12];
+    c.cpl = v_in(rec26 + 13);
+    c.k_n = rec26[16]; c.k_t = rec26[17]; c.k_b = rec26[18];
+    c.lambda_n = rec26[19]; c.lambda_t = rec26[20]; c.lambda_b = rec26[21]; c.C_n_prev = rec26[22];
+    c.mu_s = rec26[23]; c.mu_k = rec26[24]; c.use_static = rec26[25] != 0;
+    M3 hn, ht, hb;
+    dyn_constraint((uint8_t)type, c, x, rs, local, (int)dof, C3[0], C3[1], C3[2], g18, g18 + 6, g18 + 12, hn, ht, hb);
+    m3_out(hn, h27);
+    m3_out(ht, h27 + 9);
+    m3_out(hb, h27 + 18);
+    if (dof == 3) {
+        V3 g3;
+        M3 H3;
+        dyn_accumulate3((uint8_t)type, c, x, rs, local, g3, H3);
+        for (int i = 0; i < 3; i++) {
+            G[i] = g3[i];
+            for (int j = 0; j < 3; j++) H[3 * i + j] = H3.m[i][j];
+        }
+    } else {
+        V6 g6;
+        M6 H6;
+        dyn_accumulate6((uint8_t)type, c, x, rs, local, g6, H6);
+        out6(g6, H6, G, H);
+    }
+}
+
+// TombstonePool slot stack: op >= 0 frees that slot, op < 0 allocates (the
+// slot goes to out[i]; out[i] = 0xFFFFFFFF for frees).  Returns 0 or -1.
+int oracle_slot_sequence(uint32_t n_ops, const int32_t* ops, uint32_t capacity, uint32_t* out) {
+    return guard([&] {
+        SlotPool sp;
+        sp.capacity = capacity;
+        for (uint32_t i = 0; i < n_ops; i++) {
+            if (ops[i] < 0) out[i] = sp.alloc();
+            else {
+                sp.free((uint32_t)ops[i]);
+                out[i] = 0xFFFFFFFFu;
+            }
+        }
+    });
+}
+// updateCollisionFrame on (n, t, b, lambda_t, lambda_b) in frame12 =
+// n, t, b, lambda_t, lambda_b, unused; writes the updated values back.
+void oracle_update_collision_frame(double* frame12, const double* new_n3) {
+    DynContact c;
+    c.n = v_in(frame12);
+    c.t = v_in(frame12 + 3);
+    c.b = v_in(frame12 + 6);
+    c.lambda_t = frame12[9];
+    c.lambda_b = frame12[10];
+    update_collision_frame(c, v_in(new_n3));
+    for (int k = 0; k < 3; k++) {
+        frame12[k] = c.n[k];
+        frame12[3 + k] = c.t[k];
+        frame12[6 + k] = c.b[k];
+    }
+    frame12[9] = c.lambda_t;
+    frame12[10] = c.lambda_b;
+}
+
 // S3 / SO3 maps
 void oracle_exp_s3(const double* v3, double* q4) { q_out(exp_s3(v_in(v3)), q4); }
 void oracle_log_s3(const double* q4, double* v3) {
     const V3 v = log_s3(q_in(q4));
     v3[0] = v.x; v3[1] = v.y; v3[2] = v.z;
-}
-static void m3_out(const M3& m, double* o) {
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) o[3 * i + j] = m.m[i][j];
 }
 void oracle_qmat(const double* q4, double* M9) { m3_out(qmat(q_in(q4)), M9); }
 void oracle_exp_so3(const double* v3, double* M9) { m3_out(exp_so3(v_in(v3)), M9); }

@@ -56,6 +56,9 @@ bool overlaps(const double* a, const double* b) {
 
 extern "C" {
 
+void oracle_lbvh_build_boxes(uint32_t n, const double* aabb, uint64_t* morton, uint32_t* sorted, uint32_t* left,
+                             uint32_t* right, uint32_t* parent, double* node_box, uint32_t* root);
+
 // Bit-interleave of one 21-bit value (for unit tests of the closed form).
 uint64_t oracle_expand_bits21(uint64_t v) { return expand_bits21(v); }
 
@@ -68,12 +71,7 @@ uint64_t oracle_expand_bits21(uint64_t v) { return expand_bits21(v); }
 void oracle_lbvh_build(uint32_t n, const uint32_t* tris, const double* x, const double* v, double dt,
                        double* aabb, uint64_t* morton, uint32_t* sorted, uint32_t* left, uint32_t* right,
                        uint32_t* parent, double* node_box, uint32_t* root) {
-    const uint32_t INV = 0xFFFFFFFFu;
     // speculative bounds: box over p and p + dt v for the 3 vertices
-    double smin[3] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::max(),
-                      std::numeric_limits<double>::max()};
-    double smax[3] = {std::numeric_limits<double>::lowest(), std::numeric_limits<double>::lowest(),
-                      std::numeric_limits<double>::lowest()};
     for (uint32_t t = 0; t < n; t++) {
         double* b = aabb + 6 * t;
         for (int k = 0; k < 3; k++) {
@@ -91,6 +89,21 @@ void oracle_lbvh_build(uint32_t n, const uint32_t* tris, const double* x, const 
                 b[3 + k] = std::max(b[3 + k], e);
             }
         }
+    }
+    oracle_lbvh_build_boxes(n, aabb, morton, sorted, left, right, parent, node_box, root);
+}
+
+// LBVHBuilder::computeAABB_MortonCode .. assembleBVH over given boxes
+// (collision/LBVHBuilder.cpp:98-127,181-286,410-453).
+void oracle_lbvh_build_boxes(uint32_t n, const double* aabb, uint64_t* morton, uint32_t* sorted, uint32_t* left,
+                             uint32_t* right, uint32_t* parent, double* node_box, uint32_t* root) {
+    const uint32_t INV = 0xFFFFFFFFu;
+    double smin[3] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::max(),
+                      std::numeric_limits<double>::max()};
+    double smax[3] = {std::numeric_limits<double>::lowest(), std::numeric_limits<double>::lowest(),
+                      std::numeric_limits<double>::lowest()};
+    for (uint32_t t = 0; t < n; t++) {
+        const double* b = aabb + 6 * t;
         for (int k = 0; k < 3; k++) {
             smin[k] = std::min(smin[k], b[k]);
             smax[k] = std::max(smax[k], b[3 + k]);
@@ -335,6 +348,54 @@ void BroadPhase::build(const std::vector<uint32_t>& tris, const std::vector<V3>&
             st.push_back({k, true});
             st.push_back({right[k], false});
             st.push_back({left[k], false});
+        }
+    }
+}
+
+void BroadPhase::build_boxes(const std::vector<double>& boxes) {
+    n = (uint32_t)(boxes.size() / 6);
+    if (!n) return;
+    const uint32_t nn = 2 * n - 1;
+    aabb = boxes;
+    morton.resize(n);
+    sorted.resize(n);
+    left.resize(nn);
+    right.resize(nn);
+    parent.resize(nn);
+    node_box.resize(6 * nn);
+    hi.assign(nn, 0);
+    oracle_lbvh_build_boxes(n, aabb.data(), morton.data(), sorted.data(), left.data(), right.data(), parent.data(),
+                            node_box.data(), &root);
+    for (uint32_t l = 0; l < n; l++) hi[l] = l;
+    std::vector<std::pair<uint32_t, bool>> st{{root, false}};
+    while (!st.empty() && n > 1) {
+        auto [k, ex] = st.back();
+        st.pop_back();
+        if (k < n) continue;
+        if (ex) hi[k] = hi[right[k]];
+        else {
+            st.push_back({k, true});
+            st.push_back({right[k], false});
+            st.push_back({left[k], false});
+        }
+    }
+}
+
+void BroadPhase::traverse_pairs(std::vector<std::pair<uint32_t, uint32_t>>& out) const {
+    out.clear();
+    std::vector<uint32_t> stack;
+    for (uint32_t i = 0; i < n; i++) {
+        const double* bi = node_box.data() + 6 * i;
+        stack.assign(1, root);
+        while (!stack.empty()) {
+            const uint32_t k = stack.back();
+            stack.pop_back();
+            if (hi[k] <= i || !overlaps(bi, node_box.data() + 6 * k)) continue;
+            if (k < n) out.emplace_back(i, k);
+            else {
+                stack.push_back(right[k]);
+                stack.push_back(left[k]);
+            }
         }
     }
 }

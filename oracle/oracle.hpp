@@ -27,6 +27,7 @@
 #include <array>
 #include <atomic>
 #include <string>
+#include <utility>
 
 namespace oracle {
 
@@ -140,6 +141,10 @@ struct Stats {
 extern "C" void oracle_lbvh_build(uint32_t n, const uint32_t* tris, const double* x, const double* v, double dt,
                                   double* aabb, uint64_t* morton, uint32_t* sorted, uint32_t* left,
                                   uint32_t* right, uint32_t* parent, double* node_box, uint32_t* root);
+// LBVH over n given boxes (aabb[6n] is input here).
+extern "C" void oracle_lbvh_build_boxes(uint32_t n, const double* aabb, uint64_t* morton, uint32_t* sorted,
+                                        uint32_t* left, uint32_t* right, uint32_t* parent, double* node_box,
+                                        uint32_t* root);
 
 // LBVH over the collision triangles, rebuilt every step (lbvh.cpp).
 struct BroadPhase {
@@ -148,7 +153,11 @@ struct BroadPhase {
     std::vector<uint64_t> morton;
     std::vector<uint32_t> sorted, left, right, parent, hi;
     void build(const std::vector<uint32_t>& tris, const std::vector<V3>& x, const std::vector<V3>& v, double dt);
+    // the same over given primitive boxes (6 per primitive: min xyz, max xyz)
+    void build_boxes(const std::vector<double>& boxes);
     uint64_t traverse_leaves(uint32_t lo, uint32_t hi_leaf) const;   // candidate count
+    // every overlapping leaf pair (i < j, sorted-order leaf indices)
+    void traverse_pairs(std::vector<std::pair<uint32_t, uint32_t>>& out) const;
 };
 
 

@@ -250,6 +250,156 @@ def lu_solve(A, b):
     return x
 
 
+# --- dynamic contacts (collision primitives, narrow phase, lifecycle) -------
+for _k, (_r, _a2) in {
+    "oracle_add_collision_triangles": (C.c_uint32, [C.c_void_p, C.c_uint32, _U32]),
+    "oracle_add_collision_segments": (C.c_uint32, [C.c_void_p, C.c_uint32, _U32, C.c_double]),
+    "oracle_add_collision_sphere": (C.c_uint32, [C.c_void_p, C.c_uint32, C.c_double]),
+    "oracle_num_prims": (C.c_uint32, [C.c_void_p]),
+    "oracle_enable_dynamic_contacts": (None, [C.c_void_p, C.c_double, C.c_uint32]),
+    "oracle_detect_and_recolor": (C.c_int, [C.c_void_p]),
+    "oracle_num_contacts": (C.c_uint32, [C.c_void_p, C.c_uint32]),
+    "oracle_get_contacts": (C.c_uint32, [C.c_void_p, C.c_uint32, _D, C.c_uint32]),
+    "oracle_get_detected": (C.c_uint32, [C.c_void_p, C.POINTER(C.c_uint64), _U32, _U8, C.c_uint32]),
+    "oracle_get_conflicted": (C.c_uint32, [C.c_void_p, _U8]),
+    "oracle_closest_segment_triangle": (None, [_D, _D, _D, _D, _D, _D, _D, _D]),
+    "oracle_closest_point_triangle": (None, [_D, _D, _D, _D, _D]),
+    "oracle_golden_section_quadratic": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+    "oracle_unit_orthogonal": (None, [_D, _D, _D]),
+    "oracle_contact_key": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "oracle_tri_sphere_ccd": (C.c_int, [_D, _D, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, _D, _D, _D]),
+    "oracle_dyn_single": (None, [C.c_uint32, _D, _D, _D, C.c_uint32, C.c_uint32, _D, _D, _D, _D, _D]),
+    "oracle_slot_sequence": (C.c_int, [C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, _U32]),
+    "oracle_update_collision_frame": (None, [_D, _D]),
+}.items():
+    _f = getattr(lib, _k)
+    _f.restype = _r
+    _f.argtypes = _a2
+
+CONTACT_FIELDS = dict(slot=0, idx=slice(1, 6), n=slice(6, 9), t=slice(9, 12), b=slice(12, 15), barys=slice(15, 18),
+                      s=18, cpl=slice(19, 22), k_start=22, k_n=23, k_t=24, k_b=25, lambda_n=26, lambda_t=27,
+                      lambda_b=28, C_n_prev=29, use_static=30, unclamped=31)
+
+
+def _add_collision_triangles(self, tris):
+    t = np.ascontiguousarray(tris, dtype=np.uint32)
+    return int(lib.oracle_add_collision_triangles(self.h, t.shape[0], up(t)))
+
+
+def _add_collision_segments(self, nodes, radius):
+    nd = np.ascontiguousarray(nodes, dtype=np.uint32)
+    return int(lib.oracle_add_collision_segments(self.h, nd.shape[0], up(nd), float(radius)))
+
+
+def _add_collision_sphere(self, p, radius):
+    return int(lib.oracle_add_collision_sphere(self.h, int(p), float(radius)))
+
+
+def _enable_dynamic_contacts(self, margin, capacity=1 << 16):
+    lib.oracle_enable_dynamic_contacts(self.h, float(margin), int(capacity))
+
+
+def _detect_and_recolor(self):
+    _chk(lib.oracle_detect_and_recolor(self.h))
+
+
+def _contacts(self, kind):
+    """Active contacts of pool `kind` (0 triangle-rigid, 1 triangle-rod) in
+    slot order, one row of 32 per contact (CONTACT_FIELDS)."""
+    n = lib.oracle_num_contacts(self.h, kind)
+    rec = np.zeros((max(n, 1), 32))
+    got = lib.oracle_get_contacts(self.h, kind, dp(rec), n)
+    return rec[:got]
+
+
+def _detected(self):
+    cap = 1 << 16
+    keys = np.zeros(cap, dtype=np.uint64)
+    slots = np.zeros(cap, dtype=np.uint32)
+    types = np.zeros(cap, dtype=np.uint8)
+    n = lib.oracle_get_detected(self.h, keys.ctypes.data_as(C.POINTER(C.c_uint64)), up(slots),
+                                types.ctypes.data_as(_U8), cap)
+    return keys[:n], slots[:n], types[:n]
+
+
+def _conflicted(self):
+    f = np.zeros(self.n, dtype=np.uint8)
+    lib.oracle_get_conflicted(self.h, f.ctypes.data_as(_U8))
+    return f.astype(bool)
+
+
+OracleSim.add_collision_triangles = _add_collision_triangles
+OracleSim.add_collision_segments = _add_collision_segments
+OracleSim.add_collision_sphere = _add_collision_sphere
+OracleSim.enable_dynamic_contacts = _enable_dynamic_contacts
+OracleSim.detect_and_recolor = _detect_and_recolor
+OracleSim.contacts = _contacts
+OracleSim.detected = _detected
+OracleSim.conflicted = _conflicted
+
+
+def closest_segment_triangle(p0, p1, a, b, c):
+    s = C.c_double()
+    d = C.c_double()
+    bc = np.empty(3)
+    lib.oracle_closest_segment_triangle(dp(_a(p0, 3)), dp(_a(p1, 3)), dp(_a(a, 3)), dp(_a(b, 3)), dp(_a(c, 3)),
+                                        C.byref(s), dp(bc), C.byref(d))
+    return s.value, bc, d.value
+
+
+def closest_point_triangle(p, a, b, c):
+    out = np.empty(3)
+    lib.oracle_closest_point_triangle(dp(_a(p, 3)), dp(_a(a, 3)), dp(_a(b, 3)), dp(_a(c, 3)), dp(out))
+    return out
+
+
+def golden_section_quadratic(a, b, c):
+    return lib.oracle_golden_section_quadratic(a, b, c)
+
+
+def complete_basis(n):
+    t, b = np.empty(3), np.empty(3)
+    lib.oracle_unit_orthogonal(dp(_a(n, 3)), dp(t), dp(b))
+    return t, b
+
+
+def contact_key(kind, a, b, code=0):
+    return int(lib.oracle_contact_key(kind, a, b, code))
+
+
+def tri_sphere_ccd(tri, tri_vel, pos, q, v, w, radius, dt, margin):
+    n, bc, cp = np.empty(3), np.empty(3), np.empty(3)
+    hit = lib.oracle_tri_sphere_ccd(dp(_a(tri, 9)), dp(_a(tri_vel, 9)), dp(_a(pos, 3)), dp(_a(q, 4)), dp(_a(v, 3)),
+                                    dp(_a(w, 3)), radius, dt, margin, dp(n), dp(bc), dp(cp))
+    return bool(hit), n, bc, cp
+
+
+def dyn_single(kind, x5, q5, rec26, local, dof):
+    """One dynamic contact over particles 0..4: C (3), raw gradients (3 x 6),
+    rotational Hessian blocks (3 x 3 x 3), accumulated G and H."""
+    Cv, g, h = np.empty(3), np.empty(18), np.empty(27)
+    G, H = np.empty(dof), np.empty(dof * dof)
+    lib.oracle_dyn_single(kind, dp(_a(x5, 15)), dp(_a(q5, 20)), dp(_a(rec26, 26)), local, dof, dp(Cv), dp(g), dp(h),
+                          dp(G), dp(H))
+    return Cv, g.reshape(3, 6), h.reshape(3, 3, 3), G, H.reshape(dof, dof)
+
+
+def slot_sequence(ops, capacity=1 << 20):
+    """TombstonePool alloc / free sequence: op < 0 allocates, op >= 0 frees that
+    slot.  Returns the allocated slot per op (-1 for frees)."""
+    o = np.ascontiguousarray(ops, dtype=np.int32)
+    out = np.zeros(o.shape[0], dtype=np.uint32)
+    _chk(lib.oracle_slot_sequence(o.shape[0], o.ctypes.data_as(C.POINTER(C.c_int32)), capacity, up(out)))
+    return [(-1 if v == 0xFFFFFFFF else int(v)) for v in out]
+
+
+def update_collision_frame(n, t, b, lam_t, lam_b, new_n):
+    f = np.zeros(12)
+    f[0:3], f[3:6], f[6:9], f[9], f[10] = n, t, b, lam_t, lam_b
+    lib.oracle_update_collision_frame(dp(f), dp(_a(new_n, 3)))
+    return f[0:3], f[3:6], f[6:9], f[9], f[10]
+
+
 def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
     """Oracle instance for one mesh of an avbd_amd.scenes.Scene (one reference
     Simulation), followed by the scene's rods and spheres in order."""
@@ -265,10 +415,13 @@ def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
             o.set_velocities(m.v0, m.v0)
         if collide and m.triangles is not None and len(scene.meshes) == 1:
             o.set_collision_triangles(m.triangles)
+    dynamic = getattr(scene, "contact_margin", None) is not None
     spins = []
     for r in getattr(scene, "rods", []):
         first = o.add_rod(r.elements, r.length, r.curvature, r.E, r.nu, r.density, r.diameter, r.pos, r.rot,
                           r.vel, r.base_fixed, r.tip_fixed, r.ground)
+        if dynamic:
+            o.add_collision_segments(np.arange(first, first + r.elements + 1), 0.5 * r.diameter)
         if r.base_joint:
             o.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
         if getattr(r, "tip_sphere", None) is not None:
@@ -282,8 +435,14 @@ def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
             spins.append((tip, r.rot, w0))
     for sp in getattr(scene, "spheres", []):
         p = o.add_sphere(sp.radius, sp.density, sp.pos, sp.rot, sp.fixed, sp.ground)
+        if dynamic:
+            o.add_collision_sphere(p, sp.radius)
         if any(c != 0.0 for c in sp.w0):
             spins.append((p, sp.rot, sp.w0))
+        if any(c != 0.0 for c in getattr(sp, "v0", (0.0, 0.0, 0.0))):
+            vel = o.velocities()
+            vel[p] = sp.v0
+            o.set_velocities(vel, vel)
     if spins:
         cur = o.rotations()
         q, w = cur["q"].copy(), cur["w"].copy()
@@ -291,6 +450,8 @@ def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
             w[p] = w0
         o.set_rotation_state(q, w)
     o.build_graph()
+    if dynamic:
+        o.enable_dynamic_contacts(scene.contact_margin)
     return o
 
 

@@ -3,6 +3,7 @@
 // Test infrastructure only; see oracle.hpp.
 #pragma once
 
+#include "contact.hpp"
 #include "oracle.hpp"
 #include "rigid.hpp"
 
@@ -28,6 +29,18 @@ public:
     BroadPhase bp;
     uint64_t last_candidates = 0;
 
+    // Collision primitives (CollisionPrimitivePool, one object per add call)
+    // and the dynamic contact pipeline; off until enable_dynamic_contacts().
+    std::vector<Prim> prims;
+    uint32_t next_object = 0;
+    bool dynamic = false;
+    Real collision_margin = 1e-3;
+    DynPool dyn[DYN_COUNT];   // TRIANGLE_RIGID_COLLISION, TRIANGLE_ROD_COLLISION
+    std::vector<Detected> prev_det, cur_det;
+    std::vector<uint32_t> prev_order, cur_order;
+    DynAdjacency dadj;
+    Recoloring rc;
+
     StaticAdjacency adj;
     Coloring coloring;
     std::vector<uint8_t> is_conflicted;
@@ -52,6 +65,15 @@ public:
 
     void build_graph();     // adjacency + DSATUR + work list (simulation/Simulation.cpp:87-89)
 
+    // CollisionPrimitivePool::addObject (collision/CollisionPrimitivePool.cpp:36-87)
+    uint32_t add_collision_triangles(uint32_t n, const uint32_t* t);
+    uint32_t add_collision_segments(uint32_t n_nodes, const uint32_t* nodes, Real radius);
+    uint32_t add_collision_sphere(uint32_t p, Real radius);
+    // Run the reference's per-step detectCollisionsAndRecolor over every
+    // primitive (capacity: slots per dynamic pool).  After build_graph().
+    void enable_dynamic_contacts(Real margin, uint32_t capacity);
+    void detect_and_recolor();
+
     // One time step of VBDSolver::solve_Parallel, single threaded.
     void step();
     // Same step on n_threads SPMD workers with a spin barrier between phases.
@@ -69,6 +91,17 @@ public:
     bool oriented(uint32_t p) const { return !rs.rot_of.empty() && rs.rot_of[p] >= 0; }
 
 private:
+    void process_pair(uint32_t a, uint32_t b, std::vector<Detected>& out);
+    void mark_dirty(uint32_t p);
+    void add_contact(Detected& d);
+    void remove_contact(const Detected& d);
+    void update_contact(const Detected& d);
+    void handle_detected();
+    void build_dynamic_adjacency();
+    void incremental_recoloring();
+    void phase_after_timestep_dyn();
+    void phase_dual_dyn();
+    void phase_commit(uint32_t c);
     void phase_after_timestep(uint32_t lo_nh, uint32_t hi_nh, uint32_t lo_g, uint32_t hi_g);
     void phase_after_timestep_rigid(uint32_t lo, uint32_t hi);
     void phase_inertial(uint32_t lo, uint32_t hi);

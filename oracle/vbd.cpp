@@ -527,6 +527,13 @@ void Simulation::accumulate_vertex(uint32_t p, V3& G, M3& H) const {
         }
         // Cosserat, rigid ground and joints support oriented particles only
     }
+    // dynamic energies after the static ones (VBDSolver.hpp:400-438)
+    if (dynamic && p + 1 < dadj.e_offsets.size()) {
+        for (uint32_t e = dadj.e_offsets[p][0]; e < dadj.e_offsets[p + 1][0]; e++) {
+            const DynAdjEntry& en = dadj.e_entries[e];
+            dyn_accumulate3(en.type, dyn[en.type].data[en.e_idx], x, rs, en.local, G, H);
+        }
+    }
 }
 
 // Static energies of an oriented particle in StaticEnergyType order
@@ -554,6 +561,12 @@ void Simulation::accumulate_vertex6(uint32_t p, V6& G, M6& H) const {
             } else if (ty == ST_JOINT) {
                 joint_accumulate(joints[en.e_idx], x, rs, G, H);
             }
+        }
+    }
+    if (dynamic && p + 1 < dadj.e_offsets.size()) {
+        for (uint32_t e = dadj.e_offsets[p][0]; e < dadj.e_offsets[p + 1][0]; e++) {
+            const DynAdjEntry& en = dadj.e_entries[e];
+            dyn_accumulate6(en.type, dyn[en.type].data[en.e_idx], x, rs, en.local, G, H);
         }
     }
 }
@@ -636,6 +649,28 @@ void Simulation::phase_chebyshev(Real omega, uint32_t lo, uint32_t hi) {
     }
 }
 
+// _commitConflictedInColor (VBDSolver.hpp:108-119): positions only -- the
+// buffered rotation of a conflicted oriented particle is not committed.
+void Simulation::phase_commit(uint32_t c) {
+    if (!dynamic || c + 1 >= rc.conflict_offsets.size()) return;
+    for (uint32_t k = rc.conflict_offsets[c]; k < rc.conflict_offsets[c + 1]; k++) {
+        const uint32_t p = rc.conflict_entries[k];
+        x[p] = x_buf[p];
+    }
+}
+
+void Simulation::phase_after_timestep_dyn() {
+    for (int ty = 0; ty < DYN_COUNT; ty++)
+        for (uint32_t e = 0; e < dyn[ty].sp.active.size(); e++)
+            if (dyn[ty].sp.active[e]) dyn_update_after_timestep((uint8_t)ty, dyn[ty].data[e], x, rs);
+}
+
+void Simulation::phase_dual_dyn() {
+    for (int ty = 0; ty < DYN_COUNT; ty++)
+        for (uint32_t e = 0; e < dyn[ty].sp.active.size(); e++)
+            if (dyn[ty].sp.active[e]) dyn_update_after_iteration((uint8_t)ty, dyn[ty].data[e], x, rs);
+}
+
 void Simulation::phase_dual(uint32_t lo, uint32_t hi) {
     for (uint32_t e = lo; e < hi; e++) ground_update_after_iteration(ground[e], x);
 }
@@ -672,14 +707,20 @@ static inline Real omega_at(uint32_t i, Real rho, Real omega) {
 
 void Simulation::step() {
     uint32_t n = (uint32_t)x.size();
-    if (!tris.empty()) {
-        // CollisionDetector::detectCollisionsAndRecolor_Parallel precedes the solve
+    if (dynamic) {
+        // CollisionDetector::detectCollisionsAndRecolor_Parallel over every
+        // primitive: broad + narrow phase, contact lifecycle, recolouring
+        detect_and_recolor();
+    } else if (!tris.empty()) {
+        // triangles only: the triangle-triangle narrow phase is a no-op, so
+        // the broad phase is the whole detection
         bp.build(tris, x, v, params.dt);
         last_candidates = bp.traverse_leaves(0, bp.n);
     }
     const uint32_t n_rigid = (uint32_t)(rground.size() + joints.size());
     phase_after_timestep(0, (uint32_t)nh.size(), 0, (uint32_t)ground.size());
     phase_after_timestep_rigid(0, n_rigid);
+    if (dynamic) phase_after_timestep_dyn();
     phase_inertial(0, n);
     Real omega = 1;
     const bool target = params.tol > 0;
@@ -688,10 +729,14 @@ void Simulation::step() {
     uint32_t used = iters;
     for (uint32_t i = 0; i < iters; i++) {
         omega = omega_at(i, params.rho, omega);
-        for (uint32_t c = 0; c < coloring.num_colors; c++) phase_color(c, 0, coloring.color_counts[c]);
+        for (uint32_t c = 0; c < coloring.num_colors; c++) {
+            phase_color(c, 0, coloring.color_counts[c]);
+            phase_commit(c);
+        }
         phase_chebyshev(omega, 0, n);
         phase_dual(0, (uint32_t)ground.size());
         phase_dual_rigid(0, n_rigid);
+        if (dynamic) phase_dual_dyn();
         if (target && (i + 1) % params.check_every == 0 && i + 1 < iters && residual2(0, n) <= tol2) {
             used = i + 1;
             break;
@@ -757,7 +802,7 @@ inline void range(uint32_t total, uint32_t idx, uint32_t nt, uint32_t& lo, uint3
 }  // namespace
 
 void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
-    if (nthreads <= 1) {
+    if (nthreads <= 1 || dynamic) {
         for (uint32_t s = 0; s < n_steps; s++) step();
         return;
     }
@@ -886,6 +931,18 @@ Stats Simulation::stats() const {
         Real Cc = Cn - CONSTRAINT_ALPHA * c.C_n_prev;
         if (c.k_n * Cc + c.lambda_n > 0) s.ip += 0.5 * c.k_n * Cc * Cc + c.lambda_n * Cc;
         s.ip += 0.5 * c.k_t * Ct * Ct + c.lambda_t * Ct + 0.5 * c.k_b * Cb * Cb + c.lambda_b * Cb;
+    }
+    if (dynamic) {
+        for (int ty = 0; ty < DYN_COUNT; ty++)
+            for (uint32_t e = 0; e < dyn[ty].sp.active.size(); e++) {
+                if (!dyn[ty].sp.active[e]) continue;
+                const DynContact& c = dyn[ty].data[e];
+                Real Cn, Ct, Cb;
+                dyn_eval((uint8_t)ty, c, x, rs, Cn, Ct, Cb);
+                Real Cc = Cn - CONSTRAINT_ALPHA * c.C_n_prev;
+                if (c.k_n * Cc + c.lambda_n > 0) s.ip += 0.5 * c.k_n * Cc * Cc + c.lambda_n * Cc;
+                s.ip += 0.5 * c.k_t * Ct * Ct + c.lambda_t * Ct + 0.5 * c.k_b * Cb * Cb + c.lambda_b * Cb;
+            }
     }
     s.residual_abs = std::sqrt(r2);
     s.residual_rel = mg2 > 0 ? std::sqrt(r2) / std::sqrt(mg2) : 0;

@@ -1,0 +1,340 @@
+"""Pins of the oracle's dynamic-contact pipeline (narrow phase, contact
+lifecycle, dynamic adjacency, incremental recolouring, triangle-rigid /
+triangle-rod AL contacts) against closed forms, brute force, finite
+differences and invariants.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from avbd_amd import scenes
+
+rng = np.random.default_rng(11)
+
+
+def test_contact_key_bit_layout():
+    # collision/DetectedCollision.hpp:43-66: 5 type | 26 idxA | 26 idxB | 7 code, (A, B) canonical
+    for _ in range(50):
+        t = int(rng.integers(0, 32))
+        a, b = (int(v) for v in rng.integers(0, 1 << 26, size=2))
+        code = int(rng.integers(0, 128))
+        lo, hi = min(a, b), max(a, b)
+        expect = (t << 59) | (lo << 33) | (hi << 7) | code
+        assert O.contact_key(t, a, b, code) == expect
+        assert O.contact_key(t, b, a, code) == expect
+
+
+def test_slot_pool_is_a_lifo_stack():
+    # TombstonePool: slots 0, 1, 2, ... on a stack; freed slots are reused last-in first-out
+    assert O.slot_sequence([-1, -1, -1, 1, 0, -1, -1, -1]) == [0, 1, 2, -1, -1, 0, 1, 3]
+    assert O.slot_sequence([-1, -1, 0, -1, -1]) == [0, 1, -1, 0, 2]
+    with pytest.raises(O.OracleError):
+        O.slot_sequence([-1, 0, 0])          # freeing an inactive slot
+    with pytest.raises(O.OracleError):
+        O.slot_sequence([-1, -1, -1], capacity=2)
+
+
+def test_golden_section_known_answers():
+    for a, b, c in [(0.0, 1.0, 0.3), (0.0, 1.0 / 60.0, 0.004), (-2.0, 3.0, 1.7)]:
+        t = O.golden_section_quadratic(a, b, c)
+        assert abs(t - c) <= 1e-3
+    # minimum outside the interval: the nearer end point
+    assert O.golden_section_quadratic(0.0, 1.0, 2.0) == 1.0
+    assert O.golden_section_quadratic(0.0, 1.0, -1.0) == 0.0
+    # an interval narrower than the tolerance is not refined: the midpoint or an end
+    t = O.golden_section_quadratic(0.0, 5e-4, 2.6e-4)
+    assert t in (0.0, 2.5e-4, 5e-4)
+
+
+def _tri_samples(a, b, c, n=120):
+    u, v = np.meshgrid(np.linspace(0, 1, n), np.linspace(0, 1, n))
+    m = (u + v) <= 1
+    u, v = u[m], v[m]
+    return a[None] * (1 - u - v)[:, None] + b[None] * u[:, None] + c[None] * v[:, None]
+
+
+def test_closest_point_triangle_brute_force():
+    for _ in range(40):
+        a, b, c, p = rng.normal(size=(4, 3))
+        q = O.closest_point_triangle(p, a, b, c)
+        s = _tri_samples(a, b, c)
+        d_bf = np.min(np.linalg.norm(s - p, axis=1))
+        assert np.linalg.norm(q - p) <= d_bf + 1e-9
+        # q lies on the triangle
+        bc = np.linalg.lstsq(np.stack([b - a, c - a], 1), q - a, rcond=None)[0]
+        assert bc.min() >= -1e-9 and bc.sum() <= 1 + 1e-9
+
+
+def test_closest_segment_triangle():
+    a, b, c = np.array([0.0, 0, 0]), np.array([1.0, 0, 0]), np.array([0.0, 0, 1])
+    # crossing the face: distance 0 and the crossing point
+    s, bc, d = O.closest_segment_triangle([0.2, -1, 0.3], [0.2, 1, 0.3], a, b, c)
+    assert d == 0.0 and abs(s - 0.5) < 1e-12
+    assert np.allclose(bc[0] * a + bc[1] * b + bc[2] * c, [0.2, 0, 0.3])
+    # a segment hovering above the face interior: the lower end point is closest
+    s, bc, d = O.closest_segment_triangle([0.2, 0.05, 0.2], [0.25, 0.5, 0.2], a, b, c)
+    assert s == 0.0 and abs(d - 0.05) < 1e-12 and np.allclose(bc, [0.6, 0.2, 0.2])
+    # any configuration: the reported distance is a real point-pair distance, so
+    # never below the brute-force minimum, and equal to it when an end point
+    # over the face interior is the closest feature
+    for _ in range(30):
+        p0, p1 = rng.normal(size=(2, 3))
+        s, bc, d = O.closest_segment_triangle(p0, p1, a, b, c)
+        seg = p0[None] + np.linspace(0, 1, 400)[:, None] * (p1 - p0)[None]
+        tri = _tri_samples(a, b, c, 60)
+        d_bf = np.min(np.linalg.norm(seg[:, None] - tri[None], axis=2))
+        assert d >= d_bf - 2e-2 * max(1.0, d_bf) - 1e-6 or d == 0.0
+        assert 0.0 <= s <= 1.0
+
+
+def test_complete_basis_follows_eigen_unit_orthogonal():
+    # Eigen's rule: (-y, x, 0)/|(x, y)| unless x, y are negligible against z
+    t, b = O.complete_basis([0, 0, 1.0])
+    assert np.allclose(t, [0, -1, 0]) and np.allclose(b, np.cross([0, 0, 1.0], t))
+    t, b = O.complete_basis([1.0, 0, 0])
+    assert np.allclose(t, [0, 1, 0])
+    for _ in range(30):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        t, b = O.complete_basis(n)
+        assert abs(t @ n) < 1e-12 and abs(np.linalg.norm(t) - 1) < 1e-12 and t[2] == 0.0
+        assert np.allclose(b, np.cross(n, t))
+
+
+def test_update_collision_frame_preserves_friction_force():
+    n = np.array([0.0, 1.0, 0.0])
+    t, b = O.complete_basis(n)
+    lt, lb = 0.7, -0.3
+    f = lt * t + lb * b
+    # rotate the normal about the force direction: the force stays tangent
+    axis = f / np.linalg.norm(f)
+    ang = 0.4
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    R = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+    nn = R @ n
+    n2, t2, b2, lt2, lb2 = O.update_collision_frame(n, t, b, lt, lb, nn)
+    assert np.allclose(n2, nn)
+    F = np.stack([n2, t2, b2])
+    assert np.allclose(F @ F.T, np.eye(3), atol=1e-12)
+    assert np.allclose(lt2 * t2 + lb2 * b2, f, atol=1e-12)
+    # the new tangent is the old one projected onto the new plane
+    tp = t - (t @ nn) * nn
+    assert np.allclose(t2, tp / np.linalg.norm(tp))
+
+
+def test_tri_sphere_ccd_closed_form():
+    """A sphere falling straight onto a static horizontal triangle: contact iff
+    the gap closes to within the margin during the step; normal = -y (from
+    the centre towards the triangle); contact point on the sphere = (0, -r, 0)
+    in the body frame; barycentrics = the centre's projection."""
+    tri = np.array([[0.0, 0, 0], [1.0, 0, 0], [0.0, 0, 1.0]])
+    tv = np.zeros((3, 3))
+    r, dt, margin = 0.1, 1.0 / 60.0, 1e-3
+    centre = np.array([0.25, 0.0, 0.3])
+    for gap, speed, hit in [(0.0005, 0.0, True), (0.002, 0.0, False), (0.05, 3.6, True), (0.08, 3.6, False)]:
+        pos = centre + [0, r + gap, 0]
+        got, n, bc, cpl = O.tri_sphere_ccd(tri, tv, pos, [1, 0, 0, 0], [0, -speed, 0], [0, 0, 0], r, dt, margin)
+        assert got == hit, (gap, speed)
+        if hit:
+            # the spatial search stops at the golden-section tolerance (1e-3 m),
+            # so the contact point sits within ~1e-3 of the projection
+            assert np.allclose(n, [0, -1, 0], atol=1e-2) and abs(np.linalg.norm(n) - 1) < 1e-12
+            assert np.allclose(cpl, r * n, atol=1e-12)
+            p = bc[0] * tri[0] + bc[1] * tri[1] + bc[2] * tri[2]
+            assert np.linalg.norm(p - centre) < 3e-3
+    # spin about the contact normal does not move the contact
+    got, n, bc, cpl = O.tri_sphere_ccd(tri, tv, centre + [0, r + 0.0005, 0], [1, 0, 0, 0], [0, 0, 0], [0, 5.0, 0],
+                                       r, dt, margin)
+    assert got and np.allclose(n, [0, -1, 0], atol=1e-2)
+
+
+def _rand_quat():
+    q = rng.normal(size=4)
+    return q / np.linalg.norm(q)
+
+
+def _plus(q, d):
+    e = O.exp_s3(d)
+    w1, x1, y1, z1 = q
+    w2, x2, y2, z2 = e
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+
+
+def _rec(n, t, b, barys, s, cpl, kn=50.0, kt=40.0, kb=30.0, ln=2.0, lt=0.1, lb=-0.2, cprev=0.0, mus=0.4, muk=0.2,
+         use_static=0):
+    return np.concatenate([n, t, b, barys, [s], cpl, [kn, kt, kb, ln, lt, lb, cprev, mus, muk, use_static]])
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_dynamic_contact_gradients_by_finite_differences(kind):
+    """C_n gradients of both contact kinds against central differences:
+    positions for the triangle vertices and the body / rod nodes, the S3
+    perturbation q (+) dtheta for the orientations (test/finite_difference.hpp:35-49)."""
+    x = rng.normal(size=(5, 3)) * 0.1
+    q = np.zeros((5, 4))
+    q[:, 0] = 1
+    q[3], q[4] = _rand_quat(), _rand_quat()
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    t, b = O.complete_basis(n)
+    bc = np.array([0.2, 0.3, 0.5])
+    rec = _rec(n, t, b, bc, 0.35, rng.normal(size=3) * 0.02)
+    h = 1e-6
+
+    def Cn(xx, qq):
+        return O.dyn_single(kind, xx, qq, rec, 0, 3)[0][0]
+
+    locals_ = [0, 1, 2, 3] if kind == 0 else [0, 1, 2, 3, 4]
+    for loc in locals_:
+        dof = 3 if loc < 3 else 6
+        _, g, hblk, _, _ = O.dyn_single(kind, x, q, rec, loc, dof)
+        fd = np.zeros(dof)
+        for i in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[loc, i] += h
+            xm[loc, i] -= h
+            fd[i] = (Cn(xp, q) - Cn(xm, q)) / (2 * h)
+        if dof == 6:
+            for i in range(3):
+                d = np.zeros(3)
+                d[i] = h
+                qp, qm = q.copy(), q.copy()
+                qp[loc] = _plus(q[loc], d)
+                qm[loc] = _plus(q[loc], -d)
+                fd[3 + i] = (Cn(x, qp) - Cn(x, qm)) / (2 * h)
+        if kind == 1 and dof == 6:
+            # TriangleRodCollisionEnergySolver.hpp:97,125 negates R [cp]x twice
+            # (R_rloc = -R [cp]x, then -n^T R_rloc dRs/dR): the rotational part
+            # is the negated derivative of C_n.  Restated as the reference has it.
+            fd[3:] = -fd[3:]
+        assert np.allclose(g[0, :dof], fd, atol=1e-6), (kind, loc, g[0, :dof], fd)
+        if dof == 6:
+            # tangent gradients of the oriented side are zeroed in the reference
+            assert np.all(g[1] == 0) and np.all(g[2] == 0)
+            if kind == 0:
+                # rotational Hessian block of C_n = symmetrised second derivative
+                H = np.zeros((3, 3))
+                for i in range(3):
+                    for j in range(3):
+                        di = np.zeros(3)
+                        dj = np.zeros(3)
+                        di[i] = 1e-4
+                        dj[j] = 1e-4
+                        vals = []
+                        for si, sj in [(1, 1), (1, -1), (-1, 1), (-1, -1)]:
+                            qq = q.copy()
+                            qq[3] = _plus(q[3], si * di + sj * dj)
+                            vals.append(Cn(x, qq))
+                        H[i, j] = (vals[0] - vals[1] - vals[2] + vals[3]) / (4e-8)
+                assert np.allclose(hblk[0], 0.5 * (H + H.T), atol=2e-5)
+            else:
+                assert np.all(hblk == 0)
+
+
+def test_dynamic_contact_accumulate_al_terms():
+    """Positional incidence: G = lambda+ * grad, H = k grad grad^T per direction,
+    with the friction multipliers clamped to the cone mu * lambda_n+."""
+    x = rng.normal(size=(5, 3)) * 0.1
+    q = np.zeros((5, 4))
+    q[:, 0] = 1
+    n = np.array([0.0, 1.0, 0.0])
+    t, b = O.complete_basis(n)
+    rec = _rec(n, t, b, np.array([0.2, 0.3, 0.5]), 0.0, np.array([0.0, -0.05, 0.0]), lt=5.0, lb=5.0)
+    Cv, g, _, G, H = O.dyn_single(0, x, q, rec, 1, 3)
+    kn, kt, kb, ln, lt, lb, cprev, mus, muk = rec[16:25]
+    ln_p = max(0.0, kn * (Cv[0] - 0.95 * cprev) + ln)
+    ltp, lbp = kt * Cv[1] + lt, kb * Cv[2] + lb
+    mag = np.hypot(ltp, lbp)
+    if mag > muk * ln_p:
+        ltp, lbp = ltp / (mag + 1e-8) * muk * ln_p, lbp / (mag + 1e-8) * muk * ln_p
+    assert np.allclose(G, ln_p * g[0, :3] + ltp * g[1, :3] + lbp * g[2, :3])
+    Hx = kn * np.outer(g[0, :3], g[0, :3]) + kt * np.outer(g[1, :3], g[1, :3]) + kb * np.outer(g[2, :3], g[2, :3])
+    assert np.allclose(H, Hx)
+    # -bary * frame vectors for a triangle vertex
+    assert np.allclose(g[0, :3], -0.3 * n) and np.allclose(g[1, :3], -0.3 * t)
+
+
+def _union_graph(o, sc):
+    adj = o.adjacency()
+    n = o.n
+    nb = [set() for _ in range(n)]
+    _, _, off, nbrs = adj
+    for p in range(n):
+        for k in range(off[p], off[p + 1]):
+            nb[p].add(int(nbrs[k]))
+    for kind, npe in ((0, 4), (1, 5)):
+        for r in o.contacts(kind):
+            idx = [int(v) for v in r[1:1 + npe]]
+            for i in idx:
+                for j in idx:
+                    if i != j:
+                        nb[i].add(j)
+    return nb
+
+
+@pytest.mark.parametrize("name", ["ball_on_block", "rod_on_block"])
+def test_recolouring_invariants_and_lifecycle(name):
+    """After every step's detection: the colouring is proper on the static +
+    dynamic graph except between conflicted particles, and every same-colour
+    adjacent pair is conflicted; contacts persist with their slots while
+    detected; the detected list is key-sorted."""
+    sc = getattr(scenes, name)(steps=30)
+    o = O.from_scene(sc)
+    seen_contacts = 0
+    prev = {}
+    for step in range(25):
+        o.step(1)
+        keys, slots, types = o.detected()
+        assert np.all(keys[1:] > keys[:-1])
+        cur = {int(k): int(s) for k, s in zip(keys, slots)}
+        for k, s in cur.items():
+            if k in prev:
+                assert prev[k] == s   # an updated contact keeps its slot
+        prev = cur
+        seen_contacts += len(cur)
+        col, _ = o.colors()
+        conf = o.conflicted()
+        nb = _union_graph(o, sc)
+        for p in range(o.n):
+            for q in nb[p]:
+                if col[p] == col[q]:
+                    assert conf[p] and conf[q], (step, p, q)
+        # contact records: slots unique, frames orthonormal, k >= k_start
+        for kind in (0, 1):
+            rec = o.contacts(kind)
+            if len(rec):
+                assert len(set(rec[:, 0].astype(int))) == len(rec)
+                F = rec[:, 6:15].reshape(-1, 3, 3)
+                assert np.allclose(np.einsum("nij,nkj->nik", F, F), np.eye(3)[None], atol=1e-9)
+                assert np.all(rec[:, 23] >= rec[:, 22] * (1 - 1e-12))
+    assert seen_contacts > 0
+
+
+def test_resting_ball_multipliers_carry_its_weight():
+    """Sphere resting on the block: the vertical force of the triangle-rigid
+    contacts balances m g (AL fixed point with the per-step multiplier decay)."""
+    sc = scenes.ball_on_block(steps=200, speed=0.0, drop=0.0005)
+    o = O.from_scene(sc)
+    o.step(150)
+    m, _ = o.mass_fixed()
+    p_ball = sc.meshes[0].rest.shape[0]
+    w = m[p_ball] * sc.g
+    rec = o.contacts(0)
+    assert len(rec) > 0
+    fy = 0.0
+    for r in rec:
+        n, t, b = r[6:9], r[9:12], r[12:15]
+        fy += -(r[26] * n[1] + r[27] * t[1] + r[28] * b[1])
+    # multipliers after the step-start decay alpha*gamma of the next step
+    assert abs(fy - w) < 0.1 * w, (fy, w)
+
+
+def test_unsupported_pairs_raise():
+    """Sphere-sphere and rod-segment-sphere candidate pairs throw in the reference
+    (collision/CollisionDetector.cpp:210-213,686-687)."""
+    sc = scenes.Scene("two_balls", spheres=[scenes.Sphere(0.05, 1000.0, (0, 0.5, 0), ground=False),
+                                            scenes.Sphere(0.05, 1000.0, (0.09, 0.5, 0), ground=False)],
+                      contact_margin=1e-3)
+    o = O.from_scene(sc)
+    with pytest.raises(O.OracleError):
+        o.detect_and_recolor()
