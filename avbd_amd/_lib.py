@@ -155,6 +155,31 @@ class Stats(C.Structure):
     ]
 
 
+class Contact(C.Structure):
+    """avbd_contact: one dynamic triangle-rigid / triangle-rod contact."""
+    _fields_ = [
+        ("slot", C.c_uint32),
+        ("idx", C.c_uint32 * 5),
+        ("n", C.c_double * 3),
+        ("t", C.c_double * 3),
+        ("b", C.c_double * 3),
+        ("barys", C.c_double * 3),
+        ("s", C.c_double),
+        ("cp_local", C.c_double * 3),
+        ("k_start", C.c_double),
+        ("k_n", C.c_double),
+        ("k_t", C.c_double),
+        ("k_b", C.c_double),
+        ("lambda_n", C.c_double),
+        ("lambda_t", C.c_double),
+        ("lambda_b", C.c_double),
+        ("C_n_prev", C.c_double),
+        ("use_static", C.c_uint8),
+        ("unclamped", C.c_uint8),
+        ("pad_", C.c_uint8 * 6),
+    ]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _U32 = C.POINTER(C.c_uint32)
@@ -214,6 +239,12 @@ SIGNATURES = {
     "avbd_detect_collisions": (C.c_int, [_P]),
     "avbd_download_bvh": (C.c_int, [_P, C.POINTER(C.c_uint64), _U32, _U32, _U32, _U32, _D, _U32]),
     "avbd_download_candidates": (C.c_int, [_P, _U32, C.c_uint32, _U32]),
+    "avbd_add_collision_segments": (C.c_int, [_P, C.c_uint32, _U32, C.c_double]),
+    "avbd_add_collision_sphere": (C.c_int, [_P, C.c_uint32, C.c_double]),
+    "avbd_set_dynamic_contacts": (C.c_int, [_P, C.c_double, C.c_uint32]),
+    "avbd_download_contacts": (C.c_int, [_P, C.c_uint32, C.POINTER(Contact), C.c_uint32, _U32]),
+    "avbd_download_detected": (C.c_int, [_P, C.POINTER(C.c_uint64), _U32, _U8, C.c_uint32, _U32]),
+    "avbd_download_conflicts": (C.c_int, [_P, _U8]),
 }
 
 

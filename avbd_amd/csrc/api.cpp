@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "contacts.hpp"
 #include "internal.hpp"
 #include "mlp_model.hpp"
 #include "lbvh.hpp"
@@ -88,6 +89,22 @@ struct avbd_ctx {
     float* d_raw = nullptr;        // n_items x 24
     float* d_tgt = nullptr;        // n_items x 3
 
+    // dynamic contacts (avbd_set_dynamic_contacts)
+    std::vector<HostPrim> prims;   // CollisionPrimitivePool, call order
+    uint32_t next_object = 0;
+    bool dynamic = false;
+    ContactSystem cs;
+    DevPrims dp;
+    LbvhDev lbd;                   // LBVH over every primitive
+    DetRec* d_det = nullptr;
+    uint32_t det_cap = 0;
+    uint32_t* d_status = nullptr;  // [0] narrow-phase flags, [1] detected count
+    struct Grow {                  // device buffer that grows on demand
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    Grow g_ops, g_active, g_off, g_ent, g_conf;
+
     // iterations per step: the budget in residual mode, else solver_iters
     uint32_t iters() const { return res_tol > 0 ? res_max_iters : desc.solver_iters; }
 };
@@ -125,6 +142,7 @@ int guarded(avbd_ctx* c, F&& f) {
         const std::string m = e.what();
         if (m.find("Ran out of colors") != std::string::npos) return AVBD_E_COLORS;
         if (m.find("capacity") != std::string::npos) return AVBD_E_CAPACITY;
+        if (m.find("Unsupported collision") != std::string::npos) return AVBD_E_UNSUPPORTED_CONTACT;
         return AVBD_E_ARG;
     }
 }
@@ -136,6 +154,24 @@ T* dev_alloc(avbd_ctx* c, size_t count) {
     hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
     c->allocations.push_back(p);
     return static_cast<T*>(p);
+}
+
+// Make g hold at least `bytes`; the old contents are not kept.
+void* grow(avbd_ctx* c, avbd_ctx::Grow& g, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (g.bytes >= bytes) return g.p;
+    if (g.p) {
+        c->allocations.erase(std::remove(c->allocations.begin(), c->allocations.end(), g.p), c->allocations.end());
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        (void)hipFree(g.p);
+        g.p = nullptr;
+        g.bytes = 0;
+    }
+    const size_t nb = std::max(bytes, 2 * g.bytes);
+    hip_check(hipMalloc(&g.p, nb), "hipMalloc");
+    c->allocations.push_back(g.p);
+    g.bytes = nb;
+    return g.p;
 }
 
 template <class T>
@@ -321,6 +357,34 @@ void upload_rotation_range(avbd_ctx* c, uint32_t first, uint32_t n) {
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
 }
 
+// Device buffers of an LBVH over n primitives (bounds are written by the caller's kernel).
+void alloc_lbvh(avbd_ctx* c, LbvhDev& b, uint32_t n) {
+    const uint32_t nn = 2 * n - 1;
+    b.n = n;
+    b.aabb = dev_alloc<double>(c, 6 * (size_t)n);
+    b.sbox = dev_alloc<double>(c, 6);
+    b.keys = dev_alloc<uint64_t>(c, n);
+    b.keys_sorted = dev_alloc<uint64_t>(c, n);
+    b.sorted = dev_alloc<uint32_t>(c, n);
+    b.left = dev_alloc<uint32_t>(c, nn);
+    b.right = dev_alloc<uint32_t>(c, nn);
+    b.parent = dev_alloc<uint32_t>(c, nn);
+    b.range_hi = dev_alloc<uint32_t>(c, nn);
+    b.bucket_count = dev_alloc<uint32_t>(c, 1u << 13);
+    b.bucket_offset = dev_alloc<uint32_t>(c, (1u << 13) + 1);
+    b.slots = dev_alloc<uint32_t>(c, n);
+    b.table = dev_alloc<double>(c, 6 * (size_t)lbvh_table_entries(n));
+    b.node_box = dev_alloc<double>(c, 6 * (size_t)nn);
+    b.pair_cap = std::max<uint32_t>(1024u, 64u * n);
+    b.pairs = dev_alloc<uint32_t>(c, 2 * (size_t)b.pair_cap);
+    b.counters = dev_alloc<uint32_t>(c, 2);
+    // left/right of leaves stay "none" (0xFFFFFFFF) for the whole run
+    std::vector<uint32_t> none(nn, 0xFFFFFFFFu);
+    upload(c, b.left, none);
+    upload(c, b.right, none);
+    upload(c, b.parent, none);
+}
+
 double omega_at(uint32_t i, double rho, double omega) {
     // Chebyshev schedule, simulation/VBDSolver.hpp:65-74
     if (i > 2) return 4 / (4 - rho * rho * omega);
@@ -339,8 +403,10 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
     // bounds kernel has read the start-of-step state, the rest of the build
     // and the traversal run on a second stream, concurrently with the solve,
     // and join at the end of the step.
-    const bool fork = c->lb.n && c->side_stream;
-    if (c->lb.n) {
+    // (with dynamic contacts the detection round has already run, see detect_round)
+    const bool tri_bp = c->lb.n && !c->dynamic;
+    const bool fork = tri_bp && c->side_stream;
+    if (tri_bp) {
         hipStream_t sb = fork ? c->side_stream : s;
         if (with_events) hip_check(hipEventRecord(c->ev_lb0, s), "hipEventRecord");
         launch_lbvh_bounds(s, c->lb, c->d.x, c->d.v, c->desc.dt);
@@ -384,8 +450,12 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
         hip_check(hipGetLastError(), "kernel launch");
         return;
     }
+    const uint32_t* d_active = static_cast<const uint32_t*>(c->g_active.p);
+    const uint32_t n_active = c->dynamic ? (uint32_t)c->cs.active.size() : 0u;
     if (!pre) {
         if (c->damping) launch_eprev(s, c->d, dt);
+        // dynamic contacts' updateAfterTimeStep reads the start-of-step positions
+        if (n_active) launch_dyn_step_begin(s, c->d, d_active, n_active);
         launch_step_begin(s, c->d, dt, g);
     }
     const uint32_t* done = c->rc ? &c->rc->done : nullptr;
@@ -406,6 +476,7 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
             if (ce > cb) launch_commit(s, c->d, c->d_conflicts + cb, ce - cb, done);
         }
         launch_cheb_dual(s, c->d, (float)omega, done);
+        if (n_active) launch_dyn_dual(s, c->d, d_active, n_active, done);
         if (c->rc && (i + 1) % c->res_check_every == 0 && i + 1 < iters)
             launch_residual_check(s, c->d, c->cfg, c->damping, dt, c->rc, (i + 1) / c->res_check_every - 1, i + 1,
                                   c->res_tol2);
@@ -448,6 +519,64 @@ void capture_graph(avbd_ctx* c) {
     }
     hip_check(hipStreamEndCapture(c->stream, &c->graph), "hipStreamEndCapture");
     hip_check(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+// One round of CollisionDetector::detectCollisionsAndRecolor_Parallel
+// (collision/CollisionDetector.cpp:57-129) at the start of a step: device
+// bounds, LBVH and narrow phase; the detected list comes back to the host
+// for the lifecycle, the dynamic adjacency and the recolouring; the
+// resulting record ops, adjacency and (when the colouring changed) work
+// items go back before the solve.
+void detect_round(avbd_ctx* c) {
+    hipStream_t s = c->stream;
+    const double dt = c->desc.dt;
+    hip_check(hipMemsetAsync(c->d_status, 0, 2 * sizeof(uint32_t), s), "hipMemsetAsync");
+    launch_prim_bounds(s, c->dp, c->lbd, c->d, dt);
+    launch_lbvh_tree(s, c->lbd);
+    launch_narrow(s, c->dp, c->lbd, c->d, dt, c->cs.margin, c->d_det, c->det_cap, c->d_status);
+    hip_check(hipGetLastError(), "kernel launch");
+    uint32_t st[4] = {0, 0, 0, 0};
+    hip_check(hipMemcpyAsync(st, c->d_status, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(st + 2, c->lbd.counters, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (st[2] > c->lbd.pair_cap || (st[3] & 1u))
+        throw CodeError(AVBD_E_CAPACITY, "broad phase: candidate pairs exceeded capacity");
+    if (st[0] & NARROW_UNSUPPORTED) throw CodeError(AVBD_E_UNSUPPORTED_CONTACT, "Unsupported collision!");
+    if (st[1] > c->det_cap) throw CodeError(AVBD_E_CAPACITY, "narrow phase: detected contacts exceeded capacity");
+    std::vector<DetRec> det(st[1]);
+    if (!det.empty()) {
+        hip_check(hipMemcpyAsync(det.data(), c->d_det, det.size() * sizeof(DetRec), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+    const bool relayout = c->cs.process(det, c->prims, c->mass, dt, c->plan);
+    ContactSystem& cs = c->cs;
+    if (!cs.ops.empty()) {
+        auto* d_ops = static_cast<DynOp*>(grow(c, c->g_ops, cs.ops.size() * sizeof(DynOp)));
+        hip_check(hipMemcpyAsync(d_ops, cs.ops.data(), cs.ops.size() * sizeof(DynOp), hipMemcpyHostToDevice, s), "H2D");
+        launch_dyn_apply(s, c->d, d_ops, (uint32_t)cs.ops.size());
+    }
+    auto* d_act = static_cast<uint32_t*>(grow(c, c->g_active, cs.active.size() * sizeof(uint32_t)));
+    upload(c, d_act, cs.active);
+    auto* d_off = static_cast<uint32_t*>(grow(c, c->g_off, cs.dyn_off.size() * sizeof(uint32_t)));
+    upload(c, d_off, cs.dyn_off);
+    auto* d_ent = static_cast<uint32_t*>(grow(c, c->g_ent, cs.dyn_ent.size() * sizeof(uint32_t)));
+    upload(c, d_ent, cs.dyn_ent);
+    c->d.dyn_off = d_off;
+    c->d.dyn_ent = d_ent;
+    if (relayout) {
+        upload(c, c->d.items, c->plan.items);
+        upload(c, c->d.oitems, c->plan.oitems);
+        c->d_conflicts = static_cast<uint32_t*>(grow(c, c->g_conf, c->plan.conflict_list.size() * sizeof(uint32_t)));
+        upload(c, c->d_conflicts, c->plan.conflict_list);
+        const size_t need = 2 * (size_t)c->iters() * c->plan.num_colors;
+        while (c->events.size() < need) {
+            hipEvent_t ev;
+            hip_check(hipEventCreate(&ev), "hipEventCreate");
+            c->events.push_back(ev);
+        }
+    }
+    // the host vectors are reused next round: the copies must have finished
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
 // Autotuner: time the colour-phase solve of one iteration for each candidate
@@ -1166,33 +1295,9 @@ int avbd_build_graph(avbd_ctx* c) {
         if (nt) {
             for (uint32_t p : c->tris)
                 if (p >= n) throw std::runtime_error("collision triangle references a particle out of range");
-            LbvhDev& b = c->lb;
-            const uint32_t nn = 2 * nt - 1;
-            b.n = nt;
-            b.tris = dev_alloc<uint32_t>(c, 3 * (size_t)nt);
-            b.aabb = dev_alloc<double>(c, 6 * (size_t)nt);
-            b.sbox = dev_alloc<double>(c, 6);
-            b.keys = dev_alloc<uint64_t>(c, nt);
-            b.keys_sorted = dev_alloc<uint64_t>(c, nt);
-            b.sorted = dev_alloc<uint32_t>(c, nt);
-            b.left = dev_alloc<uint32_t>(c, nn);
-            b.right = dev_alloc<uint32_t>(c, nn);
-            b.parent = dev_alloc<uint32_t>(c, nn);
-            b.range_hi = dev_alloc<uint32_t>(c, nn);
-            b.bucket_count = dev_alloc<uint32_t>(c, 1u << 13);
-            b.bucket_offset = dev_alloc<uint32_t>(c, (1u << 13) + 1);
-            b.slots = dev_alloc<uint32_t>(c, nt);
-            b.table = dev_alloc<double>(c, 6 * (size_t)lbvh_table_entries(nt));
-            b.node_box = dev_alloc<double>(c, 6 * (size_t)nn);
-            b.pair_cap = std::max<uint32_t>(1024u, 64u * nt);
-            b.pairs = dev_alloc<uint32_t>(c, 2 * (size_t)b.pair_cap);
-            b.counters = dev_alloc<uint32_t>(c, 2);
-            upload(c, b.tris, c->tris);
-            // left/right of leaves stay "none" (0xFFFFFFFF) for the whole run
-            std::vector<uint32_t> none(nn, 0xFFFFFFFFu);
-            upload(c, b.left, none);
-            upload(c, b.right, none);
-            upload(c, b.parent, none);
+            alloc_lbvh(c, c->lb, nt);
+            c->lb.tris = dev_alloc<uint32_t>(c, 3 * (size_t)nt);
+            upload(c, c->lb.tris, c->tris);
         }
         // persistent step kernel: per-colour offsets, Chebyshev schedule, barrier
         {
@@ -1258,6 +1363,32 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
     return guarded(c, [&] {
         require_built(c);
         hip_check(hipEventRecord(c->ev_a, c->stream), "hipEventRecord");
+        if (c->dynamic) {
+            // per step: detection round (synchronising), then the solve
+            double total = 0;
+            uint32_t launches = 0;
+            for (uint32_t st = 0; st < n_steps; st++) {
+                detect_round(c);
+                enqueue_step(c, c->timing);
+                c->vertex_updates += (uint64_t)c->desc.solver_iters * (c->d.n_items + c->d.n_oitems);
+                if (c->timing) {
+                    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+                    for (size_t k = 0; k + 1 < 2 * (size_t)c->iters() * c->plan.num_colors; k += 2) {
+                        float ms = 0;
+                        hip_check(hipEventElapsedTime(&ms, c->events[k], c->events[k + 1]), "hipEventElapsedTime");
+                        total += ms;
+                        launches++;
+                    }
+                }
+            }
+            if (c->timing) {
+                c->solve_avg_ms = launches ? total / launches : 0;
+                c->solve_launches = launches;
+            }
+            hip_check(hipEventRecord(c->ev_b, c->stream), "hipEventRecord");
+            c->steps_taken += n_steps;
+            return;
+        }
         if (c->timing) {
             double total = 0, lbvh_total = 0;
             uint32_t launches = 0;
@@ -1808,6 +1939,7 @@ int avbd_record_steps(avbd_ctx* c, uint32_t n_steps, float* feats, float* tgts, 
     return guarded(c, [&] {
         ensure_model(c);
         require_not_warm(c);
+        if (c->dynamic) throw CodeError(AVBD_E_ARG, "recording is not available with dynamic contacts");
         const uint32_t ni = c->d.n_items;
         if ((uint64_t)n_steps * ni > cap_rows) throw std::runtime_error("record buffer too small");
         c->recording = true;
@@ -1835,6 +1967,7 @@ int avbd_record_steps(avbd_ctx* c, uint32_t n_steps, float* feats, float* tgts, 
 int avbd_set_warm_start(avbd_ctx* c, int enable) {
     return guarded(c, [&] {
         ensure_model(c);
+        if (enable && c->dynamic) throw CodeError(AVBD_E_ARG, "the warm start is not available with dynamic contacts");
         if (enable && !c->mlp->loaded) throw CodeError(AVBD_E_ARG, "the warm-start network has no trained parameters");
         if (enable) c->mlp->ensure_batch((int)c->rows_pad);
         c->warm = enable != 0;
@@ -1855,6 +1988,157 @@ int avbd_add_collision_triangles(avbd_ctx* c, uint32_t n, const uint32_t* tris) 
         require_not_planned(c);
         if (n && !tris) throw std::runtime_error("tris is null");
         c->tris.insert(c->tris.end(), tris, tris + 3 * (size_t)n);
+        const uint32_t obj = c->next_object++;
+        for (uint32_t i = 0; i < n; i++)
+            c->prims.push_back(HostPrim{PRIM_TRIANGLE, {tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]}, obj, 0.0});
+    });
+}
+
+int avbd_add_collision_segments(avbd_ctx* c, uint32_t n_nodes, const uint32_t* nodes, double radius) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (n_nodes < 2 || !nodes) throw std::runtime_error("a rod needs at least two nodes");
+        if (!(radius >= 0)) throw std::runtime_error("radius must be >= 0");
+        const uint32_t obj = c->next_object++;
+        for (uint32_t i = 0; i + 1 < n_nodes; i++)
+            c->prims.push_back(HostPrim{PRIM_ROD_SEGMENT, {nodes[i], nodes[i + 1], 0u}, obj, radius});
+    });
+}
+
+int avbd_add_collision_sphere(avbd_ctx* c, uint32_t p, double radius) {
+    return guarded(c, [&] {
+        require_not_planned(c);
+        if (!(radius > 0)) throw std::runtime_error("radius must be > 0");
+        c->prims.push_back(HostPrim{PRIM_SPHERE, {p, 0u, 0u}, c->next_object++, radius});
+    });
+}
+
+int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (c->dynamic) throw CodeError(AVBD_E_ARG, "dynamic contacts are already on");
+        if (c->prims.empty()) throw CodeError(AVBD_E_ARG, "no collision primitives");
+        if (!(margin >= 0)) throw CodeError(AVBD_E_ARG, "collision margin must be >= 0");
+        if (c->warm || c->recording || c->res_tol > 0)
+            throw CodeError(AVBD_E_ARG, "warm start, recording and residual targets are not available with dynamic contacts");
+        const uint32_t n = c->d.n;
+        auto oriented = [&](uint32_t p) { return p < n && c->rot_of[p] >= 0; };
+        for (const HostPrim& pr : c->prims) {
+            if (pr.type == PRIM_TRIANGLE) {
+                for (int k = 0; k < 3; k++)
+                    if (pr.idx[k] >= n || oriented(pr.idx[k]))
+                        throw CodeError(AVBD_E_ARG, "collision triangle needs three positional particles");
+            } else if (pr.type == PRIM_ROD_SEGMENT) {
+                if (!oriented(pr.idx[0]) || !oriented(pr.idx[1]))
+                    throw CodeError(AVBD_E_ARG, "rod segment needs two oriented particles");
+            } else if (!oriented(pr.idx[0])) {
+                throw CodeError(AVBD_E_ARG, "collision sphere needs an oriented particle");
+            }
+        }
+        const uint32_t cap = capacity ? capacity : 65536u;
+        if ((uint64_t)2 * cap > (1ull << 29)) throw CodeError(AVBD_E_ARG, "capacity too large");
+        const uint32_t np = (uint32_t)c->prims.size();
+        destroy_graph(c);
+        c->persistent = false;
+        alloc_lbvh(c, c->lbd, np);
+        std::vector<uint32_t> ty(np), id(3 * (size_t)np), ob(np);
+        std::vector<double> rad(np);
+        for (uint32_t i = 0; i < np; i++) {
+            ty[i] = c->prims[i].type;
+            for (int k = 0; k < 3; k++) id[3 * i + k] = c->prims[i].idx[k];
+            ob[i] = c->prims[i].object;
+            rad[i] = c->prims[i].radius;
+        }
+        c->dp.n = np;
+        c->dp.type = dev_alloc<uint32_t>(c, np);
+        c->dp.idx = dev_alloc<uint32_t>(c, 3 * (size_t)np);
+        c->dp.object = dev_alloc<uint32_t>(c, np);
+        c->dp.radius = dev_alloc<double>(c, np);
+        upload(c, c->dp.type, ty);
+        upload(c, c->dp.idx, id);
+        upload(c, c->dp.object, ob);
+        upload(c, c->dp.radius, rad);
+        c->det_cap = c->lbd.pair_cap;
+        c->d_det = dev_alloc<DetRec>(c, c->det_cap);
+        c->d_status = dev_alloc<uint32_t>(c, 2);
+        c->d.dyn = dev_alloc<DynRec>(c, 2 * (size_t)cap);
+        c->cs.init(n, cap, margin);
+        auto* d_off = static_cast<uint32_t*>(grow(c, c->g_off, (n + 1) * sizeof(uint32_t)));
+        upload(c, d_off, c->cs.dyn_off);
+        c->d.dyn_off = d_off;
+        c->d.dyn_ent = static_cast<uint32_t*>(grow(c, c->g_ent, 16));
+        c->dynamic = true;
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    });
+}
+
+int avbd_download_contacts(avbd_ctx* c, uint32_t kind, avbd_contact* out, uint32_t cap, uint32_t* count) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!c->dynamic) throw CodeError(AVBD_E_ARG, "dynamic contacts are off");
+        if (kind > 1) throw CodeError(AVBD_E_ARG, "kind must be 0 or 1");
+        const auto& P = c->cs.pool[kind];
+        std::vector<DynRec> recs(c->cs.cap);
+        hip_check(hipMemcpyAsync(recs.data(), c->d.dyn + (size_t)kind * c->cs.cap, recs.size() * sizeof(DynRec),
+                                 hipMemcpyDeviceToHost, c->stream), "D2H");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        uint32_t k = 0;
+        for (uint32_t sl = 0; sl < P.active.size(); sl++) {
+            if (!P.active[sl]) continue;
+            if (out && k < cap) {
+                const DynRec& r = recs[sl];
+                avbd_contact& o = out[k];
+                std::memset(&o, 0, sizeof(o));
+                o.slot = sl;
+                for (int i = 0; i < 5; i++) o.idx[i] = r.idx[i];
+                const float4* v4[6] = {&r.n, &r.t, &r.b, &r.bary, &r.cpl, nullptr};
+                double* dst[5] = {o.n, o.t, o.b, o.barys, o.cp_local};
+                for (int a = 0; a < 5; a++) {
+                    dst[a][0] = v4[a]->x;
+                    dst[a][1] = v4[a]->y;
+                    dst[a][2] = v4[a]->z;
+                }
+                o.s = r.bary.w;
+                o.k_start = r.n.w;
+                o.k_n = r.k.x;
+                o.k_t = r.k.y;
+                o.k_b = r.k.z;
+                o.lambda_n = r.lam.x;
+                o.lambda_t = r.lam.y;
+                o.lambda_b = r.lam.z;
+                o.C_n_prev = r.cpl.w;
+                const uint32_t fl = r.kind_flags >> DYN_FLAG_SHIFT;
+                o.use_static = (fl & GF_USE_STATIC) ? 1 : 0;
+                o.unclamped = (fl & GF_UNCLAMPED) ? 1 : 0;
+            }
+            k++;
+        }
+        if (count) *count = k;
+    });
+}
+
+int avbd_download_detected(avbd_ctx* c, uint64_t* keys, uint32_t* slots, uint8_t* kinds, uint32_t cap,
+                           uint32_t* count) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!c->dynamic) throw CodeError(AVBD_E_ARG, "dynamic contacts are off");
+        const auto& E = c->cs.prev;
+        for (uint32_t i = 0; i < E.size() && i < cap; i++) {
+            if (keys) keys[i] = E[i].key;
+            if (slots) slots[i] = E[i].slot;
+            if (kinds) kinds[i] = (uint8_t)(E[i].kind == DYN_TRI_RIGID ? DET_KIND_TRI_RIGID : DET_KIND_TRI_ROD);
+        }
+        if (count) *count = (uint32_t)E.size();
+    });
+}
+
+int avbd_download_conflicts(avbd_ctx* c, uint8_t* flags) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!flags) throw CodeError(AVBD_E_ARG, "flags is null");
+        const uint32_t n = c->d.n;
+        if (c->dynamic) std::memcpy(flags, c->cs.conflicted.data(), n);
+        else std::memset(flags, 0, n);
     });
 }
 

@@ -1,0 +1,282 @@
+// contacts_host.cpp -- host side of the dynamic contacts: contact lifecycle
+// (add / update / remove against the previous round, with the reference's
+// slot-stack pools), dynamic vertex adjacency and incremental recolouring.
+// The reference runs these on its first worker thread between the narrow
+// phase and the solve; here they run on the host between the device narrow
+// phase and the device solve of the same step.
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "contacts.hpp"
+
+namespace avbd {
+
+namespace {
+constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
+
+// Eigen's unitOrthogonal for 3-vectors (double precision, as the reference).
+void unit_orthogonal(const double n[3], double t[3]) {
+    const double prec = 1e-12;
+    const bool xs = std::fabs(n[0]) <= std::fabs(n[2]) * prec, ys = std::fabs(n[1]) <= std::fabs(n[2]) * prec;
+    if (!xs || !ys) {
+        const double inv = 1.0 / std::sqrt(n[0] * n[0] + n[1] * n[1]);
+        t[0] = -n[1] * inv;
+        t[1] = n[0] * inv;
+        t[2] = 0.0;
+    } else {
+        const double inv = 1.0 / std::sqrt(n[1] * n[1] + n[2] * n[2]);
+        t[0] = 0.0;
+        t[1] = -n[2] * inv;
+        t[2] = n[1] * inv;
+    }
+}
+}  // namespace
+
+// TombstonePool::allocSlot / freeSlot (common/TombstonePool.hpp:64-110)
+uint32_t ContactSystem::Slots::alloc(uint32_t capacity) {
+    if (top >= capacity) throw std::runtime_error("dynamic contact pool has exceeded its capacity");
+    const uint32_t i = top++;
+    const uint32_t slot = i < stack.size() ? stack[i] : i;
+    if (active.size() <= slot) {
+        active.resize(slot + 1, 0);
+        gen.resize(slot + 1, 0);
+        idx.resize(slot + 1);
+    }
+    active[slot] = 1;
+    gen[slot]++;
+    count++;
+    return slot;
+}
+
+void ContactSystem::Slots::free(uint32_t slot) {
+    if (slot >= active.size() || !active[slot]) throw std::runtime_error("freeSlot called on inactive slot");
+    active[slot] = 0;
+    count--;
+    const uint32_t i = --top;
+    while (stack.size() <= i) stack.push_back((uint32_t)stack.size());
+    stack[i] = slot;
+}
+
+void ContactSystem::init(uint32_t n_particles, uint32_t capacity, double collision_margin) {
+    n = n_particles;
+    cap = capacity;
+    margin = collision_margin;
+    pool[0] = Slots();
+    pool[1] = Slots();
+    prev.clear();
+    last_dirty_round.assign(n, SENTINEL);
+    candidate.assign(n, 0);
+    dirty.clear();
+    next_dirty.clear();
+    touched_this_frame.clear();
+    touched_conflicted.clear();
+    conflicted.assign(n, 0);
+    dyn_off.assign(n + 1, 0);
+    dyn_ent.clear();
+    nb_off.assign(n + 1, 0);
+    nb.clear();
+    ops.clear();
+    active.clear();
+}
+
+// ColorList::markParticleDirty (common/ColorList.hpp:366-377)
+void ContactSystem::mark_dirty(uint32_t p) {
+    if (last_dirty_round[p] != 0) {
+        last_dirty_round[p] = 0;
+        touched_this_frame.push_back(p);
+        dirty.push_back(p);
+    }
+}
+
+bool ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim>& prims,
+                            const std::vector<double>& mass, double dt, ColorPlan& plan) {
+    ops.clear();
+    // _handleDetectedCollisions (collision/CollisionDetector.cpp:272-327): key order
+    std::sort(det.begin(), det.end(), [](const DetRec& a, const DetRec& b) { return a.key < b.key; });
+    std::vector<Entry> cur(det.size());
+    auto add = [&](size_t i) {
+        const DetRec& r = det[i];
+        const uint32_t kind = r.kind == DET_KIND_TRI_RIGID ? DYN_TRI_RIGID : DYN_TRI_ROD;
+        const HostPrim& tri = prims[r.prim_a];
+        const HostPrim& other = prims[r.prim_b];
+        std::array<uint32_t, 5> id{tri.idx[0], tri.idx[1], tri.idx[2], other.idx[0],
+                                   kind == DYN_TRI_ROD ? other.idx[1] : 0u};
+        const uint32_t np = kind == DYN_TRI_RIGID ? 4u : 5u;
+        double mmin = mass[id[0]];
+        for (uint32_t k = 1; k < np; k++) mmin = std::min(mmin, mass[id[k]]);
+        const uint32_t slot = pool[kind].alloc(cap);
+        pool[kind].idx[slot] = id;
+        DynOp o{};
+        o.rec = kind * cap + slot;
+        o.op = DYN_OP_ADD;
+        o.kind = kind;
+        for (int k = 0; k < 5; k++) o.idx[k] = id[k];
+        o.k_start = (float)(mmin / (dt * dt));
+        double t[3], b[3];
+        unit_orthogonal(r.n, t);
+        b[0] = r.n[1] * t[2] - r.n[2] * t[1];
+        b[1] = r.n[2] * t[0] - r.n[0] * t[2];
+        b[2] = r.n[0] * t[1] - r.n[1] * t[0];
+        for (int k = 0; k < 3; k++) {
+            o.n[k] = (float)r.n[k];
+            o.t[k] = (float)t[k];
+            o.b[k] = (float)b[k];
+            o.bary[k] = (float)r.bary[k];
+            o.cpl[k] = (float)r.cpl[k];
+        }
+        o.s = (float)r.s;
+        ops.push_back(o);
+        cur[i] = Entry{r.key, kind, slot, 1u, 1u};
+        for (uint32_t k = 0; k < np; k++) mark_dirty(id[k]);
+    };
+    auto update = [&](size_t i, uint32_t slot) {
+        const DetRec& r = det[i];
+        const uint32_t kind = r.kind == DET_KIND_TRI_RIGID ? DYN_TRI_RIGID : DYN_TRI_ROD;
+        DynOp o{};
+        o.rec = kind * cap + slot;
+        o.op = DYN_OP_UPDATE;
+        o.kind = kind;
+        for (int k = 0; k < 3; k++) {
+            o.n[k] = (float)r.n[k];
+            o.bary[k] = (float)r.bary[k];
+            o.cpl[k] = (float)r.cpl[k];
+        }
+        o.s = (float)r.s;
+        ops.push_back(o);
+        cur[i] = Entry{r.key, kind, slot, 1u, 1u};
+    };
+    auto remove = [&](const Entry& e) { pool[e.kind].free(e.slot); };
+    size_t ci = 0, pi = 0;
+    while (ci < det.size() && pi < prev.size()) {
+        if (det[ci].key < prev[pi].key) {
+            add(ci++);
+        } else if (det[ci].key > prev[pi].key) {
+            remove(prev[pi++]);
+        } else {
+            // primitives are never removed, so both generations always match
+            update(ci++, prev[pi++].slot);
+        }
+    }
+    for (; ci < det.size(); ci++) add(ci);
+    for (; pi < prev.size(); pi++) remove(prev[pi]);
+    prev.swap(cur);
+
+    // active records and the dynamic adjacency (DynamicParticleAdjacency::buildAdjacency,
+    // common/DynamicParticleAdjacency.hpp:33-118): pool 0 then pool 1, ascending slot
+    active.clear();
+    std::vector<uint32_t> val(n + 1, 0), deg(n + 1, 0);
+    for (uint32_t kind = 0; kind < 2; kind++) {
+        const uint32_t np = kind == DYN_TRI_RIGID ? 4u : 5u;
+        const Slots& P = pool[kind];
+        for (uint32_t sl = 0; sl < P.active.size(); sl++) {
+            if (!P.active[sl]) continue;
+            active.push_back(kind * cap + sl);
+            for (uint32_t k = 0; k < np; k++) {
+                val[P.idx[sl][k] + 1]++;
+                deg[P.idx[sl][k] + 1] += np - 1;
+            }
+        }
+    }
+    for (uint32_t v = 0; v < n; v++) {
+        val[v + 1] += val[v];
+        deg[v + 1] += deg[v];
+    }
+    dyn_off = val;
+    nb_off = deg;
+    dyn_ent.resize(val[n]);
+    nb.resize(deg[n]);
+    for (uint32_t kind = 0; kind < 2; kind++) {
+        const uint32_t np = kind == DYN_TRI_RIGID ? 4u : 5u;
+        const Slots& P = pool[kind];
+        for (uint32_t sl = 0; sl < P.active.size(); sl++) {
+            if (!P.active[sl]) continue;
+            const uint32_t rec = kind * cap + sl;
+            for (uint32_t k = 0; k < np; k++) {
+                const uint32_t p = P.idx[sl][k];
+                dyn_ent[val[p]++] = (rec << 3) | k;
+                for (uint32_t k2 = 0; k2 < np; k2++)
+                    if (k2 != k) nb[deg[p]++] = P.idx[sl][k2];
+            }
+        }
+    }
+    return recolour(plan);
+}
+
+// ColorList::incrementalRecoloring + finalizeConflicts (common/ColorList.hpp:381-522).
+// Each round computes every dirty particle's candidate from the colours at
+// the start of the round and then applies them, so the result does not
+// depend on the order of the dirty list.
+bool ContactSystem::recolour(ColorPlan& plan) {
+    std::vector<uint32_t>& color = plan.color;
+    const std::vector<uint8_t> conflicted_before = conflicted;
+    const uint32_t colors_before = plan.num_colors;
+    bool changed = false;
+    auto each_nb = [&](uint32_t p, auto&& f) {
+        for (uint32_t e = plan.static_off[p]; e < plan.static_off[p + 1]; e++) f(plan.static_nb[e]);
+        for (uint32_t e = nb_off[p]; e < nb_off[p + 1]; e++) f(nb[e]);
+    };
+    for (uint32_t p : touched_conflicted) conflicted[p] = 0;
+    touched_conflicted.clear();
+    uint32_t iter = 0;
+    while (!dirty.empty() && iter < 6) {
+        for (uint32_t p : dirty) {
+            uint64_t bits[4] = {0, 0, 0, 0};
+            bool must = false;
+            each_nb(p, [&](uint32_t q) {
+                const uint32_t c = color[q];
+                bits[c >> 6] |= 1ull << (c & 63);
+                if (c == color[p] && q < p) must = true;
+            });
+            if (must) {
+                uint32_t c = 256;
+                for (uint32_t w = 0; w < 4 && c == 256; w++)
+                    if (~bits[w]) c = w * 64 + (uint32_t)__builtin_ctzll(~bits[w]);
+                if (c == 256) throw std::runtime_error("Ran out of colors!");
+                candidate[p] = c;
+                if (c >= plan.num_colors) plan.num_colors++;
+            } else {
+                candidate[p] = color[p];
+            }
+        }
+        next_dirty.clear();
+        for (uint32_t p : dirty) {
+            if (candidate[p] == color[p]) continue;
+            color[p] = candidate[p];
+            changed = true;
+            each_nb(p, [&](uint32_t q) {
+                if (last_dirty_round[q] != iter + 1) {
+                    last_dirty_round[q] = iter + 1;
+                    touched_this_frame.push_back(q);
+                    next_dirty.push_back(q);
+                }
+            });
+        }
+        dirty.swap(next_dirty);
+        iter++;
+    }
+    for (uint32_t p : dirty) {
+        each_nb(p, [&](uint32_t q) {
+            if (color[q] == color[p]) {
+                if (!conflicted[p]) {
+                    conflicted[p] = 1;
+                    touched_conflicted.push_back(p);
+                }
+                if (!conflicted[q]) {
+                    conflicted[q] = 1;
+                    touched_conflicted.push_back(q);
+                }
+            }
+        });
+    }
+    for (uint32_t p : touched_this_frame) last_dirty_round[p] = SENTINEL;
+    touched_this_frame.clear();
+    for (uint32_t p : dirty) last_dirty_round[p] = 0;
+    if (changed || plan.num_colors != colors_before || conflicted != conflicted_before) {
+        relayout_plan(plan, conflicted);
+        return true;
+    }
+    return false;
+}
+
+}  // namespace avbd

@@ -150,13 +150,14 @@ void build_color_plan(const HostModel& m, ColorPlan& plan) {
     plan.color.assign(n, 0);
     std::vector<uint32_t> scene_colors(n_scenes, 0);
     std::vector<std::string> errors(n_scenes);
+    std::vector<SceneGraph> graphs(n_scenes);
     const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> pool;
     for (uint32_t w = 0; w < hw; w++) {
         pool.emplace_back([&, w] {
             for (uint32_t s = w; s < n_scenes; s += hw) {
                 try {
-                    SceneGraph g;
+                    SceneGraph& g = graphs[s];
                     const uint32_t lo = scene_offsets[s], hi = scene_offsets[s + 1];
                     scene_adjacency(lo, hi, nh, scene_tets[s], cos, scene_rods[s], g);
                     scene_colors[s] = dsatur_csr(hi - lo, g.off.data(), g.nb.data(), plan.color.data() + lo);
@@ -171,6 +172,18 @@ void build_color_plan(const HostModel& m, ColorPlan& plan) {
         if (!errors[s].empty()) throw std::runtime_error(errors[s]);
     plan.num_colors = 0;
     for (uint32_t s = 0; s < n_scenes; s++) plan.num_colors = std::max(plan.num_colors, scene_colors[s]);
+    // global static adjacency (kept for incremental recolouring)
+    plan.static_off.assign(n + 1, 0);
+    plan.static_nb.clear();
+    for (uint32_t s = 0; s < n_scenes; s++) {
+        const uint32_t lo = scene_offsets[s], hi = scene_offsets[s + 1];
+        for (uint32_t v = lo; v < hi; v++) {
+            const SceneGraph& g = graphs[s];
+            for (uint32_t e = g.off[v - lo]; e < g.off[v - lo + 1]; e++) plan.static_nb.push_back(g.nb[e] + lo);
+            plan.static_off[v + 1] = (uint32_t)plan.static_nb.size();
+        }
+    }
+    graphs.clear();
 
     // per-vertex incidences in pool order (ascending energy index, then local)
     std::vector<std::pair<uint32_t, uint32_t>> pr;
@@ -226,6 +239,9 @@ void build_color_plan(const HostModel& m, ColorPlan& plan) {
     plan.inc.clear();
     plan.oitems.clear();
     plan.oinc.clear();
+    plan.item_kind.assign(n, 0);
+    plan.item_of.assign(n, VItem{});
+    plan.oitem_of.assign(n, OItem{});
     plan.item_offsets.assign(plan.num_colors + 1, 0);
     plan.oitem_offsets.assign(plan.num_colors + 1, 0);
     for (uint32_t c = 0; c < plan.num_colors; c++) {
@@ -241,6 +257,8 @@ void build_color_plan(const HostModel& m, ColorPlan& plan) {
                 it.ground = ground_of[v];
                 plan.inc.insert(plan.inc.end(), inc_all.begin() + inc_off[v], inc_all.begin() + inc_off[v + 1]);
                 plan.items.push_back(it);
+                plan.item_kind[v] = 1;
+                plan.item_of[v] = it;
             } else {
                 OItem it{};
                 it.v = v;
@@ -254,6 +272,8 @@ void build_color_plan(const HostModel& m, ColorPlan& plan) {
                 plan.oinc.insert(plan.oinc.end(), rg_all.begin() + rg_off[v], rg_all.begin() + rg_off[v + 1]);
                 plan.oinc.insert(plan.oinc.end(), jt_all.begin() + jt_off[v], jt_all.begin() + jt_off[v + 1]);
                 plan.oitems.push_back(it);
+                plan.item_kind[v] = 2;
+                plan.oitem_of[v] = it;
             }
         }
     }
@@ -261,6 +281,48 @@ void build_color_plan(const HostModel& m, ColorPlan& plan) {
     plan.oitem_offsets[plan.num_colors] = (uint32_t)plan.oitems.size();
     plan.conflict_offsets.assign(plan.num_colors + 1, 0);
     plan.conflict_list.clear();
+}
+
+// ColorList::rebuildWorkList (common/ColorList.hpp:345-363) over the kept
+// item templates, and the per-colour conflict lists of finalizeConflicts
+// (:476-522).  Conflicted particles without a work item (fixed ones) are
+// still committed, as the reference commits every conflicted particle.
+void relayout_plan(ColorPlan& plan, const std::vector<uint8_t>& conflicted) {
+    const uint32_t n = (uint32_t)plan.color.size();
+    const uint32_t K = plan.num_colors;
+    std::vector<uint32_t> cnt(K + 1, 0), ocnt(K + 1, 0), ccnt(K + 1, 0);
+    for (uint32_t v = 0; v < n; v++) {
+        const uint32_t c = plan.color[v];
+        if (c >= K) throw std::runtime_error("colour out of range after recolouring");
+        if (plan.item_kind[v] == 1) cnt[c + 1]++;
+        else if (plan.item_kind[v] == 2) ocnt[c + 1]++;
+        if (conflicted[v]) ccnt[c + 1]++;
+    }
+    for (uint32_t c = 0; c < K; c++) {
+        cnt[c + 1] += cnt[c];
+        ocnt[c + 1] += ocnt[c];
+        ccnt[c + 1] += ccnt[c];
+    }
+    plan.item_offsets = cnt;
+    plan.oitem_offsets = ocnt;
+    plan.conflict_offsets = ccnt;
+    plan.items.resize(cnt[K]);
+    plan.oitems.resize(ocnt[K]);
+    plan.conflict_list.resize(ccnt[K]);
+    for (uint32_t v = 0; v < n; v++) {
+        const uint32_t c = plan.color[v];
+        const bool cf = conflicted[v] != 0;
+        if (plan.item_kind[v] == 1) {
+            VItem it = plan.item_of[v];
+            it.inc_count = (it.inc_count & ~VITEM_CONFLICT) | (cf ? VITEM_CONFLICT : 0u);
+            plan.items[cnt[c]++] = it;
+        } else if (plan.item_kind[v] == 2) {
+            OItem it = plan.oitem_of[v];
+            it.n_cos = (it.n_cos & ~VITEM_CONFLICT) | (cf ? VITEM_CONFLICT : 0u);
+            plan.oitems[ocnt[c]++] = it;
+        }
+        if (cf) plan.conflict_list[ccnt[c]++] = v;
+    }
 }
 
 }  // namespace avbd

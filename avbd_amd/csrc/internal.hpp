@@ -119,6 +119,50 @@ struct alignas(16) OItem {
 };
 constexpr uint32_t PREC_JOINT = 0x80000000u;   // per-rotation record list: joint flag
 
+// Dynamic triangle-rigid / triangle-rod AL contact (CollisionConstraintEnergyInfo
+// + TriangleRigidCollisionEnergyInfo / TriangleRodCollisionEnergyInfo,
+// energy/collision/CollisionConstraintEnergyPool.hpp:9-30,
+// energy/collision/TriangleRigidCollisionEnergyPool.hpp:8-14,
+// energy/collision/TriangleRodCollisionEnergyPool.hpp:8-15).  idx: the
+// triangle's vertices, then the rigid body (kind 0) or the two rod nodes
+// (kind 1).  Record index = kind * capacity + pool slot.
+struct alignas(16) DynRec {
+    float4 n;      // normal, w = k_start
+    float4 t;      // tangent, w = mu_s
+    float4 b;      // binormal, w = mu_k
+    float4 bary;   // barycentrics on the triangle, w = s (rod parameter)
+    float4 cpl;    // contact point in the body / interpolated rod frame, w = C_n_prev
+    float4 k;      // k_n, k_t, k_b, unused
+    float4 lam;    // lambda_n, lambda_t, lambda_b, unused
+    uint32_t idx[5];
+    uint32_t kind_flags;   // kind in bits 0-7, GF_USE_STATIC / GF_UNCLAMPED in bits 8-9
+    uint32_t pad[2];
+};
+static_assert(sizeof(DynRec) == 144, "DynRec must be 144 bytes");
+constexpr uint32_t DYN_TRI_RIGID = 0u, DYN_TRI_ROD = 1u;
+constexpr uint32_t DYN_FLAG_SHIFT = 8u;
+constexpr float kContactMuS = 0.4f, kContactMuK = 0.2f;   // _addCollision (collision/CollisionDetector.cpp:365-366)
+
+// Host -> device contact update of one detection round: a new record (ADD)
+// or new normal / contact point of a persisting one (UPDATE, which rotates
+// the friction frame on the device, CollisionConstraintEnergySolver::updateCollisionFrame).
+struct alignas(16) DynOp {
+    uint32_t rec, op, kind, pad;
+    uint32_t idx[5];
+    float k_start;
+    float n[3], t[3], b[3], bary[3], s, cpl[3];
+};
+constexpr uint32_t DYN_OP_ADD = 0u, DYN_OP_UPDATE = 1u;
+
+// Narrow-phase output (device -> host): DetectedCollision fields of one pair.
+struct alignas(8) DetRec {
+    uint64_t key;
+    uint32_t kind;       // DetectedCollisionType: 2 triangle-rigid, 3 triangle-rod
+    uint32_t prim_a, prim_b, pad;
+    double n[3], bary[3], cpl[3], s;
+};
+static_assert(sizeof(DetRec) == 104, "DetRec must be 104 bytes");
+
 // ---------------------------------------------------------------------------
 // Kernel launch wrappers (defined in vbd_kernels.hip)
 // ---------------------------------------------------------------------------
@@ -163,6 +207,13 @@ struct DevState {
     uint32_t* prec_off = nullptr;  // per rotation slot: rigid-ground / joint records
     uint32_t* prec = nullptr;
     uint32_t n_rot = 0, n_cos = 0, n_rground = 0, n_joints = 0, n_oitems = 0;
+
+    // dynamic contacts (null unless enabled): per-particle CSR of incident
+    // records, entries (record << 3) | local, in the reference's dynamic
+    // adjacency order (triangle-rigid pool, then triangle-rod, ascending slot)
+    const uint32_t* dyn_off = nullptr;
+    const uint32_t* dyn_ent = nullptr;
+    DynRec* dyn = nullptr;
 };
 
 struct LaunchCfg {
@@ -236,6 +287,12 @@ void launch_warm_ip(hipStream_t s, const DevState& d, const WarmIO& io, bool dam
 void launch_warm_choose(hipStream_t s, const DevState& d, const WarmIO& io);
 void launch_warm_target(hipStream_t s, const DevState& d, const WarmIO& io, float inv_dt2g);
 void launch_velocity(hipStream_t s, const DevState& d, float dt);
+// dynamic contacts: apply a detection round's ADD / UPDATE ops; the
+// start-of-step and per-iteration AL updates of the active records
+void launch_dyn_apply(hipStream_t s, const DevState& d, const DynOp* ops, uint32_t n_ops);
+void launch_dyn_step_begin(hipStream_t s, const DevState& d, const uint32_t* active, uint32_t n_active);
+void launch_dyn_dual(hipStream_t s, const DevState& d, const uint32_t* active, uint32_t n_active,
+                     const uint32_t* done = nullptr);
 // plain streaming copy of n_float4 16-byte words (HBM counter calibration)
 void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n_float4);
 // residual/IP pass: writes 3 partial sums per block into d.reduce; returns block count
@@ -303,11 +360,24 @@ struct ColorPlan {
     std::vector<uint32_t> prec;
     std::vector<uint32_t> conflict_offsets;   // per colour
     std::vector<uint32_t> conflict_list;
+    // per particle: its work item as laid out at build time (kind 0 none /
+    // fixed, 1 positional, 2 oriented), so that a recolouring can regroup the
+    // items without moving the incidence lists; and the static vertex
+    // adjacency over all particles (deduplicated)
+    std::vector<uint8_t> item_kind;
+    std::vector<VItem> item_of;
+    std::vector<OItem> oitem_of;
+    std::vector<uint32_t> static_off, static_nb;
 };
 
 // Build adjacency and colour every scene with DSATUR; then lay out the
 // per-colour work items.  Throws std::runtime_error on failure.
 void build_color_plan(const HostModel& m, ColorPlan& plan);
+
+// Regroup the work items by plan.color (colour-major, ascending particle)
+// with the conflict flags of `conflicted`, and rebuild the per-colour conflict
+// lists; plan.num_colors must cover every colour.
+void relayout_plan(ColorPlan& plan, const std::vector<uint8_t>& conflicted);
 
 // DSATUR on one CSR graph (ColorList::greedyColor2, common/ColorList.hpp:149-241).
 uint32_t dsatur_csr(uint32_t n, const uint32_t* p_offsets, const uint32_t* p_neighbors, uint32_t* color);

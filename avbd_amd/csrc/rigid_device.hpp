@@ -6,6 +6,7 @@
 // kernel can inline every phase).
 #pragma once
 
+#include "dyn_device.hpp"
 #include "internal.hpp"
 #include "rot_math.hpp"
 
@@ -364,6 +365,7 @@ __device__ __forceinline__ void rotation_velocity(const DevState& d, uint32_t r,
 // Gradient and Hessian of every static energy on oriented particle it.v, in
 // the reference's StaticEnergyType order: Cosserat, ground (positional block),
 // rigid-ground, joint (simulation/VBDSolver.hpp:297-397).
+template <bool DYN = false>
 __device__ __forceinline__ void accumulate6(const DevState& d, const OItem& it, float3 pos, float4 q, Vec6& G,
                                             Sym6& H) {
     const uint32_t ncos = it.n_cos & ~VITEM_CONFLICT;
@@ -401,9 +403,20 @@ __device__ __forceinline__ void accumulate6(const DevState& d, const OItem& it, 
     }
     for (uint32_t e = 0; e < it.n_rg; e++, k++) rg_accumulate(d.rground[d.oinc[k]], pos, q, G, H);
     for (uint32_t e = 0; e < it.n_joint; e++, k++) joint_accumulate(d.joints[d.oinc[k]], pos, q, G, H);
+    // dynamic contacts after the static energies (simulation/VBDSolver.hpp:400-438)
+    if constexpr (DYN) {
+        const dyn::View view = dyn::view_of(d);
+        const uint32_t e1 = d.dyn_off[it.v + 1];
+        for (uint32_t e = d.dyn_off[it.v]; e < e1; e++) {
+            const uint32_t code = d.dyn_ent[e];
+            const DynRec c = d.dyn[code >> 3];
+            dyn::accumulate6(view, c, code & 7u, G, H);
+        }
+    }
 }
 
 // _solveParticle<6> (simulation/VBDSolver.hpp:450-486): one thread per oriented vertex.
+template <bool DYN = false>
 __device__ __forceinline__ void solve_oitem(const DevState& d, const OItem& it, float inv_dt2) {
     const float4 xv = d.x[it.v];
     const float3 pos = xyz(xv);
@@ -412,7 +425,7 @@ __device__ __forceinline__ void solve_oitem(const DevState& d, const OItem& it, 
     Sym6 H;
     vec6_zero(G);
     sym6_zero(H);
-    accumulate6(d, it, pos, q, G, H);
+    accumulate6<DYN>(d, it, pos, q, G, H);
     const float md = xv.w * inv_dt2;
     const float3 I = xyz(d.inertia[it.rot]);
     const float3 yv = xyz(d.y[it.v]);

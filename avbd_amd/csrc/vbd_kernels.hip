@@ -19,6 +19,7 @@
 // Oriented (6-DOF) vertices of a colour are solved in the same phase, one
 // thread each (rigid_device.hpp).
 #include "internal.hpp"
+#include "dyn_device.hpp"
 #include "rigid_device.hpp"
 
 namespace avbd {
@@ -169,6 +170,29 @@ __device__ __forceinline__ void ground_accumulate(const GroundRec& c, float3 p, 
     h.a22 += c.k_b;
 }
 
+// Dynamic contacts incident to one positional particle (the reference
+// accumulates them after the static energies, simulation/VBDSolver.hpp:400-438).
+struct DynArgs {
+    const uint32_t* off;   // null: no dynamic contacts
+    const uint32_t* ent;
+    const DynRec* rec;
+    const float4* q;
+    const int32_t* rot_of;
+};
+__host__ __device__ __forceinline__ DynArgs dyn_args(const DevState& d) {
+    return DynArgs{d.dyn_off, d.dyn_ent, d.dyn, d.q, d.rot_of};
+}
+__device__ __forceinline__ void dyn_add3(const DynArgs& a, const float4* x, uint32_t v, float3& g, Sym3& h) {
+    if (!a.off) return;
+    const dyn::View view{x, a.q, a.rot_of};
+    const uint32_t e1 = a.off[v + 1];
+    for (uint32_t e = a.off[v]; e < e1; e++) {
+        const uint32_t code = a.ent[e];
+        const DynRec c = a.rec[code >> 3];
+        dyn::accumulate3(view, c, code & 7u, g, h);
+    }
+}
+
 // Solve (m/dt^2 I + H) dx = b for symmetric positive definite H by LDL^T.
 __device__ __forceinline__ float3 ldlt3(const Sym3& A, float3 b) {
     const float d0 = A.a00;
@@ -205,12 +229,12 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 
 // One vertex block solve (_solveParticle<3>, simulation/VBDSolver.hpp:285-449,487-510)
 // by the L lanes of a group; every lane of the group must call it.
-template <int L, bool DAMP>
+template <int L, bool DAMP, bool DYN = false>
 __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const uint32_t* __restrict__ inc,
                                            const TetRec* __restrict__ irec, const TetDamp* __restrict__ damp,
                                            float4* x, const float4* __restrict__ y,
                                            const GroundRec* __restrict__ ground, float4* x_out, float4* x_buf,
-                                           float inv_dt, float inv_dt2) {
+                                           float inv_dt, float inv_dt2, const DynArgs& da) {
     const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
     // the group leader's own state is loaded first, so that its latency
     // overlaps the incidence gathers instead of following the reduction
@@ -237,6 +261,7 @@ __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const
     if (!lead) return;
     const float3 p = f3(xv);
     if (it.ground >= 0) ground_accumulate(gr, p, g, h);
+    if constexpr (DYN) dyn_add3(da, x, it.v, g, h);
     const float md = xv.w * inv_dt2;   // m / dt^2
     h.a00 += md;
     h.a11 += md;
@@ -248,7 +273,7 @@ __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const
     else x_out[it.v] = nx;
 }
 
-template <int L, bool DAMP>
+template <int L, bool DAMP, bool DYN>
 __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restrict__ items, uint32_t n_items,
                                                            const uint32_t* __restrict__ inc,
                                                            const TetRec* __restrict__ irec,
@@ -256,13 +281,13 @@ __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restri
                                                            const float4* __restrict__ y,
                                                            const GroundRec* __restrict__ ground, float4* x_out,
                                                            float4* x_buf, float inv_dt, float inv_dt2,
-                                                           const uint32_t* __restrict__ done) {
+                                                           const uint32_t* __restrict__ done, DynArgs da) {
     if (done && *done) return;   // residual target reached earlier in this step
     const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L;
     const uint32_t lane = tid % L;
     if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)
-    solve_item<L, DAMP>(items[item], lane, inc, irec, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2);
+    solve_item<L, DAMP, DYN>(items[item], lane, inc, irec, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2, da);
 }
 
 // E_prev = 0.5 (F_prev^T F_prev - I) from previous positions
@@ -378,12 +403,10 @@ __global__ void step_begin_kernel(DevState d, float dt, float g) {
     if (p < d.n) step_begin_one(d, p, dt, g);
 }
 
-// _commitConflictedInColor (VBDSolver.hpp:108-119) for one buffered particle.
-__device__ __forceinline__ void commit_one(const DevState& d, uint32_t p) {
-    d.x[p] = d.x_buf[p];
-    const int32_t r = d.n_rot ? d.rot_of[p] : -1;
-    if (r >= 0) d.q[r] = d.q_buf[r];
-}
+// _commitConflictedInColor (VBDSolver.hpp:108-119) for one buffered particle:
+// positions only, as in the reference -- the buffered rotation of a
+// conflicted oriented particle is not committed.
+__device__ __forceinline__ void commit_one(const DevState& d, uint32_t p) { d.x[p] = d.x_buf[p]; }
 
 __global__ void commit_kernel(DevState d, const uint32_t* __restrict__ list, uint32_t count,
                               const uint32_t* __restrict__ done) {
@@ -461,12 +484,63 @@ __global__ void velocity_kernel(DevState d, float inv_dt) {
     if (p < d.n) velocity_one(d, p, inv_dt);
 }
 
+// ---- dynamic contacts -------------------------------------------------------
+// One detection round's ADD / UPDATE ops (CollisionConstraintEnergyPool::addEnergy,
+// energy/collision/CollisionConstraintEnergyPool.hpp:48-69, and _updateCollision,
+// collision/CollisionDetector.cpp:454-501).
+__global__ void dyn_apply_kernel(DynRec* __restrict__ recs, const DynOp* __restrict__ ops, uint32_t n_ops) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_ops) return;
+    const DynOp o = ops[k];
+    DynRec& c = recs[o.rec];
+    if (o.op == DYN_OP_ADD) {
+        DynRec r;
+        r.n = make_float4(o.n[0], o.n[1], o.n[2], o.k_start);
+        r.t = make_float4(o.t[0], o.t[1], o.t[2], kContactMuS);
+        r.b = make_float4(o.b[0], o.b[1], o.b[2], kContactMuK);
+        r.bary = make_float4(o.bary[0], o.bary[1], o.bary[2], o.s);
+        r.cpl = make_float4(o.cpl[0], o.cpl[1], o.cpl[2], 0.f);
+        r.k = make_float4(o.k_start, o.k_start, o.k_start, 0.f);
+        r.lam = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < 5; i++) r.idx[i] = o.idx[i];
+        r.kind_flags = o.kind;
+        r.pad[0] = r.pad[1] = 0u;
+        c = r;
+    } else {
+        DynRec r = c;
+        if (dyn::kind_of(r) == DYN_TRI_ROD) r.bary.w = o.s;
+        r.cpl = make_float4(o.cpl[0], o.cpl[1], o.cpl[2], r.cpl.w);
+        r.bary = make_float4(o.bary[0], o.bary[1], o.bary[2], r.bary.w);
+        dyn::update_frame(r, make_float3(o.n[0], o.n[1], o.n[2]));
+        c = r;
+    }
+}
+
+__global__ void dyn_step_begin_kernel(DevState d, const uint32_t* __restrict__ active, uint32_t n_active) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active) return;
+    DynRec c = d.dyn[active[k]];
+    dyn::update_after_timestep(dyn::view_of(d), c);
+    d.dyn[active[k]] = c;
+}
+
+__global__ void dyn_dual_kernel(DevState d, const uint32_t* __restrict__ active, uint32_t n_active,
+                                const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active) return;
+    DynRec c = d.dyn[active[k]];
+    dyn::update_after_iteration(dyn::view_of(d), c);
+    d.dyn[active[k]] = c;
+}
+
 // Oriented vertices of one colour, one thread each.
+template <bool DYN>
 __global__ __launch_bounds__(256) void solve_ocolor_kernel(DevState d, uint32_t begin, uint32_t count,
                                                            float inv_dt2, const uint32_t* __restrict__ done) {
     if (done && *done) return;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) rigid::solve_oitem(d, d.oitems[begin + k], inv_dt2);
+    if (k < count) rigid::solve_oitem<DYN>(d, d.oitems[begin + k], inv_dt2);
 }
 
 // ---- force residual (residual-targeted iteration) --------------------------
@@ -914,7 +988,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
             const uint32_t lo = min(cnt, lb * per), hi = min(cnt, lo + per);
             for (uint32_t k = lo + grp; k < hi; k += groups)
                 solve_item<L, DAMP>(d.items[beg + k], lane, d.inc, d.irec, d.damp, d.x, d.y, d.ground, d.x, d.x_buf,
-                                    inv_dt, inv_dt2);
+                                    inv_dt, inv_dt2, dyn_args(d));
             if (d.n_oitems) {
                 const uint32_t ob = a.ocolor_offsets[c], ocnt = a.ocolor_offsets[c + 1] - ob;
                 const uint32_t oper = (ocnt + G - 1) / G;
@@ -973,7 +1047,7 @@ __global__ __launch_bounds__(1024) void stats_items_kernel(const VItem* __restri
                                                            const float4* __restrict__ y,
                                                            const GroundRec* __restrict__ ground,
                                                            double* __restrict__ out, float inv_dt, float inv_dt2,
-                                                           float g_acc) {
+                                                           float g_acc, DynArgs da) {
     __shared__ double red[3][1024 / L];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L;
@@ -993,6 +1067,7 @@ __global__ __launch_bounds__(1024) void stats_items_kernel(const VItem* __restri
         if (lane == 0) {
             const float4 xv = x[it.v], yv = y[it.v];
             if (it.ground >= 0) ground_accumulate(ground[it.ground], f3(xv), g, h);
+            dyn_add3(da, x, it.v, g, h);
             const double md = (double)xv.w * (double)inv_dt2;
             const double dx = (double)xv.x - yv.x, dy = (double)xv.y - yv.y, dz = (double)xv.z - yv.z;
             const double rx = md * dx + g.x, ry = md * dy + g.y, rz = md * dz + g.z;
@@ -1060,7 +1135,8 @@ __global__ __launch_bounds__(256) void stats_oitems_kernel(DevState d, double* _
         rot::Sym6 H;
         rot::vec6_zero(G);
         rot::sym6_zero(H);
-        rigid::accumulate6(d, it, rigid::xyz(xv), d.q[it.rot], G, H);
+        if (d.dyn_off) rigid::accumulate6<true>(d, it, rigid::xyz(xv), d.q[it.rot], G, H);
+        else rigid::accumulate6<false>(d, it, rigid::xyz(xv), d.q[it.rot], G, H);
         const float4 yv = d.y[it.v];
         const double md = (double)xv.w * (double)inv_dt2;
         const double dx = (double)xv.x - yv.x, dy = (double)xv.y - yv.y, dz = (double)xv.z - yv.z;
@@ -1088,9 +1164,14 @@ template <int L, bool DAMP>
 void solve_dispatch(hipStream_t s, const DevState& d, uint32_t block, uint32_t begin, uint32_t count, float dt,
                     float4* x_out, const uint32_t* done) {
     const uint32_t threads = count * L;
-    hipLaunchKernelGGL((solve_color_kernel<L, DAMP>), dim3(blocks_for(threads, block)), dim3(block), 0, s,
-                       d.items + begin, count, d.inc, d.irec, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
-                       1.0f / dt, 1.0f / (dt * dt), done);
+    if (d.dyn_off)
+        hipLaunchKernelGGL((solve_color_kernel<L, DAMP, true>), dim3(blocks_for(threads, block)), dim3(block), 0, s,
+                           d.items + begin, count, d.inc, d.irec, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
+                           1.0f / dt, 1.0f / (dt * dt), done, dyn_args(d));
+    else
+        hipLaunchKernelGGL((solve_color_kernel<L, DAMP, false>), dim3(blocks_for(threads, block)), dim3(block), 0,
+                           s, d.items + begin, count, d.inc, d.irec, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
+                           1.0f / dt, 1.0f / (dt * dt), done, dyn_args(d));
 }
 
 template <int L, bool DAMP>
@@ -1107,7 +1188,7 @@ uint32_t stats_dispatch(hipStream_t s, const DevState& d, uint32_t block, float 
     const uint32_t nb = blocks_for(threads, block);
     if (nb == 0) return 0;
     hipLaunchKernelGGL((stats_items_kernel<L, DAMP>), dim3(nb), dim3(block), 0, s, d.items, d.n_items, d.inc,
-                       d.tets, d.damp, d.x, d.y, d.ground, d.reduce, 1.0f / dt, 1.0f / (dt * dt), g);
+                       d.tets, d.damp, d.x, d.y, d.ground, d.reduce, 1.0f / dt, 1.0f / (dt * dt), g, dyn_args(d));
     return nb;
 }
 
@@ -1127,8 +1208,12 @@ void launch_step_begin(hipStream_t s, const DevState& d, float dt, float g) {
 void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t begin, uint32_t count, float dt,
                          const uint32_t* done) {
     if (count == 0) return;
-    hipLaunchKernelGGL(solve_ocolor_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, begin, count,
-                       1.0f / (dt * dt), done);
+    if (d.dyn_off)
+        hipLaunchKernelGGL(solve_ocolor_kernel<true>, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, begin, count,
+                           1.0f / (dt * dt), done);
+    else
+        hipLaunchKernelGGL(solve_ocolor_kernel<false>, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, begin,
+                           count, 1.0f / (dt * dt), done);
 }
 
 void launch_residual_reset(hipStream_t s, ResCtl* rc) {
@@ -1155,6 +1240,23 @@ void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cf
 
 void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters) {
     hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(64), 0, s, rc, max_iters);
+}
+
+void launch_dyn_apply(hipStream_t s, const DevState& d, const DynOp* ops, uint32_t n_ops) {
+    if (n_ops == 0) return;
+    hipLaunchKernelGGL(dyn_apply_kernel, dim3(blocks_for(n_ops, 256)), dim3(256), 0, s, d.dyn, ops, n_ops);
+}
+
+void launch_dyn_step_begin(hipStream_t s, const DevState& d, const uint32_t* active, uint32_t n_active) {
+    if (n_active == 0) return;
+    hipLaunchKernelGGL(dyn_step_begin_kernel, dim3(blocks_for(n_active, 256)), dim3(256), 0, s, d, active, n_active);
+}
+
+void launch_dyn_dual(hipStream_t s, const DevState& d, const uint32_t* active, uint32_t n_active,
+                     const uint32_t* done) {
+    if (n_active == 0) return;
+    hipLaunchKernelGGL(dyn_dual_kernel, dim3(blocks_for(n_active, 256)), dim3(256), 0, s, d, active, n_active,
+                       done);
 }
 
 void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,

@@ -225,6 +225,58 @@ class SimulationContext:
     def detect_collisions(self) -> None:
         self._chk(lib.avbd_detect_collisions(self.handle))
 
+    # --- dynamic contacts (CollisionDetector::detectCollisionsAndRecolor) ----
+    def add_collision_segments(self, nodes, radius: float) -> None:
+        """The segments between consecutive rod nodes, one object."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        self._chk(lib.avbd_add_collision_segments(self.handle, nodes.shape[0], _uptr(nodes), float(radius)))
+
+    def add_collision_sphere(self, p: int, radius: float) -> None:
+        self._chk(lib.avbd_add_collision_sphere(self.handle, int(p), float(radius)))
+
+    def set_dynamic_contacts(self, margin: float, capacity: int = 0) -> None:
+        self._chk(lib.avbd_set_dynamic_contacts(self.handle, float(margin), int(capacity)))
+
+    def contacts(self, kind: int) -> np.ndarray:
+        """Active contacts of one kind (0 triangle-rigid, 1 triangle-rod) in slot
+        order, one row of 32 per contact: slot, idx[5], n, t, b, barys, s,
+        cp_local, k_start, k_n, k_t, k_b, lambda_n, lambda_t, lambda_b,
+        C_n_prev, use_static, unclamped."""
+        cnt = C.c_uint32()
+        self._chk(lib.avbd_download_contacts(self.handle, kind, None, 0, C.byref(cnt)))
+        n = cnt.value
+        buf = (_lib.Contact * max(n, 1))()
+        self._chk(lib.avbd_download_contacts(self.handle, kind, buf, n, C.byref(cnt)))
+        out = np.zeros((n, 32))
+        for i in range(n):
+            c = buf[i]
+            out[i, 0] = c.slot
+            out[i, 1:6] = list(c.idx)
+            out[i, 6:9], out[i, 9:12], out[i, 12:15] = list(c.n), list(c.t), list(c.b)
+            out[i, 15:18] = list(c.barys)
+            out[i, 18] = c.s
+            out[i, 19:22] = list(c.cp_local)
+            out[i, 22:30] = [c.k_start, c.k_n, c.k_t, c.k_b, c.lambda_n, c.lambda_t, c.lambda_b, c.C_n_prev]
+            out[i, 30], out[i, 31] = c.use_static, c.unclamped
+        return out
+
+    def detected(self):
+        """Last detection's contacts in key order: (keys, slots, kinds)."""
+        cnt = C.c_uint32()
+        self._chk(lib.avbd_download_detected(self.handle, None, None, None, 0, C.byref(cnt)))
+        n = cnt.value
+        keys = np.zeros(max(n, 1), dtype=np.uint64)
+        slots = np.zeros(max(n, 1), dtype=np.uint32)
+        kinds = np.zeros(max(n, 1), dtype=np.uint8)
+        self._chk(lib.avbd_download_detected(self.handle, keys.ctypes.data_as(C.POINTER(C.c_uint64)), _uptr(slots),
+                                             kinds.ctypes.data_as(C.POINTER(C.c_uint8)), n, C.byref(cnt)))
+        return keys[:n], slots[:n], kinds[:n]
+
+    def conflicted(self) -> np.ndarray:
+        f = np.zeros(lib.avbd_num_particles(self.handle), dtype=np.uint8)
+        self._chk(lib.avbd_download_conflicts(self.handle, f.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return f.astype(bool)
+
     def set_residual_target(self, rel_tol: float, max_iters: int, check_every: int = 5) -> None:
         """Iterate each step until ||r|| / ||M g|| <= rel_tol (checked every
         check_every iterations) or max_iters; rel_tol = 0: fixed count."""

@@ -319,6 +319,57 @@ int avbd_download_bvh(avbd_ctx* ctx, uint64_t* morton, uint32_t* sorted, uint32_
  * (> cap means truncated).  Order is unspecified. */
 int avbd_download_candidates(avbd_ctx* ctx, uint32_t* pairs, uint32_t cap, uint32_t* count);
 
+/* Dynamic contacts --------------------------------------------------------
+ * CollisionPrimitivePool::addObject(Rod) (collision/CollisionPrimitivePool.cpp:52-72):
+ * the n_nodes - 1 segments between consecutive nodes (oriented particles),
+ * radius `radius`, as one object.  Primitives are numbered in call order
+ * across avbd_add_collision_triangles / _segments / _sphere. */
+int avbd_add_collision_segments(avbd_ctx* ctx, uint32_t n_nodes, const uint32_t* nodes, double radius);
+/* CollisionPrimitivePool::addObject(RigidSphere) (collision/CollisionPrimitivePool.cpp:74-87):
+ * a sphere of `radius` centred on oriented particle p, as one object. */
+int avbd_add_collision_sphere(avbd_ctx* ctx, uint32_t p, double radius);
+/* Turn on the reference's per-step contact pipeline (after avbd_build_graph):
+ * every avbd_step() then runs CollisionDetector::detectCollisionsAndRecolor_Parallel
+ * (collision/CollisionDetector.cpp:57-129) before the solve -- LBVH over
+ * every primitive and the narrow phase on the device; contact lifecycle
+ * (_handleDetectedCollisions, :272-501), dynamic adjacency and incremental
+ * recolouring (common/ColorList.hpp:381-522) on the host -- and the solve
+ * includes the triangle-rigid / triangle-rod AL contacts.
+ * collision_margin: SimulationParams::collision_margin; capacity: contacts
+ * per kind (0 = 65,536).  Steps then run eagerly and synchronise once per
+ * step (the host needs the narrow-phase output); the persistent kernel,
+ * the step graph, the warm start and residual-targeted iteration are not
+ * available in this mode (AVBD_E_ARG).  A sphere-sphere or rod-sphere
+ * candidate pair fails the step with AVBD_E_UNSUPPORTED_CONTACT, as the
+ * reference throws "Unsupported collision!" (:210-213,686-687). */
+int avbd_set_dynamic_contacts(avbd_ctx* ctx, double collision_margin, uint32_t capacity);
+
+/* One dynamic contact (CollisionConstraintEnergyInfo + TriangleRigid /
+ * TriangleRod info, energy/collision/CollisionConstraintEnergyPool.hpp:9-30,
+ * energy/collision/TriangleRigidCollisionEnergyPool.hpp:8-14,
+ * energy/collision/TriangleRodCollisionEnergyPool.hpp:8-15). */
+typedef struct avbd_contact {
+    uint32_t slot;          /* pool slot (TombstonePool index) */
+    uint32_t idx[5];        /* triangle vertices, then the body or both rod nodes */
+    double n[3], t[3], b[3];
+    double barys[3];
+    double s;               /* triangle-rod: contact point along the segment */
+    double cp_local[3];
+    double k_start, k_n, k_t, k_b;
+    double lambda_n, lambda_t, lambda_b, C_n_prev;
+    uint8_t use_static, unclamped, pad_[6];
+} avbd_contact;
+/* Active contacts of one kind (0 triangle-rigid, 1 triangle-rod) in slot
+ * order: up to cap into out; *count = active total. */
+int avbd_download_contacts(avbd_ctx* ctx, uint32_t kind, avbd_contact* out, uint32_t cap, uint32_t* count);
+/* Contacts detected by the last step's detection in key order
+ * (DetectedCollision::generateKey, collision/DetectedCollision.hpp:43-66):
+ * key, pool slot, DetectedCollisionType (2 triangle-rigid, 3 triangle-rod). */
+int avbd_download_detected(avbd_ctx* ctx, uint64_t* keys, uint32_t* slots, uint8_t* kinds, uint32_t cap,
+                           uint32_t* count);
+/* is_conflicted per particle after the last recolouring (n bytes). */
+int avbd_download_conflicts(avbd_ctx* ctx, uint8_t* flags);
+
 /* Learned warm start (new; replaces the inertial / adaptive initial guess of
  * _particleInertialUpdate, simulation/VBDSolver.hpp:208-248, whenever it
  * lowers the incremental potential of the scene).  A per-vertex MLP

@@ -157,14 +157,14 @@ def test_record_layouts_match_c_compiler(tmp_path):
     from avbd_amd import _lib
     src = tmp_path / "sizes.c"
     recs = ["avbd_neo_hookean_record", "avbd_ground_record", "avbd_cosserat_record", "avbd_rigid_ground_record",
-            "avbd_joint_record", "avbd_stats", "avbd_scene_desc"]
+            "avbd_joint_record", "avbd_stats", "avbd_scene_desc", "avbd_contact"]
     body = "\n".join(f'printf("%zu\\n", sizeof({r}));' for r in recs)
     src.write_text(f'#include <stdio.h>\n#include "avbd.h"\nint main(void) {{ {body} return 0; }}\n')
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     mirror = [_lib.NeoHookeanRecord, _lib.GroundRecord, _lib.CosseratRecord, _lib.RigidGroundRecord,
-              _lib.JointRecord, _lib.Stats, _lib.SceneDesc]
+              _lib.JointRecord, _lib.Stats, _lib.SceneDesc, _lib.Contact]
     assert sizes == [C.sizeof(m) for m in mirror]
 
 
