@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -1365,10 +1366,12 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
         hip_check(hipEventRecord(c->ev_a, c->stream), "hipEventRecord");
         if (c->dynamic) {
             // per step: detection round (synchronising), then the solve
-            double total = 0;
+            double total = 0, detect_ms = 0;
             uint32_t launches = 0;
             for (uint32_t st = 0; st < n_steps; st++) {
+                const auto t0 = std::chrono::steady_clock::now();
                 detect_round(c);
+                detect_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                 enqueue_step(c, c->timing);
                 c->vertex_updates += (uint64_t)c->desc.solver_iters * (c->d.n_items + c->d.n_oitems);
                 if (c->timing) {
@@ -1385,6 +1388,7 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
                 c->solve_avg_ms = launches ? total / launches : 0;
                 c->solve_launches = launches;
             }
+            c->lbvh_avg_ms = n_steps ? detect_ms / n_steps : 0;
             hip_check(hipEventRecord(c->ev_b, c->stream), "hipEventRecord");
             c->steps_taken += n_steps;
             return;

@@ -345,6 +345,24 @@ def rod_on_block(n: int = 6, edge: float = 0.2, elements: int = 12, length: floa
                  contact_margin=margin)
 
 
+def hail_on_block(n: int = 24, edge: float = 1.0, grid: int = 8, radius: float = 0.04, speed: float = 1.0,
+                  dt: float = 1.0 / 60.0, iters: int = 10, steps: int = 60, margin: float = 1e-3) -> Scene:
+    """grid x grid rigid spheres (r = 4 cm, 2000 kg/m^3, 12 cm apart, no
+    spin) falling at `speed` onto a soft block (n^3 cells, E = 1e6 Pa,
+    bottom layer held): many simultaneous triangle-rigid contacts."""
+    block = soft_block(n, edge, E=1e6, nu=0.4)
+    spacing = 0.12
+    x0 = 0.5 * edge - 0.5 * spacing * (grid - 1)
+    spheres = []
+    for i in range(grid):
+        for k in range(grid):
+            h = edge + radius + 0.004 + 0.01 * ((i + 2 * k) % 3)
+            spheres.append(Sphere(radius=radius, density=2000.0, pos=(x0 + spacing * i, h, x0 + spacing * k),
+                                  ground=False, v0=(0.0, -speed, 0.0)))
+    return Scene("hail_on_block", meshes=[block], spheres=spheres, dt=dt, iters=iters, rho=0.95, steps=steps,
+                 contact_margin=margin)
+
+
 def algorithmic_bytes_per_step(n_v: int, n_tet: int, n_g: int, iters: int) -> int:
     """FP32 gather model of one time step:
     I*(112 N_v + 576 N_tet + 136 N_g) + 156 N_v + 136 N_tet + 100 N_g."""

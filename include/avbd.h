@@ -296,7 +296,10 @@ int avbd_get_step_mode(avbd_ctx* ctx, uint32_t* persistent_grid);
 int avbd_set_timing(avbd_ctx* ctx, int enable);
 int avbd_get_solve_timing(avbd_ctx* ctx, double* avg_ms, uint32_t* launches);
 /* Average per-step duration (ms) of the broad phase over the last timed
- * avbd_step() call (0 without collision triangles). */
+ * avbd_step() call (0 without collision triangles); with dynamic contacts,
+ * of the whole detection round over the last avbd_step() call (device
+ * bounds, LBVH and narrow phase, host lifecycle and recolouring, uploads;
+ * host wall clock). */
 int avbd_get_broadphase_timing(avbd_ctx* ctx, double* avg_ms);
 
 /* Collision primitives and broad phase ------------------------------------
