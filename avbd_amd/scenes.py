@@ -186,6 +186,11 @@ class Scene:
     # step runs the triangle narrow phase against rods and spheres and keeps
     # the resulting dynamic contacts (CollisionDetector::detectCollisionsAndRecolor)
     contact_margin: Optional[float] = None
+    # several meshes: independent scenes of a batch (True) or objects of one
+    # simulation (False, as a YAML scene with several TetMeshObjects)
+    batch: bool = True
+    # creation order of the object kinds (particle and primitive numbering)
+    object_order: Tuple[str, ...] = ("meshes", "rods", "spheres")
 
     @property
     def n_vertices(self) -> int:

@@ -482,23 +482,30 @@ class Simulation:
         self._setup = False
 
     def setup(self) -> None:
+        sc = self.scene
         offs = [0]
-        for m in self.scene.meshes:
-            first = self._add_object(m)
-            offs.append(first + m.rest.shape[0])
-        self.offsets = offs
         self.rod_first: List[int] = []
-        self._spins: List[tuple] = []
-        for r in self.scene.rods:
-            self.rod_first.append(self._add_rod(r))
         self.sphere_index: List[int] = []
-        for sp in self.scene.spheres:
-            self.sphere_index.append(self._add_sphere(sp))
+        self._spins: List[tuple] = []
+        for kind in sc.object_order:
+            if kind == "meshes":
+                for m in sc.meshes:
+                    first = self._add_object(m)
+                    offs.append(first + m.rest.shape[0])
+            elif kind == "rods":
+                for r in sc.rods:
+                    self.rod_first.append(self._add_rod(r))
+            elif kind == "spheres":
+                for sp in sc.spheres:
+                    self.sphere_index.append(self._add_sphere(sp))
+            else:
+                raise ValueError(f"unknown object kind {kind!r}")
+        self.offsets = offs
         for p, _rot, w0 in self._spins:
             if any(c != 0.0 for c in w0):
                 self.ctx.set_rotation_state(p, w=np.asarray([w0], dtype=np.float64))
-        if len(self.scene.meshes) > 1:
-            if self.scene.rods or self.scene.spheres:
+        if len(sc.meshes) > 1 and sc.batch:
+            if sc.rods or sc.spheres:
                 raise ValueError("batched tet scenes cannot be mixed with rods or spheres")
             self.ctx.set_scenes(offs)
         self.ctx.build_graph()
@@ -548,7 +555,7 @@ class Simulation:
         # CollisionPrimitivePool::addObject(TetMeshObject) registers the surface
         # faces (collision/CollisionPrimitivePool.cpp:36-50); independent scenes of
         # a batch are not collided with each other, so batches skip it
-        if m.triangles is not None and len(self.scene.meshes) == 1:
+        if m.triangles is not None and (len(self.scene.meshes) == 1 or not self.scene.batch):
             self.ctx.add_collision_triangles(m.triangles + first)
         if m.fixed is not None:
             self.ctx.set_fixed(first, m.fixed)

@@ -401,48 +401,68 @@ def update_collision_frame(n, t, b, lam_t, lam_b, new_n):
 
 
 def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
-    """Oracle instance for one mesh of an avbd_amd.scenes.Scene (one reference
-    Simulation), followed by the scene's rods and spheres in order."""
+    """Oracle instance of an avbd_amd.scenes.Scene (one reference Simulation):
+    objects in the scene's object order; of a batch of independent meshes,
+    mesh `mesh_index` alone."""
     o = OracleSim(scene.dt, scene.g, scene.iters, scene.rho)
-    if scene.meshes:
-        m = scene.meshes[mesh_index]
-        o.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
-        if m.fixed is not None:
-            o.set_fixed(m.fixed)
-        if m.x0 is not None:
-            o.set_positions(m.x0, also_prev=True)
-        if m.v0 is not None:
-            o.set_velocities(m.v0, m.v0)
-        if collide and m.triangles is not None and len(scene.meshes) == 1:
-            o.set_collision_triangles(m.triangles)
+    order = getattr(scene, "object_order", ("meshes", "rods", "spheres"))
+    batch = getattr(scene, "batch", True)
+    meshes = scene.meshes[mesh_index:mesh_index + 1] if (batch and scene.meshes) else scene.meshes
     dynamic = getattr(scene, "contact_margin", None) is not None
-    spins = []
-    for r in getattr(scene, "rods", []):
-        first = o.add_rod(r.elements, r.length, r.curvature, r.E, r.nu, r.density, r.diameter, r.pos, r.rot,
-                          r.vel, r.base_fixed, r.tip_fixed, r.ground)
-        if dynamic:
-            o.add_collision_segments(np.arange(first, first + r.elements + 1), 0.5 * r.diameter)
-        if r.base_joint:
-            o.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
-        if getattr(r, "tip_sphere", None) is not None:
-            rad, m, w0 = r.tip_sphere
-            tip = first + r.elements
-            dI = np.full(3, 0.4 * m * rad * rad)
-            lib.oracle_add_mass(o.h, tip, m, dp(dI))
-            # sphere ground contact, k_start from the node's total mass (mu 0.5 / 0.2)
-            mass, _ = o.mass_fixed()
-            lib.oracle_add_rground(o.h, tip, 0, rad, mass[tip] / (scene.dt * scene.dt), 0.5, 0.2)
-            spins.append((tip, r.rot, w0))
-    for sp in getattr(scene, "spheres", []):
-        p = o.add_sphere(sp.radius, sp.density, sp.pos, sp.rot, sp.fixed, sp.ground)
-        if dynamic:
-            o.add_collision_sphere(p, sp.radius)
-        if any(c != 0.0 for c in sp.w0):
-            spins.append((p, sp.rot, sp.w0))
-        if any(c != 0.0 for c in getattr(sp, "v0", (0.0, 0.0, 0.0))):
-            vel = o.velocities()
-            vel[p] = sp.v0
-            o.set_velocities(vel, vel)
+    fixed_rows, x0_rows, v0_rows, spins = [], [], [], []
+    for kind in order:
+        if kind == "meshes":
+            for m in meshes:
+                first = o.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
+                if m.fixed is not None:
+                    fixed_rows.append((first, m.fixed))
+                if m.x0 is not None:
+                    x0_rows.append((first, m.x0))
+                if m.v0 is not None:
+                    v0_rows.append((first, m.v0))
+                if collide and m.triangles is not None and (len(scene.meshes) == 1 or not batch):
+                    o.add_collision_triangles(np.asarray(m.triangles, dtype=np.int64) + first)
+        elif kind == "rods":
+            for r in getattr(scene, "rods", []):
+                first = o.add_rod(r.elements, r.length, r.curvature, r.E, r.nu, r.density, r.diameter, r.pos, r.rot,
+                                  r.vel, r.base_fixed, r.tip_fixed, r.ground)
+                if dynamic:
+                    o.add_collision_segments(np.arange(first, first + r.elements + 1), 0.5 * r.diameter)
+                if r.base_joint:
+                    o.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
+                if getattr(r, "tip_sphere", None) is not None:
+                    rad, m, w0 = r.tip_sphere
+                    tip = first + r.elements
+                    dI = np.full(3, 0.4 * m * rad * rad)
+                    lib.oracle_add_mass(o.h, tip, m, dp(dI))
+                    # sphere ground contact, k_start from the node's total mass (mu 0.5 / 0.2)
+                    mass, _ = o.mass_fixed()
+                    lib.oracle_add_rground(o.h, tip, 0, rad, mass[tip] / (scene.dt * scene.dt), 0.5, 0.2)
+                    spins.append((tip, r.rot, w0))
+        elif kind == "spheres":
+            for sp in getattr(scene, "spheres", []):
+                p = o.add_sphere(sp.radius, sp.density, sp.pos, sp.rot, sp.fixed, sp.ground)
+                if dynamic:
+                    o.add_collision_sphere(p, sp.radius)
+                if any(c != 0.0 for c in sp.w0):
+                    spins.append((p, sp.rot, sp.w0))
+                if any(c != 0.0 for c in getattr(sp, "v0", (0.0, 0.0, 0.0))):
+                    v0_rows.append((p, np.asarray([sp.v0], dtype=np.float64)))
+    if fixed_rows:
+        _, f = o.mass_fixed()
+        for first, arr in fixed_rows:
+            f[first:first + len(arr)] = arr
+        o.set_fixed(f)
+    if x0_rows:
+        x = o.positions()
+        for first, arr in x0_rows:
+            x[first:first + len(arr)] = arr
+        o.set_positions(x, also_prev=True)
+    if v0_rows:
+        v = o.velocities()
+        for first, arr in v0_rows:
+            v[first:first + len(arr)] = arr
+        o.set_velocities(v, v)
     if spins:
         cur = o.rotations()
         q, w = cur["q"].copy(), cur["w"].copy()

@@ -113,3 +113,37 @@ def test_unsupported_pair_fails_the_step():
     with pytest.raises(AvbdError) as e:
         sim.ctx.step(1)
     assert e.value.code == -4   # AVBD_E_UNSUPPORTED_CONTACT
+
+
+@pytest.mark.parametrize("name,steps,kind", [("tumor_rod", 30, 1), ("bunny_ball", 60, 0)])
+def test_yaml_scene_matches_oracle(name, steps, kind):
+    """Scenes read from YAML files in the reference's schema (the reference's
+    tumor and bunny meshes): contact keys, slots and colourings per step while
+    the contact sets agree, positions at the end."""
+    from pathlib import Path
+    from avbd_amd import config
+    sc = config.scene_from_yaml(Path(__file__).resolve().parent / "golden" / "scenes" / f"{name}.yaml")
+    o = O.from_scene(sc)
+    sim = Simulation(sc)
+    sim.setup()
+    ctx = sim.ctx
+    bb = _bbox(o.positions())
+    same_steps = 0
+    for step in range(steps):
+        o.step(1)
+        ctx.step(1)
+        ko, so, _ = o.detected()
+        kd, sd, _ = ctx.detected()
+        if not np.array_equal(ko, kd):
+            _log(f"{name}: contact sets first differ at step {step} ({len(ko)} vs {len(kd)})")
+            break
+        assert np.array_equal(so, sd)
+        col_o, k_o = o.colors()
+        col_d, k_d = ctx.coloring()
+        assert k_o == k_d and np.array_equal(col_o, col_d), step
+        same_steps += 1
+    ctx.synchronize()
+    dx = np.max(np.linalg.norm(o.positions() - ctx.positions(), axis=1)) / bb
+    _log(f"{name}: identical contact sets for {same_steps} steps, positions max err {dx:.2e} x bbox")
+    assert same_steps >= min(steps, 20)
+    assert dx < 5e-3
