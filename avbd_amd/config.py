@@ -212,6 +212,10 @@ def scene_from_config(cfg: dict, base_dir: Union[str, Path] = ".", steps: Option
             sc.spheres.append(sphere_from_config(obj))
         else:
             raise ValueError(f"unknown object type {ty!r}")
+    if bool(cfg.get("logging", False)):
+        sc.logging = dict(output_dir=str(base_dir / str(cfg.get("logging-output-folder", "../output/"))),
+                          interval=float(cfg.get("logging-interval", 1e-2)),
+                          log_residuals=bool(cfg.get("log-residuals", False)))
     # every primitive joins the collision pool; contacts other than the
     # no-op triangle-triangle pairs need a rod or sphere next to a mesh
     if sc.meshes and (sc.rods or sc.spheres):

@@ -191,6 +191,9 @@ class Scene:
     batch: bool = True
     # creation order of the object kinds (particle and primitive numbering)
     object_order: Tuple[str, ...] = ("meshes", "rods", "spheres")
+    # logging (SimulationConfig logging / logging-output-folder /
+    # logging-interval / log-residuals): dict(output_dir, interval, log_residuals)
+    logging: Optional[dict] = None
 
     @property
     def n_vertices(self) -> int:

@@ -461,9 +461,48 @@ class VBDSolver:
             self._ctx.step(1)
 
 
+class SimulationLogger:
+    """SimulationLogger (simulation/SimulationLogger.hpp:18-77,
+    simulation/SimulationLogger.cpp): named quantities evaluated by callables,
+    one space-separated line per logging interval after a header line."""
+
+    def __init__(self, output_filename, logging_interval: float):
+        from pathlib import Path
+        path = Path(output_filename)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        self._out = open(path, "w")
+        self._interval = float(logging_interval)
+        self._last = -1e10
+        self._vars: List[tuple] = []
+        self._logging = False
+
+    def add_output(self, name: str, func) -> None:
+        if self._logging:
+            raise RuntimeError("outputs cannot be added after logging started")
+        self._vars.append((name, func))
+
+    def start_logging(self) -> None:
+        assert not self._logging
+        self._logging = True
+        self._out.write("".join(f"{n} " for n, _ in self._vars) + "\n")
+
+    def log_to_file(self, time: float) -> None:
+        if time - self._last + 1e-12 >= self._interval:
+            self._out.write("".join(f"{f()!r} " for _, f in self._vars) + "\n")
+            self._last = time
+
+    def stop_logging(self) -> None:
+        self._logging = False
+        self._out.close()
+
+
 class Simulation:
     """Simulation shell (simulation/Simulation.cpp:22-220): builds the context from
-    a Scene description, runs setup() (adjacency + colouring) and time steps."""
+    a Scene description, runs setup() (adjacency + colouring) and time steps.
+    A logger (``logger`` argument or ``Scene.logging``) records time and, with
+    ``log_residuals``, the force residual and incremental potential after
+    every logged step (the quantities of the reference's log-residuals
+    option, config/SimulationConfig.hpp:50-53)."""
 
     def __init__(self, scene: Scene, device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0,
                  use_graph: bool = True, serial_broadphase: bool = False, persistent: bool = False,
@@ -480,6 +519,25 @@ class Simulation:
         self.time = 0.0
         self.offsets: List[int] = []
         self._setup = False
+        self.logger: Optional[SimulationLogger] = None
+        lg = getattr(scene, "logging", None)
+        if lg:
+            import datetime
+            from pathlib import Path
+            name = datetime.datetime.now().strftime("%Y-%m-%d_%H:%M:%S") + ".txt"
+            self.logger = SimulationLogger(Path(lg.get("output_dir", "output")) / name, lg.get("interval", 1e-2))
+            self.logger.add_output("time", lambda: self.time)
+            if lg.get("log_residuals", False):
+                self._stats_cache = None
+                self.logger.add_output("residual_rel", lambda: self._stats().residual_rel)
+                self.logger.add_output("residual_abs", lambda: self._stats().residual_abs)
+                self.logger.add_output("incremental_potential", lambda: self._stats().incremental_potential)
+
+    def _stats(self):
+        # one device stats pass per logged line
+        if self._stats_cache is None or self._stats_cache[0] != self.time:
+            self._stats_cache = (self.time, self.ctx.stats())
+        return self._stats_cache[1]
 
     def setup(self) -> None:
         sc = self.scene
@@ -573,8 +631,16 @@ class Simulation:
         if not self._setup:
             self.setup()
         n = self.scene.steps if n_steps is None else n_steps
-        self.ctx.step(n)
-        self.time += n * self.scene.dt
+        if self.logger is None:
+            self.ctx.step(n)
+            self.time += n * self.scene.dt
+            return
+        if not self.logger._logging:
+            self.logger.start_logging()
+        for _ in range(n):
+            self.ctx.step(1)
+            self.time += self.scene.dt
+            self.logger.log_to_file(self.time)
 
     def positions(self) -> np.ndarray:
         return self.ctx.positions()

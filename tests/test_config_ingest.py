@@ -97,3 +97,31 @@ def test_reader_rejects_other_formats(tmp_path):
     p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
     with pytest.raises(config.MeshFormatError):
         config.read_msh(p)
+
+
+def test_simulation_logger_format(tmp_path):
+    """SimulationLogger: header of names, then one line per interval."""
+    from avbd_amd.sim import SimulationLogger
+    lg = SimulationLogger(tmp_path / "sub" / "log.txt", 0.01)
+    t = [0.0]
+    lg.add_output("time", lambda: t[0])
+    lg.add_output("twice", lambda: 2 * t[0])
+    lg.start_logging()
+    with pytest.raises(RuntimeError):
+        lg.add_output("late", lambda: 0.0)
+    for k in range(1, 11):
+        t[0] = k * 0.004
+        lg.log_to_file(t[0])
+    lg.stop_logging()
+    lines = (tmp_path / "sub" / "log.txt").read_text().splitlines()
+    assert lines[0] == "time twice "
+    logged = [float(l.split()[0]) for l in lines[1:]]
+    # first call logs, then every time >= last + interval
+    assert logged == [0.004, 0.016, 0.028, 0.04]
+
+
+def test_yaml_logging_fields():
+    cfg = {"time-step": 1e-3, "end-time": 0.01, "logging": True, "logging-output-folder": "out",
+           "logging-interval": 0.005, "log-residuals": True, "objects": []}
+    sc = config.scene_from_config(cfg, "/tmp/base")
+    assert sc.logging == dict(output_dir="/tmp/base/out", interval=0.005, log_residuals=True)
