@@ -245,6 +245,9 @@ SIGNATURES = {
     "avbd_download_contacts": (C.c_int, [_P, C.c_uint32, C.POINTER(Contact), C.c_uint32, _U32]),
     "avbd_download_detected": (C.c_int, [_P, C.POINTER(C.c_uint64), _U32, _U8, C.c_uint32, _U32]),
     "avbd_download_conflicts": (C.c_int, [_P, _U8]),
+    "avbd_checkpoint_size": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "avbd_save_checkpoint": (C.c_int, [_P, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "avbd_load_checkpoint": (C.c_int, [_P, C.c_void_p, C.c_uint64]),
 }
 
 

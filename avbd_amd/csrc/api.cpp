@@ -2136,6 +2136,157 @@ int avbd_download_detected(avbd_ctx* c, uint64_t* keys, uint32_t* slots, uint8_t
     });
 }
 
+namespace {
+// Checkpoint layout: header, then the device arrays in a fixed order, then
+// (dynamic contacts) the contact records, the colouring and the host
+// contact state.
+struct CkptHeader {
+    char magic[8];
+    uint32_t version, n, n_tets, n_ground, n_rot, n_rground, n_joints;
+    uint32_t dynamic, has_damp, has_rc, num_colors;
+    uint64_t steps_taken, vertex_updates;
+};
+constexpr char kCkptMagic[8] = {'A', 'V', 'B', 'D', 'C', 'K', 'P', 'T'};
+
+// (device pointer, bytes) of every mutable device array, in checkpoint order
+std::vector<std::pair<void*, size_t>> ckpt_arrays(avbd_ctx* c) {
+    const DevState& d = c->d;
+    std::vector<std::pair<void*, size_t>> a;
+    const size_t p4 = (size_t)d.n * sizeof(float4), r4 = (size_t)d.n_rot * sizeof(float4);
+    for (float4* v : {d.x, d.x_buf, d.y, d.x_prev, d.x_last, d.x_lastlast, d.v, d.v_prev}) a.emplace_back(v, p4);
+    if (c->damping) a.emplace_back(d.damp, (size_t)d.n_tets * sizeof(TetDamp));
+    a.emplace_back(d.ground, (size_t)d.n_ground * sizeof(GroundRec));
+    if (d.n_rot)
+        for (float4* v : {d.q, d.q_buf, d.q_in, d.q_prev, d.q_last, d.q_lastlast, d.w, d.w_prev}) a.emplace_back(v, r4);
+    if (d.n_rground) a.emplace_back(d.rground, (size_t)d.n_rground * sizeof(RGroundRec));
+    if (d.n_joints) a.emplace_back(d.joints, (size_t)d.n_joints * sizeof(JointRec));
+    if (c->rc) a.emplace_back(c->rc, sizeof(ResCtl));
+    if (c->dynamic) a.emplace_back(d.dyn, 2 * (size_t)c->cs.cap * sizeof(DynRec));
+    return a;
+}
+
+CkptHeader ckpt_header(avbd_ctx* c) {
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.n = c->d.n;
+    h.n_tets = c->d.n_tets;
+    h.n_ground = c->d.n_ground;
+    h.n_rot = c->d.n_rot;
+    h.n_rground = c->d.n_rground;
+    h.n_joints = c->d.n_joints;
+    h.dynamic = c->dynamic ? 1u : 0u;
+    h.has_damp = c->damping ? 1u : 0u;
+    h.has_rc = c->rc ? 1u : 0u;
+    h.num_colors = c->plan.num_colors;
+    h.steps_taken = c->steps_taken;
+    h.vertex_updates = c->vertex_updates;
+    return h;
+}
+
+std::vector<uint8_t> ckpt_host_tail(avbd_ctx* c) {
+    std::vector<uint8_t> t;
+    if (!c->dynamic) return t;
+    const uint64_t nc = c->plan.color.size();
+    t.resize(sizeof(uint64_t) + nc * sizeof(uint32_t));
+    std::memcpy(t.data(), &nc, sizeof(uint64_t));
+    std::memcpy(t.data() + sizeof(uint64_t), c->plan.color.data(), nc * sizeof(uint32_t));
+    c->cs.save(t);
+    return t;
+}
+}  // namespace
+
+int avbd_checkpoint_size(avbd_ctx* c, uint64_t* bytes) {
+    return guarded(c, [&] {
+        require_built(c);
+        uint64_t total = sizeof(CkptHeader);
+        for (const auto& a : ckpt_arrays(c)) total += a.second;
+        total += ckpt_host_tail(c).size();
+        if (bytes) *bytes = total;
+    });
+}
+
+int avbd_save_checkpoint(avbd_ctx* c, void* buf, uint64_t cap, uint64_t* written) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!buf) throw CodeError(AVBD_E_ARG, "buf is null");
+        const auto arrays = ckpt_arrays(c);
+        const std::vector<uint8_t> tail = ckpt_host_tail(c);
+        uint64_t total = sizeof(CkptHeader) + tail.size();
+        for (const auto& a : arrays) total += a.second;
+        if (cap < total) throw CodeError(AVBD_E_CAPACITY, "checkpoint buffer too small (see avbd_checkpoint_size)");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        uint8_t* p = static_cast<uint8_t*>(buf);
+        const CkptHeader h = ckpt_header(c);
+        std::memcpy(p, &h, sizeof(h));
+        p += sizeof(h);
+        for (const auto& a : arrays) {
+            hip_check(hipMemcpyAsync(p, a.first, a.second, hipMemcpyDeviceToHost, c->stream), "D2H");
+            p += a.second;
+        }
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        if (!tail.empty()) std::memcpy(p, tail.data(), tail.size());
+        if (written) *written = total;
+    });
+}
+
+int avbd_load_checkpoint(avbd_ctx* c, const void* buf, uint64_t size) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!buf || size < sizeof(CkptHeader)) throw CodeError(AVBD_E_ARG, "not a checkpoint");
+        const uint8_t* p = static_cast<const uint8_t*>(buf);
+        const uint8_t* end = p + size;
+        CkptHeader h;
+        std::memcpy(&h, p, sizeof(h));
+        p += sizeof(h);
+        const CkptHeader mine = ckpt_header(c);
+        if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1) throw CodeError(AVBD_E_ARG, "not a checkpoint");
+        if (h.n != mine.n || h.n_tets != mine.n_tets || h.n_ground != mine.n_ground || h.n_rot != mine.n_rot ||
+            h.n_rground != mine.n_rground || h.n_joints != mine.n_joints || h.dynamic != mine.dynamic ||
+            h.has_damp != mine.has_damp || h.has_rc != mine.has_rc)
+            throw CodeError(AVBD_E_ARG, "checkpoint does not match this context's scene and modes");
+        for (const auto& a : ckpt_arrays(c)) {
+            if ((uint64_t)(end - p) < a.second) throw CodeError(AVBD_E_ARG, "checkpoint is truncated");
+            hip_check(hipMemcpyAsync(a.first, p, a.second, hipMemcpyHostToDevice, c->stream), "H2D");
+            p += a.second;
+        }
+        if (c->dynamic) {
+            uint64_t nc = 0;
+            if ((uint64_t)(end - p) < sizeof(uint64_t)) throw CodeError(AVBD_E_ARG, "checkpoint is truncated");
+            std::memcpy(&nc, p, sizeof(uint64_t));
+            p += sizeof(uint64_t);
+            if (nc != c->plan.color.size() || (uint64_t)(end - p) / sizeof(uint32_t) < nc)
+                throw CodeError(AVBD_E_ARG, "checkpoint does not match this context's particles");
+            std::memcpy(c->plan.color.data(), p, nc * sizeof(uint32_t));
+            p += nc * sizeof(uint32_t);
+            c->cs.load(p, end);
+            c->plan.num_colors = h.num_colors;
+            relayout_plan(c->plan, c->cs.conflicted);
+            upload(c, c->d.items, c->plan.items);
+            upload(c, c->d.oitems, c->plan.oitems);
+            c->d_conflicts = static_cast<uint32_t*>(grow(c, c->g_conf, c->plan.conflict_list.size() * sizeof(uint32_t)));
+            upload(c, c->d_conflicts, c->plan.conflict_list);
+            auto* d_act = static_cast<uint32_t*>(grow(c, c->g_active, c->cs.active.size() * sizeof(uint32_t)));
+            upload(c, d_act, c->cs.active);
+            auto* d_off = static_cast<uint32_t*>(grow(c, c->g_off, c->cs.dyn_off.size() * sizeof(uint32_t)));
+            upload(c, d_off, c->cs.dyn_off);
+            auto* d_ent = static_cast<uint32_t*>(grow(c, c->g_ent, c->cs.dyn_ent.size() * sizeof(uint32_t)));
+            upload(c, d_ent, c->cs.dyn_ent);
+            c->d.dyn_off = d_off;
+            c->d.dyn_ent = d_ent;
+            const size_t need = 2 * (size_t)c->iters() * c->plan.num_colors;
+            while (c->events.size() < need) {
+                hipEvent_t ev;
+                hip_check(hipEventCreate(&ev), "hipEventCreate");
+                c->events.push_back(ev);
+            }
+        }
+        c->steps_taken = h.steps_taken;
+        c->vertex_updates = h.vertex_updates;
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    });
+}
+
 int avbd_download_conflicts(avbd_ctx* c, uint8_t* flags) {
     return guarded(c, [&] {
         require_built(c);

@@ -78,6 +78,10 @@ public:
     };
     std::vector<Entry> prev;
 
+    // checkpoint: append the host state to out / restore it from [p, end)
+    void save(std::vector<uint8_t>& out) const;
+    void load(const uint8_t*& p, const uint8_t* end);
+
 private:
     uint32_t n = 0;
     std::vector<uint32_t> last_dirty_round, candidate, dirty, next_dirty, touched_this_frame, touched_conflicted;

@@ -6,6 +6,7 @@
 // phase and the device solve of the same step.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 
 #include "contacts.hpp"
@@ -277,6 +278,76 @@ bool ContactSystem::recolour(ColorPlan& plan) {
         return true;
     }
     return false;
+}
+
+namespace {
+template <class T>
+void put(std::vector<uint8_t>& o, const T& v) {
+    const auto* b = reinterpret_cast<const uint8_t*>(&v);
+    o.insert(o.end(), b, b + sizeof(T));
+}
+template <class T>
+void put_vec(std::vector<uint8_t>& o, const std::vector<T>& v) {
+    put<uint64_t>(o, v.size());
+    const auto* b = reinterpret_cast<const uint8_t*>(v.data());
+    o.insert(o.end(), b, b + v.size() * sizeof(T));
+}
+template <class T>
+T get(const uint8_t*& p, const uint8_t* end) {
+    if ((size_t)(end - p) < sizeof(T)) throw std::runtime_error("checkpoint is truncated");
+    T v;
+    std::memcpy(&v, p, sizeof(T));
+    p += sizeof(T);
+    return v;
+}
+template <class T>
+void get_vec(const uint8_t*& p, const uint8_t* end, std::vector<T>& v) {
+    const uint64_t n = get<uint64_t>(p, end);
+    if ((uint64_t)(end - p) / sizeof(T) < n) throw std::runtime_error("checkpoint is truncated");
+    v.resize(n);
+    std::memcpy(v.data(), p, n * sizeof(T));
+    p += n * sizeof(T);
+}
+}  // namespace
+
+void ContactSystem::save(std::vector<uint8_t>& o) const {
+    put(o, n);
+    put(o, cap);
+    put(o, margin);
+    for (const Slots& s : pool) {
+        put(o, s.top);
+        put(o, s.count);
+        put_vec(o, s.stack);
+        put_vec(o, s.active);
+        put_vec(o, s.gen);
+        put_vec(o, s.idx);
+    }
+    put_vec(o, prev);
+    for (const auto* v : {&last_dirty_round, &candidate, &dirty, &touched_this_frame, &touched_conflicted,
+                          &active, &dyn_off, &dyn_ent, &nb_off, &nb})
+        put_vec(o, *v);
+    put_vec(o, conflicted);
+}
+
+void ContactSystem::load(const uint8_t*& p, const uint8_t* end) {
+    if (get<uint32_t>(p, end) != n || get<uint32_t>(p, end) != cap)
+        throw std::runtime_error("checkpoint does not match this context's contact pools");
+    margin = get<double>(p, end);
+    for (Slots& s : pool) {
+        s.top = get<uint32_t>(p, end);
+        s.count = get<uint32_t>(p, end);
+        get_vec(p, end, s.stack);
+        get_vec(p, end, s.active);
+        get_vec(p, end, s.gen);
+        get_vec(p, end, s.idx);
+    }
+    get_vec(p, end, prev);
+    for (auto* v : {&last_dirty_round, &candidate, &dirty, &touched_this_frame, &touched_conflicted, &active,
+                    &dyn_off, &dyn_ent, &nb_off, &nb})
+        get_vec(p, end, *v);
+    get_vec(p, end, conflicted);
+    ops.clear();
+    next_dirty.clear();
 }
 
 }  // namespace avbd

@@ -272,6 +272,19 @@ class SimulationContext:
                                              kinds.ctypes.data_as(C.POINTER(C.c_uint8)), n, C.byref(cnt)))
         return keys[:n], slots[:n], kinds[:n]
 
+    # --- checkpoint / resume -------------------------------------------------
+    def save_checkpoint(self) -> bytes:
+        n = C.c_uint64()
+        self._chk(lib.avbd_checkpoint_size(self.handle, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        w = C.c_uint64()
+        self._chk(lib.avbd_save_checkpoint(self.handle, buf, n.value, C.byref(w)))
+        return buf.raw[:w.value]
+
+    def load_checkpoint(self, blob: bytes) -> None:
+        buf = C.create_string_buffer(blob, len(blob))
+        self._chk(lib.avbd_load_checkpoint(self.handle, buf, len(blob)))
+
     def conflicted(self) -> np.ndarray:
         f = np.zeros(lib.avbd_num_particles(self.handle), dtype=np.uint8)
         self._chk(lib.avbd_download_conflicts(self.handle, f.ctypes.data_as(C.POINTER(C.c_uint8))))

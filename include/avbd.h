@@ -373,6 +373,18 @@ int avbd_download_detected(avbd_ctx* ctx, uint64_t* keys, uint32_t* slots, uint8
 /* is_conflicted per particle after the last recolouring (n bytes). */
 int avbd_download_conflicts(avbd_ctx* ctx, uint8_t* flags);
 
+/* Checkpoint / resume (new; the reference keeps no checkpoints) ---------------
+ * Every mutable part of a built context -- particle state and histories,
+ * E_prev, contact / joint multipliers and stiffnesses, rotation state, the
+ * residual-control word, the dynamic contact records and, with dynamic
+ * contacts, the colouring and the host contact state -- into a caller
+ * buffer of avbd_checkpoint_size() bytes.  Loading it into a context built
+ * from the same scene with the same modes resumes the run: the following
+ * steps reproduce the original run bit for bit.  Synchronises. */
+int avbd_checkpoint_size(avbd_ctx* ctx, uint64_t* bytes);
+int avbd_save_checkpoint(avbd_ctx* ctx, void* buf, uint64_t cap, uint64_t* written);
+int avbd_load_checkpoint(avbd_ctx* ctx, const void* buf, uint64_t size);
+
 /* Learned warm start (new; replaces the inertial / adaptive initial guess of
  * _particleInertialUpdate, simulation/VBDSolver.hpp:208-248, whenever it
  * lowers the incremental potential of the scene).  A per-vertex MLP
