@@ -376,6 +376,7 @@ void alloc_lbvh(avbd_ctx* c, LbvhDev& b, uint32_t n) {
     b.slots = dev_alloc<uint32_t>(c, n);
     b.table = dev_alloc<double>(c, 6 * (size_t)lbvh_table_entries(n));
     b.node_box = dev_alloc<double>(c, 6 * (size_t)nn);
+    b.trav = dev_alloc<TravNode>(c, n > 1 ? n - 1 : 1);
     b.pair_cap = std::max<uint32_t>(1024u, 64u * n);
     b.pairs = dev_alloc<uint32_t>(c, 2 * (size_t)b.pair_cap);
     b.counters = dev_alloc<uint32_t>(c, 2);

@@ -7,6 +7,16 @@
 
 namespace avbd {
 
+// Traversal record of one internal node: both children with their boxes and
+// leaf ranges, so a visit is one contiguous 128-byte read.
+struct alignas(16) TravNode {
+    double box[2][6];      // child boxes (min xyz, max xyz), FP64
+    uint32_t child[2];     // child node ids (leaves are < n)
+    uint32_t lo[2], hi[2]; // sorted-leaf range each child covers
+    uint32_t pad[2];
+};
+static_assert(sizeof(TravNode) == 128, "TravNode is two 64-byte halves");
+
 // Node numbering follows the reference (collision/LBVH.hpp:10-16): leaves
 // [0, n) in sorted order, internal nodes [n, 2n-1), root n (n > 1).
 struct LbvhDev {
@@ -26,6 +36,7 @@ struct LbvhDev {
     uint32_t* slots = nullptr;           // n: primitives grouped by bucket
     double* table = nullptr;        // aligned range unions, 6 per entry (lbvh_table_entries)
     double* node_box = nullptr;     // 6 per node
+    TravNode* trav = nullptr;       // n - 1 internal nodes (record g - n of node g)
     uint32_t* pairs = nullptr;      // candidate leaf pairs (a < b)
     uint32_t pair_cap = 0;
     uint32_t* counters = nullptr;   // [0] pair count, [1] status (bit0 stack overflow)

@@ -16,7 +16,8 @@
 //     min/max are exact, so any evaluation order gives the same bits;
 //   * traversal reports every leaf pair (a < b) whose closed boxes overlap,
 //     the set produced by LBVHTraversal::traverseSelfIterative
-//     (collision/LBVHTraversal.hpp:30-88); one thread walks the tree per leaf.
+//     (collision/LBVHTraversal.hpp:30-88); one thread walks the tree per leaf,
+//     reading a packed record of both children per visit.
 #include "lbvh.hpp"
 
 namespace avbd {
@@ -269,7 +270,8 @@ __device__ __forceinline__ void range_union(const double* __restrict__ table, ui
 __global__ void karras_kernel(const uint64_t* __restrict__ code, const uint32_t* __restrict__ sorted, uint32_t n,
                               uint32_t* __restrict__ left, uint32_t* __restrict__ right,
                               uint32_t* __restrict__ parent, uint32_t* __restrict__ range_hi,
-                              const double* __restrict__ table, double* __restrict__ node_box) {
+                              const double* __restrict__ table, double* __restrict__ node_box,
+                              TravNode* __restrict__ trav) {
     const int N = (int)n;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) parent[N > 1 ? N : 0] = INV;   // the root has no parent
@@ -302,28 +304,43 @@ __global__ void karras_kernel(const uint64_t* __restrict__ code, const uint32_t*
     parent[lc] = g;
     parent[rc] = g;
     range_hi[g] = (uint32_t)max(i, j);
-    double box[6];
-    range_union(table, n, (uint32_t)min(i, j), (uint32_t)max(i, j), box);
-    for (int k = 0; k < 6; k++) node_box[6 * g + k] = box[k];
+    // children cover [lo, gamma] and [gamma + 1, hi]; the node's box is the
+    // union of theirs (min/max are exact: the same bits as one range union)
+    TravNode t;
+    t.child[0] = lc;
+    t.child[1] = rc;
+    t.lo[0] = (uint32_t)min(i, j);
+    t.hi[0] = (uint32_t)gamma;
+    t.lo[1] = (uint32_t)gamma + 1;
+    t.hi[1] = (uint32_t)max(i, j);
+    t.pad[0] = t.pad[1] = 0;
+    range_union(table, n, t.lo[0], t.hi[0], t.box[0]);
+    range_union(table, n, t.lo[1], t.hi[1], t.box[1]);
+    trav[i] = t;
+    for (int k = 0; k < 3; k++) {
+        node_box[6 * g + k] = fmin(t.box[0][k], t.box[1][k]);
+        node_box[6 * g + 3 + k] = fmax(t.box[0][3 + k], t.box[1][3 + k]);
+    }
 }
 
+// Closed-box overlap; every operand is already in registers, so the six
+// tests are evaluated together (no short-circuit chain of dependent reads).
 __device__ __forceinline__ bool overlap(const double* a, const double* b) {
-    return a[0] <= b[3] && a[3] >= b[0] && a[1] <= b[4] && a[4] >= b[1] && a[2] <= b[5] && a[5] >= b[2];
+    return (a[0] <= b[3]) & (a[3] >= b[0]) & (a[1] <= b[4]) & (a[4] >= b[1]) & (a[2] <= b[5]) & (a[5] >= b[2]);
 }
 
 constexpr int TRAVERSE_STACK = 64;
 constexpr int TRAVERSE_BLOCK = 64;
 constexpr int TRAVERSE_LOCAL = 32;   // pairs buffered per thread before the block flush
 
-// One thread per leaf i; descends from the root testing both children of a
-// node at once, keeps the deferred child on a per-thread stack in LDS, and
-// records j for overlapping leaves j > i (subtrees whose last leaf is <= i
-// are skipped).  Pairs are buffered in LDS and written with one global atomic
-// reservation per block (a pair counter hit by every pair serialises at L2).
-__global__ __launch_bounds__(TRAVERSE_BLOCK) void traverse_kernel(uint32_t n, uint32_t root,
-                                                                  const uint32_t* __restrict__ left,
-                                                                  const uint32_t* __restrict__ right,
-                                                                  const uint32_t* __restrict__ range_hi,
+// One thread per leaf i; descends from the root.  A visit reads the node's
+// 128-byte TravNode (both children's boxes and ranges) in one go and tests
+// both children; the deferred child goes to a per-thread stack in LDS.
+// Children whose last leaf is <= i are skipped, leaf children that overlap
+// are recorded as pairs (i, j), j > i.  Pairs are buffered in LDS and written
+// with one global atomic reservation per block (a pair counter hit by every
+// pair serialises at L2).
+__global__ __launch_bounds__(TRAVERSE_BLOCK) void traverse_kernel(uint32_t n, const TravNode* __restrict__ trav,
                                                                   const double* __restrict__ node_box,
                                                                   uint32_t* __restrict__ pairs, uint32_t cap,
                                                                   uint32_t* __restrict__ counters) {
@@ -344,39 +361,38 @@ __global__ __launch_bounds__(TRAVERSE_BLOCK) void traverse_kernel(uint32_t n, ui
         }
         cnt++;
     };
-    if (i < n && n > 1) {
+    if (i + 1 < n) {   // the last leaf has no partner j > i; n > 1 here, so the root n is internal
         double bi[6];
         for (int k = 0; k < 6; k++) bi[k] = node_box[6 * i + k];
-        auto hit = [&](uint32_t c) { return range_hi[c] > i && overlap(bi, node_box + 6 * c); };
         int sp = 0;
-        uint32_t node = root;
-        bool live = hit(root);
-        while (live) {
-            const uint32_t a = left[node], b = right[node];
-            bool ha = hit(a), hb = hit(b);
-            if (ha && a < n) {
-                emit(a);
-                ha = false;
+        uint32_t node = n;
+        while (true) {
+            const TravNode r = trav[node - n];
+            bool h0 = (r.hi[0] > i) & overlap(bi, r.box[0]);
+            bool h1 = (r.hi[1] > i) & overlap(bi, r.box[1]);
+            if (h0 && r.child[0] < n) {
+                emit(r.child[0]);
+                h0 = false;
             }
-            if (hb && b < n) {
-                emit(b);
-                hb = false;
+            if (h1 && r.child[1] < n) {
+                emit(r.child[1]);
+                h1 = false;
             }
-            if (ha && hb) {
+            if (h0 && h1) {
                 if (sp >= TRAVERSE_STACK) {
                     atomicOr(&counters[1], 1u);   // stack overflow: reported, traversal truncated
                     break;
                 }
-                stack_mem[(sp++) * TRAVERSE_BLOCK + threadIdx.x] = b;
-                node = a;
-            } else if (ha) {
-                node = a;
-            } else if (hb) {
-                node = b;
+                stack_mem[(sp++) * TRAVERSE_BLOCK + threadIdx.x] = r.child[1];
+                node = r.child[0];
+            } else if (h0) {
+                node = r.child[0];
+            } else if (h1) {
+                node = r.child[1];
             } else if (sp > 0) {
                 node = stack_mem[(--sp) * TRAVERSE_BLOCK + threadIdx.x];
             } else {
-                live = false;
+                break;
             }
         }
     }
@@ -431,9 +447,9 @@ void launch_lbvh_tree(hipStream_t s, const LbvhDev& b) {
                        b.table, top < 8 ? top : 8u);
     if (top > 8) hipLaunchKernelGGL(upper_table_kernel, dim3(1), dim3(1024), 0, s, n, b.table, 9u, top);
     hipLaunchKernelGGL(karras_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, b.keys_sorted, b.sorted, n, b.left,
-                       b.right, b.parent, b.range_hi, b.table, b.node_box);
-    hipLaunchKernelGGL(traverse_kernel, dim3(nblk(n, TRAVERSE_BLOCK)), dim3(TRAVERSE_BLOCK), 0, s, n,
-                       n > 1 ? n : 0u, b.left, b.right, b.range_hi, b.node_box, b.pairs, b.pair_cap, b.counters);
+                       b.right, b.parent, b.range_hi, b.table, b.node_box, b.trav);
+    hipLaunchKernelGGL(traverse_kernel, dim3(nblk(n, TRAVERSE_BLOCK)), dim3(TRAVERSE_BLOCK), 0, s, n, b.trav,
+                       b.node_box, b.pairs, b.pair_cap, b.counters);
 }
 
 void launch_lbvh_build(hipStream_t s, const LbvhDev& b, const float4* x, const float4* v, double dt) {

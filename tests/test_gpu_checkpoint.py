@@ -18,12 +18,9 @@ def _log(msg):
 
 
 def _state(sim):
-    x = sim.ctx.positions()
-    out = [x]
-    q = sim.ctx.rotations()
-    if q is not None:
-        out.append(np.asarray(q["q"]))
-    return out
+    x, v = sim.ctx.state()
+    q, w = sim.ctx.rotations()
+    return [x, v, q, w]
 
 
 CASES = {
