@@ -62,6 +62,14 @@ struct avbd_ctx {
     LaunchCfg cfg;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // residual mode: the step's parts (see capture_graph), an event after
+    // each check interval, and per check the converged flag in pinned host
+    // memory (written by the check's decision kernel)
+    std::vector<hipGraphExec_t> part_exec;
+    std::vector<hipEvent_t> part_ev;
+    uint32_t* h_conv = nullptr;
+    uint32_t* d_conv = nullptr;
+    std::vector<float> omega_seq;   // Chebyshev weight of every iteration
 
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -394,8 +402,12 @@ double omega_at(uint32_t i, double rho, double omega) {
     return 1;
 }
 
-// Enqueue one time step (VBDSolver::solve_Parallel, simulation/VBDSolver.hpp:48-103).
-void enqueue_step(avbd_ctx* c, bool with_events) {
+// Enqueue part of one time step (VBDSolver::solve_Parallel,
+// simulation/VBDSolver.hpp:48-103): with `head`, the start-of-step work; then
+// iterations [i0, i1) (each followed by its residual check in residual
+// mode); with `tail`, the end-of-step work.  `join` makes the solve stream
+// wait for the concurrent broad phase at the end of this part.
+void enqueue_part(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool head, bool tail, bool join) {
     const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
     hipStream_t s = c->stream;
     // CollisionDetector::detectCollisionsAndRecolor_Parallel runs before the solve.
@@ -404,69 +416,73 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
     // nothing in this step's solve consumes the broad-phase output: after the
     // bounds kernel has read the start-of-step state, the rest of the build
     // and the traversal run on a second stream, concurrently with the solve,
-    // and join at the end of the step.
+    // and join later in the step.
     // (with dynamic contacts the detection round has already run, see detect_round)
     const bool tri_bp = c->lb.n && !c->dynamic;
     const bool fork = tri_bp && c->side_stream;
-    if (tri_bp) {
-        hipStream_t sb = fork ? c->side_stream : s;
-        if (with_events) hip_check(hipEventRecord(c->ev_lb0, s), "hipEventRecord");
-        launch_lbvh_bounds(s, c->lb, c->d.x, c->d.v, c->desc.dt);
-        if (fork) {
-            hip_check(hipEventRecord(c->ev_fork, s), "hipEventRecord");
-            hip_check(hipStreamWaitEvent(sb, c->ev_fork, 0), "hipStreamWaitEvent");
-        }
-        launch_lbvh_tree(sb, c->lb);
-        if (with_events) hip_check(hipEventRecord(c->ev_lb1, sb), "hipEventRecord");
-        if (fork) hip_check(hipEventRecord(c->ev_join, sb), "hipEventRecord");
-    }
     // learned warm start / recording: the start-of-step phase runs as separate
     // launches, then the features, the network and the choice of the start
     const bool pre = c->warm || c->recording;
     WarmIO io = c->wio;
     if (pre) {
-        if (c->damping) launch_eprev(s, c->d, dt);
-        launch_step_begin(s, c->d, dt, g);
-        launch_warm_save(s, c->d, io);
         io.raw = c->recording ? c->d_raw : nullptr;
         io.tgt = c->d_tgt;
         io.norm = c->mlp->d_norm;
-        launch_warm_features(s, c->d, io, c->cfg.lanes, dt, g, c->rows_pad);
-        if (c->warm) {
-            c->mlp->forward((int)c->rows_pad, io.x16);
-            io.out = c->mlp->z32[kMlpLayers - 1];
-            launch_warm_apply(s, c->d, io, dt * dt * g);
-            launch_warm_ip(s, c->d, io, c->damping, dt);
-            launch_warm_choose(s, c->d, io);
-        }
+        if (c->warm) io.out = c->mlp->z32[kMlpLayers - 1];
     }
-    if (c->persistent) {
-        // the whole solve in one launch with grid barriers between phases
-        if (with_events) hip_check(hipEventRecord(c->ev_st0, s), "hipEventRecord");
-        StepArgs a = c->sargs;
-        a.skip_begin = pre ? 1u : 0u;
-        launch_step_persistent(s, a, c->cfg.lanes, c->damping, c->pgrid);
-        if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
-        if (with_events) hip_check(hipEventRecord(c->ev_st1, s), "hipEventRecord");
-        if (fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
-        hip_check(hipGetLastError(), "kernel launch");
-        return;
+    if (head) {
+        if (tri_bp) {
+            hipStream_t sb = fork ? c->side_stream : s;
+            if (with_events) hip_check(hipEventRecord(c->ev_lb0, s), "hipEventRecord");
+            launch_lbvh_bounds(s, c->lb, c->d.x, c->d.v, c->desc.dt);
+            if (fork) {
+                hip_check(hipEventRecord(c->ev_fork, s), "hipEventRecord");
+                hip_check(hipStreamWaitEvent(sb, c->ev_fork, 0), "hipStreamWaitEvent");
+            }
+            launch_lbvh_tree(sb, c->lb);
+            if (with_events) hip_check(hipEventRecord(c->ev_lb1, sb), "hipEventRecord");
+            if (fork) hip_check(hipEventRecord(c->ev_join, sb), "hipEventRecord");
+        }
+        if (pre) {
+            if (c->damping) launch_eprev(s, c->d, dt);
+            launch_step_begin(s, c->d, dt, g);
+            launch_warm_save(s, c->d, io);
+            launch_warm_features(s, c->d, io, c->cfg.lanes, dt, g, c->rows_pad);
+            if (c->warm) {
+                c->mlp->forward((int)c->rows_pad, io.x16);
+                launch_warm_apply(s, c->d, io, dt * dt * g);
+                launch_warm_ip(s, c->d, io, c->damping, dt);
+                launch_warm_choose(s, c->d, io);
+            }
+        }
+        if (c->persistent) {
+            // the whole solve in one launch with grid barriers between phases
+            if (!tail || i0 != 0 || i1 != c->iters()) throw std::logic_error("persistent step enqueued in parts");
+            if (with_events) hip_check(hipEventRecord(c->ev_st0, s), "hipEventRecord");
+            StepArgs a = c->sargs;
+            a.skip_begin = pre ? 1u : 0u;
+            launch_step_persistent(s, a, c->cfg.lanes, c->damping, c->pgrid);
+            if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
+            if (with_events) hip_check(hipEventRecord(c->ev_st1, s), "hipEventRecord");
+            if (fork && join) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
+            hip_check(hipGetLastError(), "kernel launch");
+            return;
+        }
     }
     const uint32_t* d_active = static_cast<const uint32_t*>(c->g_active.p);
     const uint32_t n_active = c->dynamic ? (uint32_t)c->cs.active.size() : 0u;
-    if (!pre) {
+    if (head && !pre) {
         if (c->damping) launch_eprev(s, c->d, dt);
         // dynamic contacts' updateAfterTimeStep reads the start-of-step positions
         if (n_active) launch_dyn_step_begin(s, c->d, d_active, n_active);
         launch_step_begin(s, c->d, dt, g);
     }
     const uint32_t* done = c->rc ? &c->rc->done : nullptr;
-    if (c->rc) launch_residual_reset(s, c->rc);
-    double omega = 1;
-    size_t ev = 0;
+    if (head && c->rc) launch_residual_reset(s, c->rc);
     const uint32_t iters = c->iters();
-    for (uint32_t i = 0; i < iters; i++) {
-        omega = omega_at(i, c->desc.iter_acceleration, omega);
+    size_t ev = (size_t)2 * i0 * c->plan.num_colors;
+    for (uint32_t i = i0; i < i1; i++) {
+        const float omega = c->omega_seq[i];
         for (uint32_t col = 0; col < c->plan.num_colors; col++) {
             const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
             if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
@@ -477,18 +493,23 @@ void enqueue_step(avbd_ctx* c, bool with_events) {
             const uint32_t cb = c->plan.conflict_offsets[col], ce = c->plan.conflict_offsets[col + 1];
             if (ce > cb) launch_commit(s, c->d, c->d_conflicts + cb, ce - cb, done);
         }
-        launch_cheb_dual(s, c->d, (float)omega, done);
+        launch_cheb_dual(s, c->d, omega, done);
         if (n_active) launch_dyn_dual(s, c->d, d_active, n_active, done);
         if (c->rc && (i + 1) % c->res_check_every == 0 && i + 1 < iters)
             launch_residual_check(s, c->d, c->cfg, c->damping, dt, c->rc, (i + 1) / c->res_check_every - 1, i + 1,
-                                  c->res_tol2);
+                                  c->res_tol2, c->d_conv);
     }
-    if (c->rc) launch_residual_finish(s, c->rc, iters);
-    if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
-    launch_velocity(s, c->d, dt);
-    if (fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
+    if (tail) {
+        if (c->rc) launch_residual_finish(s, c->rc, iters);
+        if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
+        launch_velocity(s, c->d, dt);
+    }
+    if (fork && join) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
     hip_check(hipGetLastError(), "kernel launch");
 }
+
+// Enqueue one whole time step.
+void enqueue_step(avbd_ctx* c, bool with_events) { enqueue_part(c, with_events, 0, c->iters(), true, true, true); }
 
 // After a synchronisation: report (and clear) an aborted persistent step.
 void check_abort(avbd_ctx* c) {
@@ -506,21 +527,54 @@ void destroy_graph(avbd_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->graph_exec = nullptr;
     c->graph = nullptr;
+    for (hipGraphExec_t e : c->part_exec) (void)hipGraphExecDestroy(e);
+    c->part_exec.clear();
 }
 
-void capture_graph(avbd_ctx* c) {
-    destroy_graph(c);
+template <class F>
+hipGraphExec_t capture(avbd_ctx* c, F&& body) {
     hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
-        enqueue_step(c, false);
+        body();
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(c->stream, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
     }
-    hip_check(hipStreamEndCapture(c->stream, &c->graph), "hipStreamEndCapture");
-    hip_check(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    hipGraph_t g = nullptr;
+    hip_check(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+    hipGraphExec_t e = nullptr;
+    const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    hip_check(r, "hipGraphInstantiate");
+    return e;
+}
+
+// Residual mode replays a step in parts: the start of the step with the
+// first check interval (the broad phase joins there), one graph per further
+// interval, and the end of the step; the host stops launching intervals
+// once a check reports convergence (avbd_step).
+bool stepped_in_parts(const avbd_ctx* c) { return c->rc && !c->persistent && !c->dynamic && c->h_conv; }
+
+void capture_graph(avbd_ctx* c) {
+    destroy_graph(c);
+    if (stepped_in_parts(c)) {
+        const uint32_t iters = c->iters(), every = c->res_check_every;
+        const uint32_t parts = (iters + every - 1) / every;
+        for (uint32_t k = 0; k < parts; k++) {
+            const uint32_t i0 = k * every, i1 = std::min(iters, i0 + every);
+            c->part_exec.push_back(capture(c, [&] { enqueue_part(c, false, i0, i1, k == 0, false, k == 0); }));
+        }
+        c->part_exec.push_back(capture(c, [&] { enqueue_part(c, false, iters, iters, false, true, false); }));
+        while (c->part_ev.size() < parts) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            c->part_ev.push_back(e);
+        }
+        return;
+    }
+    c->graph_exec = capture(c, [&] { enqueue_step(c, false); });
 }
 
 // One round of CollisionDetector::detectCollisionsAndRecolor_Parallel
@@ -658,6 +712,8 @@ void avbd_destroy(avbd_ctx* c) {
     c->mlp.reset();
     for (void* p : c->allocations) (void)hipFree(p);
     for (auto ev : c->events) (void)hipEventDestroy(ev);
+    for (auto ev : c->part_ev) (void)hipEventDestroy(ev);
+    if (c->h_conv) (void)hipHostFree(c->h_conv);
     if (c->ev_st0) (void)hipEventDestroy(c->ev_st0);
     if (c->ev_st1) (void)hipEventDestroy(c->ev_st1);
     if (c->ev_lb0) (void)hipEventDestroy(c->ev_lb0);
@@ -1307,6 +1363,7 @@ int avbd_build_graph(avbd_ctx* c) {
             std::vector<float> om(c->iters());
             double w = 1;
             for (uint32_t i = 0; i < c->iters(); i++) om[i] = (float)(w = omega_at(i, c->desc.iter_acceleration, w));
+            c->omega_seq = om;
             uint32_t* d_co = dev_alloc<uint32_t>(c, co.size());
             uint32_t* d_oco = dev_alloc<uint32_t>(c, c->plan.oitem_offsets.size());
             upload(c, d_oco, c->plan.oitem_offsets);
@@ -1333,6 +1390,15 @@ int avbd_build_graph(avbd_ctx* c) {
                 c->res_tol2 = c->res_tol * c->res_tol * mg2;
                 c->rc = dev_alloc<ResCtl>(c, 1);
                 hip_check(hipMemsetAsync(c->rc, 0, sizeof(ResCtl), c->stream), "hipMemsetAsync");
+                void* hc = nullptr;
+                hip_check(hipHostMalloc(&hc, kMaxResidualChecks * sizeof(uint32_t),
+                                        hipHostMallocMapped | hipHostMallocCoherent),
+                          "hipHostMalloc");
+                c->h_conv = static_cast<uint32_t*>(hc);
+                std::memset(c->h_conv, 0, kMaxResidualChecks * sizeof(uint32_t));
+                void* dc = nullptr;
+                hip_check(hipHostGetDevicePointer(&dc, hc, 0), "hipHostGetDevicePointer");
+                c->d_conv = static_cast<uint32_t*>(dc);
                 c->sargs.rc = c->rc;
                 c->sargs.check_every = c->res_check_every;
                 c->sargs.tol2 = c->res_tol2;
@@ -1421,6 +1487,25 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
             c->solve_avg_ms = launches ? total / launches : 0;
             c->lbvh_avg_ms = lbvh_total / std::max(1u, n_steps);
             c->solve_launches = launches;
+        } else if (!c->part_exec.empty()) {
+            // residual mode: interval k is launched once the check of interval
+            // k - kPartsAhead has been seen unconverged, so at most
+            // kPartsAhead - 1 intervals of early-exiting launches follow the
+            // converged check instead of the rest of the iteration budget
+            constexpr uint32_t kPartsAhead = 2;
+            const uint32_t parts = (uint32_t)c->part_exec.size() - 1;
+            for (uint32_t st = 0; st < n_steps; st++) {
+                for (uint32_t k = 0; k < parts; k++) {
+                    if (k >= kPartsAhead) {
+                        // interval k - kPartsAhead < parts - 1 ends with check k - kPartsAhead
+                        hip_check(hipEventSynchronize(c->part_ev[k - kPartsAhead]), "hipEventSynchronize");
+                        if (__atomic_load_n(c->h_conv + (k - kPartsAhead), __ATOMIC_ACQUIRE)) break;
+                    }
+                    hip_check(hipGraphLaunch(c->part_exec[k], c->stream), "hipGraphLaunch");
+                    hip_check(hipEventRecord(c->part_ev[k], c->stream), "hipEventRecord");
+                }
+                hip_check(hipGraphLaunch(c->part_exec[parts], c->stream), "hipGraphLaunch");
+            }
         } else if (c->graph_exec) {
             for (uint32_t s = 0; s < n_steps; s++) hip_check(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
         } else {
@@ -2137,6 +2222,8 @@ int avbd_download_detected(avbd_ctx* c, uint64_t* keys, uint32_t* slots, uint8_t
     });
 }
 
+}  // extern "C"
+
 namespace {
 // Checkpoint layout: header, then the device arrays in a fixed order, then
 // (dynamic contacts) the contact records, the colouring and the host
@@ -2196,6 +2283,8 @@ std::vector<uint8_t> ckpt_host_tail(avbd_ctx* c) {
     return t;
 }
 }  // namespace
+
+extern "C" {
 
 int avbd_checkpoint_size(avbd_ctx* c, uint64_t* bytes) {
     return guarded(c, [&] {

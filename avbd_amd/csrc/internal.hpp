@@ -273,10 +273,11 @@ void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint3
 void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const uint32_t* done = nullptr);
 // residual-targeted iteration (per-phase path): reset at step start, one
 // check (residual pass + decision) after iteration `iters_done`, and the
-// bookkeeping at step end
+// bookkeeping at step end; `conv` (optional, host-mapped) receives the
+// converged flag after each check
 void launch_residual_reset(hipStream_t s, ResCtl* rc);
 void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
-                           ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2);
+                           ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2, uint32_t* conv);
 void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters);
 // learned warm start
 void launch_warm_save(hipStream_t s, const DevState& d, const WarmIO& io);

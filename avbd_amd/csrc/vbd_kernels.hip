@@ -603,11 +603,15 @@ __global__ __launch_bounds__(256) void residual_pass_kernel(DevState d, ResCtl* 
     block_atomic_sum(part, &rc->acc[check], red);
 }
 
-__global__ void residual_decide_kernel(ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2) {
-    if (threadIdx.x == 0 && !rc->done && rc->acc[check] <= tol2) {
+__global__ void residual_decide_kernel(ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2,
+                                       uint32_t* conv) {
+    if (threadIdx.x != 0) return;
+    if (!rc->done && rc->acc[check] <= tol2) {
         rc->done = 1;
         rc->last_iters = iters_done;
     }
+    // the host polls this flag while later intervals of the step are queued
+    if (conv) __hip_atomic_store(conv + check, rc->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __global__ void residual_reset_kernel(ResCtl* rc) {
@@ -1221,7 +1225,7 @@ void launch_residual_reset(hipStream_t s, ResCtl* rc) {
 }
 
 void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
-                           ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2) {
+                           ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2, uint32_t* conv) {
 #define AVBD_RES_CASE(LL)                                                   \
     case LL:                                                                 \
         if (damping) residual_dispatch<LL, true>(s, d, dt, rc, check);       \
@@ -1235,7 +1239,7 @@ void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cf
         default: break;
     }
 #undef AVBD_RES_CASE
-    hipLaunchKernelGGL(residual_decide_kernel, dim3(1), dim3(64), 0, s, rc, check, iters_done, tol2);
+    hipLaunchKernelGGL(residual_decide_kernel, dim3(1), dim3(64), 0, s, rc, check, iters_done, tol2, conv);
 }
 
 void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters) {
