@@ -33,6 +33,7 @@ FLAG_NO_GRAPH = 1
 FLAG_SERIAL_BROADPHASE = 2
 FLAG_NO_PERSISTENT = 4
 FLAG_PERSISTENT = 8
+FLAG_SEPARATE_CHEBYSHEV = 16
 
 
 class SceneDesc(C.Structure):
@@ -228,8 +229,8 @@ SIGNATURES = {
     "avbd_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "avbd_num_particles": (C.c_uint32, [_P]),
     "avbd_num_energies": (C.c_uint32, [_P, C.c_int]),
-    "avbd_get_launch_config": (C.c_int, [_P, _U32, _U32]),
-    "avbd_get_step_mode": (C.c_int, [_P, _U32]),
+    "avbd_get_launch_config": (C.c_int, [_P, _U32, _U32, _U32]),
+    "avbd_get_step_mode": (C.c_int, [_P, _U32, _U32]),
     "avbd_set_timing": (C.c_int, [_P, C.c_int]),
     "avbd_get_solve_timing": (C.c_int, [_P, _D, _U32]),
     "avbd_get_broadphase_timing": (C.c_int, [_P, _D]),

@@ -70,6 +70,12 @@ struct avbd_ctx {
     uint32_t* h_conv = nullptr;
     uint32_t* d_conv = nullptr;
     std::vector<float> omega_seq;   // Chebyshev weight of every iteration
+    // Chebyshev / dual update fused into each iteration's last colour phase:
+    // positions alternate between d.x (even iterations) and x_alt
+    bool fuse_cheb = false;
+    float4* x_alt = nullptr;
+    uint32_t* cheb_list = nullptr;  // particles that are not work items of the last colour
+    uint32_t n_cheb = 0;
 
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -87,6 +93,7 @@ struct avbd_ctx {
     uint32_t res_max_iters = 0, res_check_every = 0;
     double res_tol2 = 0;           // tol^2 ||M g||^2
     ResCtl* rc = nullptr;
+    double* res_partial = nullptr; // per-block partial sums of a residual check
 
     // learned warm start (mlp_host.cpp): the network, its in-step buffers,
     // and the recording of training data
@@ -480,27 +487,50 @@ void enqueue_part(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool 
     const uint32_t* done = c->rc ? &c->rc->done : nullptr;
     if (head && c->rc) launch_residual_reset(s, c->rc);
     const uint32_t iters = c->iters();
+    const bool fuse = c->fuse_cheb && !c->dynamic;
     size_t ev = (size_t)2 * i0 * c->plan.num_colors;
     for (uint32_t i = i0; i < i1; i++) {
         const float omega = c->omega_seq[i];
+        // fused: iteration i works on x_cur and extrapolates into x_next
+        DevState di = c->d;
+        float4* x_next = c->d.x;
+        if (fuse) {
+            di.x = (i & 1u) ? c->x_alt : c->d.x;
+            x_next = (i & 1u) ? c->d.x : c->x_alt;
+        }
+        const uint32_t last = c->plan.num_colors - 1;
         for (uint32_t col = 0; col < c->plan.num_colors; col++) {
             const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
             if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
-            launch_solve_color(s, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x, done);
+            if (fuse && col == last)
+                launch_solve_cheb(s, di, c->cfg, b, e - b, c->damping, dt, omega, c->cheb_list, c->n_cheb, x_next,
+                                  done);
+            else
+                launch_solve_color(s, di, c->cfg, b, e - b, c->damping, dt, di.x, done);
             const uint32_t ob = c->plan.oitem_offsets[col], oe = c->plan.oitem_offsets[col + 1];
-            launch_solve_ocolor(s, c->d, ob, oe - ob, dt, done);
+            launch_solve_ocolor(s, di, ob, oe - ob, dt, done);
             if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
             const uint32_t cb = c->plan.conflict_offsets[col], ce = c->plan.conflict_offsets[col + 1];
-            if (ce > cb) launch_commit(s, c->d, c->d_conflicts + cb, ce - cb, done);
+            if (ce > cb) launch_commit(s, di, c->d_conflicts + cb, ce - cb, done);
         }
-        launch_cheb_dual(s, c->d, omega, done);
+        if (!fuse) launch_cheb_dual(s, c->d, omega, done);
         if (n_active) launch_dyn_dual(s, c->d, d_active, n_active, done);
-        if (c->rc && (i + 1) % c->res_check_every == 0 && i + 1 < iters)
-            launch_residual_check(s, c->d, c->cfg, c->damping, dt, c->rc, (i + 1) / c->res_check_every - 1, i + 1,
-                                  c->res_tol2, c->d_conv);
+        if (c->rc && (i + 1) % c->res_check_every == 0 && i + 1 < iters) {
+            DevState dr = c->d;
+            dr.x = x_next;
+            launch_residual_check(s, dr, c->cfg, c->damping, dt, c->rc, c->res_partial,
+                                  (i + 1) / c->res_check_every - 1, i + 1, c->res_tol2, c->d_conv);
+        }
     }
     if (tail) {
         if (c->rc) launch_residual_finish(s, c->rc, iters);
+        // fused: the final positions are in x_alt after an odd number of iterations
+        if (fuse) {
+            if (c->rc && ((c->res_check_every & 1u) || (iters & 1u)))
+                launch_settle(s, c->d.x, c->x_alt, c->d.n, &c->rc->last_iters, 0);
+            else if (!c->rc && (iters & 1u))
+                launch_settle(s, c->d.x, c->x_alt, c->d.n, nullptr, iters);
+        }
         if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
         launch_velocity(s, c->d, dt);
     }
@@ -641,14 +671,16 @@ void detect_round(avbd_ctx* c) {
 void autotune(avbd_ctx* c) {
     const uint32_t lanes_c[] = {4, 8, 16, 32};
     const uint32_t block_c[] = {64, 128, 256};
+    const uint32_t batch_c[] = {1, 2, 4};
     float best = 1e30f;
     LaunchCfg best_cfg = c->cfg;
     const float dt = (float)c->desc.dt;
     for (uint32_t L : lanes_c) {
         if (c->desc.lanes_per_vertex && L != c->desc.lanes_per_vertex) continue;
-        for (uint32_t B : block_c) {
+        for (uint32_t B : block_c)
+        for (uint32_t U : batch_c) {
             if (c->desc.block_size && B != c->desc.block_size) continue;
-            LaunchCfg cfg{L, B};
+            LaunchCfg cfg{L, B, U};
             auto run = [&] {
                 for (uint32_t col = 0; col < c->plan.num_colors; col++) {
                     const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
@@ -1389,6 +1421,7 @@ int avbd_build_graph(avbd_ctx* c) {
                     if (!c->fixed[p]) mg2 += (c->mass[p] * c->desc.g_accel) * (c->mass[p] * c->desc.g_accel);
                 c->res_tol2 = c->res_tol * c->res_tol * mg2;
                 c->rc = dev_alloc<ResCtl>(c, 1);
+                c->res_partial = dev_alloc<double>(c, std::max(1u, residual_partials(c->d)));
                 hip_check(hipMemsetAsync(c->rc, 0, sizeof(ResCtl), c->stream), "hipMemsetAsync");
                 void* hc = nullptr;
                 hip_check(hipHostMalloc(&hc, kMaxResidualChecks * sizeof(uint32_t),
@@ -1408,6 +1441,23 @@ int avbd_build_graph(avbd_ctx* c) {
             c->sargs.g = (float)c->desc.g_accel;
             hip_check(hipEventCreate(&c->ev_st0), "hipEventCreate");
             hip_check(hipEventCreate(&c->ev_st1), "hipEventCreate");
+        }
+        // Chebyshev / dual update fused into the last colour phase (scenes
+        // without oriented particles or conflicted vertices)
+        if (!(c->desc.flags & AVBD_FLAG_SEPARATE_CHEBYSHEV) && c->d.n_rot == 0 && c->d.n_oitems == 0 &&
+            c->plan.num_colors > 0 && c->plan.conflict_list.empty()) {
+            const uint32_t last = c->plan.num_colors - 1;
+            std::vector<uint8_t> in_last(n, 0);
+            for (uint32_t k = c->plan.item_offsets[last]; k < c->plan.item_offsets[last + 1]; k++)
+                in_last[c->plan.items[k].v] = 1;
+            std::vector<uint32_t> rest;
+            for (uint32_t p = 0; p < n; p++)
+                if (!in_last[p]) rest.push_back(p);
+            c->x_alt = dev_alloc<float4>(c, n);
+            c->n_cheb = (uint32_t)rest.size();
+            c->cheb_list = dev_alloc<uint32_t>(c, std::max<size_t>(1, rest.size()));
+            if (!rest.empty()) upload(c, c->cheb_list, rest);
+            c->fuse_cheb = true;
         }
         c->built = true;
 
@@ -1815,16 +1865,18 @@ uint32_t avbd_num_energies(avbd_ctx* c, int type) {
     return 0;
 }
 
-int avbd_get_launch_config(avbd_ctx* c, uint32_t* lanes, uint32_t* block) {
+int avbd_get_launch_config(avbd_ctx* c, uint32_t* lanes, uint32_t* block, uint32_t* batch) {
     return guarded(c, [&] {
         if (lanes) *lanes = c->cfg.lanes;
         if (block) *block = c->persistent ? 256u : c->cfg.block;
+        if (batch) *batch = c->persistent ? 1u : c->cfg.batch;
     });
 }
 
-int avbd_get_step_mode(avbd_ctx* c, uint32_t* persistent_grid) {
+int avbd_get_step_mode(avbd_ctx* c, uint32_t* persistent_grid, uint32_t* fused_chebyshev) {
     if (!c) return AVBD_E_ARG;
     if (persistent_grid) *persistent_grid = c->persistent ? c->pgrid : 0u;
+    if (fused_chebyshev) *fused_chebyshev = (c->fuse_cheb && !c->persistent && !c->dynamic) ? 1u : 0u;
     return AVBD_OK;
 }
 
@@ -2130,6 +2182,7 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         const uint32_t np = (uint32_t)c->prims.size();
         destroy_graph(c);
         c->persistent = false;
+        c->fuse_cheb = false;
         alloc_lbvh(c, c->lbd, np);
         std::vector<uint32_t> ty(np), id(3 * (size_t)np), ob(np);
         std::vector<double> rad(np);

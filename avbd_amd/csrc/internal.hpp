@@ -219,6 +219,7 @@ struct DevState {
 struct LaunchCfg {
     uint32_t lanes = 8;
     uint32_t block = 256;
+    uint32_t batch = 1;   // incidences a lane gathers at once (1, 2 or 4)
 };
 
 // Grid barrier of the persistent step kernel (zeroed once at build time).
@@ -272,12 +273,25 @@ void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint3
                    const uint32_t* done = nullptr);
 void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const uint32_t* done = nullptr);
 // residual-targeted iteration (per-phase path): reset at step start, one
+// last colour phase of an iteration fused with the Chebyshev / dual update
+// (no oriented particles, no conflicts): reads d.x, writes the extrapolated
+// positions of every particle to x_next (cheb_list: the particles that are
+// not work items of this colour)
+void launch_solve_cheb(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
+                       bool damping, float dt, float omega, const uint32_t* cheb_list, uint32_t n_cheb,
+                       float4* x_next, const uint32_t* done);
+// end of a double-buffered step: x = x_alt when the iteration count is odd
+void launch_settle(hipStream_t s, float4* x, const float4* x_alt, uint32_t n, const uint32_t* iters_word,
+                   uint32_t iters);
 // check (residual pass + decision) after iteration `iters_done`, and the
 // bookkeeping at step end; `conv` (optional, host-mapped) receives the
 // converged flag after each check
 void launch_residual_reset(hipStream_t s, ResCtl* rc);
 void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
-                           ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2, uint32_t* conv);
+                           ResCtl* rc, double* partial, uint32_t check, uint32_t iters_done, double tol2,
+                           uint32_t* conv);
+// doubles of per-block partial sums a residual check needs (any lane count)
+uint32_t residual_partials(const DevState& d);
 void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters);
 // learned warm start
 void launch_warm_save(hipStream_t s, const DevState& d, const WarmIO& io);

@@ -55,12 +55,12 @@ __device__ __forceinline__ void load_tet(const TetRec* __restrict__ tets, uint32
 // q = Q.row(local) for local < 3, -(Q.row0 + Q.row1 + Q.row2) for local 3 (the
 // reference's sign = -1 folded into q; every term is linear in q or quadratic).
 // The record (idx, r1..r3) is already loaded; t indexes the damping record.
+// nh_incidence_pos takes the four vertex positions already gathered.
 template <bool DAMP>
-__device__ __forceinline__ void nh_incidence_rec(const float4* x, uint4 idx, float4 r1, float4 r2, float4 r3,
-                                                 uint32_t local, const TetDamp* __restrict__ damp, uint32_t t,
-                                                 float inv_dt, float3& g, Sym3& h) {
+__device__ __forceinline__ void nh_incidence_pos(float3 p0, float3 p1, float3 p2, float3 p3, float4 r1, float4 r2,
+                                                 float4 r3, uint32_t local, const TetDamp* __restrict__ damp,
+                                                 uint32_t t, float inv_dt, float3& g, Sym3& h) {
     // r1 = q00 q01 q02 q10 ; r2 = q11 q12 q20 q21 ; r3 = q22 VL VM alpha
-    const float3 p0 = f3(x[idx.x]), p1 = f3(x[idx.y]), p2 = f3(x[idx.z]), p3 = f3(x[idx.w]);
     const float3 d0 = p0 - p3, d1 = p1 - p3, d2 = p2 - p3;
     const float q00 = r1.x, q01 = r1.y, q02 = r1.z, q10 = r1.w, q11 = r2.x, q12 = r2.y, q20 = r2.z, q21 = r2.w,
                 q22 = r3.x;
@@ -118,6 +118,14 @@ __device__ __forceinline__ void nh_incidence_rec(const float4* x, uint4 idx, flo
         h.a02 += a * (q2 * ff02 + Fq.x * Fq.z);
         h.a12 += a * (q2 * ff12 + Fq.y * Fq.z);
     }
+}
+
+template <bool DAMP>
+__device__ __forceinline__ void nh_incidence_rec(const float4* x, uint4 idx, float4 r1, float4 r2, float4 r3,
+                                                 uint32_t local, const TetDamp* __restrict__ damp, uint32_t t,
+                                                 float inv_dt, float3& g, Sym3& h) {
+    nh_incidence_pos<DAMP>(f3(x[idx.x]), f3(x[idx.y]), f3(x[idx.z]), f3(x[idx.w]), r1, r2, r3, local, damp, t, inv_dt,
+                           g, h);
 }
 
 template <bool DAMP>
@@ -227,15 +235,89 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
     return x * (G / 8u) + min(x, G % 8u) + q;
 }
 
+// The incidences j0, j0 + L, ..., j0 + (U-1) L of one lane (those < cnt):
+// every record is loaded, then every vertex position, then the terms are
+// accumulated in incidence order -- the same order and arithmetic as one
+// incidence at a time, with U times fewer dependent memory round trips.
+template <int L, int U, bool DAMP>
+__device__ __forceinline__ void nh_incidences_batch(const float4* x, const TetRec* __restrict__ irec,
+                                                    const uint32_t* __restrict__ inc,
+                                                    const TetDamp* __restrict__ damp, uint32_t base, uint32_t j0,
+                                                    uint32_t cnt, float inv_dt, float3& g, Sym3& h) {
+    uint4 idx[U];
+    float4 r1[U], r2[U], r3[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (j0 + u * L < cnt) load_tet(irec, base + j0 + u * L, idx[u], r1[u], r2[u], r3[u]);
+    float3 P[U][4];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (j0 + u * L < cnt) {
+            P[u][0] = f3(x[idx[u].x]);
+            P[u][1] = f3(x[idx[u].y]);
+            P[u][2] = f3(x[idx[u].z]);
+            P[u][3] = f3(x[idx[u].w & 0x3fffffffu]);
+        }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (j0 + u * L < cnt)
+            nh_incidence_pos<DAMP>(P[u][0], P[u][1], P[u][2], P[u][3], r1[u], r2[u], r3[u], idx[u].w >> 30, damp,
+                                   DAMP ? (inc[base + j0 + u * L] >> 2) : 0u, inv_dt, g, h);
+}
+
+// The tet incidences of one work item summed over the L lanes of its group
+// (every lane gets the sums; every lane of the group must call it).  U > 1
+// gathers U incidences of a lane at once (nh_incidences_batch).
+template <int L, bool DAMP, int U>
+__device__ __forceinline__ void item_gradient(const VItem& it, uint32_t lane, const uint32_t* __restrict__ inc,
+                                              const TetRec* __restrict__ irec, const TetDamp* __restrict__ damp,
+                                              const float4* x, float inv_dt, float3& g, Sym3& h) {
+    const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
+    g = make_float3(0.f, 0.f, 0.f);
+    h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if constexpr (U > 1) {
+        for (uint32_t j0 = lane; j0 < cnt; j0 += U * L)
+            nh_incidences_batch<L, U, DAMP>(x, irec, inc, damp, it.inc_begin, j0, cnt, inv_dt, g, h);
+    } else {
+        for (uint32_t j = lane; j < cnt; j += L)
+            nh_incidence_k<DAMP>(x, irec, inc, damp, it.inc_begin + j, inv_dt, g, h);
+    }
+    g.x = group_sum<L>(g.x);
+    g.y = group_sum<L>(g.y);
+    g.z = group_sum<L>(g.z);
+    h.a00 = group_sum<L>(h.a00);
+    h.a11 = group_sum<L>(h.a11);
+    h.a22 = group_sum<L>(h.a22);
+    h.a01 = group_sum<L>(h.a01);
+    h.a02 = group_sum<L>(h.a02);
+    h.a12 = group_sum<L>(h.a12);
+}
+
+// The group leader's part of the vertex block solve: ground AL terms,
+// dynamic contacts, inertia, and the 3x3 solve; returns the new position.
+template <bool DYN>
+__device__ __forceinline__ float4 item_solve(const VItem& it, float4 xv, float4 yv, const GroundRec& gr, float3 g,
+                                             Sym3 h, const float4* x, const DynArgs& da, float inv_dt2) {
+    const float3 p = f3(xv);
+    if (it.ground >= 0) ground_accumulate(gr, p, g, h);
+    if constexpr (DYN) dyn_add3(da, x, it.v, g, h);
+    const float md = xv.w * inv_dt2;   // m / dt^2
+    h.a00 += md;
+    h.a11 += md;
+    h.a22 += md;
+    const float3 b = make_float3(-md * (p.x - yv.x) - g.x, -md * (p.y - yv.y) - g.y, -md * (p.z - yv.z) - g.z);
+    const float3 dx = ldlt3(h, b);
+    return make_float4(p.x + dx.x, p.y + dx.y, p.z + dx.z, xv.w);
+}
+
 // One vertex block solve (_solveParticle<3>, simulation/VBDSolver.hpp:285-449,487-510)
 // by the L lanes of a group; every lane of the group must call it.
-template <int L, bool DAMP, bool DYN = false>
+template <int L, bool DAMP, bool DYN = false, int U = 1>
 __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const uint32_t* __restrict__ inc,
                                            const TetRec* __restrict__ irec, const TetDamp* __restrict__ damp,
                                            float4* x, const float4* __restrict__ y,
                                            const GroundRec* __restrict__ ground, float4* x_out, float4* x_buf,
                                            float inv_dt, float inv_dt2, const DynArgs& da) {
-    const uint32_t cnt = it.inc_count & ~VITEM_CONFLICT;
     // the group leader's own state is loaded first, so that its latency
     // overlaps the incidence gathers instead of following the reduction
     float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), yv = xv;
@@ -246,35 +328,17 @@ __device__ __forceinline__ void solve_item(const VItem& it, uint32_t lane, const
         yv = y[it.v];
         if (it.ground >= 0) gr = ground[it.ground];
     }
-    float3 g = make_float3(0.f, 0.f, 0.f);
-    Sym3 h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (uint32_t j = lane; j < cnt; j += L) nh_incidence_k<DAMP>(x, irec, inc, damp, it.inc_begin + j, inv_dt, g, h);
-    g.x = group_sum<L>(g.x);
-    g.y = group_sum<L>(g.y);
-    g.z = group_sum<L>(g.z);
-    h.a00 = group_sum<L>(h.a00);
-    h.a11 = group_sum<L>(h.a11);
-    h.a22 = group_sum<L>(h.a22);
-    h.a01 = group_sum<L>(h.a01);
-    h.a02 = group_sum<L>(h.a02);
-    h.a12 = group_sum<L>(h.a12);
+    float3 g;
+    Sym3 h;
+    item_gradient<L, DAMP, U>(it, lane, inc, irec, damp, x, inv_dt, g, h);
     if (!lead) return;
-    const float3 p = f3(xv);
-    if (it.ground >= 0) ground_accumulate(gr, p, g, h);
-    if constexpr (DYN) dyn_add3(da, x, it.v, g, h);
-    const float md = xv.w * inv_dt2;   // m / dt^2
-    h.a00 += md;
-    h.a11 += md;
-    h.a22 += md;
-    const float3 b = make_float3(-md * (p.x - yv.x) - g.x, -md * (p.y - yv.y) - g.y, -md * (p.z - yv.z) - g.z);
-    const float3 dx = ldlt3(h, b);
-    const float4 nx = make_float4(p.x + dx.x, p.y + dx.y, p.z + dx.z, xv.w);
+    const float4 nx = item_solve<DYN>(it, xv, yv, gr, g, h, x, da, inv_dt2);
     if (it.inc_count & VITEM_CONFLICT) x_buf[it.v] = nx;
     else x_out[it.v] = nx;
 }
 
-template <int L, bool DAMP, bool DYN>
-__global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restrict__ items, uint32_t n_items,
+template <int L, bool DAMP, bool DYN, int U>
+__global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_color_kernel(const VItem* __restrict__ items, uint32_t n_items,
                                                            const uint32_t* __restrict__ inc,
                                                            const TetRec* __restrict__ irec,
                                                            const TetDamp* __restrict__ damp, float4* x,
@@ -287,7 +351,7 @@ __global__ __launch_bounds__(1024) void solve_color_kernel(const VItem* __restri
     const uint32_t item = tid / L;
     const uint32_t lane = tid % L;
     if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)
-    solve_item<L, DAMP, DYN>(items[item], lane, inc, irec, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2, da);
+    solve_item<L, DAMP, DYN, U>(items[item], lane, inc, irec, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2, da);
 }
 
 // E_prev = 0.5 (F_prev^T F_prev - I) from previous positions
@@ -415,16 +479,52 @@ __global__ void commit_kernel(DevState d, const uint32_t* __restrict__ list, uin
     if (k < count) commit_one(d, list[k]);
 }
 
-// Chebyshev step (positions and orientations) and the dual / stiffness update
-// of the particle's contact and joint records.
-__device__ __forceinline__ void cheb_dual_one(const DevState& d, uint32_t p, float omega) {
-    float4 xp = d.x[p];
-    const float4 xl = d.x_last[p], xll = d.x_lastlast[p];
-    // _particleChebyshevAcceleration (VBDSolver.hpp:513-536)
+// _particleChebyshevAcceleration (VBDSolver.hpp:513-536) of one position.
+__device__ __forceinline__ float4 cheb_apply(float4 xp, float4 xll, float omega) {
     xp.x = omega * (xp.x - xll.x) + xll.x;
     xp.y = omega * (xp.y - xll.y) + xll.y;
     xp.z = omega * (xp.z - xll.z) + xll.z;
-    d.x[p] = xp;
+    return xp;
+}
+
+// CollisionConstraintEnergySolver::updateAfterIteration (CollisionConstraintEnergySolver.hpp:60-134)
+// of one ground record at the extrapolated position xp.
+__device__ __forceinline__ void ground_dual_update(GroundRec& c, float4 xp) {
+    const float Cn = -xp.y, Ct = xp.x - c.cp_x, Cb = xp.z - c.cp_z;
+    const float Ccorr = Cn - kAlpha * c.C_n_prev;
+    const float lnp = c.k_n * Ccorr + c.lambda_n;
+    const float ltp = c.k_t * Ct + c.lambda_t;
+    const float lbp = c.k_b * Cb + c.lambda_b;
+    if (lnp > 0.f) c.k_n += kBeta * fabsf(Ccorr);
+    c.lambda_n = fmaxf(0.f, lnp);
+    const float mag = sqrtf(ltp * ltp + lbp * lbp);
+    float lmax = ((c.flags & GF_USE_STATIC) ? c.mu_s : c.mu_k) * c.lambda_n;
+    if (mag < lmax) {
+        c.k_t += kBeta * fabsf(Ct);
+        c.k_b += kBeta * fabsf(Cb);
+        c.lambda_t = ltp;
+        c.lambda_b = lbp;
+        c.flags |= GF_UNCLAMPED;
+    } else {
+        if (c.flags & GF_USE_STATIC) {
+            c.flags &= ~GF_USE_STATIC;
+            lmax = c.mu_k * lnp;
+        }
+        c.lambda_t = ltp / (mag + 1e-8f) * lmax;
+        c.lambda_b = lbp / (mag + 1e-8f) * lmax;
+        c.flags &= ~GF_UNCLAMPED;
+    }
+}
+
+// Chebyshev step (positions and orientations) and the dual / stiffness update
+// of the particle's contact and joint records; the position is read from
+// x_in and written to x_out (the same buffer unless the step is fused into
+// the last colour phase, solve_cheb_kernel).
+__device__ __forceinline__ void cheb_dual_io(const DevState& d, uint32_t p, float omega, const float4* x_in,
+                                             float4* x_out) {
+    const float4 xl = d.x_last[p], xll = d.x_lastlast[p];
+    const float4 xp = cheb_apply(x_in[p], xll, omega);
+    x_out[p] = xp;
     d.x_lastlast[p] = xl;
     d.x_last[p] = xp;
     const int32_t r = d.n_rot ? d.rot_of[p] : -1;
@@ -432,41 +532,86 @@ __device__ __forceinline__ void cheb_dual_one(const DevState& d, uint32_t p, flo
     if (r >= 0) q = rigid::rotation_chebyshev(d, (uint32_t)r, omega);
     const int32_t gi = d.ground_of[p];
     if (gi >= 0) {
-        // CollisionConstraintEnergySolver::updateAfterIteration (CollisionConstraintEnergySolver.hpp:60-134)
         GroundRec c = d.ground[gi];
-        const float Cn = -xp.y, Ct = xp.x - c.cp_x, Cb = xp.z - c.cp_z;
-        const float Ccorr = Cn - kAlpha * c.C_n_prev;
-        const float lnp = c.k_n * Ccorr + c.lambda_n;
-        const float ltp = c.k_t * Ct + c.lambda_t;
-        const float lbp = c.k_b * Cb + c.lambda_b;
-        if (lnp > 0.f) c.k_n += kBeta * fabsf(Ccorr);
-        c.lambda_n = fmaxf(0.f, lnp);
-        const float mag = sqrtf(ltp * ltp + lbp * lbp);
-        float lmax = ((c.flags & GF_USE_STATIC) ? c.mu_s : c.mu_k) * c.lambda_n;
-        if (mag < lmax) {
-            c.k_t += kBeta * fabsf(Ct);
-            c.k_b += kBeta * fabsf(Cb);
-            c.lambda_t = ltp;
-            c.lambda_b = lbp;
-            c.flags |= GF_UNCLAMPED;
-        } else {
-            if (c.flags & GF_USE_STATIC) {
-                c.flags &= ~GF_USE_STATIC;
-                lmax = c.mu_k * lnp;
-            }
-            c.lambda_t = ltp / (mag + 1e-8f) * lmax;
-            c.lambda_b = lbp / (mag + 1e-8f) * lmax;
-            c.flags &= ~GF_UNCLAMPED;
-        }
+        ground_dual_update(c, xp);
         d.ground[gi] = c;
     }
     if (r >= 0) rigid::records_dual(d, make_float3(xp.x, xp.y, xp.z), q, (uint32_t)r);
+}
+
+__device__ __forceinline__ void cheb_dual_one(const DevState& d, uint32_t p, float omega) {
+    cheb_dual_io(d, p, omega, d.x, d.x);
 }
 
 __global__ void cheb_dual_kernel(DevState d, float omega, const uint32_t* __restrict__ done) {
     if (done && *done) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < d.n) cheb_dual_one(d, p, omega);
+}
+
+// The last colour phase of an iteration fused with the Chebyshev / dual
+// update that follows it (scenes without oriented particles or conflicts).
+// Positions are double-buffered across iterations: this iteration's solves
+// read and write d.x; the extrapolated positions go to x_next, which the
+// next iteration uses as d.x.  Per XCD the grid holds `sx` solve blocks and
+// then `cx` Chebyshev blocks (hardware block b runs on XCD b mod 8):
+//   * a solve group computes its vertex of this colour and its leader
+//     extrapolates that result straight into x_next and updates its ground
+//     record, exactly as cheb_dual_io would after the phase;
+//   * a Chebyshev thread extrapolates one vertex of the other colours (and
+//     the unsolved ones) from d.x into x_next.
+// d.x is not written, so every solve still reads its neighbours' positions
+// before extrapolation, as with a separate Chebyshev launch.
+template <int L, bool DAMP, int U>
+__global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_cheb_kernel(
+    DevState d, const VItem* __restrict__ items, uint32_t n_items, uint32_t sx, uint32_t cx,
+    const uint32_t* __restrict__ cheb_list, uint32_t n_cheb, float4* __restrict__ x_next, float omega, float inv_dt,
+    float inv_dt2, const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    const uint32_t xcd = blockIdx.x % 8u, q = blockIdx.x / 8u;
+    if (q < sx) {
+        const uint32_t tid = (xcd * sx + q) * blockDim.x + threadIdx.x;
+        const uint32_t item = tid / L, lane = tid % L;
+        if (item >= n_items) return;   // whole L-lane groups exit together
+        const VItem it = items[item];
+        const uint32_t v = it.v;
+        float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), yv = xv, xl = xv, xll = xv;
+        GroundRec gr{};
+        const bool lead = lane == 0;
+        if (lead) {
+            xv = d.x[v];
+            yv = d.y[v];
+            xl = d.x_last[v];
+            xll = d.x_lastlast[v];
+            if (it.ground >= 0) gr = d.ground[it.ground];
+        }
+        float3 g;
+        Sym3 h;
+        item_gradient<L, DAMP, U>(it, lane, d.inc, d.irec, d.damp, d.x, inv_dt, g, h);
+        if (!lead) return;
+        const float4 xp = cheb_apply(item_solve<false>(it, xv, yv, gr, g, h, d.x, DynArgs{}, inv_dt2), xll, omega);
+        x_next[v] = xp;
+        d.x_lastlast[v] = xl;
+        d.x_last[v] = xp;
+        if (it.ground >= 0) {
+            ground_dual_update(gr, xp);
+            d.ground[it.ground] = gr;
+        }
+    } else {
+        const uint32_t k = (xcd * cx + (q - sx)) * blockDim.x + threadIdx.x;
+        if (k < n_cheb) cheb_dual_io(d, cheb_list[k], omega, d.x, x_next);
+    }
+}
+
+// End of a step whose iterations ran on two position buffers: when an odd
+// number of iterations ran (iters_word, or `iters` when it is null), the
+// final positions are in x_alt.
+__global__ void settle_kernel(float4* __restrict__ x, const float4* __restrict__ x_alt, uint32_t n,
+                              const uint32_t* __restrict__ iters_word, uint32_t iters) {
+    const uint32_t it = iters_word ? *iters_word : iters;
+    if ((it & 1u) == 0) return;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) x[p] = x_alt[p];
 }
 
 // _particleVelocityUpdate (VBDSolver.hpp:538-566) for one particle.
@@ -590,9 +735,22 @@ __device__ __forceinline__ void block_atomic_sum(double v, double* dst, double* 
     if (threadIdx.x == 0) atomicAdd(dst, red[0]);
 }
 
+// Sum of v over a block of 256 threads (fixed tree), valid in thread 0.
+__device__ __forceinline__ double block_sum256(double v, double* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// Each block writes its partial sum of |r_v|^2; residual_decide_kernel adds
+// the partials in a fixed order (no contended atomics, reproducible sums).
 template <int L, bool DAMP>
-__global__ __launch_bounds__(256) void residual_pass_kernel(DevState d, ResCtl* rc, uint32_t check, float inv_dt,
-                                                            float inv_dt2) {
+__global__ __launch_bounds__(256) void residual_pass_kernel(DevState d, ResCtl* rc, double* __restrict__ partial,
+                                                            float inv_dt, float inv_dt2) {
     if (rc->done) return;
     __shared__ double red[256];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -600,18 +758,32 @@ __global__ __launch_bounds__(256) void residual_pass_kernel(DevState d, ResCtl* 
     double part = 0;
     if (item < d.n_items) part = residual_item<L, DAMP>(d.items[item], lane, d, inv_dt, inv_dt2);
     if (tid < d.n_oitems) part += residual_oitem(d, d.oitems[tid], inv_dt2);
-    block_atomic_sum(part, &rc->acc[check], red);
+    const double s = block_sum256(part, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ void residual_decide_kernel(ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2,
-                                       uint32_t* conv) {
-    if (threadIdx.x != 0) return;
-    if (!rc->done && rc->acc[check] <= tol2) {
-        rc->done = 1;
-        rc->last_iters = iters_done;
+// One block of 256: ||r||^2 of this check from the partials, the decision,
+// and the converged flag for the host.
+__global__ __launch_bounds__(256) void residual_decide_kernel(ResCtl* rc, const double* __restrict__ partial,
+                                                              uint32_t n_partial, uint32_t check, uint32_t iters_done,
+                                                              double tol2, uint32_t* conv) {
+    __shared__ double red[256];
+    const bool was_done = rc->done != 0;
+    if (!was_done) {
+        double v = 0;
+        for (uint32_t k = threadIdx.x; k < n_partial; k += 256) v += partial[k];
+        const double total = block_sum256(v, red);
+        if (threadIdx.x == 0) {
+            rc->acc[check] = total;
+            if (total <= tol2) {
+                rc->done = 1;
+                rc->last_iters = iters_done;
+            }
+        }
     }
     // the host polls this flag while later intervals of the step are queued
-    if (conv) __hip_atomic_store(conv + check, rc->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0 && conv)
+        __hip_atomic_store(conv + check, was_done || rc->done ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __global__ void residual_reset_kernel(ResCtl* rc) {
@@ -1165,25 +1337,29 @@ __global__ __launch_bounds__(256) void stats_oitems_kernel(DevState d, double* _
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 template <int L, bool DAMP>
-void solve_dispatch(hipStream_t s, const DevState& d, uint32_t block, uint32_t begin, uint32_t count, float dt,
-                    float4* x_out, const uint32_t* done) {
+void solve_dispatch(hipStream_t s, const DevState& d, uint32_t block, uint32_t batch, uint32_t begin, uint32_t count,
+                    float dt, float4* x_out, const uint32_t* done) {
     const uint32_t threads = count * L;
-    if (d.dyn_off)
-        hipLaunchKernelGGL((solve_color_kernel<L, DAMP, true>), dim3(blocks_for(threads, block)), dim3(block), 0, s,
-                           d.items + begin, count, d.inc, d.irec, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
-                           1.0f / dt, 1.0f / (dt * dt), done, dyn_args(d));
-    else
-        hipLaunchKernelGGL((solve_color_kernel<L, DAMP, false>), dim3(blocks_for(threads, block)), dim3(block), 0,
-                           s, d.items + begin, count, d.inc, d.irec, d.damp, d.x, d.y, d.ground, x_out, d.x_buf,
-                           1.0f / dt, 1.0f / (dt * dt), done, dyn_args(d));
+    const dim3 grid(blocks_for(threads, block)), blk(block);
+#define AVBD_SOLVE_ARGS \
+    d.items + begin, count, d.inc, d.irec, d.damp, d.x, d.y, d.ground, x_out, d.x_buf, 1.0f / dt, 1.0f / (dt * dt), \
+        done, dyn_args(d)
+    if (block > 256) batch = 1;   // the batched variants are built for blocks of <= 256 threads
+    if (d.dyn_off) hipLaunchKernelGGL((solve_color_kernel<L, DAMP, true, 1>), grid, blk, 0, s, AVBD_SOLVE_ARGS);
+    else if (batch >= 4) hipLaunchKernelGGL((solve_color_kernel<L, DAMP, false, 4>), grid, blk, 0, s, AVBD_SOLVE_ARGS);
+    else if (batch >= 2) hipLaunchKernelGGL((solve_color_kernel<L, DAMP, false, 2>), grid, blk, 0, s, AVBD_SOLVE_ARGS);
+    else hipLaunchKernelGGL((solve_color_kernel<L, DAMP, false, 1>), grid, blk, 0, s, AVBD_SOLVE_ARGS);
+#undef AVBD_SOLVE_ARGS
 }
 
 template <int L, bool DAMP>
-void residual_dispatch(hipStream_t s, const DevState& d, float dt, ResCtl* rc, uint32_t check) {
+uint32_t residual_dispatch(hipStream_t s, const DevState& d, float dt, ResCtl* rc, double* partial) {
     const uint32_t threads = max(d.n_items * L, d.n_oitems);
-    if (threads == 0) return;
-    hipLaunchKernelGGL((residual_pass_kernel<L, DAMP>), dim3(blocks_for(threads, 256)), dim3(256), 0, s, d, rc,
-                       check, 1.0f / dt, 1.0f / (dt * dt));
+    if (threads == 0) return 0;
+    const uint32_t nb = blocks_for(threads, 256);
+    hipLaunchKernelGGL((residual_pass_kernel<L, DAMP>), dim3(nb), dim3(256), 0, s, d, rc, partial, 1.0f / dt,
+                       1.0f / (dt * dt));
+    return nb;
 }
 
 template <int L, bool DAMP>
@@ -1224,12 +1400,16 @@ void launch_residual_reset(hipStream_t s, ResCtl* rc) {
     hipLaunchKernelGGL(residual_reset_kernel, dim3(1), dim3(64), 0, s, rc);
 }
 
+uint32_t residual_partials(const DevState& d) { return blocks_for(max(d.n_items * 32u, d.n_oitems), 256); }
+
 void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
-                           ResCtl* rc, uint32_t check, uint32_t iters_done, double tol2, uint32_t* conv) {
-#define AVBD_RES_CASE(LL)                                                   \
-    case LL:                                                                 \
-        if (damping) residual_dispatch<LL, true>(s, d, dt, rc, check);       \
-        else residual_dispatch<LL, false>(s, d, dt, rc, check);              \
+                           ResCtl* rc, double* partial, uint32_t check, uint32_t iters_done, double tol2,
+                           uint32_t* conv) {
+    uint32_t nb = 0;
+#define AVBD_RES_CASE(LL)                                                          \
+    case LL:                                                                        \
+        nb = damping ? residual_dispatch<LL, true>(s, d, dt, rc, partial)           \
+                     : residual_dispatch<LL, false>(s, d, dt, rc, partial);         \
         break;
     switch (cfg.lanes) {
         AVBD_RES_CASE(4)
@@ -1239,7 +1419,8 @@ void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cf
         default: break;
     }
 #undef AVBD_RES_CASE
-    hipLaunchKernelGGL(residual_decide_kernel, dim3(1), dim3(64), 0, s, rc, check, iters_done, tol2, conv);
+    hipLaunchKernelGGL(residual_decide_kernel, dim3(1), dim3(256), 0, s, rc, partial, nb, check, iters_done, tol2,
+                       conv);
 }
 
 void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters) {
@@ -1268,8 +1449,8 @@ void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, 
     if (count == 0) return;
 #define AVBD_SOLVE_CASE(LL)                                                                \
     case LL:                                                                                \
-        if (damping) solve_dispatch<LL, true>(s, d, cfg.block, begin, count, dt, x_out, done); \
-        else solve_dispatch<LL, false>(s, d, cfg.block, begin, count, dt, x_out, done);      \
+        if (damping) solve_dispatch<LL, true>(s, d, cfg.block, cfg.batch, begin, count, dt, x_out, done); \
+        else solve_dispatch<LL, false>(s, d, cfg.block, cfg.batch, begin, count, dt, x_out, done);      \
         break;
     switch (cfg.lanes) {
         AVBD_SOLVE_CASE(4)
@@ -1279,6 +1460,46 @@ void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, 
         default: break;
     }
 #undef AVBD_SOLVE_CASE
+}
+
+void launch_solve_cheb(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
+                       bool damping, float dt, float omega, const uint32_t* cheb_list, uint32_t n_cheb,
+                       float4* x_next, const uint32_t* done) {
+    uint32_t block = cfg.block, batch = cfg.batch;
+    if (block > 256) batch = 1;   // the batched variants are built for blocks of <= 256 threads
+    const uint32_t L = cfg.lanes;
+    const uint32_t sx = (blocks_for(count * L, block) + 7u) / 8u, cx = (blocks_for(n_cheb, block) + 7u) / 8u;
+    if (sx + cx == 0) return;
+    const dim3 grid(8u * (sx + cx)), blk(block);
+    const float inv_dt = 1.0f / dt, inv_dt2 = 1.0f / (dt * dt);
+#define AVBD_CHEB_LAUNCH(LL, DD, UU)                                                                           \
+    hipLaunchKernelGGL((solve_cheb_kernel<LL, DD, UU>), grid, blk, 0, s, d, d.items + begin, count, sx, cx,     \
+                       cheb_list, n_cheb, x_next, omega, inv_dt, inv_dt2, done)
+#define AVBD_CHEB_U(LL, DD)                          \
+    if (batch >= 4) AVBD_CHEB_LAUNCH(LL, DD, 4);     \
+    else if (batch >= 2) AVBD_CHEB_LAUNCH(LL, DD, 2); \
+    else AVBD_CHEB_LAUNCH(LL, DD, 1);
+#define AVBD_CHEB_CASE(LL)                     \
+    case LL:                                    \
+        if (damping) { AVBD_CHEB_U(LL, true) }  \
+        else { AVBD_CHEB_U(LL, false) }         \
+        break;
+    switch (L) {
+        AVBD_CHEB_CASE(4)
+        AVBD_CHEB_CASE(8)
+        AVBD_CHEB_CASE(16)
+        AVBD_CHEB_CASE(32)
+        default: break;
+    }
+#undef AVBD_CHEB_CASE
+#undef AVBD_CHEB_U
+#undef AVBD_CHEB_LAUNCH
+}
+
+void launch_settle(hipStream_t s, float4* x, const float4* x_alt, uint32_t n, const uint32_t* iters_word,
+                   uint32_t iters) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(settle_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, x_alt, n, iters_word, iters);
 }
 
 void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count, const uint32_t* done) {

@@ -55,7 +55,7 @@ class SimulationContext:
     def __init__(self, dt: float, g_accel: float = 9.81, solver_iters: int = 20,
                  iter_acceleration: float = 0.95, particle_capacity: int = 0, energy_capacity: int = 0,
                  device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0, use_graph: bool = True,
-                 serial_broadphase: bool = False, persistent: bool = False):
+                 serial_broadphase: bool = False, persistent: bool = False, separate_chebyshev: bool = False):
         desc = _lib.SceneDesc()
         desc.particle_capacity = particle_capacity
         desc.energy_capacity = energy_capacity
@@ -64,7 +64,8 @@ class SimulationContext:
         desc.solver_iters = solver_iters
         desc.iter_acceleration = iter_acceleration
         desc.flags = ((0 if use_graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_SERIAL_BROADPHASE if serial_broadphase else 0)
-                      | (_lib.FLAG_PERSISTENT if persistent else 0))
+                      | (_lib.FLAG_PERSISTENT if persistent else 0)
+                      | (_lib.FLAG_SEPARATE_CHEBYSHEV if separate_chebyshev else 0))
         desc.device = device
         desc.lanes_per_vertex = lanes_per_vertex
         desc.block_size = block_size
@@ -425,13 +426,19 @@ class SimulationContext:
 
     def persistent_grid(self) -> int:
         g = C.c_uint32()
-        self._chk(lib.avbd_get_step_mode(self.handle, C.byref(g)))
+        self._chk(lib.avbd_get_step_mode(self.handle, C.byref(g), None))
         return g.value
 
+    def fused_chebyshev(self) -> bool:
+        f = C.c_uint32()
+        self._chk(lib.avbd_get_step_mode(self.handle, None, C.byref(f)))
+        return bool(f.value)
+
     def launch_config(self):
-        L, B = C.c_uint32(), C.c_uint32()
-        self._chk(lib.avbd_get_launch_config(self.handle, C.byref(L), C.byref(B)))
-        return L.value, B.value
+        """(lanes per vertex, block size, incidences gathered at once per lane)."""
+        L, B, U = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._chk(lib.avbd_get_launch_config(self.handle, C.byref(L), C.byref(B), C.byref(U)))
+        return L.value, B.value, U.value
 
     def set_timing(self, enable: bool) -> None:
         self._chk(lib.avbd_set_timing(self.handle, int(enable)))
@@ -519,13 +526,13 @@ class Simulation:
 
     def __init__(self, scene: Scene, device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0,
                  use_graph: bool = True, serial_broadphase: bool = False, persistent: bool = False,
-                 residual_target: Optional[tuple] = None):
+                 residual_target: Optional[tuple] = None, separate_chebyshev: bool = False):
         self.scene = scene
         self.ctx = SimulationContext(dt=scene.dt, g_accel=scene.g, solver_iters=scene.iters,
                                      iter_acceleration=scene.rho, device=device,
                                      lanes_per_vertex=lanes_per_vertex, block_size=block_size,
                                      use_graph=use_graph, serial_broadphase=serial_broadphase,
-                                     persistent=persistent)
+                                     persistent=persistent, separate_chebyshev=separate_chebyshev)
         self.solver = VBDSolver(self.ctx, scene.iters, scene.rho)
         if residual_target is not None:
             self.ctx.set_residual_target(*residual_target)

@@ -129,8 +129,8 @@ def main():
     sim = Simulation(sc)
     sim.setup()
     t_setup = time.perf_counter() - t_setup
-    lanes, block = sim.ctx.launch_config()
-    print(f"[bench] setup {t_setup:.2f}s, launch config lanes={lanes} block={block}", file=sys.stderr, flush=True)
+    lanes, block, batch = sim.ctx.launch_config()
+    print(f"[bench] setup {t_setup:.2f}s, launch config lanes={lanes} block={block} batch={batch}", file=sys.stderr, flush=True)
 
     dynamic = sc.contact_margin is not None
     if dynamic:
@@ -305,7 +305,8 @@ def main():
         "step_breakdown_us": breakdown,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "launch_config": {"lanes_per_vertex": lanes, "block_size": block, "persistent_grid": pgrid},
+        "launch_config": {"lanes_per_vertex": lanes, "block_size": block, "gather_batch": batch,
+                          "persistent_grid": pgrid, "fused_chebyshev": sim.ctx.fused_chebyshev()},
     }
     if dyn_info:
         out["dynamic_contacts"] = dyn_info

@@ -68,6 +68,10 @@ typedef enum {
                                        phase replayed from a hipGraph */
 #define AVBD_FLAG_SERIAL_BROADPHASE 2u  /* run the triangle broad phase on the solve stream
                                            instead of concurrently on a second stream */
+#define AVBD_FLAG_SEPARATE_CHEBYSHEV 16u /* run the Chebyshev / dual update of every iteration as
+                                            its own launch.  By default, in scenes without oriented
+                                            particles, it is fused into the iteration's last colour
+                                            phase over two position buffers (identical results) */
 
 /* SimulationContext capacities + SimulationParams + VBDSolver arguments.
  * Reference: simulation/SimulationContext.hpp:46-59, simulation/SimulationParams.hpp:8-15,
@@ -284,11 +288,14 @@ int avbd_get_stats(avbd_ctx* ctx, avbd_stats* out);
 /* Introspection for tests and the bench -------------------------------------- */
 uint32_t avbd_num_particles(avbd_ctx* ctx);
 uint32_t avbd_num_energies(avbd_ctx* ctx, int type);
-/* Kernel launch geometry actually in use (after autotuning). */
-int avbd_get_launch_config(avbd_ctx* ctx, uint32_t* lanes_per_vertex, uint32_t* block_size);
+/* Kernel launch geometry actually in use (after autotuning); *gather_batch =
+ * tet incidences one lane gathers at once (1, 2 or 4). Any pointer may be null. */
+int avbd_get_launch_config(avbd_ctx* ctx, uint32_t* lanes_per_vertex, uint32_t* block_size, uint32_t* gather_batch);
 /* *persistent_grid = blocks of the persistent step kernel, or 0 when the
- * step runs as one launch per phase. */
-int avbd_get_step_mode(avbd_ctx* ctx, uint32_t* persistent_grid);
+ * step runs as one launch per phase; *fused_chebyshev = 1 when the Chebyshev /
+ * dual update runs inside each iteration's last colour phase
+ * (AVBD_FLAG_SEPARATE_CHEBYSHEV). Either pointer may be null. */
+int avbd_get_step_mode(avbd_ctx* ctx, uint32_t* persistent_grid, uint32_t* fused_chebyshev);
 /* Kernel timing: while enabled, avbd_step() launches eagerly and brackets
  * every per-colour vertex-solve launch with HIP events on the ctx stream.
  * avbd_get_solve_timing() returns the average duration (ms) of those

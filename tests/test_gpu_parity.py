@@ -171,6 +171,43 @@ def test_graph_replay_matches_eager(oracle):
         a.ctx.stats()   # eager work between replays
 
 
+@pytest.mark.parametrize("mode", ["fixed_odd", "fixed_even_eager", "residual_odd_checks"])
+def test_fused_chebyshev_matches_separate_launch(oracle, mode):
+    """The Chebyshev / dual update fused into each iteration's last colour
+    phase (two position buffers) gives the same bits as its own launch after
+    the phase, including after an odd number of iterations and when a
+    residual check stops the step early."""
+    banner(f"fused Chebyshev vs separate launch ({mode})")
+    from avbd_amd.sim import Simulation
+    sc = scenes.dropped_cube(n=6, seed=3, height=0.02)
+    kw = dict(lanes_per_vertex=8, block_size=128)
+    if mode == "fixed_odd":
+        sc.iters = 7
+    elif mode == "fixed_even_eager":
+        sc.iters = 10
+        kw["use_graph"] = False
+    else:
+        kw["residual_target"] = (5e-2, 99, 3)
+    a = Simulation(sc, separate_chebyshev=False, **kw)
+    b = Simulation(sc, separate_chebyshev=True, **kw)
+    a.setup()
+    b.setup()
+    assert a.ctx.fused_chebyshev() and not b.ctx.fused_chebyshev()
+    for k in range(4):
+        a.update(5)
+        b.update(5)
+        assert np.array_equal(a.positions(), b.positions()), f"{mode}: round {k} differs"
+        va, vb = a.ctx.state()[1], b.ctx.state()[1]
+        assert np.array_equal(va, vb)
+        if mode == "residual_odd_checks":
+            sa, sb = a.ctx.stats(), b.ctx.stats()
+            assert sa.iterations_total == sb.iterations_total
+            banner(f"  round {k}: last step {sa.last_step_iterations} iterations, total {sa.iterations_total}")
+    ra, rb = a.ctx.ground_records(), b.ctx.ground_records()
+    assert bytes(ra) == bytes(rb)   # duals and stiffness ramps
+    banner("  identical positions, velocities and ground records")
+
+
 def test_stats_match_oracle(oracle):
     banner("residual / incremental potential")
     sc = scenes.hanging_cube(n=6)
