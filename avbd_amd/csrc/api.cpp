@@ -703,6 +703,53 @@ void autotune(avbd_ctx* c) {
     c->cfg = best_cfg;
 }
 
+// Fused or separate Chebyshev / dual update: time the last colour phase plus
+// the update both ways on the current state and keep the faster.  Both
+// forms write x_last, x_lastlast and the ground records (and the separate
+// one d.x), so those are saved first and restored afterwards.
+void tune_fusion(avbd_ctx* c) {
+    if (!c->fuse_cheb) return;
+    const uint32_t last = c->plan.num_colors - 1;
+    const uint32_t b = c->plan.item_offsets[last], e = c->plan.item_offsets[last + 1];
+    const float dt = (float)c->desc.dt;
+    const size_t nx = (size_t)c->d.n * sizeof(float4), ng = (size_t)c->d.n_ground * sizeof(GroundRec);
+    std::vector<uint8_t> save(3 * nx + ng);
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "D2H");
+    };
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "H2D");
+    };
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    d2h(save.data(), c->d.x, nx);
+    d2h(save.data() + nx, c->d.x_last, nx);
+    d2h(save.data() + 2 * nx, c->d.x_lastlast, nx);
+    d2h(save.data() + 3 * nx, c->d.ground, ng);
+    auto timed = [&](auto&& body) {
+        for (int w = 0; w < 3; w++) body();
+        hip_check(hipEventRecord(c->ev_a, c->stream), "hipEventRecord");
+        for (int r = 0; r < 10; r++) body();
+        hip_check(hipEventRecord(c->ev_b, c->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(c->ev_b), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, c->ev_a, c->ev_b), "hipEventElapsedTime");
+        return ms;
+    };
+    const float t_sep = timed([&] {
+        launch_solve_color(c->stream, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x_scratch, nullptr);
+        launch_cheb_dual(c->stream, c->d, 1.0f, nullptr);
+    });
+    const float t_fused = timed([&] {
+        launch_solve_cheb(c->stream, c->d, c->cfg, b, e - b, c->damping, dt, 1.0f, c->cheb_list, c->n_cheb,
+                          c->x_alt, nullptr);
+    });
+    h2d(c->d.x, save.data(), nx);
+    h2d(c->d.x_last, save.data() + nx, nx);
+    h2d(c->d.x_lastlast, save.data() + 2 * nx, nx);
+    h2d(c->d.ground, save.data() + 3 * nx, ng);
+    c->fuse_cheb = t_fused < t_sep;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1467,7 +1514,10 @@ int avbd_build_graph(avbd_ctx* c) {
         hip_check(hipEventCreate(&c->ev_lb1), "hipEventCreate");
         for (auto& ev : c->events) hip_check(hipEventCreate(&ev), "hipEventCreate");
 
-        if (c->desc.lanes_per_vertex == 0 || c->desc.block_size == 0) autotune(c);
+        if (c->desc.lanes_per_vertex == 0 || c->desc.block_size == 0) {
+            autotune(c);
+            tune_fusion(c);
+        }
         if ((c->desc.flags & AVBD_FLAG_PERSISTENT) && !(c->desc.flags & AVBD_FLAG_NO_PERSISTENT)) {
             c->pgrid = persistent_grid(c->device, c->cfg.lanes, c->damping);
             c->persistent = c->pgrid >= 8;
