@@ -326,23 +326,28 @@ __device__ T golden(T ts, T te, F&& f) {
 }
 
 // CollisionDetector::_triangleSDF_CCD for a sphere SDF (|p| - r).
+// The reference evaluates every trial point in the body frame at the trial
+// time, R(t)^T (x - c(t)) with R(t) = R0 exp(t w).  For a sphere the SDF, its
+// gradient, the normal velocity <v_rel, grad>, the barycentric coordinates
+// and every comparison of the search are invariant under that rotation, so
+// the search here runs on world-frame offsets x - c(t): the same search up
+// to rounding, without a quaternion exponential per trial point.  The world
+// angular velocity R(t) w equals R0 w (exp(t w) fixes w).  R(t) itself is
+// formed only for the body-frame contact point and for the gradient at the
+// centre (the reference's body-frame (1, 0, 0)).
 __device__ bool tri_sphere_ccd(const d3 tri[3], const d3 tv[3], d3 rb_pos, dq rb_rot, d3 rb_v, d3 rb_w, double radius,
                                double dt, double margin, d3& normal, double bc_out[3], d3& cpl) {
     auto pos_t = [&](double t) { return rb_pos + t * rb_v; };
     auto rot_t = [&](double t) { return qmul(rb_rot, exp_s3(t * rb_w)); };
-    auto rel_vel = [&](d3 xt, d3 vt, double t) {
-        const dq R = rot_t(t);
-        const d3 wg = qrot(R, rb_w);
-        const d3 lv = rb_v + cross(wg, xt - pos_t(t));
-        return qrot(qinv(R), vt - lv);
-    };
-    auto local_pos = [&](d3 xt, double t) { return qrot(qinv(rot_t(t)), xt - pos_t(t)); };
+    const d3 wg = qrot(rb_rot, rb_w);
+    auto rel_vel = [&](d3 xt, d3 vt, double t) { return vt - (rb_v + cross(wg, xt - pos_t(t))); };
+    auto local_pos = [&](d3 xt, double t) { return xt - pos_t(t); };
     auto sdf = [&](d3 p) { return norm(p) - radius; };
-    auto grad = [&](d3 p) {
+    auto grad = [&](d3 p, double t) {
         const double dd = norm(p);
-        return dd > 1e-8 ? (1.0 / dd) * p : mk(1, 0, 0);
+        return dd > 1e-8 ? (1.0 / dd) * p : qrot(rot_t(t), mk(1, 0, 0));
     };
-    auto eval0 = [&](d3 si, d3 vi) { return dot(rel_vel(si, vi, 0), grad(local_pos(si, 0))); };
+    auto eval0 = [&](d3 si, d3 vi) { return dot(rel_vel(si, vi, 0), grad(local_pos(si, 0), 0)); };
     const double e1 = eval0(tri[0], tv[0]), e2 = eval0(tri[1], tv[1]), e3 = eval0(tri[2], tv[2]);
     double t_start = 0, t_end = dt, t = 0, u, v, w;
     if (e1 < e2 && e1 < e3) {
@@ -370,14 +375,14 @@ __device__ bool tri_sphere_ccd(const d3 tri[3], const d3 tv[3], d3 rb_pos, dq rb
             t_end = fmin(t, t_end);
             t = golden(t_start, t, ud_time);
         } else {
-            const double vdg = dot(v_ti, grad(x_ti));
+            const double vdg = dot(v_ti, grad(x_ti, t));
             if (vdg > 0) t = golden(t_start, t, sd_time);
             else t = golden(t, t_end, sd_time);
         }
         xg = interp(t);
         x_ti = local_pos(xg, t);
         sdf_x = sdf(x_ti);
-        grad_x = grad(x_ti);
+        grad_x = grad(x_ti, t);
         if (sdf_x <= 0) t_end = fmin(t, t_end);
         const d3 l1 = local_pos(tri[0] + t * tv[0], t);
         const d3 l2 = local_pos(tri[1] + t * tv[1], t);
@@ -397,8 +402,9 @@ __device__ bool tri_sphere_ccd(const d3 tri[3], const d3 tv[3], d3 rb_pos, dq rb
     bc_out[0] = u;
     bc_out[1] = v;
     bc_out[2] = w;
-    cpl = x_ti - sdf_x * grad_x;
-    normal = qrot(rot_t(t), grad_x);
+    // body-frame contact point and world normal, as the reference reports them
+    cpl = qrot(qinv(rot_t(t)), x_ti - sdf_x * grad_x);
+    normal = grad_x;
     return sdf_x < margin;
 }
 
