@@ -26,18 +26,19 @@ from .scenes import SplitMix64
 DEFAULT_PATH = Path(__file__).resolve().parent / "data" / "warm_start_mlp.npz"
 
 
-def training_scenes(n_scenes: int, seed: int, n_min: int = 8, n_max: int = 14) -> List[scenes.Scene]:
-    """Dropped Kuhn cubes in the style of the 32^3 workload: random size,
-    Young's modulus (log-uniform 1e5..1e7), Poisson ratio (0.3..0.48),
-    rotation and drop height (0.1..0.6 m)."""
+def training_scenes(n_scenes: int, seed: int, n_min: int = 8, n_max: int = 14, log_e: Tuple[float, float] = (5.0, 7.0),
+                    nu: Tuple[float, float] = (0.3, 0.48), height: Tuple[float, float] = (0.1, 0.6)) -> List[scenes.Scene]:
+    """Dropped Kuhn cubes in the style of the 32^3 workload: random size
+    (n_min..n_max cells per axis), Young's modulus (log-uniform over log_e),
+    Poisson ratio, rotation and drop height (uniform over the ranges)."""
     rng = SplitMix64(seed)
     out = []
     for i in range(n_scenes):
         n = rng.randint(n_min, n_max)
-        E = float(10 ** (5 + 2 * rng.uniform()))
-        nu = 0.3 + 0.18 * rng.uniform()
-        h = 0.1 + 0.5 * rng.uniform()
-        out.append(scenes.dropped_cube(n=n, seed=1000 + 7 * seed + i, height=h, E=E, nu=nu))
+        E = float(10 ** (log_e[0] + (log_e[1] - log_e[0]) * rng.uniform()))
+        nu_i = nu[0] + (nu[1] - nu[0]) * rng.uniform()
+        h = height[0] + (height[1] - height[0]) * rng.uniform()
+        out.append(scenes.dropped_cube(n=n, seed=1000 + 7 * seed + i, height=h, E=E, nu=nu_i))
     return out
 
 
@@ -93,20 +94,26 @@ def load(path: Path = DEFAULT_PATH) -> np.ndarray:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(DEFAULT_PATH))
-    ap.add_argument("--scenes", type=int, default=24)
+    ap.add_argument("--scenes", type=int, default=16)
     ap.add_argument("--steps", type=int, default=80)
     ap.add_argument("--keep", type=float, default=0.3)
     ap.add_argument("--tol", type=float, default=1e-2)
-    ap.add_argument("--max-iters", type=int, default=300)
-    ap.add_argument("--epochs", type=int, default=15)
+    ap.add_argument("--max-iters", type=int, default=400)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--n-min", type=int, default=16)
+    ap.add_argument("--n-max", type=int, default=24)
+    ap.add_argument("--log-e", type=float, nargs=2, default=(5.7, 6.3))
+    ap.add_argument("--nu", type=float, nargs=2, default=(0.35, 0.45))
+    ap.add_argument("--height", type=float, nargs=2, default=(0.2, 0.6))
     args = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     t0 = time.perf_counter()
-    X, T = generate(training_scenes(args.scenes, args.seed), args.steps, args.tol, args.max_iters, keep=args.keep,
-                    log=log)
+    sc = training_scenes(args.scenes, args.seed, args.n_min, args.n_max, tuple(args.log_e), tuple(args.nu),
+                         tuple(args.height))
+    X, T = generate(sc, args.steps, args.tol, args.max_iters, keep=args.keep, log=log)
     log(f"[warm-start] {X.shape[0]} rows in {time.perf_counter() - t0:.1f}s")
     from .sim import SimulationContext
     ctx = SimulationContext(dt=1 / 60, solver_iters=1)
@@ -115,7 +122,9 @@ def main(argv=None):
     t1 = time.perf_counter()
     params, loss = train(ctx, X, T, args.epochs, args.batch, args.lr, args.seed)
     log(f"[warm-start] trained {args.epochs} epochs in {time.perf_counter() - t1:.1f}s, final loss {loss:.4f}")
-    save(Path(args.out), params, rows=X.shape[0], loss=loss, tol=args.tol, scenes=args.scenes, steps=args.steps)
+    save(Path(args.out), params, rows=X.shape[0], loss=loss, tol=args.tol, scenes=args.scenes, steps=args.steps,
+         max_iters=args.max_iters, epochs=args.epochs, seed=args.seed, n_range=(args.n_min, args.n_max),
+         log_e=tuple(args.log_e), nu=tuple(args.nu), height=tuple(args.height), keep=args.keep)
     log(f"[warm-start] wrote {args.out}")
     return 0
 
