@@ -110,6 +110,9 @@ def main():
     ap.add_argument("--max-iters", type=int, default=400, help="iteration budget per step of the residual leg")
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--res-steps", type=int, default=60, help="timed steps of the residual-targeted leg")
+    ap.add_argument("--res-eager", action="store_true",
+                    help="launch the residual-targeted leg eagerly instead of replaying its step graphs "
+                         "(for kernel traces of that leg; not the measured configuration)")
     ap.add_argument("--warm-weights", default=None,
                     help="trained warm-start parameters (.npz); default: avbd_amd/data/warm_start_mlp.npz if present")
     ap.add_argument("--calibrate", type=int, default=0,
@@ -192,7 +195,8 @@ def main():
     # headline leg: AVBD iterating each step to the residual target, plain
     # and (when trained parameters exist) warm-started by the network
     def residual_leg(params=None):
-        rsim = Simulation(make_scene(args.workload), residual_target=(args.tol, args.max_iters, args.check_every))
+        rsim = Simulation(make_scene(args.workload), residual_target=(args.tol, args.max_iters, args.check_every),
+                          use_graph=not args.res_eager)
         rsim.setup()
         if params is not None:
             rsim.ctx.mlp_set_params(params)

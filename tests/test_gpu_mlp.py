@@ -181,3 +181,39 @@ def test_training_and_warm_started_solver():
 
 def bbox(x):
     return float(np.max(x.max(axis=0) - x.min(axis=0)))
+
+
+def test_batched_warm_start_matches_single_scenes():
+    """Warm start over a batch of scenes: the per-scene incremental-potential
+    sums (block- and wavefront-level, blocks that straddle two scenes
+    included) choose each scene's start as a run of that scene alone does."""
+    banner("warm start: batch vs single scenes")
+    from avbd_amd import scenes
+    from avbd_amd import warm_start as W
+    from avbd_amd.sim import Simulation
+    params = W.load()
+    sc = scenes.cube_batch(n_scenes=4, seed=9, n_min=4, n_max=7, iters=8, steps=6)
+
+    def run(s):
+        sim = Simulation(s, lanes_per_vertex=8, block_size=128)
+        sim.setup()
+        sim.ctx.mlp_set_params(params)
+        sim.ctx.set_warm_start(True)
+        sim.update(6)
+        return sim
+
+    b = run(sc)
+    xb = b.positions()
+    sb = b.ctx.stats()
+    assert sb.warm_start_predictions == 6 * len(sc.meshes)
+    off, fallbacks = 0, 0
+    for m in sc.meshes:
+        one = run(scenes.Scene(name="one", meshes=[m], iters=sc.iters, steps=sc.steps))
+        xs = one.positions()
+        n = xs.shape[0]
+        err = np.abs(xb[off:off + n] - xs).max() / bbox(xs)
+        assert err < 1e-5, err
+        fallbacks += one.ctx.stats().warm_start_fallbacks
+        off += n
+    banner(f"  {len(sc.meshes)} scenes: batch fallbacks {sb.warm_start_fallbacks}, single runs {fallbacks}")
+    assert sb.warm_start_fallbacks == fallbacks
