@@ -269,7 +269,11 @@ int avbd_build_graph(avbd_ctx* ctx);
 
 /* Time stepping ------------------------------------------------------------ */
 
-/* n_steps calls of VBDSolver::solve_Parallel (asynchronous on the ctx stream). */
+/* n_steps calls of VBDSolver::solve_Parallel (asynchronous on the ctx stream;
+ * in residual-targeted mode with step graphs the call returns once the last
+ * step's final check intervals are queued: it waits for the check two
+ * intervals back to decide whether to queue more, so it runs at most two
+ * check intervals ahead of the device). */
 int avbd_step(avbd_ctx* ctx, uint32_t n_steps);
 int avbd_synchronize(avbd_ctx* ctx);
 
