@@ -456,7 +456,7 @@ void enqueue_part(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool 
             launch_warm_save(s, c->d, io);
             launch_warm_features(s, c->d, io, c->cfg.lanes, dt, g, c->rows_pad);
             if (c->warm) {
-                c->mlp->forward((int)c->rows_pad, io.x16);
+                c->mlp->forward((int)c->rows_pad, io.x16, false);
                 launch_warm_apply(s, c->d, io, dt * dt * g);
                 launch_warm_ip(s, c->d, io, c->damping, dt);
                 launch_warm_choose(s, c->d, io);

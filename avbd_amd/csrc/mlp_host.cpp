@@ -164,16 +164,18 @@ void MlpModel::ensure_batch(int rows) {
     t32 = dalloc<float>(batch_owned, (size_t)pad * kMlpOut);
 }
 
-// Forward pass of rows_pad rows of x16 (bf16, normalised, padded): hidden
-// layers keep the pre-activations (z32) for the backward pass; the output
-// layer leaves FP32 in z32[3].
-void MlpModel::forward(int rows_pad, const uint16_t* x) {
+// Forward pass of rows_pad rows of x16 (bf16, normalised, padded): with
+// keep_pre the hidden layers keep their pre-activations (z32) for the
+// backward pass (inference skips those stores); the output layer leaves FP32
+// in z32[3].
+void MlpModel::forward(int rows_pad, const uint16_t* x, bool keep_pre) {
     const uint16_t* in = x;
     for (int l = 0; l < kMlpLayers; l++) {
         const Layer& L = layers[l];
         const bool last = l == kMlpLayers - 1;
         launch_gemm_bf16(stream, false, true, last ? MLP_EPI_BIAS_F32 : MLP_EPI_BIAS_SILU, in, L.in, L.w16, L.in,
-                         z32[l], last ? nullptr : h16[l], L.out, L.b, nullptr, 0, rows_pad, L.out, L.in);
+                         (last || keep_pre) ? z32[l] : nullptr, last ? nullptr : h16[l], L.out, L.b, nullptr, 0,
+                         rows_pad, L.out, L.in);
         in = h16[l];
     }
 }

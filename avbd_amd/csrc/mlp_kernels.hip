@@ -87,6 +87,67 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
     }
 }
 
+// Four consecutive bf16 (8-byte aligned) as an MFMA operand fragment.
+__device__ __forceinline__ short4_t ld_frag(const uint16_t* p) {
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    short4_t v;
+    v[0] = (short)(u.x & 0xffffu);
+    v[1] = (short)(u.x >> 16);
+    v[2] = (short)(u.y & 0xffffu);
+    v[3] = (short)(u.y >> 16);
+    return v;
+}
+
+// The forward products: C[M x N] = A(M x K) B(K x N) with A row-major and
+// B(k, n) = b[n * ldb + k] (weights stored out x in), N = 16 NT.  One
+// wavefront owns a 16-row strip and all NT output tiles of it, so each A
+// fragment is read once (the one-tile-per-wavefront kernel above reads it NT
+// times) and the weight fragments, the same for every strip, come from the
+// caches.  The k loop and the MFMA sequence per tile are those of
+// gemm_bf16_kernel, so the results are identical.
+template <int EPI, int NT>
+__global__ __launch_bounds__(256) void gemm_rows_kernel(const uint16_t* __restrict__ a, int lda,
+                                                        const uint16_t* __restrict__ b, int ldb,
+                                                        float* __restrict__ c32, uint16_t* __restrict__ c16, int ldc,
+                                                        const float* __restrict__ bias, int M, int K) {
+    const int lane = threadIdx.x & 63;
+    const int strip = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (strip >= (M >> 4)) return;   // whole wavefronts leave together
+    const int tm = strip << 4;
+    const int r = lane & 15, g = lane >> 4;
+    float4_t acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* arow = a + (size_t)(tm + r) * lda + 4 * g;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const short4_t av = ld_frag(arow + k0);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const short4_t bv = ld_frag(b + (size_t)(t * 16 + r) * ldb + k0 + 4 * g);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const int n = t * 16 + r;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int m = tm + 4 * g + j;
+            const float v = acc[t][j];
+            const size_t o = (size_t)m * ldc + n;
+            if constexpr (EPI == MLP_EPI_F32) {
+                c32[o] = v;
+            } else if constexpr (EPI == MLP_EPI_BIAS_F32) {
+                c32[o] = v + bias[n];
+            } else if constexpr (EPI == MLP_EPI_BIAS_SILU) {
+                const float z = v + bias[n];
+                if (c32) c32[o] = z;   // pre-activation, kept for the backward pass
+                c16[o] = f2bf(z * sigmoidf_(z));
+            }
+        }
+    }
+}
+
 // Output-layer loss gradient: dY = 2 (Y - T) / rows for the first `outs`
 // columns of the first `rows` rows (zero in the padding), as bf16; adds the
 // squared-error sum to *loss.
@@ -140,6 +201,22 @@ inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
 void launch_gemm_bf16(hipStream_t s, bool ta, bool tb, int epi, const uint16_t* a, int lda, const uint16_t* b,
                       int ldb, float* c32, uint16_t* c16, int ldc, const float* bias, const float* aux, int ldaux,
                       int M, int N, int K) {
+    if (!ta && tb && epi != MLP_EPI_SILU_BWD && (N == 128 || N == 16)) {
+        const dim3 grid(blocks_for(M / 16, 4)), block(256);
+#define AVBD_ROWS(E_, NT_)                                                                                      \
+    if (epi == E_ && N == 16 * NT_) {                                                                           \
+        hipLaunchKernelGGL((gemm_rows_kernel<E_, NT_>), grid, block, 0, s, a, lda, b, ldb, c32, c16, ldc, bias, \
+                           M, K);                                                                               \
+        return;                                                                                                 \
+    }
+        AVBD_ROWS(MLP_EPI_BIAS_SILU, 8)
+        AVBD_ROWS(MLP_EPI_BIAS_F32, 8)
+        AVBD_ROWS(MLP_EPI_F32, 8)
+        AVBD_ROWS(MLP_EPI_BIAS_SILU, 1)
+        AVBD_ROWS(MLP_EPI_BIAS_F32, 1)
+        AVBD_ROWS(MLP_EPI_F32, 1)
+#undef AVBD_ROWS
+    }
     const int tiles = (M / 16) * (N / 16);
     const dim3 grid(blocks_for(tiles, 4)), block(256);
 #define AVBD_GEMM(TA_, TB_, E_)                                                                                 \

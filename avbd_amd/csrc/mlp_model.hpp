@@ -39,7 +39,7 @@ struct MlpModel {
 
     // device-side pieces used by the in-step inference
     void ensure_batch(int rows_pad);
-    void forward(int rows_pad, const uint16_t* x);
+    void forward(int rows_pad, const uint16_t* x, bool keep_pre = true);
 
     hipStream_t stream;
     Layer layers[kMlpLayers];

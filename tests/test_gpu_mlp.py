@@ -103,6 +103,15 @@ def test_mfma_gemm_layout_and_transposes():
             err = np.abs(C - want).max() / np.abs(want).max()
             banner(f"  ta={ta} tb={tb}: rel err {err:.2e}")
             assert err < 1e-5
+    # the strip kernel of the forward products (weights stored out x in, N = 128 or 16)
+    for M, N, K in ((64, 128, 128), (80, 16, 128), (32, 128, 32)):
+        A = bf16(rng.normal(size=(M, K)))
+        B = bf16(rng.normal(size=(K, N)))
+        C = ctx.mlp_gemm_test(A, B, False, True)
+        want = A.astype(np.float64) @ B.astype(np.float64)
+        err = np.abs(C - want).max() / np.abs(want).max()
+        banner(f"  strip kernel M={M} N={N} K={K}: rel err {err:.2e}")
+        assert err < 1e-5
 
 
 def test_mlp_gradients_match_numpy():
