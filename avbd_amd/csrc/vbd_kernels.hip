@@ -235,20 +235,34 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
     return x * (G / 8u) + min(x, G % 8u) + q;
 }
 
-// The incidences j0, j0 + L, ..., j0 + (U-1) L of one lane (those < cnt):
-// every record is loaded, then every vertex position, then the terms are
-// accumulated in incidence order -- the same order and arithmetic as one
-// incidence at a time, with U times fewer dependent memory round trips.
+// A lane's incidences in batches of U (j0, j0 + L, ..., j0 + (U-1) L, those
+// < cnt): a batch's records are loaded together, then its vertex positions
+// together with the next batch's records, then its terms are accumulated in
+// incidence order -- the same order and arithmetic as one incidence at a
+// time, with fewer dependent memory round trips and the next records in
+// flight during the arithmetic.
+template <int U>
+struct IncBatch {
+    uint4 idx[U];
+    float4 r1[U], r2[U], r3[U];
+};
+
+template <int L, int U>
+__device__ __forceinline__ void load_inc_batch(const TetRec* __restrict__ irec, uint32_t base, uint32_t j0,
+                                               uint32_t cnt, IncBatch<U>& b) {
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (j0 + u * L < cnt) load_tet(irec, base + j0 + u * L, b.idx[u], b.r1[u], b.r2[u], b.r3[u]);
+}
+
 template <int L, int U, bool DAMP>
 __device__ __forceinline__ void nh_incidences_batch(const float4* x, const TetRec* __restrict__ irec,
                                                     const uint32_t* __restrict__ inc,
                                                     const TetDamp* __restrict__ damp, uint32_t base, uint32_t j0,
-                                                    uint32_t cnt, float inv_dt, float3& g, Sym3& h) {
-    uint4 idx[U];
-    float4 r1[U], r2[U], r3[U];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-        if (j0 + u * L < cnt) load_tet(irec, base + j0 + u * L, idx[u], r1[u], r2[u], r3[u]);
+                                                    uint32_t cnt, float inv_dt, float3& g, Sym3& h,
+                                                    IncBatch<U>& cur) {
+    const uint4* idx = cur.idx;
+    const float4 *r1 = cur.r1, *r2 = cur.r2, *r3 = cur.r3;
     float3 P[U][4];
 #pragma unroll
     for (int u = 0; u < U; u++)
@@ -258,11 +272,14 @@ __device__ __forceinline__ void nh_incidences_batch(const float4* x, const TetRe
             P[u][2] = f3(x[idx[u].z]);
             P[u][3] = f3(x[idx[u].w & 0x3fffffffu]);
         }
+    IncBatch<U> next;
+    if (j0 + U * L < cnt) load_inc_batch<L, U>(irec, base, j0 + U * L, cnt, next);
 #pragma unroll
     for (int u = 0; u < U; u++)
         if (j0 + u * L < cnt)
             nh_incidence_pos<DAMP>(P[u][0], P[u][1], P[u][2], P[u][3], r1[u], r2[u], r3[u], idx[u].w >> 30, damp,
                                    DAMP ? (inc[base + j0 + u * L] >> 2) : 0u, inv_dt, g, h);
+    cur = next;
 }
 
 // The tet incidences of one work item summed over the L lanes of its group
@@ -276,8 +293,10 @@ __device__ __forceinline__ void item_gradient(const VItem& it, uint32_t lane, co
     g = make_float3(0.f, 0.f, 0.f);
     h = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if constexpr (U > 1) {
+        IncBatch<U> cur;
+        if (lane < cnt) load_inc_batch<L, U>(irec, it.inc_begin, lane, cnt, cur);
         for (uint32_t j0 = lane; j0 < cnt; j0 += U * L)
-            nh_incidences_batch<L, U, DAMP>(x, irec, inc, damp, it.inc_begin, j0, cnt, inv_dt, g, h);
+            nh_incidences_batch<L, U, DAMP>(x, irec, inc, damp, it.inc_begin, j0, cnt, inv_dt, g, h, cur);
     } else {
         for (uint32_t j = lane; j < cnt; j += L)
             nh_incidence_k<DAMP>(x, irec, inc, damp, it.inc_begin + j, inv_dt, g, h);
