@@ -461,6 +461,30 @@ __global__ void narrow_kernel(DevPrims p, const uint32_t* __restrict__ pairs, co
         const uint32_t c = p.idx[3 * b];
         const d3 T[3] = {v1, v2, v3};
         const d3 TV[3] = {ld(v[ti[0]]), ld(v[ti[1]]), ld(v[ti[2]])};
+        // Every point the CCD evaluates is a point of the triangle at some
+        // t in [0, dt] minus the sphere centre at t: with linear motion it
+        // lies in the box of the vertices' offsets at t = 0 and t = dt.  When
+        // that box is farther than r + margin from the centre (with 1e-6 m
+        // to spare for rounding) the CCD cannot report a contact: skip it.
+        {
+            const d3 cc = ld(x[c]), cv = ld(v[c]);
+            double lo[3] = {__DBL_MAX__, __DBL_MAX__, __DBL_MAX__}, hi[3] = {-__DBL_MAX__, -__DBL_MAX__, -__DBL_MAX__};
+            for (int k = 0; k < 3; k++) {
+                const d3 q0 = T[k] - cc, q1 = (T[k] + dt * TV[k]) - (cc + dt * cv);
+                const double a0[3] = {q0.x, q0.y, q0.z}, a1[3] = {q1.x, q1.y, q1.z};
+                for (int e = 0; e < 3; e++) {
+                    lo[e] = fmin(lo[e], fmin(a0[e], a1[e]));
+                    hi[e] = fmax(hi[e], fmax(a0[e], a1[e]));
+                }
+            }
+            double d2 = 0;
+            for (int e = 0; e < 3; e++) {
+                const double g = fmax(0.0, fmax(lo[e], -hi[e]));
+                d2 += g * g;
+            }
+            const double reach = p.radius[b] + margin + 1e-6;
+            if (d2 > reach * reach) return;
+        }
         d3 nrm, cpl;
         double bc[3];
         if (!tri_sphere_ccd(T, TV, ld(x[c]), ldq(q[rot_of[c]]), ld(v[c]), ld(w[rot_of[c]]), p.radius[b], dt, margin,
