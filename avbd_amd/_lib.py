@@ -48,6 +48,7 @@ class SceneDesc(C.Structure):
         ("device", C.c_int32),
         ("lanes_per_vertex", C.c_uint32),
         ("block_size", C.c_uint32),
+        ("gather_batch", C.c_uint32),
     ]
 
 

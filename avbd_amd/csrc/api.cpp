@@ -680,6 +680,7 @@ void autotune(avbd_ctx* c) {
         for (uint32_t B : block_c)
         for (uint32_t U : batch_c) {
             if (c->desc.block_size && B != c->desc.block_size) continue;
+            if (c->desc.gather_batch && U != c->desc.gather_batch) continue;
             LaunchCfg cfg{L, B, U};
             auto run = [&] {
                 for (uint32_t col = 0; col < c->plan.num_colors; col++) {
@@ -773,6 +774,11 @@ int avbd_create(const avbd_scene_desc* desc, avbd_ctx** out) {
         g_create_error = "block_size must be 0 or a multiple of 64 in [64, 1024]";
         return AVBD_E_ARG;
     }
+    const uint32_t U = desc->gather_batch;
+    if (U && U != 1 && U != 2 && U != 4) {
+        g_create_error = "gather_batch must be 0, 1, 2 or 4";
+        return AVBD_E_ARG;
+    }
     // HIP is initialised lazily by avbd_build_graph(): staging and the host
     // graph build work without a device.
     auto* c = new avbd_ctx();
@@ -780,6 +786,7 @@ int avbd_create(const avbd_scene_desc* desc, avbd_ctx** out) {
     c->device = desc->device;
     c->cfg.lanes = L ? L : 8;
     c->cfg.block = B ? B : 256;
+    c->cfg.batch = U ? U : 1;
     *out = c;
     return AVBD_OK;
 }

@@ -55,7 +55,8 @@ class SimulationContext:
     def __init__(self, dt: float, g_accel: float = 9.81, solver_iters: int = 20,
                  iter_acceleration: float = 0.95, particle_capacity: int = 0, energy_capacity: int = 0,
                  device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0, use_graph: bool = True,
-                 serial_broadphase: bool = False, persistent: bool = False, separate_chebyshev: bool = False):
+                 serial_broadphase: bool = False, persistent: bool = False, separate_chebyshev: bool = False,
+                 gather_batch: int = 0):
         desc = _lib.SceneDesc()
         desc.particle_capacity = particle_capacity
         desc.energy_capacity = energy_capacity
@@ -69,6 +70,7 @@ class SimulationContext:
         desc.device = device
         desc.lanes_per_vertex = lanes_per_vertex
         desc.block_size = block_size
+        desc.gather_batch = gather_batch
         h = C.c_void_p()
         rc = lib.avbd_create(C.byref(desc), C.byref(h))
         check(rc, None)
@@ -526,13 +528,14 @@ class Simulation:
 
     def __init__(self, scene: Scene, device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0,
                  use_graph: bool = True, serial_broadphase: bool = False, persistent: bool = False,
-                 residual_target: Optional[tuple] = None, separate_chebyshev: bool = False):
+                 residual_target: Optional[tuple] = None, separate_chebyshev: bool = False, gather_batch: int = 0):
         self.scene = scene
         self.ctx = SimulationContext(dt=scene.dt, g_accel=scene.g, solver_iters=scene.iters,
                                      iter_acceleration=scene.rho, device=device,
                                      lanes_per_vertex=lanes_per_vertex, block_size=block_size,
                                      use_graph=use_graph, serial_broadphase=serial_broadphase,
-                                     persistent=persistent, separate_chebyshev=separate_chebyshev)
+                                     persistent=persistent, separate_chebyshev=separate_chebyshev,
+                                     gather_batch=gather_batch)
         self.solver = VBDSolver(self.ctx, scene.iters, scene.rho)
         if residual_target is not None:
             self.ctx.set_residual_target(*residual_target)

@@ -88,6 +88,8 @@ typedef struct {
     int32_t device;                /* HIP device ordinal */
     uint32_t lanes_per_vertex;     /* 0 = autotuned; else 4, 8, 16 or 32 */
     uint32_t block_size;           /* 0 = autotuned; else 64..1024, multiple of 64 */
+    uint32_t gather_batch;         /* tet incidences a lane gathers at once: 0 = autotuned
+                                      (1 when lanes and block size are both given); else 1, 2 or 4 */
 } avbd_scene_desc;
 
 /* NeoHookeanEnergyInfo, energy/elastic/NeoHookeanEnergyPool.hpp:8-17 (FP64, row-major 3x3). */

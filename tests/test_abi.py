@@ -49,6 +49,8 @@ def test_create_rejects_bad_descriptors():
         SimulationContext(dt=1 / 60, lanes_per_vertex=3)
     with pytest.raises(_lib.AvbdError):
         SimulationContext(dt=1 / 60, block_size=100)
+    with pytest.raises(_lib.AvbdError):
+        SimulationContext(dt=1 / 60, gather_batch=3)
 
 
 def test_capacity_error_code():

@@ -208,6 +208,26 @@ def test_fused_chebyshev_matches_separate_launch(oracle, mode):
     banner("  identical positions, velocities and ground records")
 
 
+def test_gather_batches_give_identical_bits(oracle):
+    """Gathering 1, 2 or 4 tet incidences per lane at once (plain colour
+    phases and the fused last colour) accumulates in the same order: the
+    same bits after several steps."""
+    banner("gather batch 1 / 2 / 4")
+    from avbd_amd.sim import Simulation
+    sc = scenes.dropped_cube(n=6, seed=4, height=0.02)
+    xs = []
+    for U in (1, 2, 4):
+        for L in (4, 8):
+            s = Simulation(sc, lanes_per_vertex=L, block_size=128, gather_batch=U)
+            s.setup()
+            assert s.ctx.launch_config() == (L, 128, U)
+            s.update(6)
+            xs.append((L, U, s.positions()))
+    for L, U, x in xs:
+        ref = next(x0 for L0, U0, x0 in xs if L0 == L and U0 == 1)
+        assert np.array_equal(x, ref), (L, U)
+
+
 def test_stats_match_oracle(oracle):
     banner("residual / incremental potential")
     sc = scenes.hanging_cube(n=6)
