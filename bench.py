@@ -144,6 +144,9 @@ def main():
     ap.add_argument("--max-iters", type=int, default=400, help="iteration budget per step of the residual leg")
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--res-steps", type=int, default=60, help="timed steps of the residual-targeted leg")
+    ap.add_argument("--lanes-per-vertex", type=int, default=0, help="0: autotuned at set-up")
+    ap.add_argument("--block-size", type=int, default=0, help="0: autotuned at set-up")
+    ap.add_argument("--gather-batch", type=int, default=0, help="0: autotuned at set-up")
     ap.add_argument("--res-eager", action="store_true",
                     help="launch the residual-targeted leg eagerly instead of replaying its step graphs "
                          "(for kernel traces of that leg; not the measured configuration)")
@@ -163,7 +166,8 @@ def main():
     print(f"[bench] workload {sc.name}: {sc.n_vertices} vertices, {sc.n_tets} tets, "
           f"{sc.n_ground} ground contacts, {sc.iters} iterations", file=sys.stderr, flush=True)
     t_setup = time.perf_counter()
-    sim = Simulation(sc)
+    launch = dict(lanes_per_vertex=args.lanes_per_vertex, block_size=args.block_size, gather_batch=args.gather_batch)
+    sim = Simulation(sc, **launch)
     sim.setup()
     t_setup = time.perf_counter() - t_setup
     lanes, block, batch = sim.ctx.launch_config()
@@ -230,7 +234,7 @@ def main():
     # and (when trained parameters exist) warm-started by the network
     def residual_leg(params=None):
         rsim = Simulation(make_scene(args.workload), residual_target=(args.tol, args.max_iters, args.check_every),
-                          use_graph=not args.res_eager)
+                          use_graph=not args.res_eager, **launch)
         rsim.setup()
         if params is not None:
             rsim.ctx.mlp_set_params(params)
