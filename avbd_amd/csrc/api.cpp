@@ -120,6 +120,10 @@ struct avbd_ctx {
         size_t bytes = 0;
     };
     Grow g_ops, g_active, g_off, g_ent, g_conf;
+    // pinned staging for a detection round's uploads (written only after the
+    // round's stream synchronisation, so one buffer serves every round)
+    uint8_t* stage = nullptr;
+    size_t stage_cap = 0, stage_used = 0;
 
     // iterations per step: the budget in residual mode, else solver_iters
     uint32_t iters() const { return res_tol > 0 ? res_max_iters : desc.solver_iters; }
@@ -607,6 +611,36 @@ void capture_graph(avbd_ctx* c) {
     c->graph_exec = capture(c, [&] { enqueue_step(c, false); });
 }
 
+// Pinned staging of a detection round's host arrays: reserve the round's
+// total first (may reallocate: the stream is idle then), then each array is
+// copied in and sent with a truly asynchronous H2D copy.
+void stage_reserve(avbd_ctx* c, size_t bytes) {
+    c->stage_used = 0;
+    if (bytes <= c->stage_cap) return;
+    if (c->stage) hip_check(hipHostFree(c->stage), "hipHostFree");
+    c->stage = nullptr;
+    c->stage_cap = 0;
+    void* p = nullptr;
+    const size_t cap = bytes + bytes / 2 + 4096;
+    hip_check(hipHostMalloc(&p, cap, hipHostMallocDefault), "hipHostMalloc");
+    c->stage = static_cast<uint8_t*>(p);
+    c->stage_cap = cap;
+}
+
+constexpr size_t kStageAlign = 256;
+inline size_t stage_bytes(size_t bytes) { return (bytes + kStageAlign - 1) / kStageAlign * kStageAlign; }
+
+template <class T>
+void stage_upload(avbd_ctx* c, T* dst, const std::vector<T>& src) {
+    if (src.empty()) return;
+    const size_t bytes = src.size() * sizeof(T);
+    if (c->stage_used + stage_bytes(bytes) > c->stage_cap) throw std::logic_error("staging buffer overrun");
+    uint8_t* p = c->stage + c->stage_used;
+    std::memcpy(p, src.data(), bytes);
+    c->stage_used += stage_bytes(bytes);
+    hip_check(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync H2D");
+}
+
 // One round of CollisionDetector::detectCollisionsAndRecolor_Parallel
 // (collision/CollisionDetector.cpp:57-129) at the start of a step: device
 // bounds, LBVH and narrow phase; the detected list comes back to the host
@@ -636,24 +670,33 @@ void detect_round(avbd_ctx* c) {
     }
     const bool relayout = c->cs.process(det, c->prims, c->mass, dt, c->plan);
     ContactSystem& cs = c->cs;
+    // the stream is idle here (synchronised above): reserve the staging for
+    // every upload of this round, then queue them; the solve follows on the
+    // same stream, so nothing waits for the copies on the host
+    size_t total = stage_bytes(cs.ops.size() * sizeof(DynOp)) + stage_bytes(cs.active.size() * sizeof(uint32_t)) +
+                   stage_bytes(cs.dyn_off.size() * sizeof(uint32_t)) + stage_bytes(cs.dyn_ent.size() * sizeof(uint32_t));
+    if (relayout)
+        total += stage_bytes(c->plan.items.size() * sizeof(VItem)) + stage_bytes(c->plan.oitems.size() * sizeof(OItem)) +
+                 stage_bytes(c->plan.conflict_list.size() * sizeof(uint32_t));
+    stage_reserve(c, total);
     if (!cs.ops.empty()) {
         auto* d_ops = static_cast<DynOp*>(grow(c, c->g_ops, cs.ops.size() * sizeof(DynOp)));
-        hip_check(hipMemcpyAsync(d_ops, cs.ops.data(), cs.ops.size() * sizeof(DynOp), hipMemcpyHostToDevice, s), "H2D");
+        stage_upload(c, d_ops, cs.ops);
         launch_dyn_apply(s, c->d, d_ops, (uint32_t)cs.ops.size());
     }
     auto* d_act = static_cast<uint32_t*>(grow(c, c->g_active, cs.active.size() * sizeof(uint32_t)));
-    upload(c, d_act, cs.active);
+    stage_upload(c, d_act, cs.active);
     auto* d_off = static_cast<uint32_t*>(grow(c, c->g_off, cs.dyn_off.size() * sizeof(uint32_t)));
-    upload(c, d_off, cs.dyn_off);
+    stage_upload(c, d_off, cs.dyn_off);
     auto* d_ent = static_cast<uint32_t*>(grow(c, c->g_ent, cs.dyn_ent.size() * sizeof(uint32_t)));
-    upload(c, d_ent, cs.dyn_ent);
+    stage_upload(c, d_ent, cs.dyn_ent);
     c->d.dyn_off = d_off;
     c->d.dyn_ent = d_ent;
     if (relayout) {
-        upload(c, c->d.items, c->plan.items);
-        upload(c, c->d.oitems, c->plan.oitems);
+        stage_upload(c, c->d.items, c->plan.items);
+        stage_upload(c, c->d.oitems, c->plan.oitems);
         c->d_conflicts = static_cast<uint32_t*>(grow(c, c->g_conf, c->plan.conflict_list.size() * sizeof(uint32_t)));
-        upload(c, c->d_conflicts, c->plan.conflict_list);
+        stage_upload(c, c->d_conflicts, c->plan.conflict_list);
         const size_t need = 2 * (size_t)c->iters() * c->plan.num_colors;
         while (c->events.size() < need) {
             hipEvent_t ev;
@@ -661,8 +704,6 @@ void detect_round(avbd_ctx* c) {
             c->events.push_back(ev);
         }
     }
-    // the host vectors are reused next round: the copies must have finished
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
 // Autotuner: time the colour-phase solve of one iteration for each candidate
@@ -800,6 +841,7 @@ void avbd_destroy(avbd_ctx* c) {
     for (auto ev : c->events) (void)hipEventDestroy(ev);
     for (auto ev : c->part_ev) (void)hipEventDestroy(ev);
     if (c->h_conv) (void)hipHostFree(c->h_conv);
+    if (c->stage) (void)hipHostFree(c->stage);
     if (c->ev_st0) (void)hipEventDestroy(c->ev_st0);
     if (c->ev_st1) (void)hipEventDestroy(c->ev_st1);
     if (c->ev_lb0) (void)hipEventDestroy(c->ev_lb0);
