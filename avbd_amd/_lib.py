@@ -154,6 +154,7 @@ class Stats(C.Structure):
         ("pad_", C.c_uint32),
         ("warm_start_predictions", C.c_uint64),
         ("warm_start_fallbacks", C.c_uint64),
+        ("steps_at_budget", C.c_uint64),
     ]
 
 

@@ -1942,6 +1942,7 @@ int avbd_get_stats(avbd_ctx* c, avbd_stats* out) {
             st.iterations_total = rc.total_iters;
             st.last_step_iterations = rc.last_iters;
             st.vertex_updates = rc.total_iters * (uint64_t)(c->d.n_items + c->d.n_oitems);
+            st.steps_at_budget = rc.budget_steps;
         } else {
             st.iterations_total = c->steps_taken * c->desc.solver_iters;
             st.last_step_iterations = c->steps_taken ? c->desc.solver_iters : 0;

@@ -240,6 +240,7 @@ struct ResCtl {
     uint32_t done;         // set when the current step has converged
     uint32_t last_iters;   // iterations used by the last step
     unsigned long long total_iters;
+    unsigned long long budget_steps;   // steps that ended at max_iters unconverged
 };
 
 struct StepArgs {

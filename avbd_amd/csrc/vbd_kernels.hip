@@ -814,6 +814,7 @@ __global__ void residual_finish_kernel(ResCtl* rc, uint32_t max_iters) {
     if (threadIdx.x == 0) {
         if (!rc->done) rc->last_iters = max_iters;
         rc->total_iters += rc->last_iters;
+        if (!rc->done) rc->budget_steps += 1;
     }
 }
 
@@ -1271,6 +1272,7 @@ __global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
         a.rc->done = iters_used < a.iters ? 1u : 0u;
         a.rc->last_iters = iters_used;
         a.rc->total_iters += iters_used;
+        if (iters_used >= a.iters) a.rc->budget_steps += 1;
     }
     for (uint32_t p = gtid; p < d.n; p += gthreads) velocity_one(d, p, inv_dt);
 }

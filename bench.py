@@ -252,6 +252,7 @@ def main():
             "steps_per_s": round(args.res_steps / rel, 3),
             "ms_per_step": round(rel / args.res_steps * 1e3, 4),
             "mean_iterations": round(iters / args.res_steps, 2),
+            "steps_at_budget": int(s1.steps_at_budget - s0.steps_at_budget),
             "vertex_updates_per_s": round((s1.vertex_updates - s0.vertex_updates) / rel, 1),
             "last_step_iterations": s1.last_step_iterations,
             "last_step_residual_history": [float(f"{v:.4g}") for v in rsim.ctx.residual_history()],

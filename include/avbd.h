@@ -180,6 +180,8 @@ typedef struct {
     uint32_t pad_;
     uint64_t warm_start_predictions; /* scene-steps started from the network    */
     uint64_t warm_start_fallbacks;   /* of those, kept the inertial start       */
+    uint64_t steps_at_budget;        /* residual mode: steps that used max_iters
+                                        without reaching the tolerance           */
 } avbd_stats;
 
 /* Context lifetime -------------------------------------------------------- */

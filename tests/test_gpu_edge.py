@@ -49,6 +49,7 @@ def test_residual_target_first_check_and_budget(tol, expect):
     banner(f"  last step {st.last_step_iterations} iterations, total {st.iterations_total}")
     assert st.last_step_iterations == want
     assert st.iterations_total == 7 * want
+    assert st.steps_at_budget == (0 if expect == "first" else 7)
     x = runs["graph"].positions()
     for name in ("eager", "separate"):
         assert runs[name].ctx.stats().iterations_total == st.iterations_total, name
