@@ -256,6 +256,20 @@ void oracle_set_ground(void* h, const double* rec14) {
     }
 }
 
+// One ground contact on particle p with the initial values of
+// CollisionConstraintEnergyPool::addEnergy (CollisionConstraintEnergyPool.hpp:48-69).
+void oracle_add_ground(void* h, uint32_t p, double k_start, double mu_s, double mu_k) {
+    auto* s = static_cast<Simulation*>(h);
+    GroundContact c{};
+    c.p = p;
+    c.k_start = c.k_n = c.k_t = c.k_b = k_start;
+    c.mu_s = mu_s;
+    c.mu_k = mu_k;
+    c.use_static = false;
+    c.unclamped_lambda_tb = false;
+    s->ground.push_back(c);
+}
+
 uint32_t oracle_num_colors(void* h) { return static_cast<Simulation*>(h)->coloring.num_colors; }
 void oracle_get_colors(void* h, uint32_t* color) {
     auto* s = static_cast<Simulation*>(h);

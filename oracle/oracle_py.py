@@ -47,6 +47,7 @@ _SIG = {
     "oracle_set_tet_eprev": (None, [C.c_void_p, _D]),
     "oracle_get_ground": (None, [C.c_void_p, _U32, _D]),
     "oracle_set_ground": (None, [C.c_void_p, _D]),
+    "oracle_add_ground": (None, [C.c_void_p, C.c_uint32, C.c_double, C.c_double, C.c_double]),
     "oracle_num_colors": (C.c_uint32, [C.c_void_p]),
     "oracle_get_colors": (None, [C.c_void_p, _U32]),
     "oracle_num_adj_entries": (C.c_uint32, [C.c_void_p]),
@@ -204,6 +205,15 @@ class OracleSim:
         rec = np.empty((ng, 14))
         lib.oracle_get_ground(self.h, up(p), dp(rec))
         return p, rec
+
+    def set_ground(self, rec):
+        """Overwrite every ground record (rows of 14, the layout of ground())."""
+        r = np.ascontiguousarray(rec, dtype=np.float64)
+        lib.oracle_set_ground(self.h, dp(r))
+
+    def add_ground(self, p, k_start, mu_s=0.4, mu_k=0.2):
+        """A ground contact on particle p (CollisionConstraintEnergyPool::addEnergy)."""
+        lib.oracle_add_ground(self.h, p, k_start, mu_s, mu_k)
 
     def adjacency(self):
         n = self.n
