@@ -34,6 +34,8 @@ FLAG_SERIAL_BROADPHASE = 2
 FLAG_NO_PERSISTENT = 4
 FLAG_PERSISTENT = 8
 FLAG_SEPARATE_CHEBYSHEV = 16
+FLAG_FP64_STATE = 32
+FLAG_HOST_COLORING = 64
 
 
 class SceneDesc(C.Structure):
@@ -204,6 +206,7 @@ SIGNATURES = {
     "avbd_add_mass": (C.c_int, [_P, C.c_uint32, C.c_double, _D]),
     "avbd_set_residual_target": (C.c_int, [_P, C.c_double, C.c_uint32, C.c_uint32]),
     "avbd_get_residual_history": (C.c_int, [_P, _D, C.c_uint32, _U32]),
+    "avbd_get_scene_residual_history": (C.c_int, [_P, _D, C.c_uint32, _U32]),
     "avbd_mlp_param_count": (C.c_uint64, []),
     "avbd_mlp_init": (C.c_int, [_P, C.c_uint64]),
     "avbd_mlp_set_params": (C.c_int, [_P, _F, C.c_uint64]),
@@ -220,6 +223,7 @@ SIGNATURES = {
                                     C.c_double, C.c_int, C.c_double, C.c_double, _U32]),
     "avbd_set_particle_state": (C.c_int, [_P, C.c_uint32, C.c_uint32, _D, _D, _D, _D]),
     "avbd_set_fixed": (C.c_int, [_P, C.c_uint32, C.c_uint32, _U8]),
+    "avbd_set_external_accel": (C.c_int, [_P, C.c_uint32, C.c_uint32, _D]),
     "avbd_set_scenes": (C.c_int, [_P, C.c_uint32, _U32]),
     "avbd_build_graph": (C.c_int, [_P]),
     "avbd_step": (C.c_int, [_P, C.c_uint32]),

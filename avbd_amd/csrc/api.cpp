@@ -31,6 +31,7 @@ struct avbd_ctx {
     // FP64 staging (ParticlePool / EnergyRegistry layout, reference Real = double)
     std::vector<double> x, v, v_prev, x_prev, mass;
     std::vector<uint8_t> fixed;
+    std::vector<double> f_ext;   // per-particle external acceleration (3n; empty = none)
     std::vector<HostNH> nh;
     std::vector<HostGround> ground;
     std::vector<uint32_t> scene_offsets;   // empty = one scene
@@ -52,23 +53,22 @@ struct avbd_ctx {
     uint32_t reduce_blocks = 0;
     LbvhDev lb;
 
-    // persistent step kernel
-    bool persistent = false;
-    uint32_t pgrid = 0;
-    StepArgs sargs;
-    hipEvent_t ev_st0 = nullptr, ev_st1 = nullptr;
-    double step_kernel_avg_ms = 0;
+    // FP64 particle state (AVBD_FLAG_FP64_STATE)
+    bool fp64 = false;
+    F64State f;
+    // a non-finite state was reported by the device (sticky until the state is reset)
+    bool nonfinite = false;
 
     LaunchCfg cfg;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    // residual mode: the step's parts (see capture_graph), an event after
-    // each check interval, and per check the converged flag in pinned host
-    // memory (written by the check's decision kernel)
-    std::vector<hipGraphExec_t> part_exec;
+    // residual mode: the step's graphs (head + first check interval, one
+    // further interval replayed as often as needed, tail; see capture_graph),
+    // an event after each interval, and per interval its converged flag,
+    // copied back to pinned host memory after the interval
+    hipGraphExec_t head_exec = nullptr, interval_exec = nullptr, tail_exec = nullptr;
     std::vector<hipEvent_t> part_ev;
     uint32_t* h_conv = nullptr;
-    uint32_t* d_conv = nullptr;
     std::vector<float> omega_seq;   // Chebyshev weight of every iteration
     // Chebyshev / dual update fused into each iteration's last colour phase:
     // positions alternate between d.x (even iterations) and x_alt
@@ -91,9 +91,14 @@ struct avbd_ctx {
     // residual-targeted iteration (avbd_set_residual_target)
     double res_tol = 0;            // 0: fixed iteration count
     uint32_t res_max_iters = 0, res_check_every = 0;
-    double res_tol2 = 0;           // tol^2 ||M g||^2
     ResCtl* rc = nullptr;
-    double* res_partial = nullptr; // per-block partial sums of a residual check
+    uint32_t n_res_scenes = 0;
+    double res_mg2 = 0;            // ||M g||^2 over every free particle
+    double* res_rr = nullptr;      // |r_v|^2 per particle
+    uint32_t* res_scene_off = nullptr;
+    double* res_mg2_scene = nullptr;
+    double* res_acc_scene = nullptr;
+    float* omega_tab = nullptr;    // Chebyshev weight per iteration of a step
 
     // learned warm start (mlp_host.cpp): the network, its in-step buffers,
     // and the recording of training data
@@ -143,6 +148,27 @@ struct CodeError : std::runtime_error {
 
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+const char* nonfinite_message() {
+    return "a time step produced a non-finite position or velocity (NaN/Inf); the state stays unusable until it "
+           "is set again (avbd_set_particle_state, avbd_load_checkpoint)";
+}
+
+// After a synchronisation: the device's non-finite flag (sticky on the host
+// until the state is set again).
+void check_status(avbd_ctx* c) {
+    if (!c->built || !c->d.status) return;
+    uint32_t st = 0;
+    hip_check(hipMemcpy(&st, c->d.status, sizeof st, hipMemcpyDeviceToHost), "D2H");
+    if (st & 1u) c->nonfinite = true;
+    if (c->nonfinite) throw CodeError(AVBD_E_NONFINITE, nonfinite_message());
+}
+
+// The state was overwritten from the host: clear the non-finite flag.
+void clear_status(avbd_ctx* c) {
+    c->nonfinite = false;
+    if (c->d.status) hip_check(hipMemsetAsync(c->d.status, 0, sizeof(uint32_t), c->stream), "hipMemsetAsync");
 }
 
 template <class F>
@@ -295,6 +321,7 @@ void push_particle(avbd_ctx* c, const double* x, const double* v, const double* 
     c->mass.push_back(mass);
     c->fixed.push_back(fixed);
     c->rot_of.push_back(-1);
+    if (!c->f_ext.empty()) c->f_ext.insert(c->f_ext.end(), 3, 0.0);
 }
 
 // Append one oriented particle (ParticlePool::addOrientedParticle, common/ParticlePool.hpp:248-263).
@@ -351,6 +378,24 @@ void upload_particle_range(avbd_ctx* c, uint32_t first, uint32_t n) {
     cp(c->d.y, bx);
     cp(c->d.x_last, bx);
     cp(c->d.x_lastlast, bx);
+    if (c->fp64) {
+        // the FP64 streams take the staged values exactly
+        std::vector<double4> dx(n), dxp(n);
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t p = first + i;
+            dx[i] = make_double4(c->x[3 * p], c->x[3 * p + 1], c->x[3 * p + 2], c->mass[p]);
+            dxp[i] = make_double4(c->x_prev[3 * p], c->x_prev[3 * p + 1], c->x_prev[3 * p + 2], 0.0);
+        }
+        auto cpd = [&](double4* dst, const std::vector<double4>& src) {
+            hip_check(hipMemcpyAsync(dst + first, src.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream),
+                      "hipMemcpyAsync H2D");
+        };
+        cpd(c->f.x, dx);
+        cpd(c->f.x_prev, dxp);
+        cpd(c->f.y, dx);
+        cpd(c->f.x_last, dx);
+        cpd(c->f.x_lastlast, dx);
+    }
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
 }
 
@@ -413,102 +458,125 @@ double omega_at(uint32_t i, double rho, double omega) {
     return 1;
 }
 
-// Enqueue part of one time step (VBDSolver::solve_Parallel,
-// simulation/VBDSolver.hpp:48-103): with `head`, the start-of-step work; then
-// iterations [i0, i1) (each followed by its residual check in residual
-// mode); with `tail`, the end-of-step work.  `join` makes the solve stream
-// wait for the concurrent broad phase at the end of this part.
-void enqueue_part(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool head, bool tail, bool join) {
+// One time step (VBDSolver::solve_Parallel, simulation/VBDSolver.hpp:48-103) is
+// enqueued as: the head (start-of-step work), iterations, the tail
+// (end-of-step work).  In residual-targeted mode the iterations come in check
+// intervals of res_check_every iterations, each followed by a residual check;
+// the Chebyshev launches then read their weight from the device table at the
+// step's iteration counter, so every interval is the same launch sequence.
+
+// The broad phase that runs before the solve (CollisionDetector::
+// detectCollisionsAndRecolor_Parallel).  Triangle-triangle pairs produce no
+// contact energy (the reference's narrow phase for them is a no-op,
+// collision/CollisionDetector.cpp:589-599), so nothing in this step's solve
+// consumes the broad-phase output: after the bounds kernel has read the
+// start-of-step state, the rest of the build and the traversal run on a
+// second stream, concurrently with the solve, and join later in the step.
+// (With dynamic contacts the detection round has already run, see detect_round.)
+bool forks_broadphase(const avbd_ctx* c) { return c->lb.n && !c->dynamic && c->side_stream; }
+
+void join_broadphase(avbd_ctx* c) {
+    if (forks_broadphase(c)) hip_check(hipStreamWaitEvent(c->stream, c->ev_join, 0), "hipStreamWaitEvent");
+}
+
+void enqueue_head(avbd_ctx* c, bool with_events) {
     const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
     hipStream_t s = c->stream;
-    // CollisionDetector::detectCollisionsAndRecolor_Parallel runs before the solve.
-    // Triangle-triangle pairs produce no contact energy (the reference's narrow
-    // phase for them is a no-op, collision/CollisionDetector.cpp:589-599), so
-    // nothing in this step's solve consumes the broad-phase output: after the
-    // bounds kernel has read the start-of-step state, the rest of the build
-    // and the traversal run on a second stream, concurrently with the solve,
-    // and join later in the step.
-    // (with dynamic contacts the detection round has already run, see detect_round)
     const bool tri_bp = c->lb.n && !c->dynamic;
-    const bool fork = tri_bp && c->side_stream;
-    // learned warm start / recording: the start-of-step phase runs as separate
-    // launches, then the features, the network and the choice of the start
-    const bool pre = c->warm || c->recording;
-    WarmIO io = c->wio;
-    if (pre) {
+    const bool fork = forks_broadphase(c);
+    if (tri_bp) {
+        hipStream_t sb = fork ? c->side_stream : s;
+        if (with_events) hip_check(hipEventRecord(c->ev_lb0, s), "hipEventRecord");
+        launch_lbvh_bounds(s, c->lb, c->d.x, c->d.v, c->desc.dt);
+        if (fork) {
+            hip_check(hipEventRecord(c->ev_fork, s), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(sb, c->ev_fork, 0), "hipStreamWaitEvent");
+        }
+        launch_lbvh_tree(sb, c->lb);
+        if (with_events) hip_check(hipEventRecord(c->ev_lb1, sb), "hipEventRecord");
+        if (fork) hip_check(hipEventRecord(c->ev_join, sb), "hipEventRecord");
+    }
+    if (c->damping) {
+        if (c->fp64) launch_eprev_f64(s, c->f, c->d.n_tets);
+        else launch_eprev(s, c->d, dt);
+    }
+    // dynamic contacts' updateAfterTimeStep reads the start-of-step positions
+    const uint32_t n_active = c->dynamic ? (uint32_t)c->cs.active.size() : 0u;
+    if (n_active) launch_dyn_step_begin(s, c->d, static_cast<const uint32_t*>(c->g_active.p), n_active);
+    if (c->fp64) launch_step_begin_f64(s, c->f, c->d.n, c->desc.dt, c->desc.g_accel);
+    else launch_step_begin(s, c->d, dt, g);
+    // learned warm start / recording: features, the network, the choice of the start
+    if (c->warm || c->recording) {
+        WarmIO io = c->wio;
         io.raw = c->recording ? c->d_raw : nullptr;
         io.tgt = c->d_tgt;
         io.norm = c->mlp->d_norm;
         if (c->warm) io.out = c->mlp->z32[kMlpLayers - 1];
-    }
-    if (head) {
-        if (tri_bp) {
-            hipStream_t sb = fork ? c->side_stream : s;
-            if (with_events) hip_check(hipEventRecord(c->ev_lb0, s), "hipEventRecord");
-            launch_lbvh_bounds(s, c->lb, c->d.x, c->d.v, c->desc.dt);
-            if (fork) {
-                hip_check(hipEventRecord(c->ev_fork, s), "hipEventRecord");
-                hip_check(hipStreamWaitEvent(sb, c->ev_fork, 0), "hipStreamWaitEvent");
-            }
-            launch_lbvh_tree(sb, c->lb);
-            if (with_events) hip_check(hipEventRecord(c->ev_lb1, sb), "hipEventRecord");
-            if (fork) hip_check(hipEventRecord(c->ev_join, sb), "hipEventRecord");
-        }
-        if (pre) {
-            if (c->damping) launch_eprev(s, c->d, dt);
-            launch_step_begin(s, c->d, dt, g);
-            launch_warm_save(s, c->d, io);
-            launch_warm_features(s, c->d, io, c->cfg.lanes, dt, g, c->rows_pad);
-            if (c->warm) {
-                c->mlp->forward((int)c->rows_pad, io.x16, false);
-                launch_warm_apply(s, c->d, io, dt * dt * g);
-                launch_warm_ip(s, c->d, io, c->damping, dt);
-                launch_warm_choose(s, c->d, io);
-            }
-        }
-        if (c->persistent) {
-            // the whole solve in one launch with grid barriers between phases
-            if (!tail || i0 != 0 || i1 != c->iters()) throw std::logic_error("persistent step enqueued in parts");
-            if (with_events) hip_check(hipEventRecord(c->ev_st0, s), "hipEventRecord");
-            StepArgs a = c->sargs;
-            a.skip_begin = pre ? 1u : 0u;
-            launch_step_persistent(s, a, c->cfg.lanes, c->damping, c->pgrid);
-            if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
-            if (with_events) hip_check(hipEventRecord(c->ev_st1, s), "hipEventRecord");
-            if (fork && join) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
-            hip_check(hipGetLastError(), "kernel launch");
-            return;
+        launch_warm_save(s, c->d, io);
+        if (c->fp64) launch_warm_save_f64(s, c->f, c->d.n);
+        launch_warm_features(s, c->d, io, c->cfg.lanes, dt, g, c->rows_pad);
+        if (c->warm) {
+            c->mlp->forward((int)c->rows_pad, io.x16, false);
+            launch_warm_apply(s, c->d, io, dt * dt * g);
+            launch_warm_ip(s, c->d, io, c->damping, dt);
+            launch_warm_choose(s, c->d, io);
+            if (c->fp64) launch_warm_place_f64(s, c->f, io.pred, io.scene_of, io.step_of);
         }
     }
+    if (c->rc) launch_residual_reset(s, c->rc);
+    hip_check(hipGetLastError(), "kernel launch");
+}
+
+// Iterations [i0, i1) of a step.  With `check` (residual mode: i0 = 0,
+// i1 = res_check_every) the Chebyshev weights come from the device table and
+// a residual check follows the last iteration.
+void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool check) {
+    const float dt = (float)c->desc.dt;
+    hipStream_t s = c->stream;
+    const uint32_t* done = c->rc ? &c->rc->done : nullptr;
+    const ResCtl* rc = c->rc;
     const uint32_t* d_active = static_cast<const uint32_t*>(c->g_active.p);
     const uint32_t n_active = c->dynamic ? (uint32_t)c->cs.active.size() : 0u;
-    if (head && !pre) {
-        if (c->damping) launch_eprev(s, c->d, dt);
-        // dynamic contacts' updateAfterTimeStep reads the start-of-step positions
-        if (n_active) launch_dyn_step_begin(s, c->d, d_active, n_active);
-        launch_step_begin(s, c->d, dt, g);
-    }
-    const uint32_t* done = c->rc ? &c->rc->done : nullptr;
-    if (head && c->rc) launch_residual_reset(s, c->rc);
-    const uint32_t iters = c->iters();
     const bool fuse = c->fuse_cheb && !c->dynamic;
     size_t ev = (size_t)2 * i0 * c->plan.num_colors;
+    const uint32_t last = c->plan.num_colors - 1;
     for (uint32_t i = i0; i < i1; i++) {
         const float omega = c->omega_seq[i];
+        const uint32_t j = i - i0;
+        const bool odd = (i & 1u) != 0;
+        if (c->fp64) {
+            F64State fi = c->f;
+            double4* x_next = c->f.x;
+            if (fuse) {
+                fi.x = odd ? c->f.x_alt : c->f.x;
+                x_next = odd ? c->f.x : c->f.x_alt;
+            }
+            for (uint32_t col = 0; col < c->plan.num_colors; col++) {
+                const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
+                if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
+                if (fuse && col == last)
+                    launch_solve_cheb_f64(s, fi, c->cfg, b, e - b, c->damping, c->desc.dt, omega, rc, c->omega_tab, j,
+                                          c->cheb_list, c->n_cheb, fi.x, x_next);
+                else
+                    launch_solve_color_f64(s, fi, c->cfg, b, e - b, c->damping, c->desc.dt, fi.x, done);
+                if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
+            }
+            if (!fuse) launch_cheb_dual_f64(s, c->f, c->d.n, omega, rc, c->omega_tab, j);
+            continue;
+        }
         // fused: iteration i works on x_cur and extrapolates into x_next
         DevState di = c->d;
         float4* x_next = c->d.x;
         if (fuse) {
-            di.x = (i & 1u) ? c->x_alt : c->d.x;
-            x_next = (i & 1u) ? c->d.x : c->x_alt;
+            di.x = odd ? c->x_alt : c->d.x;
+            x_next = odd ? c->d.x : c->x_alt;
         }
-        const uint32_t last = c->plan.num_colors - 1;
         for (uint32_t col = 0; col < c->plan.num_colors; col++) {
             const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
             if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
             if (fuse && col == last)
-                launch_solve_cheb(s, di, c->cfg, b, e - b, c->damping, dt, omega, c->cheb_list, c->n_cheb, x_next,
-                                  done);
+                launch_solve_cheb(s, di, c->cfg, b, e - b, c->damping, dt, omega, rc, c->omega_tab, j, c->cheb_list,
+                                  c->n_cheb, x_next);
             else
                 launch_solve_color(s, di, c->cfg, b, e - b, c->damping, dt, di.x, done);
             const uint32_t ob = c->plan.oitem_offsets[col], oe = c->plan.oitem_offsets[col + 1];
@@ -517,52 +585,66 @@ void enqueue_part(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool 
             const uint32_t cb = c->plan.conflict_offsets[col], ce = c->plan.conflict_offsets[col + 1];
             if (ce > cb) launch_commit(s, di, c->d_conflicts + cb, ce - cb, done);
         }
-        if (!fuse) launch_cheb_dual(s, c->d, omega, done);
+        if (!fuse) launch_cheb_dual(s, c->d, omega, rc, c->omega_tab, j);
         if (n_active) launch_dyn_dual(s, c->d, d_active, n_active, done);
-        if (c->rc && (i + 1) % c->res_check_every == 0 && i + 1 < iters) {
-            DevState dr = c->d;
-            dr.x = x_next;
-            launch_residual_check(s, dr, c->cfg, c->damping, dt, c->rc, c->res_partial,
-                                  (i + 1) / c->res_check_every - 1, i + 1, c->res_tol2, c->d_conv);
-        }
     }
-    if (tail) {
-        if (c->rc) launch_residual_finish(s, c->rc, iters);
-        // fused: the final positions are in x_alt after an odd number of iterations
-        if (fuse) {
-            if (c->rc && ((c->res_check_every & 1u) || (iters & 1u)))
-                launch_settle(s, c->d.x, c->x_alt, c->d.n, &c->rc->last_iters, 0);
-            else if (!c->rc && (iters & 1u))
-                launch_settle(s, c->d.x, c->x_alt, c->d.n, nullptr, iters);
-        }
-        if (c->recording) launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
-        launch_velocity(s, c->d, dt);
+    if (check) {
+        // an interval has an even number of iterations whenever the step is
+        // fused (see avbd_build_graph), so the iterate is back in x
+        if (c->fp64) launch_residual_f64(s, c->f, c->cfg, c->damping, c->desc.dt, c->f.x, c->res_rr, c->rc);
+        else launch_residual_pass(s, c->d, c->cfg, c->damping, dt, c->rc, c->res_rr);
+        launch_residual_decide(s, c->rc, c->res_rr, c->res_scene_off, c->n_res_scenes, c->res_acc_scene,
+                               c->res_mg2_scene, c->res_tol * c->res_tol, c->res_check_every);
     }
-    if (fork && join) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
     hip_check(hipGetLastError(), "kernel launch");
 }
 
-// Enqueue one whole time step.
-void enqueue_step(avbd_ctx* c, bool with_events) { enqueue_part(c, with_events, 0, c->iters(), true, true, true); }
-
-// After a synchronisation: report (and clear) an aborted persistent step.
-void check_abort(avbd_ctx* c) {
-    if (!c->persistent || !c->sargs.bar) return;
-    GridBarrier b{};
-    hip_check(hipMemcpy(&b, c->sargs.bar, sizeof(b), hipMemcpyDeviceToHost), "D2H");
-    if (b.abort) {
-        hip_check(hipMemset(c->sargs.bar, 0, sizeof(GridBarrier)), "hipMemset");
-        throw HipError("persistent step kernel: a grid barrier wait timed out and the step was aborted");
+void enqueue_tail(avbd_ctx* c) {
+    const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
+    hipStream_t s = c->stream;
+    const uint32_t iters = c->iters();
+    if (c->rc) launch_residual_finish(s, c->rc);
+    // fused: the final positions are in x_alt after an odd number of
+    // iterations (never in residual mode, whose intervals are even)
+    if (c->fuse_cheb && !c->dynamic && !c->rc && (iters & 1u)) {
+        if (c->fp64) launch_settle_f64(s, c->f.x, c->f.x_alt, c->d.n, nullptr, iters);
+        else launch_settle(s, c->d.x, c->x_alt, c->d.n, nullptr, iters);
     }
+    if (c->recording) {
+        if (c->fp64) launch_warm_target_f64(s, c->f, c->d_tgt, 1.0 / (c->desc.dt * c->desc.dt * c->desc.g_accel));
+        else {
+            WarmIO io = c->wio;
+            io.tgt = c->d_tgt;
+            launch_warm_target(s, c->d, io, 1.0f / (dt * dt * g));
+        }
+    }
+    if (c->fp64) launch_velocity_f64(s, c->f, c->d.n, c->desc.dt, c->d.status);
+    else launch_velocity(s, c->d, dt);
+    hip_check(hipGetLastError(), "kernel launch");
+}
+
+// One whole step at the fixed iteration count (or, in residual mode, the
+// eager form: every interval with its check, the done flag turning the rest
+// into no-ops).
+void enqueue_step(avbd_ctx* c, bool with_events) {
+    enqueue_head(c, with_events);
+    if (c->rc) {
+        for (uint32_t k = 0; k < c->iters() / c->res_check_every; k++)
+            enqueue_iters(c, false, 0, c->res_check_every, true);
+    } else {
+        enqueue_iters(c, with_events, 0, c->iters(), false);
+    }
+    enqueue_tail(c);
+    join_broadphase(c);
 }
 
 void destroy_graph(avbd_ctx* c) {
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    for (hipGraphExec_t* e : {&c->graph_exec, &c->head_exec, &c->interval_exec, &c->tail_exec}) {
+        if (*e) (void)hipGraphExecDestroy(*e);
+        *e = nullptr;
+    }
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
     c->graph = nullptr;
-    for (hipGraphExec_t e : c->part_exec) (void)hipGraphExecDestroy(e);
-    c->part_exec.clear();
 }
 
 template <class F>
@@ -585,30 +667,58 @@ hipGraphExec_t capture(avbd_ctx* c, F&& body) {
     return e;
 }
 
-// Residual mode replays a step in parts: the start of the step with the
-// first check interval (the broad phase joins there), one graph per further
-// interval, and the end of the step; the host stops launching intervals
-// once a check reports convergence (avbd_step).
-bool stepped_in_parts(const avbd_ctx* c) { return c->rc && !c->persistent && !c->dynamic && c->h_conv; }
-
+// Residual mode replays three graphs per step: the head with the first check
+// interval (the concurrent broad phase joins there), one interval replayed as
+// often as the host decides (avbd_step), and the tail.
 void capture_graph(avbd_ctx* c) {
     destroy_graph(c);
-    if (stepped_in_parts(c)) {
-        const uint32_t iters = c->iters(), every = c->res_check_every;
-        const uint32_t parts = (iters + every - 1) / every;
-        for (uint32_t k = 0; k < parts; k++) {
-            const uint32_t i0 = k * every, i1 = std::min(iters, i0 + every);
-            c->part_exec.push_back(capture(c, [&] { enqueue_part(c, false, i0, i1, k == 0, false, k == 0); }));
-        }
-        c->part_exec.push_back(capture(c, [&] { enqueue_part(c, false, iters, iters, false, true, false); }));
-        while (c->part_ev.size() < parts) {
-            hipEvent_t e = nullptr;
-            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-            c->part_ev.push_back(e);
-        }
+    if (c->rc && !c->dynamic) {
+        const uint32_t every = c->res_check_every;
+        c->head_exec = capture(c, [&] {
+            enqueue_head(c, false);
+            enqueue_iters(c, false, 0, every, true);
+            join_broadphase(c);
+        });
+        c->interval_exec = capture(c, [&] { enqueue_iters(c, false, 0, every, true); });
+        c->tail_exec = capture(c, [&] { enqueue_tail(c); });
         return;
     }
     c->graph_exec = capture(c, [&] { enqueue_step(c, false); });
+}
+
+// Residual mode, one step: interval k is queued once the check of interval
+// k - kAhead has been seen unconverged, so at most kAhead - 1 intervals of
+// early-exiting launches follow the converged check instead of the rest of
+// the iteration budget.  Each interval's converged flag is copied to pinned
+// host memory behind the interval (a stream-ordered copy), then an event is
+// recorded.  `graphs` replays the captured graphs; otherwise the same
+// sequence is launched eagerly.
+void run_residual_step(avbd_ctx* c, bool graphs) {
+    constexpr uint32_t kAhead = 2;
+    const uint32_t intervals = c->iters() / c->res_check_every;
+    hipStream_t s = c->stream;
+    for (uint32_t k = 0; k < intervals; k++) {
+        if (k >= kAhead) {
+            hip_check(hipEventSynchronize(c->part_ev[k - kAhead]), "hipEventSynchronize");
+            if (c->h_conv[k - kAhead]) break;
+        }
+        if (k == 0) {
+            if (graphs) hip_check(hipGraphLaunch(c->head_exec, s), "hipGraphLaunch");
+            else {
+                enqueue_head(c, false);
+                enqueue_iters(c, false, 0, c->res_check_every, true);
+                join_broadphase(c);
+            }
+        } else {
+            if (graphs) hip_check(hipGraphLaunch(c->interval_exec, s), "hipGraphLaunch");
+            else enqueue_iters(c, false, 0, c->res_check_every, true);
+        }
+        hip_check(hipMemcpyAsync(c->h_conv + k, &c->rc->done, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync D2H");
+        hip_check(hipEventRecord(c->part_ev[k], s), "hipEventRecord");
+    }
+    if (graphs) hip_check(hipGraphLaunch(c->tail_exec, s), "hipGraphLaunch");
+    else enqueue_tail(c);
 }
 
 // Pinned staging of a detection round's host arrays: reserve the round's
@@ -709,24 +819,85 @@ void detect_round(avbd_ctx* c) {
 // Autotuner: time the colour-phase solve of one iteration for each candidate
 // (lanes per vertex, block size), writing into a scratch sink so that the
 // state is untouched, and keep the fastest.
+// Temporary device buffers, freed when the holder goes out of scope.
+struct DevTemp {
+    std::vector<void*> ptrs;
+    ~DevTemp() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    T* get(size_t count) {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+// DSATUR of every scene on the device (ColorList::greedyColor2,
+// common/ColorList.hpp:149-241): the static adjacency goes up, the colours
+// come back for the work-list layout.
+void device_dsatur(avbd_ctx* c, const std::vector<uint32_t>& scene_off, const std::vector<uint32_t>& off,
+                   const std::vector<uint32_t>& nb, std::vector<uint32_t>& color, std::vector<uint32_t>& scene_colors) {
+    const uint32_t n = (uint32_t)color.size(), ns = (uint32_t)scene_off.size() - 1;
+    hipStream_t s = c->stream;
+    DevTemp t;
+    uint32_t* d_so = t.get<uint32_t>(scene_off.size());
+    uint32_t* d_off = t.get<uint32_t>(off.size());
+    uint32_t* d_nb = t.get<uint32_t>(nb.size());
+    uint64_t* d_key = t.get<uint64_t>(n);
+    uint64_t* d_seen = t.get<uint64_t>(4 * (size_t)n);
+    uint32_t* d_color = t.get<uint32_t>(n);
+    uint32_t* d_sc = t.get<uint32_t>(ns);
+    uint32_t* d_status = t.get<uint32_t>(1);
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    };
+    h2d(d_so, scene_off.data(), scene_off.size() * sizeof(uint32_t));
+    h2d(d_off, off.data(), off.size() * sizeof(uint32_t));
+    h2d(d_nb, nb.data(), nb.size() * sizeof(uint32_t));
+    hip_check(hipMemsetAsync(d_seen, 0, 4 * (size_t)n * sizeof(uint64_t), s), "hipMemsetAsync");
+    hip_check(hipMemsetAsync(d_status, 0, sizeof(uint32_t), s), "hipMemsetAsync");
+    launch_dsatur(s, ns, d_so, d_off, d_nb, d_key, d_seen, d_color, d_sc, d_status);
+    hip_check(hipGetLastError(), "dsatur launch");
+    uint32_t status = 0;
+    hip_check(hipMemcpyAsync(color.data(), d_color, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipMemcpyAsync(scene_colors.data(), d_sc, (size_t)ns * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+              "D2H");
+    hip_check(hipMemcpyAsync(&status, d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (status & 1u) throw std::runtime_error("Ran out of colors!");
+    if (status & 2u) throw std::runtime_error("a vertex has more than 65535 neighbours");
+}
+
+// One colour phase of vertex solves into the scratch output (state untouched).
+void solve_into_scratch(avbd_ctx* c, const LaunchCfg& cfg, uint32_t b, uint32_t count) {
+    if (c->fp64)
+        launch_solve_color_f64(c->stream, c->f, cfg, b, count, c->damping, c->desc.dt, c->f.x_scratch, nullptr);
+    else
+        launch_solve_color(c->stream, c->d, cfg, b, count, c->damping, (float)c->desc.dt, c->d.x_scratch);
+}
+
+// Lanes per vertex, block size and gather batch: time every candidate on
+// the scene's colour phases and keep the fastest.
 void autotune(avbd_ctx* c) {
     const uint32_t lanes_c[] = {4, 8, 16, 32};
     const uint32_t block_c[] = {64, 128, 256};
     const uint32_t batch_c[] = {1, 2, 4};
     float best = 1e30f;
     LaunchCfg best_cfg = c->cfg;
-    const float dt = (float)c->desc.dt;
     for (uint32_t L : lanes_c) {
         if (c->desc.lanes_per_vertex && L != c->desc.lanes_per_vertex) continue;
         for (uint32_t B : block_c)
         for (uint32_t U : batch_c) {
             if (c->desc.block_size && B != c->desc.block_size) continue;
             if (c->desc.gather_batch && U != c->desc.gather_batch) continue;
+            if (c->fp64 && U > 2) continue;   // the FP64 solve gathers 1 or 2 incidences at once
             LaunchCfg cfg{L, B, U};
             auto run = [&] {
                 for (uint32_t col = 0; col < c->plan.num_colors; col++) {
                     const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
-                    launch_solve_color(c->stream, c->d, cfg, b, e - b, c->damping, dt, c->d.x_scratch);
+                    solve_into_scratch(c, cfg, b, e - b);
                 }
             };
             for (int w = 0; w < 3; w++) run();
@@ -748,13 +919,17 @@ void autotune(avbd_ctx* c) {
 // Fused or separate Chebyshev / dual update: time the last colour phase plus
 // the update both ways on the current state and keep the faster.  Both
 // forms write x_last, x_lastlast and the ground records (and the separate
-// one d.x), so those are saved first and restored afterwards.
+// one x), so those are saved first and restored afterwards.
 void tune_fusion(avbd_ctx* c) {
     if (!c->fuse_cheb) return;
     const uint32_t last = c->plan.num_colors - 1;
     const uint32_t b = c->plan.item_offsets[last], e = c->plan.item_offsets[last + 1];
     const float dt = (float)c->desc.dt;
-    const size_t nx = (size_t)c->d.n * sizeof(float4), ng = (size_t)c->d.n_ground * sizeof(GroundRec);
+    const size_t nx = (size_t)c->d.n * (c->fp64 ? sizeof(double4) : sizeof(float4));
+    const size_t ng = (size_t)c->d.n_ground * sizeof(GroundRec);
+    void* px = c->fp64 ? (void*)c->f.x : (void*)c->d.x;
+    void* pl = c->fp64 ? (void*)c->f.x_last : (void*)c->d.x_last;
+    void* pll = c->fp64 ? (void*)c->f.x_lastlast : (void*)c->d.x_lastlast;
     std::vector<uint8_t> save(3 * nx + ng);
     auto d2h = [&](void* dst, const void* src, size_t bytes) {
         if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "D2H");
@@ -763,9 +938,9 @@ void tune_fusion(avbd_ctx* c) {
         if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "H2D");
     };
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    d2h(save.data(), c->d.x, nx);
-    d2h(save.data() + nx, c->d.x_last, nx);
-    d2h(save.data() + 2 * nx, c->d.x_lastlast, nx);
+    d2h(save.data(), px, nx);
+    d2h(save.data() + nx, pl, nx);
+    d2h(save.data() + 2 * nx, pll, nx);
     d2h(save.data() + 3 * nx, c->d.ground, ng);
     auto timed = [&](auto&& body) {
         for (int w = 0; w < 3; w++) body();
@@ -777,17 +952,29 @@ void tune_fusion(avbd_ctx* c) {
         hip_check(hipEventElapsedTime(&ms, c->ev_a, c->ev_b), "hipEventElapsedTime");
         return ms;
     };
-    const float t_sep = timed([&] {
-        launch_solve_color(c->stream, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x_scratch, nullptr);
-        launch_cheb_dual(c->stream, c->d, 1.0f, nullptr);
-    });
-    const float t_fused = timed([&] {
-        launch_solve_cheb(c->stream, c->d, c->cfg, b, e - b, c->damping, dt, 1.0f, c->cheb_list, c->n_cheb,
-                          c->x_alt, nullptr);
-    });
-    h2d(c->d.x, save.data(), nx);
-    h2d(c->d.x_last, save.data() + nx, nx);
-    h2d(c->d.x_lastlast, save.data() + 2 * nx, nx);
+    float t_sep = 0, t_fused = 0;
+    if (c->fp64) {
+        t_sep = timed([&] {
+            solve_into_scratch(c, c->cfg, b, e - b);
+            launch_cheb_dual_f64(c->stream, c->f, c->d.n, 1.0f, nullptr, nullptr, 0);
+        });
+        t_fused = timed([&] {
+            launch_solve_cheb_f64(c->stream, c->f, c->cfg, b, e - b, c->damping, c->desc.dt, 1.0f, nullptr, nullptr, 0,
+                                  c->cheb_list, c->n_cheb, c->f.x, c->f.x_alt);
+        });
+    } else {
+        t_sep = timed([&] {
+            solve_into_scratch(c, c->cfg, b, e - b);
+            launch_cheb_dual(c->stream, c->d, 1.0f);
+        });
+        t_fused = timed([&] {
+            launch_solve_cheb(c->stream, c->d, c->cfg, b, e - b, c->damping, dt, 1.0f, nullptr, nullptr, 0,
+                              c->cheb_list, c->n_cheb, c->x_alt);
+        });
+    }
+    h2d(px, save.data(), nx);
+    h2d(pl, save.data() + nx, nx);
+    h2d(pll, save.data() + 2 * nx, nx);
     h2d(c->d.ground, save.data() + 3 * nx, ng);
     c->fuse_cheb = t_fused < t_sep;
 }
@@ -820,10 +1007,16 @@ int avbd_create(const avbd_scene_desc* desc, avbd_ctx** out) {
         g_create_error = "gather_batch must be 0, 1, 2 or 4";
         return AVBD_E_ARG;
     }
+    if (desc->flags & AVBD_FLAG_PERSISTENT) {
+        g_create_error = "AVBD_FLAG_PERSISTENT is no longer supported: one launch per solver phase, replayed from a "
+                         "hipGraph, is faster on MI355X than one persistent launch with grid barriers";
+        return AVBD_E_ARG;
+    }
     // HIP is initialised lazily by avbd_build_graph(): staging and the host
     // graph build work without a device.
     auto* c = new avbd_ctx();
     c->desc = *desc;
+    c->fp64 = (desc->flags & AVBD_FLAG_FP64_STATE) != 0;
     c->device = desc->device;
     c->cfg.lanes = L ? L : 8;
     c->cfg.block = B ? B : 256;
@@ -842,8 +1035,6 @@ void avbd_destroy(avbd_ctx* c) {
     for (auto ev : c->part_ev) (void)hipEventDestroy(ev);
     if (c->h_conv) (void)hipHostFree(c->h_conv);
     if (c->stage) (void)hipHostFree(c->stage);
-    if (c->ev_st0) (void)hipEventDestroy(c->ev_st0);
-    if (c->ev_st1) (void)hipEventDestroy(c->ev_st1);
     if (c->ev_lb0) (void)hipEventDestroy(c->ev_lb0);
     if (c->ev_lb1) (void)hipEventDestroy(c->ev_lb1);
     if (c->ev_a) (void)hipEventDestroy(c->ev_a);
@@ -1186,8 +1377,10 @@ int avbd_set_particle_state(avbd_ctx* c, uint32_t first, uint32_t n, const doubl
         if ((size_t)first + n > c->mass.size()) throw std::runtime_error("particle range out of bounds");
         if (c->built) {
             // pull the current device state so that NULL fields stay unchanged
+            // (a non-finite state is pulled too: this call is how it is replaced)
             std::vector<double> cx(3 * c->mass.size()), cv(3 * c->mass.size());
-            if (avbd_download_state(c, cx.data(), cv.data()) != AVBD_OK) throw std::runtime_error(c->err);
+            const int rc = avbd_download_state(c, cx.data(), cv.data());
+            if (rc != AVBD_OK && rc != AVBD_E_NONFINITE) throw std::runtime_error(c->err);
         }
         for (uint32_t i = 0; i < 3 * n; i++) {
             const size_t k = 3 * (size_t)first + i;
@@ -1196,7 +1389,30 @@ int avbd_set_particle_state(avbd_ctx* c, uint32_t first, uint32_t n, const doubl
             if (v) c->v[k] = v[i];
             if (v_prev) c->v_prev[k] = v_prev[i];
         }
-        if (c->built) upload_particle_range(c, first, n);
+        if (c->built) {
+            upload_particle_range(c, first, n);
+            clear_status(c);
+        }
+    });
+}
+
+int avbd_set_external_accel(avbd_ctx* c, uint32_t first, uint32_t n, const double* a) {
+    return guarded(c, [&] {
+        if ((size_t)first + n > c->mass.size()) throw std::runtime_error("particle range out of bounds");
+        if (!a) throw CodeError(AVBD_E_ARG, "a is null");
+        if (c->built) {
+            if (!c->d.a_ext)
+                throw CodeError(AVBD_E_ARG, "an external acceleration field must first be set before avbd_build_graph");
+            std::vector<float4> fa(n);
+            for (uint32_t i = 0; i < n; i++)
+                fa[i] = make_float4((float)a[3 * i], (float)a[3 * i + 1], (float)a[3 * i + 2], 0.f);
+            hip_check(hipMemcpyAsync(const_cast<float4*>(c->d.a_ext) + first, fa.data(), n * sizeof(float4),
+                                     hipMemcpyHostToDevice, c->stream),
+                      "hipMemcpyAsync H2D");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        }
+        if (c->f_ext.empty()) c->f_ext.assign(3 * c->mass.size(), 0.0);
+        std::copy(a, a + 3 * (size_t)n, c->f_ext.begin() + 3 * (size_t)first);
     });
 }
 
@@ -1252,8 +1468,8 @@ int avbd_set_residual_target(avbd_ctx* c, double rel_tol, uint32_t max_iters, ui
         if (rel_tol < 0) throw std::runtime_error("rel_tol must be >= 0");
         if (rel_tol > 0) {
             if (max_iters == 0 || check_every == 0) throw std::runtime_error("max_iters and check_every must be >= 1");
-            if ((max_iters + check_every - 1) / check_every > kMaxResidualChecks)
-                throw std::runtime_error("at most 64 residual checks per step (raise check_every)");
+            // whole check intervals: the budget is rounded up to a multiple of check_every
+            max_iters = (max_iters + check_every - 1) / check_every * check_every;
         }
         c->res_tol = rel_tol;
         c->res_max_iters = max_iters;
@@ -1261,21 +1477,27 @@ int avbd_set_residual_target(avbd_ctx* c, double rel_tol, uint32_t max_iters, ui
     });
 }
 
+namespace {
+// Per-check history of the last step (which = 0: whole scene set, 1: worst scene).
+void residual_history(avbd_ctx* c, int which, double* out, uint32_t cap, uint32_t* count) {
+    require_built(c);
+    if (!c->rc) throw std::runtime_error("residual-targeted iteration is off");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    ResCtl rc{};
+    hip_check(hipMemcpy(&rc, c->rc, sizeof(ResCtl), hipMemcpyDeviceToHost), "D2H");
+    const uint32_t n = std::min(rc.checks, kMaxResidualChecks);
+    const double* src = which ? rc.worst : rc.rel;
+    for (uint32_t k = 0; k < n && k < cap; k++) out[k] = src[k];
+    if (count) *count = n;
+}
+}  // namespace
+
 int avbd_get_residual_history(avbd_ctx* c, double* rel, uint32_t cap, uint32_t* count) {
-    return guarded(c, [&] {
-        require_built(c);
-        if (!c->rc) throw std::runtime_error("residual-targeted iteration is off");
-        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        ResCtl rc{};
-        hip_check(hipMemcpy(&rc, c->rc, sizeof(ResCtl), hipMemcpyDeviceToHost), "D2H");
-        const uint32_t it = rc.last_iters;
-        // checks run after iterations K, 2K, ... below the budget, up to the one that converged
-        uint32_t n = 0;
-        for (uint32_t k = c->res_check_every; k <= it && k < c->res_max_iters; k += c->res_check_every) n++;
-        const double mg2 = c->res_tol2 / (c->res_tol * c->res_tol);
-        for (uint32_t k = 0; k < n && k < cap; k++) rel[k] = mg2 > 0 ? std::sqrt(rc.acc[k] / mg2) : 0.0;
-        if (count) *count = n;
-    });
+    return guarded(c, [&] { residual_history(c, 0, rel, cap, count); });
+}
+
+int avbd_get_scene_residual_history(avbd_ctx* c, double* worst, uint32_t cap, uint32_t* count) {
+    return guarded(c, [&] { residual_history(c, 1, worst, cap, count); });
 }
 
 int avbd_set_scenes(avbd_ctx* c, uint32_t n_scenes, const uint32_t* offsets) {
@@ -1294,6 +1516,10 @@ int avbd_build_graph(avbd_ctx* c) {
     return guarded(c, [&] {
         require_not_built(c);
         const uint32_t n = (uint32_t)c->mass.size();
+        // the device first: the colouring runs on it when there is one
+        const hipError_t dev_err = hipSetDevice(c->device);
+        if (dev_err == hipSuccess && !c->stream)
+            hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         if (!c->planned) {
             std::vector<uint32_t> so = c->scene_offsets;
             if (so.empty()) so = {0u, n};
@@ -1308,12 +1534,18 @@ int avbd_build_graph(avbd_ctx* c) {
             m.fixed = &c->fixed;
             m.rot_of = &c->rot_of;
             m.scene_offsets = &so;
-            build_color_plan(m, c->plan);
+            SceneColourer colour;
+            if (dev_err == hipSuccess && !(c->desc.flags & AVBD_FLAG_HOST_COLORING))
+                colour = [c](const std::vector<uint32_t>& scene_off, const std::vector<uint32_t>& off,
+                             const std::vector<uint32_t>& nb, std::vector<uint32_t>& color,
+                             std::vector<uint32_t>& scene_colors) {
+                    device_dsatur(c, scene_off, off, nb, color, scene_colors);
+                };
+            build_color_plan(m, c->plan, colour);
             c->planned = true;
         }
         // device part
-        hip_check(hipSetDevice(c->device), "hipSetDevice");
-        if (!c->stream) hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(dev_err, "hipSetDevice");
         if (!c->tris.empty() && !(c->desc.flags & AVBD_FLAG_SERIAL_BROADPHASE)) {
             hip_check(hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking), "hipStreamCreate");
             hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
@@ -1341,6 +1573,18 @@ int avbd_build_graph(avbd_ctx* c) {
         d.v_prev = dev_alloc<float4>(c, n);
         d.x_scratch = dev_alloc<float4>(c, n);
         d.fixed = dev_alloc<uint8_t>(c, n);
+        if (c->fp64) {
+            if (!c->q.empty())
+                throw CodeError(AVBD_E_ARG, "the FP64 state covers positional particles only (no oriented particles)");
+            F64State& f = c->f;
+            f.x = dev_alloc<double4>(c, n);
+            f.y = dev_alloc<double4>(c, n);
+            f.x_prev = dev_alloc<double4>(c, n);
+            f.x_last = dev_alloc<double4>(c, n);
+            f.x_lastlast = dev_alloc<double4>(c, n);
+            f.x0 = dev_alloc<double4>(c, n);
+            f.x_scratch = dev_alloc<double4>(c, n);
+        }
         d.ground_of = dev_alloc<int32_t>(c, n);
         d.tets = dev_alloc<TetRec>(c, d.n_tets);
         d.damp = dev_alloc<TetDamp>(c, d.n_tets);
@@ -1426,6 +1670,7 @@ int avbd_build_graph(avbd_ctx* c) {
             c->wio.ipd = dev_alloc<double>(c, (size_t)c->wio.n_scenes * kWarmAlphas);
             c->wio.pred = dev_alloc<float4>(c, n);
             c->wio.counters = dev_alloc<unsigned long long>(c, 2);
+            c->wio.step_of = dev_alloc<float>(c, c->wio.n_scenes);
             upload(c, d_mat, mat);
             upload(c, d_scene, scene_of);
             hip_check(hipMemsetAsync(c->wio.counters, 0, 2 * sizeof(unsigned long long), c->stream), "hipMemsetAsync");
@@ -1474,6 +1719,34 @@ int avbd_build_graph(avbd_ctx* c) {
         }
         upload(c, d.items, c->plan.items);
         upload(c, c->d_conflicts, c->plan.conflict_list);
+        if (!c->f_ext.empty()) {
+            // the field, if any, is fixed at build time (its buffer is part of the captured steps)
+            std::vector<float4> fa(n);
+            for (uint32_t p = 0; p < n; p++)
+                fa[p] = make_float4((float)c->f_ext[3 * p], (float)c->f_ext[3 * p + 1], (float)c->f_ext[3 * p + 2], 0.f);
+            float4* d_fa = dev_alloc<float4>(c, n);
+            upload(c, d_fa, fa);
+            d.a_ext = d_fa;
+        }
+        if (c->fp64) {
+            F64State& f = c->f;
+            f.a_ext = d.a_ext;
+            f.x32 = d.x;
+            f.v = d.v;
+            f.v_prev = d.v_prev;
+            f.fixed = d.fixed;
+            f.ground_of = d.ground_of;
+            f.tets = d.tets;
+            f.damp = d.damp;
+            f.ground = d.ground;
+            f.inc = d.inc;
+            f.irec = d.irec;
+            f.items = d.items;
+            f.n = n;
+            f.n_tets = d.n_tets;
+            f.n_ground = d.n_ground;
+            f.n_items = d.n_items;
+        }
         upload_particle_range(c, 0, n);
 
         // LBVH broad phase buffers
@@ -1485,63 +1758,60 @@ int avbd_build_graph(avbd_ctx* c) {
             c->lb.tris = dev_alloc<uint32_t>(c, 3 * (size_t)nt);
             upload(c, c->lb.tris, c->tris);
         }
-        // persistent step kernel: per-colour offsets, Chebyshev schedule, barrier
+        // Chebyshev schedule (per iteration; residual mode reads it on the device)
         {
-            std::vector<uint32_t> co(c->plan.item_offsets.begin(), c->plan.item_offsets.end());
             std::vector<float> om(c->iters());
             double w = 1;
             for (uint32_t i = 0; i < c->iters(); i++) om[i] = (float)(w = omega_at(i, c->desc.iter_acceleration, w));
             c->omega_seq = om;
-            uint32_t* d_co = dev_alloc<uint32_t>(c, co.size());
-            uint32_t* d_oco = dev_alloc<uint32_t>(c, c->plan.oitem_offsets.size());
-            upload(c, d_oco, c->plan.oitem_offsets);
-            uint32_t* d_cfo = dev_alloc<uint32_t>(c, c->plan.conflict_offsets.size());
-            float* d_om = dev_alloc<float>(c, om.size());
-            GridBarrier* bar = dev_alloc<GridBarrier>(c, 1);
-            upload(c, d_co, co);
-            upload(c, d_cfo, c->plan.conflict_offsets);
-            upload(c, d_om, om);
-            hip_check(hipMemsetAsync(bar, 0, sizeof(GridBarrier), c->stream), "hipMemsetAsync");
-            c->sargs.d = c->d;
-            c->sargs.color_offsets = d_co;
-            c->sargs.conflict_offsets = d_cfo;
-            c->sargs.conflicts = c->d_conflicts;
-            c->sargs.ocolor_offsets = d_oco;
-            c->sargs.omega = d_om;
-            c->sargs.bar = bar;
-            c->sargs.iters = c->iters();
-            if (c->res_tol > 0) {
-                // ||M g||^2 over free particles, as in the residual definition
-                double mg2 = 0;
-                for (uint32_t p = 0; p < n; p++)
-                    if (!c->fixed[p]) mg2 += (c->mass[p] * c->desc.g_accel) * (c->mass[p] * c->desc.g_accel);
-                c->res_tol2 = c->res_tol * c->res_tol * mg2;
-                c->rc = dev_alloc<ResCtl>(c, 1);
-                c->res_partial = dev_alloc<double>(c, std::max(1u, residual_partials(c->d)));
-                hip_check(hipMemsetAsync(c->rc, 0, sizeof(ResCtl), c->stream), "hipMemsetAsync");
-                void* hc = nullptr;
-                hip_check(hipHostMalloc(&hc, kMaxResidualChecks * sizeof(uint32_t),
-                                        hipHostMallocMapped | hipHostMallocCoherent),
-                          "hipHostMalloc");
-                c->h_conv = static_cast<uint32_t*>(hc);
-                std::memset(c->h_conv, 0, kMaxResidualChecks * sizeof(uint32_t));
-                void* dc = nullptr;
-                hip_check(hipHostGetDevicePointer(&dc, hc, 0), "hipHostGetDevicePointer");
-                c->d_conv = static_cast<uint32_t*>(dc);
-                c->sargs.rc = c->rc;
-                c->sargs.check_every = c->res_check_every;
-                c->sargs.tol2 = c->res_tol2;
+        }
+        d.status = dev_alloc<uint32_t>(c, 1);
+        hip_check(hipMemsetAsync(d.status, 0, sizeof(uint32_t), c->stream), "hipMemsetAsync");
+        if (c->res_tol > 0) {
+            // per scene: particle range and ||M_s g||^2 over its free particles,
+            // as in the residual definition; |r_v|^2 per particle
+            std::vector<uint32_t> so = c->scene_offsets;
+            if (so.empty()) so = {0u, n};
+            c->n_res_scenes = (uint32_t)(so.size() - 1);
+            std::vector<double> mg2(c->n_res_scenes, 0.0);
+            for (uint32_t sc = 0; sc < c->n_res_scenes; sc++)
+                for (uint32_t p = so[sc]; p < so[sc + 1]; p++)
+                    if (!c->fixed[p]) {
+                        const double mg = c->mass[p] * c->desc.g_accel;
+                        mg2[sc] += mg * mg;
+                    }
+            double mg2_all = 0;
+            for (double v : mg2) mg2_all += v;
+            c->res_mg2 = mg2_all;
+            c->rc = dev_alloc<ResCtl>(c, 1);
+            c->res_rr = dev_alloc<double>(c, n);
+            c->res_scene_off = dev_alloc<uint32_t>(c, so.size());
+            c->res_mg2_scene = dev_alloc<double>(c, c->n_res_scenes);
+            c->res_acc_scene = dev_alloc<double>(c, c->n_res_scenes);
+            c->omega_tab = dev_alloc<float>(c, c->omega_seq.size());
+            upload(c, c->res_scene_off, so);
+            upload(c, c->res_mg2_scene, mg2);
+            upload(c, c->omega_tab, c->omega_seq);
+            hip_check(hipMemsetAsync(c->rc, 0, sizeof(ResCtl), c->stream), "hipMemsetAsync");
+            hip_check(hipMemsetAsync(c->res_rr, 0, (size_t)n * sizeof(double), c->stream), "hipMemsetAsync");
+            // converged flag of every interval of a step, copied back after the interval
+            const uint32_t intervals = c->res_max_iters / c->res_check_every;
+            void* hc = nullptr;
+            hip_check(hipHostMalloc(&hc, (size_t)intervals * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+            c->h_conv = static_cast<uint32_t*>(hc);
+            std::memset(c->h_conv, 0, (size_t)intervals * sizeof(uint32_t));
+            while (c->part_ev.size() < intervals) {
+                hipEvent_t e = nullptr;
+                hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+                c->part_ev.push_back(e);
             }
-            c->sargs.num_colors = c->plan.num_colors;
-            c->sargs.dt = (float)c->desc.dt;
-            c->sargs.g = (float)c->desc.g_accel;
-            hip_check(hipEventCreate(&c->ev_st0), "hipEventCreate");
-            hip_check(hipEventCreate(&c->ev_st1), "hipEventCreate");
         }
         // Chebyshev / dual update fused into the last colour phase (scenes
         // without oriented particles or conflicted vertices)
+        // (residual mode: with an even check interval, so that every interval
+        // starts on the same buffer and one interval graph serves them all)
         if (!(c->desc.flags & AVBD_FLAG_SEPARATE_CHEBYSHEV) && c->d.n_rot == 0 && c->d.n_oitems == 0 &&
-            c->plan.num_colors > 0 && c->plan.conflict_list.empty()) {
+            c->plan.num_colors > 0 && c->plan.conflict_list.empty() && (!c->rc || c->res_check_every % 2 == 0)) {
             const uint32_t last = c->plan.num_colors - 1;
             std::vector<uint8_t> in_last(n, 0);
             for (uint32_t k = c->plan.item_offsets[last]; k < c->plan.item_offsets[last + 1]; k++)
@@ -1549,7 +1819,8 @@ int avbd_build_graph(avbd_ctx* c) {
             std::vector<uint32_t> rest;
             for (uint32_t p = 0; p < n; p++)
                 if (!in_last[p]) rest.push_back(p);
-            c->x_alt = dev_alloc<float4>(c, n);
+            if (c->fp64) c->f.x_alt = dev_alloc<double4>(c, n);
+            else c->x_alt = dev_alloc<float4>(c, n);
             c->n_cheb = (uint32_t)rest.size();
             c->cheb_list = dev_alloc<uint32_t>(c, std::max<size_t>(1, rest.size()));
             if (!rest.empty()) upload(c, c->cheb_list, rest);
@@ -1557,7 +1828,8 @@ int avbd_build_graph(avbd_ctx* c) {
         }
         c->built = true;
 
-        const uint32_t n_solves = c->iters() * c->plan.num_colors;
+        // timing events bracket the solve launches of a fixed-count step
+        const uint32_t n_solves = c->rc ? 0u : c->iters() * c->plan.num_colors;
         c->events.resize(2 * n_solves);
         hip_check(hipEventCreate(&c->ev_lb0), "hipEventCreate");
         hip_check(hipEventCreate(&c->ev_lb1), "hipEventCreate");
@@ -1566,10 +1838,6 @@ int avbd_build_graph(avbd_ctx* c) {
         if (c->desc.lanes_per_vertex == 0 || c->desc.block_size == 0) {
             autotune(c);
             tune_fusion(c);
-        }
-        if ((c->desc.flags & AVBD_FLAG_PERSISTENT) && !(c->desc.flags & AVBD_FLAG_NO_PERSISTENT)) {
-            c->pgrid = persistent_grid(c->device, c->cfg.lanes, c->damping);
-            c->persistent = c->pgrid >= 8;
         }
         if (!(c->desc.flags & AVBD_FLAG_NO_GRAPH)) capture_graph(c);
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
@@ -1609,19 +1877,16 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
             c->steps_taken += n_steps;
             return;
         }
-        if (c->timing) {
+        if (c->nonfinite) throw CodeError(AVBD_E_NONFINITE, nonfinite_message());
+        if (c->rc) {
+            for (uint32_t st = 0; st < n_steps; st++) run_residual_step(c, c->head_exec != nullptr && !c->timing);
+        } else if (c->timing) {
             double total = 0, lbvh_total = 0;
             uint32_t launches = 0;
             for (uint32_t s = 0; s < n_steps; s++) {
                 enqueue_step(c, true);
                 hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-                if (c->persistent) {
-                    float ms = 0;
-                    hip_check(hipEventElapsedTime(&ms, c->ev_st0, c->ev_st1), "hipEventElapsedTime");
-                    total += ms;
-                    launches++;
-                }
-                for (size_t k = 0; !c->persistent && k + 1 < c->events.size(); k += 2) {
+                for (size_t k = 0; k + 1 < c->events.size(); k += 2) {
                     float ms = 0;
                     hip_check(hipEventElapsedTime(&ms, c->events[k], c->events[k + 1]), "hipEventElapsedTime");
                     total += ms;
@@ -1636,25 +1901,6 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
             c->solve_avg_ms = launches ? total / launches : 0;
             c->lbvh_avg_ms = lbvh_total / std::max(1u, n_steps);
             c->solve_launches = launches;
-        } else if (!c->part_exec.empty()) {
-            // residual mode: interval k is launched once the check of interval
-            // k - kPartsAhead has been seen unconverged, so at most
-            // kPartsAhead - 1 intervals of early-exiting launches follow the
-            // converged check instead of the rest of the iteration budget
-            constexpr uint32_t kPartsAhead = 2;
-            const uint32_t parts = (uint32_t)c->part_exec.size() - 1;
-            for (uint32_t st = 0; st < n_steps; st++) {
-                for (uint32_t k = 0; k < parts; k++) {
-                    if (k >= kPartsAhead) {
-                        // interval k - kPartsAhead < parts - 1 ends with check k - kPartsAhead
-                        hip_check(hipEventSynchronize(c->part_ev[k - kPartsAhead]), "hipEventSynchronize");
-                        if (__atomic_load_n(c->h_conv + (k - kPartsAhead), __ATOMIC_ACQUIRE)) break;
-                    }
-                    hip_check(hipGraphLaunch(c->part_exec[k], c->stream), "hipGraphLaunch");
-                    hip_check(hipEventRecord(c->part_ev[k], c->stream), "hipEventRecord");
-                }
-                hip_check(hipGraphLaunch(c->part_exec[parts], c->stream), "hipGraphLaunch");
-            }
         } else if (c->graph_exec) {
             for (uint32_t s = 0; s < n_steps; s++) hip_check(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
         } else {
@@ -1672,7 +1918,7 @@ int avbd_synchronize(avbd_ctx* c) {
         if (c->built) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, c->ev_a, c->ev_b) == hipSuccess) c->last_step_ms = ms;
-            check_abort(c);
+            check_status(c);
         }
     });
 }
@@ -1681,42 +1927,52 @@ int avbd_download_state(avbd_ctx* c, double* x, double* v) {
     return guarded(c, [&] {
         require_built(c);
         const uint32_t n = c->d.n;
-        std::vector<float4> bx(n), bv(n);
-        hip_check(hipMemcpyAsync(bx.data(), c->d.x, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
+        // positions: the FP64 stream when the state is FP64, else the FP32 one
+        std::vector<double> px(3 * (size_t)n), pxp(3 * (size_t)n);
+        std::vector<float4> bv(n), bvp(n);
+        auto positions = [&](const float4* src32, const double4* src64, std::vector<double>& out) {
+            if (c->fp64) {
+                std::vector<double4> b(n);
+                hip_check(hipMemcpyAsync(b.data(), src64, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream),
+                          "D2H");
+                hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+                for (uint32_t i = 0; i < n; i++) {
+                    out[3 * i] = b[i].x;
+                    out[3 * i + 1] = b[i].y;
+                    out[3 * i + 2] = b[i].z;
+                }
+            } else {
+                std::vector<float4> b(n);
+                hip_check(hipMemcpyAsync(b.data(), src32, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream),
+                          "D2H");
+                hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+                for (uint32_t i = 0; i < n; i++) {
+                    out[3 * i] = b[i].x;
+                    out[3 * i + 1] = b[i].y;
+                    out[3 * i + 2] = b[i].z;
+                }
+            }
+        };
+        positions(c->d.x, c->f.x, px);
+        positions(c->d.x_prev, c->f.x_prev, pxp);
         hip_check(hipMemcpyAsync(bv.data(), c->d.v, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
-        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        check_abort(c);
-        for (uint32_t i = 0; i < n; i++) {
-            if (x) {
-                x[3 * i] = bx[i].x;
-                x[3 * i + 1] = bx[i].y;
-                x[3 * i + 2] = bx[i].z;
-            }
-            if (v) {
-                v[3 * i] = bv[i].x;
-                v[3 * i + 1] = bv[i].y;
-                v[3 * i + 2] = bv[i].z;
-            }
-        }
-        // keep the staging copy current (used by avbd_set_particle_state)
-        std::vector<float4> bxp(n), bvp(n);
-        hip_check(hipMemcpyAsync(bxp.data(), c->d.x_prev, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream),
-                  "D2H");
         hip_check(hipMemcpyAsync(bvp.data(), c->d.v_prev, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream),
                   "D2H");
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         for (uint32_t i = 0; i < n; i++) {
-            const float* fx = &bx[i].x;
             const float* fv = &bv[i].x;
-            const float* fxp = &bxp[i].x;
             const float* fvp = &bvp[i].x;
             for (int k = 0; k < 3; k++) {
-                c->x[3 * i + k] = fx[k];
+                if (x) x[3 * i + k] = px[3 * i + k];
+                if (v) v[3 * i + k] = fv[k];
+                // keep the staging copy current (used by avbd_set_particle_state)
+                c->x[3 * i + k] = px[3 * i + k];
                 c->v[3 * i + k] = fv[k];
-                c->x_prev[3 * i + k] = fxp[k];
+                c->x_prev[3 * i + k] = pxp[3 * i + k];
                 c->v_prev[3 * i + k] = fvp[k];
             }
         }
+        check_status(c);   // the arrays are filled either way
     });
 }
 
@@ -1724,6 +1980,17 @@ int avbd_download_inertial(avbd_ctx* c, double* y) {
     return guarded(c, [&] {
         require_built(c);
         const uint32_t n = c->d.n;
+        if (c->fp64) {
+            std::vector<double4> by(n);
+            hip_check(hipMemcpyAsync(by.data(), c->f.y, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            for (uint32_t i = 0; i < n; i++) {
+                y[3 * i] = by[i].x;
+                y[3 * i + 1] = by[i].y;
+                y[3 * i + 2] = by[i].z;
+            }
+            return;
+        }
         std::vector<float4> by(n);
         hip_check(hipMemcpyAsync(by.data(), c->d.y, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
@@ -1745,7 +2012,6 @@ int avbd_download_rotations(avbd_ctx* c, double* q, double* w) {
             hip_check(hipMemcpyAsync(bw.data(), c->d.w, nr * sizeof(float4), hipMemcpyDeviceToHost, c->stream), "D2H");
         }
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        check_abort(c);
         for (uint32_t p = 0; p < n; p++) {
             const int32_t r = c->rot_of[p];
             if (q) {
@@ -1765,6 +2031,7 @@ int avbd_download_rotations(avbd_ctx* c, double* q, double* w) {
                 w[3 * p + 2] = r < 0 ? 0.0 : bw[r].z;
             }
         }
+        check_status(c);   // the arrays are filled either way
     });
 }
 
@@ -1908,7 +2175,10 @@ int avbd_get_stats(avbd_ctx* c, avbd_stats* out) {
         avbd_stats st{};
         hip_check(hipMemsetAsync(c->d.reduce, 0, (size_t)c->reduce_blocks * 4 * sizeof(double), c->stream),
                   "hipMemsetAsync");
-        launch_stats(c->stream, c->d, c->cfg, c->damping, (float)c->desc.dt, (float)c->desc.g_accel);
+        if (c->fp64)
+            launch_stats_f64(c->stream, c->f, c->cfg, c->damping, c->desc.dt, c->desc.g_accel, c->d.reduce);
+        else
+            launch_stats(c->stream, c->d, c->cfg, c->damping, (float)c->desc.dt, (float)c->desc.g_accel);
         hip_check(hipGetLastError(), "stats launch");
         std::vector<double> red((size_t)c->reduce_blocks * 4);
         hip_check(hipMemcpyAsync(red.data(), c->d.reduce, red.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -1950,6 +2220,7 @@ int avbd_get_stats(avbd_ctx* c, avbd_stats* out) {
         }
         st.last_step_ms = c->last_step_ms;
         *out = st;
+        check_status(c);   // the statistics are filled either way
     });
 }
 
@@ -1968,15 +2239,15 @@ uint32_t avbd_num_energies(avbd_ctx* c, int type) {
 int avbd_get_launch_config(avbd_ctx* c, uint32_t* lanes, uint32_t* block, uint32_t* batch) {
     return guarded(c, [&] {
         if (lanes) *lanes = c->cfg.lanes;
-        if (block) *block = c->persistent ? 256u : c->cfg.block;
-        if (batch) *batch = c->persistent ? 1u : c->cfg.batch;
+        if (block) *block = c->cfg.block;
+        if (batch) *batch = c->cfg.batch;
     });
 }
 
 int avbd_get_step_mode(avbd_ctx* c, uint32_t* persistent_grid, uint32_t* fused_chebyshev) {
     if (!c) return AVBD_E_ARG;
-    if (persistent_grid) *persistent_grid = c->persistent ? c->pgrid : 0u;
-    if (fused_chebyshev) *fused_chebyshev = (c->fuse_cheb && !c->persistent && !c->dynamic) ? 1u : 0u;
+    if (persistent_grid) *persistent_grid = 0u;   // one launch per phase (the persistent step kernel is gone)
+    if (fused_chebyshev) *fused_chebyshev = (c->fuse_cheb && !c->dynamic) ? 1u : 0u;
     return AVBD_OK;
 }
 
@@ -2001,13 +2272,12 @@ int avbd_time_solve_launches(avbd_ctx* c, uint32_t reps, double* avg_ms, uint32_
     return guarded(c, [&] {
         require_built(c);
         if (reps == 0) throw std::runtime_error("reps must be >= 1");
-        const float dt = (float)c->desc.dt;
         uint32_t n = 0;
         auto round = [&] {
             for (uint32_t col = 0; col < c->plan.num_colors; col++) {
                 const uint32_t b = c->plan.item_offsets[col], e = c->plan.item_offsets[col + 1];
                 if (e > b) {
-                    launch_solve_color(c->stream, c->d, c->cfg, b, e - b, c->damping, dt, c->d.x_scratch);
+                    solve_into_scratch(c, c->cfg, b, e - b);
                     n++;
                 }
             }
@@ -2192,7 +2462,7 @@ int avbd_record_steps(avbd_ctx* c, uint32_t n_steps, float* feats, float* tgts, 
                 hip_check(hipMemcpyAsync(raw.data(), c->d_raw, raw.size() * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
                 hip_check(hipMemcpyAsync(tg.data(), c->d_tgt, tg.size() * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
                 hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-                check_abort(c);
+                check_status(c);
                 std::memcpy(feats + (size_t)st * ni * kMlpIn, raw.data(), raw.size() * 4);
                 std::memcpy(tgts + (size_t)st * ni * 3, tg.data(), tg.size() * 4);
                 c->steps_taken++;
@@ -2263,6 +2533,7 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         if (!(margin >= 0)) throw CodeError(AVBD_E_ARG, "collision margin must be >= 0");
         if (c->warm || c->recording || c->res_tol > 0)
             throw CodeError(AVBD_E_ARG, "warm start, recording and residual targets are not available with dynamic contacts");
+        if (c->fp64) throw CodeError(AVBD_E_ARG, "the FP64 state is not available with dynamic contacts");
         const uint32_t n = c->d.n;
         auto oriented = [&](uint32_t p) { return p < n && c->rot_of[p] >= 0; };
         for (const HostPrim& pr : c->prims) {
@@ -2281,7 +2552,6 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         if ((uint64_t)2 * cap > (1ull << 29)) throw CodeError(AVBD_E_ARG, "capacity too large");
         const uint32_t np = (uint32_t)c->prims.size();
         destroy_graph(c);
-        c->persistent = false;
         c->fuse_cheb = false;
         alloc_lbvh(c, c->lbd, np);
         std::vector<uint32_t> ty(np), id(3 * (size_t)np), ob(np);
@@ -2385,8 +2655,11 @@ struct CkptHeader {
     char magic[8];
     uint32_t version, n, n_tets, n_ground, n_rot, n_rground, n_joints;
     uint32_t dynamic, has_damp, has_rc, num_colors;
+    uint32_t fp64, dyn_cap;   // FP64 state; contact records per kind
     uint64_t steps_taken, vertex_updates;
+    uint64_t array_bytes;     // device arrays that follow the header
 };
+constexpr uint32_t kCkptVersion = 2;
 constexpr char kCkptMagic[8] = {'A', 'V', 'B', 'D', 'C', 'K', 'P', 'T'};
 
 // (device pointer, bytes) of every mutable device array, in checkpoint order
@@ -2403,13 +2676,17 @@ std::vector<std::pair<void*, size_t>> ckpt_arrays(avbd_ctx* c) {
     if (d.n_joints) a.emplace_back(d.joints, (size_t)d.n_joints * sizeof(JointRec));
     if (c->rc) a.emplace_back(c->rc, sizeof(ResCtl));
     if (c->dynamic) a.emplace_back(d.dyn, 2 * (size_t)c->cs.cap * sizeof(DynRec));
+    if (c->fp64) {
+        const size_t p8 = (size_t)d.n * sizeof(double4);
+        for (double4* v : {c->f.x, c->f.y, c->f.x_prev, c->f.x_last, c->f.x_lastlast}) a.emplace_back(v, p8);
+    }
     return a;
 }
 
 CkptHeader ckpt_header(avbd_ctx* c) {
     CkptHeader h{};
     std::memcpy(h.magic, kCkptMagic, 8);
-    h.version = 1;
+    h.version = kCkptVersion;
     h.n = c->d.n;
     h.n_tets = c->d.n_tets;
     h.n_ground = c->d.n_ground;
@@ -2420,8 +2697,11 @@ CkptHeader ckpt_header(avbd_ctx* c) {
     h.has_damp = c->damping ? 1u : 0u;
     h.has_rc = c->rc ? 1u : 0u;
     h.num_colors = c->plan.num_colors;
+    h.fp64 = c->fp64 ? 1u : 0u;
+    h.dyn_cap = c->dynamic ? c->cs.cap : 0u;
     h.steps_taken = c->steps_taken;
     h.vertex_updates = c->vertex_updates;
+    for (const auto& a : ckpt_arrays(c)) h.array_bytes += a.second;
     return h;
 }
 
@@ -2483,16 +2763,20 @@ int avbd_load_checkpoint(avbd_ctx* c, const void* buf, uint64_t size) {
         std::memcpy(&h, p, sizeof(h));
         p += sizeof(h);
         const CkptHeader mine = ckpt_header(c);
-        if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1) throw CodeError(AVBD_E_ARG, "not a checkpoint");
+        if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != kCkptVersion)
+            throw CodeError(AVBD_E_ARG, "not a checkpoint");
+        // everything is validated on the host before any part of the context changes
         if (h.n != mine.n || h.n_tets != mine.n_tets || h.n_ground != mine.n_ground || h.n_rot != mine.n_rot ||
             h.n_rground != mine.n_rground || h.n_joints != mine.n_joints || h.dynamic != mine.dynamic ||
-            h.has_damp != mine.has_damp || h.has_rc != mine.has_rc)
+            h.has_damp != mine.has_damp || h.has_rc != mine.has_rc || h.fp64 != mine.fp64 ||
+            h.dyn_cap != mine.dyn_cap || h.array_bytes != mine.array_bytes ||
+            (!c->dynamic && h.num_colors != mine.num_colors))
             throw CodeError(AVBD_E_ARG, "checkpoint does not match this context's scene and modes");
-        for (const auto& a : ckpt_arrays(c)) {
-            if ((uint64_t)(end - p) < a.second) throw CodeError(AVBD_E_ARG, "checkpoint is truncated");
-            hip_check(hipMemcpyAsync(a.first, p, a.second, hipMemcpyHostToDevice, c->stream), "H2D");
-            p += a.second;
-        }
+        if ((uint64_t)(end - p) < h.array_bytes) throw CodeError(AVBD_E_ARG, "checkpoint is truncated");
+        const uint8_t* arrays = p;
+        p += h.array_bytes;
+        std::vector<uint32_t> color;
+        ContactSystem cs;
         if (c->dynamic) {
             uint64_t nc = 0;
             if ((uint64_t)(end - p) < sizeof(uint64_t)) throw CodeError(AVBD_E_ARG, "checkpoint is truncated");
@@ -2500,9 +2784,19 @@ int avbd_load_checkpoint(avbd_ctx* c, const void* buf, uint64_t size) {
             p += sizeof(uint64_t);
             if (nc != c->plan.color.size() || (uint64_t)(end - p) / sizeof(uint32_t) < nc)
                 throw CodeError(AVBD_E_ARG, "checkpoint does not match this context's particles");
-            std::memcpy(c->plan.color.data(), p, nc * sizeof(uint32_t));
+            color.assign(reinterpret_cast<const uint32_t*>(p), reinterpret_cast<const uint32_t*>(p) + nc);
             p += nc * sizeof(uint32_t);
-            c->cs.load(p, end);
+            cs = c->cs;
+            cs.load(p, end);   // throws on a malformed tail, before anything is applied
+        }
+        // apply
+        for (const auto& a : ckpt_arrays(c)) {
+            hip_check(hipMemcpyAsync(a.first, arrays, a.second, hipMemcpyHostToDevice, c->stream), "H2D");
+            arrays += a.second;
+        }
+        if (c->dynamic) {
+            c->plan.color = std::move(color);
+            c->cs = std::move(cs);
             c->plan.num_colors = h.num_colors;
             relayout_plan(c->plan, c->cs.conflicted);
             upload(c, c->d.items, c->plan.items);
@@ -2526,6 +2820,7 @@ int avbd_load_checkpoint(avbd_ctx* c, const void* buf, uint64_t size) {
         }
         c->steps_taken = h.steps_taken;
         c->vertex_updates = h.vertex_updates;
+        clear_status(c);
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     });
 }

@@ -102,7 +102,7 @@ void csr_by_particle(uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>
 
 }  // namespace
 
-void build_color_plan(const HostModel& m, ColorPlan& plan) {
+void build_color_plan(const HostModel& m, ColorPlan& plan, const SceneColourer& colour) {
     const uint32_t n = m.n;
     const auto& nh = *m.nh;
     const auto& ground = *m.ground;
@@ -146,43 +146,59 @@ void build_color_plan(const HostModel& m, ColorPlan& plan) {
     for (const auto& j : joints)
         if (j.p >= n || !oriented(j.p)) throw std::runtime_error("joint needs an oriented particle");
 
-    // colour every scene on its own (threads over scenes)
-    plan.color.assign(n, 0);
-    std::vector<uint32_t> scene_colors(n_scenes, 0);
-    std::vector<std::string> errors(n_scenes);
+    // vertex adjacency of every scene (threads over scenes), kept as one
+    // global CSR (also used by the incremental recolouring)
     std::vector<SceneGraph> graphs(n_scenes);
     const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (uint32_t w = 0; w < hw; w++) {
-        pool.emplace_back([&, w] {
-            for (uint32_t s = w; s < n_scenes; s += hw) {
-                try {
-                    SceneGraph& g = graphs[s];
-                    const uint32_t lo = scene_offsets[s], hi = scene_offsets[s + 1];
-                    scene_adjacency(lo, hi, nh, scene_tets[s], cos, scene_rods[s], g);
-                    scene_colors[s] = dsatur_csr(hi - lo, g.off.data(), g.nb.data(), plan.color.data() + lo);
-                } catch (const std::exception& e) {
-                    errors[s] = e.what();
-                }
-            }
-        });
+    {
+        std::vector<std::thread> pool;
+        for (uint32_t w = 0; w < hw; w++)
+            pool.emplace_back([&, w] {
+                for (uint32_t s = w; s < n_scenes; s += hw)
+                    scene_adjacency(scene_offsets[s], scene_offsets[s + 1], nh, scene_tets[s], cos, scene_rods[s],
+                                    graphs[s]);
+            });
+        for (auto& t : pool) t.join();
     }
-    for (auto& t : pool) t.join();
-    for (uint32_t s = 0; s < n_scenes; s++)
-        if (!errors[s].empty()) throw std::runtime_error(errors[s]);
-    plan.num_colors = 0;
-    for (uint32_t s = 0; s < n_scenes; s++) plan.num_colors = std::max(plan.num_colors, scene_colors[s]);
-    // global static adjacency (kept for incremental recolouring)
     plan.static_off.assign(n + 1, 0);
     plan.static_nb.clear();
     for (uint32_t s = 0; s < n_scenes; s++) {
         const uint32_t lo = scene_offsets[s], hi = scene_offsets[s + 1];
+        const SceneGraph& g = graphs[s];
         for (uint32_t v = lo; v < hi; v++) {
-            const SceneGraph& g = graphs[s];
             for (uint32_t e = g.off[v - lo]; e < g.off[v - lo + 1]; e++) plan.static_nb.push_back(g.nb[e] + lo);
             plan.static_off[v + 1] = (uint32_t)plan.static_nb.size();
         }
     }
+
+    // colour every scene on its own: DSATUR on the device when a colourer is
+    // given, else on the host (threads over scenes)
+    plan.color.assign(n, 0);
+    std::vector<uint32_t> scene_colors(n_scenes, 0);
+    if (colour) {
+        colour(scene_offsets, plan.static_off, plan.static_nb, plan.color, scene_colors);
+    } else {
+        std::vector<std::string> errors(n_scenes);
+        std::vector<std::thread> pool;
+        for (uint32_t w = 0; w < hw; w++) {
+            pool.emplace_back([&, w] {
+                for (uint32_t s = w; s < n_scenes; s += hw) {
+                    try {
+                        const SceneGraph& g = graphs[s];
+                        const uint32_t lo = scene_offsets[s], hi = scene_offsets[s + 1];
+                        scene_colors[s] = dsatur_csr(hi - lo, g.off.data(), g.nb.data(), plan.color.data() + lo);
+                    } catch (const std::exception& e) {
+                        errors[s] = e.what();
+                    }
+                }
+            });
+        }
+        for (auto& t : pool) t.join();
+        for (uint32_t s = 0; s < n_scenes; s++)
+            if (!errors[s].empty()) throw std::runtime_error(errors[s]);
+    }
+    plan.num_colors = 0;
+    for (uint32_t s = 0; s < n_scenes; s++) plan.num_colors = std::max(plan.num_colors, scene_colors[s]);
     graphs.clear();
 
     // per-vertex incidences in pool order (ascending energy index, then local)

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -214,6 +215,41 @@ struct DevState {
     const uint32_t* dyn_off = nullptr;
     const uint32_t* dyn_ent = nullptr;
     DynRec* dyn = nullptr;
+
+    // bit 0: a non-finite position or velocity was produced (set by the
+    // velocity update, read and cleared by the host)
+    uint32_t* status = nullptr;
+
+    // per-particle external acceleration added to gravity (xyz), or null
+    // when the scene has no such field (avbd_set_external_accel)
+    const float4* a_ext = nullptr;
+};
+
+// The particle state in FP64 (AVBD_FLAG_FP64_STATE, vbd_fp64.hip): double4
+// streams (xyz + mass in w) beside the FP32 ones of DevState, which then
+// hold a rounded mirror of the positions at step boundaries.
+struct F64State {
+    double4* x = nullptr;
+    double4* y = nullptr;
+    double4* x_prev = nullptr;
+    double4* x_last = nullptr;
+    double4* x_lastlast = nullptr;
+    double4* x_alt = nullptr;      // second position buffer of the fused Chebyshev phase
+    double4* x0 = nullptr;         // warm start: the initial guess of the step
+    double4* x_scratch = nullptr;  // autotuner output sink
+    float4* x32 = nullptr;         // the FP32 mirror (= DevState::x)
+    float4* v = nullptr;
+    float4* v_prev = nullptr;
+    const uint8_t* fixed = nullptr;
+    const int32_t* ground_of = nullptr;
+    const TetRec* tets = nullptr;
+    TetDamp* damp = nullptr;
+    GroundRec* ground = nullptr;
+    const uint32_t* inc = nullptr;
+    const TetRec* irec = nullptr;
+    const VItem* items = nullptr;
+    const float4* a_ext = nullptr;  // = DevState::a_ext
+    uint32_t n = 0, n_tets = 0, n_ground = 0, n_items = 0;
 };
 
 struct LaunchCfg {
@@ -222,47 +258,23 @@ struct LaunchCfg {
     uint32_t batch = 1;   // incidences a lane gathers at once (1, 2 or 4)
 };
 
-// Grid barrier of the persistent step kernel (zeroed once at build time).
-struct GridBarrier {
-    uint32_t count;   // blocks arrived at the current barrier
-    uint32_t gen;     // barrier generation
-    uint32_t abort;   // set when a wait timed out; every block then leaves
-    uint32_t pad;
-};
-
-// Residual-targeted iteration: per step, the relative force residual is
-// checked every `check_every` iterations and the iteration loop ends once
-// ||r||^2 <= tol^2 ||M g||^2.  acc[k] holds the ||r||^2 of check k of the
-// current step (zeroed at step start).
-constexpr uint32_t kMaxResidualChecks = 64;
+// Residual-targeted iteration: a step iterates in check intervals of
+// `every` iterations; after each interval the force residual is reduced per
+// scene and the step ends once every scene has ||r_s|| <= tol ||M_s g||.
+// Per check k of the current step: rel[k] = ||r|| / ||M g|| over the whole
+// scene set, worst[k] = the largest per-scene ratio (the first
+// kMaxResidualChecks checks are kept).
+constexpr uint32_t kMaxResidualChecks = 1024;
 struct ResCtl {
-    double acc[kMaxResidualChecks];
+    double rel[kMaxResidualChecks];
+    double worst[kMaxResidualChecks];
     uint32_t done;         // set when the current step has converged
     uint32_t last_iters;   // iterations used by the last step
+    uint32_t checks;       // checks run in the current step
+    uint32_t iter;         // iterations run in the current step (Chebyshev weight index)
     unsigned long long total_iters;
     unsigned long long budget_steps;   // steps that ended at max_iters unconverged
 };
-
-struct StepArgs {
-    DevState d;
-    ResCtl* rc = nullptr;         // null: fixed iteration count
-    uint32_t check_every = 0;
-    double tol2 = 0;              // tol^2 * ||M g||^2
-    const uint32_t* color_offsets = nullptr;      // num_colors + 1
-    const uint32_t* conflict_offsets = nullptr;   // num_colors + 1
-    const uint32_t* conflicts = nullptr;
-    const uint32_t* ocolor_offsets = nullptr;     // oriented items per colour, num_colors + 1
-    const float* omega = nullptr;                 // Chebyshev omega per iteration
-    GridBarrier* bar = nullptr;
-    uint32_t iters = 0, num_colors = 0;
-    uint32_t skip_begin = 0;      // the start-of-step phase already ran as separate launches
-    float dt = 0.f, g = 0.f;
-};
-
-// Blocks of 256 threads that fit on the device at once for the persistent
-// kernel with `lanes` lanes per vertex.
-uint32_t persistent_grid(int device, uint32_t lanes, bool damping);
-void launch_step_persistent(hipStream_t s, const StepArgs& a, uint32_t lanes, bool damping, uint32_t grid);
 
 void launch_eprev(hipStream_t s, const DevState& d, float dt);
 void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t item_begin, uint32_t item_count, float dt,
@@ -272,28 +284,33 @@ void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, 
                         uint32_t item_count, bool damping, float dt, float4* x_out, const uint32_t* done = nullptr);
 void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint32_t count,
                    const uint32_t* done = nullptr);
-void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const uint32_t* done = nullptr);
+// rc / omega_tab / j: in residual mode the launch takes the Chebyshev weight
+// omega_tab[rc->iter + j] and returns at once once the step has converged;
+// with rc null it uses `omega`.
+void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const ResCtl* rc = nullptr,
+                      const float* omega_tab = nullptr, uint32_t j = 0);
 // residual-targeted iteration (per-phase path): reset at step start, one
 // last colour phase of an iteration fused with the Chebyshev / dual update
 // (no oriented particles, no conflicts): reads d.x, writes the extrapolated
 // positions of every particle to x_next (cheb_list: the particles that are
 // not work items of this colour)
 void launch_solve_cheb(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
-                       bool damping, float dt, float omega, const uint32_t* cheb_list, uint32_t n_cheb,
-                       float4* x_next, const uint32_t* done);
+                       bool damping, float dt, float omega, const ResCtl* rc, const float* omega_tab, uint32_t j,
+                       const uint32_t* cheb_list, uint32_t n_cheb, float4* x_next);
 // end of a double-buffered step: x = x_alt when the iteration count is odd
 void launch_settle(hipStream_t s, float4* x, const float4* x_alt, uint32_t n, const uint32_t* iters_word,
                    uint32_t iters);
-// check (residual pass + decision) after iteration `iters_done`, and the
-// bookkeeping at step end; `conv` (optional, host-mapped) receives the
-// converged flag after each check
+// residual checks: reset at step start; the pass writes |r_v|^2 per particle
+// into rr (FP32 state; launch_residual_f64 for the FP64 state); the decision
+// sums rr per scene and sets rc->done once every scene is within tol2
+// (relative, squared); the bookkeeping at step end
 void launch_residual_reset(hipStream_t s, ResCtl* rc);
-void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
-                           ResCtl* rc, double* partial, uint32_t check, uint32_t iters_done, double tol2,
-                           uint32_t* conv);
-// doubles of per-block partial sums a residual check needs (any lane count)
-uint32_t residual_partials(const DevState& d);
-void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters);
+void launch_residual_pass(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
+                          const ResCtl* rc, double* rr);
+void launch_residual_decide(hipStream_t s, ResCtl* rc, const double* rr, const uint32_t* scene_off,
+                            uint32_t n_scenes, double* acc_scene, const double* mg2_scene, double tol2,
+                            uint32_t every);
+void launch_residual_finish(hipStream_t s, ResCtl* rc);
 // learned warm start
 void launch_warm_save(hipStream_t s, const DevState& d, const WarmIO& io);
 void launch_warm_features(hipStream_t s, const DevState& d, const WarmIO& io, uint32_t lanes, float dt, float g,
@@ -313,6 +330,30 @@ void launch_dyn_dual(hipStream_t s, const DevState& d, const uint32_t* active, u
 void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n_float4);
 // residual/IP pass: writes 3 partial sums per block into d.reduce; returns block count
 uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt, float g);
+
+// FP64-state kernels (vbd_fp64.hip)
+void launch_solve_color_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
+                            bool damping, double dt, double4* x_out, const uint32_t* done);
+void launch_solve_cheb_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
+                           bool damping, double dt, float omega, const ResCtl* rc, const float* omega_tab,
+                           uint32_t j, const uint32_t* cheb_list, uint32_t n_cheb, const double4* x_cur,
+                           double4* x_next);
+void launch_cheb_dual_f64(hipStream_t s, const F64State& f, uint32_t n, float omega, const ResCtl* rc,
+                          const float* omega_tab, uint32_t j);
+void launch_step_begin_f64(hipStream_t s, const F64State& f, uint32_t n, double dt, double g);
+void launch_eprev_f64(hipStream_t s, const F64State& f, uint32_t n_tets);
+void launch_velocity_f64(hipStream_t s, const F64State& f, uint32_t n, double dt, uint32_t* status);
+void launch_settle_f64(hipStream_t s, double4* x, const double4* x_alt, uint32_t n, const uint32_t* iters_word,
+                       uint32_t iters);
+void launch_residual_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg, bool damping, double dt,
+                         const double4* x, double* rr, const ResCtl* rc);
+// avbd_get_stats with the FP64 state: 4 sums per block into out; returns the block count
+uint32_t launch_stats_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg, bool damping, double dt, double g,
+                          double* out);
+void launch_warm_save_f64(hipStream_t s, const F64State& f, uint32_t n);
+void launch_warm_place_f64(hipStream_t s, const F64State& f, const float4* pred, const uint32_t* scene_of,
+                           const float* step_of_scene);
+void launch_warm_target_f64(hipStream_t s, const F64State& f, float* tgt, double inv_s);
 
 // ---------------------------------------------------------------------------
 // Host-side graph build (host_graph.cpp)
@@ -386,9 +427,23 @@ struct ColorPlan {
     std::vector<uint32_t> static_off, static_nb;
 };
 
-// Build adjacency and colour every scene with DSATUR; then lay out the
+// Colours every scene of a global static adjacency CSR (scene offsets,
+// vertex offsets, neighbours) into color, with the colour count per scene.
+using SceneColourer = std::function<void(const std::vector<uint32_t>&, const std::vector<uint32_t>&,
+                                         const std::vector<uint32_t>&, std::vector<uint32_t>&,
+                                         std::vector<uint32_t>&)>;
+
+// Build adjacency and colour every scene with DSATUR (with `colour` when it
+// is set -- the device colourer -- else on the host); then lay out the
 // per-colour work items.  Throws std::runtime_error on failure.
-void build_color_plan(const HostModel& m, ColorPlan& plan);
+void build_color_plan(const HostModel& m, ColorPlan& plan, const SceneColourer& colour = nullptr);
+
+// DSATUR of every scene on the device (graph_kernels.hip): one workgroup
+// per scene; key and seen are per-vertex scratch (seen zeroed, 4 words per
+// vertex); status bit 0 = more than 256 colours, bit 1 = a degree > 65535.
+void launch_dsatur(hipStream_t s, uint32_t n_scenes, const uint32_t* scene_off, const uint32_t* off,
+                   const uint32_t* nb, uint64_t* key, uint64_t* seen, uint32_t* color, uint32_t* scene_colors,
+                   uint32_t* status);
 
 // Regroup the work items by plan.color (colour-major, ascending particle)
 // with the conflict flags of `conflicted`, and rebuild the per-colour conflict

@@ -41,6 +41,7 @@ struct WarmIO {
     float4* pred = nullptr;              // per particle: predicted step from the initial guess
     double* ipd = nullptr;               // per step length and scene: IP(x0 + a pred) - IP(x0)
     unsigned long long* counters = nullptr;   // [0] scene-steps predicted, [1] fallbacks to the inertial start
+    float* step_of = nullptr;            // per scene: the chosen step along the prediction (0 = fallback), or null
     uint32_t n_scenes = 0;
 };
 

@@ -440,7 +440,7 @@ __device__ __forceinline__ void solve_oitem(const DevState& d, const OItem& it, 
     H.a[s6(4, 4)] += I.y * inv_dt2;
     H.a[s6(5, 5)] += I.z * inv_dt2;
     float dx[6];
-    ldlt6(H, b, dx);
+    lu6(H, b, dx);
     const float4 nx = make_float4(pos.x + dx[0], pos.y + dx[1], pos.z + dx[2], xv.w);
     const float4 nq = plus_s3(q, v3(dx[3], dx[4], dx[5]));
     if (it.n_cos & VITEM_CONFLICT) {

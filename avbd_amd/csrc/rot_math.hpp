@@ -197,40 +197,62 @@ __device__ __forceinline__ void sym6_outer(Sym6& h, const float a[6], float s) {
         for (int j = i; j < 6; j++) h.a[s6(i, j)] += s * a[i] * a[j];
 }
 
-// Solve A x = b for symmetric A (diagonally dominated by the inertia term)
-// by LDL^T without pivoting, fully unrolled in registers.
-__device__ __forceinline__ void ldlt6(const Sym6& A, const float b[6], float x[6]) {
-    float L[6][6];
-    float D[6];
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-        float d = A.a[s6(j, j)];
-#pragma unroll
-        for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k] * D[k];
-        D[j] = d;
-        const float inv = 1.f / d;
-#pragma unroll
-        for (int i = j + 1; i < 6; i++) {
-            float s = A.a[s6(i, j)];
-#pragma unroll
-            for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k] * D[k];
-            L[i][j] = s * inv;
-        }
-    }
-    float z[6];
+// Solve A x = b for the symmetric 6x6 block system by Gaussian elimination
+// with partial (row) pivoting -- the algorithm of Eigen's partialPivLu that the
+// reference uses (simulation/VBDSolver.hpp:472,498).  The block is not
+// always positive definite: the rotational Hessian of a rigid-ground contact
+// at a point off the centre (box, capsule) and of the triangle-rigid
+// contacts is indefinite, and a large multiplier can outweigh I/dt^2.  The
+// pivot is the first row of largest magnitude in the column; the row swaps
+// are register selects, so every index stays compile-time.
+__device__ __forceinline__ void lu6(const Sym6& S, const float b[6], float x[6]) {
+    float A[6][6], r[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        float s = b[i];
+        r[i] = b[i];
 #pragma unroll
-        for (int k = 0; k < i; k++) s -= L[i][k] * z[k];
-        z[i] = s;
+        for (int j = 0; j < 6; j++) A[i][j] = S.a[i <= j ? s6(i, j) : s6(j, i)];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        int piv = k;
+        float best = fabsf(A[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < 6; i++) {
+            const float v = fabsf(A[i][k]);
+            if (v > best) {
+                best = v;
+                piv = i;
+            }
+        }
+#pragma unroll
+        for (int i = k + 1; i < 6; i++) {
+            const bool sw = piv == i;
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const float t = A[k][j];
+                A[k][j] = sw ? A[i][j] : t;
+                A[i][j] = sw ? t : A[i][j];
+            }
+            const float t = r[k];
+            r[k] = sw ? r[i] : t;
+            r[i] = sw ? t : r[i];
+        }
+        const float inv = 1.f / A[k][k];
+#pragma unroll
+        for (int i = k + 1; i < 6; i++) {
+            const float f = A[i][k] * inv;
+#pragma unroll
+            for (int j = k + 1; j < 6; j++) A[i][j] -= f * A[k][j];
+            r[i] -= f * r[k];
+        }
     }
 #pragma unroll
     for (int i = 5; i >= 0; i--) {
-        float s = z[i] / D[i];
+        float t = r[i];
 #pragma unroll
-        for (int k = i + 1; k < 6; k++) s -= L[k][i] * x[k];
-        x[i] = s;
+        for (int j = i + 1; j < 6; j++) t -= A[i][j] * x[j];
+        x[i] = t / A[i][i];
     }
 }
 

@@ -460,8 +460,33 @@ __device__ __forceinline__ void step_begin_one(const DevState& d, uint32_t p, fl
     const int32_t r = d.n_rot ? d.rot_of[p] : -1;
     if (r >= 0) rigid::records_step_begin(d, p, (uint32_t)r);
     if (d.fixed[p]) return;
-    // _particleInertialUpdate (VBDSolver.hpp:208-248): gravity (0,-g,0)
     const float4 vv = d.v[p], vp = d.v_prev[p];
+    if (d.a_ext) {
+        // with a per-particle field f (not in the reference): the external
+        // acceleration gravity + f, in the inertial target and the adaptive
+        // initial guess alike
+        const float4 f = d.a_ext[p];
+        const float ex = f.x, ey = f.y - g, ez = f.z;
+        const float an = sqrtf(ex * ex + ey * ey + ez * ez);
+        float at = 0.f;
+        if (an > 1e-12f) {
+            const float along = ((vv.x - vp.x) * ex + (vv.y - vp.y) * ey + (vv.z - vp.z) * ez) / dt / an;
+            if (along > an) at = 1.f;
+            else if (along < 0.f) at = 0.f;
+            else at = along / an;
+        }
+        const float dt2 = dt * dt;
+        d.y[p] = make_float4(xp.x + dt * vv.x + dt2 * ex, xp.y + dt * vv.y + dt2 * ey, xp.z + dt * vv.z + dt2 * ez, 0.f);
+        xp.x = xp.x + (dt * vv.x + dt2 * (at * ex));
+        xp.y = xp.y + (dt * vv.y + dt2 * (at * ey));
+        xp.z = xp.z + (dt * vv.z + dt2 * (at * ez));
+        d.x[p] = xp;
+        d.x_last[p] = xp;
+        d.x_lastlast[p] = xp;
+        if (r >= 0) rigid::rotation_inertial(d, (uint32_t)r, dt);
+        return;
+    }
+    // _particleInertialUpdate (VBDSolver.hpp:208-248): gravity (0,-g,0)
     const float ay = (vv.y - vp.y) / dt;
     float a_tilde = 0.f;
     if (g > 1e-12f) {
@@ -562,10 +587,19 @@ __device__ __forceinline__ void cheb_dual_one(const DevState& d, uint32_t p, flo
     cheb_dual_io(d, p, omega, d.x, d.x);
 }
 
-__global__ void cheb_dual_kernel(DevState d, float omega, const uint32_t* __restrict__ done) {
-    if (done && *done) return;
+// Chebyshev weight of local iteration j of the running check interval: from
+// the per-iteration table at the step's iteration counter in residual mode
+// (one interval graph serves every interval of a step), else the weight
+// baked into the launch.
+__device__ __forceinline__ float cheb_weight(float omega, const ResCtl* rc, const float* omega_tab, uint32_t j) {
+    return rc ? omega_tab[rc->iter + j] : omega;
+}
+
+__global__ void cheb_dual_kernel(DevState d, float omega, const ResCtl* rc, const float* __restrict__ omega_tab,
+                                 uint32_t j) {
+    if (rc && rc->done) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < d.n) cheb_dual_one(d, p, omega);
+    if (p < d.n) cheb_dual_one(d, p, cheb_weight(omega, rc, omega_tab, j));
 }
 
 // The last colour phase of an iteration fused with the Chebyshev / dual
@@ -584,9 +618,10 @@ __global__ void cheb_dual_kernel(DevState d, float omega, const uint32_t* __rest
 template <int L, bool DAMP, int U>
 __global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_cheb_kernel(
     DevState d, const VItem* __restrict__ items, uint32_t n_items, uint32_t sx, uint32_t cx,
-    const uint32_t* __restrict__ cheb_list, uint32_t n_cheb, float4* __restrict__ x_next, float omega, float inv_dt,
-    float inv_dt2, const uint32_t* __restrict__ done) {
-    if (done && *done) return;
+    const uint32_t* __restrict__ cheb_list, uint32_t n_cheb, float4* __restrict__ x_next, float omega_fixed,
+    const ResCtl* rc, const float* __restrict__ omega_tab, uint32_t j, float inv_dt, float inv_dt2) {
+    if (rc && rc->done) return;
+    const float omega = cheb_weight(omega_fixed, rc, omega_tab, j);
     const uint32_t xcd = blockIdx.x % 8u, q = blockIdx.x / 8u;
     if (q < sx) {
         const uint32_t tid = (xcd * sx + q) * blockDim.x + threadIdx.x;
@@ -634,13 +669,23 @@ __global__ void settle_kernel(float4* __restrict__ x, const float4* __restrict__
 }
 
 // _particleVelocityUpdate (VBDSolver.hpp:538-566) for one particle.
+// A non-finite position or velocity (or orientation / angular velocity)
+// raises bit 0 of d.status; the host reports it at the next synchronising call.
 __device__ __forceinline__ void velocity_one(const DevState& d, uint32_t p, float inv_dt) {
     const float4 xp = d.x[p], xo = d.x_prev[p], vv = d.v[p];
     d.v_prev[p] = vv;
-    d.v[p] = make_float4((xp.x - xo.x) * inv_dt, (xp.y - xo.y) * inv_dt, (xp.z - xo.z) * inv_dt, 0.f);
+    const float4 nv = make_float4((xp.x - xo.x) * inv_dt, (xp.y - xo.y) * inv_dt, (xp.z - xo.z) * inv_dt, 0.f);
+    d.v[p] = nv;
     d.x_prev[p] = xp;
+    bool ok = isfinite(xp.x) && isfinite(xp.y) && isfinite(xp.z) && isfinite(nv.x) && isfinite(nv.y) && isfinite(nv.z);
     const int32_t r = d.n_rot ? d.rot_of[p] : -1;
-    if (r >= 0) rigid::rotation_velocity(d, (uint32_t)r, inv_dt);
+    if (r >= 0) {
+        rigid::rotation_velocity(d, (uint32_t)r, inv_dt);
+        const float4 q = d.q[r], w = d.w[r];
+        ok = ok && isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && isfinite(q.w) && isfinite(w.x) &&
+             isfinite(w.y) && isfinite(w.z);
+    }
+    if (!ok && d.status) atomicOr(d.status, 1u);
 }
 
 __global__ void velocity_kernel(DevState d, float inv_dt) {
@@ -743,76 +788,102 @@ __device__ __forceinline__ float residual_oitem(const DevState& d, const OItem& 
     return rx * rx + ry * ry + rz * rz;
 }
 
-// Sum a per-thread double over the block (256 threads) into *dst with one atomic.
-__device__ __forceinline__ void block_atomic_sum(double v, double* dst, double* red) {
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t st = blockDim.x / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(dst, red[0]);
-}
-
-// Sum of v over a block of 256 threads (fixed tree), valid in thread 0.
-__device__ __forceinline__ double block_sum256(double v, double* red) {
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
-    }
-    return red[0];
-}
-
-// Each block writes its partial sum of |r_v|^2; residual_decide_kernel adds
-// the partials in a fixed order (no contended atomics, reproducible sums).
+// |r_v|^2 of every work item into rr[v] (positional and oriented work items
+// are disjoint particles; fixed particles keep the 0 the buffer starts with).
 template <int L, bool DAMP>
-__global__ __launch_bounds__(256) void residual_pass_kernel(DevState d, ResCtl* rc, double* __restrict__ partial,
+__global__ __launch_bounds__(256) void residual_pass_kernel(DevState d, const ResCtl* rc, double* __restrict__ rr,
                                                             float inv_dt, float inv_dt2) {
     if (rc->done) return;
-    __shared__ double red[256];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L, lane = tid % L;
-    double part = 0;
-    if (item < d.n_items) part = residual_item<L, DAMP>(d.items[item], lane, d, inv_dt, inv_dt2);
-    if (tid < d.n_oitems) part += residual_oitem(d, d.oitems[tid], inv_dt2);
-    const double s = block_sum256(part, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (item < d.n_items) {
+        const VItem it = d.items[item];
+        const float r = residual_item<L, DAMP>(it, lane, d, inv_dt, inv_dt2);
+        if (lane == 0) rr[it.v] = r;
+    }
+    if (tid < d.n_oitems) {
+        const OItem it = d.oitems[tid];
+        rr[it.v] = residual_oitem(d, it, inv_dt2);
+    }
 }
 
-// One block of 256: ||r||^2 of this check from the partials, the decision,
-// and the converged flag for the host.
-__global__ __launch_bounds__(256) void residual_decide_kernel(ResCtl* rc, const double* __restrict__ partial,
-                                                              uint32_t n_partial, uint32_t check, uint32_t iters_done,
-                                                              double tol2, uint32_t* conv) {
-    __shared__ double red[256];
-    const bool was_done = rc->done != 0;
-    if (!was_done) {
-        double v = 0;
-        for (uint32_t k = threadIdx.x; k < n_partial; k += 256) v += partial[k];
-        const double total = block_sum256(v, red);
-        if (threadIdx.x == 0) {
-            rc->acc[check] = total;
-            if (total <= tol2) {
-                rc->done = 1;
-                rc->last_iters = iters_done;
-            }
+// One block per scene: sum of rr over the scene's particles in a fixed order
+// (reproducible, no atomics).
+__global__ __launch_bounds__(1024) void residual_scene_kernel(const ResCtl* rc, const double* __restrict__ rr,
+                                                              const uint32_t* __restrict__ scene_off,
+                                                              double* __restrict__ acc_scene) {
+    if (rc->done) return;
+    __shared__ double red[1024];
+    const uint32_t s = blockIdx.x, b = scene_off[s], e = scene_off[s + 1];
+    double v = 0;
+    for (uint32_t p = b + threadIdx.x; p < e; p += 1024) v += rr[p];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t st = 512; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc_scene[s] = red[0];
+}
+
+// The decision after a check interval of `every` iterations: every scene
+// must reach its own target, ||r_s||^2 <= tol^2 ||M_s g||^2 (scenes without
+// a free particle are skipped).  Records the check (the whole batch's
+// relative residual and the worst scene's) and advances the step's
+// iteration counter, which the Chebyshev launches read their weight by.
+__global__ __launch_bounds__(256) void residual_decide_kernel(ResCtl* rc, const double* __restrict__ acc_scene,
+                                                              const double* __restrict__ mg2_scene,
+                                                              uint32_t n_scenes, double tol2, uint32_t every) {
+    if (rc->done) return;
+    __shared__ double r_s[256], m_s[256], w_s[256];
+    double sum = 0, mg = 0, w = 0;
+    for (uint32_t s = threadIdx.x; s < n_scenes; s += 256) {
+        const double a = acc_scene[s], m = mg2_scene[s];
+        sum += a;
+        mg += m;
+        if (m > 0) w = fmax(w, a / m);
+    }
+    r_s[threadIdx.x] = sum;
+    m_s[threadIdx.x] = mg;
+    w_s[threadIdx.x] = w;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            r_s[threadIdx.x] += r_s[threadIdx.x + st];
+            m_s[threadIdx.x] += m_s[threadIdx.x + st];
+            w_s[threadIdx.x] = fmax(w_s[threadIdx.x], w_s[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t k = rc->checks;
+        rc->checks = k + 1;
+        rc->iter += every;
+        if (k < kMaxResidualChecks) {
+            rc->rel[k] = m_s[0] > 0 ? sqrt(r_s[0] / m_s[0]) : 0.0;
+            rc->worst[k] = sqrt(w_s[0]);
+        }
+        if (w_s[0] <= tol2) {
+            rc->done = 1;
+            rc->last_iters = rc->iter;
         }
     }
-    // the host polls this flag while later intervals of the step are queued
-    if (threadIdx.x == 0 && conv)
-        __hip_atomic_store(conv + check, was_done || rc->done ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Start of a residual-targeted step: nothing converged, no iteration run.
 __global__ void residual_reset_kernel(ResCtl* rc) {
-    if (threadIdx.x < kMaxResidualChecks) rc->acc[threadIdx.x] = 0.0;
-    if (threadIdx.x == 0) rc->done = 0;
+    if (threadIdx.x == 0) {
+        rc->done = 0;
+        rc->checks = 0;
+        rc->iter = 0;
+    }
 }
 
-__global__ void residual_finish_kernel(ResCtl* rc, uint32_t max_iters) {
+// End of a residual-targeted step: a step that never converged used the
+// whole budget (the iterations it ran).
+__global__ void residual_finish_kernel(ResCtl* rc) {
     if (threadIdx.x == 0) {
-        if (!rc->done) rc->last_iters = max_iters;
+        if (!rc->done) rc->last_iters = rc->iter;
         rc->total_iters += rc->last_iters;
         if (!rc->done) rc->budget_steps += 1;
     }
@@ -1129,8 +1200,10 @@ __device__ __forceinline__ float warm_choice(const WarmIO& io, uint32_t s) {
 __global__ void warm_choose_kernel(DevState d, WarmIO io) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < io.n_scenes) {
+        const float a = warm_choice(io, i);
         atomicAdd(&io.counters[0], 1ull);
-        if (warm_choice(io, i) == 0.f) atomicAdd(&io.counters[1], 1ull);
+        if (a == 0.f) atomicAdd(&io.counters[1], 1ull);
+        if (io.step_of) io.step_of[i] = a;
     }
     if (i >= d.n_items) return;
     const uint32_t v = d.items[i].v;
@@ -1149,132 +1222,6 @@ __global__ void warm_target_kernel(DevState d, WarmIO io, float inv_s) {
     io.tgt[(size_t)i * 3 + 0] = (x.x - x0.x) * inv_s;
     io.tgt[(size_t)i * 3 + 1] = (x.y - x0.y) * inv_s;
     io.tgt[(size_t)i * 3 + 2] = (x.z - x0.z) * inv_s;
-}
-
-// ---- persistent step kernel ------------------------------------------------
-// One launch runs a whole time step; every barrier of solve_Parallel becomes a
-// grid barrier.  The grid is sized so that all blocks are resident at once
-// (occupancy query at build time); a barrier wait is bounded: after
-// kBarrierSpinLimit polls the waiting block raises the abort word, every
-// block leaves at its next barrier, and the host reports the failure -- no
-// wave can wait forever.
-constexpr uint32_t kBarrierSpinLimit = 1u << 22;
-
-__device__ __forceinline__ uint32_t aload_u32(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Returns false (uniformly over the block) when the step was aborted.
-__device__ __forceinline__ bool grid_sync(GridBarrier* bar, uint32_t nblocks, uint32_t& gen, uint32_t* ok) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t target = gen + 1;
-        __threadfence();   // release this block's writes
-        if (atomicAdd(&bar->count, 1u) == nblocks - 1) {
-            atomicExch(&bar->count, 0u);
-            __threadfence();
-            atomicExch(&bar->gen, target);
-        } else {
-            uint32_t spins = 0;
-            while (aload_u32(&bar->gen) != target) {
-                if (aload_u32(&bar->abort)) break;
-                if (++spins > kBarrierSpinLimit) {
-                    atomicExch(&bar->abort, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        *ok = aload_u32(&bar->abort) == 0;
-    }
-    gen = gen + 1;
-    __syncthreads();
-    __threadfence();   // acquire: other blocks' writes become visible (L1 invalidated)
-    return *ok != 0;
-}
-
-template <int L, bool DAMP>
-__global__ __launch_bounds__(256) void step_persistent_kernel(StepArgs a) {
-    const DevState& d = a.d;
-    const uint32_t G = gridDim.x, B = blockDim.x;
-    const uint32_t lb = xcd_logical_block();
-    const uint32_t gtid = lb * B + threadIdx.x, gthreads = G * B;
-    __shared__ uint32_t gen_s, ok_s;
-    if (threadIdx.x == 0) gen_s = aload_u32(&a.bar->gen);
-    __syncthreads();
-    uint32_t gen = gen_s;
-    const float inv_dt = 1.0f / a.dt, inv_dt2 = inv_dt * inv_dt;
-
-    __shared__ double red_s[256];
-    if (a.rc && blockIdx.x == 0) {
-        // read by other blocks only after at least one grid barrier
-        if (threadIdx.x < kMaxResidualChecks) a.rc->acc[threadIdx.x] = 0.0;
-        if (threadIdx.x == 0) a.rc->done = 0;
-    }
-    if (!a.skip_begin) {
-        if constexpr (DAMP) {
-            for (uint32_t t = gtid; t < d.n_tets; t += gthreads) eprev_one(t, d.tets, d.damp, d.x_prev);
-        }
-        for (uint32_t p = gtid; p < d.n; p += gthreads) step_begin_one(d, p, a.dt, a.g);
-    }
-    if (!grid_sync(a.bar, G, gen, &ok_s)) return;
-    uint32_t iters_used = a.iters;
-
-    const uint32_t groups = B / L, lane = threadIdx.x % L, grp = threadIdx.x / L;
-    for (uint32_t it = 0; it < a.iters; it++) {
-        for (uint32_t c = 0; c < a.num_colors; c++) {
-            const uint32_t beg = a.color_offsets[c], cnt = a.color_offsets[c + 1] - beg;
-            // contiguous share per logical block (XCD-local slabs)
-            const uint32_t per = (cnt + G - 1) / G;
-            const uint32_t lo = min(cnt, lb * per), hi = min(cnt, lo + per);
-            for (uint32_t k = lo + grp; k < hi; k += groups)
-                solve_item<L, DAMP>(d.items[beg + k], lane, d.inc, d.irec, d.damp, d.x, d.y, d.ground, d.x, d.x_buf,
-                                    inv_dt, inv_dt2, dyn_args(d));
-            if (d.n_oitems) {
-                const uint32_t ob = a.ocolor_offsets[c], ocnt = a.ocolor_offsets[c + 1] - ob;
-                const uint32_t oper = (ocnt + G - 1) / G;
-                const uint32_t olo = min(ocnt, lb * oper), ohi = min(ocnt, olo + oper);
-                for (uint32_t k = olo + threadIdx.x; k < ohi; k += B) rigid::solve_oitem(d, d.oitems[ob + k], inv_dt2);
-            }
-            if (!grid_sync(a.bar, G, gen, &ok_s)) return;
-            const uint32_t cb = a.conflict_offsets[c], ce = a.conflict_offsets[c + 1];
-            if (ce > cb) {
-                for (uint32_t k = cb + gtid; k < ce; k += gthreads) commit_one(d, a.conflicts[k]);
-                if (!grid_sync(a.bar, G, gen, &ok_s)) return;
-            }
-        }
-        const float omega = a.omega[it];
-        for (uint32_t p = gtid; p < d.n; p += gthreads) cheb_dual_one(d, p, omega);
-        if (!grid_sync(a.bar, G, gen, &ok_s)) return;
-        if (a.rc && (it + 1) % a.check_every == 0 && it + 1 < a.iters) {
-            // relative force residual at the current iterate: contiguous share of
-            // all work items per logical block, one atomic per block, a barrier,
-            // then every block reads the same total and takes the same branch
-            const uint32_t check = (it + 1) / a.check_every - 1;
-            double part = 0;
-            const uint32_t per = (d.n_items + G - 1) / G;
-            const uint32_t lo = min(d.n_items, lb * per), hi = min(d.n_items, lo + per);
-            for (uint32_t k = lo + grp; k < hi; k += groups)
-                part += residual_item<L, DAMP>(d.items[k], lane, d, inv_dt, inv_dt2);
-            const uint32_t oper = (d.n_oitems + G - 1) / G;
-            const uint32_t olo = min(d.n_oitems, lb * oper), ohi = min(d.n_oitems, olo + oper);
-            for (uint32_t k = olo + threadIdx.x; k < ohi; k += B) part += residual_oitem(d, d.oitems[k], inv_dt2);
-            block_atomic_sum(part, &a.rc->acc[check], red_s);
-            if (!grid_sync(a.bar, G, gen, &ok_s)) return;
-            const double total = __hip_atomic_load(&a.rc->acc[check], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            if (total <= a.tol2) {
-                iters_used = it + 1;
-                break;
-            }
-        }
-    }
-    if (a.rc && blockIdx.x == 0 && threadIdx.x == 0) {
-        a.rc->done = iters_used < a.iters ? 1u : 0u;
-        a.rc->last_iters = iters_used;
-        a.rc->total_iters += iters_used;
-        if (iters_used >= a.iters) a.rc->budget_steps += 1;
-    }
-    for (uint32_t p = gtid; p < d.n; p += gthreads) velocity_one(d, p, inv_dt);
 }
 
 // ---- residual / incremental potential (new; see avbd_get_stats) ----------
@@ -1419,13 +1366,11 @@ void solve_dispatch(hipStream_t s, const DevState& d, uint32_t block, uint32_t b
 }
 
 template <int L, bool DAMP>
-uint32_t residual_dispatch(hipStream_t s, const DevState& d, float dt, ResCtl* rc, double* partial) {
+void residual_dispatch(hipStream_t s, const DevState& d, float dt, const ResCtl* rc, double* rr) {
     const uint32_t threads = max(d.n_items * L, d.n_oitems);
-    if (threads == 0) return 0;
-    const uint32_t nb = blocks_for(threads, 256);
-    hipLaunchKernelGGL((residual_pass_kernel<L, DAMP>), dim3(nb), dim3(256), 0, s, d, rc, partial, 1.0f / dt,
-                       1.0f / (dt * dt));
-    return nb;
+    if (threads == 0) return;
+    hipLaunchKernelGGL((residual_pass_kernel<L, DAMP>), dim3(blocks_for(threads, 256)), dim3(256), 0, s, d, rc, rr,
+                       1.0f / dt, 1.0f / (dt * dt));
 }
 
 template <int L, bool DAMP>
@@ -1466,16 +1411,12 @@ void launch_residual_reset(hipStream_t s, ResCtl* rc) {
     hipLaunchKernelGGL(residual_reset_kernel, dim3(1), dim3(64), 0, s, rc);
 }
 
-uint32_t residual_partials(const DevState& d) { return blocks_for(max(d.n_items * 32u, d.n_oitems), 256); }
-
-void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
-                           ResCtl* rc, double* partial, uint32_t check, uint32_t iters_done, double tol2,
-                           uint32_t* conv) {
-    uint32_t nb = 0;
-#define AVBD_RES_CASE(LL)                                                          \
-    case LL:                                                                        \
-        nb = damping ? residual_dispatch<LL, true>(s, d, dt, rc, partial)           \
-                     : residual_dispatch<LL, false>(s, d, dt, rc, partial);         \
+void launch_residual_pass(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
+                          const ResCtl* rc, double* rr) {
+#define AVBD_RES_CASE(LL)                                                                                    \
+    case LL:                                                                                                  \
+        if (damping) residual_dispatch<LL, true>(s, d, dt, rc, rr);                                          \
+        else residual_dispatch<LL, false>(s, d, dt, rc, rr);                                                 \
         break;
     switch (cfg.lanes) {
         AVBD_RES_CASE(4)
@@ -1485,12 +1426,18 @@ void launch_residual_check(hipStream_t s, const DevState& d, const LaunchCfg& cf
         default: break;
     }
 #undef AVBD_RES_CASE
-    hipLaunchKernelGGL(residual_decide_kernel, dim3(1), dim3(256), 0, s, rc, partial, nb, check, iters_done, tol2,
-                       conv);
 }
 
-void launch_residual_finish(hipStream_t s, ResCtl* rc, uint32_t max_iters) {
-    hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(64), 0, s, rc, max_iters);
+void launch_residual_decide(hipStream_t s, ResCtl* rc, const double* rr, const uint32_t* scene_off,
+                            uint32_t n_scenes, double* acc_scene, const double* mg2_scene, double tol2,
+                            uint32_t every) {
+    hipLaunchKernelGGL(residual_scene_kernel, dim3(n_scenes), dim3(1024), 0, s, rc, rr, scene_off, acc_scene);
+    hipLaunchKernelGGL(residual_decide_kernel, dim3(1), dim3(256), 0, s, rc, acc_scene, mg2_scene, n_scenes, tol2,
+                       every);
+}
+
+void launch_residual_finish(hipStream_t s, ResCtl* rc) {
+    hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(64), 0, s, rc);
 }
 
 void launch_dyn_apply(hipStream_t s, const DevState& d, const DynOp* ops, uint32_t n_ops) {
@@ -1529,8 +1476,8 @@ void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, 
 }
 
 void launch_solve_cheb(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
-                       bool damping, float dt, float omega, const uint32_t* cheb_list, uint32_t n_cheb,
-                       float4* x_next, const uint32_t* done) {
+                       bool damping, float dt, float omega, const ResCtl* rc, const float* omega_tab, uint32_t j,
+                       const uint32_t* cheb_list, uint32_t n_cheb, float4* x_next) {
     uint32_t block = cfg.block, batch = cfg.batch;
     if (block > 256) batch = 1;   // the batched variants are built for blocks of <= 256 threads
     const uint32_t L = cfg.lanes;
@@ -1540,7 +1487,7 @@ void launch_solve_cheb(hipStream_t s, const DevState& d, const LaunchCfg& cfg, u
     const float inv_dt = 1.0f / dt, inv_dt2 = 1.0f / (dt * dt);
 #define AVBD_CHEB_LAUNCH(LL, DD, UU)                                                                           \
     hipLaunchKernelGGL((solve_cheb_kernel<LL, DD, UU>), grid, blk, 0, s, d, d.items + begin, count, sx, cx,     \
-                       cheb_list, n_cheb, x_next, omega, inv_dt, inv_dt2, done)
+                       cheb_list, n_cheb, x_next, omega, rc, omega_tab, j, inv_dt, inv_dt2)
 #define AVBD_CHEB_U(LL, DD)                          \
     if (batch >= 4) AVBD_CHEB_LAUNCH(LL, DD, 4);     \
     else if (batch >= 2) AVBD_CHEB_LAUNCH(LL, DD, 2); \
@@ -1573,9 +1520,10 @@ void launch_commit(hipStream_t s, const DevState& d, const uint32_t* list, uint3
     hipLaunchKernelGGL(commit_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, d, list, count, done);
 }
 
-void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const uint32_t* done) {
+void launch_cheb_dual(hipStream_t s, const DevState& d, float omega, const ResCtl* rc, const float* omega_tab,
+                      uint32_t j) {
     if (d.n == 0) return;
-    hipLaunchKernelGGL(cheb_dual_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, omega, done);
+    hipLaunchKernelGGL(cheb_dual_kernel, dim3(blocks_for(d.n, 256)), dim3(256), 0, s, d, omega, rc, omega_tab, j);
 }
 
 void launch_velocity(hipStream_t s, const DevState& d, float dt) {
@@ -1681,48 +1629,3 @@ void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n
 
 }  // namespace avbd
 
-namespace avbd {
-
-namespace {
-template <int L, bool DAMP>
-const void* persistent_fn() {
-    return reinterpret_cast<const void*>(&step_persistent_kernel<L, DAMP>);
-}
-const void* persistent_fn(uint32_t lanes, bool damping) {
-    switch (lanes) {
-        case 4: return damping ? persistent_fn<4, true>() : persistent_fn<4, false>();
-        case 8: return damping ? persistent_fn<8, true>() : persistent_fn<8, false>();
-        case 16: return damping ? persistent_fn<16, true>() : persistent_fn<16, false>();
-        default: return damping ? persistent_fn<32, true>() : persistent_fn<32, false>();
-    }
-}
-}  // namespace
-
-uint32_t persistent_grid(int device, uint32_t lanes, bool damping) {
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_fn(lanes, damping), 256, 0) != hipSuccess)
-        return 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    // two resident blocks per CU (of the occupancy limit) keep every block on
-    // the chip together with a concurrent broad-phase stream
-    const int use = per_cu < 2 ? per_cu : 2;
-    return (uint32_t)(use * cus);
-}
-
-void launch_step_persistent(hipStream_t s, const StepArgs& a, uint32_t lanes, bool damping, uint32_t grid) {
-#define AVBD_PERSIST_CASE(LL)                                                                                  \
-    case LL:                                                                                                    \
-        if (damping) hipLaunchKernelGGL((step_persistent_kernel<LL, true>), dim3(grid), dim3(256), 0, s, a);    \
-        else hipLaunchKernelGGL((step_persistent_kernel<LL, false>), dim3(grid), dim3(256), 0, s, a);           \
-        break;
-    switch (lanes) {
-        AVBD_PERSIST_CASE(4)
-        AVBD_PERSIST_CASE(8)
-        AVBD_PERSIST_CASE(16)
-        AVBD_PERSIST_CASE(32)
-        default: break;
-    }
-#undef AVBD_PERSIST_CASE
-}
-
-}  // namespace avbd

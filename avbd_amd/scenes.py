@@ -128,6 +128,7 @@ class TetMesh:
     v0: Optional[np.ndarray] = None
     fixed: Optional[np.ndarray] = None   # uint8 [nv]
     triangles: Optional[np.ndarray] = None
+    ext_accel: Optional[np.ndarray] = None   # [nv, 3] external acceleration added to gravity (N/kg)
 
 
 @dataclass
@@ -371,6 +372,20 @@ def hail_on_block(n: int = 24, edge: float = 1.0, grid: int = 6, radius: float =
                                   ground=False, v0=(0.0, -speed, 0.0)))
     return Scene("hail_on_block", meshes=[block], spheres=spheres, dt=dt, iters=iters, rho=0.95, steps=steps,
                  contact_margin=margin)
+
+
+def gaussian_force_field(n_vertices: int, sigma: float = 10.0, seed: int = 0) -> np.ndarray:
+    """Seeded Gaussian per-vertex external acceleration field: every component of
+    every vertex drawn independently from N(0, sigma^2) (N/kg), by Box-Muller
+    over SplitMix64 uniforms in vertex-major, component order."""
+    rng = SplitMix64(seed)
+    out = np.empty((n_vertices, 3))
+    for i in range(n_vertices):
+        for k in range(3):
+            u1 = max(rng.uniform(), 1e-300)
+            u2 = rng.uniform()
+            out[i, k] = sigma * np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    return out
 
 
 def algorithmic_bytes_per_step(n_v: int, n_tet: int, n_g: int, iters: int) -> int:

@@ -55,8 +55,8 @@ class SimulationContext:
     def __init__(self, dt: float, g_accel: float = 9.81, solver_iters: int = 20,
                  iter_acceleration: float = 0.95, particle_capacity: int = 0, energy_capacity: int = 0,
                  device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0, use_graph: bool = True,
-                 serial_broadphase: bool = False, persistent: bool = False, separate_chebyshev: bool = False,
-                 gather_batch: int = 0):
+                 serial_broadphase: bool = False, separate_chebyshev: bool = False, gather_batch: int = 0,
+                 fp64_state: bool = False, host_coloring: bool = False):
         desc = _lib.SceneDesc()
         desc.particle_capacity = particle_capacity
         desc.energy_capacity = energy_capacity
@@ -65,8 +65,9 @@ class SimulationContext:
         desc.solver_iters = solver_iters
         desc.iter_acceleration = iter_acceleration
         desc.flags = ((0 if use_graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_SERIAL_BROADPHASE if serial_broadphase else 0)
-                      | (_lib.FLAG_PERSISTENT if persistent else 0)
-                      | (_lib.FLAG_SEPARATE_CHEBYSHEV if separate_chebyshev else 0))
+                      | (_lib.FLAG_SEPARATE_CHEBYSHEV if separate_chebyshev else 0)
+                      | (_lib.FLAG_FP64_STATE if fp64_state else 0)
+                      | (_lib.FLAG_HOST_COLORING if host_coloring else 0))
         desc.device = device
         desc.lanes_per_vertex = lanes_per_vertex
         desc.block_size = block_size
@@ -217,6 +218,11 @@ class SimulationContext:
         self._chk(lib.avbd_set_particle_state(self.handle, first, n, _dptr(x), _dptr(x_prev), _dptr(v),
                                               _dptr(v_prev)))
 
+    def set_external_accel(self, first: int, a: np.ndarray) -> None:
+        """Per-particle external acceleration (N/kg) added to gravity."""
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3)
+        self._chk(lib.avbd_set_external_accel(self.handle, first, a.shape[0], _dptr(a)))
+
     def set_fixed(self, first: int, fixed: np.ndarray) -> None:
         fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
         self._chk(lib.avbd_set_fixed(self.handle, first, fixed.shape[0], _bptr(fixed)))
@@ -361,12 +367,15 @@ class SimulationContext:
         self._chk(lib.avbd_mlp_gemm_test(self.handle, M, N, K, int(ta), int(tb), _fptr(A), _fptr(B), _fptr(Cm)))
         return Cm
 
-    def residual_history(self) -> np.ndarray:
-        """Relative residuals at the checks of the last step (residual-targeted mode)."""
-        buf = np.empty(64)
+    def residual_history(self, worst_scene: bool = False) -> np.ndarray:
+        """Relative residuals at the checks of the last step (residual-targeted
+        mode): over all scenes, or (worst_scene) the largest per-scene one."""
+        cap = 1024
+        buf = np.empty(cap)
         n = C.c_uint32()
-        self._chk(lib.avbd_get_residual_history(self.handle, _dptr(buf), 64, C.byref(n)))
-        return buf[:min(64, n.value)].copy()
+        fn = lib.avbd_get_scene_residual_history if worst_scene else lib.avbd_get_residual_history
+        self._chk(fn(self.handle, _dptr(buf), cap, C.byref(n)))
+        return buf[:min(cap, n.value)].copy()
 
     def set_scenes(self, offsets: Sequence[int]) -> None:
         off = np.ascontiguousarray(offsets, dtype=np.uint32)
@@ -527,15 +536,16 @@ class Simulation:
     option, config/SimulationConfig.hpp:50-53)."""
 
     def __init__(self, scene: Scene, device: int = 0, lanes_per_vertex: int = 0, block_size: int = 0,
-                 use_graph: bool = True, serial_broadphase: bool = False, persistent: bool = False,
-                 residual_target: Optional[tuple] = None, separate_chebyshev: bool = False, gather_batch: int = 0):
+                 use_graph: bool = True, serial_broadphase: bool = False,
+                 residual_target: Optional[tuple] = None, separate_chebyshev: bool = False, gather_batch: int = 0,
+                 fp64_state: bool = False, host_coloring: bool = False):
         self.scene = scene
         self.ctx = SimulationContext(dt=scene.dt, g_accel=scene.g, solver_iters=scene.iters,
                                      iter_acceleration=scene.rho, device=device,
                                      lanes_per_vertex=lanes_per_vertex, block_size=block_size,
                                      use_graph=use_graph, serial_broadphase=serial_broadphase,
-                                     persistent=persistent, separate_chebyshev=separate_chebyshev,
-                                     gather_batch=gather_batch)
+                                     separate_chebyshev=separate_chebyshev, gather_batch=gather_batch,
+                                     fp64_state=fp64_state, host_coloring=host_coloring)
         self.solver = VBDSolver(self.ctx, scene.iters, scene.rho)
         if residual_target is not None:
             self.ctx.set_residual_target(*residual_target)
@@ -644,6 +654,8 @@ class Simulation:
             x0 = m.rest if m.x0 is None else m.x0
             v0 = np.zeros_like(x0) if m.v0 is None else m.v0
             self.ctx.set_particle_state(first, x=x0, x_prev=x0, v=v0, v_prev=v0)
+        if getattr(m, "ext_accel", None) is not None:
+            self.ctx.set_external_accel(first, m.ext_accel)
         return first
 
     def _timeStep(self) -> None:

@@ -27,10 +27,13 @@ DEFAULT_PATH = Path(__file__).resolve().parent / "data" / "warm_start_mlp.npz"
 
 
 def training_scenes(n_scenes: int, seed: int, n_min: int = 8, n_max: int = 14, log_e: Tuple[float, float] = (5.0, 7.0),
-                    nu: Tuple[float, float] = (0.3, 0.48), height: Tuple[float, float] = (0.1, 0.6)) -> List[scenes.Scene]:
+                    nu: Tuple[float, float] = (0.3, 0.48), height: Tuple[float, float] = (0.1, 0.6),
+                    field_sigma: float = 0.0, field_fraction: float = 0.0) -> List[scenes.Scene]:
     """Dropped Kuhn cubes in the style of the 32^3 workload: random size
     (n_min..n_max cells per axis), Young's modulus (log-uniform over log_e),
-    Poisson ratio, rotation and drop height (uniform over the ranges)."""
+    Poisson ratio, rotation and drop height (uniform over the ranges); a
+    fraction field_fraction of them also carries a seeded Gaussian per-vertex
+    external acceleration field of standard deviation field_sigma (N/kg)."""
     rng = SplitMix64(seed)
     out = []
     for i in range(n_scenes):
@@ -38,7 +41,12 @@ def training_scenes(n_scenes: int, seed: int, n_min: int = 8, n_max: int = 14, l
         E = float(10 ** (log_e[0] + (log_e[1] - log_e[0]) * rng.uniform()))
         nu_i = nu[0] + (nu[1] - nu[0]) * rng.uniform()
         h = height[0] + (height[1] - height[0]) * rng.uniform()
-        out.append(scenes.dropped_cube(n=n, seed=1000 + 7 * seed + i, height=h, E=E, nu=nu_i))
+        with_field = field_sigma > 0 and rng.uniform() < field_fraction
+        sc = scenes.dropped_cube(n=n, seed=1000 + 7 * seed + i, height=h, E=E, nu=nu_i)
+        if with_field:
+            m = sc.meshes[0]
+            m.ext_accel = scenes.gaussian_force_field(m.rest.shape[0], field_sigma, seed=5000 + 7 * seed + i)
+        out.append(sc)
     return out
 
 
@@ -108,11 +116,14 @@ def main(argv=None):
     ap.add_argument("--log-e", type=float, nargs=2, default=(5.7, 6.3))
     ap.add_argument("--nu", type=float, nargs=2, default=(0.35, 0.45))
     ap.add_argument("--height", type=float, nargs=2, default=(0.2, 0.6))
+    ap.add_argument("--field-sigma", type=float, default=10.0,
+                    help="standard deviation (N/kg) of the Gaussian per-vertex force fields")
+    ap.add_argument("--field-fraction", type=float, default=0.5, help="fraction of scenes that carry a field")
     args = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     t0 = time.perf_counter()
     sc = training_scenes(args.scenes, args.seed, args.n_min, args.n_max, tuple(args.log_e), tuple(args.nu),
-                         tuple(args.height))
+                         tuple(args.height), args.field_sigma, args.field_fraction)
     X, T = generate(sc, args.steps, args.tol, args.max_iters, keep=args.keep, log=log)
     log(f"[warm-start] {X.shape[0]} rows in {time.perf_counter() - t0:.1f}s")
     from .sim import SimulationContext
@@ -124,7 +135,8 @@ def main(argv=None):
     log(f"[warm-start] trained {args.epochs} epochs in {time.perf_counter() - t1:.1f}s, final loss {loss:.4f}")
     save(Path(args.out), params, rows=X.shape[0], loss=loss, tol=args.tol, scenes=args.scenes, steps=args.steps,
          max_iters=args.max_iters, epochs=args.epochs, seed=args.seed, n_range=(args.n_min, args.n_max),
-         log_e=tuple(args.log_e), nu=tuple(args.nu), height=tuple(args.height), keep=args.keep)
+         log_e=tuple(args.log_e), nu=tuple(args.nu), height=tuple(args.height), keep=args.keep,
+         field_sigma=args.field_sigma, field_fraction=args.field_fraction)
     log(f"[warm-start] wrote {args.out}")
     return 0
 

@@ -61,13 +61,24 @@ typedef enum {
 /* ---- context creation flags ---- */
 #define AVBD_FLAG_NO_GRAPH 1u    /* launch kernels eagerly instead of replaying a hipGraph */
 #define AVBD_FLAG_NO_PERSISTENT 4u  /* one kernel launch per solver phase (the default) */
-#define AVBD_FLAG_PERSISTENT 8u     /* one persistent launch per step with grid barriers between
-                                       phases.  Opt-in: every barrier needs agent-scope release /
-                                       acquire fences, which write back and invalidate the L2 of
-                                       every XCD, so on MI355X it is slower than one launch per
-                                       phase replayed from a hipGraph */
+#define AVBD_FLAG_PERSISTENT 8u     /* rejected by avbd_create (AVBD_E_ARG): the persistent step
+                                       kernel was removed.  Every grid barrier needed agent-scope
+                                       release / acquire fences, which write back and invalidate
+                                       the L2 of every XCD, so on MI355X one launch per phase
+                                       replayed from a hipGraph was 25x faster */
 #define AVBD_FLAG_SERIAL_BROADPHASE 2u  /* run the triangle broad phase on the solve stream
                                            instead of concurrently on a second stream */
+#define AVBD_FLAG_FP64_STATE 32u /* keep positions, the inertial target and the Chebyshev history in
+                                    FP64 on the device and evaluate the vertex-block solve, the
+                                    ground terms and the force residual in FP64 (material and contact
+                                    records stay FP32).  An FP32 position resolves ~6e-8 m at 0.5 m,
+                                    which bounds the relative force residual an FP32 state can reach
+                                    near 1e-3..1e-2 on fine stiff meshes; this mode reaches 1e-4 and
+                                    below.  Positional particles only (no oriented particles), no
+                                    dynamic contacts (AVBD_E_ARG) */
+#define AVBD_FLAG_HOST_COLORING 64u /* run the DSATUR colouring of avbd_build_graph on the host
+                                       instead of the device (same colours; without a device the
+                                       host colours anyway) */
 #define AVBD_FLAG_SEPARATE_CHEBYSHEV 16u /* run the Chebyshev / dual update of every iteration as
                                             its own launch.  By default, in scenes without oriented
                                             particles, it is fused into the iteration's last colour
@@ -241,6 +252,14 @@ int avbd_add_tet_mesh(avbd_ctx* ctx, uint32_t nv, const double* rest_x, uint32_t
 int avbd_set_particle_state(avbd_ctx* ctx, uint32_t first, uint32_t n, const double* x, const double* x_prev,
                             const double* v, const double* v_prev);
 int avbd_set_fixed(avbd_ctx* ctx, uint32_t first, uint32_t n, const uint8_t* fixed);
+/* Per-particle external acceleration field (N/kg, 3n doubles) added to gravity
+ * for particles [first, first+n) (new; the reference's only external
+ * acceleration is gravity, simulation/VBDSolver.hpp:51).  It enters the
+ * inertial target and the adaptive initial guess of _particleInertialUpdate
+ * (simulation/VBDSolver.hpp:208-248) as gravity + a.  A field set before
+ * avbd_build_graph can be changed afterwards; a scene built without one
+ * keeps none (AVBD_E_ARG). */
+int avbd_set_external_accel(avbd_ctx* ctx, uint32_t first, uint32_t n, const double* a);
 /* Add mass dm and principal inertia dI (3, NULL = none; oriented particles
  * only) to particle p: a rigid body welded to it, e.g. a sphere at a rod tip
  * (new; the reference builds such bodies as separate objects). */
@@ -251,16 +270,21 @@ int avbd_add_mass(avbd_ctx* ctx, uint32_t p, double dm, const double* dI);
 int avbd_set_rotation_state(avbd_ctx* ctx, uint32_t first, uint32_t n, const double* q, const double* w);
 
 /* Residual-targeted iteration (new; the reference runs a fixed solver_iters):
- * every step iterates until the relative force residual
- * ||r||_2 / ||M g||_2 (free particles; r = m/dt^2 (x - y) + dE/dx at the
- * current iterate) is <= rel_tol, checked every check_every iterations, or
- * until max_iters iterations.  rel_tol = 0 restores the fixed count.  Call
- * before avbd_build_graph.  At most 64 checks per step. */
+ * every step iterates until, in every scene s (avbd_set_scenes; one scene by
+ * default), the relative force residual ||r_s||_2 / ||M_s g||_2 (free
+ * particles of the scene; r = m/dt^2 (x - y) + dE/dx at the current iterate)
+ * is <= rel_tol, checked every check_every iterations, or until max_iters
+ * iterations (rounded up to a multiple of check_every).  rel_tol = 0
+ * restores the fixed count.  Call before avbd_build_graph. */
 int avbd_set_residual_target(avbd_ctx* ctx, double rel_tol, uint32_t max_iters, uint32_t check_every);
-/* Relative residuals measured at the checks of the last step (residual-
- * targeted mode): rel[k] after iteration (k+1)*check_every; *count = checks
- * that ran (at most cap are written). */
+/* Relative residual ||r||_2 / ||M g||_2 over all scenes at each check of the
+ * last step (residual-targeted mode): rel[k] after iteration
+ * (k+1)*check_every; *count = checks that ran (the first 1024 are kept; at
+ * most cap are written). */
 int avbd_get_residual_history(avbd_ctx* ctx, double* rel, uint32_t cap, uint32_t* count);
+/* The same checks' largest per-scene relative residual (the quantity the
+ * stopping rule compares with rel_tol). */
+int avbd_get_scene_residual_history(avbd_ctx* ctx, double* worst, uint32_t cap, uint32_t* count);
 
 /* Scene partition for batching: scene s owns particles
  * [particle_offsets[s], particle_offsets[s+1]); each scene is coloured on its
@@ -274,11 +298,16 @@ int avbd_build_graph(avbd_ctx* ctx);
 /* Time stepping ------------------------------------------------------------ */
 
 /* n_steps calls of VBDSolver::solve_Parallel (asynchronous on the ctx stream;
- * in residual-targeted mode with step graphs the call returns once the last
- * step's final check intervals are queued: it waits for the check two
- * intervals back to decide whether to queue more, so it runs at most two
- * check intervals ahead of the device). */
+ * in residual-targeted mode the call returns once the last step's final
+ * check intervals are queued: it waits for the check two intervals back to
+ * decide whether to queue more, so it runs at most two check intervals ahead
+ * of the device).  Fails with AVBD_E_NONFINITE once a synchronising call has
+ * reported a non-finite state, until the state is set again. */
 int avbd_step(avbd_ctx* ctx, uint32_t n_steps);
+/* Waits for the ctx stream.  Every step checks its new positions and
+ * velocities on the device; AVBD_E_NONFINITE when one was NaN or Inf (the
+ * downloads and avbd_get_stats report it too, after filling their outputs).
+ * avbd_set_particle_state and avbd_load_checkpoint clear it. */
 int avbd_synchronize(avbd_ctx* ctx);
 
 /* Readback (synchronising) -------------------------------------------------- */

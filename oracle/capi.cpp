@@ -417,6 +417,36 @@ void oracle_add_rground(void* h, uint32_t p, uint32_t shape, double radius, doub
     s->rground.push_back(c);
 }
 
+// Per-particle external acceleration added to gravity (3n doubles; NULL clears it).
+void oracle_set_external_accel(void* h, const double* a) {
+    auto* s = static_cast<Simulation*>(h);
+    s->f_ext.clear();
+    if (!a) return;
+    s->f_ext.resize(s->x.size());
+    for (size_t p = 0; p < s->x.size(); p++) s->f_ext[p] = v_in(a + 3 * p);
+}
+
+// RigidBodyGroundCollisionEnergyPool::addEnergy for a shape without the
+// lowest-point rule (box, capsule): a fixed contact point in the body frame
+// (energy/collision/RigidBodyGroundCollisionEnergySolver.hpp:111-120,161-183).
+void oracle_add_rground_point(void* h, uint32_t p, const double* cp_local3, double k_start, double mu_s,
+                              double mu_k) {
+    auto* s = static_cast<Simulation*>(h);
+    RigidGround c{};
+    c.p = p;
+    c.shape = SHAPE_OTHER;
+    c.radius = 0;
+    c.cp_rb_local = v_in(cp_local3);
+    c.cp_ground = V3();
+    c.n = V3(0, 1, 0);
+    c.t = V3(1, 0, 0);
+    c.b = V3(0, 0, 1);
+    c.k_start = c.k_n = c.k_t = c.k_b = k_start;
+    c.mu_s = mu_s;
+    c.mu_k = mu_k;
+    s->rground.push_back(c);
+}
+
 uint32_t oracle_num_cosserat(void* h) { return (uint32_t)static_cast<Simulation*>(h)->cos.size(); }
 uint32_t oracle_num_rground(void* h) { return (uint32_t)static_cast<Simulation*>(h)->rground.size(); }
 uint32_t oracle_num_joints(void* h) { return (uint32_t)static_cast<Simulation*>(h)->joints.size(); }

@@ -419,11 +419,13 @@ def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
     batch = getattr(scene, "batch", True)
     meshes = scene.meshes[mesh_index:mesh_index + 1] if (batch and scene.meshes) else scene.meshes
     dynamic = getattr(scene, "contact_margin", None) is not None
-    fixed_rows, x0_rows, v0_rows, spins = [], [], [], []
+    fixed_rows, x0_rows, v0_rows, spins, field_rows = [], [], [], [], []
     for kind in order:
         if kind == "meshes":
             for m in meshes:
                 first = o.add_tet_mesh(m.rest, m.tets, m.E, m.nu, m.density, m.kd, m.ground, m.mu_s, m.mu_k)
+                if getattr(m, "ext_accel", None) is not None:
+                    field_rows.append((first, m.ext_accel))
                 if m.fixed is not None:
                     fixed_rows.append((first, m.fixed))
                 if m.x0 is not None:
@@ -473,6 +475,11 @@ def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
         for first, arr in v0_rows:
             v[first:first + len(arr)] = arr
         o.set_velocities(v, v)
+    if field_rows:
+        a = np.zeros((o.n, 3))
+        for first, arr in field_rows:
+            a[first:first + len(arr)] = arr
+        o.set_external_accel(a)
     if spins:
         cur = o.rotations()
         q, w = cur["q"].copy(), cur["w"].copy()
@@ -556,6 +563,8 @@ for _k, (_r, _a) in {
     "oracle_get_mass": (None, [C.c_void_p, _D, _U8]),
     "oracle_set_rotation_state": (None, [C.c_void_p, _D, _D]),
     "oracle_add_mass": (None, [C.c_void_p, C.c_uint32, C.c_double, _D]),
+    "oracle_add_rground_point": (None, [C.c_void_p, C.c_uint32, _D, C.c_double, C.c_double, C.c_double]),
+    "oracle_set_external_accel": (None, [C.c_void_p, _D]),
     "oracle_add_rground": (None, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                   C.c_double]),
     "oracle_get_rground": (None, [C.c_void_p, _D]),
@@ -595,6 +604,19 @@ def _add_rod(self, elements, length, curvature=(0, 0, 0), E=1e7, nu=0.3, density
 def _add_sphere(self, radius, density, pos, rot=(1, 0, 0, 0), fixed=False, with_ground=True):
     return int(lib.oracle_add_sphere(self.h, radius, density, dp(_a(pos, 3)), dp(_a(rot, 4)), int(fixed),
                                      int(with_ground)))
+
+
+def _add_rground_point(self, p, cp_local, k_start, mu_s=0.4, mu_k=0.2):
+    """A rigid-ground contact at a fixed body-frame point (box / capsule shapes)."""
+    lib.oracle_add_rground_point(self.h, int(p), dp(_a(cp_local, 3)), float(k_start), float(mu_s), float(mu_k))
+
+
+def _set_external_accel(self, a):
+    """Per-particle external acceleration added to gravity ((n, 3); None clears it)."""
+    if a is None:
+        lib.oracle_set_external_accel(self.h, None)
+        return
+    lib.oracle_set_external_accel(self.h, dp(_a(np.asarray(a, dtype=np.float64).reshape(-1), 3 * self.n)))
 
 
 def _add_oriented(self, pos, rot, mass, inertia):
@@ -644,6 +666,8 @@ def _joints(self):
 OracleSim.add_rod = _add_rod
 OracleSim.add_sphere = _add_sphere
 OracleSim.add_oriented = _add_oriented
+OracleSim.add_rground_point = _add_rground_point
+OracleSim.set_external_accel = _set_external_accel
 OracleSim.add_cosserat = _add_cosserat
 OracleSim.add_joint = _add_joint
 def _set_rotation_state(self, q, w):

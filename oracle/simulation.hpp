@@ -14,6 +14,9 @@ public:
     Params params;
     // ParticlePool, common/ParticlePool.hpp:78-90
     std::vector<V3> x, x_buf, y, x_prev, x_last, x_lastlast, v, v_prev;
+    // per-particle external acceleration added to gravity (empty = none; an
+    // extension of the reference, whose only external acceleration is gravity)
+    std::vector<V3> f_ext;
     std::vector<Real> mass;
     std::vector<uint8_t> fixed;
     RotState rs;   // RotationPool through rotation_idx

@@ -467,11 +467,16 @@ void Simulation::phase_after_timestep_rigid(uint32_t lo, uint32_t hi) {
 }
 
 // _particleInertialUpdate + _particleRotationInertialUpdate, VBDSolver.hpp:208-283
+// _particleInertialUpdate (simulation/VBDSolver.hpp:208-248).  With a
+// per-particle field f_ext (not in the reference) the external acceleration
+// of particle p is gravity + f_ext[p], in the inertial target and in the
+// adaptive initial guess alike.
 void Simulation::phase_inertial(uint32_t lo, uint32_t hi) {
-    V3 a_ext(0, -params.g, 0);
+    const V3 gravity(0, -params.g, 0);
     Real dt = params.dt;
     for (uint32_t p = lo; p < hi; p++) {
         if (fixed[p]) continue;
+        const V3 a_ext = f_ext.empty() ? gravity : gravity + f_ext[p];
         V3 a = (v[p] - v_prev[p]) / dt;
         V3 a_tilde_vec;
         Real an = norm(a_ext);

@@ -1,5 +1,5 @@
 """Edge cases of the step on the device: a scene with no free vertex, and
-residual targets met at the first check or never met (the part-graph
+residual targets met at the first check or never met (the interval-graph
 stepping's early stop and its full budget), each against the separate
 Chebyshev launch and the eager path."""
 import numpy as np
@@ -32,11 +32,11 @@ def test_all_fixed_scene_does_not_move():
 @pytest.mark.parametrize("tol,expect", [(10.0, "first"), (1e-9, "budget")])
 def test_residual_target_first_check_and_budget(tol, expect):
     """tol = 10: every step stops at its first check; tol = 1e-9: no check
-    converges and every step uses the whole budget.  The part-graph replay,
+    converges and every step uses the whole budget.  The interval-graph replay,
     the eager launches and the separate Chebyshev launch agree bit for bit."""
     banner(f"residual target {tol:g} ({expect})")
     from avbd_amd.sim import Simulation
-    K, budget = 4, 27
+    K, budget = 4, 28
     sc = scenes.dropped_cube(n=5, seed=11, height=0.03)
     runs = {}
     for name, kw in (("graph", {}), ("eager", dict(use_graph=False)), ("separate", dict(separate_chebyshev=True))):

@@ -171,12 +171,13 @@ def test_graph_replay_matches_eager(oracle):
         a.ctx.stats()   # eager work between replays
 
 
-@pytest.mark.parametrize("mode", ["fixed_odd", "fixed_even_eager", "residual_odd_checks"])
+@pytest.mark.parametrize("mode", ["fixed_odd", "fixed_even_eager", "residual_checks"])
 def test_fused_chebyshev_matches_separate_launch(oracle, mode):
     """The Chebyshev / dual update fused into each iteration's last colour
     phase (two position buffers) gives the same bits as its own launch after
     the phase, including after an odd number of iterations and when a
-    residual check stops the step early."""
+    residual check stops the step early (residual mode fuses with an even
+    check interval only)."""
     banner(f"fused Chebyshev vs separate launch ({mode})")
     from avbd_amd.sim import Simulation
     sc = scenes.dropped_cube(n=6, seed=3, height=0.02)
@@ -187,7 +188,7 @@ def test_fused_chebyshev_matches_separate_launch(oracle, mode):
         sc.iters = 10
         kw["use_graph"] = False
     else:
-        kw["residual_target"] = (5e-2, 99, 3)
+        kw["residual_target"] = (5e-2, 100, 4)
     a = Simulation(sc, separate_chebyshev=False, **kw)
     b = Simulation(sc, separate_chebyshev=True, **kw)
     a.setup()
@@ -199,7 +200,7 @@ def test_fused_chebyshev_matches_separate_launch(oracle, mode):
         assert np.array_equal(a.positions(), b.positions()), f"{mode}: round {k} differs"
         va, vb = a.ctx.state()[1], b.ctx.state()[1]
         assert np.array_equal(va, vb)
-        if mode == "residual_odd_checks":
+        if mode == "residual_checks":
             sa, sb = a.ctx.stats(), b.ctx.stats()
             assert sa.iterations_total == sb.iterations_total
             banner(f"  round {k}: last step {sa.last_step_iterations} iterations, total {sa.iterations_total}")
@@ -273,47 +274,25 @@ def test_full_size_dropped_cube_one_step(oracle):
     assert x[:, 1].min() == pytest.approx(xo[:, 1].min(), abs=5e-3)
 
 
-@pytest.mark.parametrize("damping", [False, True])
-def test_persistent_step_kernel_matches_phase_launches(damping):
-    """One persistent launch per step (grid barriers between phases) runs the
-    same per-vertex arithmetic as one launch per phase: identical bits."""
-    banner(f"persistent vs per-phase launches, damping={damping}")
-    from avbd_amd.sim import Simulation
-    sc = scenes.dropped_cube(n=10, seed=3)
-    if damping:
-        sc.meshes[0].kd = 0.3
-    a = Simulation(sc, lanes_per_vertex=8, block_size=256, persistent=True)
-    b = Simulation(sc, lanes_per_vertex=8, block_size=256, persistent=False)
-    a.setup()
-    b.setup()
-    assert a.ctx.persistent_grid() > 0 and b.ctx.persistent_grid() == 0
-    for _ in range(3):
-        a.update(10)
-        b.update(10)
-        xa, va = a.ctx.state()
-        xb, vb = b.ctx.state()
-        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
-
-
 def test_residual_target_mode(oracle):
     """Residual-targeted iteration on the device: the step stops at the first
     check whose relative residual is <= tol; the state it leaves satisfies the
     tolerance (FP64 oracle evaluation of the same positions allows 10 % slack
-    for FP32 rounding), the persistent kernel and the per-phase launches take
+    for FP32 rounding), the replayed step graphs and the eager launches take
     the same decisions and produce the same bits, and one step equals the
     oracle's fixed-count step with the iteration count the device used."""
     banner("residual-targeted iteration")
     from avbd_amd.sim import Simulation
     sc = scenes.hanging_cube(n=6)
     tol, K = 3e-3, 5
-    a = Simulation(sc, lanes_per_vertex=8, block_size=256, persistent=True, residual_target=(tol, 300, K))
-    b = Simulation(sc, lanes_per_vertex=8, block_size=256, persistent=False, residual_target=(tol, 300, K))
+    a = Simulation(sc, lanes_per_vertex=8, block_size=256, residual_target=(tol, 300, K))
+    b = Simulation(sc, lanes_per_vertex=8, block_size=256, use_graph=False, residual_target=(tol, 300, K))
     a.setup()
     b.setup()
     a.update(1)
     b.update(1)
     sa, sb = a.ctx.stats(), b.ctx.stats()
-    banner(f"  iterations persistent {sa.last_step_iterations} per-phase {sb.last_step_iterations}; "
+    banner(f"  iterations graphs {sa.last_step_iterations} eager {sb.last_step_iterations}; "
            f"residual {sa.residual_rel:.3e}")
     assert sa.last_step_iterations == sb.last_step_iterations
     assert np.array_equal(a.positions(), b.positions())
@@ -334,3 +313,65 @@ def test_residual_target_mode(oracle):
     sa, sb = a.ctx.stats(), b.ctx.stats()
     assert sa.iterations_total == sb.iterations_total and np.array_equal(a.positions(), b.positions())
     assert sa.vertex_updates == sa.iterations_total * int((sc.meshes[0].fixed == 0).sum())
+
+
+@pytest.mark.parametrize("workload", ["dropped_cube_32", "cube_batch_512"])
+def test_device_dsatur_matches_host_and_oracle(oracle, workload):
+    """DSATUR runs on the device (one workgroup per scene): its colours equal
+    the host restatement's for the full 32^3 cube and for all 512 scenes of
+    the batch, and the oracle's for the cube (ColorList::greedyColor2,
+    common/ColorList.hpp:149-241)."""
+    from avbd_amd.sim import Simulation
+    banner(f"device DSATUR vs host: {workload}")
+    sc = scenes.dropped_cube(n=32) if workload == "dropped_cube_32" else scenes.cube_batch(n_scenes=512)
+    kw = dict(lanes_per_vertex=8, block_size=128)
+    dev = Simulation(sc, **kw)
+    host = Simulation(sc, host_coloring=True, **kw)
+    import time
+    t0 = time.perf_counter()
+    dev.setup()
+    t_dev = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    host.setup()
+    t_host = time.perf_counter() - t0
+    cd, kd = dev.ctx.coloring()
+    ch, kh = host.ctx.coloring()
+    banner(f"  {len(cd)} vertices, {kd} colours; set-up {t_dev:.2f} s (device colouring) vs {t_host:.2f} s (host)")
+    assert kd == kh and np.array_equal(cd, ch)
+    if workload == "dropped_cube_32":
+        o = oracle.from_scene(sc)
+        co, ko = o.colors()
+        assert ko == kd and np.array_equal(co, cd)
+    dev.update(2)
+    host.update(2)
+    assert np.array_equal(dev.positions(), host.positions())
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_external_force_field_matches_oracle(oracle, fp64):
+    """A seeded Gaussian per-vertex acceleration field (sigma = 10 N/kg) on a
+    hanging cube: one step within 1e-5 of the bounding box and 30 steps
+    within 1e-3, against the oracle with the same field."""
+    from avbd_amd.sim import Simulation
+    banner(f"external force field vs oracle (fp64={fp64})")
+    sc = scenes.hanging_cube(n=6)
+    m = sc.meshes[0]
+    m.ext_accel = scenes.gaussian_force_field(m.rest.shape[0], sigma=10.0, seed=5)
+    g = Simulation(sc, fp64_state=fp64)
+    g.setup()
+    o = oracle.from_scene(sc)
+    g.update(1)
+    o.step(1)
+    xo = o.positions()
+    e1 = np.abs(g.positions() - xo).max() / bbox(xo)
+    g.update(29)
+    o.step(29)
+    xo = o.positions()
+    e30 = np.abs(g.positions() - xo).max() / bbox(xo)
+    banner(f"  1 step {e1:.2e}, 30 steps {e30:.2e}")
+    assert e1 <= 1e-5 and e30 <= 1e-3
+    # the field moved the cube differently from gravity alone
+    plain = Simulation(scenes.hanging_cube(n=6), fp64_state=fp64)
+    plain.setup()
+    plain.update(30)
+    assert np.abs(plain.positions() - g.positions()).max() > 1e-4

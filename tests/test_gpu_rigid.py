@@ -130,15 +130,14 @@ def test_rods_and_spheres_contact_and_joints(oracle):
     assert lam_g[-4:].sum() == pytest.approx(lam_o[-4:].sum(), rel=2e-2)
 
 
-def test_oriented_persistent_matches_phase_launches():
-    banner("oriented scene: persistent kernel vs per-phase launches (bitwise)")
+def test_oriented_graph_matches_eager_launches():
+    banner("oriented scene: step graph replay vs eager launches (bitwise)")
     from avbd_amd.sim import Simulation
     sc = scenes.rods_and_spheres(n_rods=3, elements=12)
-    a = Simulation(sc, persistent=True)
-    b = Simulation(sc, persistent=False)
+    a = Simulation(sc)
+    b = Simulation(sc, use_graph=False)
     a.setup()
     b.setup()
-    assert a.ctx.persistent_grid() > 0 and b.ctx.persistent_grid() == 0
     for _ in range(3):
         a.update(10)
         b.update(10)
@@ -196,3 +195,67 @@ def test_rod_grid_with_tip_spheres(oracle):
     err = np.abs(g.positions() - xo).max() / bbox(xo)
     banner(f"  30-step pos rel err {err:.3e}")
     assert err <= 2e-3
+
+
+def test_box_corner_indefinite_block_matches_oracle(oracle):
+    """A rigid box balanced on one corner: the 6x6 block of its vertex solve
+    (inertia plus the corner's AL contact) becomes indefinite -- the
+    contact's rotational Hessian at an off-centre point
+    (energy/collision/RigidBodyGroundCollisionEnergySolver.hpp:111-120),
+    scaled by the normal force, outweighs I/dt^2 in the twists that keep the
+    corner in place.  The device solves it by LU with partial pivoting, as
+    the reference's partialPivLu (simulation/VBDSolver.hpp:472), and follows
+    the oracle step by step."""
+    import ctypes as C
+    from avbd_amd import _lib
+    from avbd_amd.sim import SimulationContext
+    banner("box on one corner: indefinite 6x6 block vs oracle")
+    dt, m, inertia, iters = 1 / 60, 20.0, 0.02, 10
+    axis = np.array([1.0, 0.0, 1.0]) / np.sqrt(2.0)
+    ang = 0.35
+    q0 = np.concatenate([[np.cos(ang / 2)], np.sin(ang / 2) * axis])
+    cp = np.array([0.1, -0.5, 0.1])
+    w_, x_, y_, z_ = q0
+    R = np.array([[1 - 2 * (y_ * y_ + z_ * z_), 2 * (x_ * y_ - w_ * z_), 2 * (x_ * z_ + w_ * y_)],
+                  [2 * (x_ * y_ + w_ * z_), 1 - 2 * (x_ * x_ + z_ * z_), 2 * (y_ * z_ - w_ * x_)],
+                  [2 * (x_ * z_ - w_ * y_), 2 * (y_ * z_ + w_ * x_), 1 - 2 * (x_ * x_ + y_ * y_)]])
+    pos = np.array([0.0, -(R @ cp)[1] + 0.002, 0.0])
+    k_start = m / (dt * dt)
+    ctx = SimulationContext(dt=dt, solver_iters=iters, iter_acceleration=0.95)
+    p = ctx.add_oriented_particles(pos[None], inertia=np.full((1, 3), inertia), q=q0[None], mass=np.array([m]))
+    r = _lib.RigidGroundRecord()
+    r.particle_index = p
+    r.shape = 3   # AVBD_SHAPE_BOX: a fixed contact point in the body frame
+    for k in range(3):
+        r.cp_rb_local[k] = cp[k]
+    r.k_start = r.k_n = r.k_t = r.k_b = k_start
+    r.mu_s, r.mu_k = 0.5, 0.2
+    ctx.add_energies(_lib.RIGID_BODY_GROUND_COLLISION, (_lib.RigidGroundRecord * 1)(r))
+    ctx.build_graph()
+    o = oracle.OracleSim(dt, 9.81, iters, 0.95)
+    o.add_oriented(pos, q0, m, [inertia] * 3)
+    o.add_rground_point(0, cp, k_start, 0.5, 0.2)
+    o.build_graph()
+    min_eig = np.inf
+    # the first steps, up to touch-down: the tangential AL terms, whose
+    # constraint is measured from the body's centre (cp_ground takes the
+    # particle's x and z, RigidBodyGroundCollisionEnergySolver.hpp:161-170),
+    # already make the block indefinite; after touch-down the restated
+    # dynamics of this shape run away in the FP64 oracle as well
+    for step in range(5):
+        ctx.step(1)
+        o.step(1)
+        xg, xo = ctx.positions(), o.positions()
+        qg, _ = ctx.rotations()
+        qo = o.rotations()["q"]
+        ex, eq = float(np.abs(xg - xo).max()), qerr(qg, qo)
+        # the block at the end of the step, from the oracle's state
+        rec = o.rground()[0]
+        rec12 = [rec[1], rec[2], rec[3], rec[4], rec[5], rec[6], rec[7], 0.5, 0.2, rec[14], 0.0, 2]
+        _, _, H = oracle.rground_single(xo[0], qo[0], rec12, rec[8:11], rec[11:14])
+        H = H + np.diag([m / dt ** 2] * 3 + [inertia / dt ** 2] * 3)
+        ev = float(np.linalg.eigvalsh(0.5 * (H + H.T)).min())
+        min_eig = min(min_eig, ev)
+        banner(f"  step {step}: |dx| {ex:.2e} m, |dq| {eq:.2e}, normal force {rec[4]:.1f} N, min eigenvalue {ev:.3g}")
+        assert ex <= 2e-4 * (step + 1) and eq <= 2e-4 * (step + 1), step
+    assert min_eig < 0.0    # the solves went through indefinite blocks

@@ -360,3 +360,34 @@ def test_residual_target_stops_at_tolerance(oracle):
     assert np.array_equal(h.positions(), first.positions())
     assert h.stats()["residual_rel"] <= tol
     assert fixed_count(used - K).stats()["residual_rel"] > tol
+
+
+def test_external_acceleration_field_free_fall_exact(oracle):
+    """Per-particle external acceleration added to gravity: unconnected
+    particles fall exactly as x_n = x_0 + (g + f) dt^2 n (n + 1) / 2 (the
+    adaptive initial guess follows the total acceleration), and a field of
+    +g holds a particle still."""
+    dt = 1 / 60
+    o = oracle.OracleSim(dt, 9.81, 4, 0.95)
+    x0 = np.array([[0.0, 5.0, 0.0], [1.0, 5.0, 0.0], [2.0, 5.0, 0.0]])
+    o.add_particles(x0, np.ones(3))
+    o.build_graph()
+    f = np.array([[3.0, -2.0, 1.0], [0.0, 9.81, 0.0], [-7.5, 20.0, 4.0]])
+    o.set_external_accel(f)
+    a = f + np.array([0.0, -9.81, 0.0])
+    for n in range(1, 6):
+        o.step(1)
+        want = x0 + a * dt * dt * n * (n + 1) / 2
+        assert np.abs(o.positions() - want).max() < 1e-12
+    assert np.array_equal(o.positions()[1], x0[1])
+
+
+def test_gaussian_force_field_statistics():
+    """The seeded Gaussian field: reproducible, zero mean and the requested
+    standard deviation per component (up to sampling error)."""
+    from avbd_amd import scenes
+    a = scenes.gaussian_force_field(4000, sigma=10.0, seed=3)
+    b = scenes.gaussian_force_field(4000, sigma=10.0, seed=3)
+    assert np.array_equal(a, b)
+    assert np.abs(a.mean(axis=0)).max() < 0.5
+    assert np.abs(a.std(axis=0) - 10.0).max() < 0.4
