@@ -255,6 +255,8 @@ SIGNATURES = {
     "avbd_checkpoint_size": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "avbd_save_checkpoint": (C.c_int, [_P, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "avbd_load_checkpoint": (C.c_int, [_P, C.c_void_p, C.c_uint64]),
+    "avbd_detect_collisions_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double,
+                                                 C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
 }
 
 

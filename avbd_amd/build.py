@@ -16,14 +16,14 @@ LIB = OUT_DIR / "libavbd_amd.so"
 INCLUDE = PKG.parent / "include"
 
 SOURCES = ["api.cpp", "host_graph.cpp", "contacts_host.cpp", "mlp_host.cpp", "vbd_kernels.hip", "lbvh_kernels.hip",
-           "contact_kernels.hip", "mlp_kernels.hip", "vbd_fp64.hip",
+           "contact_kernels.hip", "mlp_kernels.hip", "lbvh_static.hip", "vbd_fp64.hip",
            "graph_kernels.hip"]
 # FP64 Morton codes must round exactly like the reference's CPU code: no FMA
 # contraction in the LBVH kernels.  The solver kernels contract only within
 # one expression (-ffp-contract=on), so a device function rounds the same way
 # wherever it is inlined: the fused and separate Chebyshev phases then
 # produce identical bits.
-EXTRA = {"lbvh_kernels.hip": ["-ffp-contract=off"], "contact_kernels.hip": ["-ffp-contract=off"],
+EXTRA = {"lbvh_kernels.hip": ["-ffp-contract=off"], "lbvh_static.hip": ["-ffp-contract=off"], "contact_kernels.hip": ["-ffp-contract=off"],
          "vbd_kernels.hip": ["-ffp-contract=on"], "vbd_fp64.hip": ["-ffp-contract=on"]}
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 VALU ops

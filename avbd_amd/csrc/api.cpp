@@ -981,6 +981,10 @@ void tune_fusion(avbd_ctx* c) {
 
 }  // namespace
 
+namespace avbd {
+void set_context_free_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace avbd
+
 extern "C" {
 
 int avbd_create(const avbd_scene_desc* desc, avbd_ctx** out) {

@@ -450,6 +450,10 @@ void launch_dsatur(hipStream_t s, uint32_t n_scenes, const uint32_t* scene_off, 
 // lists; plan.num_colors must cover every colour.
 void relayout_plan(ColorPlan& plan, const std::vector<uint8_t>& conflicted);
 
+// Message of the last failure of an entry point that takes no context
+// (returned by avbd_last_error(NULL)).
+void set_context_free_error(const std::string& msg);
+
 // DSATUR on one CSR graph (ColorList::greedyColor2, common/ColorList.hpp:149-241).
 uint32_t dsatur_csr(uint32_t n, const uint32_t* p_offsets, const uint32_t* p_neighbors, uint32_t* color);
 

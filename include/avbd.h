@@ -479,6 +479,30 @@ int avbd_time_solve_launches(avbd_ctx* ctx, uint32_t reps, double* avg_ms, uint3
  * HBM byte counters can be scaled against a kernel of known traffic. */
 int avbd_calibrate_copy(avbd_ctx* ctx, uint64_t bytes, uint32_t iters, double* avg_ms);
 
+/* Allocation-free broad phase.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc).  The same LBVH build and
+ * self traversal as avbd_detect_collisions + avbd_download_candidates, over
+ * caller-owned device buffers and without a context:
+ *   x4, v4    device arrays of float4 per particle (xyz + any w), the
+ *             positions and velocities the speculative bounds are built from;
+ *   tris      device array of 3 particle indices per triangle (n_tris);
+ *   pairs     device output of up to pair_cap candidate leaf pairs (a < b,
+ *             sorted-order leaf indices, 2 uint32 each, order unspecified);
+ *   counters  device output of 2 uint32: [0] pairs found (> pair_cap means
+ *             truncated), [1] status (bit 0: traversal stack overflow);
+ *   stream    hipStream_t to launch on (NULL = the null stream).
+ * Asynchronous: outputs are valid once the stream has reached this point.
+ * Every intermediate array lives in one static device array of the library
+ * (about 6.7 MiB per GPU, held from the first call on a device until the
+ * process ends), so a call allocates nothing and never synchronises.  At most
+ * AVBD_NOALLOC_MAX_TRIANGLES triangles (AVBD_E_CAPACITY beyond).  Calls on one
+ * GPU share that array: calls that overlap in time on one GPU race.  Errors
+ * are reported through avbd_last_error(NULL). */
+#define AVBD_NOALLOC_MAX_TRIANGLES 16384u
+int avbd_detect_collisions_noalloc(const float* x4, const float* v4, const uint32_t* tris, uint32_t n_tris,
+                                   double dt, uint32_t* pairs, uint32_t pair_cap, uint32_t* counters, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
