@@ -99,6 +99,12 @@ struct avbd_ctx {
     double* res_mg2_scene = nullptr;
     double* res_acc_scene = nullptr;
     float* omega_tab = nullptr;    // Chebyshev weight per iteration of a step
+    // a batch freezes each scene once it meets the target (per scene: flag,
+    // iterations it ran, its work items; per particle: its scene)
+    uint32_t* res_frozen = nullptr;
+    uint32_t* res_iters = nullptr;
+    uint32_t* res_items = nullptr;
+    uint32_t* res_scene_of = nullptr;
 
     // learned warm start (mlp_host.cpp): the network, its in-step buffers,
     // and the recording of training data
@@ -479,6 +485,20 @@ void join_broadphase(avbd_ctx* c) {
     if (forks_broadphase(c)) hip_check(hipStreamWaitEvent(c->stream, c->ev_join, 0), "hipStreamWaitEvent");
 }
 
+// Converged-scene flags of a residual-targeted batch (none for one scene):
+// a scene that meets the target stops iterating while the others go on, so
+// each scene of a batch ends its step as it would on its own.
+SceneFreeze scene_freeze(const avbd_ctx* c) {
+    SceneFreeze fz;
+    if (c->res_frozen) {
+        fz.frozen = c->res_frozen;
+        fz.iters = c->res_iters;
+        fz.items = c->res_items;
+        fz.scene_of = c->res_scene_of;
+    }
+    return fz;
+}
+
 void enqueue_head(avbd_ctx* c, bool with_events) {
     const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
     hipStream_t s = c->stream;
@@ -518,12 +538,13 @@ void enqueue_head(avbd_ctx* c, bool with_events) {
         if (c->warm) {
             c->mlp->forward((int)c->rows_pad, io.x16, false);
             launch_warm_apply(s, c->d, io, dt * dt * g);
-            launch_warm_ip(s, c->d, io, c->damping, dt);
+            if (c->fp64) launch_warm_ip_f64(s, c->f, io, c->damping, c->desc.dt);
+            else launch_warm_ip(s, c->d, io, c->damping, dt);
             launch_warm_choose(s, c->d, io);
             if (c->fp64) launch_warm_place_f64(s, c->f, io.pred, io.scene_of, io.step_of);
         }
     }
-    if (c->rc) launch_residual_reset(s, c->rc);
+    if (c->rc) launch_residual_reset(s, c->rc, scene_freeze(c), c->n_res_scenes);
     hip_check(hipGetLastError(), "kernel launch");
 }
 
@@ -540,12 +561,18 @@ void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool
     const bool fuse = c->fuse_cheb && !c->dynamic;
     size_t ev = (size_t)2 * i0 * c->plan.num_colors;
     const uint32_t last = c->plan.num_colors - 1;
+    // the step's state with the converged-scene flags of a batch
+    const SceneFreeze fz = scene_freeze(c);
+    DevState dz = c->d;
+    F64State f0 = c->f;
+    dz.frozen = f0.frozen = fz.frozen;
+    dz.scene_of = f0.scene_of = fz.scene_of;
     for (uint32_t i = i0; i < i1; i++) {
         const float omega = c->omega_seq[i];
         const uint32_t j = i - i0;
         const bool odd = (i & 1u) != 0;
         if (c->fp64) {
-            F64State fi = c->f;
+            F64State fi = f0;
             double4* x_next = c->f.x;
             if (fuse) {
                 fi.x = odd ? c->f.x_alt : c->f.x;
@@ -561,11 +588,11 @@ void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool
                     launch_solve_color_f64(s, fi, c->cfg, b, e - b, c->damping, c->desc.dt, fi.x, done);
                 if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
             }
-            if (!fuse) launch_cheb_dual_f64(s, c->f, c->d.n, omega, rc, c->omega_tab, j);
+            if (!fuse) launch_cheb_dual_f64(s, f0, c->d.n, omega, rc, c->omega_tab, j);
             continue;
         }
         // fused: iteration i works on x_cur and extrapolates into x_next
-        DevState di = c->d;
+        DevState di = dz;
         float4* x_next = c->d.x;
         if (fuse) {
             di.x = odd ? c->x_alt : c->d.x;
@@ -585,16 +612,16 @@ void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool
             const uint32_t cb = c->plan.conflict_offsets[col], ce = c->plan.conflict_offsets[col + 1];
             if (ce > cb) launch_commit(s, di, c->d_conflicts + cb, ce - cb, done);
         }
-        if (!fuse) launch_cheb_dual(s, c->d, omega, rc, c->omega_tab, j);
+        if (!fuse) launch_cheb_dual(s, dz, omega, rc, c->omega_tab, j);
         if (n_active) launch_dyn_dual(s, c->d, d_active, n_active, done);
     }
     if (check) {
         // an interval has an even number of iterations whenever the step is
         // fused (see avbd_build_graph), so the iterate is back in x
-        if (c->fp64) launch_residual_f64(s, c->f, c->cfg, c->damping, c->desc.dt, c->f.x, c->res_rr, c->rc);
-        else launch_residual_pass(s, c->d, c->cfg, c->damping, dt, c->rc, c->res_rr);
+        if (c->fp64) launch_residual_f64(s, f0, c->cfg, c->damping, c->desc.dt, c->f.x, c->res_rr, c->rc);
+        else launch_residual_pass(s, dz, c->cfg, c->damping, dt, c->rc, c->res_rr);
         launch_residual_decide(s, c->rc, c->res_rr, c->res_scene_off, c->n_res_scenes, c->res_acc_scene,
-                               c->res_mg2_scene, c->res_tol * c->res_tol, c->res_check_every);
+                               c->res_mg2_scene, c->res_tol * c->res_tol, c->res_check_every, fz);
     }
     hip_check(hipGetLastError(), "kernel launch");
 }
@@ -603,7 +630,8 @@ void enqueue_tail(avbd_ctx* c) {
     const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
     hipStream_t s = c->stream;
     const uint32_t iters = c->iters();
-    if (c->rc) launch_residual_finish(s, c->rc);
+    if (c->rc)
+        launch_residual_finish(s, c->rc, scene_freeze(c), c->n_res_scenes, c->d.n_items + c->d.n_oitems);
     // fused: the final positions are in x_alt after an odd number of
     // iterations (never in residual mode, whose intervals are even)
     if (c->fuse_cheb && !c->dynamic && !c->rc && (iters & 1u)) {
@@ -1504,6 +1532,25 @@ int avbd_get_scene_residual_history(avbd_ctx* c, double* worst, uint32_t cap, ui
     return guarded(c, [&] { residual_history(c, 1, worst, cap, count); });
 }
 
+int avbd_get_scene_iterations(avbd_ctx* c, uint32_t* iters, uint32_t cap, uint32_t* count) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!c->rc) throw std::runtime_error("residual-targeted iteration is off");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        const uint32_t ns = c->n_res_scenes;
+        std::vector<uint32_t> h(ns, 0);
+        if (c->res_iters) {
+            hip_check(hipMemcpy(h.data(), c->res_iters, ns * sizeof(uint32_t), hipMemcpyDeviceToHost), "D2H");
+        } else {
+            ResCtl rc{};
+            hip_check(hipMemcpy(&rc, c->rc, sizeof(ResCtl), hipMemcpyDeviceToHost), "D2H");
+            h.assign(ns, rc.last_iters);
+        }
+        for (uint32_t s = 0; s < ns && s < cap; s++) iters[s] = h[s];
+        if (count) *count = ns;
+    });
+}
+
 int avbd_set_scenes(avbd_ctx* c, uint32_t n_scenes, const uint32_t* offsets) {
     return guarded(c, [&] {
         require_not_planned(c);
@@ -1796,6 +1843,23 @@ int avbd_build_graph(avbd_ctx* c) {
             upload(c, c->res_scene_off, so);
             upload(c, c->res_mg2_scene, mg2);
             upload(c, c->omega_tab, c->omega_seq);
+            if (c->n_res_scenes > 1) {
+                std::vector<uint32_t> scene_of(n, 0), items(c->n_res_scenes, 0);
+                for (uint32_t sc = 0; sc < c->n_res_scenes; sc++)
+                    for (uint32_t p = so[sc]; p < so[sc + 1]; p++) scene_of[p] = sc;
+                for (const VItem& it : c->plan.items) items[scene_of[it.v]]++;
+                for (const OItem& it : c->plan.oitems) items[scene_of[it.v]]++;
+                c->res_frozen = dev_alloc<uint32_t>(c, c->n_res_scenes);
+                c->res_iters = dev_alloc<uint32_t>(c, c->n_res_scenes);
+                c->res_items = dev_alloc<uint32_t>(c, c->n_res_scenes);
+                c->res_scene_of = dev_alloc<uint32_t>(c, n);
+                upload(c, c->res_items, items);
+                upload(c, c->res_scene_of, scene_of);
+                hip_check(hipMemsetAsync(c->res_frozen, 0, c->n_res_scenes * sizeof(uint32_t), c->stream),
+                          "hipMemsetAsync");
+                hip_check(hipMemsetAsync(c->res_iters, 0, c->n_res_scenes * sizeof(uint32_t), c->stream),
+                          "hipMemsetAsync");
+            }
             hip_check(hipMemsetAsync(c->rc, 0, sizeof(ResCtl), c->stream), "hipMemsetAsync");
             hip_check(hipMemsetAsync(c->res_rr, 0, (size_t)n * sizeof(double), c->stream), "hipMemsetAsync");
             // converged flag of every interval of a step, copied back after the interval
@@ -2215,7 +2279,7 @@ int avbd_get_stats(avbd_ctx* c, avbd_stats* out) {
             hip_check(hipMemcpy(&rc, c->rc, sizeof(ResCtl), hipMemcpyDeviceToHost), "D2H");
             st.iterations_total = rc.total_iters;
             st.last_step_iterations = rc.last_iters;
-            st.vertex_updates = rc.total_iters * (uint64_t)(c->d.n_items + c->d.n_oitems);
+            st.vertex_updates = rc.vertex_solves;
             st.steps_at_budget = rc.budget_steps;
         } else {
             st.iterations_total = c->steps_taken * c->desc.solver_iters;

@@ -220,6 +220,11 @@ struct DevState {
     // velocity update, read and cleared by the host)
     uint32_t* status = nullptr;
 
+    // residual mode over several scenes: per-scene convergence flags (null:
+    // no freezing) and the scene of every particle
+    const uint32_t* frozen = nullptr;
+    const uint32_t* scene_of = nullptr;
+
     // per-particle external acceleration added to gravity (xyz), or null
     // when the scene has no such field (avbd_set_external_accel)
     const float4* a_ext = nullptr;
@@ -249,6 +254,8 @@ struct F64State {
     const TetRec* irec = nullptr;
     const VItem* items = nullptr;
     const float4* a_ext = nullptr;  // = DevState::a_ext
+    const uint32_t* frozen = nullptr;    // = DevState::frozen
+    const uint32_t* scene_of = nullptr;  // = DevState::scene_of
     uint32_t n = 0, n_tets = 0, n_ground = 0, n_items = 0;
 };
 
@@ -274,6 +281,17 @@ struct ResCtl {
     uint32_t iter;         // iterations run in the current step (Chebyshev weight index)
     unsigned long long total_iters;
     unsigned long long budget_steps;   // steps that ended at max_iters unconverged
+    unsigned long long vertex_solves;  // vertex-block solves run (frozen scenes skip theirs)
+};
+
+// Per-scene freezing of a batch in residual mode: a scene that has reached
+// the target stops iterating (its solves, Chebyshev steps and dual updates
+// return at once) while the others go on, as independent runs would.
+struct SceneFreeze {
+    uint32_t* frozen = nullptr;          // per scene: 1 once converged in the current step
+    uint32_t* iters = nullptr;           // per scene: iterations it ran in the current step
+    const uint32_t* items = nullptr;     // per scene: free work items (vertex-block solves per iteration)
+    const uint32_t* scene_of = nullptr;  // per particle
 };
 
 void launch_eprev(hipStream_t s, const DevState& d, float dt);
@@ -304,13 +322,14 @@ void launch_settle(hipStream_t s, float4* x, const float4* x_alt, uint32_t n, co
 // into rr (FP32 state; launch_residual_f64 for the FP64 state); the decision
 // sums rr per scene and sets rc->done once every scene is within tol2
 // (relative, squared); the bookkeeping at step end
-void launch_residual_reset(hipStream_t s, ResCtl* rc);
 void launch_residual_pass(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
                           const ResCtl* rc, double* rr);
 void launch_residual_decide(hipStream_t s, ResCtl* rc, const double* rr, const uint32_t* scene_off,
                             uint32_t n_scenes, double* acc_scene, const double* mg2_scene, double tol2,
-                            uint32_t every);
-void launch_residual_finish(hipStream_t s, ResCtl* rc);
+                            uint32_t every, const SceneFreeze& fz);
+void launch_residual_reset(hipStream_t s, ResCtl* rc, const SceneFreeze& fz, uint32_t n_scenes);
+void launch_residual_finish(hipStream_t s, ResCtl* rc, const SceneFreeze& fz, uint32_t n_scenes,
+                            uint32_t n_items);
 // learned warm start
 void launch_warm_save(hipStream_t s, const DevState& d, const WarmIO& io);
 void launch_warm_features(hipStream_t s, const DevState& d, const WarmIO& io, uint32_t lanes, float dt, float g,
@@ -351,6 +370,7 @@ void launch_residual_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg,
 uint32_t launch_stats_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg, bool damping, double dt, double g,
                           double* out);
 void launch_warm_save_f64(hipStream_t s, const F64State& f, uint32_t n);
+void launch_warm_ip_f64(hipStream_t s, const F64State& f, const WarmIO& io, bool damping, double dt);
 void launch_warm_place_f64(hipStream_t s, const F64State& f, const float4* pred, const uint32_t* scene_of,
                            const float* step_of_scene);
 void launch_warm_target_f64(hipStream_t s, const F64State& f, float* tgt, double inv_s);

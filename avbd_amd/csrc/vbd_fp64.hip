@@ -22,6 +22,7 @@
 // accumulate, energy/elastic/NeoHookeanEnergySolver.hpp:121-227; the ground AL
 // terms, energy/collision/CollisionConstraintEnergySolver.hpp:60-285).
 #include "internal.hpp"
+#include "warm_device.hpp"
 
 namespace avbd {
 namespace {
@@ -290,12 +291,15 @@ __global__ __launch_bounds__(256) void solve_color_f64_kernel(const VItem* __res
                                                               const double4* __restrict__ y,
                                                               const GroundRec* __restrict__ ground, double4* x_out,
                                                               double inv_dt, double inv_dt2,
-                                                              const uint32_t* __restrict__ done) {
+                                                              const uint32_t* __restrict__ done,
+                                                              const uint32_t* __restrict__ frozen,
+                                                              const uint32_t* __restrict__ scene_of) {
     if (done && *done) return;
     const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L, lane = tid % L;
     if (item >= n_items) return;   // whole L-lane groups exit together
     const VItem it = items[item];
+    if (frozen && frozen[scene_of[it.v]]) return;   // the item's scene has converged in this step
     double4 xv = make_double4(0.0, 0.0, 0.0, 0.0), yv = xv;
     GroundRec gr{};
     const bool lead = lane == 0;
@@ -336,6 +340,10 @@ __global__ __launch_bounds__(256) void solve_cheb_f64_kernel(F64State f, const V
         if (item >= n_items) return;
         const VItem it = items[item];
         const uint32_t v = it.v;
+        if (f.frozen && f.frozen[f.scene_of[v]]) {   // converged scene: keep its position in both buffers
+            if (lane == 0) x_next[v] = x_cur[v];
+            return;
+        }
         double4 xv = make_double4(0.0, 0.0, 0.0, 0.0), yv = xv, xll = xv;
         GroundRec gr{};
         const bool lead = lane == 0;
@@ -363,6 +371,10 @@ __global__ __launch_bounds__(256) void solve_cheb_f64_kernel(F64State f, const V
         const uint32_t k = (xcd * cx + (q - sx)) * blockDim.x + threadIdx.x;
         if (k >= n_cheb) return;
         const uint32_t p = cheb_list[k];
+        if (f.frozen && f.frozen[f.scene_of[p]]) {
+            x_next[p] = x_cur[p];
+            return;
+        }
         const double4 xl = f.x_last[p], xll = f.x_lastlast[p], xc = x_cur[p];
         const double4 xp = make_double4(w * (xc.x - xll.x) + xll.x, w * (xc.y - xll.y) + xll.y,
                                         w * (xc.z - xll.z) + xll.z, xc.w);
@@ -383,7 +395,7 @@ __global__ void cheb_dual_f64_kernel(F64State f, uint32_t n, float omega, const 
                                      uint32_t j) {
     if (rc && rc->done) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= n || (f.frozen && f.frozen[f.scene_of[p]])) return;
     const double w = cheb_omega(omega, rc, omega_tab, j);
     const double4 xl = f.x_last[p], xll = f.x_lastlast[p], xc = f.x[p];
     const double4 xp = make_double4(w * (xc.x - xll.x) + xll.x, w * (xc.y - xll.y) + xll.y,
@@ -515,6 +527,7 @@ __global__ __launch_bounds__(256) void residual_f64_kernel(F64State f, const VIt
     const uint32_t item = tid / L, lane = tid % L;
     if (item >= n_items) return;
     const VItem it = items[item];
+    if (f.frozen && f.frozen[f.scene_of[it.v]]) return;   // keeps the residual it converged with
     D3 g;
     SymD h;
     item_gradient<L, 1, DAMP, false>(it, lane, f.inc, f.irec, f.damp, x, inv_dt, g, h);
@@ -641,9 +654,108 @@ __global__ void warm_target_f64_kernel(F64State f, const VItem* __restrict__ ite
     tgt[(size_t)i * 3 + 2] = (float)((x.z - x0.z) * inv_s);
 }
 
+// Per scene and step length a: incremental potential at x0 + a pred minus at
+// x0, in FP64 from the FP64 start (the terms that change along the
+// prediction: Neo-Hookean with damping, ground AL, inertia).  An FP32
+// evaluation cannot resolve these differences once the start is already
+// close to the solution (the energies are ~1e4 times larger than their
+// change there), and would then pick predictions that raise the potential.
+__device__ __forceinline__ D3 along(const double4* x0, const float4* pred, uint32_t p, double a) {
+    const double4 b = x0[p];
+    const float4 d = pred[p];
+    return D3{b.x + a * d.x, b.y + a * d.y, b.z + a * d.z};
+}
+
+template <bool DAMP>
+__device__ __forceinline__ double nh_energy_at(D3 p0, D3 p1, D3 p2, D3 p3, float4 r1, float4 r2, float4 r3,
+                                               const TetDamp* __restrict__ damp, uint32_t t, double inv_dt) {
+    D3 f0, f1, f2;
+    deform(p0, p1, p2, p3, r1, r2, r3, f0, f1, f2);
+    const double J = dot3(f0, cross3(f1, f2));
+    const double VL = r3.y, VM = r3.z, al = r3.w;
+    const double trc = dot3(f0, f0) + dot3(f1, f1) + dot3(f2, f2);
+    double e = 0.5 * VL * (J - al) * (J - al) + 0.5 * VM * (trc - 3.0);
+    if constexpr (DAMP) {
+        const TetDamp dr = damp[t];
+        const double E[6] = {0.5 * (dot3(f0, f0) - 1.0), 0.5 * (dot3(f1, f1) - 1.0), 0.5 * (dot3(f2, f2) - 1.0),
+                             0.5 * dot3(f0, f1), 0.5 * dot3(f0, f2), 0.5 * dot3(f1, f2)};
+        const double Pv[6] = {dr.e00, dr.e11, dr.e22, dr.e01, dr.e02, dr.e12};
+        double sq = 0;
+        for (int i = 0; i < 6; i++) {
+            const double dd = (E[i] - Pv[i]) * inv_dt;
+            sq += (i < 3 ? 1.0 : 2.0) * dd * dd;
+        }
+        e += 0.5 * dr.damp / inv_dt * sq;
+    }
+    return e;
+}
+
+__device__ __forceinline__ double ground_energy_at(const GroundRec& c, D3 p) {
+    return ground_energy(c, make_double4(p.x, p.y, p.z, 0.0));
+}
+
+template <bool DAMP>
+__global__ __launch_bounds__(256) void warm_ip_f64_kernel(F64State f, WarmIO io, double inv_dt) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    double e[kWarmAlphas] = {0.0, 0.0, 0.0};
+    uint32_t scene = 0;
+    bool active = false;
+    if (k < f.n_tets) {
+        uint4 idx;
+        float4 r1, r2, r3;
+        load_rec(f.tets, k, idx, r1, r2, r3);
+        const uint32_t v[4] = {idx.x, idx.y, idx.z, idx.w};
+        D3 P[4];
+        for (int i = 0; i < 4; i++) P[i] = along(f.x0, io.pred, v[i], 0.0);
+        const double e0 = nh_energy_at<DAMP>(P[0], P[1], P[2], P[3], r1, r2, r3, f.damp, k, inv_dt);
+        for (int j = 0; j < kWarmAlphas; j++) {
+            for (int i = 0; i < 4; i++) P[i] = along(f.x0, io.pred, v[i], kWarmSteps[j]);
+            e[j] = nh_energy_at<DAMP>(P[0], P[1], P[2], P[3], r1, r2, r3, f.damp, k, inv_dt) - e0;
+        }
+        scene = io.scene_of[idx.x];
+        active = true;
+    } else if (k < f.n_tets + f.n_ground) {
+        const GroundRec c = f.ground[k - f.n_tets];
+        const double e0 = ground_energy_at(c, along(f.x0, io.pred, c.p, 0.0));
+        for (int j = 0; j < kWarmAlphas; j++) e[j] = ground_energy_at(c, along(f.x0, io.pred, c.p, kWarmSteps[j])) - e0;
+        scene = io.scene_of[c.p];
+        active = true;
+    } else if (k < f.n_tets + f.n_ground + f.n_items) {
+        const uint32_t v = f.items[k - f.n_tets - f.n_ground].v;
+        const double4 x0 = f.x0[v], y = f.y[v];
+        const double md = 0.5 * x0.w * inv_dt * inv_dt;
+        const D3 d0 = D3{x0.x - y.x, x0.y - y.y, x0.z - y.z};
+        const double b = dot3(d0, d0);
+        for (int j = 0; j < kWarmAlphas; j++) {
+            const D3 x = along(f.x0, io.pred, v, kWarmSteps[j]);
+            const D3 d = D3{x.x - y.x, x.y - y.y, x.z - y.z};
+            e[j] = md * (dot3(d, d) - b);
+        }
+        scene = io.scene_of[v];
+        active = true;
+    }
+    add_per_scene_block<kWarmAlphas>(io.ipd, io.n_scenes, scene, e, active);
+}
+
+__global__ void warm_ip_reset_f64_kernel(WarmIO io) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < io.n_scenes * kWarmAlphas) io.ipd[s] = 0.0;
+}
+
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 }  // namespace
+
+void launch_warm_ip_f64(hipStream_t s, const F64State& f, const WarmIO& io, bool damping, double dt) {
+    hipLaunchKernelGGL(warm_ip_reset_f64_kernel, dim3(blocks_for(io.n_scenes * kWarmAlphas, 256)), dim3(256), 0, s,
+                       io);
+    const uint32_t n = f.n_tets + f.n_ground + f.n_items;
+    if (n == 0) return;
+    if (damping)
+        hipLaunchKernelGGL(warm_ip_f64_kernel<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, f, io, 1.0 / dt);
+    else
+        hipLaunchKernelGGL(warm_ip_f64_kernel<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, f, io, 1.0 / dt);
+}
 
 // ---- launch wrappers ---------------------------------------------------------
 
@@ -656,7 +768,7 @@ void launch_solve_color_f64(hipStream_t s, const F64State& f, const LaunchCfg& c
     const double inv_dt = 1.0 / dt, inv_dt2 = inv_dt * inv_dt;
 #define AVBD_F64_LAUNCH(LL, DD, UU)                                                                                  \
     hipLaunchKernelGGL((solve_color_f64_kernel<LL, DD, UU>), grid, blk, 0, s, f.items + begin, count, f.inc, f.irec, \
-                       f.damp, f.x, f.y, f.ground, x_out, inv_dt, inv_dt2, done)
+                       f.damp, f.x, f.y, f.ground, x_out, inv_dt, inv_dt2, done, f.frozen, f.scene_of)
 #define AVBD_F64_U(LL, DD)                           \
     if (cfg.batch >= 2) AVBD_F64_LAUNCH(LL, DD, 2);  \
     else AVBD_F64_LAUNCH(LL, DD, 1);

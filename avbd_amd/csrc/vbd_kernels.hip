@@ -21,6 +21,7 @@
 #include "internal.hpp"
 #include "dyn_device.hpp"
 #include "rigid_device.hpp"
+#include "warm_device.hpp"
 
 namespace avbd {
 
@@ -364,13 +365,17 @@ __global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_color_kernel(const V
                                                            const float4* __restrict__ y,
                                                            const GroundRec* __restrict__ ground, float4* x_out,
                                                            float4* x_buf, float inv_dt, float inv_dt2,
-                                                           const uint32_t* __restrict__ done, DynArgs da) {
+                                                           const uint32_t* __restrict__ done, DynArgs da,
+                                                           const uint32_t* __restrict__ frozen,
+                                                           const uint32_t* __restrict__ scene_of) {
     if (done && *done) return;   // residual target reached earlier in this step
     const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L;
     const uint32_t lane = tid % L;
     if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)
-    solve_item<L, DAMP, DYN, U>(items[item], lane, inc, irec, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2, da);
+    const VItem it = items[item];
+    if (frozen && frozen[scene_of[it.v]]) return;   // the item's scene has converged in this step
+    solve_item<L, DAMP, DYN, U>(it, lane, inc, irec, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2, da);
 }
 
 // E_prev = 0.5 (F_prev^T F_prev - I) from previous positions
@@ -599,7 +604,8 @@ __global__ void cheb_dual_kernel(DevState d, float omega, const ResCtl* rc, cons
                                  uint32_t j) {
     if (rc && rc->done) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < d.n) cheb_dual_one(d, p, cheb_weight(omega, rc, omega_tab, j));
+    if (p >= d.n || (d.frozen && d.frozen[d.scene_of[p]])) return;
+    cheb_dual_one(d, p, cheb_weight(omega, rc, omega_tab, j));
 }
 
 // The last colour phase of an iteration fused with the Chebyshev / dual
@@ -629,6 +635,10 @@ __global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_cheb_kernel(
         if (item >= n_items) return;   // whole L-lane groups exit together
         const VItem it = items[item];
         const uint32_t v = it.v;
+        if (d.frozen && d.frozen[d.scene_of[v]]) {   // converged scene: keep its position in both buffers
+            if (lane == 0) x_next[v] = d.x[v];
+            return;
+        }
         float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), yv = xv, xl = xv, xll = xv;
         GroundRec gr{};
         const bool lead = lane == 0;
@@ -653,7 +663,11 @@ __global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_cheb_kernel(
         }
     } else {
         const uint32_t k = (xcd * cx + (q - sx)) * blockDim.x + threadIdx.x;
-        if (k < n_cheb) cheb_dual_io(d, cheb_list[k], omega, d.x, x_next);
+        if (k < n_cheb) {
+            const uint32_t p = cheb_list[k];
+            if (d.frozen && d.frozen[d.scene_of[p]]) x_next[p] = d.x[p];
+            else cheb_dual_io(d, p, omega, d.x, x_next);
+        }
     }
 }
 
@@ -749,7 +763,10 @@ __global__ __launch_bounds__(256) void solve_ocolor_kernel(DevState d, uint32_t 
                                                            float inv_dt2, const uint32_t* __restrict__ done) {
     if (done && *done) return;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) rigid::solve_oitem<DYN>(d, d.oitems[begin + k], inv_dt2);
+    if (k >= count) return;
+    const OItem it = d.oitems[begin + k];
+    if (d.frozen && d.frozen[d.scene_of[it.v]]) return;
+    rigid::solve_oitem<DYN>(d, it, inv_dt2);
 }
 
 // ---- force residual (residual-targeted iteration) --------------------------
@@ -796,25 +813,33 @@ __global__ __launch_bounds__(256) void residual_pass_kernel(DevState d, const Re
     if (rc->done) return;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L, lane = tid % L;
+    // a converged scene keeps the residual it converged with
     if (item < d.n_items) {
         const VItem it = d.items[item];
-        const float r = residual_item<L, DAMP>(it, lane, d, inv_dt, inv_dt2);
-        if (lane == 0) rr[it.v] = r;
+        if (!(d.frozen && d.frozen[d.scene_of[it.v]])) {
+            const float r = residual_item<L, DAMP>(it, lane, d, inv_dt, inv_dt2);
+            if (lane == 0) rr[it.v] = r;
+        }
     }
     if (tid < d.n_oitems) {
         const OItem it = d.oitems[tid];
-        rr[it.v] = residual_oitem(d, it, inv_dt2);
+        if (!(d.frozen && d.frozen[d.scene_of[it.v]])) rr[it.v] = residual_oitem(d, it, inv_dt2);
     }
 }
 
 // One block per scene: sum of rr over the scene's particles in a fixed order
 // (reproducible, no atomics).
+// A scene that is within the target now (or has no free particle) freezes
+// for the rest of the step; its iteration count is the step's count so far.
 __global__ __launch_bounds__(1024) void residual_scene_kernel(const ResCtl* rc, const double* __restrict__ rr,
                                                               const uint32_t* __restrict__ scene_off,
-                                                              double* __restrict__ acc_scene) {
+                                                              double* __restrict__ acc_scene,
+                                                              const double* __restrict__ mg2_scene, double tol2,
+                                                              uint32_t every, SceneFreeze fz) {
     if (rc->done) return;
     __shared__ double red[1024];
     const uint32_t s = blockIdx.x, b = scene_off[s], e = scene_off[s + 1];
+    if (fz.frozen && fz.frozen[s]) return;   // its sum is the one it converged with
     double v = 0;
     for (uint32_t p = b + threadIdx.x; p < e; p += 1024) v += rr[p];
     red[threadIdx.x] = v;
@@ -823,7 +848,13 @@ __global__ __launch_bounds__(1024) void residual_scene_kernel(const ResCtl* rc, 
         if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) acc_scene[s] = red[0];
+    if (threadIdx.x == 0) {
+        acc_scene[s] = red[0];
+        if (fz.frozen && red[0] <= tol2 * mg2_scene[s]) {
+            fz.frozen[s] = 1;
+            fz.iters[s] = rc->iter + every;
+        }
+    }
 }
 
 // The decision after a check interval of `every` iterations: every scene
@@ -871,21 +902,44 @@ __global__ __launch_bounds__(256) void residual_decide_kernel(ResCtl* rc, const 
 }
 
 // Start of a residual-targeted step: nothing converged, no iteration run.
-__global__ void residual_reset_kernel(ResCtl* rc) {
-    if (threadIdx.x == 0) {
+__global__ void residual_reset_kernel(ResCtl* rc, SceneFreeze fz, uint32_t n_scenes) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
         rc->done = 0;
         rc->checks = 0;
         rc->iter = 0;
     }
+    if (fz.frozen && t < n_scenes) fz.frozen[t] = 0;
 }
 
 // End of a residual-targeted step: a step that never converged used the
-// whole budget (the iterations it ran).
-__global__ void residual_finish_kernel(ResCtl* rc) {
+// whole budget (the iterations it ran); a scene still running at the end ran
+// all of them; the vertex-block solves of the step are every scene's
+// iterations times its work items.
+__global__ __launch_bounds__(256) void residual_finish_kernel(ResCtl* rc, SceneFreeze fz, uint32_t n_scenes,
+                                                              uint32_t n_items) {
+    __shared__ unsigned long long red[256];
+    const uint32_t iters_run = rc->done ? rc->last_iters : rc->iter;
+    unsigned long long v = 0;
+    if (fz.frozen) {
+        for (uint32_t s = threadIdx.x; s < n_scenes; s += 256) {
+            if (!fz.frozen[s]) fz.iters[s] = iters_run;
+            v += (unsigned long long)fz.iters[s] * fz.items[s];
+        }
+    } else if (threadIdx.x == 0) {
+        v = (unsigned long long)iters_run * n_items;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
         if (!rc->done) rc->last_iters = rc->iter;
         rc->total_iters += rc->last_iters;
         if (!rc->done) rc->budget_steps += 1;
+        rc->vertex_solves += red[0];
     }
 }
 
@@ -1036,69 +1090,6 @@ __global__ void warm_apply_kernel(DevState d, WarmIO io, float s) {
     io.pred[v] = make_float4(s * (o[0] * tn[3] + tn[0]), s * (o[1] * tn[4] + tn[1]), s * (o[2] * tn[5] + tn[2]), 0.f);
 }
 
-// Add v[j] to acc[j * stride + scene] for NV values over a block of 256
-// threads.  When every active lane of the block has one scene (a one-scene
-// run, or the interior of a scene in a batch) the block adds one sum per
-// value; otherwise each wavefront whose active lanes share a scene adds its
-// sums, and the lanes of a mixed wavefront add their own values.  Every
-// thread of the block must call it.
-template <int NV>
-__device__ __forceinline__ void add_per_scene_block(double* acc, uint32_t stride, uint32_t scene, const double* v,
-                                                    bool active) {
-    __shared__ double part[4][NV];
-    __shared__ uint32_t wscene[4], wstate[4];   // state: 0 no active lane, 1 one scene, 2 mixed
-    const unsigned long long act = __ballot(active);
-    const uint32_t wave = threadIdx.x / 64u, lane = __lane_id();
-    uint32_t s0 = 0, state = 0;
-    if (act) {
-        const int first = __ffsll((long long)act) - 1;
-        s0 = __shfl(scene, first);
-        state = __all(!active || scene == s0) ? 1u : 2u;
-    }
-    double t[NV];
-#pragma unroll
-    for (int j = 0; j < NV; j++) {
-        t[j] = active ? v[j] : 0.0;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) t[j] += __shfl_xor(t[j], off);
-    }
-    if (lane == 0) {
-        wscene[wave] = s0;
-        wstate[wave] = state;
-#pragma unroll
-        for (int j = 0; j < NV; j++) part[wave][j] = t[j];
-    }
-    __syncthreads();
-    bool uniform = true;
-    uint32_t sb = 0;
-    bool any = false;
-    for (int w = 0; w < 4; w++) {
-        if (wstate[w] == 2u) uniform = false;
-        if (wstate[w] == 1u) {
-            if (any && wscene[w] != sb) uniform = false;
-            sb = wscene[w];
-            any = true;
-        }
-    }
-    if (uniform) {
-        if (any && threadIdx.x == 0) {
-#pragma unroll
-            for (int j = 0; j < NV; j++)
-                atomicAdd(&acc[j * stride + sb], part[0][j] + part[1][j] + part[2][j] + part[3][j]);
-        }
-        return;
-    }
-    if (state == 1u) {
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < NV; j++) atomicAdd(&acc[j * stride + s0], t[j]);
-        }
-    } else if (state == 2u && active) {
-#pragma unroll
-        for (int j = 0; j < NV; j++) atomicAdd(&acc[j * stride + scene], v[j]);
-    }
-}
-
 __device__ __forceinline__ double ground_al_energy(const GroundRec& c, float4 xp) {
     const double Cn = -xp.y, Ct = (double)xp.x - c.cp_x, Cb = (double)xp.z - c.cp_z;
     const double Cc = Cn - (double)kAlpha * c.C_n_prev;
@@ -1107,8 +1098,6 @@ __device__ __forceinline__ double ground_al_energy(const GroundRec& c, float4 xp
     return e + 0.5 * c.k_t * Ct * Ct + (double)c.lambda_t * Ct + 0.5 * c.k_b * Cb * Cb + (double)c.lambda_b * Cb;
 }
 
-// Step lengths tried along the prediction: x0 + a * pred.
-__device__ constexpr float kWarmSteps[kWarmAlphas] = {1.0f, 0.5f, 0.25f};
 
 __device__ __forceinline__ float4 along(const float4* x0, const float4* pred, uint32_t p, float a) {
     const float4 b = x0[p], d = pred[p];
@@ -1356,7 +1345,7 @@ void solve_dispatch(hipStream_t s, const DevState& d, uint32_t block, uint32_t b
     const dim3 grid(blocks_for(threads, block)), blk(block);
 #define AVBD_SOLVE_ARGS \
     d.items + begin, count, d.inc, d.irec, d.damp, d.x, d.y, d.ground, x_out, d.x_buf, 1.0f / dt, 1.0f / (dt * dt), \
-        done, dyn_args(d)
+        done, dyn_args(d), d.frozen, d.scene_of
     if (block > 256) batch = 1;   // the batched variants are built for blocks of <= 256 threads
     if (d.dyn_off) hipLaunchKernelGGL((solve_color_kernel<L, DAMP, true, 1>), grid, blk, 0, s, AVBD_SOLVE_ARGS);
     else if (batch >= 4) hipLaunchKernelGGL((solve_color_kernel<L, DAMP, false, 4>), grid, blk, 0, s, AVBD_SOLVE_ARGS);
@@ -1407,8 +1396,9 @@ void launch_solve_ocolor(hipStream_t s, const DevState& d, uint32_t begin, uint3
                            count, 1.0f / (dt * dt), done);
 }
 
-void launch_residual_reset(hipStream_t s, ResCtl* rc) {
-    hipLaunchKernelGGL(residual_reset_kernel, dim3(1), dim3(64), 0, s, rc);
+void launch_residual_reset(hipStream_t s, ResCtl* rc, const SceneFreeze& fz, uint32_t n_scenes) {
+    const uint32_t n = fz.frozen ? (n_scenes > 0 ? n_scenes : 1u) : 1u;
+    hipLaunchKernelGGL(residual_reset_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, rc, fz, n_scenes);
 }
 
 void launch_residual_pass(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt,
@@ -1430,14 +1420,15 @@ void launch_residual_pass(hipStream_t s, const DevState& d, const LaunchCfg& cfg
 
 void launch_residual_decide(hipStream_t s, ResCtl* rc, const double* rr, const uint32_t* scene_off,
                             uint32_t n_scenes, double* acc_scene, const double* mg2_scene, double tol2,
-                            uint32_t every) {
-    hipLaunchKernelGGL(residual_scene_kernel, dim3(n_scenes), dim3(1024), 0, s, rc, rr, scene_off, acc_scene);
+                            uint32_t every, const SceneFreeze& fz) {
+    hipLaunchKernelGGL(residual_scene_kernel, dim3(n_scenes), dim3(1024), 0, s, rc, rr, scene_off, acc_scene,
+                       mg2_scene, tol2, every, fz);
     hipLaunchKernelGGL(residual_decide_kernel, dim3(1), dim3(256), 0, s, rc, acc_scene, mg2_scene, n_scenes, tol2,
                        every);
 }
 
-void launch_residual_finish(hipStream_t s, ResCtl* rc) {
-    hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(64), 0, s, rc);
+void launch_residual_finish(hipStream_t s, ResCtl* rc, const SceneFreeze& fz, uint32_t n_scenes, uint32_t n_items) {
+    hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(256), 0, s, rc, fz, n_scenes, n_items);
 }
 
 void launch_dyn_apply(hipStream_t s, const DevState& d, const DynOp* ops, uint32_t n_ops) {

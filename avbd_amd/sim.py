@@ -377,6 +377,14 @@ class SimulationContext:
         self._chk(fn(self.handle, _dptr(buf), cap, C.byref(n)))
         return buf[:min(cap, n.value)].copy()
 
+    def scene_iterations(self) -> np.ndarray:
+        """Iterations each scene ran in the last step (residual-targeted mode)."""
+        n = C.c_uint32()
+        self._chk(lib.avbd_get_scene_iterations(self.handle, None, 0, C.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint32)
+        self._chk(lib.avbd_get_scene_iterations(self.handle, _uptr(buf), n.value, C.byref(n)))
+        return buf
+
     def set_scenes(self, offsets: Sequence[int]) -> None:
         off = np.ascontiguousarray(offsets, dtype=np.uint32)
         self._chk(lib.avbd_set_scenes(self.handle, off.shape[0] - 1, _uptr(off)))

@@ -285,6 +285,12 @@ int avbd_get_residual_history(avbd_ctx* ctx, double* rel, uint32_t cap, uint32_t
 /* The same checks' largest per-scene relative residual (the quantity the
  * stopping rule compares with rel_tol). */
 int avbd_get_scene_residual_history(avbd_ctx* ctx, double* worst, uint32_t cap, uint32_t* count);
+/* Iterations each scene ran in the last step (residual-targeted mode; new).
+ * In a batch a scene stops iterating at the first check where it meets
+ * rel_tol, while the others go on, so every scene ends its step where it
+ * would on its own; the step's count is the largest.  *count = scenes (at
+ * most cap are written). */
+int avbd_get_scene_iterations(avbd_ctx* ctx, uint32_t* iters, uint32_t cap, uint32_t* count);
 
 /* Scene partition for batching: scene s owns particles
  * [particle_offsets[s], particle_offsets[s+1]); each scene is coloured on its
