@@ -208,6 +208,7 @@ SIGNATURES = {
     "avbd_get_residual_history": (C.c_int, [_P, _D, C.c_uint32, _U32]),
     "avbd_get_scene_residual_history": (C.c_int, [_P, _D, C.c_uint32, _U32]),
     "avbd_get_scene_iterations": (C.c_int, [_P, _U32, C.c_uint32, _U32]),
+    "avbd_get_warm_start_decision": (C.c_int, [_P, _D, _F, C.c_uint32, _U32]),
     "avbd_mlp_param_count": (C.c_uint64, []),
     "avbd_mlp_init": (C.c_int, [_P, C.c_uint64]),
     "avbd_mlp_set_params": (C.c_int, [_P, _F, C.c_uint64]),

@@ -1783,6 +1783,8 @@ int avbd_build_graph(avbd_ctx* c) {
             F64State& f = c->f;
             f.a_ext = d.a_ext;
             f.x32 = d.x;
+            f.y32 = d.y;
+            f.x_prev32 = d.x_prev;
             f.v = d.v;
             f.v_prev = d.v_prev;
             f.fixed = d.fixed;
@@ -2544,6 +2546,25 @@ int avbd_record_steps(avbd_ctx* c, uint32_t n_steps, float* feats, float* tgts, 
     });
 }
 
+int avbd_get_warm_start_decision(avbd_ctx* c, double* ip_delta, float* step, uint32_t cap, uint32_t* n_scenes) {
+    return guarded(c, [&] {
+        require_built(c);
+        if (!c->warm) throw CodeError(AVBD_E_ARG, "the warm start is off");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        const uint32_t ns = c->wio.n_scenes;
+        std::vector<double> ipd((size_t)kWarmAlphas * ns);
+        std::vector<float> st(ns);
+        hip_check(hipMemcpy(ipd.data(), c->wio.ipd, ipd.size() * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpy(st.data(), c->wio.step_of, ns * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+        for (uint32_t s = 0; s < ns && s < cap; s++) {
+            if (ip_delta)
+                for (int j = 0; j < kWarmAlphas; j++) ip_delta[(size_t)kWarmAlphas * s + j] = ipd[(size_t)j * ns + s];
+            if (step) step[s] = st[s];
+        }
+        if (n_scenes) *n_scenes = ns;
+    });
+}
+
 int avbd_set_warm_start(avbd_ctx* c, int enable) {
     return guarded(c, [&] {
         ensure_model(c);
@@ -2727,7 +2748,7 @@ struct CkptHeader {
     uint64_t steps_taken, vertex_updates;
     uint64_t array_bytes;     // device arrays that follow the header
 };
-constexpr uint32_t kCkptVersion = 2;
+constexpr uint32_t kCkptVersion = 3;
 constexpr char kCkptMagic[8] = {'A', 'V', 'B', 'D', 'C', 'K', 'P', 'T'};
 
 // (device pointer, bytes) of every mutable device array, in checkpoint order

@@ -243,6 +243,8 @@ struct F64State {
     double4* x0 = nullptr;         // warm start: the initial guess of the step
     double4* x_scratch = nullptr;  // autotuner output sink
     float4* x32 = nullptr;         // the FP32 mirror (= DevState::x)
+    float4* y32 = nullptr;         // FP32 mirrors of y and x_prev (= DevState::y, ::x_prev), read by
+    float4* x_prev32 = nullptr;    // the warm-start features
     float4* v = nullptr;
     float4* v_prev = nullptr;
     const uint8_t* fixed = nullptr;

@@ -455,8 +455,10 @@ __global__ void step_begin_f64_kernel(F64State f, uint32_t n, double dt, double 
             else a_tilde = along / an;
         }
         const double dt2 = dt * dt;
-        f.y[p] = make_double4(xp.x + dt * vv.x + dt2 * ex, xp.y + dt * vv.y + dt2 * ey, xp.z + dt * vv.z + dt2 * ez,
-                              0.0);
+        const double4 yv = make_double4(xp.x + dt * vv.x + dt2 * ex, xp.y + dt * vv.y + dt2 * ey,
+                                        xp.z + dt * vv.z + dt2 * ez, 0.0);
+        f.y[p] = yv;
+        f.y32[p] = mirror(yv);
         xp.x = xp.x + (dt * vv.x + dt2 * (a_tilde * ex));
         xp.y = xp.y + (dt * vv.y + dt2 * (a_tilde * ey));
         xp.z = xp.z + (dt * vv.z + dt2 * (a_tilde * ez));
@@ -501,6 +503,7 @@ __global__ void velocity_f64_kernel(F64State f, uint32_t n, double inv_dt, uint3
     f.v[p] = nv;
     f.x_prev[p] = xp;
     f.x32[p] = mirror(xp);
+    f.x_prev32[p] = mirror(xp);
     const bool bad = !(isfinite(xp.x) && isfinite(xp.y) && isfinite(xp.z) && isfinite(nv.x) && isfinite(nv.y) &&
                        isfinite(nv.z));
     if (bad) atomicOr(status, 1u);

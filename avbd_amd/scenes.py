@@ -181,7 +181,10 @@ class Scene:
     dt: float = 1.0 / 60.0
     g: float = 9.81
     iters: int = 20
-    rho: float = 0.95
+    # Chebyshev rho: the reference's SimulationConfig default
+    # (config/SimulationConfig.hpp:148) for scenes that do not set it; its
+    # YAML scenes set 0.95
+    rho: float = 0.5
     steps: int = 100
     # collision margin (SimulationParams::collision_margin); when set, every
     # step runs the triangle narrow phase against rods and spheres and keeps
@@ -389,6 +392,10 @@ def gaussian_force_field(n_vertices: int, sigma: float = 10.0, seed: int = 0) ->
 
 
 def algorithmic_bytes_per_step(n_v: int, n_tet: int, n_g: int, iters: int) -> int:
-    """FP32 gather model of one time step:
-    I*(112 N_v + 576 N_tet + 136 N_g) + 156 N_v + 136 N_tet + 100 N_g."""
-    return iters * (112 * n_v + 576 * n_tet + 136 * n_g) + 156 * n_v + 136 * n_tet + 100 * n_g
+    """FP32 gather model of one time step: per iteration the vertex-block
+    solve (40 B per vertex, 100 B per tet incidence, four per tet, 48 B per
+    ground record), Chebyshev (72 B per vertex) and dual update (88 B per
+    contact); per step inertial (84 B per vertex), velocity (72 B per vertex),
+    E_prev (136 B per tet) and contact update (100 B per contact):
+    I*(112 N_v + 400 N_tet + 136 N_g) + 156 N_v + 136 N_tet + 100 N_g."""
+    return iters * (112 * n_v + 400 * n_tet + 136 * n_g) + 156 * n_v + 136 * n_tet + 100 * n_g

@@ -377,6 +377,17 @@ class SimulationContext:
         self._chk(fn(self.handle, _dptr(buf), cap, C.byref(n)))
         return buf[:min(cap, n.value)].copy()
 
+    def warm_start_decision(self):
+        """(ip_delta [scenes, 3], step [scenes]) of the last warm-started step:
+        the incremental-potential change at step lengths 1, 1/2, 1/4 along
+        the prediction, and the chosen length (0 = inertial start kept)."""
+        n = C.c_uint32()
+        self._chk(lib.avbd_get_warm_start_decision(self.handle, None, None, 0, C.byref(n)))
+        ipd = np.zeros((n.value, 3))
+        st = np.zeros(n.value, dtype=np.float32)
+        self._chk(lib.avbd_get_warm_start_decision(self.handle, _dptr(ipd), _fptr(st), n.value, C.byref(n)))
+        return ipd, st
+
     def scene_iterations(self) -> np.ndarray:
         """Iterations each scene ran in the last step (residual-targeted mode)."""
         n = C.c_uint32()

@@ -74,25 +74,25 @@ def free_counts(sc):
 
 def solve_bytes_per_launch(sc, n_colors: int, fp64: bool) -> float:
     """Average algorithmic bytes of one colour-phase launch of the vertex-block
-    solve.  FP32 state: 40 B own state + 144 B per incident tet (4 B incidence
-    + 104 B record + 3 x 12 B neighbour positions) + 48 B per ground record.
-    FP64 state: 80 B own state (x read + write 2 x 24 B, y 24 B, mass 8 B) +
-    136 B per incident tet (64 B record + 3 x 24 B neighbour positions) + 48 B
-    per ground record."""
+    solve, with the same terms in both state precisions: own state (x read
+    + write, y, mass), per incident tet its 64 B record (indices, rest shape,
+    material) and the three other corners' positions, 48 B per ground record.
+    FP32 state: 40 B + 100 B per incidence; FP64 state: 80 B + 136 B."""
     n_items, n_inc, n_g = free_counts(sc)
     if fp64:
         total = 80 * n_items + 136 * n_inc + 48 * n_g
     else:
-        total = 40 * n_items + 144 * n_inc + 48 * n_g
+        total = 40 * n_items + 100 * n_inc + 48 * n_g
     return total / n_colors
 
 
-def residual_leg(workload: str, args, launch: dict, params=None):
+def residual_leg(workload: str, args, launch: dict, params=None, budget=None):
     """Every step iterated to the residual target; `warmup` untimed steps,
     then `steps` steps each timed from launch to completion."""
     from avbd_amd.sim import Simulation
     t0 = time.perf_counter()
-    sim = Simulation(make_scene(workload), residual_target=(args.tol, args.max_iters, args.check_every),
+    budget = budget or args.max_iters
+    sim = Simulation(make_scene(workload), residual_target=(args.tol, budget, args.check_every),
                      use_graph=not args.res_eager, fp64_state=not args.fp32, **launch)
     sim.setup()
     if params is not None:
@@ -104,6 +104,8 @@ def residual_leg(workload: str, args, launch: dict, params=None):
     s0 = sim.ctx.stats()
     per_ms, per_it = [], []
     prev_it = s0.iterations_total
+    multi = len(sim.scene.meshes) > 1
+    scene_budget_hits, scene_iters = 0, 0
     for _ in range(args.steps):
         t = time.perf_counter()
         sim.ctx.step(1)
@@ -112,6 +114,10 @@ def residual_leg(workload: str, args, launch: dict, params=None):
         st = sim.ctx.stats()
         per_it.append(int(st.iterations_total - prev_it))
         prev_it = st.iterations_total
+        if multi:
+            its = sim.ctx.scene_iterations()
+            scene_budget_hits += int((its >= budget).sum())
+            scene_iters += int(its.sum())
     s1 = sim.ctx.stats()
     total_s = sum(per_ms) * 1e-3
     out = {
@@ -126,7 +132,11 @@ def residual_leg(workload: str, args, launch: dict, params=None):
         "last_step_residual_history": [float(f"{v:.4g}") for v in sim.ctx.residual_history()][-40:],
         "setup_s": round(t_setup, 2),
     }
-    if len(sim.scene.meshes) > 1:
+    if multi:
+        n_sc = len(sim.scene.meshes)
+        out["scene_steps_at_budget"] = scene_budget_hits
+        out["scene_steps"] = n_sc * args.steps
+        out["mean_scene_iterations"] = round(scene_iters / (n_sc * args.steps), 2)
         out["last_step_worst_scene_history"] = [float(f"{v:.4g}") for v in
                                                 sim.ctx.residual_history(worst_scene=True)][-40:]
     if params is not None:
@@ -147,16 +157,16 @@ def warm_params(args):
     return warm_start.load(wpath), rel
 
 
-def headline_pair(workload: str, args, launch: dict):
+def headline_pair(workload: str, args, launch: dict, budget=None):
     """Plain and warm-started residual legs; returns (plain, warm, the warm
     leg's simulation for kernel timing)."""
-    sim_p, plain = residual_leg(workload, args, launch)
+    sim_p, plain = residual_leg(workload, args, launch, budget=budget)
     log(f"{workload} residual leg (plain): {plain['steps_per_s']:.2f} steps/s, mean {plain['mean_iterations']} "
         f"iterations, {plain['steps_at_budget']} at budget")
     params, wrel = warm_params(args)
     warm, sim_w = None, None
     if params is not None:
-        sim_w, warm = residual_leg(workload, args, launch, params)
+        sim_w, warm = residual_leg(workload, args, launch, params, budget=budget)
         warm["weights"] = wrel
         log(f"{workload} residual leg (warm start): {warm['steps_per_s']:.2f} steps/s, mean "
             f"{warm['mean_iterations']} iterations, {warm['fallbacks_to_inertial_start']} fallbacks")
@@ -267,7 +277,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU work of the baseline")
     ap.add_argument("--tol", type=float, default=1e-4, help="relative force residual target")
-    ap.add_argument("--max-iters", type=int, default=20000, help="iteration budget per step")
+    ap.add_argument("--max-iters", type=int, default=50000, help="iteration budget per step")
+    ap.add_argument("--batch-max-iters", type=int, default=20000, help="iteration budget per step of the batch legs")
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--fp32", action="store_true", help="FP32 particle state in the residual legs")
     ap.add_argument("--fixed-steps", type=int, default=200, help="timed steps of the fixed-iteration leg")
@@ -373,11 +384,12 @@ def main():
     if not dynamic and not args.no_batch and args.batch_workload != args.workload:
         bsc = make_scene(args.batch_workload)
         log(f"batch {bsc.name}: {len(bsc.meshes)} scenes, {bsc.n_vertices} vertices, {bsc.n_tets} tets")
-        bplain, bwarm, bsim = headline_pair(args.batch_workload, args, launch)
+        bplain, bwarm, bsim = headline_pair(args.batch_workload, args, launch, budget=args.batch_max_iters)
         if bsim is not None:
             bsim.ctx.close()
         batch_res = {"workload": bsc.name, "scenes": len(bsc.meshes), "vertices": bsc.n_vertices,
-                     "tets": bsc.n_tets, "tol": args.tol, "rule": "every scene reaches the tolerance on its own",
+                     "tets": bsc.n_tets, "tol": args.tol, "max_iters": args.batch_max_iters,
+                     "rule": "every scene reaches the tolerance on its own; a scene that has stops iterating",
                      "plain": bplain, "warm_start": bwarm,
                      "speedup_warm_vs_plain": round(bwarm["steps_per_s"] / bplain["steps_per_s"], 3) if bwarm else None}
 

@@ -467,6 +467,12 @@ int avbd_record_steps(avbd_ctx* ctx, uint32_t n_steps, float* feats, float* tgts
 /* Start every step from the network's prediction (with the per-scene
  * incremental-potential fallback) or, with enable = 0, from the inertial guess. */
 int avbd_set_warm_start(avbd_ctx* ctx, int enable);
+/* The warm start's choice in the last step, per scene (new): ip_delta[3 s + j]
+ * = IP(x0 + a_j p) - IP(x0) for the step lengths a = 1, 1/2, 1/4 along the
+ * prediction p from the inertial start x0, and step[s] = the a whose change
+ * is the most negative, or 0 when none is negative (the inertial start is
+ * kept).  *n_scenes = scenes (at most cap are written). */
+int avbd_get_warm_start_decision(avbd_ctx* ctx, double* ip_delta, float* step, uint32_t cap, uint32_t* n_scenes);
 /* Test hook: C (M x N) = A (M x K) B (K x N) through the BF16 MFMA GEMM
  * kernel, with A (ta) or B (tb) stored transposed on the device; FP32 row-major
  * host arrays, sizes multiples of 16. */

@@ -226,3 +226,53 @@ def test_batched_warm_start_matches_single_scenes():
         off += n
     banner(f"  {len(sc.meshes)} scenes: batch fallbacks {sb.warm_start_fallbacks}, single runs {fallbacks}")
     assert sb.warm_start_fallbacks == fallbacks
+
+
+def test_warm_start_on_the_32_cube_at_its_workload():
+    """The learned warm start at its workload: the 32^3 dropped cube (35,937
+    vertices), FP64 state, every step iterated to a relative force residual
+    of 1e-4, the committed network, 24 steps from the drop (free fall, the
+    impact and the first steps on the ground).  Every step predicts; the
+    prediction is kept at the step length whose incremental-potential change
+    is the most negative, and the inertial start is kept exactly when no
+    length lowers it; every step meets the target; the warm run stays with
+    the plain one; the iteration counts of both are reported."""
+    banner("warm start at its workload: 32^3 cube, residual 1e-4")
+    from avbd_amd import scenes
+    from avbd_amd import warm_start as W
+    from avbd_amd.sim import Simulation
+    params = W.load()
+    sc = scenes.dropped_cube(n=32)
+    tol, budget, K, steps = 1e-4, 40000, 10, 24
+    runs = {}
+    for warm in (False, True):
+        sim = Simulation(sc, fp64_state=True, residual_target=(tol, budget, K))
+        sim.setup()
+        if warm:
+            sim.ctx.mlp_set_params(params)
+            sim.ctx.set_warm_start(True)
+        its, used = [], 0
+        for k in range(steps):
+            sim.update(1)
+            st = sim.ctx.stats()
+            its.append(st.last_step_iterations)
+            assert st.residual_rel <= tol * 1.01, (warm, k, st.residual_rel)
+            if warm:
+                ipd, step = sim.ctx.warm_start_decision()
+                best = ipd[0].min()
+                if best < 0.0:
+                    assert step[0] == (1.0, 0.5, 0.25)[int(np.argmin(ipd[0]))], (k, ipd, step)
+                    used += 1
+                else:
+                    assert step[0] == 0.0, (k, ipd, step)
+        st = sim.ctx.stats()
+        assert st.steps_at_budget == 0
+        if warm:
+            assert st.warm_start_predictions == steps
+            assert st.warm_start_fallbacks == steps - used
+        runs[warm] = (sim.positions(), its, used)
+    xp, ip, _ = runs[False]
+    xw, iw, used = runs[True]
+    banner(f"  iterations plain {sum(ip)} warm {sum(iw)} (gain {sum(ip) / sum(iw):.3f}); "
+           f"prediction kept in {used} of {steps} steps; per step plain {ip} warm {iw}")
+    assert np.abs(xw - xp).max() < 1e-3 * bbox(xp)
