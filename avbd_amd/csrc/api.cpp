@@ -2528,7 +2528,9 @@ int avbd_record_steps(avbd_ctx* c, uint32_t n_steps, float* feats, float* tgts, 
         std::vector<float> raw((size_t)ni * kMlpIn), tg((size_t)ni * 3);
         try {
             for (uint32_t st = 0; st < n_steps; st++) {
-                enqueue_step(c, false);
+                // residual mode: intervals stop being queued once converged
+                if (c->rc) run_residual_step(c, false);
+                else enqueue_step(c, false);
                 hip_check(hipMemcpyAsync(raw.data(), c->d_raw, raw.size() * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
                 hip_check(hipMemcpyAsync(tg.data(), c->d_tgt, tg.size() * 4, hipMemcpyDeviceToHost, c->stream), "D2H");
                 hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");

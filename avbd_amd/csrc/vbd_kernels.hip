@@ -1608,9 +1608,19 @@ uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bo
 namespace avbd {
 
 namespace {
+// Plain device copy with four independent 16-byte loads in flight per thread
+// per trip (all issued before the stores), then a one-load tail.
 __global__ void calib_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const float4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
 }
 }  // namespace
 

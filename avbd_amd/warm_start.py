@@ -51,22 +51,35 @@ def training_scenes(n_scenes: int, seed: int, n_min: int = 8, n_max: int = 14, l
 
 
 def generate(scene_list, steps: int, tol: float, max_iters: int, check_every: int = 10, keep: float = 0.25,
-             seed: int = 0, log=print) -> Tuple[np.ndarray, np.ndarray]:
-    """Run every scene for `steps` steps with the plain solver iterating to the
-    residual target, recording features and targets; keep a random fraction
-    of the rows."""
+             seed: int = 0, log=print, fp64: bool = False, per_batch: int = 1,
+             chunk: int = 16) -> Tuple[np.ndarray, np.ndarray]:
+    """Run the scenes for `steps` steps with the plain solver iterating every
+    step to the residual target, recording features and targets; keep a
+    random fraction of the rows.  `per_batch` scenes run together as one
+    batch (each still stops at its own target); the steps are recorded
+    `chunk` at a time."""
     from .sim import Simulation
     rng = np.random.default_rng(seed)
     Xs, Ts = [], []
-    for k, sc in enumerate(scene_list):
-        sim = Simulation(sc, residual_target=(tol, max_iters, check_every))
+    for b0 in range(0, len(scene_list), per_batch):
+        group = scene_list[b0:b0 + per_batch]
+        sc = group[0] if len(group) == 1 else scenes.Scene(
+            name=f"train_batch_{b0}", meshes=[m for g in group for m in g.meshes], iters=group[0].iters,
+            rho=group[0].rho, dt=group[0].dt, g=group[0].g, steps=steps)
+        sim = Simulation(sc, residual_target=(tol, max_iters, check_every), fp64_state=fp64)
         sim.setup()
         n_items = int(sum(int((m.fixed == 0).sum()) if m.fixed is not None else m.rest.shape[0] for m in sc.meshes))
-        X, T = sim.ctx.record_steps(steps, n_items)
-        sel = rng.random(X.shape[0]) < keep
-        Xs.append(X[sel])
-        Ts.append(T[sel])
-        log(f"[warm-start] scene {k}: {sc.n_vertices} vertices, {steps} steps, {int(sel.sum())} rows kept")
+        kept, its, at_budget = 0, 0, 0
+        for k0 in range(0, steps, chunk):
+            n = min(chunk, steps - k0)
+            X, T = sim.ctx.record_steps(n, n_items)
+            sel = rng.random(X.shape[0]) < keep
+            Xs.append(X[sel])
+            Ts.append(T[sel])
+            kept += int(sel.sum())
+        st = sim.ctx.stats()
+        log(f"[warm-start] scenes {b0}..{b0 + len(group) - 1}: {sc.n_vertices} vertices, {steps} steps, "
+            f"{st.iterations_total} iterations, {st.steps_at_budget} steps at the budget, {kept} rows kept")
         sim.ctx.close()
     return np.concatenate(Xs), np.concatenate(Ts)
 
@@ -102,20 +115,22 @@ def load(path: Path = DEFAULT_PATH) -> np.ndarray:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(DEFAULT_PATH))
-    ap.add_argument("--scenes", type=int, default=16)
-    ap.add_argument("--steps", type=int, default=80)
-    ap.add_argument("--keep", type=float, default=0.3)
-    ap.add_argument("--tol", type=float, default=1e-2)
-    ap.add_argument("--max-iters", type=int, default=400)
+    ap.add_argument("--scenes", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=160)
+    ap.add_argument("--per-batch", type=int, default=64, help="scenes stepped together as one batch")
+    ap.add_argument("--keep", type=float, default=0.1)
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--max-iters", type=int, default=4000)
+    ap.add_argument("--fp32", action="store_true", help="FP32 particle state while generating (default FP64)")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--n-min", type=int, default=16)
-    ap.add_argument("--n-max", type=int, default=24)
-    ap.add_argument("--log-e", type=float, nargs=2, default=(5.7, 6.3))
-    ap.add_argument("--nu", type=float, nargs=2, default=(0.35, 0.45))
-    ap.add_argument("--height", type=float, nargs=2, default=(0.2, 0.6))
+    ap.add_argument("--n-min", type=int, default=8)
+    ap.add_argument("--n-max", type=int, default=16)
+    ap.add_argument("--log-e", type=float, nargs=2, default=(5.5, 6.5))
+    ap.add_argument("--nu", type=float, nargs=2, default=(0.3, 0.45))
+    ap.add_argument("--height", type=float, nargs=2, default=(0.1, 0.6))
     ap.add_argument("--field-sigma", type=float, default=10.0,
                     help="standard deviation (N/kg) of the Gaussian per-vertex force fields")
     ap.add_argument("--field-fraction", type=float, default=0.5, help="fraction of scenes that carry a field")
@@ -124,7 +139,8 @@ def main(argv=None):
     t0 = time.perf_counter()
     sc = training_scenes(args.scenes, args.seed, args.n_min, args.n_max, tuple(args.log_e), tuple(args.nu),
                          tuple(args.height), args.field_sigma, args.field_fraction)
-    X, T = generate(sc, args.steps, args.tol, args.max_iters, keep=args.keep, log=log)
+    X, T = generate(sc, args.steps, args.tol, args.max_iters, keep=args.keep, log=log, fp64=not args.fp32,
+                    per_batch=args.per_batch)
     log(f"[warm-start] {X.shape[0]} rows in {time.perf_counter() - t0:.1f}s")
     from .sim import SimulationContext
     ctx = SimulationContext(dt=1 / 60, solver_iters=1)
@@ -136,7 +152,8 @@ def main(argv=None):
     save(Path(args.out), params, rows=X.shape[0], loss=loss, tol=args.tol, scenes=args.scenes, steps=args.steps,
          max_iters=args.max_iters, epochs=args.epochs, seed=args.seed, n_range=(args.n_min, args.n_max),
          log_e=tuple(args.log_e), nu=tuple(args.nu), height=tuple(args.height), keep=args.keep,
-         field_sigma=args.field_sigma, field_fraction=args.field_fraction)
+         field_sigma=args.field_sigma, field_fraction=args.field_fraction, fp64=not args.fp32,
+         per_batch=args.per_batch, scene_steps=args.scenes * args.steps)
     log(f"[warm-start] wrote {args.out}")
     return 0
 

@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--check-every", type=int, default=10)
     ap.add_argument("--fp32", action="store_true", help="FP32 particle state in the residual legs")
     ap.add_argument("--fixed-steps", type=int, default=200, help="timed steps of the fixed-iteration leg")
+    ap.add_argument("--no-fixed", action="store_true", help="skip the fixed-iteration leg")
+    ap.add_argument("--no-residual", action="store_true", help="skip the residual-targeted legs of the workload")
     ap.add_argument("--lanes-per-vertex", type=int, default=0, help="0: autotuned at set-up")
     ap.add_argument("--block-size", type=int, default=0, help="0: autotuned at set-up")
     ap.add_argument("--gather-batch", type=int, default=0, help="0: autotuned at set-up")
@@ -291,6 +293,9 @@ def main():
                     help="trained warm-start parameters (.npz); default: avbd_amd/data/warm_start_mlp.npz")
     ap.add_argument("--calibrate", type=int, default=0,
                     help="also run one plain copy kernel of this many bytes (HBM counter calibration)")
+    ap.add_argument("--solve-only", type=int, default=0,
+                    help="profiling run: set up the headline workload, take the warm-up steps, time this many "
+                         "rounds of its colour-phase solve launches (the roofline's kernel) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the solver on a single MI355X "
@@ -305,9 +310,28 @@ def main():
     dynamic = sc.contact_margin is not None
     fp64 = not args.fp32
 
+    if args.solve_only:
+        sim = Simulation(sc, residual_target=(args.tol, args.max_iters, args.check_every), fp64_state=fp64,
+                         use_graph=False, **launch)
+        sim.setup()
+        sim.ctx.step(args.warmup)
+        sim.ctx.synchronize()
+        avg_ms, launches = sim.ctx.time_solve_launches(args.solve_only)
+        if args.calibrate:
+            ms = sim.ctx.calibrate_copy(args.calibrate, 10)
+            log(f"copy kernel: {args.calibrate} bytes in {ms:.4f} ms ({2 * args.calibrate / (ms * 1e-3) / 1e9:.1f} GB/s "
+                f"read + write)")
+        _, n_colors = sim.ctx.coloring()
+        lanes, block, batch = sim.ctx.launch_config()
+        nb = solve_bytes_per_launch(sc, n_colors, fp64)
+        log(f"solve launches: {launches} x {avg_ms * 1e3:.3f} us, {nb:.0f} algorithmic bytes each "
+            f"({nb / (avg_ms * 1e-3) / 1e9:.1f} GB/s); lanes {lanes} block {block} gather batch {batch}")
+        sim.ctx.close()
+        return
+
     # headline: residual-targeted legs
     res, roof, sim_w = None, None, None
-    if not dynamic:
+    if not dynamic and not args.no_residual:
         plain, warm, sim_w = headline_pair(args.workload, args, launch)
         best = warm if warm is not None else plain
         res = {"tol": args.tol, "max_iters": args.max_iters, "check_every": args.check_every,
@@ -328,56 +352,66 @@ def main():
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                     "avg_launch_us": round(avg_ms * 1e3, 3), "algorithmic_bytes_per_launch": int(bytes_launch),
                     "launches_timed": launches}
+            # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE
+            # runs, profiles/), used only when they were taken on the variant
+            # timed here
+            lanes, _, batch = ksim.ctx.launch_config()
+            damp = any(m.kd > 0 for m in sc.meshes)
+            variant = (f"solve_color_f64_kernel<{lanes}, {str(damp).lower()}, {batch}>" if fp64 else
+                       f"solve_color_kernel<{lanes}, {str(damp).lower()}, false, {batch}>")
+            roof["variant"] = variant
             prof = ROOT / "profiles" / f"{kernel}_{sc.name}_traffic.json"
             if prof.exists():
-                try:
-                    roof["traffic"] = json.loads(prof.read_text()).get("bytes_per_launch")
-                except Exception:
-                    pass
+                t = json.loads(prof.read_text())
+                if t.get("variant") == variant:
+                    roof["traffic"] = t.get("bytes_per_launch")
+                    roof["traffic_source"] = t.get("source")
             ksim.ctx.close()
 
     # the reference's fixed iteration count, FP32 state
-    t_setup = time.perf_counter()
-    sim = Simulation(sc, **launch)
-    sim.setup()
-    t_setup = time.perf_counter() - t_setup
-    sim.ctx.step(args.warmup)
-    sim.ctx.synchronize()
-    t0 = time.perf_counter()
-    sim.ctx.step(args.fixed_steps)
-    sim.ctx.synchronize()
-    el = time.perf_counter() - t0
-    st = sim.ctx.stats()
-    fixed_rate = args.fixed_steps / el
-    vups = st.vertex_updates / max(st.steps_taken, 1) * fixed_rate
-    sim.ctx.set_timing(True)
-    sim.ctx.step(2)
-    lbvh_ms = sim.ctx.broadphase_timing()
-    sim.ctx.set_timing(False)
-    f_avg_ms, f_launches = sim.ctx.time_solve_launches(20)
-    _, n_colors = sim.ctx.coloring()
-    f_bytes = solve_bytes_per_launch(sc, n_colors, False)
-    lanes, block, batch = sim.ctx.launch_config()
-    fixed = {"iterations": sc.iters, "state": "fp32", "steps": args.fixed_steps,
-             "steps_per_s": round(fixed_rate, 3), "ms_per_step": round(el / args.fixed_steps * 1e3, 4),
-             "vertex_updates_per_s": round(vups, 1), "final_residual_rel": st.residual_rel,
-             "setup_s": round(t_setup, 2),
-             "solve_kernel": {"kernel": "solve_color_kernel", "avg_launch_us": round(f_avg_ms * 1e3, 3),
-                              "algorithmic_bytes_per_launch": int(f_bytes),
-                              "achieved_GBps": round(f_bytes / (f_avg_ms * 1e-3) / 1e9, 1) if f_avg_ms > 0 else 0},
-             "launch_config": {"lanes_per_vertex": lanes, "block_size": block, "gather_batch": batch,
-                               "fused_chebyshev": sim.ctx.fused_chebyshev()}}
-    log(f"fixed-count leg: {fixed_rate:.1f} steps/s, residual_rel {st.residual_rel:.3e}")
-    if roof is None and f_launches:
-        roof = {"kernel": "solve_color_kernel", "bound": "hbm", "achieved": fixed["solve_kernel"]["achieved_GBps"],
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fixed["solve_kernel"]["achieved_GBps"] / HBM_PEAK_GBS, 4),
-                "traffic": None, "avg_launch_us": round(f_avg_ms * 1e3, 3),
-                "algorithmic_bytes_per_launch": int(f_bytes), "launches_timed": f_launches}
-    calib = None
-    if args.calibrate:
-        ms = sim.ctx.calibrate_copy(args.calibrate, 1)
-        calib = {"bytes": args.calibrate, "avg_ms": ms, "GBps_read_plus_write": 2 * args.calibrate / (ms * 1e-3) / 1e9}
-    sim.ctx.close()
+    fixed, calib, lbvh_ms = None, None, 0.0
+    if not args.no_fixed:
+        t_setup = time.perf_counter()
+        sim = Simulation(sc, **launch)
+        sim.setup()
+        t_setup = time.perf_counter() - t_setup
+        sim.ctx.step(args.warmup)
+        sim.ctx.synchronize()
+        t0 = time.perf_counter()
+        sim.ctx.step(args.fixed_steps)
+        sim.ctx.synchronize()
+        el = time.perf_counter() - t0
+        st = sim.ctx.stats()
+        fixed_rate = args.fixed_steps / el
+        vups = st.vertex_updates / max(st.steps_taken, 1) * fixed_rate
+        sim.ctx.set_timing(True)
+        sim.ctx.step(2)
+        lbvh_ms = sim.ctx.broadphase_timing()
+        sim.ctx.set_timing(False)
+        f_avg_ms, f_launches = sim.ctx.time_solve_launches(20)
+        _, n_colors = sim.ctx.coloring()
+        f_bytes = solve_bytes_per_launch(sc, n_colors, False)
+        lanes, block, batch = sim.ctx.launch_config()
+        fixed = {"iterations": sc.iters, "state": "fp32", "steps": args.fixed_steps,
+                 "steps_per_s": round(fixed_rate, 3), "ms_per_step": round(el / args.fixed_steps * 1e3, 4),
+                 "vertex_updates_per_s": round(vups, 1), "final_residual_rel": st.residual_rel,
+                 "setup_s": round(t_setup, 2),
+                 "solve_kernel": {"kernel": "solve_color_kernel", "avg_launch_us": round(f_avg_ms * 1e3, 3),
+                                  "algorithmic_bytes_per_launch": int(f_bytes),
+                                  "achieved_GBps": round(f_bytes / (f_avg_ms * 1e-3) / 1e9, 1) if f_avg_ms > 0 else 0},
+                 "launch_config": {"lanes_per_vertex": lanes, "block_size": block, "gather_batch": batch,
+                                   "fused_chebyshev": sim.ctx.fused_chebyshev()}}
+        log(f"fixed-count leg: {fixed_rate:.1f} steps/s, residual_rel {st.residual_rel:.3e}")
+        if roof is None and f_launches:
+            roof = {"kernel": "solve_color_kernel", "bound": "hbm", "achieved": fixed["solve_kernel"]["achieved_GBps"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fixed["solve_kernel"]["achieved_GBps"] / HBM_PEAK_GBS, 4),
+                    "traffic": None, "avg_launch_us": round(f_avg_ms * 1e3, 3),
+                    "algorithmic_bytes_per_launch": int(f_bytes), "launches_timed": f_launches}
+        calib = None
+        if args.calibrate:
+            ms = sim.ctx.calibrate_copy(args.calibrate, 10)
+            calib = {"bytes": args.calibrate, "avg_ms": ms, "GBps_read_plus_write": 2 * args.calibrate / (ms * 1e-3) / 1e9}
+        sim.ctx.close()
 
     # the same legs on the scene batch
     batch_res = None
@@ -399,8 +433,8 @@ def main():
         cpu = cpu_baseline(sc, args, res["plain"] if res else None)
 
     n_tris = sum(len(m.triangles) for m in sc.meshes if m.triangles is not None) if len(sc.meshes) == 1 else 0
-    head_value = res["steps_per_s"] if res else round(fixed_rate, 3)
-    head_ms = res["ms_per_step"] if res else round(el / args.fixed_steps * 1e3, 4)
+    head_value = res["steps_per_s"] if res else fixed["steps_per_s"]
+    head_ms = res["ms_per_step"] if res else fixed["ms_per_step"]
     out = {
         "metric": METRIC,
         "value": head_value,

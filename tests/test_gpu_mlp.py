@@ -236,7 +236,8 @@ def test_warm_start_on_the_32_cube_at_its_workload():
     prediction is kept at the step length whose incremental-potential change
     is the most negative, and the inertial start is kept exactly when no
     length lowers it; every step meets the target; the warm run stays with
-    the plain one; the iteration counts of both are reported."""
+    the plain one and needs fewer iterations in total (the committed network
+    saves about 7 % here, most of it on the hardest contact step)."""
     banner("warm start at its workload: 32^3 cube, residual 1e-4")
     from avbd_amd import scenes
     from avbd_amd import warm_start as W
@@ -246,7 +247,8 @@ def test_warm_start_on_the_32_cube_at_its_workload():
     tol, budget, K, steps = 1e-4, 40000, 10, 24
     runs = {}
     for warm in (False, True):
-        sim = Simulation(sc, fp64_state=True, residual_target=(tol, budget, K))
+        sim = Simulation(sc, fp64_state=True, residual_target=(tol, budget, K), lanes_per_vertex=8, block_size=64,
+                         gather_batch=1)
         sim.setup()
         if warm:
             sim.ctx.mlp_set_params(params)
@@ -276,3 +278,4 @@ def test_warm_start_on_the_32_cube_at_its_workload():
     banner(f"  iterations plain {sum(ip)} warm {sum(iw)} (gain {sum(ip) / sum(iw):.3f}); "
            f"prediction kept in {used} of {steps} steps; per step plain {ip} warm {iw}")
     assert np.abs(xw - xp).max() < 1e-3 * bbox(xp)
+    assert sum(iw) < sum(ip)
