@@ -17,7 +17,7 @@ INCLUDE = PKG.parent / "include"
 
 SOURCES = ["api.cpp", "host_graph.cpp", "contacts_host.cpp", "mlp_host.cpp", "vbd_kernels.hip", "lbvh_kernels.hip",
            "contact_kernels.hip", "mlp_kernels.hip", "lbvh_static.hip", "vbd_fp64.hip",
-           "graph_kernels.hip"]
+           "graph_kernels.hip", "recolour_kernels.hip"]
 # FP64 Morton codes must round exactly like the reference's CPU code: no FMA
 # contraction in the LBVH kernels.  The solver kernels contract only within
 # one expression (-ffp-contract=on), so a device function rounds the same way

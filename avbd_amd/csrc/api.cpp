@@ -126,11 +126,13 @@ struct avbd_ctx {
     DetRec* d_det = nullptr;
     uint32_t det_cap = 0;
     uint32_t* d_status = nullptr;  // [0] narrow-phase flags, [1] detected count
+    RecolourDev rdev;              // device recolouring and work-list layout
+    uint32_t* h_layout = nullptr;  // pinned: the 3 x 257 per-colour offsets, then the 4 recolouring counters
     struct Grow {                  // device buffer that grows on demand
         void* p = nullptr;
         size_t bytes = 0;
     };
-    Grow g_ops, g_active, g_off, g_ent, g_conf;
+    Grow g_ops, g_active, g_off, g_ent, g_nboff, g_nb, g_seeds;
     // pinned staging for a detection round's uploads (written only after the
     // round's stream synchronisation, so one buffer serves every round)
     uint8_t* stage = nullptr;
@@ -785,6 +787,32 @@ void stage_upload(avbd_ctx* c, T* dst, const std::vector<T>& src) {
 // for the lifecycle, the dynamic adjacency and the recolouring; the
 // resulting record ops, adjacency and (when the colouring changed) work
 // items go back before the solve.
+// The per-colour offsets of the work list the device laid out, and the
+// colour count (synchronising): the host sizes the colour-phase launches by
+// them.
+void read_layout(avbd_ctx* c) {
+    hipStream_t s = c->stream;
+    hip_check(hipMemcpyAsync(c->h_layout, c->rdev.offsets, 3 * 257 * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+              "D2H");
+    hip_check(hipMemcpyAsync(c->h_layout + 3 * 257, c->rdev.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+              "D2H");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    const uint32_t* cnt = c->h_layout + 3 * 257;
+    if (cnt[3] & 1u) throw CodeError(AVBD_E_COLORS, "Ran out of colors!");
+    const uint32_t K = cnt[2];
+    ColorPlan& plan = c->plan;
+    plan.num_colors = K;
+    plan.item_offsets.assign(c->h_layout, c->h_layout + K + 1);
+    plan.oitem_offsets.assign(c->h_layout + 257, c->h_layout + 257 + K + 1);
+    plan.conflict_offsets.assign(c->h_layout + 514, c->h_layout + 514 + K + 1);
+    const size_t need = 2 * (size_t)c->iters() * K;
+    while (c->events.size() < need) {
+        hipEvent_t ev;
+        hip_check(hipEventCreate(&ev), "hipEventCreate");
+        c->events.push_back(ev);
+    }
+}
+
 void detect_round(avbd_ctx* c) {
     hipStream_t s = c->stream;
     const double dt = c->desc.dt;
@@ -806,16 +834,16 @@ void detect_round(avbd_ctx* c) {
         hip_check(hipMemcpyAsync(det.data(), c->d_det, det.size() * sizeof(DetRec), hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     }
-    const bool relayout = c->cs.process(det, c->prims, c->mass, dt, c->plan);
+    c->cs.process(det, c->prims, c->mass, dt);
     ContactSystem& cs = c->cs;
     // the stream is idle here (synchronised above): reserve the staging for
-    // every upload of this round, then queue them; the solve follows on the
-    // same stream, so nothing waits for the copies on the host
-    size_t total = stage_bytes(cs.ops.size() * sizeof(DynOp)) + stage_bytes(cs.active.size() * sizeof(uint32_t)) +
-                   stage_bytes(cs.dyn_off.size() * sizeof(uint32_t)) + stage_bytes(cs.dyn_ent.size() * sizeof(uint32_t));
-    if (relayout)
-        total += stage_bytes(c->plan.items.size() * sizeof(VItem)) + stage_bytes(c->plan.oitems.size() * sizeof(OItem)) +
-                 stage_bytes(c->plan.conflict_list.size() * sizeof(uint32_t));
+    // every upload of this round, then queue them; the recolouring and the
+    // solve follow on the same stream
+    const size_t total = stage_bytes(cs.ops.size() * sizeof(DynOp)) + stage_bytes(cs.active.size() * sizeof(uint32_t)) +
+                         stage_bytes(cs.dyn_off.size() * sizeof(uint32_t)) +
+                         stage_bytes(cs.dyn_ent.size() * sizeof(uint32_t)) +
+                         stage_bytes(cs.nb_off.size() * sizeof(uint32_t)) + stage_bytes(cs.nb.size() * sizeof(uint32_t)) +
+                         stage_bytes(cs.seeds.size() * sizeof(uint32_t));
     stage_reserve(c, total);
     if (!cs.ops.empty()) {
         auto* d_ops = static_cast<DynOp*>(grow(c, c->g_ops, cs.ops.size() * sizeof(DynOp)));
@@ -830,18 +858,19 @@ void detect_round(avbd_ctx* c) {
     stage_upload(c, d_ent, cs.dyn_ent);
     c->d.dyn_off = d_off;
     c->d.dyn_ent = d_ent;
-    if (relayout) {
-        stage_upload(c, c->d.items, c->plan.items);
-        stage_upload(c, c->d.oitems, c->plan.oitems);
-        c->d_conflicts = static_cast<uint32_t*>(grow(c, c->g_conf, c->plan.conflict_list.size() * sizeof(uint32_t)));
-        stage_upload(c, c->d_conflicts, c->plan.conflict_list);
-        const size_t need = 2 * (size_t)c->iters() * c->plan.num_colors;
-        while (c->events.size() < need) {
-            hipEvent_t ev;
-            hip_check(hipEventCreate(&ev), "hipEventCreate");
-            c->events.push_back(ev);
-        }
-    }
+    // incremental recolouring and the new work list, on the device
+    auto* d_nboff = static_cast<uint32_t*>(grow(c, c->g_nboff, cs.nb_off.size() * sizeof(uint32_t)));
+    stage_upload(c, d_nboff, cs.nb_off);
+    auto* d_nb = static_cast<uint32_t*>(grow(c, c->g_nb, std::max<size_t>(cs.nb.size(), 1) * sizeof(uint32_t)));
+    stage_upload(c, d_nb, cs.nb);
+    auto* d_seeds = static_cast<uint32_t*>(grow(c, c->g_seeds, std::max<size_t>(cs.seeds.size(), 1) * sizeof(uint32_t)));
+    stage_upload(c, d_seeds, cs.seeds);
+    c->rdev.d_off = d_nboff;
+    c->rdev.d_nb = d_nb;
+    launch_recolour(s, c->rdev, d_seeds, (uint32_t)cs.seeds.size());
+    launch_relayout(s, c->rdev, c->d.items, c->d.oitems, c->d_conflicts, false);
+    hip_check(hipGetLastError(), "kernel launch");
+    read_layout(c);
 }
 
 // Autotuner: time the colour-phase solve of one iteration for each candidate
@@ -920,7 +949,6 @@ void autotune(avbd_ctx* c) {
         for (uint32_t U : batch_c) {
             if (c->desc.block_size && B != c->desc.block_size) continue;
             if (c->desc.gather_batch && U != c->desc.gather_batch) continue;
-            if (c->fp64 && U > 2) continue;   // the FP64 solve gathers 1 or 2 incidences at once
             LaunchCfg cfg{L, B, U};
             auto run = [&] {
                 for (uint32_t col = 0; col < c->plan.num_colors; col++) {
@@ -1066,6 +1094,7 @@ void avbd_destroy(avbd_ctx* c) {
     for (auto ev : c->events) (void)hipEventDestroy(ev);
     for (auto ev : c->part_ev) (void)hipEventDestroy(ev);
     if (c->h_conv) (void)hipHostFree(c->h_conv);
+    if (c->h_layout) (void)hipHostFree(c->h_layout);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->ev_lb0) (void)hipEventDestroy(c->ev_lb0);
     if (c->ev_lb1) (void)hipEventDestroy(c->ev_lb1);
@@ -2230,9 +2259,16 @@ int avbd_download_energies(avbd_ctx* c, int type, void* records, uint32_t count)
 
 int avbd_download_coloring(avbd_ctx* c, uint32_t* color, uint32_t* num_colors) {
     return guarded(c, [&] {
-        // the colouring is host-side: available once avbd_build_graph() has
-        // planned it, even if the device part of that call failed
+        // the set-up colouring is host-side: available once avbd_build_graph()
+        // has planned it, even if the device part of that call failed; with
+        // dynamic contacts the device recolours every step
         if (!c->planned) throw CodeError(AVBD_E_NOT_BUILT, "avbd_build_graph() has not been called");
+        if (c->dynamic && color) {
+            hip_check(hipMemcpyAsync(c->plan.color.data(), c->rdev.color, c->plan.color.size() * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        }
         if (color) std::copy(c->plan.color.begin(), c->plan.color.end(), color);
         if (num_colors) *num_colors = c->plan.num_colors;
     });
@@ -2671,6 +2707,51 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         upload(c, d_off, c->cs.dyn_off);
         c->d.dyn_off = d_off;
         c->d.dyn_ent = static_cast<uint32_t*>(grow(c, c->g_ent, 16));
+        // device recolouring state: the set-up colouring, the static
+        // adjacency, each particle's work item; nothing dirty or conflicted
+        RecolourDev& r = c->rdev;
+        const ColorPlan& plan = c->plan;
+        r.n = n;
+        r.color = dev_alloc<uint32_t>(c, n);
+        r.last_dirty = dev_alloc<uint32_t>(c, n);
+        r.candidate = dev_alloc<uint32_t>(c, n);
+        r.dirty = dev_alloc<uint32_t>(c, n);
+        r.next = dev_alloc<uint32_t>(c, n);
+        r.touched = dev_alloc<uint32_t>(c, 7 * (size_t)n);
+        r.touched_conf = dev_alloc<uint32_t>(c, n);
+        r.conflicted = dev_alloc<uint32_t>(c, n);
+        r.counters = dev_alloc<uint32_t>(c, 8);
+        r.offsets = dev_alloc<uint32_t>(c, 3 * 257);
+        upload(c, r.color, plan.color);
+        hip_check(hipMemsetAsync(r.last_dirty, 0xFF, (size_t)n * sizeof(uint32_t), c->stream), "hipMemsetAsync");
+        hip_check(hipMemsetAsync(r.conflicted, 0, (size_t)n * sizeof(uint32_t), c->stream), "hipMemsetAsync");
+        const std::vector<uint32_t> cnt0{0u, 0u, plan.num_colors, 0u, 0u, 0u, 0u, 0u};
+        upload(c, r.counters, cnt0);
+        uint32_t* s_off = dev_alloc<uint32_t>(c, plan.static_off.size());
+        uint32_t* s_nb = dev_alloc<uint32_t>(c, std::max<size_t>(plan.static_nb.size(), 1));
+        upload(c, s_off, plan.static_off);
+        upload(c, s_nb, plan.static_nb);
+        r.s_off = s_off;
+        r.s_nb = s_nb;
+        uint8_t* kind = dev_alloc<uint8_t>(c, n);
+        VItem* item_of = dev_alloc<VItem>(c, n);
+        OItem* oitem_of = dev_alloc<OItem>(c, n);
+        upload(c, kind, plan.item_kind);
+        upload(c, item_of, plan.item_of);
+        upload(c, oitem_of, plan.oitem_of);
+        r.kind = kind;
+        r.item_of = item_of;
+        r.oitem_of = oitem_of;
+        c->d_conflicts = dev_alloc<uint32_t>(c, n);   // at most every particle is conflicted
+        if (!c->h_layout) {
+            void* hl = nullptr;
+            hip_check(hipHostMalloc(&hl, (3 * 257 + 4) * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+            c->h_layout = static_cast<uint32_t*>(hl);
+        }
+        // the per-colour offsets of the set-up layout (later rounds keep them
+        // when nothing changes)
+        launch_relayout(c->stream, r, c->d.items, c->d.oitems, c->d_conflicts, true);
+        hip_check(hipGetLastError(), "kernel launch");
         c->dynamic = true;
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     });
@@ -2766,7 +2847,14 @@ std::vector<std::pair<void*, size_t>> ckpt_arrays(avbd_ctx* c) {
     if (d.n_rground) a.emplace_back(d.rground, (size_t)d.n_rground * sizeof(RGroundRec));
     if (d.n_joints) a.emplace_back(d.joints, (size_t)d.n_joints * sizeof(JointRec));
     if (c->rc) a.emplace_back(c->rc, sizeof(ResCtl));
-    if (c->dynamic) a.emplace_back(d.dyn, 2 * (size_t)c->cs.cap * sizeof(DynRec));
+    if (c->dynamic) {
+        a.emplace_back(d.dyn, 2 * (size_t)c->cs.cap * sizeof(DynRec));
+        // the device recolouring state (colours, dirty marks and list, conflicts)
+        const RecolourDev& r = c->rdev;
+        const size_t pu = (size_t)d.n * sizeof(uint32_t);
+        for (uint32_t* v : {r.color, r.last_dirty, r.dirty, r.touched_conf, r.conflicted}) a.emplace_back(v, pu);
+        a.emplace_back(r.counters, 8 * sizeof(uint32_t));
+    }
     if (c->fp64) {
         const size_t p8 = (size_t)d.n * sizeof(double4);
         for (double4* v : {c->f.x, c->f.y, c->f.x_prev, c->f.x_last, c->f.x_lastlast}) a.emplace_back(v, p8);
@@ -2798,12 +2886,7 @@ CkptHeader ckpt_header(avbd_ctx* c) {
 
 std::vector<uint8_t> ckpt_host_tail(avbd_ctx* c) {
     std::vector<uint8_t> t;
-    if (!c->dynamic) return t;
-    const uint64_t nc = c->plan.color.size();
-    t.resize(sizeof(uint64_t) + nc * sizeof(uint32_t));
-    std::memcpy(t.data(), &nc, sizeof(uint64_t));
-    std::memcpy(t.data() + sizeof(uint64_t), c->plan.color.data(), nc * sizeof(uint32_t));
-    c->cs.save(t);
+    if (c->dynamic) c->cs.save(t);
     return t;
 }
 }  // namespace
@@ -2866,17 +2949,8 @@ int avbd_load_checkpoint(avbd_ctx* c, const void* buf, uint64_t size) {
         if ((uint64_t)(end - p) < h.array_bytes) throw CodeError(AVBD_E_ARG, "checkpoint is truncated");
         const uint8_t* arrays = p;
         p += h.array_bytes;
-        std::vector<uint32_t> color;
         ContactSystem cs;
         if (c->dynamic) {
-            uint64_t nc = 0;
-            if ((uint64_t)(end - p) < sizeof(uint64_t)) throw CodeError(AVBD_E_ARG, "checkpoint is truncated");
-            std::memcpy(&nc, p, sizeof(uint64_t));
-            p += sizeof(uint64_t);
-            if (nc != c->plan.color.size() || (uint64_t)(end - p) / sizeof(uint32_t) < nc)
-                throw CodeError(AVBD_E_ARG, "checkpoint does not match this context's particles");
-            color.assign(reinterpret_cast<const uint32_t*>(p), reinterpret_cast<const uint32_t*>(p) + nc);
-            p += nc * sizeof(uint32_t);
             cs = c->cs;
             cs.load(p, end);   // throws on a malformed tail, before anything is applied
         }
@@ -2886,14 +2960,11 @@ int avbd_load_checkpoint(avbd_ctx* c, const void* buf, uint64_t size) {
             arrays += a.second;
         }
         if (c->dynamic) {
-            c->plan.color = std::move(color);
             c->cs = std::move(cs);
-            c->plan.num_colors = h.num_colors;
-            relayout_plan(c->plan, c->cs.conflicted);
-            upload(c, c->d.items, c->plan.items);
-            upload(c, c->d.oitems, c->plan.oitems);
-            c->d_conflicts = static_cast<uint32_t*>(grow(c, c->g_conf, c->plan.conflict_list.size() * sizeof(uint32_t)));
-            upload(c, c->d_conflicts, c->plan.conflict_list);
+            // the work list of the restored colouring, laid out on the device
+            launch_relayout(c->stream, c->rdev, c->d.items, c->d.oitems, c->d_conflicts, true);
+            hip_check(hipGetLastError(), "kernel launch");
+            read_layout(c);
             auto* d_act = static_cast<uint32_t*>(grow(c, c->g_active, c->cs.active.size() * sizeof(uint32_t)));
             upload(c, d_act, c->cs.active);
             auto* d_off = static_cast<uint32_t*>(grow(c, c->g_off, c->cs.dyn_off.size() * sizeof(uint32_t)));
@@ -2902,12 +2973,6 @@ int avbd_load_checkpoint(avbd_ctx* c, const void* buf, uint64_t size) {
             upload(c, d_ent, c->cs.dyn_ent);
             c->d.dyn_off = d_off;
             c->d.dyn_ent = d_ent;
-            const size_t need = 2 * (size_t)c->iters() * c->plan.num_colors;
-            while (c->events.size() < need) {
-                hipEvent_t ev;
-                hip_check(hipEventCreate(&ev), "hipEventCreate");
-                c->events.push_back(ev);
-            }
         }
         c->steps_taken = h.steps_taken;
         c->vertex_updates = h.vertex_updates;
@@ -2921,8 +2986,16 @@ int avbd_download_conflicts(avbd_ctx* c, uint8_t* flags) {
         require_built(c);
         if (!flags) throw CodeError(AVBD_E_ARG, "flags is null");
         const uint32_t n = c->d.n;
-        if (c->dynamic) std::memcpy(flags, c->cs.conflicted.data(), n);
-        else std::memset(flags, 0, n);
+        if (c->dynamic) {
+            std::vector<uint32_t> f(n);
+            hip_check(hipMemcpyAsync(f.data(), c->rdev.conflicted, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                     c->stream),
+                      "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            for (uint32_t p = 0; p < n; p++) flags[p] = f[p] ? 1u : 0u;
+        } else {
+            std::memset(flags, 0, n);
+        }
     });
 }
 

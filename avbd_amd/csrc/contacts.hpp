@@ -41,6 +41,38 @@ void launch_prim_bounds(hipStream_t s, const DevPrims& p, const LbvhDev& b, cons
 void launch_narrow(hipStream_t s, const DevPrims& p, const LbvhDev& b, const DevState& d, double dt, double margin,
                    DetRec* det, uint32_t det_cap, uint32_t* status);
 
+// Device state of the incremental recolouring (ColorList::incrementalRecoloring,
+// common/ColorList.hpp:366-522) and of the work-list layout that follows it.
+struct RecolourDev {
+    uint32_t n = 0;
+    uint32_t* color = nullptr;        // per particle
+    uint32_t* last_dirty = nullptr;   // per particle: round that queued it, 0 = dirty, ~0 = clean
+    uint32_t* candidate = nullptr;
+    uint32_t* dirty = nullptr;        // [0, counters[0]): dirty particles left unresolved by the last call
+    uint32_t* next = nullptr;
+    uint32_t* touched = nullptr;      // particles queued during a call (capacity 7n)
+    uint32_t* touched_conf = nullptr; // [0, counters[1]): the conflicted particles
+    uint32_t* conflicted = nullptr;   // per particle flag
+    uint32_t* counters = nullptr;     // [0] dirty, [1] conflicted, [2] colours, [3] status (1: out of colours),
+                                      // [4] layout changed by the last recolouring
+    const uint32_t* s_off = nullptr;  // static vertex adjacency (CSR)
+    const uint32_t* s_nb = nullptr;
+    const uint32_t* d_off = nullptr;  // dynamic vertex adjacency of this round (CSR)
+    const uint32_t* d_nb = nullptr;
+    const uint8_t* kind = nullptr;    // per particle: 0 no item, 1 positional, 2 oriented
+    const VItem* item_of = nullptr;   // per particle: its work item
+    const OItem* oitem_of = nullptr;
+    uint32_t* offsets = nullptr;      // 3 x 257 per-colour offsets: items, oriented items, conflicted
+};
+
+// One workgroup: mark the particles of this round's new contacts dirty, then
+// recolour and flag the conflicts; the new colour count is in counters[2].
+void launch_recolour(hipStream_t s, const RecolourDev& r, const uint32_t* seeds, uint32_t n_seeds);
+// One workgroup: lay the work list out again from the colours and conflicts
+// (only when the last recolouring changed them, unless forced).
+void launch_relayout(hipStream_t s, const RecolourDev& r, VItem* items, OItem* oitems, uint32_t* conflicts,
+                     bool force);
+
 // Host side of one detection round.
 class ContactSystem {
 public:
@@ -50,16 +82,17 @@ public:
     void init(uint32_t n_particles, uint32_t capacity, double collision_margin);
 
     // Merge this round's detected contacts (any order) with the previous
-    // round's, then rebuild the dynamic adjacency and recolour incrementally.
-    // Fills ops / active / dyn_off / dyn_ent; returns true when the colouring
-    // or the conflict set changed (plan relaid out).
-    bool process(std::vector<DetRec>& det, const std::vector<HostPrim>& prims, const std::vector<double>& mass,
-                 double dt, ColorPlan& plan);
+    // round's and rebuild the dynamic adjacency.  Fills ops / active /
+    // dyn_off / dyn_ent / nb_off / nb and the particles of the added contacts
+    // (seeds), which the device recolouring marks dirty.
+    void process(std::vector<DetRec>& det, const std::vector<HostPrim>& prims, const std::vector<double>& mass,
+                 double dt);
 
     std::vector<DynOp> ops;
     std::vector<uint32_t> active;            // record indices, pool 0 then pool 1, ascending slot
     std::vector<uint32_t> dyn_off, dyn_ent;  // per-particle CSR of (record << 3) | local
-    std::vector<uint8_t> conflicted;
+    std::vector<uint32_t> nb_off, nb;        // dynamic vertex adjacency (with duplicates)
+    std::vector<uint32_t> seeds;             // particles of the contacts added this round, in add order
 
     struct Slots {   // TombstonePool (common/TombstonePool.hpp): LIFO stack of free slots
         uint32_t top = 0, count = 0;
@@ -84,10 +117,6 @@ public:
 
 private:
     uint32_t n = 0;
-    std::vector<uint32_t> last_dirty_round, candidate, dirty, next_dirty, touched_this_frame, touched_conflicted;
-    std::vector<uint32_t> nb_off, nb;   // dynamic vertex adjacency (with duplicates)
-    void mark_dirty(uint32_t p);
-    bool recolour(ColorPlan& plan);
 };
 
 }  // namespace avbd

@@ -1,9 +1,9 @@
 // contacts_host.cpp -- host side of the dynamic contacts: contact lifecycle
 // (add / update / remove against the previous round, with the reference's
-// slot-stack pools), dynamic vertex adjacency and incremental recolouring.
-// The reference runs these on its first worker thread between the narrow
-// phase and the solve; here they run on the host between the device narrow
-// phase and the device solve of the same step.
+// slot-stack pools) and the dynamic adjacency.  The reference runs these on
+// its first worker thread between the narrow phase and the solve; here they
+// run on the host between the device narrow phase and the device
+// recolouring (recolour_kernels.hip) of the same step.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -14,7 +14,6 @@
 namespace avbd {
 
 namespace {
-constexpr uint32_t SENTINEL = 0xFFFFFFFFu;
 
 // Eigen's unitOrthogonal for 3-vectors (double precision, as the reference).
 void unit_orthogonal(const double n[3], double t[3]) {
@@ -66,13 +65,7 @@ void ContactSystem::init(uint32_t n_particles, uint32_t capacity, double collisi
     pool[0] = Slots();
     pool[1] = Slots();
     prev.clear();
-    last_dirty_round.assign(n, SENTINEL);
-    candidate.assign(n, 0);
-    dirty.clear();
-    next_dirty.clear();
-    touched_this_frame.clear();
-    touched_conflicted.clear();
-    conflicted.assign(n, 0);
+    seeds.clear();
     dyn_off.assign(n + 1, 0);
     dyn_ent.clear();
     nb_off.assign(n + 1, 0);
@@ -81,18 +74,10 @@ void ContactSystem::init(uint32_t n_particles, uint32_t capacity, double collisi
     active.clear();
 }
 
-// ColorList::markParticleDirty (common/ColorList.hpp:366-377)
-void ContactSystem::mark_dirty(uint32_t p) {
-    if (last_dirty_round[p] != 0) {
-        last_dirty_round[p] = 0;
-        touched_this_frame.push_back(p);
-        dirty.push_back(p);
-    }
-}
-
-bool ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim>& prims,
-                            const std::vector<double>& mass, double dt, ColorPlan& plan) {
+void ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim>& prims,
+                            const std::vector<double>& mass, double dt) {
     ops.clear();
+    seeds.clear();
     // _handleDetectedCollisions (collision/CollisionDetector.cpp:272-327): key order
     std::sort(det.begin(), det.end(), [](const DetRec& a, const DetRec& b) { return a.key < b.key; });
     std::vector<Entry> cur(det.size());
@@ -129,7 +114,7 @@ bool ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim
         o.s = (float)r.s;
         ops.push_back(o);
         cur[i] = Entry{r.key, kind, slot, 1u, 1u};
-        for (uint32_t k = 0; k < np; k++) mark_dirty(id[k]);
+        for (uint32_t k = 0; k < np; k++) seeds.push_back(id[k]);   // ColorList::markParticleDirty
     };
     auto update = [&](size_t i, uint32_t slot) {
         const DetRec& r = det[i];
@@ -201,83 +186,6 @@ bool ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim
             }
         }
     }
-    return recolour(plan);
-}
-
-// ColorList::incrementalRecoloring + finalizeConflicts (common/ColorList.hpp:381-522).
-// Each round computes every dirty particle's candidate from the colours at
-// the start of the round and then applies them, so the result does not
-// depend on the order of the dirty list.
-bool ContactSystem::recolour(ColorPlan& plan) {
-    std::vector<uint32_t>& color = plan.color;
-    const std::vector<uint8_t> conflicted_before = conflicted;
-    const uint32_t colors_before = plan.num_colors;
-    bool changed = false;
-    auto each_nb = [&](uint32_t p, auto&& f) {
-        for (uint32_t e = plan.static_off[p]; e < plan.static_off[p + 1]; e++) f(plan.static_nb[e]);
-        for (uint32_t e = nb_off[p]; e < nb_off[p + 1]; e++) f(nb[e]);
-    };
-    for (uint32_t p : touched_conflicted) conflicted[p] = 0;
-    touched_conflicted.clear();
-    uint32_t iter = 0;
-    while (!dirty.empty() && iter < 6) {
-        for (uint32_t p : dirty) {
-            uint64_t bits[4] = {0, 0, 0, 0};
-            bool must = false;
-            each_nb(p, [&](uint32_t q) {
-                const uint32_t c = color[q];
-                bits[c >> 6] |= 1ull << (c & 63);
-                if (c == color[p] && q < p) must = true;
-            });
-            if (must) {
-                uint32_t c = 256;
-                for (uint32_t w = 0; w < 4 && c == 256; w++)
-                    if (~bits[w]) c = w * 64 + (uint32_t)__builtin_ctzll(~bits[w]);
-                if (c == 256) throw std::runtime_error("Ran out of colors!");
-                candidate[p] = c;
-                if (c >= plan.num_colors) plan.num_colors++;
-            } else {
-                candidate[p] = color[p];
-            }
-        }
-        next_dirty.clear();
-        for (uint32_t p : dirty) {
-            if (candidate[p] == color[p]) continue;
-            color[p] = candidate[p];
-            changed = true;
-            each_nb(p, [&](uint32_t q) {
-                if (last_dirty_round[q] != iter + 1) {
-                    last_dirty_round[q] = iter + 1;
-                    touched_this_frame.push_back(q);
-                    next_dirty.push_back(q);
-                }
-            });
-        }
-        dirty.swap(next_dirty);
-        iter++;
-    }
-    for (uint32_t p : dirty) {
-        each_nb(p, [&](uint32_t q) {
-            if (color[q] == color[p]) {
-                if (!conflicted[p]) {
-                    conflicted[p] = 1;
-                    touched_conflicted.push_back(p);
-                }
-                if (!conflicted[q]) {
-                    conflicted[q] = 1;
-                    touched_conflicted.push_back(q);
-                }
-            }
-        });
-    }
-    for (uint32_t p : touched_this_frame) last_dirty_round[p] = SENTINEL;
-    touched_this_frame.clear();
-    for (uint32_t p : dirty) last_dirty_round[p] = 0;
-    if (changed || plan.num_colors != colors_before || conflicted != conflicted_before) {
-        relayout_plan(plan, conflicted);
-        return true;
-    }
-    return false;
 }
 
 namespace {
@@ -323,10 +231,7 @@ void ContactSystem::save(std::vector<uint8_t>& o) const {
         put_vec(o, s.idx);
     }
     put_vec(o, prev);
-    for (const auto* v : {&last_dirty_round, &candidate, &dirty, &touched_this_frame, &touched_conflicted,
-                          &active, &dyn_off, &dyn_ent, &nb_off, &nb})
-        put_vec(o, *v);
-    put_vec(o, conflicted);
+    for (const auto* v : {&active, &dyn_off, &dyn_ent, &nb_off, &nb}) put_vec(o, *v);
 }
 
 void ContactSystem::load(const uint8_t*& p, const uint8_t* end) {
@@ -342,12 +247,9 @@ void ContactSystem::load(const uint8_t*& p, const uint8_t* end) {
         get_vec(p, end, s.idx);
     }
     get_vec(p, end, prev);
-    for (auto* v : {&last_dirty_round, &candidate, &dirty, &touched_this_frame, &touched_conflicted, &active,
-                    &dyn_off, &dyn_ent, &nb_off, &nb})
-        get_vec(p, end, *v);
-    get_vec(p, end, conflicted);
+    for (auto* v : {&active, &dyn_off, &dyn_ent, &nb_off, &nb}) get_vec(p, end, *v);
     ops.clear();
-    next_dirty.clear();
+    seeds.clear();
 }
 
 }  // namespace avbd

@@ -299,46 +299,4 @@ void build_color_plan(const HostModel& m, ColorPlan& plan, const SceneColourer& 
     plan.conflict_list.clear();
 }
 
-// ColorList::rebuildWorkList (common/ColorList.hpp:345-363) over the kept
-// item templates, and the per-colour conflict lists of finalizeConflicts
-// (:476-522).  Conflicted particles without a work item (fixed ones) are
-// still committed, as the reference commits every conflicted particle.
-void relayout_plan(ColorPlan& plan, const std::vector<uint8_t>& conflicted) {
-    const uint32_t n = (uint32_t)plan.color.size();
-    const uint32_t K = plan.num_colors;
-    std::vector<uint32_t> cnt(K + 1, 0), ocnt(K + 1, 0), ccnt(K + 1, 0);
-    for (uint32_t v = 0; v < n; v++) {
-        const uint32_t c = plan.color[v];
-        if (c >= K) throw std::runtime_error("colour out of range after recolouring");
-        if (plan.item_kind[v] == 1) cnt[c + 1]++;
-        else if (plan.item_kind[v] == 2) ocnt[c + 1]++;
-        if (conflicted[v]) ccnt[c + 1]++;
-    }
-    for (uint32_t c = 0; c < K; c++) {
-        cnt[c + 1] += cnt[c];
-        ocnt[c + 1] += ocnt[c];
-        ccnt[c + 1] += ccnt[c];
-    }
-    plan.item_offsets = cnt;
-    plan.oitem_offsets = ocnt;
-    plan.conflict_offsets = ccnt;
-    plan.items.resize(cnt[K]);
-    plan.oitems.resize(ocnt[K]);
-    plan.conflict_list.resize(ccnt[K]);
-    for (uint32_t v = 0; v < n; v++) {
-        const uint32_t c = plan.color[v];
-        const bool cf = conflicted[v] != 0;
-        if (plan.item_kind[v] == 1) {
-            VItem it = plan.item_of[v];
-            it.inc_count = (it.inc_count & ~VITEM_CONFLICT) | (cf ? VITEM_CONFLICT : 0u);
-            plan.items[cnt[c]++] = it;
-        } else if (plan.item_kind[v] == 2) {
-            OItem it = plan.oitem_of[v];
-            it.n_cos = (it.n_cos & ~VITEM_CONFLICT) | (cf ? VITEM_CONFLICT : 0u);
-            plan.oitems[ocnt[c]++] = it;
-        }
-        if (cf) plan.conflict_list[ccnt[c]++] = v;
-    }
-}
-
 }  // namespace avbd

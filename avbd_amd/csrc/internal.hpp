@@ -467,10 +467,6 @@ void launch_dsatur(hipStream_t s, uint32_t n_scenes, const uint32_t* scene_off, 
                    const uint32_t* nb, uint64_t* key, uint64_t* seen, uint32_t* color, uint32_t* scene_colors,
                    uint32_t* status);
 
-// Regroup the work items by plan.color (colour-major, ascending particle)
-// with the conflict flags of `conflicted`, and rebuild the per-colour conflict
-// lists; plan.num_colors must cover every colour.
-void relayout_plan(ColorPlan& plan, const std::vector<uint8_t>& conflicted);
 
 // Message of the last failure of an entry point that takes no context
 // (returned by avbd_last_error(NULL)).

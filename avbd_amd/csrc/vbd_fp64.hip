@@ -773,7 +773,8 @@ void launch_solve_color_f64(hipStream_t s, const F64State& f, const LaunchCfg& c
     hipLaunchKernelGGL((solve_color_f64_kernel<LL, DD, UU>), grid, blk, 0, s, f.items + begin, count, f.inc, f.irec, \
                        f.damp, f.x, f.y, f.ground, x_out, inv_dt, inv_dt2, done, f.frozen, f.scene_of)
 #define AVBD_F64_U(LL, DD)                           \
-    if (cfg.batch >= 2) AVBD_F64_LAUNCH(LL, DD, 2);  \
+    if (cfg.batch >= 4) AVBD_F64_LAUNCH(LL, DD, 4);  \
+    else if (cfg.batch >= 2) AVBD_F64_LAUNCH(LL, DD, 2); \
     else AVBD_F64_LAUNCH(LL, DD, 1);
 #define AVBD_F64_CASE(LL)                      \
     case LL:                                    \
@@ -806,7 +807,8 @@ void launch_solve_cheb_f64(hipStream_t s, const F64State& f, const LaunchCfg& cf
     hipLaunchKernelGGL((solve_cheb_f64_kernel<LL, DD, UU>), grid, blk, 0, s, f, f.items + begin, count, sx, cx,   \
                        cheb_list, n_cheb, x_cur, x_next, omega, rc, omega_tab, j, inv_dt, inv_dt2)
 #define AVBD_F64C_U(LL, DD)                           \
-    if (cfg.batch >= 2) AVBD_F64C_LAUNCH(LL, DD, 2);  \
+    if (cfg.batch >= 4) AVBD_F64C_LAUNCH(LL, DD, 4);  \
+    else if (cfg.batch >= 2) AVBD_F64C_LAUNCH(LL, DD, 2); \
     else AVBD_F64C_LAUNCH(LL, DD, 1);
 #define AVBD_F64C_CASE(LL)                     \
     case LL:                                    \

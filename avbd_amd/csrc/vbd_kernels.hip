@@ -837,18 +837,31 @@ __global__ __launch_bounds__(1024) void residual_scene_kernel(const ResCtl* rc, 
                                                               const double* __restrict__ mg2_scene, double tol2,
                                                               uint32_t every, SceneFreeze fz) {
     if (rc->done) return;
-    __shared__ double red[1024];
+    __shared__ double red[16];
     const uint32_t s = blockIdx.x, b = scene_off[s], e = scene_off[s + 1];
     if (fz.frozen && fz.frozen[s]) return;   // its sum is the one it converged with
-    double v = 0;
-    for (uint32_t p = b + threadIdx.x; p < e; p += 1024) v += rr[p];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (uint32_t st = 512; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
+    // four independent loads in flight per thread; the grouping depends only
+    // on the offset from the scene's first particle, so a scene sums the
+    // same way wherever it sits in a batch
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    uint32_t p = b + threadIdx.x;
+    for (; p + 3 * 1024 < e; p += 4 * 1024) {
+        const double a0 = rr[p], a1 = rr[p + 1024], a2 = rr[p + 2048], a3 = rr[p + 3072];
+        v0 += a0;
+        v1 += a1;
+        v2 += a2;
+        v3 += a3;
     }
+    for (; p < e; p += 1024) v0 += rr[p];
+    double v = (v0 + v1) + (v2 + v3);
+    // wavefront butterfly, then the 16 wavefront sums in order
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < 16; w++) t += red[w];
+        red[0] = t;
         acc_scene[s] = red[0];
         if (fz.frozen && red[0] <= tol2 * mg2_scene[s]) {
             fz.frozen[s] = 1;
