@@ -2676,7 +2676,7 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
             }
         }
         const uint32_t cap = capacity ? capacity : 65536u;
-        if ((uint64_t)2 * cap > (1ull << 29)) throw CodeError(AVBD_E_ARG, "capacity too large");
+        if ((uint64_t)DYN_KINDS * cap > (1ull << 29)) throw CodeError(AVBD_E_ARG, "capacity too large");
         const uint32_t np = (uint32_t)c->prims.size();
         destroy_graph(c);
         c->fuse_cheb = false;
@@ -2701,7 +2701,7 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         c->det_cap = c->lbd.pair_cap;
         c->d_det = dev_alloc<DetRec>(c, c->det_cap);
         c->d_status = dev_alloc<uint32_t>(c, 2);
-        c->d.dyn = dev_alloc<DynRec>(c, 2 * (size_t)cap);
+        c->d.dyn = dev_alloc<DynRec>(c, DYN_KINDS * (size_t)cap);
         c->cs.init(n, cap, margin);
         auto* d_off = static_cast<uint32_t*>(grow(c, c->g_off, (n + 1) * sizeof(uint32_t)));
         upload(c, d_off, c->cs.dyn_off);
@@ -2761,7 +2761,7 @@ int avbd_download_contacts(avbd_ctx* c, uint32_t kind, avbd_contact* out, uint32
     return guarded(c, [&] {
         require_built(c);
         if (!c->dynamic) throw CodeError(AVBD_E_ARG, "dynamic contacts are off");
-        if (kind > 1) throw CodeError(AVBD_E_ARG, "kind must be 0 or 1");
+        if (kind >= DYN_KINDS) throw CodeError(AVBD_E_ARG, "kind must be 0, 1 or 2");
         const auto& P = c->cs.pool[kind];
         std::vector<DynRec> recs(c->cs.cap);
         hip_check(hipMemcpyAsync(recs.data(), c->d.dyn + (size_t)kind * c->cs.cap, recs.size() * sizeof(DynRec),
@@ -2811,7 +2811,9 @@ int avbd_download_detected(avbd_ctx* c, uint64_t* keys, uint32_t* slots, uint8_t
         for (uint32_t i = 0; i < E.size() && i < cap; i++) {
             if (keys) keys[i] = E[i].key;
             if (slots) slots[i] = E[i].slot;
-            if (kinds) kinds[i] = (uint8_t)(E[i].kind == DYN_TRI_RIGID ? DET_KIND_TRI_RIGID : DET_KIND_TRI_ROD);
+            if (kinds)
+                kinds[i] = (uint8_t)(E[i].kind == DYN_TRI_RIGID ? DET_KIND_TRI_RIGID
+                                     : (E[i].kind == DYN_TRI_ROD ? DET_KIND_TRI_ROD : DET_KIND_RIGID_RIGID));
         }
         if (count) *count = (uint32_t)E.size();
     });
@@ -2848,7 +2850,7 @@ std::vector<std::pair<void*, size_t>> ckpt_arrays(avbd_ctx* c) {
     if (d.n_joints) a.emplace_back(d.joints, (size_t)d.n_joints * sizeof(JointRec));
     if (c->rc) a.emplace_back(c->rc, sizeof(ResCtl));
     if (c->dynamic) {
-        a.emplace_back(d.dyn, 2 * (size_t)c->cs.cap * sizeof(DynRec));
+        a.emplace_back(d.dyn, DYN_KINDS * (size_t)c->cs.cap * sizeof(DynRec));
         // the device recolouring state (colours, dirty marks and list, conflicts)
         const RecolourDev& r = c->rdev;
         const size_t pu = (size_t)d.n * sizeof(uint32_t);

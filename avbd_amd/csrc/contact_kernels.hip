@@ -447,7 +447,43 @@ __global__ void narrow_kernel(DevPrims p, const uint32_t* __restrict__ pairs, co
     }
     if (ta == PRIM_TRIANGLE && (tb == PRIM_TRIANGLE)) return;            // _triangleTriangle: no-op
     if (ta == PRIM_ROD_SEGMENT && tb == PRIM_ROD_SEGMENT) return;        // _rodRod: no-op
-    if (ta != PRIM_TRIANGLE) {                                          // sphere-sphere, rod-sphere: unsupported
+    if (ta == PRIM_SPHERE) {
+        // sphere-sphere (new; the reference throws): in contact when the
+        // centres are closer than r_a + r_b + margin; n from b to a, contact
+        // points on the facing surfaces in body frames; the lower primitive
+        // index is the first body
+        if (a > b) {
+            const uint32_t s = a;
+            a = b;
+            b = s;
+        }
+        const uint32_t ca = p.idx[3 * a], cb = p.idx[3 * b];
+        const double ra = p.radius[a], rb = p.radius[b];
+        const d3 d = ld(x[ca]) - ld(x[cb]);
+        const double dist = norm(d);
+        if (!(dist < ra + rb + margin)) return;
+        const d3 nrm = dist > 1e-12 ? (1.0 / dist) * d : mk(1, 0, 0);
+        const d3 cpa = qrot(qinv(ldq(q[rot_of[ca]])), (-ra) * nrm);
+        const d3 cpb = qrot(qinv(ldq(q[rot_of[cb]])), rb * nrm);
+        DetRec r;
+        r.prim_a = a;
+        r.prim_b = b;
+        r.pad = 0;
+        r.kind = DET_KIND_RIGID_RIGID;
+        r.n[0] = nrm.x; r.n[1] = nrm.y; r.n[2] = nrm.z;
+        r.bary[0] = cpb.x; r.bary[1] = cpb.y; r.bary[2] = cpb.z;
+        r.cpl[0] = cpa.x; r.cpl[1] = cpa.y; r.cpl[2] = cpa.z;
+        r.s = 0;
+        r.key = contact_key(r.kind, a, b);
+        const uint32_t slot = atomicAdd(status + 1, 1u);
+        if (slot < det_cap) det[slot] = r;
+        return;
+    }
+    if (ta != PRIM_TRIANGLE) {
+        // rod segment - sphere: a sphere welded to one of the segment's nodes
+        // does not collide with it; any other pair is unsupported (the
+        // reference throws)
+        if (p.idx[3 * b] == p.idx[3 * a] || p.idx[3 * b] == p.idx[3 * a + 1]) return;
         atomicOr(status, NARROW_UNSUPPORTED);
         return;
     }

@@ -17,8 +17,10 @@ namespace avbd {
 
 // CollisionGeometryType values (collision/CollisionGeometryType.hpp)
 constexpr uint32_t PRIM_TRIANGLE = 0u, PRIM_ROD_SEGMENT = 1u, PRIM_SPHERE = 2u;
-// DetectedCollisionType values (collision/DetectedCollision.hpp:6-15)
-constexpr uint32_t DET_KIND_TRI_RIGID = 2u, DET_KIND_TRI_ROD = 3u;
+// DetectedCollisionType values (collision/DetectedCollision.hpp:6-15); the
+// rigid-rigid (sphere-sphere) pair is detected here, where the reference
+// throws (collision/CollisionDetector.cpp:678-686)
+constexpr uint32_t DET_KIND_TRI_RIGID = 2u, DET_KIND_TRI_ROD = 3u, DET_KIND_RIGID_RIGID = 6u;
 // narrow-phase status word bits
 constexpr uint32_t NARROW_UNSUPPORTED = 1u;
 
@@ -103,7 +105,7 @@ public:
         uint32_t alloc(uint32_t cap);
         void free(uint32_t slot);
     };
-    Slots pool[2];
+    Slots pool[3];   // triangle-rigid, triangle-rod, rigid-rigid
 
     struct Entry {   // one detected contact of the previous round, key order
         uint64_t key;

@@ -62,8 +62,7 @@ void ContactSystem::init(uint32_t n_particles, uint32_t capacity, double collisi
     n = n_particles;
     cap = capacity;
     margin = collision_margin;
-    pool[0] = Slots();
-    pool[1] = Slots();
+    for (Slots& P : pool) P = Slots();
     prev.clear();
     seeds.clear();
     dyn_off.assign(n + 1, 0);
@@ -81,14 +80,20 @@ void ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim
     // _handleDetectedCollisions (collision/CollisionDetector.cpp:272-327): key order
     std::sort(det.begin(), det.end(), [](const DetRec& a, const DetRec& b) { return a.key < b.key; });
     std::vector<Entry> cur(det.size());
+    auto kind_of = [](uint32_t det_kind) {
+        return det_kind == DET_KIND_TRI_RIGID ? DYN_TRI_RIGID : (det_kind == DET_KIND_TRI_ROD ? DYN_TRI_ROD : DYN_RIGID_RIGID);
+    };
     auto add = [&](size_t i) {
         const DetRec& r = det[i];
-        const uint32_t kind = r.kind == DET_KIND_TRI_RIGID ? DYN_TRI_RIGID : DYN_TRI_ROD;
-        const HostPrim& tri = prims[r.prim_a];
-        const HostPrim& other = prims[r.prim_b];
-        std::array<uint32_t, 5> id{tri.idx[0], tri.idx[1], tri.idx[2], other.idx[0],
-                                   kind == DYN_TRI_ROD ? other.idx[1] : 0u};
-        const uint32_t np = kind == DYN_TRI_RIGID ? 4u : 5u;
+        const uint32_t kind = kind_of(r.kind);
+        const HostPrim& pa = prims[r.prim_a];
+        const HostPrim& pb = prims[r.prim_b];
+        // triangle-rigid: triangle, body; triangle-rod: triangle, two nodes;
+        // rigid-rigid: the two bodies
+        std::array<uint32_t, 5> id{};
+        if (kind == DYN_RIGID_RIGID) id = {pa.idx[0], pb.idx[0], 0u, 0u, 0u};
+        else id = {pa.idx[0], pa.idx[1], pa.idx[2], pb.idx[0], kind == DYN_TRI_ROD ? pb.idx[1] : 0u};
+        const uint32_t np = dyn_particles(kind);
         double mmin = mass[id[0]];
         for (uint32_t k = 1; k < np; k++) mmin = std::min(mmin, mass[id[k]]);
         const uint32_t slot = pool[kind].alloc(cap);
@@ -118,7 +123,7 @@ void ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim
     };
     auto update = [&](size_t i, uint32_t slot) {
         const DetRec& r = det[i];
-        const uint32_t kind = r.kind == DET_KIND_TRI_RIGID ? DYN_TRI_RIGID : DYN_TRI_ROD;
+        const uint32_t kind = kind_of(r.kind);
         DynOp o{};
         o.rec = kind * cap + slot;
         o.op = DYN_OP_UPDATE;
@@ -152,8 +157,8 @@ void ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim
     // common/DynamicParticleAdjacency.hpp:33-118): pool 0 then pool 1, ascending slot
     active.clear();
     std::vector<uint32_t> val(n + 1, 0), deg(n + 1, 0);
-    for (uint32_t kind = 0; kind < 2; kind++) {
-        const uint32_t np = kind == DYN_TRI_RIGID ? 4u : 5u;
+    for (uint32_t kind = 0; kind < DYN_KINDS; kind++) {
+        const uint32_t np = dyn_particles(kind);
         const Slots& P = pool[kind];
         for (uint32_t sl = 0; sl < P.active.size(); sl++) {
             if (!P.active[sl]) continue;
@@ -172,8 +177,8 @@ void ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim
     nb_off = deg;
     dyn_ent.resize(val[n]);
     nb.resize(deg[n]);
-    for (uint32_t kind = 0; kind < 2; kind++) {
-        const uint32_t np = kind == DYN_TRI_RIGID ? 4u : 5u;
+    for (uint32_t kind = 0; kind < DYN_KINDS; kind++) {
+        const uint32_t np = dyn_particles(kind);
         const Slots& P = pool[kind];
         for (uint32_t sl = 0; sl < P.active.size(); sl++) {
             if (!P.active[sl]) continue;

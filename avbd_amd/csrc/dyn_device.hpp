@@ -39,6 +39,16 @@ struct Geo {
 
 __device__ __forceinline__ Geo geometry(const View& d, const DynRec& c) {
     Geo g;
+    if (kind_of(c) == DYN_RIGID_RIGID) {
+        // cp_2 - cp_1 with cp_k = x_k + R_k cpl_k (cpl_1 in cpl, cpl_2 in bary)
+        const float4 q1 = d.q[d.rot_of[c.idx[0]]], q2 = d.q[d.rot_of[c.idx[1]]];
+        const float3 cp1 = add(p3(d.x[c.idx[0]]), qrot(q1, make_float3(c.cpl.x, c.cpl.y, c.cpl.z)));
+        const float3 cp2 = add(p3(d.x[c.idx[1]]), qrot(q2, make_float3(c.bary.x, c.bary.y, c.bary.z)));
+        g.q = q1;
+        g.q_diff = make_float3(0.f, 0.f, 0.f);
+        g.diff = sub(cp2, cp1);
+        return g;
+    }
     const float3 x0 = p3(d.x[c.idx[0]]), x1 = p3(d.x[c.idx[1]]), x2 = p3(d.x[c.idx[2]]);
     const float3 cp_tri = add(add(scl(c.bary.x, x0), scl(c.bary.y, x1)), scl(c.bary.z, x2));
     const float3 cpl = make_float3(c.cpl.x, c.cpl.y, c.cpl.z);
@@ -124,7 +134,47 @@ __device__ __forceinline__ void add_rot_hess(Sym6& H, float w, float3 vb, float3
 // local 3 / 4).  The tangent gradients of the oriented side are zero in the
 // reference (TriangleRigidCollisionEnergySolver.hpp:90,
 // TriangleRodCollisionEnergySolver.hpp:130).
+// Rigid-rigid contact, body `local` (0 first, sign -1; 1 second, sign +1):
+// per direction dir = n, t, b the gradient sign * [dir ; cpl x R^T dir], the
+// Gauss-Newton terms k g g^T, and the rotational blocks
+// sign * sym(R^T [dir]x R [cpl]x) weighted by the unclamped k C + lambda,
+// with the friction multipliers clamped to the cone (new; the same terms as
+// the body side of the triangle-rigid contact, plus friction).
+__device__ __forceinline__ void accumulate6_rigid_rigid(const View& d, const DynRec& c, uint32_t local, Vec6& G,
+                                                        Sym6& H) {
+    const Geo g = geometry(d, c);
+    const float3 n = p3(c.n), t = p3(c.t), b = p3(c.b);
+    const float Cn = dotv(n, g.diff), Ct = dotv(t, g.diff), Cb = dotv(b, g.diff);
+    const Mult m = multipliers(c, Cn, Ct, Cb);
+    const float sg = local == 0 ? -1.f : 1.f;
+    const uint32_t body = c.idx[local];
+    const float3 cpl = local == 0 ? make_float3(c.cpl.x, c.cpl.y, c.cpl.z) : make_float3(c.bary.x, c.bary.y, c.bary.z);
+    const M3f R = qmat(d.q[d.rot_of[body]]);
+    float gn[6], gt[6], gb[6];
+    auto grad = [&](float3 dir, float* gv) {
+        const float3 r = crossv(cpl, mtvec(R, dir));
+        gv[0] = sg * dir.x; gv[1] = sg * dir.y; gv[2] = sg * dir.z;
+        gv[3] = sg * r.x; gv[4] = sg * r.y; gv[5] = sg * r.z;
+    };
+    grad(n, gn);
+    grad(t, gt);
+    grad(b, gb);
+#pragma unroll
+    for (int i = 0; i < 6; i++) G.v[i] += m.lnp * gn[i] + m.lt * gt[i] + m.lb * gb[i];
+    sym6_outer(H, gn, c.k.x);
+    sym6_outer(H, gt, c.k.y);
+    sym6_outer(H, gb, c.k.z);
+    const float an = c.k.x * m.Ccorr + c.lam.x, at = c.k.y * Ct + c.lam.y, ab = c.k.z * Cb + c.lam.z;
+    add_rot_hess(H, sg * an, mtvec(R, n), cpl);
+    add_rot_hess(H, sg * at, mtvec(R, t), cpl);
+    add_rot_hess(H, sg * ab, mtvec(R, b), cpl);
+}
+
 __device__ __forceinline__ void accumulate6(const View& d, const DynRec& c, uint32_t local, Vec6& G, Sym6& H) {
+    if (kind_of(c) == DYN_RIGID_RIGID) {
+        accumulate6_rigid_rigid(d, c, local, G, H);
+        return;
+    }
     const Geo g = geometry(d, c);
     const float3 n = p3(c.n), t = p3(c.t), b = p3(c.b);
     const float Cn = dotv(n, g.diff), Ct = dotv(t, g.diff), Cb = dotv(b, g.diff);

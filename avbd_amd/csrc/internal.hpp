@@ -126,12 +126,14 @@ constexpr uint32_t PREC_JOINT = 0x80000000u;   // per-rotation record list: join
 // energy/collision/TriangleRigidCollisionEnergyPool.hpp:8-14,
 // energy/collision/TriangleRodCollisionEnergyPool.hpp:8-15).  idx: the
 // triangle's vertices, then the rigid body (kind 0) or the two rod nodes
-// (kind 1).  Record index = kind * capacity + pool slot.
+// (kind 1); kind 2 (rigid-rigid, new) holds the two bodies in idx[0..1] and
+// their body-frame contact points in cpl and bary.  Record index = kind *
+// capacity + pool slot.
 struct alignas(16) DynRec {
     float4 n;      // normal, w = k_start
     float4 t;      // tangent, w = mu_s
     float4 b;      // binormal, w = mu_k
-    float4 bary;   // barycentrics on the triangle, w = s (rod parameter)
+    float4 bary;   // barycentrics on the triangle, w = s (rod parameter); rigid-rigid: the second body's contact point
     float4 cpl;    // contact point in the body / interpolated rod frame, w = C_n_prev
     float4 k;      // k_n, k_t, k_b, unused
     float4 lam;    // lambda_n, lambda_t, lambda_b, unused
@@ -140,7 +142,9 @@ struct alignas(16) DynRec {
     uint32_t pad[2];
 };
 static_assert(sizeof(DynRec) == 144, "DynRec must be 144 bytes");
-constexpr uint32_t DYN_TRI_RIGID = 0u, DYN_TRI_ROD = 1u;
+constexpr uint32_t DYN_TRI_RIGID = 0u, DYN_TRI_ROD = 1u, DYN_RIGID_RIGID = 2u, DYN_KINDS = 3u;
+// particles per record of each kind: triangle + body, triangle + two rod nodes, two bodies
+__host__ __device__ constexpr uint32_t dyn_particles(uint32_t kind) { return kind == 0u ? 4u : (kind == 1u ? 5u : 2u); }
 constexpr uint32_t DYN_FLAG_SHIFT = 8u;
 constexpr float kContactMuS = 0.4f, kContactMuK = 0.2f;   // _addCollision (collision/CollisionDetector.cpp:365-366)
 

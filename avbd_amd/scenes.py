@@ -362,8 +362,7 @@ def hail_on_block(n: int = 24, edge: float = 1.0, grid: int = 6, radius: float =
     """grid x grid rigid spheres (r = 4 cm, 2000 kg/m^3, 16 cm apart, no
     spin) falling at `speed` onto a soft block (n^3 cells, E = 5e6 Pa,
     bottom layer held): many simultaneous triangle-rigid contacts.  The
-    spheres stay apart over the benchmarked steps (a sphere-sphere pair is
-    unsupported, as in the reference)."""
+    spheres stay apart over the benchmarked steps."""
     block = soft_block(n, edge, E=5e6, nu=0.4)
     spacing = 0.16
     x0 = 0.5 * edge - 0.5 * spacing * (grid - 1)
@@ -375,6 +374,27 @@ def hail_on_block(n: int = 24, edge: float = 1.0, grid: int = 6, radius: float =
                                   ground=False, v0=(0.0, -speed, 0.0)))
     return Scene("hail_on_block", meshes=[block], spheres=spheres, dt=dt, iters=iters, rho=0.95, steps=steps,
                  contact_margin=margin)
+
+
+def sphere_pile(radius: float = 0.05, density: float = 1000.0, spin: float = 3.0, seed: int = 11,
+                dt: float = 1.0 / 120.0, iters: int = 20, steps: int = 60, margin: float = 1e-3) -> Scene:
+    """Six rigid spheres (r = 5 cm) with ground contact: three in a row just
+    above the ground, two dropped into the grooves between them and one on
+    top, each spinning at up to `spin` rad/s about a seeded axis, so that
+    rigid-rigid (sphere-sphere) contacts with friction form and persist
+    next to the rigid-ground ones."""
+    rng = SplitMix64(seed)
+    r = radius
+    h0 = r + 0.002
+    pts = [(-2.02 * r, h0, 0.0), (0.0, h0, 0.0), (2.02 * r, h0, 0.0),
+           (-1.01 * r, h0 + 1.75 * r + 0.006, 0.01 * r), (1.01 * r, h0 + 1.75 * r + 0.006, -0.01 * r),
+           (0.0, h0 + 3.5 * r + 0.012, 0.0)]
+    spheres = []
+    for pos in pts:
+        w = np.array([rng.uniform() - 0.5 for _ in range(3)])
+        w = spin * w / max(np.linalg.norm(w), 1e-12) * rng.uniform()
+        spheres.append(Sphere(radius=r, density=density, pos=pos, w0=tuple(float(v) for v in w), ground=True))
+    return Scene("sphere_pile", spheres=spheres, dt=dt, iters=iters, rho=0.95, steps=steps, contact_margin=margin)
 
 
 def gaussian_force_field(n_vertices: int, sigma: float = 10.0, seed: int = 0) -> np.ndarray:

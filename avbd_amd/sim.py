@@ -247,7 +247,7 @@ class SimulationContext:
         self._chk(lib.avbd_set_dynamic_contacts(self.handle, float(margin), int(capacity)))
 
     def contacts(self, kind: int) -> np.ndarray:
-        """Active contacts of one kind (0 triangle-rigid, 1 triangle-rod) in slot
+        """Active contacts of one kind (0 triangle-rigid, 1 triangle-rod, 2 rigid-rigid) in slot
         order, one row of 32 per contact: slot, idx[5], n, t, b, barys, s,
         cp_local, k_start, k_n, k_t, k_b, lambda_n, lambda_t, lambda_b,
         C_n_prev, use_static, unclamped."""

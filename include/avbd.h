@@ -385,16 +385,21 @@ int avbd_add_collision_sphere(avbd_ctx* ctx, uint32_t p, double radius);
  * every avbd_step() then runs CollisionDetector::detectCollisionsAndRecolor_Parallel
  * (collision/CollisionDetector.cpp:57-129) before the solve -- LBVH over
  * every primitive and the narrow phase on the device; contact lifecycle
- * (_handleDetectedCollisions, :272-501), dynamic adjacency and incremental
- * recolouring (common/ColorList.hpp:381-522) on the host -- and the solve
- * includes the triangle-rigid / triangle-rod AL contacts.
+ * (_handleDetectedCollisions, :272-501) and dynamic adjacency on the host;
+ * incremental recolouring (common/ColorList.hpp:381-522) and the work-list
+ * layout on the device -- and the solve includes the triangle-rigid /
+ * triangle-rod AL contacts.
  * collision_margin: SimulationParams::collision_margin; capacity: contacts
  * per kind (0 = 65,536).  Steps then run eagerly and synchronise once per
  * step (the host needs the narrow-phase output); the persistent kernel,
  * the step graph, the warm start and residual-targeted iteration are not
- * available in this mode (AVBD_E_ARG).  A sphere-sphere or rod-sphere
- * candidate pair fails the step with AVBD_E_UNSUPPORTED_CONTACT, as the
- * reference throws "Unsupported collision!" (:210-213,686-687). */
+ * available in this mode (AVBD_E_ARG).  Sphere-sphere pairs become
+ * rigid-rigid AL contacts (new; the reference throws at :686-687): normal
+ * and friction on both bodies, C = dir . (x_2 + R_2 cp_2 - x_1 - R_1 cp_1).
+ * A rod-sphere pair that shares no particle fails the step with
+ * AVBD_E_UNSUPPORTED_CONTACT, as the reference throws "Unsupported
+ * collision!" (:210-213); a sphere welded to a rod's node does not collide
+ * with that rod. */
 int avbd_set_dynamic_contacts(avbd_ctx* ctx, double collision_margin, uint32_t capacity);
 
 /* One dynamic contact (CollisionConstraintEnergyInfo + TriangleRigid /
@@ -403,21 +408,23 @@ int avbd_set_dynamic_contacts(avbd_ctx* ctx, double collision_margin, uint32_t c
  * energy/collision/TriangleRodCollisionEnergyPool.hpp:8-15). */
 typedef struct avbd_contact {
     uint32_t slot;          /* pool slot (TombstonePool index) */
-    uint32_t idx[5];        /* triangle vertices, then the body or both rod nodes */
+    uint32_t idx[5];        /* triangle vertices, then the body or both rod nodes;
+                               rigid-rigid: the two bodies */
     double n[3], t[3], b[3];
-    double barys[3];
+    double barys[3];        /* rigid-rigid: the second body's contact point (body frame) */
     double s;               /* triangle-rod: contact point along the segment */
     double cp_local[3];
     double k_start, k_n, k_t, k_b;
     double lambda_n, lambda_t, lambda_b, C_n_prev;
     uint8_t use_static, unclamped, pad_[6];
 } avbd_contact;
-/* Active contacts of one kind (0 triangle-rigid, 1 triangle-rod) in slot
- * order: up to cap into out; *count = active total. */
+/* Active contacts of one kind (0 triangle-rigid, 1 triangle-rod, 2
+ * rigid-rigid) in slot order: up to cap into out; *count = active total. */
 int avbd_download_contacts(avbd_ctx* ctx, uint32_t kind, avbd_contact* out, uint32_t cap, uint32_t* count);
 /* Contacts detected by the last step's detection in key order
  * (DetectedCollision::generateKey, collision/DetectedCollision.hpp:43-66):
- * key, pool slot, DetectedCollisionType (2 triangle-rigid, 3 triangle-rod). */
+ * key, pool slot, DetectedCollisionType (2 triangle-rigid, 3 triangle-rod,
+ * 6 rigid-rigid). */
 int avbd_download_detected(avbd_ctx* ctx, uint64_t* keys, uint32_t* slots, uint8_t* kinds, uint32_t cap,
                            uint32_t* count);
 /* is_conflicted per particle after the last recolouring (n bytes). */

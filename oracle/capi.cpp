@@ -98,8 +98,8 @@ uint32_t oracle_num_contacts(void* h, uint32_t type) {
     return type < DYN_COUNT ? s->dyn[type].sp.count : 0;
 }
 // Active contacts of one dynamic pool in slot order, 32 doubles each:
-// slot, idx[5], n, t, b, barys, s, cpl, k_start, k_n, k_t, k_b,
-// lambda_n, lambda_t, lambda_b, C_n_prev, use_static, unclamped.
+// slot, idx[5], n, t, b, barys (rigid-rigid: cpl2), s, cpl, k_start, k_n,
+// k_t, k_b, lambda_n, lambda_t, lambda_b, C_n_prev, use_static, unclamped.
 uint32_t oracle_get_contacts(void* h, uint32_t type, double* rec32, uint32_t cap) {
     auto* s = static_cast<Simulation*>(h);
     if (type >= DYN_COUNT) return 0;
@@ -116,7 +116,7 @@ uint32_t oracle_get_contacts(void* h, uint32_t type, double* rec32, uint32_t cap
                 r[6 + i] = c.n[i];
                 r[9 + i] = c.t[i];
                 r[12 + i] = c.b[i];
-                r[15 + i] = c.barys[i];
+                r[15 + i] = type == DYN_RIGID_RIGID ? c.cpl2[i] : c.barys[i];
                 r[19 + i] = c.cpl[i];
             }
             r[18] = c.s;
@@ -616,7 +616,8 @@ int oracle_tri_sphere_ccd(const double* tri9, const double* tv9, const double* p
     return hit ? 1 : 0;
 }
 // One dynamic contact over particles 0..4 (0-2 triangle, 3 body or 3-4 rod
-// nodes; q20: quaternions (w, x, y, z) of particles 0..4, only 3-4 used).
+// nodes; rigid-rigid: bodies 3 and 4, cpl2 in the barys slot; q20:
+// quaternions (w, x, y, z) of particles 0..4, only 3-4 used).
 // rec: n, t, b, barys, s, cpl (19), then k_n, k_t, k_b, lambda_n, lambda_t,
 // lambda_b, C_n_prev, mu_s, mu_k, use_static (10).  Writes C (3), the raw
 // gradients g (3 x 6: n, t, b), the rotational Hessian blocks h (3 x 9) and
@@ -632,10 +633,15 @@ void oracle_dyn_single(uint32_t type, const double* x15, const double* q20, cons
     rs.q = {q_in(q20 + 12), q_in(q20 + 16)};
     DynContact c;
     for (int i = 0; i < 5; i++) c.idx[i] = i;
+    if (type == DYN_RIGID_RIGID) {   // the two bodies are the oriented particles 3 and 4
+        c.idx[0] = 3;
+        c.idx[1] = 4;
+    }
     c.n = v_in(rec26);
     c.t = v_in(rec26 + 3);
     c.b = v_in(rec26 + 6);
     c.barys = v_in(rec26 + 9);
+    if (type == DYN_RIGID_RIGID) c.cpl2 = c.barys;   // the rigid-rigid record carries cpl2 there
     c.s = rec26[12];
     c.cpl = v_in(rec26 + 13);
     c.k_n = rec26[16]; c.k_t = rec26[17]; c.k_b = rec26[18];

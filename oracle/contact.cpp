@@ -382,6 +382,15 @@ M3 sym(const M3& a) {
 // TriangleRodCollisionConstraintSolver::evaluateConstraint (:14-49)
 void dyn_eval(uint8_t type, const DynContact& c, const std::vector<V3>& x, const RotState& rs, Real& Cn, Real& Ct,
               Real& Cb) {
+    if (type == DYN_RIGID_RIGID) {
+        const V3 cp1 = x[c.idx[0]] + qrot(rs.rot(c.idx[0]), c.cpl);
+        const V3 cp2 = x[c.idx[1]] + qrot(rs.rot(c.idx[1]), c.cpl2);
+        const V3 d = cp2 - cp1;
+        Cn = dot(c.n, d);
+        Ct = dot(c.t, d);
+        Cb = dot(c.b, d);
+        return;
+    }
     const V3 cp_tri = c.barys.x * x[c.idx[0]] + c.barys.y * x[c.idx[1]] + c.barys.z * x[c.idx[2]];
     V3 cp;
     if (type == DYN_TRI_RIGID) {
@@ -403,6 +412,37 @@ void dyn_constraint(uint8_t type, const DynContact& c, const std::vector<V3>& x,
     dyn_eval(type, c, x, rs, Cn, Ct, Cb);
     for (int i = 0; i < 6; i++) gn[i] = gt[i] = gb[i] = 0;
     hn = ht = hb = M3();
+    if (type == DYN_RIGID_RIGID) {
+        // C = dir . (x_2 + R_2 cpl_2 - x_1 - R_1 cpl_1): per body sign * [dir ; -(R [cpl]x)^T dir],
+        // rotational Hessians sign * sym(R^T [dir]x R [cpl]x), as the tri-rigid body side
+        if (dof != 6 || local > 1) throw std::runtime_error("dynamic contact: local index out of bounds");
+        const Real sg = local == 0 ? -1.0 : 1.0;
+        const uint32_t body = c.idx[local];
+        const V3 cpl = local == 0 ? c.cpl : c.cpl2;
+        const M3 R = qmat(rs.rot(body));
+        const M3 Sc = skew3(cpl);
+        const M3 Rr = matmul(R, Sc);
+        auto grad = [&](V3 dir, Real g[6]) {
+            const V3 r = matvec(transpose(Rr), dir);
+            for (int i = 0; i < 3; i++) {
+                g[i] = sg * dir[i];
+                g[3 + i] = -sg * r[i];
+            }
+        };
+        auto hblock = [&](V3 v) {
+            M3 h = sym(matmul(matmul(matmul(transpose(R), skew3(v)), R), Sc));
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) h.m[i][j] *= sg;
+            return h;
+        };
+        grad(c.n, gn);
+        grad(c.t, gt);
+        grad(c.b, gb);
+        hn = hblock(c.n);
+        ht = hblock(c.t);
+        hb = hblock(c.b);
+        return;
+    }
     if (dof == 3) {
         if (local >= 3) throw std::runtime_error("dynamic contact: local index out of bounds for a positional particle");
         const Real bary = c.barys[(int)local];
@@ -581,6 +621,17 @@ void update_collision_frame(DynContact& c, V3 nn) {
 // ---------------------------------------------------------------------------
 // Narrow phase: CollisionDetector::_triangleSDF_CCD (:689-876), sphere SDF
 // ---------------------------------------------------------------------------
+bool sphere_sphere(V3 xa, const Quat& qa, Real ra, V3 xb, const Quat& qb, Real rb, Real margin, V3& normal, V3& cpl_a,
+                   V3& cpl_b) {
+    const V3 d = xa - xb;
+    const Real dist = norm(d);
+    if (!(dist < ra + rb + margin)) return false;
+    normal = dist > 1e-12 ? d / dist : V3(1, 0, 0);
+    cpl_a = qrot(qinverse(qa), -ra * normal);
+    cpl_b = qrot(qinverse(qb), rb * normal);
+    return true;
+}
+
 bool tri_sphere_ccd(const V3 tri[3], const V3 tv[3], V3 rb_pos, const Quat& rb_rot, V3 rb_v, V3 rb_w, Real radius,
                     Real dt, Real margin, V3& normal, V3& cp_barys, V3& cpl) {
     auto pos_t = [&](Real t) { return rb_pos + t * rb_v; };
@@ -735,8 +786,28 @@ void Simulation::process_pair(uint32_t a, uint32_t b, std::vector<Detected>& out
     const Real dt = params.dt;
     if (ta == PRIM_TRIANGLE && tb == PRIM_TRIANGLE) return;        // _triangleTriangle: no-op
     if (ta == PRIM_ROD_SEGMENT && tb == PRIM_ROD_SEGMENT) return;  // _rodRod: no-op
-    if (ta == PRIM_SPHERE && tb == PRIM_SPHERE)
-        throw std::runtime_error("Sphere-sphere collision detection not implemented.");
+    if (ta == PRIM_SPHERE && tb == PRIM_SPHERE) {
+        // new: the reference throws here (collision/CollisionDetector.cpp:678-686)
+        if (a > b) std::swap(a, b);
+        const Prim& A = prims[a];
+        const Prim& B = prims[b];
+        const uint32_t pa = A.idx[0], pb = B.idx[0];
+        V3 nrm, ca, cb;
+        if (sphere_sphere(x[pa], rs.rot(pa), A.radius, x[pb], rs.rot(pb), B.radius, collision_margin, nrm, ca, cb)) {
+            Detected d;
+            d.type = DET_RIGID_RIGID;
+            d.key = contact_key(DET_RIGID_RIGID, a, b, 0);
+            d.gen1 = A.gen;
+            d.gen2 = B.gen;
+            d.normal = nrm;
+            d.rb = pa;
+            d.rb2 = pb;
+            d.cpl = ca;
+            d.cpl2 = cb;
+            out.push_back(d);
+        }
+        return;
+    }
     if (ta == PRIM_TRIANGLE && tb == PRIM_SPHERE) {
         const Prim& tr = prims[a];
         const Prim& sp = prims[b];
@@ -793,7 +864,17 @@ void Simulation::process_pair(uint32_t a, uint32_t b, std::vector<Detected>& out
         }
         return;
     }
-    throw std::runtime_error("Unsupported collision!");   // rod segment vs sphere
+    // rod segment vs sphere: a sphere welded to a rod's node does not collide
+    // with that rod (the shared-particle rule of _shouldSkip); any other pair
+    // is unsupported, as in the reference
+    for (int i = 0; i < prims[a].np; i++)
+        for (int j = 0; j < prims[b].np; j++)
+            if (prims[a].idx[i] == prims[b].idx[j]) return;
+    throw std::runtime_error("Unsupported collision!");
+}
+
+static uint8_t dyn_type_of(uint8_t det_type) {
+    return det_type == DET_TRI_RIGID ? DYN_TRI_RIGID : (det_type == DET_TRI_ROD ? DYN_TRI_ROD : DYN_RIGID_RIGID);
 }
 
 // ColorList::markParticleDirty (common/ColorList.hpp:366-377)
@@ -811,7 +892,13 @@ void Simulation::add_contact(Detected& d) {
     DynContact c;
     uint8_t ty;
     Real min_mass;
-    if (d.type == DET_TRI_RIGID) {
+    if (d.type == DET_RIGID_RIGID) {
+        ty = DYN_RIGID_RIGID;
+        min_mass = std::min(mass[d.rb], mass[d.rb2]);
+        c.idx[0] = d.rb;
+        c.idx[1] = d.rb2;
+        c.cpl2 = d.cpl2;
+    } else if (d.type == DET_TRI_RIGID) {
         ty = DYN_TRI_RIGID;
         min_mass = std::min({mass[d.tri[0]], mass[d.tri[1]], mass[d.tri[2]], mass[d.rb]});
         c.idx[3] = d.rb;
@@ -822,7 +909,8 @@ void Simulation::add_contact(Detected& d) {
         c.idx[4] = d.rod[1];
         c.s = d.s;
     }
-    for (int k = 0; k < 3; k++) c.idx[k] = d.tri[k];
+    if (ty != DYN_RIGID_RIGID)
+        for (int k = 0; k < 3; k++) c.idx[k] = d.tri[k];
     const Real k_start = min_mass / (dt * dt);
     // CollisionConstraintEnergyPool::addEnergy (energy/collision/CollisionConstraintEnergyPool.hpp:48-69)
     c.k_start = c.k_n = c.k_t = c.k_b = k_start;
@@ -837,6 +925,11 @@ void Simulation::add_contact(Detected& d) {
     c.barys = d.barys;
     c.cpl = d.cpl;
     d.e_idx = dyn[ty].add(c);
+    if (ty == DYN_RIGID_RIGID) {
+        mark_dirty(d.rb);
+        mark_dirty(d.rb2);
+        return;
+    }
     for (int k = 0; k < 3; k++) mark_dirty(d.tri[k]);
     if (ty == DYN_TRI_RIGID) {
         mark_dirty(d.rb);
@@ -846,15 +939,14 @@ void Simulation::add_contact(Detected& d) {
     }
 }
 
-void Simulation::remove_contact(const Detected& d) {
-    dyn[d.type == DET_TRI_RIGID ? DYN_TRI_RIGID : DYN_TRI_ROD].sp.free(d.e_idx);
-}
+void Simulation::remove_contact(const Detected& d) { dyn[dyn_type_of(d.type)].sp.free(d.e_idx); }
 
 // _updateCollision (:454-501)
 void Simulation::update_contact(const Detected& d) {
-    DynContact& c = dyn[d.type == DET_TRI_RIGID ? DYN_TRI_RIGID : DYN_TRI_ROD].data[d.e_idx];
+    DynContact& c = dyn[dyn_type_of(d.type)].data[d.e_idx];
     if (d.type == DET_TRI_ROD) c.s = d.s;
     c.cpl = d.cpl;
+    c.cpl2 = d.cpl2;
     c.barys = d.barys;
     update_collision_frame(c, d.normal);
 }
@@ -894,9 +986,9 @@ void Simulation::handle_detected() {
 // DynamicParticleAdjacency::buildAdjacency (common/DynamicParticleAdjacency.hpp:33-118)
 void Simulation::build_dynamic_adjacency() {
     const uint32_t n = (uint32_t)x.size();
-    std::vector<std::array<uint32_t, DYN_COUNT>> val(n, {0, 0});
+    std::vector<std::array<uint32_t, DYN_COUNT>> val(n, {0, 0, 0});
     std::vector<uint32_t> deg(n, 0);
-    const uint32_t npe[DYN_COUNT] = {4, 5};
+    const uint32_t npe[DYN_COUNT] = {4, 5, 2};
     for (int ty = 0; ty < DYN_COUNT; ty++) {
         const DynPool& P = dyn[ty];
         for (uint32_t e = 0; e < P.sp.active.size(); e++) {
@@ -907,11 +999,11 @@ void Simulation::build_dynamic_adjacency() {
             }
         }
     }
-    dadj.e_offsets.assign(n + 1, {0, 0});
+    dadj.e_offsets.assign(n + 1, {0, 0, 0});
     dadj.p_offsets.assign(n + 1, 0);
     for (uint32_t i = 0; i < n; i++) {
-        dadj.e_offsets[i][1] = dadj.e_offsets[i][0] + val[i][0];
-        dadj.e_offsets[i + 1][0] = dadj.e_offsets[i][1] + val[i][1];
+        for (int ty = 1; ty < DYN_COUNT; ty++) dadj.e_offsets[i][ty] = dadj.e_offsets[i][ty - 1] + val[i][ty - 1];
+        dadj.e_offsets[i + 1][0] = dadj.e_offsets[i][DYN_COUNT - 1] + val[i][DYN_COUNT - 1];
         dadj.p_offsets[i + 1] = dadj.p_offsets[i] + deg[i];
     }
     dadj.e_entries.resize(dadj.e_offsets[n][0]);

@@ -72,12 +72,17 @@ struct SlotPool {
     uint32_t slot_at(uint32_t i) const { return i < slots.size() ? slots[i] : i; }
 };
 
-// DynamicEnergyType order (common/common.hpp) and DetectedCollisionType values.
-enum DynType : uint8_t { DYN_TRI_RIGID = 0, DYN_TRI_ROD = 1, DYN_COUNT = 2 };
-enum DetType : uint8_t { DET_TRI_RIGID = 2, DET_TRI_ROD = 3 };
+// DynamicEnergyType order (common/common.hpp) and DetectedCollisionType
+// values; the rigid-rigid (sphere-sphere) contact is new: the reference
+// declares its detection record (collision/DetectedCollision.hpp:33) but
+// throws on sphere-sphere pairs (collision/CollisionDetector.cpp:678-686).
+enum DynType : uint8_t { DYN_TRI_RIGID = 0, DYN_TRI_ROD = 1, DYN_RIGID_RIGID = 2, DYN_COUNT = 3 };
+enum DetType : uint8_t { DET_TRI_RIGID = 2, DET_TRI_ROD = 3, DET_RIGID_RIGID = 6 };
 
 // CollisionConstraintEnergyInfo + TriangleRigid / TriangleRod info.
-// idx: triangle vertices, then the rigid body (tri-rigid) or both rod nodes.
+// idx: triangle vertices, then the rigid body (tri-rigid) or both rod nodes;
+// rigid-rigid: the two bodies (idx[0], idx[1]) with their body-frame contact
+// points cpl and cpl2.
 struct DynContact {
     uint32_t idx[5] = {0, 0, 0, 0, 0};
     Real k_start = 0, k_n = 0, k_t = 0, k_b = 0;
@@ -88,6 +93,7 @@ struct DynContact {
     V3 barys;
     Real s = 0;   // tri-rod: contact point along the segment
     V3 cpl;       // contact point in the body (tri-rigid) or interpolated rod frame
+    V3 cpl2;      // rigid-rigid: contact point in the second body's frame
 };
 
 struct DynPool {
@@ -106,10 +112,12 @@ struct Detected {
     V3 normal;
     uint32_t tri[3] = {0, 0, 0};
     V3 barys;
-    uint32_t rb = 0;                 // tri-rigid
+    uint32_t rb = 0;                 // tri-rigid; rigid-rigid: the first body
     uint32_t rod[2] = {0, 0};        // tri-rod
     Real s = 0;
     V3 cpl;
+    uint32_t rb2 = 0;                // rigid-rigid: the second body
+    V3 cpl2;
 };
 
 struct DynAdjEntry {
@@ -133,7 +141,10 @@ struct Recoloring {
 
 // Constraint value and gradients of one dynamic contact for one of its
 // particles (constraintGradientHessian<DOF>).  g*: 3 (local < 3) or 6 entries;
-// h*: the rotational 3x3 block (6-DOF, tri-rigid only; zero otherwise).
+// h*: the rotational 3x3 block (6-DOF, tri-rigid and rigid-rigid; zero
+// otherwise).  Rigid-rigid: C = dir . (cp_2 - cp_1) for dir = n, t, b with
+// cp_k = x_k + R_k cpl_k; local 0 is the first body (sign -1), local 1 the
+// second (sign +1); both bodies carry the normal and the friction terms.
 void dyn_constraint(uint8_t type, const DynContact& c, const std::vector<V3>& x, const RotState& rs, uint32_t local,
                     int dof, Real& Cn, Real& Ct, Real& Cb, Real gn[6], Real gt[6], Real gb[6], M3& hn, M3& ht, M3& hb);
 void dyn_eval(uint8_t type, const DynContact& c, const std::vector<V3>& x, const RotState& rs, Real& Cn, Real& Ct,
@@ -149,6 +160,12 @@ void update_collision_frame(DynContact& c, V3 new_normal);
 // CollisionDetector::_triangleSDF_CCD for a sphere SDF.  tri/tv: triangle
 // vertices and velocities; rb_*: body position, orientation, velocity and
 // body-frame angular velocity.
+// Sphere-sphere detection (new): bodies at xa, xb with radii ra, rb are in
+// contact when |xa - xb| < ra + rb + margin; n points from b to a, and the
+// contact points are the surface points facing each other, in body frames.
+bool sphere_sphere(V3 xa, const Quat& qa, Real ra, V3 xb, const Quat& qb, Real rb, Real margin, V3& normal, V3& cpl_a,
+                   V3& cpl_b);
+
 bool tri_sphere_ccd(const V3 tri[3], const V3 tv[3], V3 rb_pos, const Quat& rb_rot, V3 rb_v, V3 rb_w, Real radius,
                     Real dt, Real margin, V3& normal, V3& barys, V3& cpl);
 

@@ -314,7 +314,7 @@ def _detect_and_recolor(self):
 
 
 def _contacts(self, kind):
-    """Active contacts of pool `kind` (0 triangle-rigid, 1 triangle-rod) in
+    """Active contacts of pool `kind` (0 triangle-rigid, 1 triangle-rod, 2 rigid-rigid) in
     slot order, one row of 32 per contact (CONTACT_FIELDS)."""
     n = lib.oracle_num_contacts(self.h, kind)
     rec = np.zeros((max(n, 1), 32))

@@ -38,7 +38,7 @@ public:
     uint32_t next_object = 0;
     bool dynamic = false;
     Real collision_margin = 1e-3;
-    DynPool dyn[DYN_COUNT];   // TRIANGLE_RIGID_COLLISION, TRIANGLE_ROD_COLLISION
+    DynPool dyn[DYN_COUNT];   // TRIANGLE_RIGID_COLLISION, TRIANGLE_ROD_COLLISION, rigid-rigid (new)
     std::vector<Detected> prev_det, cur_det;
     std::vector<uint32_t> prev_order, cur_order;
     DynAdjacency dadj;

@@ -1,7 +1,8 @@
 """Dynamic contacts on the GPU against the FP64 oracle, through the C ABI:
-the device broad + narrow phase, the host contact lifecycle, dynamic
-adjacency and incremental recolouring, and the triangle-rigid /
-triangle-rod AL contacts in the 3-DOF and 6-DOF vertex solves.
+the device broad + narrow phase, the host contact lifecycle and dynamic
+adjacency, the device incremental recolouring, and the triangle-rigid /
+triangle-rod / rigid-rigid AL contacts in the 3-DOF and 6-DOF vertex
+solves.
 
 Bars: the detected contact set (keys), their pool slots, the colouring and
 the conflict set are bit-exact per step while the two trajectories agree
@@ -139,15 +140,51 @@ def test_contact_residual_matches_oracle():
 
 
 def test_unsupported_pair_fails_the_step():
+    """A rod segment against a sphere it is not welded to fails the step, as
+    the reference throws (collision/CollisionDetector.cpp:210-213)."""
     from avbd_amd._lib import AvbdError
-    sc = scenes.Scene("two_balls", spheres=[scenes.Sphere(0.05, 1000.0, (0, 0.5, 0), ground=False),
-                                            scenes.Sphere(0.05, 1000.0, (0.09, 0.5, 0), ground=False)],
+    rod = scenes.Rod(elements=2, length=0.2, E=1e7, nu=0.3, density=1000.0, diameter=0.02,
+                     pos=(-0.1, 0.5, 0.0), rot=scenes.quat_about((0, 1, 0), np.pi / 2), ground=False)
+    sc = scenes.Scene("rod_ball", rods=[rod], spheres=[scenes.Sphere(0.05, 1000.0, (0.0, 0.5, 0.055), ground=False)],
                       contact_margin=1e-3)
     sim = Simulation(sc)
     sim.setup()
     with pytest.raises(AvbdError) as e:
         sim.ctx.step(1)
     assert e.value.code == -4   # AVBD_E_UNSUPPORTED_CONTACT
+
+
+def test_sphere_pile_matches_oracle():
+    """Rigid-rigid (sphere-sphere) contacts, new: six spinning spheres settling
+    into a pile on the ground.  Per step the detected keys, pool slots and
+    kinds, the colouring and the conflict set are bit-exact with the oracle's,
+    the contact records (frame, both body-frame contact points) within 1e-3,
+    and the positions follow the oracle's; after 30 steps within 2e-3 x bbox."""
+    sc, o, sim = _pair("sphere_pile", steps=40)
+    ctx = sim.ctx
+    bb = _bbox(o.positions())
+    seen = 0
+    for step in range(30):
+        o.step(1)
+        ctx.step(1)
+        ctx.synchronize()
+        ko, so, to = o.detected()
+        kd, sd, td = ctx.detected()
+        assert np.array_equal(ko, kd), (step, ko, kd)
+        assert np.array_equal(so, sd) and np.array_equal(to, td)
+        assert set(to.tolist()) <= {6}
+        seen += len(ko)
+        col_o, k_o = o.colors()
+        col_d, k_d = ctx.coloring()
+        assert k_o == k_d and np.array_equal(col_o, col_d), step
+        assert np.array_equal(o.conflicted(), ctx.conflicted()), step
+        err = _compare_records(o.contacts(2), ctx.contacts(2), 1e-3)
+        xo, xd = o.positions(), ctx.positions()
+        dx = np.max(np.linalg.norm(xo - xd, axis=1)) / bb
+        if step % 5 == 0:
+            _log(f"sphere pile step {step}: {len(ko)} contacts, records err {err:.2e}, positions err {dx:.2e} x bbox")
+        assert dx < (1e-5 if step == 0 else min(2e-3, 1e-4 * (step + 1))), (step, dx)
+    assert seen > 0
 
 
 @pytest.mark.parametrize("name,steps,kind", [("tumor_rod", 30, 1), ("bunny_ball", 60, 0)])

@@ -152,6 +152,13 @@ def _rand_quat():
     return q / np.linalg.norm(q)
 
 
+def _qmat(q):
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
 def _plus(q, d):
     e = O.exp_s3(d)
     w1, x1, y1, z1 = q
@@ -330,11 +337,130 @@ def test_resting_ball_multipliers_carry_its_weight():
 
 
 def test_unsupported_pairs_raise():
-    """Sphere-sphere and rod-segment-sphere candidate pairs throw in the reference
-    (collision/CollisionDetector.cpp:210-213,686-687)."""
+    """Rod-segment-sphere candidate pairs throw as in the reference
+    (collision/CollisionDetector.cpp:210-213); sphere-sphere pairs, which the
+    reference also throws on (:686-687), are detected here (rigid-rigid
+    contacts, new)."""
     sc = scenes.Scene("two_balls", spheres=[scenes.Sphere(0.05, 1000.0, (0, 0.5, 0), ground=False),
                                             scenes.Sphere(0.05, 1000.0, (0.09, 0.5, 0), ground=False)],
                       contact_margin=1e-3)
     o = O.from_scene(sc)
+    o.detect_and_recolor()
+    keys, _, types = o.detected()
+    assert list(types) == [6] and int(keys[0]) == O.contact_key(6, 0, 1)
+    rod = scenes.Rod(elements=2, length=0.2, E=1e7, nu=0.3, density=1000.0, diameter=0.02,
+                     pos=(-0.1, 0.5, 0.0), rot=scenes.quat_about((0, 1, 0), np.pi / 2), ground=False)
+    sc2 = scenes.Scene("rod_ball", rods=[rod], spheres=[scenes.Sphere(0.05, 1000.0, (0.0, 0.5, 0.055), ground=False)],
+                       contact_margin=1e-3)
+    o2 = O.from_scene(sc2)
     with pytest.raises(O.OracleError):
-        o.detect_and_recolor()
+        o2.detect_and_recolor()
+
+
+def test_rigid_rigid_gradients_by_finite_differences():
+    """Rigid-rigid contact (new): C = dir . (x_2 + R_2 cpl_2 - x_1 - R_1 cpl_1)
+    for dir = n, t, b; the gradients of both bodies (position and the S3
+    perturbation q (+) dtheta) and the rotational Hessian blocks against
+    central differences.  Both bodies carry all three directions."""
+    x = rng.normal(size=(5, 3)) * 0.1
+    q = np.zeros((5, 4))
+    q[:, 0] = 1
+    q[3], q[4] = _rand_quat(), _rand_quat()
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    t, b = O.complete_basis(n)
+    cpl1, cpl2 = rng.normal(size=3) * 0.03, rng.normal(size=3) * 0.03
+    rec = _rec(n, t, b, cpl2, 0.0, cpl1)
+    h = 1e-6
+
+    def C(xx, qq):
+        return O.dyn_single(2, xx, qq, rec, 0, 6)[0]
+
+    c0 = C(x, q)
+    R1, R2 = _qmat(q[3]), _qmat(q[4])
+    d = x[4] + R2 @ cpl2 - x[3] - R1 @ cpl1
+    assert np.allclose(c0, [n @ d, t @ d, b @ d], atol=1e-14)
+    for loc in (0, 1):
+        p = 3 + loc
+        _, g, hblk, _, _ = O.dyn_single(2, x, q, rec, loc, 6)
+        fd = np.zeros((3, 6))
+        for i in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[p, i] += h
+            xm[p, i] -= h
+            fd[:, i] = (C(xp, q) - C(xm, q)) / (2 * h)
+            dv = np.zeros(3)
+            dv[i] = h
+            qp, qm = q.copy(), q.copy()
+            qp[p] = _plus(q[p], dv)
+            qm[p] = _plus(q[p], -dv)
+            fd[:, 3 + i] = (C(x, qp) - C(x, qm)) / (2 * h)
+        assert np.allclose(g, fd, atol=1e-6), (loc, g, fd)
+        for k in range(3):
+            H = np.zeros((3, 3))
+            for i in range(3):
+                for j in range(3):
+                    di, dj = np.zeros(3), np.zeros(3)
+                    di[i] = 1e-4
+                    dj[j] = 1e-4
+                    vals = []
+                    for si, sj in [(1, 1), (1, -1), (-1, 1), (-1, -1)]:
+                        qq = q.copy()
+                        qq[p] = _plus(q[p], si * di + sj * dj)
+                        vals.append(C(x, qq)[k])
+                    H[i, j] = (vals[0] - vals[1] - vals[2] + vals[3]) / (4e-8)
+            assert np.allclose(hblk[k], 0.5 * (H + H.T), atol=2e-5), (loc, k)
+
+
+def test_head_on_spheres_conserve_momentum_and_stop_at_contact():
+    """Two equal free spheres approaching head-on without gravity: the contact
+    forces are equal and opposite, so the total linear momentum stays zero and
+    the motion stays symmetric about the midpoint; the spheres stop at
+    contact (the AL normal force removes the approach speed, no sliding
+    friction acts on a head-on contact), and never overlap by more than a
+    fraction of a millimetre."""
+    r, rho, v0 = 0.05, 1000.0, 0.3
+    sc = scenes.Scene("head_on", spheres=[scenes.Sphere(r, rho, (-0.06, 0.5, 0), ground=False),
+                                          scenes.Sphere(r, rho, (0.06, 0.5, 0), ground=False)],
+                      contact_margin=1e-3, g=0.0, dt=1.0 / 120.0, iters=100)
+    o = O.from_scene(sc)
+    v = np.zeros((2, 3))
+    v[0, 0], v[1, 0] = v0, -v0
+    o.set_velocities(v, v)
+    m, _ = o.mass_fixed()
+    min_gap, seen = 1.0, 0
+    for _ in range(40):
+        o.step(1)
+        x = o.positions()
+        vv = o.velocities()
+        p = m[:, None] * vv
+        assert np.abs(p.sum(axis=0)).max() < 1e-6 * m[0] * v0
+        assert np.abs(x[0, 0] + x[1, 0]) < 1e-6
+        min_gap = min(min_gap, np.linalg.norm(x[1] - x[0]) - 2 * r)
+        seen = max(seen, len(o.contacts(2)))
+    assert seen == 1
+    assert min_gap > -5e-4
+    assert abs(o.velocities()[0, 0]) < 0.05 * v0
+
+
+def test_resting_sphere_on_a_fixed_sphere_carries_its_weight():
+    """A sphere dropped onto a fixed sphere straight below it comes to rest on
+    top; the normal multiplier of the rigid-rigid contact balances its weight
+    (AL fixed point with the per-step multiplier decay alpha * gamma)."""
+    r, rho = 0.05, 1000.0
+    sc = scenes.Scene("stack", spheres=[scenes.Sphere(r, rho, (0, 0.5, 0), ground=False, fixed=True),
+                                        scenes.Sphere(r, rho, (0, 0.5 + 2 * r + 0.002, 0), ground=False)],
+                      contact_margin=1e-3, dt=1.0 / 120.0, iters=50)
+    o = O.from_scene(sc)
+    o.step(240)
+    m, _ = o.mass_fixed()
+    w = m[1] * sc.g
+    x = o.positions()
+    assert abs(x[1, 0]) < 1e-9 and abs(x[1, 2]) < 1e-9          # stays on the axis
+    assert abs(np.linalg.norm(x[1] - x[0]) - 2 * r) < 2e-3      # resting on top
+    rec = o.contacts(2)
+    assert len(rec) == 1
+    n = rec[0, 6:9]
+    assert abs(abs(n[1]) - 1.0) < 1e-9
+    fy = rec[0, 26] * abs(n[1])
+    assert abs(fy - w) < 0.05 * w, (fy, w)
