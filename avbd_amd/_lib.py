@@ -28,6 +28,7 @@ RIGID_BODY_GROUND_COLLISION = 3
 TRIANGLE_RIGID_COLLISION = 4
 TRIANGLE_ROD_COLLISION = 5
 ONE_SIDED_FIXED_JOINT = 6
+FIXED_JOINT = 7   # two-sided fixed joint (new)
 
 FLAG_NO_GRAPH = 1
 FLAG_SERIAL_BROADPHASE = 2
@@ -133,6 +134,18 @@ class JointRecord(C.Structure):
         ("rotation_offset", C.c_double * 4),
         ("ref_position", C.c_double * 3),
         ("ref_rotation", C.c_double * 4),
+        ("k", C.c_double * 6),
+        ("lambda_", C.c_double * 6),
+        ("C_prev", C.c_double * 6),
+    ]
+
+
+class FixedJointRecord(C.Structure):
+    """avbd_fixed_joint_record: two-sided fixed joint between two oriented particles."""
+    _fields_ = [
+        ("particle_index", C.c_uint32 * 2),
+        ("position_offset", (C.c_double * 3) * 2),
+        ("rotation_offset", (C.c_double * 4) * 2),
         ("k", C.c_double * 6),
         ("lambda_", C.c_double * 6),
         ("C_prev", C.c_double * 6),

@@ -42,6 +42,7 @@ struct avbd_ctx {
     std::vector<HostCos> cos;
     std::vector<HostRGround> rground;
     std::vector<HostJoint> joints;
+    std::vector<HostJoint2> joints2;       // two-sided fixed joints (new)
 
     bool planned = false;   // host colour plan done
     bool built = false;     // device buffers uploaded
@@ -313,7 +314,23 @@ JointRec to_dev(const HostJoint& h) {
         r.C_prev[i] = (float)h.C_prev[i];
     }
     r.p = h.p;
-    r.pad = 0;
+    r.p2 = JOINT_ONE_SIDED;
+    return r;
+}
+
+JointRec to_dev(const HostJoint2& h) {
+    JointRec r{};
+    r.pos_off = make_float4((float)h.pos_offset[0][0], (float)h.pos_offset[0][1], (float)h.pos_offset[0][2], 0.f);
+    r.rot_off = qf4(h.rot_offset[0]);
+    r.ref_pos = make_float4((float)h.pos_offset[1][0], (float)h.pos_offset[1][1], (float)h.pos_offset[1][2], 0.f);
+    r.ref_rot = qf4(h.rot_offset[1]);
+    for (int i = 0; i < 6; i++) {
+        r.k[i] = (float)h.k[i];
+        r.lambda[i] = (float)h.lambda[i];
+        r.C_prev[i] = (float)h.C_prev[i];
+    }
+    r.p = h.p[0];
+    r.p2 = h.p[1];
     return r;
 }
 
@@ -525,8 +542,12 @@ void enqueue_head(avbd_ctx* c, bool with_events) {
     // dynamic contacts' updateAfterTimeStep reads the start-of-step positions
     const uint32_t n_active = c->dynamic ? (uint32_t)c->cs.active.size() : 0u;
     if (n_active) launch_dyn_step_begin(s, c->d, static_cast<const uint32_t*>(c->g_active.p), n_active);
-    if (c->fp64) launch_step_begin_f64(s, c->f, c->d.n, c->desc.dt, c->desc.g_accel);
-    else launch_step_begin(s, c->d, dt, g);
+    if (c->fp64) {
+        launch_step_begin_f64(s, c->f, c->d.n, c->desc.dt, c->desc.g_accel);
+    } else {
+        launch_joint2_step_begin(s, c->d);   // reads both bodies before the inertial update
+        launch_step_begin(s, c->d, dt, g);
+    }
     // learned warm start / recording: features, the network, the choice of the start
     if (c->warm || c->recording) {
         WarmIO io = c->wio;
@@ -615,6 +636,7 @@ void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool
             if (ce > cb) launch_commit(s, di, c->d_conflicts + cb, ce - cb, done);
         }
         if (!fuse) launch_cheb_dual(s, dz, omega, rc, c->omega_tab, j);
+        launch_joint2_dual(s, dz, done);   // oriented particles: never fused
         if (n_active) launch_dyn_dual(s, c->d, d_active, n_active, done);
     }
     if (check) {
@@ -1217,6 +1239,19 @@ int avbd_add_energies(avbd_ctx* c, int type, const void* records, uint32_t count
                 std::memcpy(h.C_prev, r[i].C_prev, sizeof(h.C_prev));
                 c->joints.push_back(h);
             }
+        } else if (type == AVBD_FIXED_JOINT) {
+            check_capacity(c, c->joints2.size() + count, c->desc.energy_capacity);
+            const auto* r = static_cast<const avbd_fixed_joint_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                HostJoint2 h{};
+                std::memcpy(h.p, r[i].particle_index, sizeof(h.p));
+                std::memcpy(h.pos_offset, r[i].position_offset, sizeof(h.pos_offset));
+                std::memcpy(h.rot_offset, r[i].rotation_offset, sizeof(h.rot_offset));
+                std::memcpy(h.k, r[i].k, sizeof(h.k));
+                std::memcpy(h.lambda, r[i].lambda, sizeof(h.lambda));
+                std::memcpy(h.C_prev, r[i].C_prev, sizeof(h.C_prev));
+                c->joints2.push_back(h);
+            }
         } else {
             throw CodeError(AVBD_E_ARG, "energy type not supported by this build");
         }
@@ -1611,6 +1646,7 @@ int avbd_build_graph(avbd_ctx* c) {
             m.cos = &c->cos;
             m.rground = &c->rground;
             m.joints = &c->joints;
+            m.joints2 = &c->joints2;
             m.fixed = &c->fixed;
             m.rot_of = &c->rot_of;
             m.scene_offsets = &so;
@@ -1677,7 +1713,8 @@ int avbd_build_graph(avbd_ctx* c) {
         d.n_rot = (uint32_t)(c->q.size() / 4);
         d.n_cos = (uint32_t)c->cos.size();
         d.n_rground = (uint32_t)c->rground.size();
-        d.n_joints = (uint32_t)c->joints.size();
+        d.n_joints1 = (uint32_t)c->joints.size();
+        d.n_joints = d.n_joints1 + (uint32_t)c->joints2.size();
         d.n_oitems = (uint32_t)c->plan.oitems.size();
         d.rot_of = dev_alloc<int32_t>(c, n);
         d.q = dev_alloc<float4>(c, d.n_rot);
@@ -1702,7 +1739,8 @@ int avbd_build_graph(avbd_ctx* c) {
             std::vector<RGroundRec> rr(d.n_rground);
             for (uint32_t i = 0; i < d.n_rground; i++) rr[i] = to_dev(c->rground[i]);
             std::vector<JointRec> jr(d.n_joints);
-            for (uint32_t i = 0; i < d.n_joints; i++) jr[i] = to_dev(c->joints[i]);
+            for (uint32_t i = 0; i < d.n_joints1; i++) jr[i] = to_dev(c->joints[i]);
+            for (uint32_t i = d.n_joints1; i < d.n_joints; i++) jr[i] = to_dev(c->joints2[i - d.n_joints1]);
             upload(c, d.cos, cr);
             upload(c, d.rground, rr);
             upload(c, d.joints, jr);
@@ -2229,8 +2267,8 @@ int avbd_download_energies(avbd_ctx* c, int type, void* records, uint32_t count)
                 r[i].mu_k = g.mu_k;
             }
         } else if (type == AVBD_ONE_SIDED_FIXED_JOINT) {
-            if (count > c->d.n_joints) throw std::runtime_error("count exceeds the pool");
-            std::vector<JointRec> jr(c->d.n_joints);
+            if (count > c->d.n_joints1) throw std::runtime_error("count exceeds the pool");
+            std::vector<JointRec> jr(c->d.n_joints1);
             if (!jr.empty())
                 hip_check(hipMemcpyAsync(jr.data(), c->d.joints, jr.size() * sizeof(JointRec), hipMemcpyDeviceToHost,
                                          c->stream),
@@ -2245,6 +2283,27 @@ int avbd_download_energies(avbd_ctx* c, int type, void* records, uint32_t count)
                 std::memcpy(r[i].rotation_offset, h.rot_offset, sizeof(h.rot_offset));
                 std::memcpy(r[i].ref_position, h.ref_position, sizeof(h.ref_position));
                 std::memcpy(r[i].ref_rotation, h.ref_rotation, sizeof(h.ref_rotation));
+                for (int k = 0; k < 6; k++) {
+                    r[i].k[k] = jr[i].k[k];
+                    r[i].lambda[k] = jr[i].lambda[k];
+                    r[i].C_prev[k] = jr[i].C_prev[k];
+                }
+            }
+        } else if (type == AVBD_FIXED_JOINT) {
+            const uint32_t n1 = c->d.n_joints1, n2 = c->d.n_joints - c->d.n_joints1;
+            if (count > n2) throw std::runtime_error("count exceeds the pool");
+            std::vector<JointRec> jr(n2);
+            if (n2)
+                hip_check(hipMemcpyAsync(jr.data(), c->d.joints + n1, n2 * sizeof(JointRec), hipMemcpyDeviceToHost,
+                                         c->stream),
+                          "D2H");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            auto* r = static_cast<avbd_fixed_joint_record*>(records);
+            for (uint32_t i = 0; i < count; i++) {
+                const HostJoint2& h = c->joints2[i];
+                std::memcpy(r[i].particle_index, h.p, sizeof(h.p));
+                std::memcpy(r[i].position_offset, h.pos_offset, sizeof(h.pos_offset));
+                std::memcpy(r[i].rotation_offset, h.rot_offset, sizeof(h.rot_offset));
                 for (int k = 0; k < 6; k++) {
                     r[i].k[k] = jr[i].k[k];
                     r[i].lambda[k] = jr[i].lambda[k];
@@ -2339,6 +2398,7 @@ uint32_t avbd_num_energies(avbd_ctx* c, int type) {
     if (type == AVBD_COSSERAT_ROD) return (uint32_t)c->cos.size();
     if (type == AVBD_RIGID_BODY_GROUND_COLLISION) return (uint32_t)c->rground.size();
     if (type == AVBD_ONE_SIDED_FIXED_JOINT) return (uint32_t)c->joints.size();
+    if (type == AVBD_FIXED_JOINT) return (uint32_t)c->joints2.size();
     return 0;
 }
 

@@ -63,9 +63,11 @@ struct SceneGraph {
 
 // Vertex-vertex adjacency of one scene (StaticParticleAdjacency::buildAdjacency,
 // common/StaticParticleAdjacency.hpp:42-133): deduplicated neighbour sets of the
-// static multi-particle energies (Neo-Hookean tets and Cosserat elements).
+// static multi-particle energies (Neo-Hookean tets and Cosserat elements, and
+// the two-sided fixed joints, new).
 void scene_adjacency(uint32_t lo, uint32_t hi, const std::vector<HostNH>& nh, const std::vector<uint32_t>& tets,
-                     const std::vector<HostCos>& cos, const std::vector<uint32_t>& rods, SceneGraph& g) {
+                     const std::vector<HostCos>& cos, const std::vector<uint32_t>& rods,
+                     const std::vector<HostJoint2>& joints2, const std::vector<uint32_t>& links, SceneGraph& g) {
     const uint32_t n = hi - lo;
     std::vector<std::vector<uint32_t>> sets(n);
     for (uint32_t t : tets) {
@@ -77,6 +79,10 @@ void scene_adjacency(uint32_t lo, uint32_t hi, const std::vector<HostNH>& nh, co
     for (uint32_t e : rods) {
         sets[cos[e].idx[0] - lo].push_back(cos[e].idx[1] - lo);
         sets[cos[e].idx[1] - lo].push_back(cos[e].idx[0] - lo);
+    }
+    for (uint32_t e : links) {
+        sets[joints2[e].p[0] - lo].push_back(joints2[e].p[1] - lo);
+        sets[joints2[e].p[1] - lo].push_back(joints2[e].p[0] - lo);
     }
     g.off.assign(n + 1, 0);
     for (uint32_t v = 0; v < n; v++) {
@@ -109,6 +115,8 @@ void build_color_plan(const HostModel& m, ColorPlan& plan, const SceneColourer& 
     const auto& cos = *m.cos;
     const auto& rground = *m.rground;
     const auto& joints = *m.joints;
+    static const std::vector<HostJoint2> no_joints2;
+    const auto& joints2 = m.joints2 ? *m.joints2 : no_joints2;
     const auto& fixed = *m.fixed;
     const auto& rot_of = *m.rot_of;
     const auto& scene_offsets = *m.scene_offsets;
@@ -145,6 +153,15 @@ void build_color_plan(const HostModel& m, ColorPlan& plan, const SceneColourer& 
         if (r.p >= n || !oriented(r.p)) throw std::runtime_error("rigid-ground contact needs an oriented particle");
     for (const auto& j : joints)
         if (j.p >= n || !oriented(j.p)) throw std::runtime_error("joint needs an oriented particle");
+    std::vector<std::vector<uint32_t>> scene_links(n_scenes);
+    for (uint32_t e = 0; e < joints2.size(); e++) {
+        const auto& j = joints2[e];
+        for (int k = 0; k < 2; k++)
+            if (j.p[k] >= n || !oriented(j.p[k])) throw std::runtime_error("joint needs an oriented particle");
+        if (j.p[0] == j.p[1]) throw std::runtime_error("two-sided joint with one particle twice");
+        if (scene_of[j.p[0]] != scene_of[j.p[1]]) throw std::runtime_error("joint spans two scenes");
+        scene_links[scene_of[j.p[0]]].push_back(e);
+    }
 
     // vertex adjacency of every scene (threads over scenes), kept as one
     // global CSR (also used by the incremental recolouring)
@@ -156,7 +173,7 @@ void build_color_plan(const HostModel& m, ColorPlan& plan, const SceneColourer& 
             pool.emplace_back([&, w] {
                 for (uint32_t s = w; s < n_scenes; s += hw)
                     scene_adjacency(scene_offsets[s], scene_offsets[s + 1], nh, scene_tets[s], cos, scene_rods[s],
-                                    graphs[s]);
+                                    joints2, scene_links[s], graphs[s]);
             });
         for (auto& t : pool) t.join();
     }
@@ -220,6 +237,15 @@ void build_color_plan(const HostModel& m, ColorPlan& plan, const SceneColourer& 
     for (uint32_t e = 0; e < joints.size(); e++) pr.emplace_back(joints[e].p, e);
     std::vector<uint32_t> jt_off, jt_all;
     csr_by_particle(n, pr, jt_off, jt_all);
+    // the solve's joint terms: the one-sided joints, then both bodies of every
+    // two-sided one (records follow the one-sided ones on the device)
+    const uint32_t n_j1 = (uint32_t)joints.size();
+    for (uint32_t e = 0; e < joints2.size(); e++) {
+        pr.emplace_back(joints2[e].p[0], n_j1 + e);
+        pr.emplace_back(joints2[e].p[1], (n_j1 + e) | OINC_JOINT_SIDE1);
+    }
+    std::vector<uint32_t> js_off, js_all;
+    csr_by_particle(n, pr, js_off, js_all);
 
     std::vector<int32_t> ground_of(n, -1);
     for (uint32_t gi = 0; gi < ground.size(); gi++) {
@@ -282,11 +308,11 @@ void build_color_plan(const HostModel& m, ColorPlan& plan, const SceneColourer& 
                 it.inc_begin = (uint32_t)plan.oinc.size();
                 it.n_cos = cos_off[v + 1] - cos_off[v];
                 it.n_rg = rg_off[v + 1] - rg_off[v];
-                it.n_joint = jt_off[v + 1] - jt_off[v];
+                it.n_joint = js_off[v + 1] - js_off[v];
                 it.ground = ground_of[v];
                 plan.oinc.insert(plan.oinc.end(), cos_all.begin() + cos_off[v], cos_all.begin() + cos_off[v + 1]);
                 plan.oinc.insert(plan.oinc.end(), rg_all.begin() + rg_off[v], rg_all.begin() + rg_off[v + 1]);
-                plan.oinc.insert(plan.oinc.end(), jt_all.begin() + jt_off[v], jt_all.begin() + jt_off[v + 1]);
+                plan.oinc.insert(plan.oinc.end(), js_all.begin() + js_off[v], js_all.begin() + js_off[v + 1]);
                 plan.oitems.push_back(it);
                 plan.item_kind[v] = 2;
                 plan.oitem_of[v] = it;

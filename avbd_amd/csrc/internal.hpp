@@ -97,16 +97,23 @@ constexpr uint32_t SHAPE_SPHERE = 2u;
 
 // One-sided fixed joint (OneSidedFixedJointEnergyInfo,
 // energy/joint/OneSidedFixedJointEnergyPool.hpp:8-16) with its vector AL state.
+// A two-sided fixed joint (new) uses the same record: p2 is its second body,
+// and ref_pos / ref_rot hold that body's position and rotation offsets
+// instead of a fixed world pose.
 struct alignas(16) JointRec {
     float4 pos_off;   // xyz
     float4 rot_off;   // quaternion (x, y, z, w)
-    float4 ref_pos;   // xyz
-    float4 ref_rot;   // quaternion (x, y, z, w)
+    float4 ref_pos;   // xyz (two-sided: body p2's position offset)
+    float4 ref_rot;   // quaternion (x, y, z, w) (two-sided: body p2's rotation offset)
     float k[6], lambda[6], C_prev[6];
-    uint32_t p, pad;
+    uint32_t p, p2;   // p2 = JOINT_ONE_SIDED for the one-sided joint
 };
 static_assert(sizeof(JointRec) == 144, "JointRec must be 144 bytes");
 constexpr float kJointKStart = 1e4f;   // OneSidedFixedJointEnergyPool k_start
+constexpr uint32_t JOINT_ONE_SIDED = 0xFFFFFFFFu;
+// oriented incidence entry of a joint: the joint record, with this bit set
+// when the particle is the second body of a two-sided joint
+constexpr uint32_t OINC_JOINT_SIDE1 = 0x80000000u;
 
 // Work item of an oriented particle: vertex, rotation slot, first entry in
 // the oriented incidence list, counts of Cosserat incidences ((e << 1) | local,
@@ -212,6 +219,7 @@ struct DevState {
     uint32_t* prec_off = nullptr;  // per rotation slot: rigid-ground / joint records
     uint32_t* prec = nullptr;
     uint32_t n_rot = 0, n_cos = 0, n_rground = 0, n_joints = 0, n_oitems = 0;
+    uint32_t n_joints1 = 0;   // joints [0, n_joints1) are one-sided, the rest two-sided
 
     // dynamic contacts (null unless enabled): per-particle CSR of incident
     // records, entries (record << 3) | local, in the reference's dynamic
@@ -351,6 +359,10 @@ void launch_dyn_apply(hipStream_t s, const DevState& d, const DynOp* ops, uint32
 void launch_dyn_step_begin(hipStream_t s, const DevState& d, const uint32_t* active, uint32_t n_active);
 void launch_dyn_dual(hipStream_t s, const DevState& d, const uint32_t* active, uint32_t n_active,
                      const uint32_t* done = nullptr);
+// two-sided fixed joints: start-of-step update (before launch_step_begin) and
+// dual update (after the Chebyshev step); no-ops without such joints
+void launch_joint2_step_begin(hipStream_t s, const DevState& d);
+void launch_joint2_dual(hipStream_t s, const DevState& d, const uint32_t* done = nullptr);
 // plain streaming copy of n_float4 16-byte words (HBM counter calibration)
 void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n_float4);
 // residual/IP pass: writes 3 partial sums per block into d.reduce; returns block count
@@ -416,6 +428,12 @@ struct HostJoint {
     double ref_position[3], ref_rotation[4];
     double k[6], lambda[6], C_prev[6];
 };
+// Two-sided fixed joint between oriented particles p[0] and p[1] (new).
+struct HostJoint2 {
+    uint32_t p[2];
+    double pos_offset[2][3], rot_offset[2][4];   // quaternions (w, x, y, z)
+    double k[6], lambda[6], C_prev[6];
+};
 
 // Everything the colour plan is built from.
 struct HostModel {
@@ -425,6 +443,7 @@ struct HostModel {
     const std::vector<HostCos>* cos = nullptr;
     const std::vector<HostRGround>* rground = nullptr;
     const std::vector<HostJoint>* joints = nullptr;
+    const std::vector<HostJoint2>* joints2 = nullptr;
     const std::vector<uint8_t>* fixed = nullptr;
     const std::vector<int32_t>* rot_of = nullptr;
     const std::vector<uint32_t>* scene_offsets = nullptr;

@@ -289,11 +289,98 @@ __device__ __forceinline__ void joint_dual(JointRec& j, float3 p, float4 q) {
 }
 
 // ---------------------------------------------------------------------------
+// Two-sided fixed joint (new: the reference's joint holds one particle to a
+// fixed world pose).  Frame i = (x_i + R_i o_i, q_i r_i) with body 0's
+// offsets in pos_off / rot_off and body 1's in ref_pos / ref_rot;
+// C = [frame_0 position - frame_1 position ; Minus_S3(frame_0 rotation,
+// frame_1 rotation)], so a fixed body 1 is the one-sided joint.  Rows of the
+// gradient: body 0 [I, -R_0 [o_0]x ; 0, J_r^-1(theta) R(r_0)^T], body 1
+// [-I, R_1 [o_1]x ; 0, -J_r^-1(-theta) R(r_1)^T].  Same vector AL.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void joint2_eval(const JointRec& j, float3 p0, float4 q0, float3 p1, float4 q1,
+                                            float C[6]) {
+    const float3 a = add(p0, qrot(q0, xyz(j.pos_off)));
+    const float3 b = add(p1, qrot(q1, xyz(j.ref_pos)));
+    const float3 th = minus_s3(qmul(q0, j.rot_off), qmul(q1, j.ref_rot));
+    C[0] = a.x - b.x;
+    C[1] = a.y - b.y;
+    C[2] = a.z - b.z;
+    C[3] = th.x;
+    C[4] = th.y;
+    C[5] = th.z;
+}
+
+// G, H of body `side` (pos, q: that body; po, qo: the other one)
+__device__ __forceinline__ void joint2_accumulate(const JointRec& j, uint32_t side, float3 pos, float4 q, float3 po,
+                                                  float4 qo, Vec6& G, Sym6& H) {
+    float C[6];
+    if (side == 0) joint2_eval(j, pos, q, po, qo, C);
+    else joint2_eval(j, po, qo, pos, q, C);
+    const float sg = side == 0 ? 1.f : -1.f;
+    const float3 off = side == 0 ? xyz(j.pos_off) : xyz(j.ref_pos);
+    const float4 roff = side == 0 ? j.rot_off : j.ref_rot;
+    const M3f dCp = mscale(-sg, mmul(qmat(q), skew(off)));
+    const M3f dCR = mscale(sg, mmul(jr_inv(v3(sg * C[3], sg * C[4], sg * C[5])), mtrans(qmat(roff))));
+#pragma unroll
+    for (int ci = 0; ci < 6; ci++) {
+        float g[6];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            g[a] = (ci < 3 && a == ci) ? sg : 0.f;
+            g[3 + a] = ci < 3 ? dCp.m[ci][a] : dCR.m[ci - 3][a];
+        }
+        const float lam = j.k[ci] * (C[ci] - kAlpha * j.C_prev[ci]) + j.lambda[ci];
+#pragma unroll
+        for (int a = 0; a < 6; a++) G.v[a] += g[a] * lam;
+        sym6_outer(H, g, j.k[ci]);
+    }
+}
+
+// Body frames (x, q) of both bodies of a two-sided joint from the state.
+__device__ __forceinline__ void joint2_bodies(const DevState& d, const JointRec& j, float3& p0, float4& q0,
+                                              float3& p1, float4& q1) {
+    p0 = xyz(d.x[j.p]);
+    q0 = d.q[d.rot_of[j.p]];
+    p1 = xyz(d.x[j.p2]);
+    q1 = d.q[d.rot_of[j.p2]];
+}
+
+// HardConstraintEnergySolver::updateAfterTimeStep / updateAfterIteration
+// (HardConstraintEnergySolver.hpp:42-88) with the two-sided constraint.
+__device__ __forceinline__ void joint2_step_begin(const DevState& d, JointRec& j) {
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        j.lambda[i] = kAlpha * kGamma * j.lambda[i];
+        j.k[i] = fmaxf(kJointKStart, kGamma * j.k[i]);
+    }
+    float3 p0, p1;
+    float4 q0, q1;
+    joint2_bodies(d, j, p0, q0, p1, q1);
+    joint2_eval(j, p0, q0, p1, q1, j.C_prev);
+}
+
+__device__ __forceinline__ void joint2_dual(const DevState& d, JointRec& j) {
+    float3 p0, p1;
+    float4 q0, q1;
+    joint2_bodies(d, j, p0, q0, p1, q1);
+    float C[6];
+    joint2_eval(j, p0, q0, p1, q1, C);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const float Cc = C[i] - kAlpha * j.C_prev[i];
+        const float lp = j.k[i] * Cc + j.lambda[i];
+        if (lp > -3.402823466e38f && lp < 3.402823466e38f) j.k[i] += kBeta * Cc;
+        j.lambda[i] = lp;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Per-particle phases of an oriented particle (rotation slot r)
 // ---------------------------------------------------------------------------
 
-// Start-of-step updates of the particle's rigid-ground and joint records
-// (all read the start-of-step state, before the inertial update).
+// Start-of-step updates of the particle's rigid-ground and one-sided joint
+// records (all read the start-of-step state, before the inertial update).
+// Two-sided joints depend on two particles and have kernels of their own.
 __device__ __forceinline__ void records_step_begin(const DevState& d, uint32_t p, uint32_t r) {
     const float3 pos = xyz(d.x[p]);
     const float4 q = d.q[r];
@@ -402,7 +489,17 @@ __device__ __forceinline__ void accumulate6(const DevState& d, const OItem& it, 
         H.a[s6(2, 2)] += c.k_b;
     }
     for (uint32_t e = 0; e < it.n_rg; e++, k++) rg_accumulate(d.rground[d.oinc[k]], pos, q, G, H);
-    for (uint32_t e = 0; e < it.n_joint; e++, k++) joint_accumulate(d.joints[d.oinc[k]], pos, q, G, H);
+    for (uint32_t e = 0; e < it.n_joint; e++, k++) {
+        const uint32_t code = d.oinc[k];
+        const JointRec& j = d.joints[code & ~OINC_JOINT_SIDE1];
+        if (j.p2 == JOINT_ONE_SIDED) {
+            joint_accumulate(j, pos, q, G, H);
+        } else {
+            const uint32_t side = code >> 31;
+            const uint32_t other = side ? j.p : j.p2;
+            joint2_accumulate(j, side, pos, q, xyz(d.x[other]), d.q[d.rot_of[other]], G, H);
+        }
+    }
     // dynamic contacts after the static energies (simulation/VBDSolver.hpp:400-438)
     if constexpr (DYN) {
         const dyn::View view = dyn::view_of(d);

@@ -747,6 +747,28 @@ __global__ void dyn_step_begin_kernel(DevState d, const uint32_t* __restrict__ a
     d.dyn[active[k]] = c;
 }
 
+// Two-sided fixed joints (records [n_joints1, n_joints)): their start-of-step
+// update, before the inertial update moves either body, and their dual /
+// stiffness update after the Chebyshev step of both bodies.  Kept out of the
+// per-particle kernels because each reads two particles.
+__global__ void joint2_step_begin_kernel(DevState d) {
+    const uint32_t e = d.n_joints1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.n_joints) return;
+    JointRec j = d.joints[e];
+    rigid::joint2_step_begin(d, j);
+    d.joints[e] = j;
+}
+
+__global__ void joint2_dual_kernel(DevState d, const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    const uint32_t e = d.n_joints1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.n_joints) return;
+    JointRec j = d.joints[e];
+    if (d.frozen && d.frozen[d.scene_of[j.p]]) return;
+    rigid::joint2_dual(d, j);
+    d.joints[e] = j;
+}
+
 __global__ void dyn_dual_kernel(DevState d, const uint32_t* __restrict__ active, uint32_t n_active,
                                 const uint32_t* __restrict__ done) {
     if (done && *done) return;
@@ -1459,6 +1481,16 @@ void launch_dyn_dual(hipStream_t s, const DevState& d, const uint32_t* active, u
     if (n_active == 0) return;
     hipLaunchKernelGGL(dyn_dual_kernel, dim3(blocks_for(n_active, 256)), dim3(256), 0, s, d, active, n_active,
                        done);
+}
+
+void launch_joint2_step_begin(hipStream_t s, const DevState& d) {
+    const uint32_t n2 = d.n_joints - d.n_joints1;
+    if (n2) hipLaunchKernelGGL(joint2_step_begin_kernel, dim3(blocks_for(n2, 256)), dim3(256), 0, s, d);
+}
+
+void launch_joint2_dual(hipStream_t s, const DevState& d, const uint32_t* done) {
+    const uint32_t n2 = d.n_joints - d.n_joints1;
+    if (n2) hipLaunchKernelGGL(joint2_dual_kernel, dim3(blocks_for(n2, 256)), dim3(256), 0, s, d, done);
 }
 
 void launch_solve_color(hipStream_t s, const DevState& d, const LaunchCfg& cfg, uint32_t begin, uint32_t count,

@@ -151,10 +151,37 @@ class Rod:
     tip_fixed: bool = False
     ground: bool = False
     base_joint: bool = False
-    # rigid sphere welded at the tip node: (radius, mass, body-frame angular
-    # velocity); adds its mass and inertia 2/5 m r^2 to the node and a sphere
-    # ground contact
+    # rigid sphere at the tip: (radius, mass, body-frame angular velocity).
+    # Welded (tip_joint False): its mass and inertia 2/5 m r^2 are added to
+    # the tip node, which gets a sphere ground contact.  Jointed (tip_joint
+    # True, straight rods only): a sphere body of its own centred one radius
+    # beyond the tip along the rod axis, held to the tip node by a two-sided
+    # fixed joint at the tip, with the sphere's ground contact.
     tip_sphere: Optional[Tuple[float, float, Tuple[float, float, float]]] = None
+    tip_joint: bool = False
+    collide: bool = True   # rod segments are collision primitives when the scene has contacts
+
+
+def quat_rotate(q, v) -> np.ndarray:
+    """Rotate v by the unit quaternion q = (w, x, y, z)."""
+    w, u = q[0], np.asarray(q[1:], dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def tip_body(r: "Rod"):
+    """Centre and density of a jointed tip sphere (mass m exactly as the
+    sphere constructor's 4/3 pi r^3 density) and the joint's offsets: at the
+    tip node's origin, and one radius back along the sphere's body z axis."""
+    if any(c != 0.0 for c in r.curvature):
+        raise ValueError("a jointed tip sphere needs a straight rod")
+    rad, m, _ = r.tip_sphere
+    centre = np.asarray(r.pos, dtype=np.float64) + quat_rotate(r.rot, (0.0, 0.0, r.length + rad))
+    density = m / (4.0 / 3.0 * np.pi * rad ** 3)
+    pos_offsets = ((0.0, 0.0, 0.0), (0.0, 0.0, -rad))
+    rot_offsets = ((1.0, 0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0))
+    return centre, density, pos_offsets, rot_offsets
 
 
 @dataclass
@@ -299,14 +326,18 @@ def rods_and_spheres(n_rods: int = 4, elements: int = 16, seed: int = 3, dt: flo
 
 
 def rod_grid(nx: int = 16, nz: int = 16, segments: int = 128, seed: int = 3, dt: float = 1.0 / 120.0,
-             iters: int = 20, steps: int = 120) -> Scene:
+             iters: int = 20, steps: int = 120, tip_joint: bool = False,
+             contact_margin: Optional[float] = None) -> Scene:
     """Rods + rigid bodies: nx x nz Cosserat rods at 5 cm spacing, 1 m long,
     radius 5 mm, E = 1e7 Pa, nu = 0.3 (G = E/2.6), density 1100 kg/m^3, hanging
     down from a one-sided fixed joint at the top node, a rigid sphere
-    (r = 2 cm, m = 0.05 kg) welded at the tip with angular velocity
-    U[-5, 5]^3 rad/s (SplitMix64(seed)); every node and sphere has a
-    rigid-ground contact.  Tops at y = 1.05 m, so the spheres hang 3 cm above
-    the ground."""
+    (r = 2 cm, m = 0.05 kg) at the tip with angular velocity U[-5, 5]^3 rad/s
+    (SplitMix64(seed)); every node and sphere has a rigid-ground contact.
+    Tops at y = 1.05 m.  tip_joint False: the sphere is welded into the tip
+    node (centre at the tip, 3 cm above the ground); True: it is a body of its
+    own below the tip (1 cm above the ground), held by a two-sided fixed
+    joint.  contact_margin: dynamic contacts between the spheres (the rod
+    segments are not collision primitives)."""
     rng = SplitMix64(seed)
     down = quat_about((1.0, 0.0, 0.0), np.pi / 2)   # body z -> world -y
     rods = []
@@ -315,8 +346,8 @@ def rod_grid(nx: int = 16, nz: int = 16, segments: int = 128, seed: int = 3, dt:
             w0 = tuple(float(10.0 * rng.uniform() - 5.0) for _ in range(3))
             rods.append(Rod(elements=segments, length=1.0, E=1e7, nu=0.3, density=1100.0, diameter=0.01,
                             pos=(0.05 * i, 1.05, 0.05 * k), rot=down, ground=True, base_joint=True,
-                            tip_sphere=(0.02, 0.05, w0)))
-    return Scene("rod_grid", rods=rods, dt=dt, iters=iters, rho=0.95, steps=steps)
+                            tip_sphere=(0.02, 0.05, w0), tip_joint=tip_joint, collide=contact_margin is None))
+    return Scene("rod_grid", rods=rods, dt=dt, iters=iters, rho=0.95, steps=steps, contact_margin=contact_margin)
 
 
 def soft_block(n: int, edge: float, E: float = 2e4, nu: float = 0.45, density: float = 1000.0) -> TetMesh:

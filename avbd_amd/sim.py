@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .scenes import Rod, Scene, Sphere, TetMesh
+from .scenes import Rod, Scene, Sphere, TetMesh, tip_body
 
 
 def _dptr(a: Optional[np.ndarray]):
@@ -175,6 +175,21 @@ class SimulationContext:
         recs = (_lib.JointRecord * 1)(r)
         self.add_energies(_lib.ONE_SIDED_FIXED_JOINT, recs)
 
+    def add_fixed_joint(self, p0: int, p1: int, pos_offsets, rot_offsets) -> None:
+        """Two-sided fixed joint (new) between oriented particles p0 and p1:
+        pos_offsets (2, 3) and rot_offsets (2, 4) place the joint frame on each
+        body; k = 1e4 (the one-sided joint's k_start), lambda = 0, C_prev = 0."""
+        r = _lib.FixedJointRecord()
+        r.particle_index[0], r.particle_index[1] = p0, p1
+        for i in range(2):
+            for k in range(3):
+                r.position_offset[i][k] = pos_offsets[i][k]
+            for k in range(4):
+                r.rotation_offset[i][k] = rot_offsets[i][k]
+        for k in range(6):
+            r.k[k] = 1e4
+        self.add_energies(_lib.FIXED_JOINT, (_lib.FixedJointRecord * 1)(r))
+
     def add_cosserat(self, a: int, b: int, stiffness, rest_length: float, rest_curvature=(0, 0, 0)) -> None:
         r = _lib.CosseratRecord()
         r.particle_indices[0], r.particle_indices[1] = a, b
@@ -209,6 +224,12 @@ class SimulationContext:
         n = lib.avbd_num_energies(self.handle, _lib.ONE_SIDED_FIXED_JOINT)
         recs = (_lib.JointRecord * n)()
         self._chk(lib.avbd_download_energies(self.handle, _lib.ONE_SIDED_FIXED_JOINT, C.cast(recs, C.c_void_p), n))
+        return recs
+
+    def fixed_joint_records(self):
+        n = lib.avbd_num_energies(self.handle, _lib.FIXED_JOINT)
+        recs = (_lib.FixedJointRecord * n)()
+        self._chk(lib.avbd_download_energies(self.handle, _lib.FIXED_JOINT, C.cast(recs, C.c_void_p), n))
         return recs
 
     def set_particle_state(self, first: int, x=None, x_prev=None, v=None, v_prev=None) -> None:
@@ -625,12 +646,18 @@ class Simulation:
 
     def _add_rod(self, r: Rod) -> int:
         first = self.ctx.add_rod(r)
-        if self.scene.contact_margin is not None:
+        if self.scene.contact_margin is not None and r.collide:
             self.ctx.add_collision_segments(np.arange(first, first + r.elements + 1), 0.5 * r.diameter)
         if r.base_joint:
             # hold the first node at its initial pose
             self.ctx.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
-        if r.tip_sphere is not None:
+        if r.tip_sphere is not None and r.tip_joint:
+            rad, m, w0 = r.tip_sphere
+            centre, density, offs, rots = tip_body(r)
+            body = self._add_sphere(Sphere(radius=rad, density=density, pos=tuple(centre), rot=r.rot, w0=w0,
+                                           ground=r.ground))
+            self.ctx.add_fixed_joint(first + r.elements, body, offs, rots)
+        elif r.tip_sphere is not None:
             rad, m, w0 = r.tip_sphere
             tip = first + r.elements
             self.ctx.add_mass(tip, m, np.full(3, 0.4 * m * rad * rad))

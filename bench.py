@@ -55,6 +55,10 @@ def make_scene(name: str):
         return scenes.cube_batch(n_scenes=512)
     if name == "rod_grid":
         return scenes.rod_grid()
+    if name == "rod_grid_jointed":   # tip spheres as bodies held by two-sided fixed joints
+        return scenes.rod_grid(tip_joint=True)
+    if name == "rod_grid_contacts":  # ... and sphere-sphere contacts through the LBVH
+        return scenes.rod_grid(tip_joint=True, contact_margin=1e-3)
     if name in ("ball_on_block", "rod_on_block", "hail_on_block"):
         return getattr(scenes, name)()
     raise SystemExit(f"unknown workload {name}")
@@ -473,6 +477,17 @@ def main():
     }
     if calib:
         out["copy_calibration"] = calib
+    if sc.rods or sc.spheres:
+        tips = [r for r in sc.rods if r.tip_sphere is not None]
+        out["config"].update({
+            "workload": args.workload,
+            "rods": len(sc.rods), "rod_nodes": sum(r.elements + 1 for r in sc.rods),
+            "rigid_spheres": len(sc.spheres) + sum(1 for r in tips if r.tip_joint),
+            "tip_spheres": ("bodies held by two-sided fixed joints" if tips and all(r.tip_joint for r in tips)
+                            else "welded into the tip nodes" if tips else "none"),
+            "broad_phase": (f"LBVH over the spheres' collision primitives every step (margin {sc.contact_margin:g})"
+                            if dynamic else "none"),
+        })
     print(json.dumps(out), flush=True)
 
 

@@ -55,7 +55,8 @@ typedef enum {
     AVBD_RIGID_BODY_GROUND_COLLISION = 3,
     AVBD_TRIANGLE_RIGID_COLLISION = 4,
     AVBD_TRIANGLE_ROD_COLLISION = 5,
-    AVBD_ONE_SIDED_FIXED_JOINT = 6
+    AVBD_ONE_SIDED_FIXED_JOINT = 6,
+    AVBD_FIXED_JOINT = 7 /* new: two-sided fixed joint between two oriented particles */
 } avbd_energy_type;
 
 /* ---- context creation flags ---- */
@@ -173,6 +174,25 @@ typedef struct {
     double C_prev[6];
 } avbd_joint_record;
 
+/* Two-sided fixed joint (new; the reference's joint holds one particle to a
+ * fixed world pose, energy/joint/OneSidedFixedJointEnergySolver.hpp:16-34,
+ * so a rigid body attached to a rod tip has to be welded into the node).
+ * Joint frame i = (x_i + R_i position_offset[i], q_i rotation_offset[i]) on
+ * oriented particle particle_index[i];
+ * C = [frame_0 position - frame_1 position ; Minus_S3(frame_0 rotation,
+ * frame_1 rotation)] under the one-sided joint's vector AL (k_start 1e4,
+ * unbounded multipliers).  With particle 1 fixed it is the one-sided joint
+ * with ref = frame 1.  The two particles are adjacent in the colouring.
+ * Quaternions are (w, x, y, z). */
+typedef struct {
+    uint32_t particle_index[2];
+    double position_offset[2][3];
+    double rotation_offset[2][4];
+    double k[6];
+    double lambda[6];
+    double C_prev[6];
+} avbd_fixed_joint_record;
+
 /* Per-step statistics (new: the reference has no residual; see the
  * definitions in the oracle and the solver sources). */
 typedef struct {
@@ -235,7 +255,7 @@ int avbd_add_sphere(avbd_ctx* ctx, double radius, double density, const double* 
 
 /* EnergyRegistry pools: append count records of the given type
  * (avbd_neo_hookean_record / avbd_ground_record / avbd_cosserat_record /
- * avbd_rigid_ground_record / avbd_joint_record). */
+ * avbd_rigid_ground_record / avbd_joint_record / avbd_fixed_joint_record). */
 int avbd_add_energies(avbd_ctx* ctx, int type, const void* records, uint32_t count);
 
 /* TetMeshObject::setup formulas (simobject/TetMeshObject.cpp:46-96) for a mesh

@@ -390,6 +390,13 @@ int64_t oracle_add_joint(void* h, uint32_t p, const double* pos_offset3, const d
                                                    q_in(ref_rot4));
 }
 
+// Two-sided fixed joint (new): offsets3 = body 0's then body 1's position
+// offset, rot_offsets4 = body 0's then body 1's rotation offset.
+int64_t oracle_add_joint2(void* h, uint32_t p0, uint32_t p1, const double* offsets6, const double* rot_offsets8) {
+    return static_cast<Simulation*>(h)->add_joint2(p0, p1, v_in(offsets6), q_in(rot_offsets8), v_in(offsets6 + 3),
+                                                    q_in(rot_offsets8 + 4));
+}
+
 // Add mass (and, for an oriented particle, principal inertia) to particle p:
 // a rigid body welded to it (a sphere at a rod tip).
 void oracle_add_mass(void* h, uint32_t p, double dm, const double* dI3) {
@@ -450,6 +457,7 @@ void oracle_add_rground_point(void* h, uint32_t p, const double* cp_local3, doub
 uint32_t oracle_num_cosserat(void* h) { return (uint32_t)static_cast<Simulation*>(h)->cos.size(); }
 uint32_t oracle_num_rground(void* h) { return (uint32_t)static_cast<Simulation*>(h)->rground.size(); }
 uint32_t oracle_num_joints(void* h) { return (uint32_t)static_cast<Simulation*>(h)->joints.size(); }
+uint32_t oracle_num_joints2(void* h) { return (uint32_t)static_cast<Simulation*>(h)->joints2.size(); }
 
 // Per particle: orientation (4, identity for positional particles), angular
 // velocity (3), inertia (3), oriented flag.
@@ -517,6 +525,19 @@ void oracle_get_joints(void* h, double* rec18) {
     }
 }
 
+// Two-sided joint records: k(6), lambda(6), C_prev(6) (18 per record)
+void oracle_get_joints2(void* h, double* rec18) {
+    auto* s = static_cast<Simulation*>(h);
+    for (size_t e = 0; e < s->joints2.size(); e++) {
+        const auto& j = s->joints2[e];
+        for (int k = 0; k < 6; k++) {
+            rec18[18 * e + k] = j.k[k];
+            rec18[18 * e + 6 + k] = j.lambda[k];
+            rec18[18 * e + 12 + k] = j.C_prev[k];
+        }
+    }
+}
+
 // Single-energy evaluations for unit tests (G: 6, H: 36 row-major)
 static void out6(const V6& G, const M6& H, double* G6, double* H36) {
     for (int i = 0; i < 6; i++) {
@@ -556,6 +577,31 @@ void oracle_joint_single(const double* x3, const double* q4, const double* pos_o
     joint_eval(j, x, rs, C6);
     V6 G; M6 H;
     joint_accumulate(j, x, rs, G, H);
+    out6(G, H, G6, H36);
+}
+
+// Two-sided joint between particles 0 and 1 (x6: both positions, q8: both
+// orientations), accumulated for body `side`; writes C (6) and the gradient
+// rows dC/d(x, theta) of that side (36, row-major) as well.
+void oracle_joint2_single(const double* x6, const double* q8, const double* offsets6, const double* rot_offsets8,
+                          const double* k6, const double* lambda6, const double* C_prev6, uint32_t side, double* C6,
+                          double* grad36, double* G6, double* H36) {
+    std::vector<V3> x = {v_in(x6), v_in(x6 + 3)};
+    RotState rs;
+    rs.rot_of = {0, 1};
+    rs.q = {q_in(q8), q_in(q8 + 4)};
+    Joint2 j{};
+    j.p[0] = 0; j.p[1] = 1;
+    j.pos_offset[0] = v_in(offsets6); j.pos_offset[1] = v_in(offsets6 + 3);
+    j.rot_offset[0] = q_in(rot_offsets8); j.rot_offset[1] = q_in(rot_offsets8 + 4);
+    for (int k = 0; k < 6; k++) { j.k[k] = k6[k]; j.lambda[k] = lambda6[k]; j.C_prev[k] = C_prev6[k]; }
+    joint2_eval(j, x, rs, C6);
+    Real grad[6][6];
+    joint2_grad(j, side, x, rs, grad);
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) grad36[6 * a + b] = grad[a][b];
+    V6 G; M6 H;
+    joint2_accumulate(j, side, x, rs, G, H);
     out6(G, H, G6, H36);
 }
 

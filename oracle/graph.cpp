@@ -15,7 +15,8 @@ namespace oracle {
 // depends on (DSATUR only uses it as a set), so it is kept ascending here.
 void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh, const std::vector<CosseratRod>& cos,
                             const std::vector<GroundContact>& ground, const std::vector<RigidGround>& rground,
-                            const std::vector<Joint>& joints, StaticAdjacency& adj) {
+                            const std::vector<Joint>& joints, const std::vector<Joint2>& joints2,
+                            StaticAdjacency& adj) {
     std::vector<std::array<uint32_t, ST_COUNT>> val(n);
     for (auto& a : val) a.fill(0);
     std::vector<std::vector<uint32_t>> nb(n);
@@ -33,6 +34,12 @@ void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh, const std:
     for (const auto& g : ground) val[g.p][ST_GROUND]++;
     for (const auto& g : rground) val[g.p][ST_RIGID_GROUND]++;
     for (const auto& j : joints) val[j.p][ST_JOINT]++;
+    // a two-sided joint (new) couples its two bodies like a rod element
+    for (const auto& j : joints2)
+        for (int k = 0; k < 2; k++) {
+            val[j.p[k]][ST_JOINT]++;
+            nb[j.p[k]].push_back(j.p[1 - k]);
+        }
 
     adj.e_offsets.assign(n + 1, {});
     adj.e_offsets[0][0] = 0;
@@ -75,6 +82,11 @@ void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh, const std:
         uint32_t p = joints[e].p;
         adj.e_entries[cursor[p][ST_JOINT]++] = AdjEntry{ST_JOINT, e, 0};
     }
+    for (uint32_t e = 0; e < joints2.size(); e++)
+        for (uint16_t k = 0; k < 2; k++) {
+            uint32_t p = joints2[e].p[k];
+            adj.e_entries[cursor[p][ST_JOINT]++] = AdjEntry{ST_JOINT, (uint32_t)joints.size() + e, k};
+        }
     adj.p_neighbors.resize(adj.p_offsets[n]);
     for (uint32_t i = 0; i < n; i++)
         std::copy(nb[i].begin(), nb[i].end(), adj.p_neighbors.begin() + adj.p_offsets[i]);

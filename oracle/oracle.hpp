@@ -178,9 +178,13 @@ void ground_update_after_iteration(GroundContact& c, const std::vector<V3>& x);
 struct CosseratRod;
 struct RigidGround;
 struct Joint;
+struct Joint2;
+// joint entries: the one-sided joints (e_idx = joint index, local 0), then
+// both sides of every two-sided joint (e_idx = |joints| + index, local = side)
 void build_static_adjacency(uint32_t n, const std::vector<NHTet>& nh, const std::vector<CosseratRod>& cos,
                             const std::vector<GroundContact>& ground, const std::vector<RigidGround>& rground,
-                            const std::vector<Joint>& joints, StaticAdjacency& adj);
+                            const std::vector<Joint>& joints, const std::vector<Joint2>& joints2,
+                            StaticAdjacency& adj);
 void dsatur(const StaticAdjacency& adj, uint32_t n, Coloring& col);
 void rebuild_work_list(uint32_t n, Coloring& col);
 

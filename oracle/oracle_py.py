@@ -438,11 +438,27 @@ def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
             for r in getattr(scene, "rods", []):
                 first = o.add_rod(r.elements, r.length, r.curvature, r.E, r.nu, r.density, r.diameter, r.pos, r.rot,
                                   r.vel, r.base_fixed, r.tip_fixed, r.ground)
-                if dynamic:
+                if dynamic and getattr(r, "collide", True):
                     o.add_collision_segments(np.arange(first, first + r.elements + 1), 0.5 * r.diameter)
                 if r.base_joint:
                     o.add_joint(first, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), r.pos, r.rot)
-                if getattr(r, "tip_sphere", None) is not None:
+                if getattr(r, "tip_sphere", None) is not None and getattr(r, "tip_joint", False):
+                    # a sphere body of its own below the tip, held by a two-sided joint
+                    rad, m, w0 = r.tip_sphere
+                    if any(c != 0.0 for c in r.curvature):
+                        raise ValueError("a jointed tip sphere needs a straight rod")
+                    qr = np.asarray(r.rot, dtype=np.float64)
+                    u = qr[1:]
+                    t = 2.0 * np.cross(u, (0.0, 0.0, r.length + rad))
+                    centre = np.asarray(r.pos, dtype=np.float64) + (0.0, 0.0, r.length + rad) + qr[0] * t + np.cross(u, t)
+                    density = m / (4.0 / 3.0 * np.pi * rad ** 3)
+                    body = o.add_sphere(rad, density, centre, r.rot, False, r.ground)
+                    if dynamic:
+                        o.add_collision_sphere(body, rad)
+                    spins.append((body, r.rot, w0))
+                    o.add_joint2(first + r.elements, body, [(0.0, 0.0, 0.0), (0.0, 0.0, -rad)],
+                                 [(1.0, 0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0)])
+                elif getattr(r, "tip_sphere", None) is not None:
                     rad, m, w0 = r.tip_sphere
                     tip = first + r.elements
                     dI = np.full(3, 0.4 * m * rad * rad)
@@ -559,6 +575,10 @@ for _k, (_r, _a) in {
     "oracle_num_cosserat": (C.c_uint32, [C.c_void_p]),
     "oracle_num_rground": (C.c_uint32, [C.c_void_p]),
     "oracle_num_joints": (C.c_uint32, [C.c_void_p]),
+    "oracle_add_joint2": (C.c_int64, [C.c_void_p, C.c_uint32, C.c_uint32, _D, _D]),
+    "oracle_num_joints2": (C.c_uint32, [C.c_void_p]),
+    "oracle_get_joints2": (None, [C.c_void_p, _D]),
+    "oracle_joint2_single": (None, [_D, _D, _D, _D, _D, _D, _D, C.c_uint32, _D, _D, _D, _D]),
     "oracle_get_rotations": (None, [C.c_void_p, _D, _D, _D, _U8]),
     "oracle_get_mass": (None, [C.c_void_p, _D, _U8]),
     "oracle_set_rotation_state": (None, [C.c_void_p, _D, _D]),
@@ -632,6 +652,12 @@ def _add_joint(self, p, pos_offset, rot_offset, ref_pos, ref_rot):
                                     dp(_a(ref_rot, 4))))
 
 
+def _add_joint2(self, p0, p1, pos_offsets, rot_offsets):
+    """Two-sided fixed joint (new): pos_offsets (2, 3) and rot_offsets (2, 4)
+    of body p0 and body p1."""
+    return int(lib.oracle_add_joint2(self.h, p0, p1, dp(_a(pos_offsets, 6)), dp(_a(rot_offsets, 8))))
+
+
 def _rotations(self):
     n = self.n
     q = np.empty((n, 4))
@@ -663,6 +689,13 @@ def _joints(self):
     return dict(k=rec[:, :6], lam=rec[:, 6:12], C_prev=rec[:, 12:])
 
 
+def _joints2(self):
+    n = lib.oracle_num_joints2(self.h)
+    rec = np.empty((n, 18))
+    lib.oracle_get_joints2(self.h, dp(rec))
+    return dict(k=rec[:, :6], lam=rec[:, 6:12], C_prev=rec[:, 12:])
+
+
 OracleSim.add_rod = _add_rod
 OracleSim.add_sphere = _add_sphere
 OracleSim.add_oriented = _add_oriented
@@ -670,6 +703,7 @@ OracleSim.add_rground_point = _add_rground_point
 OracleSim.set_external_accel = _set_external_accel
 OracleSim.add_cosserat = _add_cosserat
 OracleSim.add_joint = _add_joint
+OracleSim.add_joint2 = _add_joint2
 def _set_rotation_state(self, q, w):
     lib.oracle_set_rotation_state(self.h, dp(_a(q, 4 * self.n)), dp(_a(w, 3 * self.n)))
 
@@ -679,6 +713,7 @@ OracleSim.set_rotation_state = _set_rotation_state
 OracleSim.mass_fixed = _mass_fixed
 OracleSim.rground = _rground
 OracleSim.joints = _joints
+OracleSim.joints2 = _joints2
 
 
 def cosserat_single(x2, q2, stiffness, L, kappa, local):
@@ -695,6 +730,15 @@ def joint_single(x, q, pos_offset, rot_offset, ref_pos, ref_rot, k, lam, C_prev=
                             dp(_a(ref_pos, 3)), dp(_a(ref_rot, 4)), dp(_a(k, 6)), dp(_a(lam, 6)), dp(_a(C_prev, 6)),
                             dp(Cv), dp(G), dp(H))
     return Cv, G, H.reshape(6, 6)
+
+
+def joint2_single(x2, q2, pos_offsets, rot_offsets, k, lam, side, C_prev=(0,) * 6):
+    """Two-sided joint between bodies 0 and 1 accumulated for `side`:
+    C (6), the gradient rows dC/d(x, theta) of that side (6, 6), G, H."""
+    Cv, grad, G, H = np.empty(6), np.empty(36), np.empty(6), np.empty(36)
+    lib.oracle_joint2_single(dp(_a(x2, 6)), dp(_a(q2, 8)), dp(_a(pos_offsets, 6)), dp(_a(rot_offsets, 8)),
+                             dp(_a(k, 6)), dp(_a(lam, 6)), dp(_a(C_prev, 6)), side, dp(Cv), dp(grad), dp(G), dp(H))
+    return Cv, grad.reshape(6, 6), G, H.reshape(6, 6)
 
 
 def rground_single(x, q, rec12, cp_local, cp_ground):

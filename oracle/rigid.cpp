@@ -375,4 +375,75 @@ void joint_update_after_iteration(Joint& j, const std::vector<V3>& x, const RotS
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two-sided fixed joint (new; same vector AL as the one-sided joint).
+// Perturbing body 0 by q0 [+] d: frame rotation f0 exp(R(r0)^T d), so
+// dC_rot/dtheta_0 = J_r^-1(theta) R(r0)^T, the one-sided joint's form.
+// Perturbing body 1: f1 exp(R(r1)^T d), and
+// (f1 exp(u))^-1 f0 = exp(-u) f1^-1 f0, whose Log is theta - J_l^-1(theta) u
+// to first order with J_l^-1(theta) = J_r^-1(-theta); so
+// dC_rot/dtheta_1 = -J_r^-1(-theta) R(r1)^T.  Positions: dC/dx_0 = I,
+// dC/dx_1 = -I, dC/dtheta_0 = -R0 [p0]x, dC/dtheta_1 = +R1 [p1]x.
+// ---------------------------------------------------------------------------
+void joint2_eval(const Joint2& j, const std::vector<V3>& x, const RotState& r, Real C[6]) {
+    const Quat& q0 = r.rot(j.p[0]);
+    const Quat& q1 = r.rot(j.p[1]);
+    const V3 d = (x[j.p[0]] + qrot(q0, j.pos_offset[0])) - (x[j.p[1]] + qrot(q1, j.pos_offset[1]));
+    const V3 th = minus_s3(qmul(q0, j.rot_offset[0]), qmul(q1, j.rot_offset[1]));
+    for (int i = 0; i < 3; i++) {
+        C[i] = d[i];
+        C[3 + i] = th[i];
+    }
+}
+
+void joint2_grad(const Joint2& j, uint32_t side, const std::vector<V3>& x, const RotState& r, Real grad[6][6]) {
+    Real C[6];
+    joint2_eval(j, x, r, C);
+    const Real sgn = side == 0 ? 1.0 : -1.0;
+    const M3 dCp = matmul(qmat(r.rot(j.p[side])), skew3(j.pos_offset[side]));   // R [p]x
+    const V3 th = side == 0 ? V3(C[3], C[4], C[5]) : V3(-C[3], -C[4], -C[5]);
+    const M3 dCR = matmul(jr_inv(th), transpose(qmat(j.rot_offset[side])));
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) grad[a][b] = 0;
+    for (int i = 0; i < 3; i++) {
+        grad[i][i] = sgn;
+        for (int k = 0; k < 3; k++) {
+            grad[i][3 + k] = -sgn * dCp.m[i][k];
+            grad[3 + i][3 + k] = sgn * dCR.m[i][k];
+        }
+    }
+}
+
+void joint2_accumulate(const Joint2& j, uint32_t side, const std::vector<V3>& x, const RotState& r, V6& G, M6& H) {
+    Real Craw[6], grad[6][6];
+    joint2_eval(j, x, r, Craw);
+    joint2_grad(j, side, x, r, grad);
+    for (int ci = 0; ci < 6; ci++) {
+        const Real C = Craw[ci] - CONSTRAINT_ALPHA * j.C_prev[ci];
+        const Real lam = j.k[ci] * C + j.lambda[ci];
+        for (int a = 0; a < 6; a++) G.v[a] += grad[ci][a] * lam;
+        outer_add(H, grad[ci], j.k[ci]);
+    }
+}
+
+void joint2_update_after_timestep(Joint2& j, const std::vector<V3>& x, const RotState& r) {
+    for (int i = 0; i < 6; i++) {
+        j.lambda[i] = CONSTRAINT_ALPHA * STIFFNESS_GAMMA * j.lambda[i];
+        j.k[i] = std::max(JOINT_K_START, STIFFNESS_GAMMA * j.k[i]);
+    }
+    joint2_eval(j, x, r, j.C_prev);
+}
+
+void joint2_update_after_iteration(Joint2& j, const std::vector<V3>& x, const RotState& r) {
+    Real C[6];
+    joint2_eval(j, x, r, C);
+    for (int i = 0; i < 6; i++) {
+        const Real Cc = C[i] - CONSTRAINT_ALPHA * j.C_prev[i];
+        const Real lp = j.k[i] * Cc + j.lambda[i];
+        if (lp > std::numeric_limits<Real>::lowest() && lp < std::numeric_limits<Real>::max())
+            j.k[i] += STIFFNESS_BETA * Cc;
+        j.lambda[i] = lp;
+    }
+}
+
 }  // namespace oracle

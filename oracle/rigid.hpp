@@ -1,5 +1,5 @@
 // rigid.hpp -- FP64 oracle: quaternions, S3/SO3 maps and the 6-DOF energies
-// (Cosserat rod, rigid-body ground contact, one-sided fixed joint).
+// (Cosserat rod, rigid-body ground contact, one-sided and two-sided fixed joints).
 // Test infrastructure only; see oracle.hpp.
 //
 // Restates:
@@ -83,6 +83,19 @@ struct Joint {
 };
 constexpr Real JOINT_K_START = 1e4;   // OneSidedFixedJointEnergyPool ctor
 
+// Two-sided fixed joint (new: the reference's joint holds one particle to a
+// fixed world pose, energy/joint/OneSidedFixedJointEnergySolver.hpp:16-34).
+// Joint frame i = (x_i + R_i pos_offset[i], q_i rot_offset[i]) on body
+// p[i]; C = [frame_0 position - frame_1 position ; Minus_S3(frame_0
+// rotation, frame_1 rotation)] under the same vector AL, so that a fixed body
+// 1 reproduces the one-sided joint with ref = frame 1.
+struct Joint2 {
+    uint32_t p[2];
+    V3 pos_offset[2];
+    Quat rot_offset[2];
+    Real k[6], lambda[6], C_prev[6];
+};
+
 // Kernels (positions x, rotations by particle through rot_of)
 struct RotState;
 void cosserat_accumulate(const CosseratRod& e, const std::vector<V3>& x, const RotState& r, uint32_t local, V6& G,
@@ -99,6 +112,14 @@ void joint_eval(const Joint& j, const std::vector<V3>& x, const RotState& r, Rea
 void joint_accumulate(const Joint& j, const std::vector<V3>& x, const RotState& r, V6& G, M6& H);
 void joint_update_after_timestep(Joint& j, const std::vector<V3>& x, const RotState& r);
 void joint_update_after_iteration(Joint& j, const std::vector<V3>& x, const RotState& r);
+
+void joint2_eval(const Joint2& j, const std::vector<V3>& x, const RotState& r, Real C[6]);
+// gradient rows of C with respect to body `side`'s (x, theta), theta the
+// body-frame increment of q [+] theta
+void joint2_grad(const Joint2& j, uint32_t side, const std::vector<V3>& x, const RotState& r, Real grad[6][6]);
+void joint2_accumulate(const Joint2& j, uint32_t side, const std::vector<V3>& x, const RotState& r, V6& G, M6& H);
+void joint2_update_after_timestep(Joint2& j, const std::vector<V3>& x, const RotState& r);
+void joint2_update_after_iteration(Joint2& j, const std::vector<V3>& x, const RotState& r);
 
 // RotationPool (common/ParticlePool.hpp:7-69), indexed by particle via rot_of
 struct RotState {

@@ -27,6 +27,7 @@ public:
     std::vector<GroundContact> ground;
     std::vector<RigidGround> rground;
     std::vector<Joint> joints;
+    std::vector<Joint2> joints2;   // two-sided fixed joints (new)
 
     std::vector<uint32_t> tris;     // collision triangles (3 particle indices each)
     BroadPhase bp;
@@ -65,6 +66,9 @@ public:
     uint32_t add_sphere(Real radius, Real density, V3 pos, Quat rot, bool fixed_, bool with_ground);
     // OneSidedFixedJointEnergyPool::addEnergy (energy/joint/OneSidedFixedJointEnergyPool.hpp:33-50).
     uint32_t add_joint(uint32_t p, V3 pos_offset, Quat rot_offset, V3 ref_position, Quat ref_rotation);
+    // Two-sided fixed joint between oriented particles p0 and p1 (new), k
+    // starting at the one-sided joint's k_start.
+    uint32_t add_joint2(uint32_t p0, uint32_t p1, V3 pos_offset0, Quat rot_offset0, V3 pos_offset1, Quat rot_offset1);
 
     void build_graph();     // adjacency + DSATUR + work list (simulation/Simulation.cpp:87-89)
 

@@ -434,9 +434,23 @@ uint32_t Simulation::add_joint(uint32_t p, V3 pos_offset, Quat rot_offset, V3 re
     return (uint32_t)joints.size() - 1;
 }
 
+uint32_t Simulation::add_joint2(uint32_t p0, uint32_t p1, V3 pos_offset0, Quat rot_offset0, V3 pos_offset1,
+                                Quat rot_offset1) {
+    if (p0 == p1) throw std::runtime_error("two-sided joint with one particle twice");
+    if (p0 >= x.size() || p1 >= x.size() || rs.rot_of[p0] < 0 || rs.rot_of[p1] < 0)
+        throw std::runtime_error("joint needs oriented particles");
+    Joint2 j{};
+    j.p[0] = p0; j.p[1] = p1;
+    j.pos_offset[0] = pos_offset0; j.pos_offset[1] = pos_offset1;
+    j.rot_offset[0] = rot_offset0; j.rot_offset[1] = rot_offset1;
+    for (int i = 0; i < 6; i++) { j.k[i] = JOINT_K_START; j.lambda[i] = 0; j.C_prev[i] = 0; }
+    joints2.push_back(j);
+    return (uint32_t)joints2.size() - 1;
+}
+
 void Simulation::build_graph() {
     uint32_t n = (uint32_t)x.size();
-    build_static_adjacency(n, nh, cos, ground, rground, joints, adj);
+    build_static_adjacency(n, nh, cos, ground, rground, joints, joints2, adj);
     dsatur(adj, n, coloring);
     rebuild_work_list(n, coloring);
     is_conflicted.assign(n, 0);
@@ -455,13 +469,16 @@ void Simulation::phase_after_timestep(uint32_t lo_nh, uint32_t hi_nh, uint32_t l
 }
 
 void Simulation::phase_after_timestep_rigid(uint32_t lo, uint32_t hi) {
-    // records [0, |rground|) then [|rground|, |rground| + |joints|)
+    // records [0, |rground|), then the joints, then the two-sided joints
+    const size_t nj = rground.size() + joints.size();
     for (uint32_t e = lo; e < hi; e++) {
         if (e < rground.size()) {
             rground_update_contact_points(rground[e], x, rs);
             rground_update_after_timestep(rground[e], x, rs);
-        } else {
+        } else if (e < nj) {
             joint_update_after_timestep(joints[e - rground.size()], x, rs);
+        } else {
+            joint2_update_after_timestep(joints2[e - nj], x, rs);
         }
     }
 }
@@ -564,7 +581,8 @@ void Simulation::accumulate_vertex6(uint32_t p, V6& G, M6& H) const {
             } else if (ty == ST_RIGID_GROUND) {
                 rground_accumulate(rground[en.e_idx], x, rs, G, H);
             } else if (ty == ST_JOINT) {
-                joint_accumulate(joints[en.e_idx], x, rs, G, H);
+                if (en.e_idx < joints.size()) joint_accumulate(joints[en.e_idx], x, rs, G, H);
+                else joint2_accumulate(joints2[en.e_idx - joints.size()], en.local, x, rs, G, H);
             }
         }
     }
@@ -683,7 +701,8 @@ void Simulation::phase_dual(uint32_t lo, uint32_t hi) {
 void Simulation::phase_dual_rigid(uint32_t lo, uint32_t hi) {
     for (uint32_t e = lo; e < hi; e++) {
         if (e < rground.size()) rground_update_after_iteration(rground[e], x, rs);
-        else joint_update_after_iteration(joints[e - rground.size()], x, rs);
+        else if (e < rground.size() + joints.size()) joint_update_after_iteration(joints[e - rground.size()], x, rs);
+        else joint2_update_after_iteration(joints2[e - rground.size() - joints.size()], x, rs);
     }
 }
 
@@ -722,7 +741,7 @@ void Simulation::step() {
         bp.build(tris, x, v, params.dt);
         last_candidates = bp.traverse_leaves(0, bp.n);
     }
-    const uint32_t n_rigid = (uint32_t)(rground.size() + joints.size());
+    const uint32_t n_rigid = (uint32_t)(rground.size() + joints.size() + joints2.size());
     phase_after_timestep(0, (uint32_t)nh.size(), 0, (uint32_t)ground.size());
     phase_after_timestep_rigid(0, n_rigid);
     if (dynamic) phase_after_timestep_dyn();
@@ -817,7 +836,7 @@ void Simulation::step_mt(uint32_t nthreads, uint32_t n_steps) {
     const bool target = params.tol > 0;
     const uint32_t iters = target ? params.max_iters : params.iters;
     const Real tol2 = target ? params.tol * params.tol * mg2() : 0;
-    const uint32_t n_rigid = (uint32_t)(rground.size() + joints.size());
+    const uint32_t n_rigid = (uint32_t)(rground.size() + joints.size() + joints2.size());
     auto worker = [&](uint32_t w) {
         uint32_t n = (uint32_t)x.size();
         uint32_t lo, hi, lo2, hi2;

@@ -159,14 +159,14 @@ def test_record_layouts_match_c_compiler(tmp_path):
     from avbd_amd import _lib
     src = tmp_path / "sizes.c"
     recs = ["avbd_neo_hookean_record", "avbd_ground_record", "avbd_cosserat_record", "avbd_rigid_ground_record",
-            "avbd_joint_record", "avbd_stats", "avbd_scene_desc", "avbd_contact"]
+            "avbd_joint_record", "avbd_fixed_joint_record", "avbd_stats", "avbd_scene_desc", "avbd_contact"]
     body = "\n".join(f'printf("%zu\\n", sizeof({r}));' for r in recs)
     src.write_text(f'#include <stdio.h>\n#include "avbd.h"\nint main(void) {{ {body} return 0; }}\n')
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     mirror = [_lib.NeoHookeanRecord, _lib.GroundRecord, _lib.CosseratRecord, _lib.RigidGroundRecord,
-              _lib.JointRecord, _lib.Stats, _lib.SceneDesc, _lib.Contact]
+              _lib.JointRecord, _lib.FixedJointRecord, _lib.Stats, _lib.SceneDesc, _lib.Contact]
     assert sizes == [C.sizeof(m) for m in mirror]
 
 
@@ -192,6 +192,42 @@ def test_rod_and_sphere_coloring_bit_exact_vs_oracle(oracle):
     assert _lib.lib.avbd_num_energies(sim.ctx.handle, _lib.COSSERAT_ROD) == 27
     assert _lib.lib.avbd_num_energies(sim.ctx.handle, _lib.RIGID_BODY_GROUND_COLLISION) == 30 + 3
     assert _lib.lib.avbd_num_energies(sim.ctx.handle, _lib.ONE_SIDED_FIXED_JOINT) == 3
+
+
+def test_jointed_tip_spheres_coloring_bit_exact_vs_oracle(oracle):
+    """Tip spheres as bodies of their own, held by two-sided fixed joints: the
+    joint couples the tip node and its sphere in the colouring (different
+    colours), and the colouring is the oracle's."""
+    from avbd_amd import _lib
+    sc = scenes.rod_grid(nx=2, nz=3, segments=8, tip_joint=True)
+    sim = _rod_scene_ctx(sc)
+    col, k = _planned_coloring(sim.ctx)
+    o = oracle.from_scene(sc)
+    ocol, ok = o.colors()
+    assert (k, col.tolist()) == (ok, ocol.tolist())
+    assert _lib.lib.avbd_num_energies(sim.ctx.handle, _lib.FIXED_JOINT) == 6
+    assert _lib.lib.avbd_num_energies(sim.ctx.handle, _lib.ONE_SIDED_FIXED_JOINT) == 6
+    for r in range(6):
+        tip, body = r * 10 + 8, r * 10 + 9   # 9 rod nodes, then the sphere
+        assert col[tip] != col[body]
+
+
+def test_fixed_joint_validated():
+    """A two-sided joint needs two distinct oriented particles."""
+    from avbd_amd import _lib
+    ctx = make_ctx()
+    ctx.add_oriented_particles(np.zeros((1, 3)), inertia=np.ones((1, 3)), mass=np.ones(1))
+    ctx.add_particles(np.ones((1, 3)), mass=np.ones(1))
+    ctx.add_fixed_joint(0, 1, [(0, 0, 0)] * 2, [(1, 0, 0, 0)] * 2)
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.build_graph()
+    assert "oriented particle" in str(e.value)
+    ctx = make_ctx()
+    ctx.add_oriented_particles(np.zeros((2, 3)), inertia=np.ones((2, 3)), mass=np.ones(2))
+    ctx.add_fixed_joint(1, 1, [(0, 0, 0)] * 2, [(1, 0, 0, 0)] * 2)
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.build_graph()
+    assert "one particle twice" in str(e.value)
 
 
 def test_energy_particle_kind_validated():

@@ -197,6 +197,43 @@ def test_rod_grid_with_tip_spheres(oracle):
     assert err <= 2e-3
 
 
+def test_rod_grid_with_jointed_tip_spheres(oracle):
+    """The tip spheres as 6-DOF bodies of their own, held to the tip nodes by
+    two-sided fixed joints (new), spinning: colouring bit-exact, positions and
+    orientations after one step within 1e-5 x bbox, after 30 steps within
+    2e-3 x bbox, and the joints' multipliers follow the oracle's (both
+    bodies' terms in the solves, the per-joint start-of-step and dual
+    updates)."""
+    banner("rod grid with jointed tip spheres vs oracle: 1 and 30 steps")
+    sc = scenes.rod_grid(nx=3, nz=3, segments=16, tip_joint=True)
+    g = gpu_sim(sc)
+    o = oracle.from_scene(sc)
+    col, k = g.ctx.coloring()
+    ocol, ok = o.colors()
+    assert k == ok and np.array_equal(col, ocol)
+    _, wg0 = g.ctx.rotations()
+    assert np.allclose(wg0, o.rotations()["w"], atol=1e-6)
+    g.update(1)
+    o.step(1)
+    xo = o.positions()
+    err = np.abs(g.positions() - xo).max() / bbox(xo)
+    qg, _ = g.ctx.rotations()
+    banner(f"  1-step pos rel err {err:.3e}, q err {qerr(qg, o.rotations()['q']):.3e}")
+    assert err <= 1e-5 and qerr(qg, o.rotations()["q"]) <= 1e-5
+    g.update(29)
+    o.step(29)
+    xo = o.positions()
+    err = np.abs(g.positions() - xo).max() / bbox(xo)
+    jg = np.array([list(r.lambda_) for r in g.ctx.fixed_joint_records()])
+    jo = o.joints2()["lam"]
+    lam_err = np.abs(jg - jo).max() / np.abs(jo).max()
+    banner(f"  30-step pos rel err {err:.3e}, joint lambda rel err {lam_err:.3e}")
+    assert err <= 2e-3
+    assert jg.shape == jo.shape == (9, 6) and lam_err <= 2e-2
+    # the joint carries the sphere: vertical multiplier ~ its weight
+    assert np.all(np.abs(np.abs(jo[:, 1]) - 0.05 * 9.81) < 0.5 * 0.05 * 9.81)
+
+
 def test_box_corner_indefinite_block_matches_oracle(oracle):
     """A rigid box balanced on one corner: the 6x6 block of its vertex solve
     (inertia plus the corner's AL contact) becomes indefinite -- the

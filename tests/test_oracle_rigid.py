@@ -214,6 +214,150 @@ def test_joint_zero_at_reference():
 
 
 # ---------------------------------------------------------------------------
+# two-sided fixed joint (new: the reference's joint is one-sided)
+# ---------------------------------------------------------------------------
+J2X = np.array([[0.05, -0.1, 0.2], [0.3, 0.45, -0.15]])
+J2Q = np.array([euler_xyz([15, -25, 40]), euler_xyz([-35, 10, 70])])
+J2OFF = (np.array([[0.4, 0.6, 0.2], [-0.2, 0.1, 0.35]]), np.array([euler_xyz([60, -30, 20]), euler_xyz([-60, 50, -20])]))
+
+
+def _perturb_side(side, d):
+    x = J2X.copy()
+    q = J2Q.copy()
+    x[side], q[side] = perturb(x[side], q[side], d)
+    return x, q
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_joint2_constraint_jacobian_fd(side):
+    """Rows of dC/d(x, theta) of either body by central differences under
+    q [+] dtheta, and the accumulated gradient with k = 0, lambda = e_i is that
+    row (a sign, a transposed offset rotation or J_r^-1(theta) in place of
+    J_r^-1(-theta) on body 1 fails this)."""
+    def C(d):
+        x, q = _perturb_side(side, d)
+        return O.joint2_single(x, q, *J2OFF, np.zeros(6), np.zeros(6), side)[0]
+
+    Cv, grad, _, _ = O.joint2_single(J2X, J2Q, *J2OFF, np.zeros(6), np.zeros(6), side)
+    assert np.abs(Cv).max() > 0.1   # far from closed: the rotation Jacobians are not identity
+    for i in range(6):
+        lam = np.zeros(6)
+        lam[i] = 1.0
+        _, _, G, H = O.joint2_single(J2X, J2Q, *J2OFF, np.zeros(6), lam, side)
+        row = fd_grad(lambda d: C(d)[i])
+        assert np.allclose(grad[i], row, atol=1e-8), (side, i)
+        assert np.allclose(G, row, atol=1e-8), (side, i)
+        assert np.abs(H).max() == 0.0
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_joint2_energy_gradient_fd(side):
+    """E = sum 1/2 k C^2 + lambda C: G = dE/d(x, theta) of the body, and H is
+    the Gauss-Newton k grad grad^T (symmetric, positive semi-definite)."""
+    k = np.array([1e4, 2e4, 3e4, 5e3, 6e3, 7e3])
+    lam = np.array([1.0, -2.0, 0.5, 0.2, -0.3, 0.1])
+
+    def energy(d):
+        x, q = _perturb_side(side, d)
+        Cv = O.joint2_single(x, q, *J2OFF, k, lam, side)[0]
+        return float(np.sum(0.5 * k * Cv * Cv + lam * Cv))
+
+    _, grad, G, H = O.joint2_single(J2X, J2Q, *J2OFF, k, lam, side)
+    g_fd = fd_grad(energy)
+    assert np.allclose(G, g_fd, rtol=1e-6, atol=1e-6 * np.abs(g_fd).max())
+    assert np.allclose(H, grad.T @ np.diag(k) @ grad, rtol=1e-12, atol=1e-9)
+    assert np.linalg.eigvalsh(H).min() > -1e-6
+
+
+def test_joint2_zero_when_frames_meet_and_antisymmetric():
+    """C = 0 when both joint frames coincide; swapping the bodies negates the
+    position part and the rotation part (Log(b^-1 a) = -Log(a^-1 b))."""
+    q0 = euler_xyz([20, 30, -10])
+    off_p, off_q = J2OFF
+    frame_q = qmul(q0, off_q[0])
+    q1 = qmul(frame_q, np.array([off_q[1][0], -off_q[1][1], -off_q[1][2], -off_q[1][3]]))
+    x0 = np.array([1.0, 2.0, 3.0])
+    frame_p = x0 + O.qmat(q0) @ off_p[0]
+    x1 = frame_p - O.qmat(q1) @ off_p[1]
+    Cv = O.joint2_single([x0, x1], [q0, q1], off_p, off_q, np.zeros(6), np.zeros(6), 0)[0]
+    assert np.abs(Cv).max() < 1e-14
+    Ca = O.joint2_single(J2X, J2Q, *J2OFF, np.zeros(6), np.zeros(6), 0)[0]
+    Cb = O.joint2_single(J2X[::-1], J2Q[::-1], J2OFF[0][::-1], J2OFF[1][::-1], np.zeros(6), np.zeros(6), 0)[0]
+    assert np.allclose(Ca, -Cb, atol=1e-13)
+
+
+def test_joint2_to_a_fixed_body_is_the_one_sided_joint():
+    """A two-sided joint whose second body is fixed is the one-sided joint
+    with the reference pose at that body's joint frame: a hanging, spinning
+    body follows the same trajectory and multipliers over 120 steps."""
+    dt = 1 / 60
+    m, inertia = 3.0, [0.1, 0.2, 0.15]
+    x0, q0 = [0.1, 1.2, -0.05], euler_xyz([10, -20, 30])
+    anchor_x, anchor_q = np.array([0.3, 1.5, 0.2]), euler_xyz([-40, 25, 60])
+    off_b, rot_b = np.array([-0.1, 0.2, 0.05]), euler_xyz([30, 10, -50])
+    off_a, rot_a = np.array([0.05, 0.1, -0.02]), euler_xyz([5, 15, 25])
+    ref_p = anchor_x + O.qmat(anchor_q) @ off_b
+    ref_q = qmul(anchor_q, rot_b)
+
+    one = O.OracleSim(dt, 9.81, 10, 0.95)
+    p = one.add_oriented(x0, q0, m, inertia)
+    one.add_joint(p, off_a, rot_a, ref_p, ref_q)
+    one.build_graph()
+    two = O.OracleSim(dt, 9.81, 10, 0.95)
+    p2 = two.add_oriented(x0, q0, m, inertia)
+    a = two.add_oriented(anchor_x, anchor_q, 1.0, [0.1, 0.1, 0.1])
+    two.set_fixed(np.array([0, 1], dtype=np.uint8))
+    two.add_joint2(p2, a, [off_a, off_b], [rot_a, rot_b])
+    two.build_graph()
+    w0 = np.array([[0.5, -1.0, 2.0]])
+    one.set_rotation_state(np.array([q0]), w0)
+    two.set_rotation_state(np.array([q0, anchor_q]), np.vstack([w0, np.zeros((1, 3))]))
+    for _ in range(12):
+        one.step(10)
+        two.step(10)
+        assert np.allclose(two.positions()[0], one.positions()[0], rtol=0, atol=1e-12)
+        assert np.allclose(two.rotations()["q"][0], one.rotations()["q"][0], rtol=0, atol=1e-12)
+    assert np.allclose(two.positions()[1], anchor_x, rtol=0, atol=0)
+    j1, j2 = one.joints(), two.joints2()
+    assert np.allclose(j2["lam"], j1["lam"], rtol=1e-9, atol=1e-9)
+    assert j1["lam"][0, 1] == pytest.approx(-m * 9.81, rel=5e-2)
+
+
+def test_joint2_welded_pair_falls_as_one_body():
+    """Two free bodies welded by a two-sided joint and set spinning together:
+    the joint stays closed (the second body's gradient enters its own solve
+    with the right sign) and the pair's centre of mass follows the backward
+    Euler free fall (the joint forces are internal)."""
+    dt = 1 / 120
+    s = O.OracleSim(dt, 9.81, 20, 0.0)
+    ma, mb = 2.0, 0.5
+    xa, xb = np.array([0.0, 2.0, 0.0]), np.array([0.0, 1.8, 0.0])
+    qa, qb = euler_xyz([0, 0, 0]), euler_xyz([20, -10, 5])
+    a = s.add_oriented(xa, qa, ma, [0.02, 0.02, 0.02])
+    b = s.add_oriented(xb, qb, mb, [0.005, 0.005, 0.005])
+    # the joint frame at the midpoint, with body b's offsets taken from the start pose
+    mid = 0.5 * (xa + xb)
+    off_a = O.qmat(qa).T @ (mid - xa)
+    off_b = O.qmat(qb).T @ (mid - xb)
+    rot_b = qmul(np.array([qb[0], -qb[1], -qb[2], -qb[3]]), qa)   # q_b rot_b = q_a
+    s.add_joint2(a, b, [off_a, off_b], [[1.0, 0, 0, 0], rot_b])
+    s.build_graph()
+    s.set_rotation_state(np.array([qa, qb]), np.array([[0.0, 0.0, 0.3], [0.0, 0.0, 0.3]]))
+    com0 = (ma * xa + mb * xb) / (ma + mb)
+    steps = 60
+    s.step(steps)
+    x = s.positions()
+    q = s.rotations()["q"]
+    com = (ma * x[0] + mb * x[1]) / (ma + mb)
+    # backward Euler free fall: y_n = y_0 - g dt^2 n (n + 1) / 2
+    fall = com0 + np.array([0.0, -9.81 * dt * dt * steps * (steps + 1) / 2, 0.0])
+    assert np.linalg.norm(com - fall) < 1e-4
+    Cv = O.joint2_single(x, q, [off_a, off_b], [[1.0, 0, 0, 0], rot_b], np.zeros(6), np.zeros(6), 0)[0]
+    assert np.abs(Cv[:3]).max() < 1e-3 and np.abs(Cv[3:]).max() < 1e-2
+    assert np.linalg.norm(x[0] - x[1]) == pytest.approx(0.2, abs=1e-3)
+
+
+# ---------------------------------------------------------------------------
 # rigid-body ground contact (energy/collision/RigidBodyGroundCollisionEnergySolver.hpp)
 # ---------------------------------------------------------------------------
 def rg_rec(kn=0.0, kt=0.0, kb=0.0, ln=0.0, lt=0.0, lb=0.0, cprev=0.0, mu_s=10.0, mu_k=10.0, static=0,
