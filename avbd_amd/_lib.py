@@ -264,6 +264,7 @@ SIGNATURES = {
     "avbd_add_collision_segments": (C.c_int, [_P, C.c_uint32, _U32, C.c_double]),
     "avbd_add_collision_sphere": (C.c_int, [_P, C.c_uint32, C.c_double]),
     "avbd_set_dynamic_contacts": (C.c_int, [_P, C.c_double, C.c_uint32]),
+    "avbd_set_self_collision": (C.c_int, [_P, C.c_int]),
     "avbd_download_contacts": (C.c_int, [_P, C.c_uint32, C.POINTER(Contact), C.c_uint32, _U32]),
     "avbd_download_detected": (C.c_int, [_P, C.POINTER(C.c_uint64), _U32, _U8, C.c_uint32, _U32]),
     "avbd_download_conflicts": (C.c_int, [_P, _U8]),

@@ -121,6 +121,7 @@ struct avbd_ctx {
     std::vector<HostPrim> prims;   // CollisionPrimitivePool, call order
     uint32_t next_object = 0;
     bool dynamic = false;
+    bool self_collision = false;   // vertex-face contacts (avbd_set_self_collision)
     ContactSystem cs;
     DevPrims dp;
     LbvhDev lbd;                   // LBVH over every primitive
@@ -2704,6 +2705,13 @@ int avbd_add_collision_segments(avbd_ctx* c, uint32_t n_nodes, const uint32_t* n
     });
 }
 
+int avbd_set_self_collision(avbd_ctx* c, int enable) {
+    return guarded(c, [&] {
+        if (c->dynamic) throw CodeError(AVBD_E_ARG, "set self collision before avbd_set_dynamic_contacts");
+        c->self_collision = enable != 0;
+    });
+}
+
 int avbd_add_collision_sphere(avbd_ctx* c, uint32_t p, double radius) {
     return guarded(c, [&] {
         require_not_planned(c);
@@ -2750,6 +2758,17 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
             rad[i] = c->prims[i].radius;
         }
         c->dp.n = np;
+        if (c->self_collision) {
+            // owner of every particle: the lowest-numbered triangle holding it
+            std::vector<uint32_t> owner(n, 0xFFFFFFFFu);
+            for (uint32_t i = np; i-- > 0;)
+                if (c->prims[i].type == PRIM_TRIANGLE)
+                    for (int k = 0; k < 3; k++) owner[c->prims[i].idx[k]] = i;
+            c->dp.owner = dev_alloc<uint32_t>(c, n);
+            upload(c, c->dp.owner, owner);
+            c->dp.vertex_face = 1;
+            c->dp.tri_pad = margin;
+        }
         c->dp.type = dev_alloc<uint32_t>(c, np);
         c->dp.idx = dev_alloc<uint32_t>(c, 3 * (size_t)np);
         c->dp.object = dev_alloc<uint32_t>(c, np);
@@ -2821,7 +2840,7 @@ int avbd_download_contacts(avbd_ctx* c, uint32_t kind, avbd_contact* out, uint32
     return guarded(c, [&] {
         require_built(c);
         if (!c->dynamic) throw CodeError(AVBD_E_ARG, "dynamic contacts are off");
-        if (kind >= DYN_KINDS) throw CodeError(AVBD_E_ARG, "kind must be 0, 1 or 2");
+        if (kind >= DYN_KINDS) throw CodeError(AVBD_E_ARG, "kind must be 0, 1, 2 or 3");
         const auto& P = c->cs.pool[kind];
         std::vector<DynRec> recs(c->cs.cap);
         hip_check(hipMemcpyAsync(recs.data(), c->d.dyn + (size_t)kind * c->cs.cap, recs.size() * sizeof(DynRec),
@@ -2872,8 +2891,10 @@ int avbd_download_detected(avbd_ctx* c, uint64_t* keys, uint32_t* slots, uint8_t
             if (keys) keys[i] = E[i].key;
             if (slots) slots[i] = E[i].slot;
             if (kinds)
-                kinds[i] = (uint8_t)(E[i].kind == DYN_TRI_RIGID ? DET_KIND_TRI_RIGID
-                                     : (E[i].kind == DYN_TRI_ROD ? DET_KIND_TRI_ROD : DET_KIND_RIGID_RIGID));
+                kinds[i] = (uint8_t)(E[i].kind == DYN_TRI_RIGID    ? DET_KIND_TRI_RIGID
+                                     : E[i].kind == DYN_TRI_ROD    ? DET_KIND_TRI_ROD
+                                     : E[i].kind == DYN_VERTEX_TRI ? DET_KIND_VERTEX_FACE
+                                                                   : DET_KIND_RIGID_RIGID);
         }
         if (count) *count = (uint32_t)E.size();
     });
@@ -2893,7 +2914,7 @@ struct CkptHeader {
     uint64_t steps_taken, vertex_updates;
     uint64_t array_bytes;     // device arrays that follow the header
 };
-constexpr uint32_t kCkptVersion = 3;
+constexpr uint32_t kCkptVersion = 4;
 constexpr char kCkptMagic[8] = {'A', 'V', 'B', 'D', 'C', 'K', 'P', 'T'};
 
 // (device pointer, bytes) of every mutable device array, in checkpoint order

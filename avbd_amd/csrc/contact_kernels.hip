@@ -110,6 +110,11 @@ __global__ void prim_bounds_kernel(DevPrims p, const float4* __restrict__ x, con
             expand(pa, pa);
             expand(pe, pe);
         }
+        // self collision (new): widened by the contact margin
+        for (int i = 0; i < 3; i++) {
+            b[i] -= p.tri_pad;
+            b[3 + i] += p.tri_pad;
+        }
     } else if (ty == PRIM_ROD_SEGMENT) {
         const double r = p.radius[k];
         const d3 p1 = ld(x[id[0]]), p2 = ld(x[id[1]]);
@@ -409,15 +414,16 @@ __device__ bool tri_sphere_ccd(const d3 tri[3], const d3 tv[3], d3 rb_pos, dq rb
 }
 
 // collision/DetectedCollision.hpp:43-66
-__device__ __forceinline__ uint64_t contact_key(uint32_t type, uint32_t a, uint32_t b) {
+__device__ __forceinline__ uint64_t contact_key(uint32_t type, uint32_t a, uint32_t b, uint32_t code = 0) {
     if (a > b) {
         const uint32_t t = a;
         a = b;
         b = t;
     }
     const uint32_t M = (1u << 26) - 1;
-    return ((((uint64_t)type & 31u) << 26 | (a & M)) << 26 | (b & M)) << 7;
+    return (((((uint64_t)type & 31u) << 26 | (a & M)) << 26 | (b & M)) << 7) | (code & 127u);
 }
+
 
 __global__ void narrow_kernel(DevPrims p, const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ counters,
                               uint32_t pair_cap, const uint32_t* __restrict__ sorted, const float4* __restrict__ x,
@@ -445,7 +451,47 @@ __global__ void narrow_kernel(DevPrims p, const uint32_t* __restrict__ pairs, co
         a = b;
         b = s;
     }
-    if (ta == PRIM_TRIANGLE && (tb == PRIM_TRIANGLE)) return;            // _triangleTriangle: no-op
+    if (ta == PRIM_TRIANGLE && tb == PRIM_TRIANGLE) {
+        // _triangleTriangle: empty in the reference; with self collision on
+        // (new), each vertex of either triangle against the other one.  A
+        // vertex is tested from its owner (the lowest-numbered triangle
+        // holding it) only, so a vertex-face pair is reported once; key =
+        // generateKey(TriangleTriangle_VertexFace, owner, face, code), code =
+        // the vertex's corner in the owner, + 3 when the owner is the
+        // higher-numbered primitive.  Normal: the face normal turned towards
+        // the face as seen from the vertex.
+        if (!p.vertex_face) return;
+        for (uint32_t side = 0; side < 2; side++) {
+            const uint32_t own = side == 0 ? a : b, face = side == 0 ? b : a;
+            const uint32_t* fi = p.idx + 3 * face;
+            const d3 fa = ld(x[fi[0]]), fb = ld(x[fi[1]]), fc = ld(x[fi[2]]);
+            for (uint32_t k = 0; k < 3; k++) {
+                const uint32_t pv = p.idx[3 * own + k];
+                if (p.owner[pv] != own || pv == fi[0] || pv == fi[1] || pv == fi[2]) continue;
+                const d3 pp = ld(x[pv]);
+                const d3 cp = closest_point_triangle(pp, fa, fb, fc);
+                if (!(norm(pp - cp) < margin)) continue;
+                const d3 f = cross(fb - fa, fc - fa);
+                const double fm = norm(f);
+                const d3 fu = fm > 0 ? (1.0 / fm) * f : mk(0, 1, 0);
+                const d3 nrm = dot(fu, pp - cp) >= 0 ? (-1.0) * fu : fu;
+                const d3 bc = barycentric(cp, fa, fb, fc);
+                DetRec r;
+                r.prim_a = face;
+                r.prim_b = own;
+                r.pad = k;
+                r.kind = DET_KIND_VERTEX_FACE;
+                r.n[0] = nrm.x; r.n[1] = nrm.y; r.n[2] = nrm.z;
+                r.bary[0] = bc.x; r.bary[1] = bc.y; r.bary[2] = bc.z;
+                r.cpl[0] = 0; r.cpl[1] = 0; r.cpl[2] = 0;
+                r.s = 0;
+                r.key = contact_key(r.kind, own, face, own < face ? k : 3 + k);
+                const uint32_t slot = atomicAdd(status + 1, 1u);
+                if (slot < det_cap) det[slot] = r;
+            }
+        }
+        return;
+    }
     if (ta == PRIM_ROD_SEGMENT && tb == PRIM_ROD_SEGMENT) return;        // _rodRod: no-op
     if (ta == PRIM_SPHERE) {
         // sphere-sphere (new; the reference throws): in contact when the

@@ -19,8 +19,11 @@ namespace avbd {
 constexpr uint32_t PRIM_TRIANGLE = 0u, PRIM_ROD_SEGMENT = 1u, PRIM_SPHERE = 2u;
 // DetectedCollisionType values (collision/DetectedCollision.hpp:6-15); the
 // rigid-rigid (sphere-sphere) pair is detected here, where the reference
-// throws (collision/CollisionDetector.cpp:678-686)
-constexpr uint32_t DET_KIND_TRI_RIGID = 2u, DET_KIND_TRI_ROD = 3u, DET_KIND_RIGID_RIGID = 6u;
+// throws (collision/CollisionDetector.cpp:678-686), and so is the
+// triangle-triangle vertex-face pair, whose narrow phase the reference leaves
+// empty (collision/CollisionDetector.cpp:589-599)
+constexpr uint32_t DET_KIND_VERTEX_FACE = 0u, DET_KIND_TRI_RIGID = 2u, DET_KIND_TRI_ROD = 3u,
+                   DET_KIND_RIGID_RIGID = 6u;
 // narrow-phase status word bits
 constexpr uint32_t NARROW_UNSUPPORTED = 1u;
 
@@ -31,6 +34,13 @@ struct DevPrims {
     uint32_t* idx = nullptr;      // 3 per primitive: triangle vertices / rod nodes / sphere centre
     uint32_t* object = nullptr;
     double* radius = nullptr;
+    // vertex-face contacts (self collision): on/off, the owner of every
+    // particle (the lowest-numbered triangle holding it, ~0 for none), and
+    // the widening of the triangle bounds (the contact margin) that lets the
+    // broad phase report every face within the margin of a vertex
+    uint32_t vertex_face = 0;
+    uint32_t* owner = nullptr;
+    double tri_pad = 0;
 };
 
 struct HostPrim {
@@ -105,7 +115,7 @@ public:
         uint32_t alloc(uint32_t cap);
         void free(uint32_t slot);
     };
-    Slots pool[3];   // triangle-rigid, triangle-rod, rigid-rigid
+    Slots pool[DYN_KINDS];   // triangle-rigid, triangle-rod, rigid-rigid, vertex-triangle
 
     struct Entry {   // one detected contact of the previous round, key order
         uint64_t key;

@@ -81,6 +81,7 @@ void ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim
     std::sort(det.begin(), det.end(), [](const DetRec& a, const DetRec& b) { return a.key < b.key; });
     std::vector<Entry> cur(det.size());
     auto kind_of = [](uint32_t det_kind) {
+        if (det_kind == DET_KIND_VERTEX_FACE) return DYN_VERTEX_TRI;
         return det_kind == DET_KIND_TRI_RIGID ? DYN_TRI_RIGID : (det_kind == DET_KIND_TRI_ROD ? DYN_TRI_ROD : DYN_RIGID_RIGID);
     };
     auto add = [&](size_t i) {
@@ -89,9 +90,11 @@ void ContactSystem::process(std::vector<DetRec>& det, const std::vector<HostPrim
         const HostPrim& pa = prims[r.prim_a];
         const HostPrim& pb = prims[r.prim_b];
         // triangle-rigid: triangle, body; triangle-rod: triangle, two nodes;
-        // rigid-rigid: the two bodies
+        // rigid-rigid: the two bodies; vertex-triangle: the face (prim_a),
+        // then corner `pad` of the vertex's owner (prim_b)
         std::array<uint32_t, 5> id{};
         if (kind == DYN_RIGID_RIGID) id = {pa.idx[0], pb.idx[0], 0u, 0u, 0u};
+        else if (kind == DYN_VERTEX_TRI) id = {pa.idx[0], pa.idx[1], pa.idx[2], pb.idx[r.pad], 0u};
         else id = {pa.idx[0], pa.idx[1], pa.idx[2], pb.idx[0], kind == DYN_TRI_ROD ? pb.idx[1] : 0u};
         const uint32_t np = dyn_particles(kind);
         double mmin = mass[id[0]];

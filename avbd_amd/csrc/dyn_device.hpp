@@ -53,7 +53,12 @@ __device__ __forceinline__ Geo geometry(const View& d, const DynRec& c) {
     const float3 cp_tri = add(add(scl(c.bary.x, x0), scl(c.bary.y, x1)), scl(c.bary.z, x2));
     const float3 cpl = make_float3(c.cpl.x, c.cpl.y, c.cpl.z);
     float3 cp;
-    if (kind_of(c) == DYN_TRI_RIGID) {
+    if (kind_of(c) == DYN_VERTEX_TRI) {
+        // vertex-triangle (new): the vertex itself, no orientation
+        g.q = make_float4(0.f, 0.f, 0.f, 1.f);
+        g.q_diff = make_float3(0.f, 0.f, 0.f);
+        cp = p3(d.x[c.idx[3]]);
+    } else if (kind_of(c) == DYN_TRI_RIGID) {
         g.q = d.q[d.rot_of[c.idx[3]]];
         g.q_diff = make_float3(0.f, 0.f, 0.f);
         cp = add(p3(d.x[c.idx[3]]), qrot(g.q, cpl));
@@ -97,13 +102,14 @@ __device__ __forceinline__ Mult multipliers(const DynRec& c, float Cn, float Ct,
     return m;
 }
 
-// 3-DOF contribution of a triangle vertex (local 0..2): gradients -bary n, t, b.
+// 3-DOF contribution of a triangle vertex (local 0..2): gradients -bary n, t, b;
+// and of the vertex of a vertex-triangle contact (local 3): +n, t, b.
 template <class Sym>
 __device__ __forceinline__ void accumulate3(const View& d, const DynRec& c, uint32_t local, float3& G, Sym& H) {
     float Cn, Ct, Cb;
     eval(d, c, Cn, Ct, Cb);
     const Mult m = multipliers(c, Cn, Ct, Cb);
-    const float beta = local == 0 ? c.bary.x : (local == 1 ? c.bary.y : c.bary.z);
+    const float beta = local == 0 ? c.bary.x : (local == 1 ? c.bary.y : (local == 2 ? c.bary.z : -1.f));
     const float3 n = p3(c.n), t = p3(c.t), b = p3(c.b);
     G = add(G, scl(-beta, add(add(scl(m.lnp, n), scl(m.lt, t)), scl(m.lb, b))));
     const float b2 = beta * beta;

@@ -134,8 +134,9 @@ constexpr uint32_t PREC_JOINT = 0x80000000u;   // per-rotation record list: join
 // energy/collision/TriangleRodCollisionEnergyPool.hpp:8-15).  idx: the
 // triangle's vertices, then the rigid body (kind 0) or the two rod nodes
 // (kind 1); kind 2 (rigid-rigid, new) holds the two bodies in idx[0..1] and
-// their body-frame contact points in cpl and bary.  Record index = kind *
-// capacity + pool slot.
+// their body-frame contact points in cpl and bary; kind 3 (vertex-triangle,
+// new) holds the triangle's vertices and then the vertex, all positional.
+// Record index = kind * capacity + pool slot.
 struct alignas(16) DynRec {
     float4 n;      // normal, w = k_start
     float4 t;      // tangent, w = mu_s
@@ -149,9 +150,12 @@ struct alignas(16) DynRec {
     uint32_t pad[2];
 };
 static_assert(sizeof(DynRec) == 144, "DynRec must be 144 bytes");
-constexpr uint32_t DYN_TRI_RIGID = 0u, DYN_TRI_ROD = 1u, DYN_RIGID_RIGID = 2u, DYN_KINDS = 3u;
-// particles per record of each kind: triangle + body, triangle + two rod nodes, two bodies
-__host__ __device__ constexpr uint32_t dyn_particles(uint32_t kind) { return kind == 0u ? 4u : (kind == 1u ? 5u : 2u); }
+constexpr uint32_t DYN_TRI_RIGID = 0u, DYN_TRI_ROD = 1u, DYN_RIGID_RIGID = 2u, DYN_VERTEX_TRI = 3u, DYN_KINDS = 4u;
+// particles per record of each kind: triangle + body, triangle + two rod
+// nodes, two bodies, triangle + vertex
+__host__ __device__ constexpr uint32_t dyn_particles(uint32_t kind) {
+    return kind == 1u ? 5u : (kind == 2u ? 2u : 4u);
+}
 constexpr uint32_t DYN_FLAG_SHIFT = 8u;
 constexpr float kContactMuS = 0.4f, kContactMuK = 0.2f;   // _addCollision (collision/CollisionDetector.cpp:365-366)
 
@@ -169,8 +173,8 @@ constexpr uint32_t DYN_OP_ADD = 0u, DYN_OP_UPDATE = 1u;
 // Narrow-phase output (device -> host): DetectedCollision fields of one pair.
 struct alignas(8) DetRec {
     uint64_t key;
-    uint32_t kind;       // DetectedCollisionType: 2 triangle-rigid, 3 triangle-rod
-    uint32_t prim_a, prim_b, pad;
+    uint32_t kind;       // DetectedCollisionType: 0 vertex-face, 2 triangle-rigid, 3 triangle-rod, 6 rigid-rigid
+    uint32_t prim_a, prim_b, pad;   // vertex-face: face, the vertex's owner, the vertex's corner in it
     double n[3], bary[3], cpl[3], s;
 };
 static_assert(sizeof(DetRec) == 104, "DetRec must be 104 bytes");

@@ -734,7 +734,11 @@ __global__ void dyn_apply_kernel(DynRec* __restrict__ recs, const DynOp* __restr
         if (dyn::kind_of(r) == DYN_TRI_ROD) r.bary.w = o.s;
         r.cpl = make_float4(o.cpl[0], o.cpl[1], o.cpl[2], r.cpl.w);
         r.bary = make_float4(o.bary[0], o.bary[1], o.bary[2], r.bary.w);
-        dyn::update_frame(r, make_float3(o.n[0], o.n[1], o.n[2]));
+        float3 nn = make_float3(o.n[0], o.n[1], o.n[2]);
+        // a vertex-triangle contact keeps the side it was created with
+        if (dyn::kind_of(r) == DYN_VERTEX_TRI && nn.x * r.n.x + nn.y * r.n.y + nn.z * r.n.z < 0.f)
+            nn = make_float3(-nn.x, -nn.y, -nn.z);
+        dyn::update_frame(r, nn);
         c = r;
     }
 }

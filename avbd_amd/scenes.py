@@ -217,6 +217,8 @@ class Scene:
     # step runs the triangle narrow phase against rods and spheres and keeps
     # the resulting dynamic contacts (CollisionDetector::detectCollisionsAndRecolor)
     contact_margin: Optional[float] = None
+    # vertex-face contacts between the collision triangles (new; needs contact_margin)
+    self_collision: bool = False
     # several meshes: independent scenes of a batch (True) or objects of one
     # simulation (False, as a YAML scene with several TetMeshObjects)
     batch: bool = True
@@ -258,9 +260,11 @@ def hanging_cube(n: int = 8, jitter: float = 1e-3, seed: int = 1, iters: int = 2
 
 
 def dropped_cube(n: int = 32, seed: int = 2, height: float = 0.5, E: float = 1e6, nu: float = 0.4,
-                 iters: int = 10, steps: int = 200) -> Scene:
+                 iters: int = 10, steps: int = 200, self_collision_margin: Optional[float] = None) -> Scene:
     """1 m Kuhn cube rotated about its centre by a random rotation, translated so
-    its lowest point is at y = height, with ground contact on every vertex."""
+    its lowest point is at y = height, with ground contact on every vertex.
+    self_collision_margin: the surface's vertex-face self collision through
+    the dynamic-contact pipeline, with that margin."""
     verts, tets = kuhn_cube(n)
     R = random_rotation(SplitMix64(seed))
     c = verts.mean(axis=0)
@@ -268,6 +272,9 @@ def dropped_cube(n: int = 32, seed: int = 2, height: float = 0.5, E: float = 1e6
     p[:, 1] += height - p[:, 1].min()
     mesh = TetMesh(rest=p, tets=tets, E=E, nu=nu, density=1000.0, kd=0.0, ground=True,
                    triangles=surface_triangles(n))
+    if self_collision_margin is not None:
+        return Scene(name=f"dropped_cube_{n}", meshes=[mesh], iters=iters, steps=steps,
+                     contact_margin=self_collision_margin, self_collision=True)
     return Scene(name=f"dropped_cube_{n}", meshes=[mesh], iters=iters, steps=steps)
 
 
@@ -371,6 +378,23 @@ def ball_on_block(n: int = 6, edge: float = 0.2, radius: float = 0.04, drop: flo
                     w0=(0.0, 2.0, 0.0), ground=False, v0=(0.0, -speed, 0.0))
     return Scene("ball_on_block", meshes=[block], spheres=[sphere], dt=dt, iters=iters, rho=0.95, steps=steps,
                  contact_margin=margin)
+
+
+def block_stack(n: int = 6, edge: float = 0.2, n_top: int = 3, top_edge: float = 0.08, gap: float = 5e-4,
+                E_top: float = 5e4, dt: float = 1.0 / 120.0, iters: int = 20, steps: int = 60,
+                margin: float = 3e-3) -> Scene:
+    """A soft cube set down on a soft block, both with their boundary
+    triangles as collision primitives and vertex-face contacts on (new): the
+    cube's bottom vertices meet the block's top faces and the block's top
+    vertices the cube's bottom faces.  The cube starts `gap` above the block,
+    off the block's vertex grid."""
+    block = soft_block(n, edge)
+    verts, tets = kuhn_cube(n_top, top_edge)
+    off = np.array([0.5 * (edge - top_edge) + 0.013, edge + gap, 0.5 * (edge - top_edge) - 0.007])
+    top = TetMesh(rest=verts + off, tets=tets, E=E_top, nu=0.4, density=1000.0, kd=0.0, ground=False,
+                  triangles=surface_triangles(n_top))
+    return Scene("block_stack", meshes=[block, top], dt=dt, iters=iters, rho=0.95, steps=steps,
+                 contact_margin=margin, self_collision=True, batch=False)
 
 
 def rod_on_block(n: int = 6, edge: float = 0.2, elements: int = 12, length: float = 0.16, diameter: float = 0.01,

@@ -267,8 +267,12 @@ class SimulationContext:
     def set_dynamic_contacts(self, margin: float, capacity: int = 0) -> None:
         self._chk(lib.avbd_set_dynamic_contacts(self.handle, float(margin), int(capacity)))
 
+    def set_self_collision(self, enable: bool = True) -> None:
+        """Vertex-face contacts between collision triangles (new); before set_dynamic_contacts."""
+        self._chk(lib.avbd_set_self_collision(self.handle, int(bool(enable))))
+
     def contacts(self, kind: int) -> np.ndarray:
-        """Active contacts of one kind (0 triangle-rigid, 1 triangle-rod, 2 rigid-rigid) in slot
+        """Active contacts of one kind (0 triangle-rigid, 1 triangle-rod, 2 rigid-rigid, 3 vertex-triangle) in slot
         order, one row of 32 per contact: slot, idx[5], n, t, b, barys, s,
         cp_local, k_start, k_n, k_t, k_b, lambda_n, lambda_t, lambda_b,
         C_n_prev, use_static, unclamped."""
@@ -641,6 +645,8 @@ class Simulation:
             self.ctx.set_scenes(offs)
         self.ctx.build_graph()
         if self.scene.contact_margin is not None:
+            if self.scene.self_collision:
+                self.ctx.set_self_collision(True)
             self.ctx.set_dynamic_contacts(self.scene.contact_margin)
         self._setup = True
 

@@ -49,6 +49,8 @@ def make_scene(name: str):
     from avbd_amd import scenes
     if name == "dropped_cube_32":
         return scenes.dropped_cube(n=32)
+    if name == "dropped_cube_32_self":   # with the surface's vertex-face self collision (margin 1 mm)
+        return scenes.dropped_cube(n=32, self_collision_margin=1e-3)
     if name == "hanging_cube_8":
         return scenes.hanging_cube(n=8)
     if name == "cube_batch_512":
@@ -386,6 +388,7 @@ def main():
         sim.ctx.synchronize()
         el = time.perf_counter() - t0
         st = sim.ctx.stats()
+        n_detected = len(sim.ctx.detected()[0]) if dynamic else None
         fixed_rate = args.fixed_steps / el
         vups = st.vertex_updates / max(st.steps_taken, 1) * fixed_rate
         sim.ctx.set_timing(True)
@@ -400,6 +403,7 @@ def main():
                  "steps_per_s": round(fixed_rate, 3), "ms_per_step": round(el / args.fixed_steps * 1e3, 4),
                  "vertex_updates_per_s": round(vups, 1), "final_residual_rel": st.residual_rel,
                  "setup_s": round(t_setup, 2),
+                 "contacts_detected_last_step": n_detected,
                  "solve_kernel": {"kernel": "solve_color_kernel", "avg_launch_us": round(f_avg_ms * 1e3, 3),
                                   "algorithmic_bytes_per_launch": int(f_bytes),
                                   "achieved_GBps": round(f_bytes / (f_avg_ms * 1e-3) / 1e9, 1) if f_avg_ms > 0 else 0},
@@ -453,7 +457,7 @@ def main():
         "dtype": ("f64" if fp64 else "f32") if res else "f32",
         "data": "synthetic",
         "config": {
-            "workload": sc.name,
+            "workload": args.workload,
             "vertices": sc.n_vertices,
             "tets": sc.n_tets,
             "ground_contacts": sc.n_ground,
@@ -462,9 +466,12 @@ def main():
                      f"checked every {args.check_every}); {'FP64' if fp64 else 'FP32'} particle state; "
                      + ("warm-started by the learned MLP" if res and res["warm_start"] else "plain start")) if res
                     else f"plain AVBD, {sc.iters} iterations per step",
-            "broad_phase": (f"LBVH over the {n_tris:,} surface triangles rebuilt and self-traversed every step"
+            "broad_phase": (f"LBVH over the {n_tris:,} surface triangles (widened by the {sc.contact_margin:g} m "
+                            f"margin) rebuilt and self-traversed, vertex-face narrow phase, every step"
+                            if n_tris and sc.self_collision else
+                            f"LBVH over the {n_tris:,} surface triangles rebuilt and self-traversed every step"
                             if n_tris else "none"),
-            "self_collision": False,
+            "self_collision": bool(sc.self_collision),
             "parallelism": "single GPU",
         },
         "fixed_residual": res,

@@ -421,6 +421,23 @@ int avbd_add_collision_sphere(avbd_ctx* ctx, uint32_t p, double radius);
  * collision!" (:210-213); a sphere welded to a rod's node does not collide
  * with that rod. */
 int avbd_set_dynamic_contacts(avbd_ctx* ctx, double collision_margin, uint32_t capacity);
+/* New: vertex-face contacts between collision triangles, of one mesh (self
+ * collision) or of two.  The reference declares TriangleTriangle_VertexFace
+ * (collision/DetectedCollision.hpp:7,29) but its triangle-triangle narrow
+ * phase is empty (collision/CollisionDetector.cpp:589-599), which is the
+ * behaviour with this off (the default).  On: the triangle bounds are
+ * widened by the collision margin, and for every candidate triangle pair
+ * each vertex of either triangle that is not a vertex of the other, tested
+ * from its owner (the lowest-numbered triangle holding it, so that each
+ * vertex-face pair is found once), is in contact with the other triangle
+ * when its distance to it is below the margin.  The normal is the face
+ * normal turned towards the face as seen from the vertex (a persisting
+ * contact keeps its side), and the contact is the triangle-rigid AL contact
+ * with the vertex as the contact point: C = dir . (x_v - sum_i barys_i x_i)
+ * for dir = n, t, b, normal and friction on all four particles (kind 3 of
+ * avbd_download_contacts; DetectedCollisionType 0).  Call before
+ * avbd_set_dynamic_contacts. */
+int avbd_set_self_collision(avbd_ctx* ctx, int enable);
 
 /* One dynamic contact (CollisionConstraintEnergyInfo + TriangleRigid /
  * TriangleRod info, energy/collision/CollisionConstraintEnergyPool.hpp:9-30,
@@ -429,7 +446,8 @@ int avbd_set_dynamic_contacts(avbd_ctx* ctx, double collision_margin, uint32_t c
 typedef struct avbd_contact {
     uint32_t slot;          /* pool slot (TombstonePool index) */
     uint32_t idx[5];        /* triangle vertices, then the body or both rod nodes;
-                               rigid-rigid: the two bodies */
+                               rigid-rigid: the two bodies; vertex-triangle:
+                               the face's vertices, then the vertex */
     double n[3], t[3], b[3];
     double barys[3];        /* rigid-rigid: the second body's contact point (body frame) */
     double s;               /* triangle-rod: contact point along the segment */
@@ -439,12 +457,13 @@ typedef struct avbd_contact {
     uint8_t use_static, unclamped, pad_[6];
 } avbd_contact;
 /* Active contacts of one kind (0 triangle-rigid, 1 triangle-rod, 2
- * rigid-rigid) in slot order: up to cap into out; *count = active total. */
+ * rigid-rigid, 3 vertex-triangle) in slot order: up to cap into out;
+ * *count = active total. */
 int avbd_download_contacts(avbd_ctx* ctx, uint32_t kind, avbd_contact* out, uint32_t cap, uint32_t* count);
 /* Contacts detected by the last step's detection in key order
  * (DetectedCollision::generateKey, collision/DetectedCollision.hpp:43-66):
- * key, pool slot, DetectedCollisionType (2 triangle-rigid, 3 triangle-rod,
- * 6 rigid-rigid). */
+ * key, pool slot, DetectedCollisionType (0 vertex-face, 2 triangle-rigid,
+ * 3 triangle-rod, 6 rigid-rigid). */
 int avbd_download_detected(avbd_ctx* ctx, uint64_t* keys, uint32_t* slots, uint8_t* kinds, uint32_t cap,
                            uint32_t* count);
 /* is_conflicted per particle after the last recolouring (n bytes). */

@@ -86,6 +86,7 @@ uint32_t oracle_add_collision_sphere(void* h, uint32_t p, double radius) {
     return static_cast<Simulation*>(h)->add_collision_sphere(p, radius);
 }
 uint32_t oracle_num_prims(void* h) { return (uint32_t)static_cast<Simulation*>(h)->prims.size(); }
+void oracle_set_self_collision(void* h, int enable) { static_cast<Simulation*>(h)->self_collision = enable != 0; }
 void oracle_enable_dynamic_contacts(void* h, double margin, uint32_t capacity) {
     static_cast<Simulation*>(h)->enable_dynamic_contacts(margin, capacity);
 }
@@ -647,6 +648,16 @@ void oracle_unit_orthogonal(const double* n3, double* t3, double* b3) {
     }
 }
 uint64_t oracle_contact_key(uint32_t type, uint32_t a, uint32_t b, uint32_t code) { return contact_key(type, a, b, code); }
+int oracle_vertex_face(const double* p3, const double* tri9, double margin, double* normal3, double* barys3) {
+    V3 n, bc;
+    const bool hit = vertex_face(v_in(p3), v_in(tri9), v_in(tri9 + 3), v_in(tri9 + 6), margin, n, bc);
+    for (int k = 0; k < 3; k++) {
+        normal3[k] = n[k];
+        barys3[k] = bc[k];
+    }
+    return hit ? 1 : 0;
+}
+
 int oracle_tri_sphere_ccd(const double* tri9, const double* tv9, const double* pos3, const double* q4, const double* v3,
                           const double* w3, double radius, double dt, double margin, double* normal3, double* barys3,
                           double* cpl3) {

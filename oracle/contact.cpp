@@ -257,7 +257,7 @@ uint64_t contact_key(uint32_t type, uint32_t a, uint32_t b, uint32_t code) {
 
 // collision/CollisionPrimitivePool.hpp:160-236 (sphere: SDF::localBounds, collision/SDF.hpp:16-23)
 void prim_bounds(const Prim& pr, const std::vector<V3>& x, const std::vector<V3>& v, const RotState& rs, Real dt,
-                 double box[6]) {
+                 double box[6], Real tri_pad) {
     for (int k = 0; k < 3; k++) {
         box[k] = std::numeric_limits<double>::max();
         box[3 + k] = std::numeric_limits<double>::lowest();
@@ -278,6 +278,10 @@ void prim_bounds(const Prim& pr, const std::vector<V3>& x, const std::vector<V3>
         for (int c = 0; c < 3; c++) {
             expand_pt(x[pr.idx[c]]);
             expand_pt(x[pr.idx[c]] + dt * v[pr.idx[c]]);
+        }
+        for (int k = 0; k < 3; k++) {   // self collision (new): widened by the contact margin
+            box[k] -= tri_pad;
+            box[3 + k] += tri_pad;
         }
     } else if (pr.type == PRIM_ROD_SEGMENT) {
         const V3 p1 = x[pr.idx[0]], p2 = x[pr.idx[1]];
@@ -393,7 +397,9 @@ void dyn_eval(uint8_t type, const DynContact& c, const std::vector<V3>& x, const
     }
     const V3 cp_tri = c.barys.x * x[c.idx[0]] + c.barys.y * x[c.idx[1]] + c.barys.z * x[c.idx[2]];
     V3 cp;
-    if (type == DYN_TRI_RIGID) {
+    if (type == DYN_VERTEX_TRI) {
+        cp = x[c.idx[3]];
+    } else if (type == DYN_TRI_RIGID) {
         cp = x[c.idx[3]] + qrot(rs.rot(c.idx[3]), c.cpl);
     } else {
         const RodMid m = rod_mid(c, x, rs);
@@ -441,6 +447,16 @@ void dyn_constraint(uint8_t type, const DynContact& c, const std::vector<V3>& x,
         hn = hblock(c.n);
         ht = hblock(c.t);
         hb = hblock(c.b);
+        return;
+    }
+    if (type == DYN_VERTEX_TRI) {
+        if (dof != 3 || local > 3) throw std::runtime_error("dynamic contact: local index out of bounds");
+        const Real w = local == 3 ? 1.0 : -c.barys[(int)local];
+        for (int i = 0; i < 3; i++) {
+            gn[i] = w * c.n[i];
+            gt[i] = w * c.t[i];
+            gb[i] = w * c.b[i];
+        }
         return;
     }
     if (dof == 3) {
@@ -632,6 +648,17 @@ bool sphere_sphere(V3 xa, const Quat& qa, Real ra, V3 xb, const Quat& qb, Real r
     return true;
 }
 
+bool vertex_face(V3 p, V3 a, V3 b, V3 c, Real margin, V3& normal, V3& barys) {
+    const V3 cp = closest_point_triangle(p, a, b, c);
+    if (!(norm(p - cp) < margin)) return false;
+    const V3 f = cross(b - a, c - a);
+    const Real fm = norm(f);
+    const V3 fu = fm > 0 ? (1.0 / fm) * f : V3(0, 1, 0);
+    normal = dot(fu, p - cp) >= 0 ? -1.0 * fu : fu;
+    barys = barycentric(cp, a, b, c);
+    return true;
+}
+
 bool tri_sphere_ccd(const V3 tri[3], const V3 tv[3], V3 rb_pos, const Quat& rb_rot, V3 rb_v, V3 rb_w, Real radius,
                     Real dt, Real margin, V3& normal, V3& cp_barys, V3& cpl) {
     auto pos_t = [&](Real t) { return rb_pos + t * rb_v; };
@@ -784,7 +811,39 @@ void Simulation::process_pair(uint32_t a, uint32_t b, std::vector<Detected>& out
         std::swap(a, b);
     }
     const Real dt = params.dt;
-    if (ta == PRIM_TRIANGLE && tb == PRIM_TRIANGLE) return;        // _triangleTriangle: no-op
+    if (ta == PRIM_TRIANGLE && tb == PRIM_TRIANGLE) {
+        if (!self_collision) return;   // the reference's _triangleTriangle is empty
+        // _triangleTriangle, vertex-face only (new; the reference's is empty):
+        // each vertex of either triangle against the other triangle.  A
+        // vertex is tested from its owner, the lowest-numbered triangle that
+        // holds it, so that a vertex-face pair is found once; the key is the
+        // reference's generateKey(TriangleTriangle_VertexFace, owner, face,
+        // code) with code = the vertex's corner in the owner, + 3 when the
+        // owner is the higher-numbered primitive.
+        for (int side = 0; side < 2; side++) {
+            const uint32_t own = side == 0 ? a : b, face = side == 0 ? b : a;
+            const Prim& O = prims[own];
+            const Prim& F = prims[face];
+            for (uint32_t k = 0; k < 3; k++) {
+                const uint32_t pv = O.idx[k];
+                if (vertex_owner[pv] != own) continue;
+                if (pv == F.idx[0] || pv == F.idx[1] || pv == F.idx[2]) continue;
+                V3 nrm, bc;
+                if (!vertex_face(x[pv], x[F.idx[0]], x[F.idx[1]], x[F.idx[2]], collision_margin, nrm, bc)) continue;
+                Detected d;
+                d.type = DET_VERTEX_FACE;
+                d.key = contact_key(DET_VERTEX_FACE, own, face, own < face ? k : 3 + k);
+                d.gen1 = F.gen;
+                d.gen2 = O.gen;
+                d.normal = nrm;
+                for (int q = 0; q < 3; q++) d.tri[q] = F.idx[q];
+                d.barys = bc;
+                d.vtx = pv;
+                out.push_back(d);
+            }
+        }
+        return;
+    }
     if (ta == PRIM_ROD_SEGMENT && tb == PRIM_ROD_SEGMENT) return;  // _rodRod: no-op
     if (ta == PRIM_SPHERE && tb == PRIM_SPHERE) {
         // new: the reference throws here (collision/CollisionDetector.cpp:678-686)
@@ -874,6 +933,7 @@ void Simulation::process_pair(uint32_t a, uint32_t b, std::vector<Detected>& out
 }
 
 static uint8_t dyn_type_of(uint8_t det_type) {
+    if (det_type == DET_VERTEX_FACE) return DYN_VERTEX_TRI;
     return det_type == DET_TRI_RIGID ? DYN_TRI_RIGID : (det_type == DET_TRI_ROD ? DYN_TRI_ROD : DYN_RIGID_RIGID);
 }
 
@@ -898,6 +958,10 @@ void Simulation::add_contact(Detected& d) {
         c.idx[0] = d.rb;
         c.idx[1] = d.rb2;
         c.cpl2 = d.cpl2;
+    } else if (d.type == DET_VERTEX_FACE) {
+        ty = DYN_VERTEX_TRI;
+        min_mass = std::min({mass[d.tri[0]], mass[d.tri[1]], mass[d.tri[2]], mass[d.vtx]});
+        c.idx[3] = d.vtx;
     } else if (d.type == DET_TRI_RIGID) {
         ty = DYN_TRI_RIGID;
         min_mass = std::min({mass[d.tri[0]], mass[d.tri[1]], mass[d.tri[2]], mass[d.rb]});
@@ -931,7 +995,9 @@ void Simulation::add_contact(Detected& d) {
         return;
     }
     for (int k = 0; k < 3; k++) mark_dirty(d.tri[k]);
-    if (ty == DYN_TRI_RIGID) {
+    if (ty == DYN_VERTEX_TRI) {
+        mark_dirty(d.vtx);
+    } else if (ty == DYN_TRI_RIGID) {
         mark_dirty(d.rb);
     } else {
         mark_dirty(d.rod[0]);
@@ -948,7 +1014,9 @@ void Simulation::update_contact(const Detected& d) {
     c.cpl = d.cpl;
     c.cpl2 = d.cpl2;
     c.barys = d.barys;
-    update_collision_frame(c, d.normal);
+    // a vertex-face contact keeps the side it was created with
+    const bool flip = d.type == DET_VERTEX_FACE && dot(d.normal, c.n) < 0;
+    update_collision_frame(c, flip ? -1.0 * d.normal : d.normal);
 }
 
 // _handleDetectedCollisions (:272-327): merge the key-sorted lists.
@@ -986,9 +1054,9 @@ void Simulation::handle_detected() {
 // DynamicParticleAdjacency::buildAdjacency (common/DynamicParticleAdjacency.hpp:33-118)
 void Simulation::build_dynamic_adjacency() {
     const uint32_t n = (uint32_t)x.size();
-    std::vector<std::array<uint32_t, DYN_COUNT>> val(n, {0, 0, 0});
+    std::vector<std::array<uint32_t, DYN_COUNT>> val(n, {0, 0, 0, 0});
     std::vector<uint32_t> deg(n, 0);
-    const uint32_t npe[DYN_COUNT] = {4, 5, 2};
+    const uint32_t npe[DYN_COUNT] = {4, 5, 2, 4};
     for (int ty = 0; ty < DYN_COUNT; ty++) {
         const DynPool& P = dyn[ty];
         for (uint32_t e = 0; e < P.sp.active.size(); e++) {
@@ -999,7 +1067,7 @@ void Simulation::build_dynamic_adjacency() {
             }
         }
     }
-    dadj.e_offsets.assign(n + 1, {0, 0, 0});
+    dadj.e_offsets.assign(n + 1, {0, 0, 0, 0});
     dadj.p_offsets.assign(n + 1, 0);
     for (uint32_t i = 0; i < n; i++) {
         for (int ty = 1; ty < DYN_COUNT; ty++) dadj.e_offsets[i][ty] = dadj.e_offsets[i][ty - 1] + val[i][ty - 1];
@@ -1110,7 +1178,14 @@ void Simulation::detect_and_recolor() {
     last_candidates = 0;
     if (!prims.empty()) {
         std::vector<double> boxes(6 * prims.size());
-        for (size_t i = 0; i < prims.size(); i++) prim_bounds(prims[i], x, v, rs, params.dt, boxes.data() + 6 * i);
+        const Real pad = self_collision ? collision_margin : 0.0;
+        for (size_t i = 0; i < prims.size(); i++)
+            prim_bounds(prims[i], x, v, rs, params.dt, boxes.data() + 6 * i, pad);
+        // owner of every triangle vertex: the lowest-numbered triangle holding it
+        vertex_owner.assign(x.size(), UINT32_MAX);
+        for (uint32_t i = (uint32_t)prims.size(); i-- > 0;)
+            if (prims[i].type == PRIM_TRIANGLE)
+                for (int k = 0; k < 3; k++) vertex_owner[prims[i].idx[k]] = i;
         bp.build_boxes(boxes);
         std::vector<std::pair<uint32_t, uint32_t>> pairs;
         bp.traverse_pairs(pairs);

@@ -57,9 +57,10 @@ Real golden_section_quadratic(Real a, Real b, Real c);
 // DetectedCollision::generateKey: 5 type bits | 26 | 26 | 7 code bits.
 uint64_t contact_key(uint32_t type, uint32_t a, uint32_t b, uint32_t code);
 
-// CollisionPrimitivePool::speculativeGlobalBounds: box[6] = min xyz, max xyz.
+// CollisionPrimitivePool::speculativeGlobalBounds: box[6] = min xyz, max xyz;
+// triangle boxes widened by tri_pad on every side (self collision, new).
 void prim_bounds(const Prim& pr, const std::vector<V3>& x, const std::vector<V3>& v, const RotState& rs, Real dt,
-                 double box[6]);
+                 double box[6], Real tri_pad = 0);
 
 // TombstonePool: a stack of free slots (initially 0, 1, 2, ...), LIFO reuse.
 struct SlotPool {
@@ -76,13 +77,18 @@ struct SlotPool {
 // values; the rigid-rigid (sphere-sphere) contact is new: the reference
 // declares its detection record (collision/DetectedCollision.hpp:33) but
 // throws on sphere-sphere pairs (collision/CollisionDetector.cpp:678-686).
-enum DynType : uint8_t { DYN_TRI_RIGID = 0, DYN_TRI_ROD = 1, DYN_RIGID_RIGID = 2, DYN_COUNT = 3 };
-enum DetType : uint8_t { DET_TRI_RIGID = 2, DET_TRI_ROD = 3, DET_RIGID_RIGID = 6 };
+// The vertex-face contact between surface triangles is new as well: the
+// reference declares TriangleTriangle_VertexFace (collision/DetectedCollision.hpp:7,29)
+// but its triangle-triangle narrow phase is empty
+// (collision/CollisionDetector.cpp:589-599) and adding one throws (:333-336).
+enum DynType : uint8_t { DYN_TRI_RIGID = 0, DYN_TRI_ROD = 1, DYN_RIGID_RIGID = 2, DYN_VERTEX_TRI = 3, DYN_COUNT = 4 };
+enum DetType : uint8_t { DET_VERTEX_FACE = 0, DET_TRI_RIGID = 2, DET_TRI_ROD = 3, DET_RIGID_RIGID = 6 };
 
 // CollisionConstraintEnergyInfo + TriangleRigid / TriangleRod info.
 // idx: triangle vertices, then the rigid body (tri-rigid) or both rod nodes;
 // rigid-rigid: the two bodies (idx[0], idx[1]) with their body-frame contact
-// points cpl and cpl2.
+// points cpl and cpl2; vertex-triangle: the triangle's vertices, then the
+// vertex (idx[3]).
 struct DynContact {
     uint32_t idx[5] = {0, 0, 0, 0, 0};
     Real k_start = 0, k_n = 0, k_t = 0, k_b = 0;
@@ -118,6 +124,7 @@ struct Detected {
     V3 cpl;
     uint32_t rb2 = 0;                // rigid-rigid: the second body
     V3 cpl2;
+    uint32_t vtx = 0;                // vertex-face: the vertex (tri: the face)
 };
 
 struct DynAdjEntry {
@@ -145,6 +152,8 @@ struct Recoloring {
 // otherwise).  Rigid-rigid: C = dir . (cp_2 - cp_1) for dir = n, t, b with
 // cp_k = x_k + R_k cpl_k; local 0 is the first body (sign -1), local 1 the
 // second (sign +1); both bodies carry the normal and the friction terms.
+// Vertex-triangle: C = dir . (x_v - sum_i barys_i x_i); the triangle's
+// vertices take -barys_i dir, the vertex (local 3) +dir, friction included.
 void dyn_constraint(uint8_t type, const DynContact& c, const std::vector<V3>& x, const RotState& rs, uint32_t local,
                     int dof, Real& Cn, Real& Ct, Real& Cb, Real gn[6], Real gt[6], Real gb[6], M3& hn, M3& ht, M3& hb);
 void dyn_eval(uint8_t type, const DynContact& c, const std::vector<V3>& x, const RotState& rs, Real& Cn, Real& Ct,
@@ -160,6 +169,16 @@ void update_collision_frame(DynContact& c, V3 new_normal);
 // CollisionDetector::_triangleSDF_CCD for a sphere SDF.  tri/tv: triangle
 // vertices and velocities; rb_*: body position, orientation, velocity and
 // body-frame angular velocity.
+// Vertex-face detection (new): vertex p is in contact with triangle (a, b, c)
+// when its distance to the triangle is below the margin.  The normal is the
+// face normal f = unit((b - a) x (c - a)) turned towards the face as seen
+// from the vertex (-f when f . (p - cp) >= 0, else f), so that
+// C_n = n . (p - cp_tri) is negative while the vertex is apart and positive
+// once it is through; barys: the closest point.  An existing contact keeps
+// the side it was created with: its update turns a new normal that points
+// against the current one around (a vertex that went through the face is
+// pushed back, not further).
+bool vertex_face(V3 p, V3 a, V3 b, V3 c, Real margin, V3& normal, V3& barys);
 // Sphere-sphere detection (new): bodies at xa, xb with radii ra, rb are in
 // contact when |xa - xb| < ra + rb + margin; n points from b to a, and the
 // contact points are the surface points facing each other, in body frames.

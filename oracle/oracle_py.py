@@ -267,6 +267,7 @@ for _k, (_r, _a2) in {
     "oracle_add_collision_sphere": (C.c_uint32, [C.c_void_p, C.c_uint32, C.c_double]),
     "oracle_num_prims": (C.c_uint32, [C.c_void_p]),
     "oracle_enable_dynamic_contacts": (None, [C.c_void_p, C.c_double, C.c_uint32]),
+    "oracle_set_self_collision": (None, [C.c_void_p, C.c_int]),
     "oracle_detect_and_recolor": (C.c_int, [C.c_void_p]),
     "oracle_num_contacts": (C.c_uint32, [C.c_void_p, C.c_uint32]),
     "oracle_get_contacts": (C.c_uint32, [C.c_void_p, C.c_uint32, _D, C.c_uint32]),
@@ -278,6 +279,7 @@ for _k, (_r, _a2) in {
     "oracle_unit_orthogonal": (None, [_D, _D, _D]),
     "oracle_contact_key": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "oracle_tri_sphere_ccd": (C.c_int, [_D, _D, _D, _D, _D, _D, C.c_double, C.c_double, C.c_double, _D, _D, _D]),
+    "oracle_vertex_face": (C.c_int, [_D, _D, C.c_double, _D, _D]),
     "oracle_dyn_single": (None, [C.c_uint32, _D, _D, _D, C.c_uint32, C.c_uint32, _D, _D, _D, _D, _D]),
     "oracle_slot_sequence": (C.c_int, [C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, _U32]),
     "oracle_update_collision_frame": (None, [_D, _D]),
@@ -305,7 +307,8 @@ def _add_collision_sphere(self, p, radius):
     return int(lib.oracle_add_collision_sphere(self.h, int(p), float(radius)))
 
 
-def _enable_dynamic_contacts(self, margin, capacity=1 << 16):
+def _enable_dynamic_contacts(self, margin, capacity=1 << 16, self_collision=False):
+    lib.oracle_set_self_collision(self.h, int(bool(self_collision)))
     lib.oracle_enable_dynamic_contacts(self.h, float(margin), int(capacity))
 
 
@@ -382,6 +385,13 @@ def tri_sphere_ccd(tri, tri_vel, pos, q, v, w, radius, dt, margin):
     hit = lib.oracle_tri_sphere_ccd(dp(_a(tri, 9)), dp(_a(tri_vel, 9)), dp(_a(pos, 3)), dp(_a(q, 4)), dp(_a(v, 3)),
                                     dp(_a(w, 3)), radius, dt, margin, dp(n), dp(bc), dp(cp))
     return bool(hit), n, bc, cp
+
+
+def vertex_face(p, tri, margin):
+    """Vertex-face detection (new): hit, normal, barycentrics of the closest point."""
+    n, bc = np.empty(3), np.empty(3)
+    hit = lib.oracle_vertex_face(dp(_a(p, 3)), dp(_a(tri, 9)), margin, dp(n), dp(bc))
+    return bool(hit), n, bc
 
 
 def dyn_single(kind, x5, q5, rec26, local, dof):
@@ -504,7 +514,7 @@ def from_scene(scene, mesh_index=0, collide=True) -> OracleSim:
         o.set_rotation_state(q, w)
     o.build_graph()
     if dynamic:
-        o.enable_dynamic_contacts(scene.contact_margin)
+        o.enable_dynamic_contacts(scene.contact_margin, self_collision=getattr(scene, "self_collision", False))
     return o
 
 

@@ -39,7 +39,9 @@ public:
     uint32_t next_object = 0;
     bool dynamic = false;
     Real collision_margin = 1e-3;
-    DynPool dyn[DYN_COUNT];   // TRIANGLE_RIGID_COLLISION, TRIANGLE_ROD_COLLISION, rigid-rigid (new)
+    bool self_collision = false;   // vertex-face contacts between collision triangles (new)
+    DynPool dyn[DYN_COUNT];   // TRIANGLE_RIGID_COLLISION, TRIANGLE_ROD_COLLISION, rigid-rigid, vertex-triangle (new)
+    std::vector<uint32_t> vertex_owner;   // per particle: lowest-numbered triangle primitive holding it
     std::vector<Detected> prev_det, cur_det;
     std::vector<uint32_t> prev_order, cur_order;
     DynAdjacency dadj;

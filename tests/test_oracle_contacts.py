@@ -269,7 +269,7 @@ def _union_graph(o, sc):
     for p in range(n):
         for k in range(off[p], off[p + 1]):
             nb[p].add(int(nbrs[k]))
-    for kind, npe in ((0, 4), (1, 5)):
+    for kind, npe in ((0, 4), (1, 5), (2, 2), (3, 4)):
         for r in o.contacts(kind):
             idx = [int(v) for v in r[1:1 + npe]]
             for i in idx:
@@ -279,7 +279,7 @@ def _union_graph(o, sc):
     return nb
 
 
-@pytest.mark.parametrize("name", ["ball_on_block", "rod_on_block"])
+@pytest.mark.parametrize("name", ["ball_on_block", "rod_on_block", "block_stack"])
 def test_recolouring_invariants_and_lifecycle(name):
     """After every step's detection: the colouring is proper on the static +
     dynamic graph except between conflicted particles, and every same-colour
@@ -307,7 +307,7 @@ def test_recolouring_invariants_and_lifecycle(name):
                 if col[p] == col[q]:
                     assert conf[p] and conf[q], (step, p, q)
         # contact records: slots unique, frames orthonormal, k >= k_start
-        for kind in (0, 1):
+        for kind in (0, 1, 2, 3):
             rec = o.contacts(kind)
             if len(rec):
                 assert len(set(rec[:, 0].astype(int))) == len(rec)
@@ -464,3 +464,134 @@ def test_resting_sphere_on_a_fixed_sphere_carries_its_weight():
     assert abs(abs(n[1]) - 1.0) < 1e-9
     fy = rec[0, 26] * abs(n[1])
     assert abs(fy - w) < 0.05 * w, (fy, w)
+
+
+# ---------------------------------------------------------------------------
+# vertex-face contacts between triangles (new: the reference's triangle-
+# triangle narrow phase is empty, collision/CollisionDetector.cpp:589-599)
+# ---------------------------------------------------------------------------
+def test_vertex_face_closed_form():
+    """A vertex near a triangle: contact iff its distance to the triangle is
+    below the margin; the normal is the face normal turned towards the face
+    from the vertex's side (either side); barycentrics of the closest point,
+    on an edge when the vertex is beyond it."""
+    tri = np.array([[0.0, 0, 0], [1.0, 0, 0], [0.0, 0, 1.0]])   # (b - a) x (c - a) = -y
+    margin = 2e-3
+    for d, hit in [(1e-3, True), (3e-3, False)]:
+        for side in (1.0, -1.0):
+            p = np.array([0.25, side * d, 0.3])
+            got, n, bc = O.vertex_face(p, tri, margin)
+            assert got == hit
+            if hit:
+                assert np.allclose(n, [0, -side, 0], atol=1e-15)
+                assert np.allclose(bc, [0.45, 0.25, 0.3], atol=1e-14)
+    # beyond the hypotenuse: the closest point is on edge b-c
+    p = np.array([0.5 + 1e-3, 0.0, 0.5 + 1e-3])
+    got, n, bc = O.vertex_face(p, tri, margin)
+    assert got and np.allclose(bc, [0.0, 0.5, 0.5], atol=1e-12)
+    got, _, _ = O.vertex_face(np.array([0.5 + 2e-3, 0.0, 0.5 + 2e-3]), tri, margin)   # 2.8 mm away
+    assert not got
+
+
+def test_vertex_triangle_gradients_by_finite_differences():
+    """C = dir . (x_v - sum_i barys_i x_i) for dir = n, t, b: every direction's
+    gradient for the three face vertices (-barys_i dir) and the vertex (+dir)
+    against central differences, and the AL accumulate G = sum lambda+ grad."""
+    x = rng.normal(size=(5, 3)) * 0.1
+    q = np.zeros((5, 4))
+    q[:, 0] = 1
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    t, b = O.complete_basis(n)
+    bc = np.array([0.2, 0.3, 0.5])
+    rec = _rec(n, t, b, bc, 0.0, np.zeros(3))
+    h = 1e-6
+    for loc in range(4):
+        Cv, g, hblk, G, H = O.dyn_single(3, x, q, rec, loc, 3)
+        for dirn in range(3):
+            fd = np.zeros(3)
+            for i in range(3):
+                xp, xm = x.copy(), x.copy()
+                xp[loc, i] += h
+                xm[loc, i] -= h
+                fd[i] = (O.dyn_single(3, xp, q, rec, 0, 3)[0][dirn] - O.dyn_single(3, xm, q, rec, 0, 3)[0][dirn]) / (2 * h)
+            assert np.allclose(g[dirn, :3], fd, atol=1e-8), (loc, dirn)
+        assert np.all(hblk == 0)
+        w = 1.0 if loc == 3 else -bc[loc]
+        assert np.allclose(g[0, :3], w * n, atol=1e-15)
+
+
+def _key_pair(key, prims):
+    """(vertex, face) of a vertex-face key: type 0, owner / face, corner code."""
+    code = int(key) & 127
+    lo = (int(key) >> 33) & ((1 << 26) - 1)
+    hi = (int(key) >> 7) & ((1 << 26) - 1)
+    own, face = (lo, hi) if code < 3 else (hi, lo)
+    return int(prims[own][code % 3]), face
+
+
+def test_vertex_face_pairs_found_once_and_match_brute_force():
+    """Per step of the stacked blocks: every detected vertex-face pair is
+    reported once, and the set equals the brute-force set of (vertex, face)
+    pairs closer than the margin, excluding a vertex of the face and pairs
+    whose vertex's owner triangle shares a particle with the face in the
+    same mesh (the reference's _shouldSkip on the pair that tests it)."""
+    sc = scenes.block_stack(steps=20)
+    o = O.from_scene(sc)
+    nb = sc.meshes[0].rest.shape[0]
+    prims = np.vstack([sc.meshes[0].triangles.astype(np.int64), sc.meshes[1].triangles.astype(np.int64) + nb])
+    obj = np.r_[np.zeros(len(sc.meshes[0].triangles), int), np.ones(len(sc.meshes[1].triangles), int)]
+    owner = {}
+    for i in range(len(prims) - 1, -1, -1):
+        for v in prims[i]:
+            owner[int(v)] = i
+    def brute(x):
+        want = set()
+        for f in range(len(prims)):
+            tri = x[prims[f]]
+            lo, hi = tri.min(0) - sc.contact_margin, tri.max(0) + sc.contact_margin
+            for v, ow in owner.items():
+                if v in prims[f] or np.any(x[v] < lo) or np.any(x[v] > hi):
+                    continue
+                if obj[ow] == obj[f] and len(set(prims[ow]) & set(prims[f])):
+                    continue
+                if O.vertex_face(x[v], tri, sc.contact_margin)[0]:
+                    want.add((v, f))
+        return want
+
+    seen = 0
+    for step in range(12):
+        want = brute(o.positions())   # the detection runs on the start-of-step positions
+        o.step(1)
+        keys, _, types = o.detected()
+        assert np.all(types == 0)
+        got = [_key_pair(k, prims) for k in keys]
+        assert len(set(got)) == len(got)
+        assert set(got) == want, step
+        seen += len(got)
+    assert seen > 0
+
+
+def test_cube_resting_on_a_block_is_carried_by_vertex_face_contacts():
+    """The stacked cube settles on the block: the vertical forces of the
+    vertex-face contacts on the cube (vertex forces -(lambda . dir) on its
+    vertices, face forces +(lambda . dir) on its faces) balance its weight
+    within 2 %, and no vertex ends more than 0.5 mm behind a face."""
+    sc = scenes.block_stack(steps=300)
+    o = O.from_scene(sc)
+    nb = sc.meshes[0].rest.shape[0]
+    m, _ = o.mass_fixed()
+    W = m[nb:].sum() * sc.g
+    o.step(200)
+    x = o.positions()
+    rec = o.contacts(3)
+    assert len(rec) > 8
+    fy, pen = 0.0, []
+    for r in rec:
+        idx = r[1:5].astype(int)
+        n, t, b, bc = r[6:9], r[9:12], r[12:15], r[15:18]
+        f = r[26] * n + r[27] * t + r[28] * b
+        pen.append(np.dot(n, x[idx[3]] - (bc[0] * x[idx[0]] + bc[1] * x[idx[1]] + bc[2] * x[idx[2]])))
+        fy += -f[1] if idx[3] >= nb else f[1]
+    assert abs(fy - W) < 0.02 * W, (fy, W)
+    assert max(pen) < 5e-4
