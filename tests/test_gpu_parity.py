@@ -157,6 +157,34 @@ def test_batched_scenes_match_individual_oracles(oracle):
         assert err <= 2e-3
 
 
+def test_full_batch_512_matches_sampled_oracles(oracle):
+    """The batch workload at its full size (512 scenes, 941,649 vertices, 10
+    iterations): every 16th scene and the largest one, each against an
+    oracle that runs that scene alone -- colours bit-exact, positions within
+    1e-5 x the scene's bbox after one step and 2e-4 after ten."""
+    banner("512-scene batch vs sampled per-scene oracles")
+    sc = scenes.cube_batch(n_scenes=512)
+    g = gpu_sim(sc)
+    offs = g.offsets
+    sizes = [m.rest.shape[0] for m in sc.meshes]
+    picks = sorted(set(range(0, 512, 16)) | {int(np.argmax(sizes))})
+    col, _ = g.ctx.coloring()
+    orcs = {i: oracle.from_scene(sc, i) for i in picks}
+    for i, o in orcs.items():
+        assert np.array_equal(col[offs[i]:offs[i + 1]], o.colors()[0]), i
+    for steps, tol in ((1, 1e-5), (9, 2e-4)):
+        g.update(steps)
+        xg = g.positions()
+        worst = 0.0
+        for i, o in orcs.items():
+            o.step(steps)
+            xo = o.positions()
+            err = np.abs(xg[offs[i]:offs[i + 1]] - xo).max() / bbox(xo)
+            worst = max(worst, err)
+            assert err <= tol, (i, steps, err)
+        banner(f"  after {steps if steps == 1 else 10} steps: worst rel err {worst:.3e} over {len(orcs)} scenes")
+
+
 def test_graph_replay_matches_eager(oracle):
     """hipGraph replays are checked against eager launches of the same kernels,
     with eager work and synchronisations between replays."""
