@@ -519,6 +519,18 @@ SceneFreeze scene_freeze(const avbd_ctx* c) {
     return fz;
 }
 
+// The FP64 state with this round's dynamic-contact arrays (the adjacency
+// buffers can move when a detection round grows them).
+F64State f64_state(const avbd_ctx* c) {
+    F64State f = c->f;
+    if (c->dynamic) {
+        f.dyn_off = c->d.dyn_off;
+        f.dyn_ent = c->d.dyn_ent;
+        f.dyn = c->d.dyn;
+    }
+    return f;
+}
+
 void enqueue_head(avbd_ctx* c, bool with_events) {
     const float dt = (float)c->desc.dt, g = (float)c->desc.g_accel;
     hipStream_t s = c->stream;
@@ -542,7 +554,11 @@ void enqueue_head(avbd_ctx* c, bool with_events) {
     }
     // dynamic contacts' updateAfterTimeStep reads the start-of-step positions
     const uint32_t n_active = c->dynamic ? (uint32_t)c->cs.active.size() : 0u;
-    if (n_active) launch_dyn_step_begin(s, c->d, static_cast<const uint32_t*>(c->g_active.p), n_active);
+    if (n_active) {
+        const uint32_t* act = static_cast<const uint32_t*>(c->g_active.p);
+        if (c->fp64) launch_dyn_step_begin_f64(s, f64_state(c), act, n_active);
+        else launch_dyn_step_begin(s, c->d, act, n_active);
+    }
     if (c->fp64) {
         launch_step_begin_f64(s, c->f, c->d.n, c->desc.dt, c->desc.g_accel);
     } else {
@@ -588,7 +604,7 @@ void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool
     // the step's state with the converged-scene flags of a batch
     const SceneFreeze fz = scene_freeze(c);
     DevState dz = c->d;
-    F64State f0 = c->f;
+    F64State f0 = f64_state(c);
     dz.frozen = f0.frozen = fz.frozen;
     dz.scene_of = f0.scene_of = fz.scene_of;
     for (uint32_t i = i0; i < i1; i++) {
@@ -611,8 +627,11 @@ void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool
                 else
                     launch_solve_color_f64(s, fi, c->cfg, b, e - b, c->damping, c->desc.dt, fi.x, done);
                 if (with_events) hip_check(hipEventRecord(c->events[ev++], s), "hipEventRecord");
+                const uint32_t cb = c->plan.conflict_offsets[col], ce = c->plan.conflict_offsets[col + 1];
+                if (ce > cb) launch_commit_f64(s, fi, c->d_conflicts + cb, ce - cb, done);
             }
             if (!fuse) launch_cheb_dual_f64(s, f0, c->d.n, omega, rc, c->omega_tab, j);
+            if (n_active) launch_dyn_dual_f64(s, f0, d_active, n_active, done);
             continue;
         }
         // fused: iteration i works on x_cur and extrapolates into x_next
@@ -828,7 +847,9 @@ void read_layout(avbd_ctx* c) {
     plan.item_offsets.assign(c->h_layout, c->h_layout + K + 1);
     plan.oitem_offsets.assign(c->h_layout + 257, c->h_layout + 257 + K + 1);
     plan.conflict_offsets.assign(c->h_layout + 514, c->h_layout + 514 + K + 1);
-    const size_t need = 2 * (size_t)c->iters() * K;
+    // per-launch timing events (fixed iteration counts only; residual mode
+    // runs without them)
+    const size_t need = c->rc ? 0 : 2 * (size_t)c->iters() * K;
     while (c->events.size() < need) {
         hipEvent_t ev;
         hip_check(hipEventCreate(&ev), "hipEventCreate");
@@ -1994,6 +2015,12 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
                 const auto t0 = std::chrono::steady_clock::now();
                 detect_round(c);
                 detect_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (c->rc) {
+                    // residual-targeted: the interval sequence, launched eagerly
+                    // (the work list may change from step to step)
+                    run_residual_step(c, false);
+                    continue;
+                }
                 enqueue_step(c, c->timing);
                 c->vertex_updates += (uint64_t)c->desc.solver_iters * (c->d.n_items + c->d.n_oitems);
                 if (c->timing) {
@@ -2342,7 +2369,7 @@ int avbd_get_stats(avbd_ctx* c, avbd_stats* out) {
         hip_check(hipMemsetAsync(c->d.reduce, 0, (size_t)c->reduce_blocks * 4 * sizeof(double), c->stream),
                   "hipMemsetAsync");
         if (c->fp64)
-            launch_stats_f64(c->stream, c->f, c->cfg, c->damping, c->desc.dt, c->desc.g_accel, c->d.reduce);
+            launch_stats_f64(c->stream, f64_state(c), c->cfg, c->damping, c->desc.dt, c->desc.g_accel, c->d.reduce);
         else
             launch_stats(c->stream, c->d, c->cfg, c->damping, (float)c->desc.dt, (float)c->desc.g_accel);
         hip_check(hipGetLastError(), "stats launch");
@@ -2667,7 +2694,6 @@ int avbd_get_warm_start_decision(avbd_ctx* c, double* ip_delta, float* step, uin
 int avbd_set_warm_start(avbd_ctx* c, int enable) {
     return guarded(c, [&] {
         ensure_model(c);
-        if (enable && c->dynamic) throw CodeError(AVBD_E_ARG, "the warm start is not available with dynamic contacts");
         if (enable && !c->mlp->loaded) throw CodeError(AVBD_E_ARG, "the warm-start network has no trained parameters");
         if (enable) c->mlp->ensure_batch((int)c->rows_pad);
         c->warm = enable != 0;
@@ -2726,9 +2752,11 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         if (c->dynamic) throw CodeError(AVBD_E_ARG, "dynamic contacts are already on");
         if (c->prims.empty()) throw CodeError(AVBD_E_ARG, "no collision primitives");
         if (!(margin >= 0)) throw CodeError(AVBD_E_ARG, "collision margin must be >= 0");
-        if (c->warm || c->recording || c->res_tol > 0)
-            throw CodeError(AVBD_E_ARG, "warm start, recording and residual targets are not available with dynamic contacts");
-        if (c->fp64) throw CodeError(AVBD_E_ARG, "the FP64 state is not available with dynamic contacts");
+        if (c->recording) throw CodeError(AVBD_E_ARG, "recording is not available with dynamic contacts");
+        if (c->fp64)
+            for (const HostPrim& pr : c->prims)
+                if (pr.type != PRIM_TRIANGLE)
+                    throw CodeError(AVBD_E_ARG, "the FP64 state takes collision triangles only (vertex-face contacts)");
         const uint32_t n = c->d.n;
         auto oriented = [&](uint32_t p) { return p < n && c->rot_of[p] >= 0; };
         for (const HostPrim& pr : c->prims) {
@@ -2781,6 +2809,7 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         c->d_det = dev_alloc<DetRec>(c, c->det_cap);
         c->d_status = dev_alloc<uint32_t>(c, 2);
         c->d.dyn = dev_alloc<DynRec>(c, DYN_KINDS * (size_t)cap);
+        if (c->fp64) c->f.x_buf = dev_alloc<double4>(c, n);   // conflicted particles' buffered positions
         c->cs.init(n, cap, margin);
         auto* d_off = static_cast<uint32_t*>(grow(c, c->g_off, (n + 1) * sizeof(uint32_t)));
         upload(c, d_off, c->cs.dyn_off);

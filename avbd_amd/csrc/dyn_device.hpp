@@ -232,10 +232,9 @@ __device__ __forceinline__ void accumulate6(const View& d, const DynRec& c, uint
     sym6_outer(H, gn, c.k.x);
 }
 
-// CollisionConstraintEnergySolver::updateAfterIteration (:60-134)
-__device__ __forceinline__ void update_after_iteration(const View& d, DynRec& c) {
-    float Cn, Ct, Cb;
-    eval(d, c, Cn, Ct, Cb);
+// CollisionConstraintEnergySolver::updateAfterIteration (:60-134) at the
+// constraint values Cn, Ct, Cb of the current state.
+__device__ __forceinline__ void al_update(DynRec& c, float Cn, float Ct, float Cb) {
     const float Ccorr = Cn - kAlpha * c.cpl.w;
     const float lnp = c.k.x * Ccorr + c.lam.x;
     const float ltp = c.k.y * Ct + c.lam.y;
@@ -263,13 +262,17 @@ __device__ __forceinline__ void update_after_iteration(const View& d, DynRec& c)
     c.kind_flags = (c.kind_flags & 0xffu) | (fl << DYN_FLAG_SHIFT);
 }
 
-// CollisionConstraintEnergySolver::updateAfterTimeStep (:141-167)
-__device__ __forceinline__ void update_after_timestep(const View& d, DynRec& c) {
-    if (c.lam.x > 0.f) {
-        float Cn, Ct, Cb;
-        eval(d, c, Cn, Ct, Cb);
-        c.cpl.w = Cn;
-    }
+__device__ __forceinline__ void update_after_iteration(const View& d, DynRec& c) {
+    float Cn, Ct, Cb;
+    eval(d, c, Cn, Ct, Cb);
+    al_update(c, Cn, Ct, Cb);
+}
+
+// CollisionConstraintEnergySolver::updateAfterTimeStep (:141-167) with the
+// normal constraint value Cn of the start-of-step state (read only when the
+// normal multiplier is active).
+__device__ __forceinline__ void al_step_begin(DynRec& c, float Cn) {
+    if (c.lam.x > 0.f) c.cpl.w = Cn;
     uint32_t fl = flags_of(c);
     if ((fl & GF_UNCLAMPED) && !(fl & GF_USE_STATIC)) fl |= GF_USE_STATIC;
     c.kind_flags = (c.kind_flags & 0xffu) | (fl << DYN_FLAG_SHIFT);
@@ -280,6 +283,12 @@ __device__ __forceinline__ void update_after_timestep(const View& d, DynRec& c) 
     c.k.x = fmaxf(kGamma * c.k.x, ks);
     c.k.y = fmaxf(kGamma * c.k.y, ks);
     c.k.z = fmaxf(kGamma * c.k.z, ks);
+}
+
+__device__ __forceinline__ void update_after_timestep(const View& d, DynRec& c) {
+    float Cn = 0.f, Ct, Cb;
+    if (c.lam.x > 0.f) eval(d, c, Cn, Ct, Cb);
+    al_step_begin(c, Cn);
 }
 
 // Eigen's unitOrthogonal for 3-vectors with the double-precision threshold

@@ -274,6 +274,14 @@ struct F64State {
     const float4* a_ext = nullptr;  // = DevState::a_ext
     const uint32_t* frozen = nullptr;    // = DevState::frozen
     const uint32_t* scene_of = nullptr;  // = DevState::scene_of
+    // dynamic contacts (null unless enabled; = DevState::dyn_off / dyn_ent /
+    // dyn): with the FP64 state every particle is positional, so the only
+    // kind is the vertex-triangle contact; x_buf: the buffered positions of
+    // conflicted particles
+    const uint32_t* dyn_off = nullptr;
+    const uint32_t* dyn_ent = nullptr;
+    DynRec* dyn = nullptr;
+    double4* x_buf = nullptr;
     uint32_t n = 0, n_tets = 0, n_ground = 0, n_items = 0;
 };
 
@@ -373,6 +381,12 @@ void launch_calib_copy(hipStream_t s, const float4* src, float4* dst, uint64_t n
 uint32_t launch_stats(hipStream_t s, const DevState& d, const LaunchCfg& cfg, bool damping, float dt, float g);
 
 // FP64-state kernels (vbd_fp64.hip)
+// FP64 state with dynamic contacts: the vertex-triangle records' start-of-step
+// and per-iteration AL updates, and the commit of buffered positions
+void launch_dyn_step_begin_f64(hipStream_t s, const F64State& f, const uint32_t* active, uint32_t n_active);
+void launch_dyn_dual_f64(hipStream_t s, const F64State& f, const uint32_t* active, uint32_t n_active,
+                         const uint32_t* done);
+void launch_commit_f64(hipStream_t s, const F64State& f, const uint32_t* list, uint32_t count, const uint32_t* done);
 void launch_solve_color_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg, uint32_t begin, uint32_t count,
                             bool damping, double dt, double4* x_out, const uint32_t* done);
 void launch_solve_cheb_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg, uint32_t begin, uint32_t count,

@@ -21,6 +21,7 @@
 // (VBDSolver::solve_Parallel, simulation/VBDSolver.hpp:48-103; the Neo-Hookean
 // accumulate, energy/elastic/NeoHookeanEnergySolver.hpp:121-227; the ground AL
 // terms, energy/collision/CollisionConstraintEnergySolver.hpp:60-285).
+#include "dyn_device.hpp"
 #include "internal.hpp"
 #include "warm_device.hpp"
 
@@ -183,6 +184,58 @@ __device__ __forceinline__ void ground_dual_update(GroundRec& c, D3 xp) {
     }
 }
 
+// Vertex-triangle contact (the only dynamic kind with the FP64 state, where
+// every particle is positional) for particle `local` of record c: the
+// triangle-rigid AL contact with the vertex as the contact point,
+// C = dir . (x_v - sum_i barys_i x_i) for dir = n, t, b, evaluated in FP64
+// from the FP64 positions (CollisionConstraintEnergySolver::accumulate<3>,
+// energy/collision/CollisionConstraintEnergySolver.hpp:179-285).
+__device__ __forceinline__ void vt_constraint(const double4* x, const DynRec& c, double& Cn, double& Ct, double& Cb) {
+    const D3 x0 = dd3(x[c.idx[0]]), x1 = dd3(x[c.idx[1]]), x2 = dd3(x[c.idx[2]]), xv = dd3(x[c.idx[3]]);
+    const D3 cp = (double)c.bary.x * x0 + (double)c.bary.y * x1 + (double)c.bary.z * x2;
+    const D3 d = xv - cp;
+    Cn = (double)c.n.x * d.x + (double)c.n.y * d.y + (double)c.n.z * d.z;
+    Ct = (double)c.t.x * d.x + (double)c.t.y * d.y + (double)c.t.z * d.z;
+    Cb = (double)c.b.x * d.x + (double)c.b.y * d.y + (double)c.b.z * d.z;
+}
+
+__device__ __forceinline__ void vt_accumulate(const double4* x, const DynRec& c, uint32_t local, D3& g, SymD& h) {
+    if (dyn::kind_of(c) != DYN_VERTEX_TRI) return;
+    double Cn, Ct, Cb;
+    vt_constraint(x, c, Cn, Ct, Cb);
+    const double Ccorr = Cn - (double)kAlpha * c.cpl.w;
+    const double lnp = fmax(0.0, (double)c.k.x * Ccorr + c.lam.x);
+    double lt = (double)c.k.y * Ct + c.lam.y, lb = (double)c.k.z * Cb + c.lam.z;
+    const double mag = sqrt(lt * lt + lb * lb);
+    const double lmax = (double)((dyn::flags_of(c) & GF_USE_STATIC) ? c.t.w : c.b.w) * lnp;
+    if (mag > lmax) {
+        const double nrm = mag + 1e-8;
+        lt = lt / nrm * lmax;
+        lb = lb / nrm * lmax;
+    }
+    const double w = local == 3 ? 1.0 : -(double)(local == 0 ? c.bary.x : (local == 1 ? c.bary.y : c.bary.z));
+    const D3 n = D3{c.n.x, c.n.y, c.n.z}, t = D3{c.t.x, c.t.y, c.t.z}, b = D3{c.b.x, c.b.y, c.b.z};
+    g = g + w * (lnp * n + lt * t + lb * b);
+    const double w2 = w * w, kn = w2 * c.k.x, kt = w2 * c.k.y, kb = w2 * c.k.z;
+    h.a00 += kn * n.x * n.x + kt * t.x * t.x + kb * b.x * b.x;
+    h.a11 += kn * n.y * n.y + kt * t.y * t.y + kb * b.y * b.y;
+    h.a22 += kn * n.z * n.z + kt * t.z * t.z + kb * b.z * b.z;
+    h.a01 += kn * n.x * n.y + kt * t.x * t.y + kb * b.x * b.y;
+    h.a02 += kn * n.x * n.z + kt * t.x * t.z + kb * b.x * b.z;
+    h.a12 += kn * n.y * n.z + kt * t.y * t.z + kb * b.y * b.z;
+}
+
+// The dynamic contacts of particle v, after its static energies (simulation/VBDSolver.hpp:400-438).
+__device__ __forceinline__ void dyn_terms(const uint32_t* __restrict__ off, const uint32_t* __restrict__ ent,
+                                          const DynRec* __restrict__ rec, const double4* x, uint32_t v, D3& g,
+                                          SymD& h) {
+    const uint32_t e1 = off[v + 1];
+    for (uint32_t e = off[v]; e < e1; e++) {
+        const uint32_t code = ent[e];
+        vt_accumulate(x, rec[code >> 3], code & 7u, g, h);
+    }
+}
+
 __device__ __forceinline__ D3 ldlt3(const SymD& A, D3 b) {
     const double d0 = A.a00;
     const double l10 = A.a01 / d0, l20 = A.a02 / d0;
@@ -283,7 +336,17 @@ __device__ __forceinline__ double4 item_solve(const VItem& it, double4 xv, doubl
 }
 
 // One colour phase (_solveParticle<3>, simulation/VBDSolver.hpp:285-449,487-510).
-template <int L, bool DAMP, int U>
+// DYN: the vertex-triangle contacts are added after the static energies and
+// a conflicted particle's result is buffered in dyn.x_buf
+// (_commitConflictedInColor, VBDSolver.hpp:108-119).
+struct DynF64 {
+    const uint32_t* off;
+    const uint32_t* ent;
+    const DynRec* rec;
+    double4* x_buf;
+};
+
+template <int L, bool DAMP, int U, bool DYN = false>
 __global__ __launch_bounds__(256) void solve_color_f64_kernel(const VItem* __restrict__ items, uint32_t n_items,
                                                               const uint32_t* __restrict__ inc,
                                                               const TetRec* __restrict__ irec,
@@ -293,7 +356,8 @@ __global__ __launch_bounds__(256) void solve_color_f64_kernel(const VItem* __res
                                                               double inv_dt, double inv_dt2,
                                                               const uint32_t* __restrict__ done,
                                                               const uint32_t* __restrict__ frozen,
-                                                              const uint32_t* __restrict__ scene_of) {
+                                                              const uint32_t* __restrict__ scene_of,
+                                                              DynF64 dy = DynF64{}) {
     if (done && *done) return;
     const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L, lane = tid % L;
@@ -312,7 +376,18 @@ __global__ __launch_bounds__(256) void solve_color_f64_kernel(const VItem* __res
     SymD h;
     item_gradient<L, U, DAMP, true>(it, lane, inc, irec, damp, x, inv_dt, g, h);
     if (!lead) return;
-    x_out[it.v] = item_solve(it, xv, yv, gr, g, h, inv_dt2);
+    if constexpr (DYN) {
+        // static energies (tets, ground) first, then the dynamic contacts
+        VItem st = it;
+        if (it.ground >= 0) ground_accumulate(gr, dd3(xv), g, h);
+        st.ground = -1;
+        dyn_terms(dy.off, dy.ent, dy.rec, x, it.v, g, h);
+        const double4 nx = item_solve(st, xv, yv, gr, g, h, inv_dt2);
+        if (it.inc_count & VITEM_CONFLICT) dy.x_buf[it.v] = nx;
+        else x_out[it.v] = nx;
+    } else {
+        x_out[it.v] = item_solve(it, xv, yv, gr, g, h, inv_dt2);
+    }
 }
 
 // Chebyshev weight of local iteration j of the running interval: from the
@@ -537,6 +612,7 @@ __global__ __launch_bounds__(256) void residual_f64_kernel(F64State f, const VIt
     if (lane != 0) return;
     const double4 xv = x[it.v], yv = f.y[it.v];
     if (it.ground >= 0) ground_accumulate(f.ground[it.ground], dd3(xv), g, h);
+    if (f.dyn_off) dyn_terms(f.dyn_off, f.dyn_ent, f.dyn, x, it.v, g, h);
     const double md = xv.w * inv_dt2;
     const double rx = md * (xv.x - yv.x) + g.x, ry = md * (xv.y - yv.y) + g.y, rz = md * (xv.z - yv.z) + g.z;
     rr[it.v] = rx * rx + ry * ry + rz * rz;
@@ -597,6 +673,7 @@ __global__ __launch_bounds__(256) void stats_f64_kernel(F64State f, const VItem*
         if (lane == 0) {
             const double4 xv = f.x[it.v], yv = f.y[it.v];
             if (it.ground >= 0) ground_accumulate(f.ground[it.ground], dd3(xv), g, h);
+            if (f.dyn_off) dyn_terms(f.dyn_off, f.dyn_ent, f.dyn, f.x, it.v, g, h);
             const double md = xv.w * inv_dt2;
             const double dx = xv.x - yv.x, dy = xv.y - yv.y, dz = xv.z - yv.z;
             const double rx = md * dx + g.x, ry = md * dy + g.y, rz = md * dz + g.z;
@@ -621,6 +698,40 @@ __global__ __launch_bounds__(256) void stats_f64_kernel(F64State f, const VItem*
         __syncthreads();
     }
     if (threadIdx.x < 4) out[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Dynamic vertex-triangle records with the FP64 state: the AL updates of
+// dyn_device.hpp at constraint values evaluated from the FP64 positions.
+__global__ void dyn_step_begin_f64_kernel(const double4* __restrict__ x, DynRec* __restrict__ dyn,
+                                          const uint32_t* __restrict__ active, uint32_t n_active) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active) return;
+    DynRec c = dyn[active[k]];
+    double Cn = 0, Ct, Cb;
+    if (c.lam.x > 0.f) vt_constraint(x, c, Cn, Ct, Cb);
+    dyn::al_step_begin(c, (float)Cn);
+    dyn[active[k]] = c;
+}
+
+__global__ void dyn_dual_f64_kernel(const double4* __restrict__ x, DynRec* __restrict__ dyn,
+                                    const uint32_t* __restrict__ active, uint32_t n_active,
+                                    const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_active) return;
+    DynRec c = dyn[active[k]];
+    double Cn, Ct, Cb;
+    vt_constraint(x, c, Cn, Ct, Cb);
+    dyn::al_update(c, (float)Cn, (float)Ct, (float)Cb);
+    dyn[active[k]] = c;
+}
+
+// _commitConflictedInColor (VBDSolver.hpp:108-119): buffered positions of one colour.
+__global__ void commit_f64_kernel(double4* __restrict__ x, const double4* __restrict__ x_buf,
+                                  const uint32_t* __restrict__ list, uint32_t count, const uint32_t* __restrict__ done) {
+    if (done && *done) return;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) x[list[k]] = x_buf[list[k]];
 }
 
 // Learned warm start with the FP64 state: keep the start (x0 = x), apply the
@@ -771,10 +882,15 @@ void launch_solve_color_f64(hipStream_t s, const F64State& f, const LaunchCfg& c
     const double inv_dt = 1.0 / dt, inv_dt2 = inv_dt * inv_dt;
 #define AVBD_F64_LAUNCH(LL, DD, UU)                                                                                  \
     hipLaunchKernelGGL((solve_color_f64_kernel<LL, DD, UU>), grid, blk, 0, s, f.items + begin, count, f.inc, f.irec, \
-                       f.damp, f.x, f.y, f.ground, x_out, inv_dt, inv_dt2, done, f.frozen, f.scene_of)
-#define AVBD_F64_U(LL, DD)                           \
-    if (cfg.batch >= 4) AVBD_F64_LAUNCH(LL, DD, 4);  \
-    else if (cfg.batch >= 2) AVBD_F64_LAUNCH(LL, DD, 2); \
+                       f.damp, f.x, f.y, f.ground, x_out, inv_dt, inv_dt2, done, f.frozen, f.scene_of, DynF64{})
+    // with dynamic contacts: one incidence gathered per lane at a time
+#define AVBD_F64_U(LL, DD)                                                                                         \
+    if (f.dyn_off)                                                                                                 \
+        hipLaunchKernelGGL((solve_color_f64_kernel<LL, DD, 1, true>), grid, blk, 0, s, f.items + begin, count,    \
+                           f.inc, f.irec, f.damp, f.x, f.y, f.ground, x_out, inv_dt, inv_dt2, done, f.frozen,      \
+                           f.scene_of, DynF64{f.dyn_off, f.dyn_ent, f.dyn, f.x_buf});                               \
+    else if (cfg.batch >= 4) AVBD_F64_LAUNCH(LL, DD, 4);                                                           \
+    else if (cfg.batch >= 2) AVBD_F64_LAUNCH(LL, DD, 2);                                                           \
     else AVBD_F64_LAUNCH(LL, DD, 1);
 #define AVBD_F64_CASE(LL)                      \
     case LL:                                    \
@@ -791,6 +907,25 @@ void launch_solve_color_f64(hipStream_t s, const F64State& f, const LaunchCfg& c
 #undef AVBD_F64_CASE
 #undef AVBD_F64_U
 #undef AVBD_F64_LAUNCH
+}
+
+void launch_dyn_step_begin_f64(hipStream_t s, const F64State& f, const uint32_t* active, uint32_t n_active) {
+    if (n_active)
+        hipLaunchKernelGGL(dyn_step_begin_f64_kernel, dim3(blocks_for(n_active, 256)), dim3(256), 0, s, f.x, f.dyn,
+                           active, n_active);
+}
+
+void launch_dyn_dual_f64(hipStream_t s, const F64State& f, const uint32_t* active, uint32_t n_active,
+                         const uint32_t* done) {
+    if (n_active)
+        hipLaunchKernelGGL(dyn_dual_f64_kernel, dim3(blocks_for(n_active, 256)), dim3(256), 0, s, f.x, f.dyn, active,
+                           n_active, done);
+}
+
+void launch_commit_f64(hipStream_t s, const F64State& f, const uint32_t* list, uint32_t count, const uint32_t* done) {
+    if (count)
+        hipLaunchKernelGGL(commit_f64_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, s, f.x, f.x_buf, list, count,
+                           done);
 }
 
 void launch_solve_cheb_f64(hipStream_t s, const F64State& f, const LaunchCfg& cfg, uint32_t begin, uint32_t count,

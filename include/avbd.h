@@ -75,8 +75,10 @@ typedef enum {
                                     records stay FP32).  An FP32 position resolves ~6e-8 m at 0.5 m,
                                     which bounds the relative force residual an FP32 state can reach
                                     near 1e-3..1e-2 on fine stiff meshes; this mode reaches 1e-4 and
-                                    below.  Positional particles only (no oriented particles), no
-                                    dynamic contacts (AVBD_E_ARG) */
+                                    below.  Positional particles only (no oriented particles);
+                                    dynamic contacts between collision triangles only (the
+                                    vertex-face contacts of avbd_set_self_collision), AVBD_E_ARG
+                                    for rod segments and spheres */
 #define AVBD_FLAG_HOST_COLORING 64u /* run the DSATUR colouring of avbd_build_graph on the host
                                        instead of the device (same colours; without a device the
                                        host colours anyway) */
@@ -411,9 +413,11 @@ int avbd_add_collision_sphere(avbd_ctx* ctx, uint32_t p, double radius);
  * triangle-rod AL contacts.
  * collision_margin: SimulationParams::collision_margin; capacity: contacts
  * per kind (0 = 65,536).  Steps then run eagerly and synchronise once per
- * step (the host needs the narrow-phase output); the persistent kernel,
- * the step graph, the warm start and residual-targeted iteration are not
- * available in this mode (AVBD_E_ARG).  Sphere-sphere pairs become
+ * step (the host needs the narrow-phase output): no step graph; a residual
+ * target is met by launching the check intervals eagerly.  Recording warm-
+ * start training data is not available in this mode (AVBD_E_ARG).  With
+ * AVBD_FLAG_FP64_STATE only collision triangles are allowed (their
+ * vertex-face contacts are evaluated from the FP64 positions).  Sphere-sphere pairs become
  * rigid-rigid AL contacts (new; the reference throws at :686-687): normal
  * and friction on both bodies, C = dir . (x_2 + R_2 cp_2 - x_1 - R_1 cp_1).
  * A rod-sphere pair that shares no particle fails the step with

@@ -233,3 +233,84 @@ def test_batch_scene_ends_its_step_as_on_its_own(fp64):
            f"{batch.ctx.stats().vertex_updates} alone {vs}")
     assert batch.ctx.stats().vertex_updates == vs
     assert max(spread) > 1          # the scenes did stop at different checks
+
+
+def test_fp64_vertex_face_contacts_match_oracle(oracle):
+    """The FP64 state with dynamic contacts (vertex-face, the only kind that
+    takes positional particles alone): the stacked blocks per step for 10
+    steps -- detected keys, slots, colours and conflicts bit-exact, positions
+    within 1e-7 x bbox after one step and 1e-6 after ten."""
+    from avbd_amd.sim import Simulation
+    banner("fp64 state + vertex-face contacts vs oracle: block stack")
+    sc = scenes.block_stack(steps=20)
+    g = Simulation(sc, fp64_state=True)
+    g.setup()
+    o = oracle.from_scene(sc)
+    bb = bbox(o.positions())
+    for step in range(10):
+        g.update(1)
+        o.step(1)
+        ko, so, to = o.detected()
+        kd, sd, td = g.ctx.detected()
+        assert np.array_equal(ko, kd) and np.array_equal(so, sd) and np.array_equal(to, td), step
+        col_o, k_o = o.colors()
+        col_d, k_d = g.ctx.coloring()
+        assert k_o == k_d and np.array_equal(col_o, col_d), step
+        assert np.array_equal(o.conflicted(), g.ctx.conflicted()), step
+        err = np.abs(g.positions() - o.positions()).max() / bb
+        if step in (0, 9):
+            banner(f"  step {step}: {len(kd)} contacts, rel err {err:.2e}")
+        assert err <= (1e-7 if step == 0 else 1e-6), (step, err)
+
+
+def test_fp64_vertex_face_residual_target_follows_oracle(oracle):
+    """Residual-targeted iteration with dynamic contacts (the steps launched
+    eagerly, the work list re-laid out every detection round): the stacked
+    blocks at 1e-4 for 20 steps -- every step ends within the target, the
+    per-step iteration counts follow the FP64 oracle's under the same rule,
+    and the positions stay within 1e-5 x bbox."""
+    from avbd_amd.sim import Simulation
+    banner("fp64 residual target 1e-4 with vertex-face contacts: block stack")
+    sc = scenes.block_stack(steps=30)
+    tol, budget, K = 1e-4, 5000, 10
+    g = Simulation(sc, fp64_state=True, residual_target=(tol, budget, K))
+    g.setup()
+    o = oracle.from_scene(sc)
+    o.set_residual_target(tol, budget, K)
+    gi, oi = [], []
+    for step in range(20):
+        g.update(1)
+        st = g.ctx.stats()
+        gi.append(st.last_step_iterations)
+        o.step(1)
+        oi.append(o.last_iters())
+        assert st.residual_rel <= tol * 1.01, (step, st.residual_rel)
+    banner(f"  iterations gpu {gi}, oracle {oi}")
+    assert g.ctx.stats().steps_at_budget == 0
+    assert abs(sum(gi) - sum(oi)) <= 0.1 * sum(oi)
+    xo = o.positions()
+    assert np.abs(g.positions() - xo).max() / bbox(xo) <= 1e-5
+
+
+def test_fp64_residual_cube_with_self_collision_on():
+    """The headline configuration on a small cube: FP64 state, residual
+    target 1e-4, the surface's vertex-face self collision on.  Through the
+    fall and the impact of a convex cube no self contact is found, and the
+    run equals the one without self collision (same iteration counts,
+    positions within 1e-9 x bbox)."""
+    from avbd_amd.sim import Simulation
+    banner("fp64 residual 1e-4, self collision on vs off: dropped cube n=8")
+    kw = dict(fp64_state=True, residual_target=(1e-4, 5000, 10), lanes_per_vertex=8, block_size=64, gather_batch=1)
+    on = Simulation(scenes.dropped_cube(n=8, height=0.05, self_collision_margin=1e-3), **kw)
+    off = Simulation(scenes.dropped_cube(n=8, height=0.05), **kw)
+    on.setup()
+    off.setup()
+    for step in range(20):
+        on.update(1)
+        off.update(1)
+        assert len(on.ctx.detected()[0]) == 0, step
+        assert on.ctx.stats().last_step_iterations == off.ctx.stats().last_step_iterations, step
+    x = off.positions()
+    err = np.abs(on.positions() - x).max() / bbox(x)
+    banner(f"  20 steps: {on.ctx.stats().iterations_total} iterations, on vs off rel err {err:.2e}")
+    assert err <= 1e-9
