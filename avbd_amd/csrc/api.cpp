@@ -744,7 +744,10 @@ hipGraphExec_t capture(avbd_ctx* c, F&& body) {
 // often as the host decides (avbd_step), and the tail.
 void capture_graph(avbd_ctx* c) {
     destroy_graph(c);
-    if (c->rc && !c->dynamic) {
+    // dynamic contacts: every step launches eagerly after its detection round
+    // (the work list can change from step to step), so nothing is captured
+    if (c->dynamic) return;
+    if (c->rc) {
         const uint32_t every = c->res_check_every;
         c->head_exec = capture(c, [&] {
             enqueue_head(c, false);
@@ -974,7 +977,7 @@ void device_dsatur(avbd_ctx* c, const std::vector<uint32_t>& scene_off, const st
 // One colour phase of vertex solves into the scratch output (state untouched).
 void solve_into_scratch(avbd_ctx* c, const LaunchCfg& cfg, uint32_t b, uint32_t count) {
     if (c->fp64)
-        launch_solve_color_f64(c->stream, c->f, cfg, b, count, c->damping, c->desc.dt, c->f.x_scratch, nullptr);
+        launch_solve_color_f64(c->stream, f64_state(c), cfg, b, count, c->damping, c->desc.dt, c->f.x_scratch, nullptr);
     else
         launch_solve_color(c->stream, c->d, cfg, b, count, c->damping, (float)c->desc.dt, c->d.x_scratch);
 }

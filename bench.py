@@ -277,7 +277,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps of the residual-targeted legs")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="dropped_cube_32")
+    # the 32^3 cube as configured: ground contact and the surface's
+    # vertex-face self collision through the LBVH every step
+    ap.add_argument("--workload", default="dropped_cube_32_self")
     ap.add_argument("--batch-workload", default="cube_batch_512")
     ap.add_argument("--no-batch", action="store_true", help="skip the batch legs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -337,7 +339,10 @@ def main():
 
     # headline: residual-targeted legs
     res, roof, sim_w = None, None, None
-    if not dynamic and not args.no_residual:
+    # residual legs: static scenes, and dynamic ones whose contacts the FP64
+    # state takes (vertex-face self collision between triangles)
+    res_ok = not dynamic or args.fp32 or not (sc.rods or sc.spheres)
+    if res_ok and not args.no_residual:
         plain, warm, sim_w = headline_pair(args.workload, args, launch)
         best = warm if warm is not None else plain
         res = {"tol": args.tol, "max_iters": args.max_iters, "check_every": args.check_every,
@@ -363,10 +368,14 @@ def main():
             # timed here
             lanes, _, batch = ksim.ctx.launch_config()
             damp = any(m.kd > 0 for m in sc.meshes)
-            variant = (f"solve_color_f64_kernel<{lanes}, {str(damp).lower()}, {batch}>" if fp64 else
-                       f"solve_color_kernel<{lanes}, {str(damp).lower()}, false, {batch}>")
+            if dynamic:   # the contact-aware variants (one incidence gathered per lane in FP64)
+                variant = (f"solve_color_f64_kernel<{lanes}, {str(damp).lower()}, 1, true>" if fp64 else
+                           f"solve_color_kernel<{lanes}, {str(damp).lower()}, true, {batch}>")
+            else:
+                variant = (f"solve_color_f64_kernel<{lanes}, {str(damp).lower()}, {batch}>" if fp64 else
+                           f"solve_color_kernel<{lanes}, {str(damp).lower()}, false, {batch}>")
             roof["variant"] = variant
-            prof = ROOT / "profiles" / f"{kernel}_{sc.name}_traffic.json"
+            prof = ROOT / "profiles" / f"{kernel}_{args.workload}_traffic.json"
             if prof.exists():
                 t = json.loads(prof.read_text())
                 if t.get("variant") == variant:
