@@ -2010,6 +2010,7 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
     return guarded(c, [&] {
         require_built(c);
         hip_check(hipEventRecord(c->ev_a, c->stream), "hipEventRecord");
+        if (c->nonfinite) throw CodeError(AVBD_E_NONFINITE, nonfinite_message());
         if (c->dynamic) {
             // per step: detection round (synchronising), then the solve
             double total = 0, detect_ms = 0;
@@ -2045,7 +2046,6 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
             c->steps_taken += n_steps;
             return;
         }
-        if (c->nonfinite) throw CodeError(AVBD_E_NONFINITE, nonfinite_message());
         if (c->rc) {
             for (uint32_t st = 0; st < n_steps; st++) run_residual_step(c, c->head_exec != nullptr && !c->timing);
         } else if (c->timing) {

@@ -28,12 +28,17 @@ CASES = {
     "dropped_cube": lambda: scenes.dropped_cube(n=6, height=0.05),
     "rods_and_spheres": lambda: scenes.rods_and_spheres(n_rods=2, elements=8),
     "ball_on_block": lambda: scenes.ball_on_block(),
+    "block_stack": lambda: scenes.block_stack(),
+    "jointed_rod_grid": lambda: scenes.rod_grid(nx=2, nz=2, segments=8, tip_joint=True),
 }
+# the FP64 state with vertex-face contacts, iterated to a residual target
+KW = {"block_stack_fp64_residual": dict(fp64_state=True, residual_target=(1e-4, 5000, 10))}
+CASES["block_stack_fp64_residual"] = lambda: scenes.block_stack()
 
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_resume_reproduces_the_run(name):
-    sim = Simulation(CASES[name]())
+    sim = Simulation(CASES[name](), **KW.get(name, {}))
     sim.setup()
     sim.ctx.step(6)
     blob = sim.ctx.save_checkpoint()
@@ -44,7 +49,7 @@ def test_resume_reproduces_the_run(name):
     sim.ctx.step(9)
     again = _state(sim)
     # a fresh context built from the same scene
-    sim2 = Simulation(CASES[name]())
+    sim2 = Simulation(CASES[name](), **KW.get(name, {}))
     sim2.setup()
     sim2.ctx.load_checkpoint(blob)
     sim2.ctx.step(9)
