@@ -129,12 +129,20 @@ struct avbd_ctx {
     uint32_t det_cap = 0;
     uint32_t* d_status = nullptr;  // [0] narrow-phase flags, [1] detected count
     RecolourDev rdev;              // device recolouring and work-list layout
-    uint32_t* h_layout = nullptr;  // pinned: the 3 x 257 per-colour offsets, then the 4 recolouring counters
+    uint32_t* h_layout = nullptr;  // pinned: the 3 x 257 per-colour offsets, then the 5 recolouring counters
+    // dynamic contacts: a detection round has changed the colours or the
+    // conflict set since set-up (the work list is no longer the set-up one)
+    bool layout_changed = false;
+    std::vector<uint32_t> setup_color;   // the set-up colouring (dynamic contacts)
     struct Grow {                  // device buffer that grows on demand
         void* p = nullptr;
         size_t bytes = 0;
     };
     Grow g_ops, g_active, g_off, g_ent, g_nboff, g_nb, g_seeds;
+    // residual mode with dynamic contacts: what the captured step graphs
+    // were recorded with (launch sizes and device buffers); they are
+    // captured again when a detection round changes any of it
+    std::vector<uint64_t> dyn_graph_key;
     // pinned staging for a detection round's uploads (written only after the
     // round's stream synchronisation, so one buffer serves every round)
     uint8_t* stage = nullptr;
@@ -519,6 +527,13 @@ SceneFreeze scene_freeze(const avbd_ctx* c) {
     return fz;
 }
 
+// Dynamic contacts, but none active and the work list still the set-up one
+// (no colour change, no conflict): the step is the static one.
+bool dyn_quiet(const avbd_ctx* c) {
+    return c->dynamic && c->cs.active.empty() && !c->layout_changed && !c->plan.conflict_offsets.empty() &&
+           c->plan.conflict_offsets.back() == 0;
+}
+
 // The FP64 state with this round's dynamic-contact arrays (the adjacency
 // buffers can move when a detection round grows them).
 F64State f64_state(const avbd_ctx* c) {
@@ -598,7 +613,10 @@ void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool
     const ResCtl* rc = c->rc;
     const uint32_t* d_active = static_cast<const uint32_t*>(c->g_active.p);
     const uint32_t n_active = c->dynamic ? (uint32_t)c->cs.active.size() : 0u;
-    const bool fuse = c->fuse_cheb && !c->dynamic;
+    // a dynamic-contact step with no active contact on the set-up work list
+    // is the static step: the static kernels, fused as at set-up
+    const bool quiet = dyn_quiet(c);
+    const bool fuse = c->fuse_cheb && (!c->dynamic || quiet);
     size_t ev = (size_t)2 * i0 * c->plan.num_colors;
     const uint32_t last = c->plan.num_colors - 1;
     // the step's state with the converged-scene flags of a batch
@@ -607,6 +625,10 @@ void enqueue_iters(avbd_ctx* c, bool with_events, uint32_t i0, uint32_t i1, bool
     F64State f0 = f64_state(c);
     dz.frozen = f0.frozen = fz.frozen;
     dz.scene_of = f0.scene_of = fz.scene_of;
+    if (quiet) {
+        dz.dyn_off = f0.dyn_off = nullptr;
+        dz.dyn_ent = f0.dyn_ent = nullptr;
+    }
     for (uint32_t i = i0; i < i1; i++) {
         const float omega = c->omega_seq[i];
         const uint32_t j = i - i0;
@@ -678,7 +700,7 @@ void enqueue_tail(avbd_ctx* c) {
         launch_residual_finish(s, c->rc, scene_freeze(c), c->n_res_scenes, c->d.n_items + c->d.n_oitems);
     // fused: the final positions are in x_alt after an odd number of
     // iterations (never in residual mode, whose intervals are even)
-    if (c->fuse_cheb && !c->dynamic && !c->rc && (iters & 1u)) {
+    if (c->fuse_cheb && (!c->dynamic || dyn_quiet(c)) && !c->rc && (iters & 1u)) {
         if (c->fp64) launch_settle_f64(s, c->f.x, c->f.x_alt, c->d.n, nullptr, iters);
         else launch_settle(s, c->d.x, c->x_alt, c->d.n, nullptr, iters);
     }
@@ -744,8 +766,10 @@ hipGraphExec_t capture(avbd_ctx* c, F&& body) {
 // often as the host decides (avbd_step), and the tail.
 void capture_graph(avbd_ctx* c) {
     destroy_graph(c);
-    // dynamic contacts: every step launches eagerly after its detection round
-    // (the work list can change from step to step), so nothing is captured
+    c->dyn_graph_key.clear();
+    // dynamic contacts: the fixed-count step launches eagerly after its
+    // detection round; residual-mode graphs are captured per detection round
+    // when its layout differs from the captured one (dyn_residual_graphs)
     if (c->dynamic) return;
     if (c->rc) {
         const uint32_t every = c->res_check_every;
@@ -759,6 +783,34 @@ void capture_graph(avbd_ctx* c) {
         return;
     }
     c->graph_exec = capture(c, [&] { enqueue_step(c, false); });
+}
+
+// Residual mode with dynamic contacts: the head / interval / tail graphs bake
+// in the colour-phase and conflict launch sizes, the number of active
+// contacts and the dynamic-adjacency buffers; capture them again when a
+// detection round changed any of these (the cube of the headline keeps one
+// layout for every step while it has no self contact).
+void dyn_residual_graphs(avbd_ctx* c) {
+    const ColorPlan& p = c->plan;
+    std::vector<uint64_t> key;
+    key.push_back(p.num_colors);
+    key.insert(key.end(), p.item_offsets.begin(), p.item_offsets.end());
+    key.insert(key.end(), p.oitem_offsets.begin(), p.oitem_offsets.end());
+    key.insert(key.end(), p.conflict_offsets.begin(), p.conflict_offsets.end());
+    key.push_back(c->cs.active.size());
+    key.push_back(dyn_quiet(c) ? 1u : 0u);
+    for (const void* ptr : {(const void*)c->g_active.p, (const void*)c->d.dyn_off, (const void*)c->d.dyn_ent})
+        key.push_back((uint64_t)(uintptr_t)ptr);
+    if (c->head_exec && key == c->dyn_graph_key) return;
+    destroy_graph(c);
+    const uint32_t every = c->res_check_every;
+    c->head_exec = capture(c, [&] {
+        enqueue_head(c, false);
+        enqueue_iters(c, false, 0, every, true);
+    });
+    c->interval_exec = capture(c, [&] { enqueue_iters(c, false, 0, every, true); });
+    c->tail_exec = capture(c, [&] { enqueue_tail(c); });
+    c->dyn_graph_key = std::move(key);
 }
 
 // Residual mode, one step: interval k is queued once the check of interval
@@ -839,11 +891,12 @@ void read_layout(avbd_ctx* c) {
     hipStream_t s = c->stream;
     hip_check(hipMemcpyAsync(c->h_layout, c->rdev.offsets, 3 * 257 * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
               "D2H");
-    hip_check(hipMemcpyAsync(c->h_layout + 3 * 257, c->rdev.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+    hip_check(hipMemcpyAsync(c->h_layout + 3 * 257, c->rdev.counters, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
               "D2H");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     const uint32_t* cnt = c->h_layout + 3 * 257;
     if (cnt[3] & 1u) throw CodeError(AVBD_E_COLORS, "Ran out of colors!");
+    if (cnt[4]) c->layout_changed = true;
     const uint32_t K = cnt[2];
     ColorPlan& plan = c->plan;
     plan.num_colors = K;
@@ -977,9 +1030,14 @@ void device_dsatur(avbd_ctx* c, const std::vector<uint32_t>& scene_off, const st
 // One colour phase of vertex solves into the scratch output (state untouched).
 void solve_into_scratch(avbd_ctx* c, const LaunchCfg& cfg, uint32_t b, uint32_t count) {
     if (c->fp64)
-        launch_solve_color_f64(c->stream, f64_state(c), cfg, b, count, c->damping, c->desc.dt, c->f.x_scratch, nullptr);
-    else
-        launch_solve_color(c->stream, c->d, cfg, b, count, c->damping, (float)c->desc.dt, c->d.x_scratch);
+        // the variant the next step launches: contact-aware unless the step is quiet
+        launch_solve_color_f64(c->stream, dyn_quiet(c) ? c->f : f64_state(c), cfg, b, count, c->damping, c->desc.dt,
+                               c->f.x_scratch, nullptr);
+    else {
+        DevState d = c->d;
+        if (dyn_quiet(c)) d.dyn_off = d.dyn_ent = nullptr;
+        launch_solve_color(c->stream, d, cfg, b, count, c->damping, (float)c->desc.dt, c->d.x_scratch);
+    }
 }
 
 // Lanes per vertex, block size and gather batch: time every candidate on
@@ -2020,9 +2078,12 @@ int avbd_step(avbd_ctx* c, uint32_t n_steps) {
                 detect_round(c);
                 detect_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                 if (c->rc) {
-                    // residual-targeted: the interval sequence, launched eagerly
-                    // (the work list may change from step to step)
-                    run_residual_step(c, false);
+                    // residual-targeted: the interval sequence, replayed from
+                    // graphs captured for this round's layout (eager with
+                    // AVBD_FLAG_NO_GRAPH or timing)
+                    const bool graphs = !(c->desc.flags & AVBD_FLAG_NO_GRAPH) && !c->timing;
+                    if (graphs) dyn_residual_graphs(c);
+                    run_residual_step(c, graphs);
                     continue;
                 }
                 enqueue_step(c, c->timing);
@@ -2444,7 +2505,7 @@ int avbd_get_launch_config(avbd_ctx* c, uint32_t* lanes, uint32_t* block, uint32
 int avbd_get_step_mode(avbd_ctx* c, uint32_t* persistent_grid, uint32_t* fused_chebyshev) {
     if (!c) return AVBD_E_ARG;
     if (persistent_grid) *persistent_grid = 0u;   // one launch per phase (the persistent step kernel is gone)
-    if (fused_chebyshev) *fused_chebyshev = (c->fuse_cheb && !c->dynamic) ? 1u : 0u;
+    if (fused_chebyshev) *fused_chebyshev = (c->fuse_cheb && (!c->dynamic || dyn_quiet(c))) ? 1u : 0u;
     return AVBD_OK;
 }
 
@@ -2778,7 +2839,6 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         if ((uint64_t)DYN_KINDS * cap > (1ull << 29)) throw CodeError(AVBD_E_ARG, "capacity too large");
         const uint32_t np = (uint32_t)c->prims.size();
         destroy_graph(c);
-        c->fuse_cheb = false;
         alloc_lbvh(c, c->lbd, np);
         std::vector<uint32_t> ty(np), id(3 * (size_t)np), ob(np);
         std::vector<double> rad(np);
@@ -2834,6 +2894,8 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         r.counters = dev_alloc<uint32_t>(c, 8);
         r.offsets = dev_alloc<uint32_t>(c, 3 * 257);
         upload(c, r.color, plan.color);
+        c->setup_color = plan.color;
+        c->layout_changed = false;
         hip_check(hipMemsetAsync(r.last_dirty, 0xFF, (size_t)n * sizeof(uint32_t), c->stream), "hipMemsetAsync");
         hip_check(hipMemsetAsync(r.conflicted, 0, (size_t)n * sizeof(uint32_t), c->stream), "hipMemsetAsync");
         const std::vector<uint32_t> cnt0{0u, 0u, plan.num_colors, 0u, 0u, 0u, 0u, 0u};
@@ -2856,7 +2918,7 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         c->d_conflicts = dev_alloc<uint32_t>(c, n);   // at most every particle is conflicted
         if (!c->h_layout) {
             void* hl = nullptr;
-            hip_check(hipHostMalloc(&hl, (3 * 257 + 4) * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc(&hl, (3 * 257 + 8) * sizeof(uint32_t), hipHostMallocDefault), "hipHostMalloc");
             c->h_layout = static_cast<uint32_t*>(hl);
         }
         // the per-colour offsets of the set-up layout (later rounds keep them
@@ -3080,6 +3142,11 @@ int avbd_load_checkpoint(avbd_ctx* c, const void* buf, uint64_t size) {
             launch_relayout(c->stream, c->rdev, c->d.items, c->d.oitems, c->d_conflicts, true);
             hip_check(hipGetLastError(), "kernel launch");
             read_layout(c);
+            // the restored work list is the set-up one only if the colours are
+            // and nothing is conflicted
+            std::vector<uint32_t> col(c->d.n);
+            hip_check(hipMemcpy(col.data(), c->rdev.color, col.size() * sizeof(uint32_t), hipMemcpyDeviceToHost), "D2H");
+            c->layout_changed = col != c->setup_color || c->plan.conflict_offsets.back() != 0;
             auto* d_act = static_cast<uint32_t*>(grow(c, c->g_active, c->cs.active.size() * sizeof(uint32_t)));
             upload(c, d_act, c->cs.active);
             auto* d_off = static_cast<uint32_t*>(grow(c, c->g_off, c->cs.dyn_off.size() * sizeof(uint32_t)));

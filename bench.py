@@ -101,6 +101,7 @@ def residual_leg(workload: str, args, launch: dict, params=None, budget=None):
     sim = Simulation(make_scene(workload), residual_target=(args.tol, budget, args.check_every),
                      use_graph=not args.res_eager, fp64_state=not args.fp32, **launch)
     sim.setup()
+    sim.setup_colors = sim.ctx.coloring()[0].copy()
     if params is not None:
         sim.ctx.mlp_set_params(params)
         sim.ctx.set_warm_start(True)
@@ -161,6 +162,16 @@ def warm_params(args):
         return None, None
     rel = str(wpath.relative_to(ROOT)) if wpath.is_relative_to(ROOT) else str(wpath)
     return warm_start.load(wpath), rel
+
+
+def contact_aware_step(sim) -> bool:
+    """Whether the next step of a dynamic-contact simulation launches the
+    contact-aware solve variants: it does unless no contact is active and the
+    work list is still the set-up one (same colours, nothing conflicted), when
+    the step is the static one."""
+    if any(len(sim.ctx.contacts(k)) for k in range(4)) or sim.ctx.conflicted().any():
+        return True
+    return not np.array_equal(sim.ctx.coloring()[0], sim.setup_colors)
 
 
 def headline_pair(workload: str, args, launch: dict, budget=None):
@@ -379,19 +390,22 @@ def main():
             # timed here
             lanes, _, batch = ksim.ctx.launch_config()
             damp = any(m.kd > 0 for m in sc.meshes)
-            if dynamic:   # the contact-aware variants (one incidence gathered per lane in FP64)
+            if dynamic and contact_aware_step(ksim):   # one incidence gathered per lane in FP64
                 variant = (f"solve_color_f64_kernel<{lanes}, {str(damp).lower()}, 1, true>" if fp64 else
                            f"solve_color_kernel<{lanes}, {str(damp).lower()}, true, {batch}>")
             else:
-                variant = (f"solve_color_f64_kernel<{lanes}, {str(damp).lower()}, {batch}>" if fp64 else
+                variant = (f"solve_color_f64_kernel<{lanes}, {str(damp).lower()}, {batch}, false>" if fp64 else
                            f"solve_color_kernel<{lanes}, {str(damp).lower()}, false, {batch}>")
             roof["variant"] = variant
-            prof = ROOT / "profiles" / f"{kernel}_{args.workload}_traffic.json"
-            if prof.exists():
-                t = json.loads(prof.read_text())
+            # the workload's own profile, else the one of its scene (the same
+            # cube without the self-collision pass runs the same static kernel)
+            for stem in (args.workload, sc.name):
+                prof = ROOT / "profiles" / f"{kernel}_{stem}_traffic.json"
+                t = json.loads(prof.read_text()) if prof.exists() else {}
                 if t.get("variant") == variant:
                     roof["traffic"] = t.get("bytes_per_launch")
                     roof["traffic_source"] = t.get("source")
+                    break
             ksim.ctx.close()
 
     # the reference's fixed iteration count, FP32 state
