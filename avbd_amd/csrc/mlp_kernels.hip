@@ -3,16 +3,15 @@
 // displacement from 1-ring features and replaces the inertial initial guess.
 //
 // Every matrix product, in inference and in training (forward and backward),
-// runs through one BF16 MFMA GEMM kernel with FP32 accumulation
-// (v_mfma_f32_16x16x16_bf16): bf16 operands in HBM, one wavefront per 16x16
-// output tile, operand transposition chosen at compile time, and fused
-// epilogues (bias, SiLU, SiLU derivative, FP32 / BF16 stores).  Weights keep
-// an FP32 master copy updated by Adam; the bf16 copy is refreshed after each
-// update.
+// runs on gfx950's BF16 MFMA with FP32 accumulation
+// (v_mfma_f32_16x16x32_bf16, 32-deep products): bf16 operands in HBM,
+// operand transposition chosen at compile time, and fused epilogues (bias,
+// SiLU, SiLU derivative, FP32 / BF16 stores).  Weights keep an FP32 master
+// copy updated by Adam; the bf16 copy is refreshed after each update.
 //
-// MFMA operand layout (16x16x16, 64 lanes, lane l, r = l % 16, g = l / 16):
-//   A (16 x 16, rows x k): lane holds A[r][4g .. 4g+3]
-//   B (16 x 16, k x cols): lane holds B[4g .. 4g+3][r]
+// MFMA operand layout (16x16x32, 64 lanes, lane l, r = l % 16, g = l / 16):
+//   A (16 x 32, rows x k): lane holds A[r][8g .. 8g+7]
+//   B (32 x 16, k x cols): lane holds B[8g .. 8g+7][r]
 //   D (16 x 16):           lane holds D[4g .. 4g+3][r]
 #include <hip/hip_runtime.h>
 
@@ -25,8 +24,10 @@ namespace avbd {
 
 namespace {
 
-typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __bf16 as_bf16(uint16_t u) { return __builtin_bit_cast(__bf16, u); }
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     // round to nearest even (NaN kept quiet)
@@ -40,7 +41,8 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.f / (1.f + __expf(-z)); }
 
 // C[M x N] = A(M x K) B(K x N) with A(m, k) = TA ? a[k * lda + m] : a[m * lda + k]
-// and B(k, n) = TB ? b[n * ldb + k] : b[k * ldb + n].  M, N, K multiples of 16.
+// and B(k, n) = TB ? b[n * ldb + k] : b[k * ldb + n].  M, N, K multiples of
+// 16; a K that is not a multiple of 32 is zero-filled to the next one.
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restrict__ a, int lda,
                                                         const uint16_t* __restrict__ b, int ldb,
@@ -54,16 +56,17 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
     const int tm = (tile / tiles_n) << 4, tn = (tile % tiles_n) << 4;
     const int r = lane & 15, g = lane >> 4;
     float4_t acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        short4_t av, bv;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8_t av, bv;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = k0 + 4 * g + j;
+        for (int j = 0; j < 8; j++) {
+            const int k = k0 + 8 * g + j;
             const int m = tm + r, n = tn + r;
-            av[j] = (short)(TA ? a[(size_t)k * lda + m] : a[(size_t)m * lda + k]);
-            bv[j] = (short)(TB ? b[(size_t)n * ldb + k] : b[(size_t)k * ldb + n]);
+            const bool in = k < K;
+            av[j] = as_bf16(in ? (TA ? a[(size_t)k * lda + m] : a[(size_t)m * lda + k]) : (uint16_t)0);
+            bv[j] = as_bf16(in ? (TB ? b[(size_t)n * ldb + k] : b[(size_t)k * ldb + n]) : (uint16_t)0);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
     }
     const int n = tn + r;
 #pragma unroll
@@ -87,19 +90,14 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
     }
 }
 
-// Four consecutive bf16 (8-byte aligned) as an MFMA operand fragment.
-__device__ __forceinline__ short4_t ld_frag(const uint16_t* p) {
-    const uint2 u = *reinterpret_cast<const uint2*>(p);
-    short4_t v;
-    v[0] = (short)(u.x & 0xffffu);
-    v[1] = (short)(u.x >> 16);
-    v[2] = (short)(u.y & 0xffffu);
-    v[3] = (short)(u.y >> 16);
-    return v;
+// Eight consecutive bf16 (16-byte aligned) as an MFMA operand fragment.
+__device__ __forceinline__ bf16x8_t ld_frag(const uint16_t* p) {
+    return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4*>(p));
 }
 
 // The forward products: C[M x N] = A(M x K) B(K x N) with A row-major and
-// B(k, n) = b[n * ldb + k] (weights stored out x in), N = 16 NT.  One
+// B(k, n) = b[n * ldb + k] (weights stored out x in), N = 16 NT, K a
+// multiple of 32, lda and ldb multiples of 8 (16-byte fragments).  One
 // wavefront owns a 16-row strip and all NT output tiles of it, so each A
 // fragment is read once (the one-tile-per-wavefront kernel above reads it NT
 // times) and the weight fragments, the same for every strip, come from the
@@ -118,13 +116,13 @@ __global__ __launch_bounds__(256) void gemm_rows_kernel(const uint16_t* __restri
     float4_t acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = float4_t{0.f, 0.f, 0.f, 0.f};
-    const uint16_t* arow = a + (size_t)(tm + r) * lda + 4 * g;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const short4_t av = ld_frag(arow + k0);
+    const uint16_t* arow = a + (size_t)(tm + r) * lda + 8 * g;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const bf16x8_t av = ld_frag(arow + k0);
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            const short4_t bv = ld_frag(b + (size_t)(t * 16 + r) * ldb + k0 + 4 * g);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+            const bf16x8_t bv = ld_frag(b + (size_t)(t * 16 + r) * ldb + k0 + 8 * g);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
         }
     }
 #pragma unroll
@@ -201,7 +199,8 @@ inline int blocks_for(int n, int b) { return (n + b - 1) / b; }
 void launch_gemm_bf16(hipStream_t s, bool ta, bool tb, int epi, const uint16_t* a, int lda, const uint16_t* b,
                       int ldb, float* c32, uint16_t* c16, int ldc, const float* bias, const float* aux, int ldaux,
                       int M, int N, int K) {
-    if (!ta && tb && epi != MLP_EPI_SILU_BWD && (N == 128 || N == 16)) {
+    if (!ta && tb && epi != MLP_EPI_SILU_BWD && (N == 128 || N == 16) && K % 32 == 0 && lda % 8 == 0 &&
+        ldb % 8 == 0) {
         const dim3 grid(blocks_for(M / 16, 4)), block(256);
 #define AVBD_ROWS(E_, NT_)                                                                                      \
     if (epi == E_ && N == 16 * NT_) {                                                                           \
