@@ -412,10 +412,13 @@ int avbd_add_collision_sphere(avbd_ctx* ctx, uint32_t p, double radius);
  * layout on the device -- and the solve includes the triangle-rigid /
  * triangle-rod AL contacts.
  * collision_margin: SimulationParams::collision_margin; capacity: contacts
- * per kind (0 = 65,536).  Steps then run eagerly and synchronise once per
- * step (the host needs the narrow-phase output): no step graph; a residual
- * target is met by launching the check intervals eagerly.  Recording warm-
- * start training data is not available in this mode (AVBD_E_ARG).  With
+ * per kind (0 = 65,536).  Steps then synchronise once per step (the host
+ * needs the narrow-phase output); fixed-count steps launch eagerly, and a
+ * residual target replays head / interval / tail graphs captured for the
+ * round's layout (captured again when the launch sizes, the active contacts
+ * or the adjacency buffers change).  A step with no active contact on the
+ * set-up work list runs the static kernels.  Recording warm-start training
+ * data is not available in this mode (AVBD_E_ARG).  With
  * AVBD_FLAG_FP64_STATE only collision triangles are allowed (their
  * vertex-face contacts are evaluated from the FP64 positions).  Sphere-sphere pairs become
  * rigid-rigid AL contacts (new; the reference throws at :686-687): normal
