@@ -967,6 +967,14 @@ void detect_round(avbd_ctx* c) {
     stage_upload(c, d_seeds, cs.seeds);
     c->rdev.d_off = d_nboff;
     c->rdev.d_nb = d_nb;
+    // no new contact, nothing left dirty or conflicted by the last round: the
+    // recolouring would change nothing (ColorList only recolours from dirty
+    // particles), so the colours, the work list and its offsets stand
+    const uint32_t* last = c->h_layout + 3 * 257;
+    if (cs.seeds.empty() && last[0] == 0 && last[1] == 0) {
+        hip_check(hipGetLastError(), "kernel launch");
+        return;
+    }
     launch_recolour(s, c->rdev, d_seeds, (uint32_t)cs.seeds.size());
     launch_relayout(s, c->rdev, c->d.items, c->d.oitems, c->d_conflicts, false);
     hip_check(hipGetLastError(), "kernel launch");
