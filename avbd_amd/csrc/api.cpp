@@ -2825,6 +2825,9 @@ int avbd_set_dynamic_contacts(avbd_ctx* c, double margin, uint32_t capacity) {
         if (c->prims.empty()) throw CodeError(AVBD_E_ARG, "no collision primitives");
         if (!(margin >= 0)) throw CodeError(AVBD_E_ARG, "collision margin must be >= 0");
         if (c->recording) throw CodeError(AVBD_E_ARG, "recording is not available with dynamic contacts");
+        // the scenes of a batch are independent: their primitives must not meet
+        if (c->scene_offsets.size() > 2)
+            throw CodeError(AVBD_E_ARG, "dynamic contacts take one scene, not a batch (avbd_set_scenes)");
         if (c->fp64)
             for (const HostPrim& pr : c->prims)
                 if (pr.type != PRIM_TRIANGLE)

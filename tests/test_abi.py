@@ -259,3 +259,18 @@ def test_energy_particle_kind_validated():
     with pytest.raises(_lib.AvbdError) as e:
         ctx.build_graph()
     assert "oriented particle" in str(e.value)
+
+
+def test_dynamic_contacts_refuse_a_batch():
+    """The scenes of a batch are independent, so dynamic contacts (whose
+    primitives would meet across scenes) are refused for one (AVBD_E_ARG).
+    Without a device avbd_set_dynamic_contacts stops at the build check
+    first; both are argument errors."""
+    from avbd_amd import _lib
+    from avbd_amd.sim import SimulationContext
+    ctx = SimulationContext(dt=1 / 60, solver_iters=4)
+    ctx.add_particles(np.zeros((6, 3)), mass=np.ones(6))
+    ctx.set_scenes([0, 3, 6])
+    with pytest.raises(_lib.AvbdError) as e:
+        ctx.set_dynamic_contacts(1e-3)
+    assert e.value.code in (-1, -7)
