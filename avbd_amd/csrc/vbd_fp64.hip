@@ -23,6 +23,7 @@
 // terms, energy/collision/CollisionConstraintEnergySolver.hpp:60-285).
 #include "dyn_device.hpp"
 #include "internal.hpp"
+#include "kernarg.hpp"
 #include "warm_device.hpp"
 
 namespace avbd {
@@ -358,11 +359,13 @@ __global__ __launch_bounds__(256) void solve_color_f64_kernel(const VItem* __res
                                                               const uint32_t* __restrict__ frozen,
                                                               const uint32_t* __restrict__ scene_of,
                                                               DynF64 dy = DynF64{}) {
-    if (done && *done) return;
+    kernarg_prefetch(items, n_items, inc, irec, x, y, ground, x_out, done, frozen, scene_of, inv_dt, inv_dt2,
+                     blockDim.x, gridDim.x);
     const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L, lane = tid % L;
     if (item >= n_items) return;   // whole L-lane groups exit together
-    const VItem it = items[item];
+    const VItem it = items[item];   // in flight with the converged flag's load
+    if (done && *done) return;      // residual target reached earlier in this step
     if (frozen && frozen[scene_of[it.v]]) return;   // the item's scene has converged in this step
     double4 xv = make_double4(0.0, 0.0, 0.0, 0.0), yv = xv;
     GroundRec gr{};
@@ -406,14 +409,16 @@ __global__ __launch_bounds__(256) void solve_cheb_f64_kernel(F64State f, const V
                                                              const double4* __restrict__ x_cur, double4* x_next,
                                                              float omega, const ResCtl* rc, const float* omega_tab,
                                                              uint32_t j, double inv_dt, double inv_dt2) {
-    if (rc && rc->done) return;
-    const double w = cheb_omega(omega, rc, omega_tab, j);
+    kernarg_prefetch(f.inc, f.irec, f.damp, f.y, f.ground, f.x_last, f.x_lastlast, f.ground_of, f.frozen, f.scene_of,
+                     items, n_items, sx, cx, cheb_list, n_cheb, x_cur, x_next, rc, inv_dt, inv_dt2, blockDim.x);
     const uint32_t xcd = blockIdx.x % 8u, q = blockIdx.x / 8u;
     if (q < sx) {
         const uint32_t tid = (xcd * sx + q) * blockDim.x + threadIdx.x;
         const uint32_t item = tid / L, lane = tid % L;
         if (item >= n_items) return;
-        const VItem it = items[item];
+        const VItem it = items[item];   // in flight with the converged flag's load
+        if (rc && rc->done) return;
+        const double w = cheb_omega(omega, rc, omega_tab, j);
         const uint32_t v = it.v;
         if (f.frozen && f.frozen[f.scene_of[v]]) {   // converged scene: keep its position in both buffers
             if (lane == 0) x_next[v] = x_cur[v];
@@ -446,6 +451,8 @@ __global__ __launch_bounds__(256) void solve_cheb_f64_kernel(F64State f, const V
         const uint32_t k = (xcd * cx + (q - sx)) * blockDim.x + threadIdx.x;
         if (k >= n_cheb) return;
         const uint32_t p = cheb_list[k];
+        if (rc && rc->done) return;
+        const double w = cheb_omega(omega, rc, omega_tab, j);
         if (f.frozen && f.frozen[f.scene_of[p]]) {
             x_next[p] = x_cur[p];
             return;
@@ -600,11 +607,13 @@ __global__ __launch_bounds__(256) void residual_f64_kernel(F64State f, const VIt
                                                            uint32_t n_items, const double4* __restrict__ x,
                                                            double* __restrict__ rr, const ResCtl* rc, double inv_dt,
                                                            double inv_dt2) {
-    if (rc && rc->done) return;
+    kernarg_prefetch(f.inc, f.irec, f.damp, f.y, f.ground, f.frozen, f.scene_of, f.dyn_off, items, n_items, x, rr, rc,
+                     inv_dt, inv_dt2, blockDim.x);
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L, lane = tid % L;
     if (item >= n_items) return;
-    const VItem it = items[item];
+    const VItem it = items[item];   // in flight with the converged flag's load
+    if (rc && rc->done) return;
     if (f.frozen && f.frozen[f.scene_of[it.v]]) return;   // keeps the residual it converged with
     D3 g;
     SymD h;

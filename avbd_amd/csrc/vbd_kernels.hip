@@ -19,6 +19,7 @@
 // Oriented (6-DOF) vertices of a colour are solved in the same phase, one
 // thread each (rigid_device.hpp).
 #include "internal.hpp"
+#include "kernarg.hpp"
 #include "dyn_device.hpp"
 #include "rigid_device.hpp"
 #include "warm_device.hpp"
@@ -368,12 +369,14 @@ __global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_color_kernel(const V
                                                            const uint32_t* __restrict__ done, DynArgs da,
                                                            const uint32_t* __restrict__ frozen,
                                                            const uint32_t* __restrict__ scene_of) {
-    if (done && *done) return;   // residual target reached earlier in this step
+    kernarg_prefetch(items, n_items, inc, irec, x, y, ground, x_out, done, frozen, scene_of, inv_dt, inv_dt2,
+                     blockDim.x, gridDim.x);
     const uint32_t tid = xcd_logical_block() * blockDim.x + threadIdx.x;
     const uint32_t item = tid / L;
     const uint32_t lane = tid % L;
     if (item >= n_items) return;   // whole L-lane groups exit together (L divides the block)
-    const VItem it = items[item];
+    const VItem it = items[item];   // in flight with the converged flag's load
+    if (done && *done) return;      // residual target reached earlier in this step
     if (frozen && frozen[scene_of[it.v]]) return;   // the item's scene has converged in this step
     solve_item<L, DAMP, DYN, U>(it, lane, inc, irec, damp, x, y, ground, x_out, x_buf, inv_dt, inv_dt2, da);
 }
@@ -626,14 +629,16 @@ __global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_cheb_kernel(
     DevState d, const VItem* __restrict__ items, uint32_t n_items, uint32_t sx, uint32_t cx,
     const uint32_t* __restrict__ cheb_list, uint32_t n_cheb, float4* __restrict__ x_next, float omega_fixed,
     const ResCtl* rc, const float* __restrict__ omega_tab, uint32_t j, float inv_dt, float inv_dt2) {
-    if (rc && rc->done) return;
-    const float omega = cheb_weight(omega_fixed, rc, omega_tab, j);
+    kernarg_prefetch(d.x, d.y, d.x_last, d.x_lastlast, d.ground, d.ground_of, d.inc, d.irec, d.damp, d.frozen,
+                     d.scene_of, items, n_items, sx, cx, cheb_list, n_cheb, x_next, rc, inv_dt, inv_dt2, blockDim.x);
     const uint32_t xcd = blockIdx.x % 8u, q = blockIdx.x / 8u;
     if (q < sx) {
         const uint32_t tid = (xcd * sx + q) * blockDim.x + threadIdx.x;
         const uint32_t item = tid / L, lane = tid % L;
         if (item >= n_items) return;   // whole L-lane groups exit together
-        const VItem it = items[item];
+        const VItem it = items[item];   // in flight with the converged flag's load
+        if (rc && rc->done) return;
+        const float omega = cheb_weight(omega_fixed, rc, omega_tab, j);
         const uint32_t v = it.v;
         if (d.frozen && d.frozen[d.scene_of[v]]) {   // converged scene: keep its position in both buffers
             if (lane == 0) x_next[v] = d.x[v];
@@ -665,6 +670,8 @@ __global__ __launch_bounds__(U > 1 ? 256 : 1024) void solve_cheb_kernel(
         const uint32_t k = (xcd * cx + (q - sx)) * blockDim.x + threadIdx.x;
         if (k < n_cheb) {
             const uint32_t p = cheb_list[k];
+            if (rc && rc->done) return;
+            const float omega = cheb_weight(omega_fixed, rc, omega_tab, j);
             if (d.frozen && d.frozen[d.scene_of[p]]) x_next[p] = d.x[p];
             else cheb_dual_io(d, p, omega, d.x, x_next);
         }
