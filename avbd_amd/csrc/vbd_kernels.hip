@@ -869,9 +869,10 @@ __global__ __launch_bounds__(1024) void residual_scene_kernel(const ResCtl* rc, 
                                                               double* __restrict__ acc_scene,
                                                               const double* __restrict__ mg2_scene, double tol2,
                                                               uint32_t every, SceneFreeze fz) {
-    if (rc->done) return;
+    kernarg_prefetch(rc, rr, scene_off, acc_scene, mg2_scene, tol2, every, fz.frozen, fz.iters);
     __shared__ double red[16];
-    const uint32_t s = blockIdx.x, b = scene_off[s], e = scene_off[s + 1];
+    const uint32_t s = blockIdx.x, b = scene_off[s], e = scene_off[s + 1];   // in flight with the flag's load
+    if (rc->done) return;
     if (fz.frozen && fz.frozen[s]) return;   // its sum is the one it converged with
     // four independent loads in flight per thread; the grouping depends only
     // on the offset from the scene's first particle, so a scene sums the
@@ -911,7 +912,7 @@ __global__ __launch_bounds__(1024) void residual_scene_kernel(const ResCtl* rc, 
 __global__ __launch_bounds__(256) void residual_decide_kernel(ResCtl* rc, const double* __restrict__ acc_scene,
                                                               const double* __restrict__ mg2_scene,
                                                               uint32_t n_scenes, double tol2, uint32_t every) {
-    if (rc->done) return;
+    kernarg_prefetch(rc, acc_scene, mg2_scene, n_scenes, tol2, every);
     __shared__ double r_s[256], m_s[256], w_s[256];
     double sum = 0, mg = 0, w = 0;
     for (uint32_t s = threadIdx.x; s < n_scenes; s += 256) {
@@ -920,6 +921,7 @@ __global__ __launch_bounds__(256) void residual_decide_kernel(ResCtl* rc, const 
         mg += m;
         if (m > 0) w = fmax(w, a / m);
     }
+    if (rc->done) return;   // the same for the whole block, tested after the sums' loads are in flight
     r_s[threadIdx.x] = sum;
     m_s[threadIdx.x] = mg;
     w_s[threadIdx.x] = w;
